@@ -1,0 +1,205 @@
+/*
+ * yannq_amd C ABI: the variational Monte Carlo step of yannq (Metropolis
+ * sampling of an RBM, local energies, the log-derivative matrix O and the
+ * stochastic-reconfiguration solve) on one AMD Instinct MI355X (gfx950).
+ *
+ * The reference exposes this path as compile-time C++ concepts, not an FFI.
+ * Each entry point below replaces one member of those concepts; the citation
+ * names the reference member (paths under the yannq source tree).  The
+ * header-only C++ wrappers in include/yannq_amd/ rebuild the concepts
+ * (Machine, Sampler, SRMat, Runner) on top of these calls.
+ *
+ * Conventions
+ *   - Every function returns a yq_status; 0 is success.  yq_last_error()
+ *     gives the message of the last failure on that context.  Nothing throws.
+ *   - Host arrays are caller-owned and copied in or out.  Device buffers are
+ *     owned by the context.  One host thread per context; all work is
+ *     ordered on the context's HIP stream and calls return after it finished.
+ *   - Real parameters are doubles in the layout [W (m x n, column-major) |
+ *     a (n) | b (m)] of Machines/RBM.hpp:271-281 (W block only without bias).
+ *   - Spins are int8 (+1 / -1), one row of n per configuration.
+ */
+#ifndef YANNQ_AMD_H
+#define YANNQ_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YQ_API __attribute__((visibility("default")))
+
+typedef struct yq_ctx yq_ctx;
+
+typedef enum {
+  YQ_OK = 0,
+  YQ_ERR_ARG = 1,          /* invalid argument or size */
+  YQ_ERR_STATE = 2,        /* call out of order (e.g. no samples yet) */
+  YQ_ERR_HIP = 3,          /* HIP / rocBLAS / rocSOLVER failure */
+  YQ_ERR_NAN = 4,          /* NaN or Inf in parameters or results */
+  YQ_ERR_NOT_SPD = 5,      /* Cholesky of S + shift I failed */
+  YQ_ERR_NOT_CONVERGED = 6,/* iterative solve hit max_iter */
+  YQ_ERR_OOM = 7,          /* device allocation failed */
+  YQ_ERR_UNSUPPORTED = 8   /* combination not implemented */
+} yq_status;
+
+typedef enum { YQ_TFI = 0, YQ_XXZ = 1, YQ_J1J2 = 2 } yq_ham_kind;
+typedef enum { YQ_LOCAL = 0, YQ_EXCHANGE = 1 } yq_sweeper_kind;
+/* YQ_FP64: every operation in double.  YQ_MIXED: sampling and local energies
+ * in float with theta refreshed from double once per sweep; the CG products
+ * as split-bf16 MFMA with float accumulation; residual, refinement and update
+ * in double. */
+typedef enum { YQ_FP64 = 0, YQ_MIXED = 1 } yq_precision;
+/* YQ_SOLVE_EXPLICIT: S = O_c^T O_c / Ns and a Cholesky solve.
+ * YQ_SOLVE_CG: matrix-free conjugate gradients on O_c (materialised O in
+ *   FP64, the reference's apply), no S.
+ * YQ_SOLVE_CG_FACTORED: matrix-free CG on the factored O (sigma and
+ *   tanh(theta) only, O never stored); with YQ_MIXED the inner products run
+ *   on split-bf16 MFMA and an FP64 iterative refinement drives the residual. */
+typedef enum {
+  YQ_SOLVE_EXPLICIT = 0,
+  YQ_SOLVE_CG = 1,
+  YQ_SOLVE_CG_FACTORED = 2
+} yq_solver;
+typedef enum {
+  YQ_SR_F = 0,      /* energy gradient F (dim) */
+  YQ_SR_OMEAN = 1,  /* column mean of O (dim) */
+  YQ_SR_O = 2,      /* centred O (Ns x dim, row-major); explicit/CG only */
+  YQ_SR_S = 3,      /* S (dim x dim); explicit only */
+  YQ_SR_ELOC = 4    /* local energies (Ns) */
+} yq_sr_item;
+
+typedef struct {
+  int kind;       /* yq_ham_kind */
+  int n;          /* sites, periodic chain */
+  double J;       /* TFI: J; XXZ: J; J1J2: J1 */
+  double h;       /* TFI: coefficient of the transverse term */
+  double delta;   /* XXZ: anisotropy */
+  double J2;      /* J1J2: next-nearest coupling */
+  int sign_rule;  /* XXZ / J1J2: 1 applies the Marshall sign rule */
+} yq_ham_desc;
+
+/* ---- context ---------------------------------------------------------- */
+/* No reference counterpart (the reference has no device); owns the stream. */
+YQ_API int yq_ctx_create(int device, yq_ctx** out);
+YQ_API int yq_ctx_destroy(yq_ctx* ctx);
+YQ_API const char* yq_last_error(const yq_ctx* ctx);
+
+/* ---- Machine: Machines/RBM.hpp ---------------------------------------- */
+/* RBM(n, m, useBias), all parameters zero: RBM.hpp:50-56.  Only real double
+ * machines are supported by this build (dtype must be 1). */
+YQ_API int yq_rbm_create(yq_ctx* ctx, int dtype, int n, int m, int use_bias);
+/* getN / getM / getDim: RBM.hpp:115-130 */
+YQ_API int yq_rbm_dims(const yq_ctx* ctx, int* n, int* m, int* dim);
+/* setParams / getParams: RBM.hpp:271-291 */
+YQ_API int yq_rbm_set_params(yq_ctx* ctx, const double* params);
+YQ_API int yq_rbm_get_params(yq_ctx* ctx, double* params);
+/* updateParams (params += delta): RBM.hpp:261-269 */
+YQ_API int yq_rbm_update_params(yq_ctx* ctx, const double* delta);
+/* initializeRandom(re, sigma): RBM.hpp:301-322, normals drawn in the order
+ * a, b, W from a Philox stream keyed by seed (values differ from the
+ * reference's std::normal_distribution by design). */
+YQ_API int yq_rbm_init_random(yq_ctx* ctx, uint64_t seed, double sigma);
+/* hasNaN: RBM.hpp:293-296 */
+YQ_API int yq_rbm_has_nan(yq_ctx* ctx, int* out);
+
+/* ---- Hamiltonian: Hamiltonians/{TFIsing,XXZ,J1J2}.hpp ------------------ */
+/* A POD descriptor replaces the E_loc functor operator()(const State&):
+ * TFIsing.hpp:28-39, XXZ.hpp:37-49, J1J2.hpp:35-55. */
+YQ_API int yq_ham_set(yq_ctx* ctx, const yq_ham_desc* desc);
+
+/* ---- Sampler: Samplers/SamplerMT.hpp + Local/SwapSweeper --------------- */
+/* SamplerMT(qs, nTmps, nChainsPer, sweeper): SamplerMT.hpp:157-173, with the
+ * sweeper's nSweep (LocalSweeper.hpp:19, SwapSweeper.hpp:70) as
+ * sweep_factor.  Only n_temps == 1 is supported.  seed keys the Philox
+ * streams of every chain. */
+YQ_API int yq_sampler_config(yq_ctx* ctx, int sweeper, int n_temps,
+                             int n_chains_per, int sweep_factor, int precision,
+                             uint64_t seed);
+/* randomizeSigma() / randomizeSigma(nup): SamplerMT.hpp:194-211; nup < 0
+ * draws unconstrained spins. */
+YQ_API int yq_sampler_randomize(yq_ctx* ctx, int nup);
+/* randomize(randomizer) with caller-chosen configurations (chains x n):
+ * SamplerMT.hpp:216-223. */
+YQ_API int yq_sampler_set_sigma(yq_ctx* ctx, const int8_t* sigma);
+/* Chain state: spins (chains x n), per-chain proposal counters and accepted
+ * proposal counts. */
+YQ_API int yq_sampler_get_state(yq_ctx* ctx, int8_t* sigma, uint64_t* ctr,
+                                uint64_t* acc);
+/* sample(nSweeps, nDiscard): SamplerMT.hpp:293-317.  Samples stay on the
+ * device; *n_samples = n_chains_per * n_sweeps, ordered sweep-major. */
+YQ_API int yq_sample(yq_ctx* ctx, int n_sweeps, int n_discard, int* n_samples);
+/* Copy the current batch out: spins (Ns x n) and theta (Ns x m, doubles). */
+YQ_API int yq_samples_get(yq_ctx* ctx, int8_t* sigma, double* theta);
+/* Replace the batch by caller configurations (Ns x n); theta = W sigma + b is
+ * computed on the device (makeData, RBM.hpp:362-365). precision selects the
+ * arithmetic of the following yq_eloc / yq_sr_build. */
+YQ_API int yq_samples_set(yq_ctx* ctx, int ns, const int8_t* sigma,
+                          int precision);
+
+/* ---- Energy: Observables/Energy.hpp + GroundState/utils.hpp:108-126 ---- */
+/* E_loc of every sample (eloc may be null), mean = Re <E_loc>,
+ * var = <Re E_loc^2> - mean^2: Energy.hpp:59-70. */
+YQ_API int yq_eloc(yq_ctx* ctx, double* eloc, double* mean, double* var);
+
+/* ---- SR: GroundState/SRMat.hpp ----------------------------------------- */
+/* constructFromSamples: O rows, column mean, centring, F = O_c^T E / Ns
+ * (SRMat.hpp:89-103); with YQ_SOLVE_EXPLICIT also S (corrMat, :142-149).
+ * Computes E_loc first if yq_eloc was not called on this batch. */
+YQ_API int yq_sr_build(yq_ctx* ctx, int solver, int precision);
+/* Copy one of the built items to the host. */
+YQ_API int yq_sr_get(yq_ctx* ctx, int item, double* out);
+/* apply: O_c^T (O_c v) / Ns + shift v, SRMat.hpp:170-189. */
+YQ_API int yq_sr_apply(yq_ctx* ctx, double shift, const double* v,
+                       double* out);
+/* Solve (S + shift I) x = rhs (rhs = F when null): solveExact
+ * (SRMat.hpp:231-237, RunRBM.hpp:194-203) for the explicit solver,
+ * solveCG (SRMat.hpp:197-223) otherwise.  resid = |(S+shift)x - rhs|/|rhs|,
+ * iters = CG iterations (0 for Cholesky).  tol is the CG relative residual
+ * (for YQ_MIXED factored CG: the outer FP64 refinement target). */
+YQ_API int yq_sr_solve(yq_ctx* ctx, double shift, double tol, int max_iter,
+                       const double* rhs, double* x, double* resid,
+                       int* iters);
+
+/* ---- Runner: Runners/RunRBM.hpp:151-220 --------------------------------- */
+typedef struct {
+  int n_sweeps;      /* sample(nSweeps, .) */
+  int n_discard;     /* sample(., nDiscard) */
+  int randomize;     /* re-randomise chains before sampling (RunRBM.hpp:166) */
+  int nup;           /* constraint for randomize, < 0 for none */
+  int solver;        /* yq_solver */
+  int precision;     /* yq_precision */
+  double lambda;     /* diagonal shift for this step */
+  double cg_tol;     /* CG / refinement relative residual */
+  int cg_max_iter;
+  double lr;         /* SGD step eta (already max(alpha/t^p, 1e-4)) */
+  double grad_clip;  /* 0 = off (RunRBM.hpp:205-209) */
+} yq_step_cfg;
+typedef struct {
+  double energy;     /* srm.eloc() */
+  double energy_var;
+  double update_norm;/* |v| before clipping */
+  double cg_err;     /* |apply(v) - F| / |F| */
+  int cg_iters;
+  int n_samples;
+  double sample_ms;  /* sampling time (device) */
+  double solve_ms;   /* E_loc + SR build + solve + update (device) */
+} yq_step_out;
+/* One epoch body: [randomize], sample, constructFromSamples, solve, SGD
+ * update, all on the device. */
+YQ_API int yq_sr_step(yq_ctx* ctx, const yq_step_cfg* cfg, yq_step_out* out);
+
+/* ---- instrumentation (no reference counterpart) ------------------------- */
+/* Time every launch of the named kernels with HIP events on the launch's
+ * stream.  labels: comma-separated kernel labels (e.g. "sample,bwd_mfma"),
+ * NULL or "" to disable.  yq_prof_read returns the accumulated device time
+ * and launch count of one label and resets it. */
+YQ_API int yq_prof_enable(yq_ctx* ctx, const char* labels);
+YQ_API int yq_prof_read(yq_ctx* ctx, const char* label, double* total_ms,
+                        long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YANNQ_AMD_H */

@@ -1,0 +1,699 @@
+// CPU oracle for the VMC / SR step -- test infrastructure, see oracle.h.
+// Reference citations are paths under the yannq source tree.
+#include "oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "../yannq_amd/csrc/yq_math.h"
+
+namespace {
+
+inline double sgn(int8_t s) { return s > 0 ? 1.0 : -1.0; }
+
+// Parameter layout [W col-major m x n | a (n) | b (m)]: Machines/RBM.hpp:271-281.
+inline double Wat(const double* W, int m, int j, int i) {
+  return W[(size_t)i * m + j];
+}
+
+// Bond list of the off-diagonal terms of one Hamiltonian, in the order its
+// E_loc functor visits them (TFIsing.hpp:33-37, XXZ.hpp:37-49, J1J2.hpp:35-55).
+struct Bond {
+  int k, l;
+};
+std::vector<Bond> bonds_of(const or_ham* H) {
+  std::vector<Bond> b;
+  const int n = H->n;
+  if (H->kind == OR_TFI) {
+    for (int i = 0; i < n; ++i) b.push_back({i, -1});
+  } else if (H->kind == OR_XXZ) {
+    for (int i = 0; i < n; ++i) b.push_back({i, (i + 1) % n});
+  } else {
+    for (int i = 0; i < n; ++i) b.push_back({i, (i + 1) % n});
+    for (int i = 0; i < n; ++i) b.push_back({i, (i + 2) % n});
+  }
+  return b;
+}
+
+// The diagonal + ratio accumulation of the E_loc functors, given the ratio of
+// every bond (ratio[b] is read only where the functor's prefactor is used).
+double eloc_combine(const or_ham* H, const int8_t* s, const double* ratio) {
+  const int n = H->n;
+  double e = 0.0;
+  if (H->kind == OR_TFI) {  // TFIsing.hpp:31-38
+    for (int i = 0; i < n; ++i) {
+      e += H->J * (double)s[i] * (double)s[(i + 1) % n];
+      e += H->h * ratio[i];
+    }
+  } else if (H->kind == OR_XXZ) {  // XXZ.hpp:37-49
+    const double sign = H->sign_rule ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) {
+      const int zz = (int)s[i] * (int)s[(i + 1) % n];
+      e += H->J * H->delta * zz;
+      e += sign * H->J * (1 - zz) * ratio[i];
+    }
+  } else {  // J1J2.hpp:35-55
+    const int sign = H->sign_rule ? -1 : 1;
+    for (int i = 0; i < n; ++i) {
+      const double yy = -(double)((int)s[i] * (int)s[(i + 1) % n]);
+      e += -H->J * yy;
+      e += H->J * sign * (1.0 + yy) * ratio[i];
+    }
+    for (int i = 0; i < n; ++i) {
+      const double yy = -(double)((int)s[i] * (int)s[(i + 2) % n]);
+      e += -H->J2 * yy;
+      e += H->J2 * (1.0 + yy) * ratio[n + i];
+    }
+  }
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ============================================================ restatement
+// Utilities/Utility.hpp:15-24
+double or_logcosh(double x) {
+  const double xp = fabs(x);
+  if (xp <= 12.) return log(cosh(xp));
+  return xp - log(2.);
+}
+
+// Machines/RBM.hpp:137-141 (theta = W sigma + b; sum over i in index order)
+void or_theta(int n, int m, const double* W, const double* b,
+              const int8_t* sigma, double* theta) {
+  for (int j = 0; j < m; ++j) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += Wat(W, m, j, i) * sgn(sigma[i]);
+    theta[j] = acc + b[j];
+  }
+}
+
+// States/RBMState.hpp:48-63
+double or_log_ratio1(int n, int m, const double* W, const double* a,
+                     const int8_t* sigma, const double* theta, int k) {
+  (void)n;
+  double res = -2.0 * a[k] * sgn(sigma[k]);
+  for (int j = 0; j < m; ++j)
+    res += or_logcosh(theta[j] - 2.0 * sgn(sigma[k]) * Wat(W, m, j, k)) -
+           or_logcosh(theta[j]);
+  return res;
+}
+
+// States/RBMState.hpp:78-93
+double or_log_ratio2(int n, int m, const double* W, const double* a,
+                     const int8_t* sigma, const double* theta, int k, int l) {
+  (void)n;
+  double res = -2.0 * a[k] * sgn(sigma[k]) - 2.0 * a[l] * sgn(sigma[l]);
+  for (int j = 0; j < m; ++j) {
+    const double t = theta[j] - 2.0 * sgn(sigma[k]) * Wat(W, m, j, k) -
+                     2.0 * sgn(sigma[l]) * Wat(W, m, j, l);
+    res += or_logcosh(t) - or_logcosh(theta[j]);
+  }
+  return res;
+}
+
+// Machines/RBM.hpp:367-378
+double or_log_coeff(int n, int m, const double* a, const int8_t* sigma,
+                    const double* theta) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += a[i] * sgn(sigma[i]);
+  for (int j = 0; j < m; ++j) s += or_logcosh(theta[j]);
+  return s;
+}
+
+// Machines/RBM.hpp:390-406
+void or_logderiv(int n, int m, int use_bias, const int8_t* sigma,
+                 const double* theta, double* out) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < m; ++j)
+      out[(size_t)i * m + j] = sgn(sigma[i]) * tanh(theta[j]);
+  if (!use_bias) return;
+  for (int i = 0; i < n; ++i) out[(size_t)n * m + i] = sgn(sigma[i]);
+  for (int j = 0; j < m; ++j) out[(size_t)n * m + n + j] = tanh(theta[j]);
+}
+
+// Hamiltonian E_loc functors with ratio = exp(logRatio): TFIsing.hpp:28-39,
+// XXZ.hpp:37-49, J1J2.hpp:35-55; ratio: RBMState.hpp:65-68, 95-98.
+double or_eloc(const or_ham* H, int n, int m, const double* W, const double* a,
+               const int8_t* sigma, const double* theta) {
+  std::vector<Bond> bl = bonds_of(H);
+  std::vector<double> ratio(bl.size());
+  for (size_t b = 0; b < bl.size(); ++b) {
+    if (bl[b].l < 0)
+      ratio[b] = exp(or_log_ratio1(n, m, W, a, sigma, theta, bl[b].k));
+    else
+      ratio[b] =
+          exp(or_log_ratio2(n, m, W, a, sigma, theta, bl[b].k, bl[b].l));
+  }
+  return eloc_combine(H, sigma, ratio.data());
+}
+
+// Column functors for exact diagonalisation: TFIsing.hpp:53-65,
+// XXZ.hpp:51-64, J1J2.hpp:57-79.  Basis bit i set <=> sigma_i = -1.
+int or_ham_column(const or_ham* H, uint32_t col, uint32_t* rows, double* vals,
+                  int cap) {
+  std::vector<std::pair<uint32_t, double>> e;
+  auto add = [&](uint32_t r, double v) {
+    for (auto& p : e)
+      if (p.first == r) {
+        p.second += v;
+        return;
+      }
+    e.push_back({r, v});
+  };
+  const int n = H->n;
+  if (H->kind == OR_TFI) {
+    for (int i = 0; i < n; ++i) {
+      const int s1 = (col >> i) & 1, s2 = (col >> ((i + 1) % n)) & 1;
+      add(col ^ (1u << i), H->h);
+      add(col, H->J * (1 - 2 * s1) * (1 - 2 * s2));
+    }
+  } else if (H->kind == OR_XXZ) {
+    const double sign = H->sign_rule ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) {
+      const int b1 = (col >> i) & 1, b2 = (col >> ((i + 1) % n)) & 1;
+      const int zz = (1 - 2 * b1) * (1 - 2 * b2);
+      add(col, H->J * H->delta * zz);
+      add(col ^ ((1u << i) | (1u << ((i + 1) % n))), sign * H->J * (1 - zz));
+    }
+  } else {
+    const int sign = H->sign_rule ? -1 : 1;
+    for (int d = 1; d <= 2; ++d) {
+      const double Jd = d == 1 ? H->J : H->J2;
+      const double sg = d == 1 ? sign : 1;
+      for (int i = 0; i < n; ++i) {
+        const int b1 = (col >> i) & 1, b2 = (col >> ((i + d) % n)) & 1;
+        const int zz = (1 - 2 * b1) * (1 - 2 * b2);
+        add(col ^ ((1u << i) | (1u << ((i + d) % n))), Jd * sg * (1.0 - zz));
+        add(col, Jd * zz);
+      }
+    }
+  }
+  const int cnt = (int)e.size();
+  for (int q = 0; q < cnt && q < cap; ++q) {
+    rows[q] = e[q].first;
+    vals[q] = e[q].second;
+  }
+  return cnt;
+}
+
+// GroundState/utils.hpp:13-51 (O rows), SRMat.hpp:89-103 (mean, centring,
+// F = O_c^T E / Ns), SRMat.hpp:105-118 (weighted variant), SRMat.hpp:142-149
+// (S = O_c^T O_c / Ns or O_c^T diag(w) O_c).
+void or_sr_build(int ns, int n, int m, int use_bias, const int8_t* sigma,
+                 const double* theta, const double* eloc, const double* weights,
+                 double* O, double* omean, double* F, double* S) {
+  const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
+  for (int s = 0; s < ns; ++s)
+    or_logderiv(n, m, use_bias, sigma + (size_t)s * n, theta + (size_t)s * m,
+                O + (size_t)s * dim);
+  for (size_t c = 0; c < dim; ++c) {
+    double acc = 0.0;
+    if (weights)
+      for (int s = 0; s < ns; ++s) acc += weights[s] * O[(size_t)s * dim + c];
+    else {
+      for (int s = 0; s < ns; ++s) acc += O[(size_t)s * dim + c];
+      acc /= ns;
+    }
+    omean[c] = acc;
+  }
+  for (int s = 0; s < ns; ++s)
+    for (size_t c = 0; c < dim; ++c) O[(size_t)s * dim + c] -= omean[c];
+  for (size_t c = 0; c < dim; ++c) {
+    double acc = 0.0;
+    for (int s = 0; s < ns; ++s)
+      acc += O[(size_t)s * dim + c] * eloc[s] * (weights ? weights[s] : 1.0);
+    F[c] = weights ? acc : acc / ns;
+  }
+  if (!S) return;
+  for (size_t c = 0; c < dim; ++c)
+    for (size_t d = 0; d <= c; ++d) {
+      double acc = 0.0;
+      for (int s = 0; s < ns; ++s)
+        acc += O[(size_t)s * dim + c] * O[(size_t)s * dim + d] *
+               (weights ? weights[s] : 1.0);
+      if (!weights) acc /= ns;
+      S[c * dim + d] = acc;
+      S[d * dim + c] = acc;
+    }
+}
+
+// Runners/RunRBM.hpp:197-201 / SRMat.hpp:231-237: (S + shift I) x = F by a
+// Cholesky factorisation.  Returns 0, or the failing pivot + 1 if not SPD.
+int or_llt_solve(int dim, const double* S, double shift, const double* F,
+                 double* x) {
+  std::vector<double> L((size_t)dim * dim);
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j)
+      L[(size_t)i * dim + j] = S[(size_t)i * dim + j] + (i == j ? shift : 0.0);
+  for (int j = 0; j < dim; ++j) {
+    double d = L[(size_t)j * dim + j];
+    for (int k = 0; k < j; ++k) d -= L[(size_t)j * dim + k] * L[(size_t)j * dim + k];
+    if (!(d > 0.0)) return j + 1;
+    d = sqrt(d);
+    L[(size_t)j * dim + j] = d;
+    for (int i = j + 1; i < dim; ++i) {
+      double v = L[(size_t)i * dim + j];
+      for (int k = 0; k < j; ++k) v -= L[(size_t)i * dim + k] * L[(size_t)j * dim + k];
+      L[(size_t)i * dim + j] = v / d;
+    }
+  }
+  std::vector<double> y(dim);
+  for (int i = 0; i < dim; ++i) {
+    double v = F[i];
+    for (int k = 0; k < i; ++k) v -= L[(size_t)i * dim + k] * y[k];
+    y[i] = v / L[(size_t)i * dim + i];
+  }
+  for (int i = dim - 1; i >= 0; --i) {
+    double v = y[i];
+    for (int k = i + 1; k < dim; ++k) v -= L[(size_t)k * dim + i] * x[k];
+    x[i] = v / L[(size_t)i * dim + i];
+  }
+  return 0;
+}
+
+namespace {
+// SRMat::apply, SRMat.hpp:170-189: O_c^T (O_c v) / Ns + shift v (or the
+// weighted form O_c^T diag(w) O_c v + shift v).
+void sr_apply(int ns, int dim, const double* Oc, const double* w, double shift,
+              const double* v, double* out, std::vector<double>& r) {
+  r.assign(ns, 0.0);
+  for (int s = 0; s < ns; ++s) {
+    double acc = 0.0;
+    for (int c = 0; c < dim; ++c) acc += Oc[(size_t)s * dim + c] * v[c];
+    r[s] = w ? acc * w[s] : acc / ns;
+  }
+  for (int c = 0; c < dim; ++c) out[c] = 0.0;
+  for (int s = 0; s < ns; ++s)
+    for (int c = 0; c < dim; ++c) out[c] += Oc[(size_t)s * dim + c] * r[s];
+  for (int c = 0; c < dim; ++c) out[c] += shift * v[c];
+}
+double dot(int d, const double* a, const double* b) {
+  double s = 0.0;
+  for (int i = 0; i < d; ++i) s += a[i] * b[i];
+  return s;
+}
+}  // namespace
+
+// SRMat::solveCG, SRMat.hpp:197-223: unpreconditioned conjugate gradients
+// from x = 0, stopping when |r| <= tol |b| (Eigen's ConjugateGradient
+// criterion), at most max_iter iterations.  Returns 0 on convergence, 1 if
+// max_iter was reached.
+int or_cg_solve(int ns, int dim, const double* Oc, const double* weights,
+                double shift, const double* b, double tol, int max_iter,
+                double* x, int* iters, double* err) {
+  std::vector<double> r(b, b + dim), p(dim), Ap(dim), tmp;
+  for (int i = 0; i < dim; ++i) x[i] = 0.0;
+  const double bn2 = dot(dim, b, b);
+  if (bn2 == 0.0) {
+    *iters = 0;
+    *err = 0.0;
+    return 0;
+  }
+  const double thr = tol * tol * bn2;
+  double rn2 = bn2;
+  p = r;
+  double absNew = rn2;
+  int it = 0;
+  while (rn2 >= thr && it < max_iter) {
+    sr_apply(ns, dim, Oc, weights, shift, p.data(), Ap.data(), tmp);
+    const double alpha = absNew / dot(dim, p.data(), Ap.data());
+    for (int i = 0; i < dim; ++i) x[i] += alpha * p[i];
+    for (int i = 0; i < dim; ++i) r[i] -= alpha * Ap[i];
+    rn2 = dot(dim, r.data(), r.data());
+    ++it;
+    if (rn2 < thr) break;
+    const double absOld = absNew;
+    absNew = rn2;
+    const double beta = absNew / absOld;
+    for (int i = 0; i < dim; ++i) p[i] = r[i] + beta * p[i];
+  }
+  *iters = it;
+  *err = sqrt(rn2 / bn2);
+  return rn2 < thr ? 0 : 1;
+}
+
+// Optimizers/SGD.hpp:48-54: update = -max(alpha / t^p, 1e-4) v, t counted
+// from 1 at the first call.
+void or_sgd_update(int dim, double alpha, double p, int t, const double* v,
+                   double* out) {
+  const double eta = std::max(alpha / pow((double)t, p), 1e-4);
+  for (int i = 0; i < dim; ++i) out[i] = -eta * v[i];
+}
+
+// ============================================================ replay
+void rp_theta(int n, int m, const double* W, const double* b,
+              const int8_t* sigma, double* theta) {
+  // theta_j = (sum_i sigma_i W_ji in index order, as +/-W) + b_j: the order
+  // every theta-refresh kernel uses.
+  for (int j = 0; j < m; ++j) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i)
+      acc = sigma[i] > 0 ? acc + Wat(W, m, j, i) : acc - Wat(W, m, j, i);
+    theta[j] = acc + b[j];
+  }
+}
+double rp_logcosh_d(double x) { return yq::logcosh_d(x); }
+float rp_logcosh_f(float x) { return yq::logcosh_f(x); }
+double rp_exp_d(double x) { return yq::exp_d(x); }
+float rp_exp_f(float x) { return yq::exp_f(x); }
+void rp_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+               uint32_t k1, uint32_t* out4) {
+  yq::u32x4 c{c0, c1, c2, c3};
+  yq::u32x4 r = yq::philox4x32_10(c, k0, k1);
+  out4[0] = r.x;
+  out4[1] = r.y;
+  out4[2] = r.z;
+  out4[3] = r.w;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Canonical reduction of one chain's per-hidden-unit terms: lane l = j % L
+// sums its terms in increasing j, then lanes are combined by a tree with
+// strides L/2, L/4, ..., 1 (what an xor-butterfly over L lanes produces).
+template <typename R>
+R tree_sum(std::vector<R>& acc, int L) {
+  for (int s = L / 2; s >= 1; s /= 2)
+    for (int l = 0; l < s; ++l) acc[l] = acc[l] + acc[l + s];
+  return acc[0];
+}
+
+template <typename R>
+struct ChainState {
+  std::vector<R> theta, lc, Wr, ar;  // working precision copies
+};
+
+// One chain of the sampler (LocalSweeper.hpp:32-53 / SwapSweeper.hpp:28-56
+// semantics, SamplerMT.hpp:180-204 schedule) with the kernel's arithmetic.
+template <typename R>
+void replay_chain(const rp_cfg* cfg, uint32_t stream, const double* W,
+                  const double* a, const double* b, int8_t* sig, uint64_t* ctr,
+                  uint64_t* acc, int n_sweeps, int n_discard, int C, int chain,
+                  int8_t* out_sigma, double* out_theta) {
+  const int n = cfg->n, m = cfg->m, L = cfg->lanes;
+  const bool mixed = cfg->precision == 1;
+  std::vector<R> Wr((size_t)n * m), ar(n), th(m), lc(m), t(m), lt(m),
+      part(L);
+  for (size_t q = 0; q < (size_t)n * m; ++q) Wr[q] = (R)W[q];
+  for (int i = 0; i < n; ++i) ar[i] = (R)a[i];
+  std::vector<double> th64(m);
+  auto refresh = [&]() {
+    rp_theta(n, m, W, b, sig, th64.data());
+    for (int j = 0; j < m; ++j) {
+      th[j] = (R)th64[j];
+      lc[j] = yq::Math<R>::logcosh(th[j]);
+    }
+  };
+  refresh();
+  const int nprop = n * cfg->sweep_factor;
+  const R beta = (R)cfg->beta;
+  for (int sw = 0; sw < n_discard + n_sweeps; ++sw) {
+    if (mixed && sw > 0) refresh();
+    for (int p = 0; p < nprop; ++p) {
+      const yq::u32x4 x = yq::draw(cfg->seed, (*ctr)++, stream, yq::kDomSweep);
+      const int k = (int)yq::index_from(x.x, (uint32_t)n);
+      int l = -1;
+      R cst;
+      if (cfg->kind == 1) {
+        l = (int)yq::index_from(x.y, (uint32_t)n);
+        if (sig[k] == sig[l]) continue;
+        cst = ((R)-2 * ar[k]) * (R)sig[k] - ((R)2 * ar[l]) * (R)sig[l];
+      } else {
+        cst = ((R)-2 * ar[k]) * (R)sig[k];
+      }
+      for (int q = 0; q < L; ++q) part[q] = (R)0;
+      for (int j = 0; j < m; ++j) {
+        R tj = th[j] - (R)(2 * sig[k]) * Wr[(size_t)k * m + j];
+        if (l >= 0) tj = tj - (R)(2 * sig[l]) * Wr[(size_t)l * m + j];
+        t[j] = tj;
+        lt[j] = yq::Math<R>::logcosh(tj);
+        part[j % L] = part[j % L] + (lt[j] - lc[j]);
+      }
+      const R delta = cst + tree_sum(part, L);
+      bool ok;
+      if (mixed)
+        ok = yq::accept_f((float)beta, (float)delta, yq::u01_float(x.z));
+      else
+        ok = yq::accept_d((double)beta, (double)delta, yq::u01_double(x.z, x.w));
+      if (ok) {
+        for (int j = 0; j < m; ++j) {
+          th[j] = t[j];
+          lc[j] = lt[j];
+        }
+        sig[k] = -sig[k];
+        if (l >= 0) sig[l] = -sig[l];
+        ++(*acc);
+      }
+    }
+    if (sw >= n_discard) {
+      const size_t row = (size_t)(sw - n_discard) * C + chain;
+      memcpy(out_sigma + row * n, sig, n);
+      if (mixed) {
+        rp_theta(n, m, W, b, sig, th64.data());
+        for (int j = 0; j < m; ++j) out_theta[row * m + j] = (double)(float)th64[j];
+      } else {
+        for (int j = 0; j < m; ++j) out_theta[row * m + j] = (double)th[j];
+      }
+    }
+  }
+}
+
+int sample_range(const rp_cfg* cfg, int C, const uint32_t* stream,
+                 const double* W, const double* a, const double* b,
+                 int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
+                 int n_discard, int8_t* out_sigma, double* out_theta, int c0,
+                 int c1) {
+  for (int c = c0; c < c1; ++c) {
+    int8_t* sig = sigma + (size_t)c * cfg->n;
+    if (cfg->precision == 1)
+      replay_chain<float>(cfg, stream[c], W, a, b, sig, ctr + c, acc + c,
+                          n_sweeps, n_discard, C, c, out_sigma, out_theta);
+    else
+      replay_chain<double>(cfg, stream[c], W, a, b, sig, ctr + c, acc + c,
+                           n_sweeps, n_discard, C, c, out_sigma, out_theta);
+  }
+  return 0;
+}
+
+// E_loc of one sample with the kernel arithmetic: each bond's log-ratio is
+// summed sequentially over j in the reference order (RBMState.hpp:54-62),
+// then the functor's accumulation in double.
+template <typename R>
+double eloc_one(const or_ham* H, const std::vector<Bond>& bl, int n, int m,
+                const std::vector<R>& Wr, const std::vector<R>& ar,
+                const int8_t* s, const double* theta64, std::vector<R>& th,
+                std::vector<R>& lc, std::vector<double>& ratio) {
+  for (int j = 0; j < m; ++j) {
+    th[j] = (R)theta64[j];
+    lc[j] = yq::Math<R>::logcosh(th[j]);
+  }
+  for (size_t b = 0; b < bl.size(); ++b) {
+    const int k = bl[b].k, l = bl[b].l;
+    if (l >= 0 && s[k] == s[l]) {
+      ratio[b] = 0.0;
+      continue;
+    }
+    R res = ((R)-2 * ar[k]) * (R)s[k];
+    if (l >= 0) res = res - ((R)2 * ar[l]) * (R)s[l];
+    for (int j = 0; j < m; ++j) {
+      R tj = th[j] - (R)(2 * s[k]) * Wr[(size_t)k * m + j];
+      if (l >= 0) tj = tj - (R)(2 * s[l]) * Wr[(size_t)l * m + j];
+      res = res + (yq::Math<R>::logcosh(tj) - lc[j]);
+    }
+    ratio[b] = (double)yq::Math<R>::exp(res);
+  }
+  (void)n;
+  return eloc_combine(H, s, ratio.data());
+}
+
+template <typename R>
+void eloc_range(const or_ham* H, int n, int m, const double* W,
+                const double* a, const int8_t* sigma, const double* theta,
+                double* eloc, int s0, int s1) {
+  std::vector<Bond> bl = bonds_of(H);
+  std::vector<R> Wr((size_t)n * m), ar(n), th(m), lc(m);
+  for (size_t q = 0; q < (size_t)n * m; ++q) Wr[q] = (R)W[q];
+  for (int i = 0; i < n; ++i) ar[i] = (R)a[i];
+  std::vector<double> ratio(bl.size());
+  for (int s = s0; s < s1; ++s)
+    eloc[s] = eloc_one<R>(H, bl, n, m, Wr, ar, sigma + (size_t)s * n,
+                          theta + (size_t)s * m, th, lc, ratio);
+}
+
+template <typename F>
+void parallel_for(int total, int nthreads, F&& f) {
+  if (nthreads <= 1 || total <= 1) {
+    f(0, total);
+    return;
+  }
+  std::vector<std::thread> ts;
+  const int chunk = (total + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    const int lo = t * chunk, hi = std::min(total, lo + chunk);
+    if (lo >= hi) break;
+    ts.emplace_back([&f, lo, hi]() { f(lo, hi); });
+  }
+  for (auto& th : ts) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int rp_sample(const rp_cfg* cfg, int C, const uint32_t* stream,
+              const double* W, const double* a, const double* b,
+              int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
+              int n_discard, int8_t* out_sigma, double* out_theta) {
+  if (cfg->lanes <= 0 || (cfg->lanes & (cfg->lanes - 1))) return 1;
+  return sample_range(cfg, C, stream, W, a, b, sigma, ctr, acc, n_sweeps,
+                      n_discard, out_sigma, out_theta, 0, C);
+}
+
+void rp_eloc(const or_ham* H, int precision, int ns, int n, int m,
+             const double* W, const double* a, const int8_t* sigma,
+             const double* theta, double* eloc) {
+  if (precision == 1)
+    eloc_range<float>(H, n, m, W, a, sigma, theta, eloc, 0, ns);
+  else
+    eloc_range<double>(H, n, m, W, a, sigma, theta, eloc, 0, ns);
+}
+
+// Initial configurations (SamplerMT::randomizeSigma, SamplerMT.hpp:194-211,
+// Utility.hpp:64-88): uniform +/-1 spins, or a uniformly random arrangement
+// with exactly nup spins at -1 (Fisher-Yates shuffle driven by Philox).
+void rp_randomize(uint64_t seed, uint64_t epoch, int C, const uint32_t* stream,
+                  int n, int nup, int8_t* sigma) {
+  for (int c = 0; c < C; ++c) {
+    int8_t* s = sigma + (size_t)c * n;
+    const uint64_t base = epoch << 20;
+    if (nup < 0) {
+      for (int i = 0; i < n; i += 128) {
+        const yq::u32x4 x =
+            yq::draw(seed, base + (uint64_t)(i / 128), stream[c], yq::kDomRandomize);
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+        for (int q = 0; q < 128 && i + q < n; ++q)
+          s[i + q] = ((w[q >> 5] >> (q & 31)) & 1u) ? -1 : 1;
+      }
+    } else {
+      for (int i = 0; i < n; ++i) s[i] = i < nup ? -1 : 1;
+      for (int i = n - 1; i > 0; --i) {
+        const yq::u32x4 x =
+            yq::draw(seed, base + (uint64_t)(n - 1 - i), stream[c], yq::kDomRandomize);
+        const int j = (int)yq::index_from(x.x, (uint32_t)(i + 1));
+        std::swap(s[i], s[j]);
+      }
+    }
+  }
+}
+
+// ============================================================ CPU baseline
+int bl_threads(void) {
+  const unsigned h = std::thread::hardware_concurrency();
+  return h ? (int)h : 1;
+}
+
+int bl_sample(const rp_cfg* cfg, int C, const uint32_t* stream,
+              const double* W, const double* a, const double* b,
+              int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
+              int n_discard, int8_t* out_sigma, double* out_theta,
+              int nthreads) {
+  parallel_for(C, nthreads, [&](int lo, int hi) {
+    sample_range(cfg, C, stream, W, a, b, sigma, ctr, acc, n_sweeps, n_discard,
+                 out_sigma, out_theta, lo, hi);
+  });
+  return 0;
+}
+
+void bl_eloc(const or_ham* H, int ns, int n, int m, const double* W,
+             const double* a, const int8_t* sigma, const double* theta,
+             double* eloc, int nthreads) {
+  parallel_for(ns, nthreads, [&](int lo, int hi) {
+    eloc_range<double>(H, n, m, W, a, sigma, theta, eloc, lo, hi);
+  });
+}
+
+void bl_sr_build(int ns, int n, int m, int use_bias, const int8_t* sigma,
+                 const double* theta, const double* eloc, double* O,
+                 double* omean, double* F, int nthreads) {
+  const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
+  parallel_for(ns, nthreads, [&](int lo, int hi) {
+    for (int s = lo; s < hi; ++s)
+      or_logderiv(n, m, use_bias, sigma + (size_t)s * n, theta + (size_t)s * m,
+                  O + (size_t)s * dim);
+  });
+  // column blocks: mean, centring and F
+  parallel_for((int)dim, nthreads, [&](int lo, int hi) {
+    for (int c = lo; c < hi; ++c) omean[c] = 0.0;
+    for (int s = 0; s < ns; ++s)
+      for (int c = lo; c < hi; ++c) omean[c] += O[(size_t)s * dim + c];
+    for (int c = lo; c < hi; ++c) omean[c] /= ns;
+    for (int c = lo; c < hi; ++c) F[c] = 0.0;
+    for (int s = 0; s < ns; ++s)
+      for (int c = lo; c < hi; ++c) {
+        O[(size_t)s * dim + c] -= omean[c];
+        F[c] += O[(size_t)s * dim + c] * eloc[s];
+      }
+    for (int c = lo; c < hi; ++c) F[c] /= ns;
+  });
+}
+
+int bl_cg_solve(int ns, int dim, const double* Oc, double shift,
+                const double* b, double tol, int max_iter, double* x,
+                int* iters, double* err, int nthreads) {
+  std::vector<double> r(b, b + dim), p(dim), Ap(dim), y(ns);
+  auto apply = [&](const double* v, double* out) {
+    parallel_for(ns, nthreads, [&](int lo, int hi) {
+      for (int s = lo; s < hi; ++s) {
+        double acc = 0.0;
+        const double* row = Oc + (size_t)s * dim;
+        for (int c = 0; c < dim; ++c) acc += row[c] * v[c];
+        y[s] = acc / ns;
+      }
+    });
+    parallel_for(dim, nthreads, [&](int lo, int hi) {
+      for (int c = lo; c < hi; ++c) out[c] = shift * v[c];
+      for (int s = 0; s < ns; ++s) {
+        const double* row = Oc + (size_t)s * dim;
+        const double ys = y[s];
+        for (int c = lo; c < hi; ++c) out[c] += row[c] * ys;
+      }
+    });
+  };
+  for (int i = 0; i < dim; ++i) x[i] = 0.0;
+  const double bn2 = dot(dim, b, b);
+  if (bn2 == 0.0) {
+    *iters = 0;
+    *err = 0.0;
+    return 0;
+  }
+  const double thr = tol * tol * bn2;
+  double rn2 = bn2, absNew = bn2;
+  p = r;
+  int it = 0;
+  while (rn2 >= thr && it < max_iter) {
+    apply(p.data(), Ap.data());
+    const double alpha = absNew / dot(dim, p.data(), Ap.data());
+    for (int i = 0; i < dim; ++i) x[i] += alpha * p[i];
+    for (int i = 0; i < dim; ++i) r[i] -= alpha * Ap[i];
+    rn2 = dot(dim, r.data(), r.data());
+    ++it;
+    if (rn2 < thr) break;
+    const double beta = rn2 / absNew;
+    absNew = rn2;
+    for (int i = 0; i < dim; ++i) p[i] = r[i] + beta * p[i];
+  }
+  *iters = it;
+  *err = sqrt(rn2 / bn2);
+  return rn2 < thr ? 0 : 1;
+}
+
+}  // extern "C"

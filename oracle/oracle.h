@@ -1,0 +1,115 @@
+// CPU oracle for the VMC / SR step.  TEST INFRASTRUCTURE ONLY: it is loaded
+// by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
+// always as the checker or the timed CPU baseline, never by the library.
+//
+// Two layers:
+//  * or_*  restate the reference algorithm in double precision with libm and
+//          the reference's own summation order (citations per function).
+//  * rp_*  replay the exact arithmetic the gfx950 kernels perform (canonical
+//          lane-strided + tree reductions, the replayable exp/logcosh of
+//          yannq_amd/csrc/yq_math.h, the same Philox counters), so spin
+//          configurations, accept decisions and local energies can be
+//          compared bit for bit.
+//  * bl_*  the multi-threaded CPU baseline (std::thread over chains and
+//          samples, mirroring the reference's TBB parallel_for sites).
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_TFI = 0, OR_XXZ = 1, OR_J1J2 = 2 };
+
+typedef struct {
+  int kind;        // OR_TFI / OR_XXZ / OR_J1J2
+  int n;           // sites (periodic chain)
+  double J;        // TFI: J; XXZ: J; J1J2: J1
+  double h;        // TFI: h (coefficient of the ratio term)
+  double delta;    // XXZ: Delta
+  double J2;       // J1J2: J2
+  int sign_rule;   // XXZ / J1J2 sign rule (1 -> sign = -1)
+} or_ham;
+
+// ---- restatement (reference order, libm) ---------------------------------
+double or_logcosh(double x);
+void or_theta(int n, int m, const double* W, const double* b,
+              const int8_t* sigma, double* theta);
+double or_log_ratio1(int n, int m, const double* W, const double* a,
+                     const int8_t* sigma, const double* theta, int k);
+double or_log_ratio2(int n, int m, const double* W, const double* a,
+                     const int8_t* sigma, const double* theta, int k, int l);
+double or_log_coeff(int n, int m, const double* a, const int8_t* sigma,
+                    const double* theta);
+void or_logderiv(int n, int m, int use_bias, const int8_t* sigma,
+                 const double* theta, double* out);
+double or_eloc(const or_ham* H, int n, int m, const double* W, const double* a,
+               const int8_t* sigma, const double* theta);
+int or_ham_column(const or_ham* H, uint32_t col, uint32_t* rows, double* vals,
+                  int cap);
+// O (ns x dim row-major), centred in place; omean (dim); F (dim);
+// S (dim x dim) if non-null; weights (ns) optional (null -> 1/ns, the
+// unweighted SRMat::constructFromSamples path).
+void or_sr_build(int ns, int n, int m, int use_bias, const int8_t* sigma,
+                 const double* theta, const double* eloc, const double* weights,
+                 double* O, double* omean, double* F, double* S);
+int or_llt_solve(int dim, const double* S, double shift, const double* F,
+                 double* x);
+int or_cg_solve(int ns, int dim, const double* Oc, const double* weights,
+                double shift, const double* b, double tol, int max_iter,
+                double* x, int* iters, double* err);
+void or_sgd_update(int dim, double alpha, double p, int t, const double* v,
+                   double* out);
+
+// ---- replay of the kernels' arithmetic -------------------------------------
+typedef struct {
+  int n, m, use_bias;
+  int kind;          // 0 local single-spin flips, 1 exchange (swap) moves
+  int precision;     // 0 fp64, 1 mixed (fp32 theta refreshed from fp64)
+  int lanes;         // reduction width of one chain (power of two)
+  int sweep_factor;  // proposals per sweep = n * sweep_factor
+  double beta;
+  uint64_t seed;
+} rp_cfg;
+
+void rp_theta(int n, int m, const double* W, const double* b,
+              const int8_t* sigma, double* theta);
+double rp_logcosh_d(double x);
+float rp_logcosh_f(float x);
+double rp_exp_d(double x);
+float rp_exp_f(float x);
+void rp_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+               uint32_t k1, uint32_t* out4);
+// chains: C; stream ids; W,a,b fp64 master parameters; sigma C x n in/out;
+// ctr, acc per chain in/out; samples (n_sweeps*C rows) out.
+int rp_sample(const rp_cfg* cfg, int C, const uint32_t* stream,
+              const double* W, const double* a, const double* b,
+              int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
+              int n_discard, int8_t* out_sigma, double* out_theta);
+// Local energies with the kernel's arithmetic. precision 0: fp64, 1: fp32.
+void rp_eloc(const or_ham* H, int precision, int ns, int n, int m,
+             const double* W, const double* a, const int8_t* sigma,
+             const double* theta, double* eloc);
+void rp_randomize(uint64_t seed, uint64_t epoch, int C, const uint32_t* stream,
+                  int n, int nup, int8_t* sigma);
+
+// ---- multi-threaded CPU baseline --------------------------------------------
+int bl_threads(void);
+int bl_sample(const rp_cfg* cfg, int C, const uint32_t* stream,
+              const double* W, const double* a, const double* b,
+              int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
+              int n_discard, int8_t* out_sigma, double* out_theta,
+              int nthreads);
+void bl_eloc(const or_ham* H, int ns, int n, int m, const double* W,
+             const double* a, const int8_t* sigma, const double* theta,
+             double* eloc, int nthreads);
+void bl_sr_build(int ns, int n, int m, int use_bias, const int8_t* sigma,
+                 const double* theta, const double* eloc, double* O,
+                 double* omean, double* F, int nthreads);
+int bl_cg_solve(int ns, int dim, const double* Oc, double shift,
+                const double* b, double tol, int max_iter, double* x,
+                int* iters, double* err, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,253 @@
+"""yannq_amd: the VMC / SR step of yannq on one AMD Instinct MI355X (gfx950).
+
+The compute path is the in-tree HIP library ``libyannq_amd.so`` behind the C
+ABI declared in ``include/yannq_amd.h``; this module is a thin ctypes binding
+of that ABI (used by the tests and the benchmark).  There is no CPU fallback:
+if the library is missing or no GPU is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libyannq_amd.so")
+HEADER = os.path.join(HERE, "..", "include", "yannq_amd.h")
+
+# enums of include/yannq_amd.h
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NAN, ERR_NOT_SPD, ERR_NOT_CONVERGED, ERR_OOM, \
+    ERR_UNSUPPORTED = range(9)
+TFI, XXZ, J1J2 = 0, 1, 2
+LOCAL, EXCHANGE = 0, 1
+FP64, MIXED = 0, 1
+SOLVE_EXPLICIT, SOLVE_CG, SOLVE_CG_FACTORED = 0, 1, 2
+SR_F, SR_OMEAN, SR_O, SR_S, SR_ELOC = range(5)
+
+STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_STATE", 3: "ERR_HIP", 4: "ERR_NAN",
+                5: "ERR_NOT_SPD", 6: "ERR_NOT_CONVERGED", 7: "ERR_OOM", 8: "ERR_UNSUPPORTED"}
+
+
+class YqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class HamDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n", C.c_int), ("J", C.c_double), ("h", C.c_double),
+                ("delta", C.c_double), ("J2", C.c_double), ("sign_rule", C.c_int)]
+
+
+class StepCfg(C.Structure):
+    _fields_ = [("n_sweeps", C.c_int), ("n_discard", C.c_int), ("randomize", C.c_int),
+                ("nup", C.c_int), ("solver", C.c_int), ("precision", C.c_int),
+                ("lambda_", C.c_double), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int),
+                ("lr", C.c_double), ("grad_clip", C.c_double)]
+
+
+class StepOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("energy_var", C.c_double),
+                ("update_norm", C.c_double), ("cg_err", C.c_double), ("cg_iters", C.c_int),
+                ("n_samples", C.c_int), ("sample_ms", C.c_double), ("solve_ms", C.c_double)]
+
+
+_lib = None
+
+
+def header_symbols():
+    """Names of every entry point declared in include/yannq_amd.h."""
+    import re
+    txt = open(HEADER).read()
+    return re.findall(r"YQ_API\s+[^;(]*?\b(yq_\w+)\s*\(", txt)
+
+
+def lib():
+    """Load libyannq_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m yannq_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    i, d, p, u64 = C.c_int, C.c_double, C.c_void_p, C.c_uint64
+    sig = {
+        "yq_ctx_create": [i, C.POINTER(p)], "yq_ctx_destroy": [p],
+        "yq_rbm_create": [p, i, i, i, i], "yq_rbm_dims": [p, p, p, p],
+        "yq_rbm_set_params": [p, p], "yq_rbm_get_params": [p, p],
+        "yq_rbm_update_params": [p, p], "yq_rbm_init_random": [p, u64, d],
+        "yq_rbm_has_nan": [p, p], "yq_ham_set": [p, p],
+        "yq_sampler_config": [p, i, i, i, i, i, u64], "yq_sampler_randomize": [p, i],
+        "yq_sampler_set_sigma": [p, p], "yq_sampler_get_state": [p, p, p, p],
+        "yq_sample": [p, i, i, p], "yq_samples_get": [p, p, p],
+        "yq_samples_set": [p, i, p, i], "yq_eloc": [p, p, p, p],
+        "yq_sr_build": [p, i, i], "yq_sr_get": [p, i, p], "yq_sr_apply": [p, d, p, p],
+        "yq_sr_solve": [p, d, d, i, p, p, p, p], "yq_sr_step": [p, p, p],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i
+    L.yq_last_error.argtypes = [p]
+    L.yq_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """One device context (include/yannq_amd.h); methods mirror the ABI."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.yq_ctx_create(device, C.byref(h))
+        if rc:
+            raise YqError(rc, "yq_ctx_create failed (is a gfx950 GPU visible?)")
+        self._h = h
+        self.n = self.m = self.dim = 0
+        self.chains = 0
+        self.ns = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.yq_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc, allow=()):
+        if rc and rc not in allow:
+            raise YqError(rc, self._L.yq_last_error(self._h).decode())
+        return rc
+
+    # machine
+    def rbm_create(self, n, m, use_bias=True):
+        self._chk(self._L.yq_rbm_create(self._h, 1, n, m, int(use_bias)))
+        self.n, self.m = n, m
+        self.dim = n * m + (n + m if use_bias else 0)
+
+    def set_params(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.dim,)
+        self._chk(self._L.yq_rbm_set_params(self._h, _ptr(v)))
+
+    def get_params(self):
+        out = np.empty(self.dim)
+        self._chk(self._L.yq_rbm_get_params(self._h, _ptr(out)))
+        return out
+
+    def update_params(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        self._chk(self._L.yq_rbm_update_params(self._h, _ptr(d)))
+
+    def init_random(self, seed, sigma):
+        self._chk(self._L.yq_rbm_init_random(self._h, seed, sigma))
+
+    def has_nan(self):
+        out = np.zeros(1, np.int32)
+        self._chk(self._L.yq_rbm_has_nan(self._h, _ptr(out)))
+        return bool(out[0])
+
+    def ham_set(self, kind, n, J=-1.0, h=-1.0, delta=1.0, J2=0.0, sign_rule=0):
+        d = HamDesc(kind, n, J, h, delta, J2, sign_rule)
+        self._chk(self._L.yq_ham_set(self._h, C.byref(d)))
+
+    # sampler
+    def sampler_config(self, sweeper, chains, precision=FP64, seed=1, sweep_factor=1, n_temps=1):
+        self._chk(self._L.yq_sampler_config(self._h, sweeper, n_temps, chains, sweep_factor,
+                                            precision, seed))
+        self.chains = chains
+
+    def randomize(self, nup=-1):
+        self._chk(self._L.yq_sampler_randomize(self._h, nup))
+
+    def set_sigma(self, sigma):
+        sigma = np.ascontiguousarray(sigma, dtype=np.int8)
+        assert sigma.shape == (self.chains, self.n)
+        self._chk(self._L.yq_sampler_set_sigma(self._h, _ptr(sigma)))
+
+    def sampler_state(self):
+        s = np.empty((self.chains, self.n), np.int8)
+        ctr = np.empty(self.chains, np.uint64)
+        acc = np.empty(self.chains, np.uint64)
+        self._chk(self._L.yq_sampler_get_state(self._h, _ptr(s), _ptr(ctr), _ptr(acc)))
+        return s, ctr, acc
+
+    def sample(self, n_sweeps, n_discard):
+        ns = C.c_int()
+        self._chk(self._L.yq_sample(self._h, n_sweeps, n_discard, C.byref(ns)))
+        self.ns = ns.value
+        return ns.value
+
+    def samples(self):
+        s = np.empty((self.ns, self.n), np.int8)
+        t = np.empty((self.ns, self.m))
+        self._chk(self._L.yq_samples_get(self._h, _ptr(s), _ptr(t)))
+        return s, t
+
+    def samples_set(self, sigma, precision=FP64):
+        sigma = np.ascontiguousarray(sigma, dtype=np.int8)
+        self._chk(self._L.yq_samples_set(self._h, sigma.shape[0], _ptr(sigma), precision))
+        self.ns = sigma.shape[0]
+
+    def eloc(self):
+        e = np.empty(self.ns)
+        mean, var = C.c_double(), C.c_double()
+        self._chk(self._L.yq_eloc(self._h, _ptr(e), C.byref(mean), C.byref(var)))
+        return e, mean.value, var.value
+
+    # SR
+    def sr_build(self, solver, precision=FP64):
+        self._chk(self._L.yq_sr_build(self._h, solver, precision))
+
+    def sr_get(self, item):
+        size = {SR_F: self.dim, SR_OMEAN: self.dim, SR_O: self.ns * self.dim,
+                SR_S: self.dim * self.dim, SR_ELOC: self.ns}[item]
+        out = np.empty(size)
+        self._chk(self._L.yq_sr_get(self._h, item, _ptr(out)))
+        if item == SR_O:
+            return out.reshape(self.ns, self.dim)
+        if item == SR_S:
+            return out.reshape(self.dim, self.dim)
+        return out
+
+    def sr_apply(self, shift, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty(self.dim)
+        self._chk(self._L.yq_sr_apply(self._h, shift, _ptr(v), _ptr(out)))
+        return out
+
+    def sr_solve(self, shift, tol=1e-10, max_iter=10000, rhs=None, allow_unconverged=False):
+        x = np.empty(self.dim)
+        res, it = C.c_double(), C.c_int()
+        r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
+        self._chk(self._L.yq_sr_solve(self._h, shift, tol, max_iter, _ptr(r), _ptr(x),
+                                      C.byref(res), C.byref(it)),
+                  allow=(ERR_NOT_CONVERGED,) if allow_unconverged else ())
+        return x, res.value, it.value
+
+    def sr_step(self, **kw):
+        cfg = StepCfg(kw.get("n_sweeps", 1), kw.get("n_discard", 0), int(kw.get("randomize", 0)),
+                      kw.get("nup", -1), kw.get("solver", SOLVE_CG_FACTORED),
+                      kw.get("precision", FP64), kw.get("lam", 1e-3), kw.get("cg_tol", 1e-8),
+                      kw.get("cg_max_iter", 20000), kw.get("lr", 0.01), kw.get("grad_clip", 0.0))
+        out = StepOut()
+        self._chk(self._L.yq_sr_step(self._h, C.byref(cfg), C.byref(out)),
+                  allow=(ERR_NOT_CONVERGED,) if kw.get("allow_unconverged") else ())
+        self.ns = out.n_samples
+        return {f: getattr(out, f) for f, _ in StepOut._fields_}

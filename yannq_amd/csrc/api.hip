@@ -1,0 +1,1076 @@
+// C-ABI implementation of include/yannq_amd.h: device state, argument
+// checks, the SR solvers and the epoch body.  Reference citations are paths
+// under the yannq source tree.
+#include <math.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mixed.h"
+#include "yq_internal.h"
+#include "yq_math.h"
+
+using namespace yq;
+
+// ---------------------------------------------------------------- profiling
+namespace {
+struct ProfEntry {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> open;
+  double total_ms = 0.0;
+  long long launches = 0;
+};
+struct ProfState {
+  std::vector<std::string> labels;
+  std::vector<ProfEntry> entries;
+  int find(const char* l) const {
+    for (size_t i = 0; i < labels.size(); ++i)
+      if (labels[i] == l) return (int)i;
+    return -1;
+  }
+};
+ProfState g_prof;
+bool g_prof_any = false;
+}  // namespace
+
+namespace yq {
+bool prof_enabled(const char* label) { return g_prof_any && g_prof.find(label) >= 0; }
+void prof_begin(const char* label, hipStream_t st) {
+  ProfEntry& e = g_prof.entries[g_prof.find(label)];
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  (void)hipEventRecord(ev, st);
+  e.open.push_back(ev);
+}
+void prof_end(const char* label, hipStream_t st) {
+  ProfEntry& e = g_prof.entries[g_prof.find(label)];
+  if (e.open.empty()) return;
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  (void)hipEventRecord(ev, st);
+  e.pending.push_back({e.open.back(), ev});
+  e.open.pop_back();
+}
+}  // namespace yq
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace
+
+struct yq_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  rocblas_handle blas = nullptr;
+  std::vector<Buf*> bufs;
+
+  // machine
+  bool have_rbm = false;
+  int n = 0, m = 0, use_bias = 1;
+  size_t dim = 0, nparam_alloc = 0;
+  Buf params, WT64, W32, WT32, a32;
+
+  // hamiltonian
+  bool have_ham = false;
+  yq_ham_desc ham{};
+
+  // sampler
+  bool have_sampler = false;
+  int skind = 0, chains = 0, sweep_factor = 1, sprec = 0;
+  uint64_t seed = 0, rand_epoch = 0;
+  Buf sig_state, ctr, acc;
+
+  // current batch
+  bool have_batch = false, have_eloc = false;
+  int ns = 0, bprec = 0;
+  Buf bsig, btheta, ratio, eloc, estat;
+  double e_mean = 0.0, e_var = 0.0;
+
+  // SR
+  bool have_sr = false;
+  int sr_solver = 0, sr_prec = 0;
+  Buf T64, T32, O, omean, F, S, M, partial, ecent, info;
+  Buf vx, vr, vp, vAp, vy, vtmp, vb, dotp;
+  yq::MixedWork mixed;
+
+  yq_ctx() {
+    for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &bsig,
+                   &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
+                   &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp})
+      bufs.push_back(b);
+  }
+};
+
+namespace {
+
+int fail(yq_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, x)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess)                                                   \
+      return fail(c, YQ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(c) HIPCHK(c, hipGetLastError())
+
+#define BLASCHK(c, x)                                                        \
+  do {                                                                       \
+    rocblas_status s_ = (x);                                                 \
+    if (s_ != rocblas_status_success)                                        \
+      return fail(c, YQ_ERR_HIP, std::string(#x) + ": " + rocblas_status_to_string(s_)); \
+  } while (0)
+
+#define TRY(x)                 \
+  do {                         \
+    int rc_ = (x);             \
+    if (rc_ != YQ_OK) return rc_; \
+  } while (0)
+
+int ensure(yq_ctx* c, Buf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.cap >= bytes) return YQ_OK;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    if (e != hipSuccess) return fail(c, YQ_ERR_HIP, "hipFree failed");
+  }
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return fail(c, YQ_ERR_OOM,
+                "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.cap = bytes;
+  return YQ_OK;
+}
+
+int blas(yq_ctx* c) {
+  if (c->blas) return YQ_OK;
+  BLASCHK(c, rocblas_create_handle(&c->blas));
+  BLASCHK(c, rocblas_set_stream(c->blas, c->st));
+  return YQ_OK;
+}
+
+int sync(yq_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return YQ_OK;
+}
+
+double* W64(yq_ctx* c) { return c->params.as<double>(); }
+double* a64(yq_ctx* c) { return c->params.as<double>() + (size_t)c->n * c->m; }
+double* b64(yq_ctx* c) { return c->params.as<double>() + (size_t)c->n * c->m + c->n; }
+
+// working copies derived from the fp64 master parameters
+int refresh_params(yq_ctx* c) {
+  const int n = c->n, m = c->m;
+  launch_transpose(W64(c), m, n, c->WT64.as<double>(), c->st);
+  launch_to_float(W64(c), c->W32.as<float>(), (size_t)n * m, c->st);
+  launch_transpose_f(W64(c), m, n, c->WT32.as<float>(), c->st);
+  launch_to_float(a64(c), c->a32.as<float>(), (size_t)n, c->st);
+  LAUNCHCHK(c);
+  c->have_eloc = false;
+  c->have_sr = false;
+  return YQ_OK;
+}
+
+int need_rbm(yq_ctx* c) {
+  if (!c) return YQ_ERR_ARG;
+  if (!c->have_rbm) return fail(c, YQ_ERR_STATE, "no machine: call yq_rbm_create first");
+  return YQ_OK;
+}
+
+// deterministic dot: device partials, host sum in block order
+int dot(yq_ctx* c, const double* x, const double* y, size_t n, double* out) {
+  const int nb = dot_blocks(n);
+  TRY(ensure(c, c->dotp, sizeof(double) * 512));
+  launch_dot(x, y, n, c->dotp.as<double>(), c->st);
+  LAUNCHCHK(c);
+  double h[512];
+  HIPCHK(c, hipMemcpyAsync(h, c->dotp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += h[i];
+  *out = s;
+  return YQ_OK;
+}
+
+// ---------------------------------------------------------------- E_loc
+int compute_eloc(yq_ctx* c) {
+  if (!c->have_batch) return fail(c, YQ_ERR_STATE, "no samples: call yq_sample or yq_samples_set");
+  if (!c->have_ham) return fail(c, YQ_ERR_STATE, "no Hamiltonian: call yq_ham_set");
+  if (c->ham.n != c->n) return fail(c, YQ_ERR_ARG, "Hamiltonian size differs from the machine's n");
+  const int B = num_bonds(c->ham);
+  TRY(ensure(c, c->ratio, sizeof(double) * (size_t)c->ns * B));
+  TRY(ensure(c, c->eloc, sizeof(double) * (size_t)c->ns));
+  TRY(ensure(c, c->estat, sizeof(double) * 2));
+  ElocArgs a;
+  a.sigma = c->bsig.as<int8_t>();
+  a.theta = c->btheta.p;
+  a.use_float = c->bprec == YQ_MIXED;
+  a.WT = a.use_float ? (const void*)c->WT32.p : (const void*)c->WT64.p;
+  a.ar = a.use_float ? (const void*)c->a32.p : (const void*)a64(c);
+  a.ratio = c->ratio.as<double>();
+  a.eloc = c->eloc.as<double>();
+  a.ns = c->ns;
+  a.n = c->n;
+  a.m = c->m;
+  a.B = B;
+  a.ham = c->ham;
+  const size_t rs = a.use_float ? 4 : 8;
+  const size_t lds = (size_t)(256 / B + 2) * c->m * 2 * rs + (size_t)(256 / B + 2) * c->n;
+  if (lds > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
+  launch_eloc(a, c->st);
+  LAUNCHCHK(c);
+  launch_eloc_stats(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+  LAUNCHCHK(c);
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, c->estat.p, sizeof(h), hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  c->e_mean = h[0];
+  c->e_var = h[1];
+  c->have_eloc = true;
+  return YQ_OK;
+}
+
+// ---------------------------------------------------------------- SR build
+size_t aug_partial_doubles(int ns, int n, int m) {
+  return (size_t)colsum_chunks(ns) * (n + 1) * (m + 1) + ns;
+}
+
+int sr_build(yq_ctx* c, int solver, int precision) {
+  if (!c->have_eloc) TRY(compute_eloc(c));
+  const int ns = c->ns, n = c->n, m = c->m;
+  const size_t dim = c->dim;
+  if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG &&
+      solver != YQ_SOLVE_CG_FACTORED)
+    return fail(c, YQ_ERR_ARG, "unknown solver");
+  if (precision != YQ_FP64 && precision != YQ_MIXED)
+    return fail(c, YQ_ERR_ARG, "unknown precision");
+  if (precision == YQ_MIXED && solver != YQ_SOLVE_CG_FACTORED)
+    return fail(c, YQ_ERR_UNSUPPORTED,
+                "mixed precision is implemented for the factored CG solver only");
+  TRY(ensure(c, c->omean, sizeof(double) * dim));
+  TRY(ensure(c, c->F, sizeof(double) * dim));
+  TRY(ensure(c, c->T64, sizeof(double) * (size_t)ns * m));
+  // T64 = tanh(theta) in double from the batch's theta (float batches promote)
+  if (c->bprec == YQ_MIXED) {
+    TRY(ensure(c, c->vtmp, sizeof(double) * (size_t)ns * m));
+    launch_float_to_double(c->btheta.as<float>(), c->vtmp.as<double>(), (size_t)ns * m, c->st);
+    launch_tanh(c->vtmp.p, c->T64.p, (size_t)ns * m, 0, c->st);
+  } else {
+    launch_tanh(c->btheta.p, c->T64.p, (size_t)ns * m, 0, c->st);
+  }
+  LAUNCHCHK(c);
+  if (solver == YQ_SOLVE_CG_FACTORED) {
+    TRY(ensure(c, c->partial, sizeof(double) * aug_partial_doubles(ns, n, m)));
+    TRY(ensure(c, c->ecent, sizeof(double) * ns));
+    launch_factored_mean64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias,
+                           c->omean.as<double>(), c->partial.as<double>(), c->st);
+    // F = O_c^T E / Ns = O^T (E - <E>) / Ns
+    HIPCHK(c, hipMemcpyAsync(c->ecent.p, c->eloc.p, sizeof(double) * ns,
+                             hipMemcpyDeviceToDevice, c->st));
+    launch_add_const(c->ecent.as<double>(), -c->e_mean, ns, c->st);
+    launch_scale_add(c->ecent.as<double>(), 1.0 / ns, c->ecent.as<double>(), 0.0, ns, c->st);
+    launch_factored_bwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias,
+                          c->ecent.as<double>(), c->F.as<double>(), c->partial.as<double>(),
+                          c->st);
+    LAUNCHCHK(c);
+    if (precision == YQ_MIXED) {
+      TRY(ensure(c, c->T32, sizeof(float) * (size_t)ns * m));
+      launch_to_float(c->T64.as<double>(), c->T32.as<float>(), (size_t)ns * m, c->st);
+      LAUNCHCHK(c);
+      int rc = mixed_prepare(c->mixed, c->bsig.as<int8_t>(), c->T32.as<float>(), ns, n, m,
+                             c->use_bias, c->st, c->err);
+      if (rc) return rc;
+    }
+  } else {
+    const size_t obytes = sizeof(double) * (size_t)ns * dim;
+    TRY(ensure(c, c->O, obytes));
+    const int chunks = colsum_chunks(ns);
+    TRY(ensure(c, c->partial, sizeof(double) * std::max((size_t)chunks * dim,
+                                                       aug_partial_doubles(ns, n, m))));
+    launch_build_O(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias,
+                   c->O.as<double>(), c->st);
+    launch_colsum(c->O.as<double>(), ns, (int)dim, c->partial.as<double>(), c->st);
+    launch_reduce_partials(c->partial.as<double>(), chunks, (int)dim, 1.0 / ns,
+                           c->omean.as<double>(), c->st);
+    launch_center_force(c->O.as<double>(), c->omean.as<double>(), c->eloc.as<double>(), ns,
+                        (int)dim, c->partial.as<double>(), c->st);
+    launch_reduce_partials(c->partial.as<double>(), chunks, (int)dim, 1.0 / ns,
+                           c->F.as<double>(), c->st);
+    LAUNCHCHK(c);
+    if (solver == YQ_SOLVE_EXPLICIT) {
+      TRY(blas(c));
+      TRY(ensure(c, c->S, sizeof(double) * dim * dim));
+      const double alpha = 1.0 / ns, beta = 0.0;
+      // O_c is ns x dim row-major = dim x ns column-major: S = A A^T / ns
+      ProfGuard g("dsyrk", c->st);
+      BLASCHK(c, rocblas_dsyrk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                               (rocblas_int)dim, ns, &alpha, c->O.as<double>(),
+                               (rocblas_int)dim, &beta, c->S.as<double>(), (rocblas_int)dim));
+    }
+  }
+  TRY(sync(c));
+  c->have_sr = true;
+  c->sr_solver = solver;
+  c->sr_prec = precision;
+  return YQ_OK;
+}
+
+// out = (S + shift) v on device pointers, FP64
+int apply64(yq_ctx* c, double shift, const double* v, double* out) {
+  const int ns = c->ns;
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->vy, sizeof(double) * ns));
+  double* y = c->vy.as<double>();
+  if (c->sr_solver == YQ_SOLVE_CG_FACTORED) {
+    launch_factored_fwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
+                          c->use_bias, v, y, c->st);
+    LAUNCHCHK(c);
+    double ov;
+    TRY(dot(c, c->omean.as<double>(), v, dim, &ov));
+    launch_add_const(y, -ov, ns, c->st);                 // O_c v
+    launch_scale_add(y, 1.0 / ns, y, 0.0, ns, c->st);     // / Ns
+    launch_factored_bwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
+                          c->use_bias, y, out, c->partial.as<double>(), c->st);
+    LAUNCHCHK(c);
+    TRY(ensure(c, c->vb, sizeof(double) * ns));
+    double sy;
+    {
+      // sum of y via dot with a ones vector kept in vb
+      std::vector<double> ones(ns, 1.0);
+      HIPCHK(c, hipMemcpyAsync(c->vb.p, ones.data(), sizeof(double) * ns,
+                               hipMemcpyHostToDevice, c->st));
+      TRY(dot(c, y, c->vb.as<double>(), ns, &sy));
+    }
+    launch_axpy(-sy, c->omean.as<double>(), out, dim, c->st);
+    launch_axpy(shift, v, out, dim, c->st);
+    LAUNCHCHK(c);
+  } else if (c->sr_solver == YQ_SOLVE_CG || c->sr_solver == YQ_SOLVE_EXPLICIT) {
+    TRY(blas(c));
+    const double one = 1.0, zero = 0.0, inv = 1.0 / ns;
+    const double* A = c->O.as<double>();  // dim x ns column-major
+    BLASCHK(c, rocblas_dgemv(c->blas, rocblas_operation_transpose, (rocblas_int)dim, ns,
+                             &inv, A, (rocblas_int)dim, v, 1, &zero, y, 1));
+    HIPCHK(c, hipMemcpyAsync(out, v, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    BLASCHK(c, rocblas_dgemv(c->blas, rocblas_operation_none, (rocblas_int)dim, ns, &one, A,
+                             (rocblas_int)dim, y, 1, &shift, out, 1));
+  }
+  return YQ_OK;
+}
+
+// Conjugate gradients from x = 0 (SRMat.hpp:197-223 through Eigen's
+// ConjugateGradient): stop when |r| < tol |b|.
+int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int max_iter,
+         int* iters, double* err) {
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->vr, sizeof(double) * dim));
+  TRY(ensure(c, c->vp, sizeof(double) * dim));
+  TRY(ensure(c, c->vAp, sizeof(double) * dim));
+  double *r = c->vr.as<double>(), *p = c->vp.as<double>(), *Ap = c->vAp.as<double>();
+  HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+  HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(p, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  double bn2;
+  TRY(dot(c, b, b, dim, &bn2));
+  *iters = 0;
+  *err = 0.0;
+  if (bn2 == 0.0) return YQ_OK;
+  const double thr = tol * tol * bn2;
+  double rn2 = bn2, absNew = bn2;
+  int it = 0;
+  while (rn2 >= thr && it < max_iter) {
+    TRY(apply64(c, shift, p, Ap));
+    double pAp;
+    TRY(dot(c, p, Ap, dim, &pAp));
+    const double alpha = absNew / pAp;
+    launch_axpy(alpha, p, x, dim, c->st);
+    launch_axpy(-alpha, Ap, r, dim, c->st);
+    LAUNCHCHK(c);
+    TRY(dot(c, r, r, dim, &rn2));
+    ++it;
+    if (rn2 < thr) break;
+    const double beta = rn2 / absNew;
+    absNew = rn2;
+    launch_xpby(r, beta, p, dim, c->st);
+    LAUNCHCHK(c);
+  }
+  *iters = it;
+  *err = sqrt(rn2 / bn2);
+  return rn2 < thr ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+}
+
+// |(S + shift) x - b| / |b| in FP64
+int residual64(yq_ctx* c, double shift, const double* x, const double* b, double* out) {
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)c->ns * c->m)));
+  double* t = c->vtmp.as<double>();
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+    TRY(blas(c));
+    HIPCHK(c, hipMemcpyAsync(t, x, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    const double one = 1.0;
+    BLASCHK(c, rocblas_dsymv(c->blas, rocblas_fill_lower, (rocblas_int)dim, &one,
+                             c->S.as<double>(), (rocblas_int)dim, x, 1, &shift, t, 1));
+  } else {
+    TRY(apply64(c, shift, x, t));
+  }
+  launch_axpy(-1.0, b, t, dim, c->st);
+  LAUNCHCHK(c);
+  double rr, bb;
+  TRY(dot(c, t, t, dim, &rr));
+  TRY(dot(c, b, b, dim, &bb));
+  *out = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+  return YQ_OK;
+}
+
+int solve_explicit(yq_ctx* c, double shift, const double* b, double* x) {
+  const size_t dim = c->dim;
+  TRY(blas(c));
+  TRY(ensure(c, c->M, sizeof(double) * dim * dim));
+  TRY(ensure(c, c->info, sizeof(rocblas_int)));
+  launch_copy_lower_shift(c->S.as<double>(), c->M.as<double>(), (int)dim, shift, c->st);
+  LAUNCHCHK(c);
+  {
+    ProfGuard g("dpotrf", c->st);
+    BLASCHK(c, rocsolver_dpotrf(c->blas, rocblas_fill_lower, (rocblas_int)dim,
+                                c->M.as<double>(), (rocblas_int)dim,
+                                c->info.as<rocblas_int>()));
+  }
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  if (info != 0)
+    return fail(c, YQ_ERR_NOT_SPD,
+                "Cholesky of S + shift I failed at pivot " + std::to_string(info));
+  HIPCHK(c, hipMemcpyAsync(x, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  BLASCHK(c, rocsolver_dpotrs(c->blas, rocblas_fill_lower, (rocblas_int)dim, 1,
+                              c->M.as<double>(), (rocblas_int)dim, x, (rocblas_int)dim));
+  return YQ_OK;
+}
+
+// FP64 iterative refinement around the mixed-precision inner CG:
+// x += CG_mixed((S+shift) d = r), r = b - (S+shift) x in FP64.
+int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
+                int max_iter, int* iters, double* err) {
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->vr, sizeof(double) * dim));
+  TRY(ensure(c, c->vAp, sizeof(double) * dim));
+  TRY(ensure(c, c->vp, sizeof(double) * dim));
+  double *r = c->vr.as<double>(), *Ax = c->vAp.as<double>(), *d = c->vp.as<double>();
+  HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+  HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  double bn2;
+  TRY(dot(c, b, b, dim, &bn2));
+  *iters = 0;
+  *err = 0.0;
+  if (bn2 == 0.0) return YQ_OK;
+  double rel = 1.0;
+  int total = 0;
+  for (int outer = 0; outer < 30 && total < max_iter; ++outer) {
+    int it = 0;
+    const double inner_tol = std::max(tol / rel, 1e-4) < 1.0 ? std::max(tol / rel, 1e-4) : 0.5;
+    int rc = mixed_cg(c->mixed, shift, c->omean.as<double>(), r, d, inner_tol,
+                      max_iter - total, &it, c->st, c->err);
+    total += it;
+    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+    launch_axpy(1.0, d, x, dim, c->st);
+    LAUNCHCHK(c);
+    TRY(apply64(c, shift, x, Ax));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    launch_axpy(-1.0, Ax, r, dim, c->st);
+    LAUNCHCHK(c);
+    double rn2;
+    TRY(dot(c, r, r, dim, &rn2));
+    rel = sqrt(rn2 / bn2);
+    if (rel < tol) break;
+  }
+  *iters = total;
+  *err = rel;
+  return rel < tol ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+}
+
+int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev,
+          double* x_dev, double* resid, int* iters) {
+  if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
+  if (!(shift >= 0.0)) return fail(c, YQ_ERR_ARG, "shift must be >= 0");
+  int it = 0;
+  double err = 0.0;
+  int rc = YQ_OK;
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+    rc = solve_explicit(c, shift, b_dev, x_dev);
+    if (rc) return rc;
+    TRY(residual64(c, shift, x_dev, b_dev, &err));
+  } else if (c->sr_prec == YQ_MIXED) {
+    if (tol <= 0 || max_iter <= 0) return fail(c, YQ_ERR_ARG, "tol and max_iter must be > 0");
+    rc = solve_mixed(c, shift, b_dev, x_dev, tol, max_iter, &it, &err);
+    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+  } else {
+    if (tol <= 0 || max_iter <= 0) return fail(c, YQ_ERR_ARG, "tol and max_iter must be > 0");
+    rc = cg64(c, shift, b_dev, x_dev, tol, max_iter, &it, &err);
+    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+    double e2;
+    TRY(residual64(c, shift, x_dev, b_dev, &e2));
+    err = e2;
+  }
+  TRY(sync(c));
+  if (resid) *resid = err;
+  if (iters) *iters = it;
+  if (rc == YQ_ERR_NOT_CONVERGED)
+    return fail(c, rc, "iterative solve reached max_iter with relative residual " +
+                           std::to_string(err));
+  return YQ_OK;
+}
+
+int has_nan(yq_ctx* c, const double* dev, size_t n, int* out) {
+  std::vector<double> h(n);
+  HIPCHK(c, hipMemcpyAsync(h.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  *out = 0;
+  for (double v : h)
+    if (!std::isfinite(v)) {
+      *out = 1;
+      break;
+    }
+  return YQ_OK;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+YQ_API int yq_ctx_create(int device, yq_ctx** out) {
+  if (!out) return YQ_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return YQ_ERR_HIP;
+  }
+  if (device < 0 || device >= count) return YQ_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return YQ_ERR_HIP;
+  yq_ctx* c = new yq_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return YQ_ERR_HIP;
+  }
+  *out = c;
+  return YQ_OK;
+}
+
+YQ_API int yq_ctx_destroy(yq_ctx* c) {
+  if (!c) return YQ_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->st);
+  mixed_release(c->mixed);
+  for (Buf* b : c->bufs)
+    if (b->p) (void)hipFree(b->p);
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  (void)hipStreamDestroy(c->st);
+  delete c;
+  return YQ_OK;
+}
+
+YQ_API const char* yq_last_error(const yq_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+// ---------------------------------------------------------------- machine
+YQ_API int yq_rbm_create(yq_ctx* c, int dtype, int n, int m, int use_bias) {
+  if (!c) return YQ_ERR_ARG;
+  if (dtype != 1) return fail(c, YQ_ERR_UNSUPPORTED, "only real double machines (dtype 1)");
+  if (n < 2 || n > kMaxSites || m < 1 || m > kMaxHidden)
+    return fail(c, YQ_ERR_ARG, "need 2 <= n <= 128 and 1 <= m <= 512");
+  c->n = n;
+  c->m = m;
+  c->use_bias = use_bias ? 1 : 0;
+  c->dim = (size_t)n * m + (use_bias ? (size_t)n + m : 0);
+  const size_t np = (size_t)n * m + n + m;
+  TRY(ensure(c, c->params, sizeof(double) * np));
+  TRY(ensure(c, c->WT64, sizeof(double) * (size_t)n * m));
+  TRY(ensure(c, c->W32, sizeof(float) * (size_t)n * m));
+  TRY(ensure(c, c->WT32, sizeof(float) * (size_t)n * m));
+  TRY(ensure(c, c->a32, sizeof(float) * (size_t)n));
+  HIPCHK(c, hipMemsetAsync(c->params.p, 0, sizeof(double) * np, c->st));
+  c->have_rbm = true;
+  c->have_batch = false;
+  c->have_sampler = false;
+  TRY(refresh_params(c));
+  return sync(c);
+}
+
+YQ_API int yq_rbm_dims(const yq_ctx* c, int* n, int* m, int* dim) {
+  if (!c || !c->have_rbm) return YQ_ERR_STATE;
+  if (n) *n = c->n;
+  if (m) *m = c->m;
+  if (dim) *dim = (int)c->dim;
+  return YQ_OK;
+}
+
+YQ_API int yq_rbm_set_params(yq_ctx* c, const double* p) {
+  TRY(need_rbm(c));
+  if (!p) return fail(c, YQ_ERR_ARG, "null params");
+  const size_t nm = (size_t)c->n * c->m;
+  std::vector<double> h(nm + c->n + c->m, 0.0);
+  memcpy(h.data(), p, sizeof(double) * c->dim);
+  for (double v : h)
+    if (!std::isfinite(v)) return fail(c, YQ_ERR_NAN, "non-finite parameter");
+  HIPCHK(c, hipMemcpyAsync(c->params.p, h.data(), sizeof(double) * h.size(),
+                           hipMemcpyHostToDevice, c->st));
+  TRY(refresh_params(c));
+  return sync(c);
+}
+
+YQ_API int yq_rbm_get_params(yq_ctx* c, double* p) {
+  TRY(need_rbm(c));
+  if (!p) return fail(c, YQ_ERR_ARG, "null output");
+  HIPCHK(c, hipMemcpyAsync(p, c->params.p, sizeof(double) * c->dim, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_rbm_update_params(yq_ctx* c, const double* delta) {
+  TRY(need_rbm(c));
+  if (!delta) return fail(c, YQ_ERR_ARG, "null delta");
+  TRY(ensure(c, c->vtmp, sizeof(double) * c->dim));
+  HIPCHK(c, hipMemcpyAsync(c->vtmp.p, delta, sizeof(double) * c->dim, hipMemcpyHostToDevice, c->st));
+  launch_axpy(1.0, c->vtmp.as<double>(), c->params.as<double>(), c->dim, c->st);
+  LAUNCHCHK(c);
+  TRY(refresh_params(c));
+  return sync(c);
+}
+
+YQ_API int yq_rbm_init_random(yq_ctx* c, uint64_t seed, double sigma) {
+  TRY(need_rbm(c));
+  const int n = c->n, m = c->m;
+  const size_t nm = (size_t)n * m;
+  std::vector<double> h(nm + n + m, 0.0);
+  // draw order of RBM::initializeRandom: a, b, then W column by column
+  std::vector<double*> order;
+  if (c->use_bias) {
+    for (int i = 0; i < n; ++i) order.push_back(&h[nm + i]);
+    for (int j = 0; j < m; ++j) order.push_back(&h[nm + n + j]);
+  }
+  for (size_t q = 0; q < nm; ++q) order.push_back(&h[q]);
+  for (size_t q = 0; q < order.size(); ++q) {
+    const u32x4 x = draw(seed, q / 2, 0u, kDomParams);
+    const double u1 = 1.0 - u01_double(x.x, x.y);
+    const double u2 = u01_double(x.z, x.w);
+    const double rad = sqrt(-2.0 * log(u1));
+    const double z = (q & 1) ? rad * sin(2.0 * M_PI * u2) : rad * cos(2.0 * M_PI * u2);
+    *order[q] = sigma * z;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->params.p, h.data(), sizeof(double) * h.size(),
+                           hipMemcpyHostToDevice, c->st));
+  TRY(refresh_params(c));
+  return sync(c);
+}
+
+YQ_API int yq_rbm_has_nan(yq_ctx* c, int* out) {
+  TRY(need_rbm(c));
+  if (!out) return fail(c, YQ_ERR_ARG, "null output");
+  return has_nan(c, c->params.as<double>(), c->dim, out);
+}
+
+// ---------------------------------------------------------------- Hamiltonian
+YQ_API int yq_ham_set(yq_ctx* c, const yq_ham_desc* d) {
+  if (!c) return YQ_ERR_ARG;
+  if (!d) return fail(c, YQ_ERR_ARG, "null descriptor");
+  if (d->kind < YQ_TFI || d->kind > YQ_J1J2) return fail(c, YQ_ERR_ARG, "unknown Hamiltonian kind");
+  if (d->n < 3) return fail(c, YQ_ERR_ARG, "Hamiltonian needs n >= 3");
+  for (double v : {d->J, d->h, d->delta, d->J2})
+    if (!std::isfinite(v)) return fail(c, YQ_ERR_ARG, "non-finite coupling");
+  c->ham = *d;
+  c->have_ham = true;
+  c->have_eloc = false;
+  c->have_sr = false;
+  return YQ_OK;
+}
+
+// ---------------------------------------------------------------- sampler
+YQ_API int yq_sampler_config(yq_ctx* c, int sweeper, int n_temps, int n_chains_per,
+                             int sweep_factor, int precision, uint64_t seed) {
+  TRY(need_rbm(c));
+  if (sweeper != YQ_LOCAL && sweeper != YQ_EXCHANGE) return fail(c, YQ_ERR_ARG, "unknown sweeper");
+  if (n_temps != 1) return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering (n_temps > 1)");
+  if (n_chains_per < 1 || sweep_factor < 1) return fail(c, YQ_ERR_ARG, "chains and sweep factor >= 1");
+  if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
+  c->skind = sweeper;
+  c->chains = n_chains_per;
+  c->sweep_factor = sweep_factor;
+  c->sprec = precision;
+  c->seed = seed;
+  c->rand_epoch = 0;
+  TRY(ensure(c, c->sig_state, (size_t)c->chains * c->n));
+  TRY(ensure(c, c->ctr, sizeof(uint64_t) * c->chains));
+  TRY(ensure(c, c->acc, sizeof(uint64_t) * c->chains));
+  HIPCHK(c, hipMemsetAsync(c->ctr.p, 0, sizeof(uint64_t) * c->chains, c->st));
+  HIPCHK(c, hipMemsetAsync(c->acc.p, 0, sizeof(uint64_t) * c->chains, c->st));
+  c->have_sampler = true;
+  TRY(sync(c));
+  return yq_sampler_randomize(c, sweeper == YQ_EXCHANGE ? c->n / 2 : -1);
+}
+
+YQ_API int yq_sampler_randomize(yq_ctx* c, int nup) {
+  TRY(need_rbm(c));
+  if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
+  const int n = c->n;
+  if (nup > n) return fail(c, YQ_ERR_ARG, "nup > n");
+  std::vector<int8_t> h((size_t)c->chains * n);
+  const uint64_t base = c->rand_epoch << 20;
+  for (int ch = 0; ch < c->chains; ++ch) {
+    int8_t* s = h.data() + (size_t)ch * n;
+    if (nup < 0) {
+      for (int i = 0; i < n; i += 128) {
+        const u32x4 x = draw(c->seed, base + (uint64_t)(i / 128), (uint32_t)ch, kDomRandomize);
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+        for (int q = 0; q < 128 && i + q < n; ++q) s[i + q] = ((w[q >> 5] >> (q & 31)) & 1u) ? -1 : 1;
+      }
+    } else {
+      for (int i = 0; i < n; ++i) s[i] = i < nup ? -1 : 1;
+      for (int i = n - 1; i > 0; --i) {
+        const u32x4 x = draw(c->seed, base + (uint64_t)(n - 1 - i), (uint32_t)ch, kDomRandomize);
+        const int j = (int)index_from(x.x, (uint32_t)(i + 1));
+        std::swap(s[i], s[j]);
+      }
+    }
+  }
+  ++c->rand_epoch;
+  HIPCHK(c, hipMemcpyAsync(c->sig_state.p, h.data(), h.size(), hipMemcpyHostToDevice, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_sampler_set_sigma(yq_ctx* c, const int8_t* sigma) {
+  TRY(need_rbm(c));
+  if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
+  if (!sigma) return fail(c, YQ_ERR_ARG, "null sigma");
+  const size_t cnt = (size_t)c->chains * c->n;
+  for (size_t q = 0; q < cnt; ++q)
+    if (sigma[q] != 1 && sigma[q] != -1) return fail(c, YQ_ERR_ARG, "spins must be +1 or -1");
+  HIPCHK(c, hipMemcpyAsync(c->sig_state.p, sigma, cnt, hipMemcpyHostToDevice, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_sampler_get_state(yq_ctx* c, int8_t* sigma, uint64_t* ctr, uint64_t* acc) {
+  TRY(need_rbm(c));
+  if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
+  if (sigma)
+    HIPCHK(c, hipMemcpyAsync(sigma, c->sig_state.p, (size_t)c->chains * c->n,
+                             hipMemcpyDeviceToHost, c->st));
+  if (ctr)
+    HIPCHK(c, hipMemcpyAsync(ctr, c->ctr.p, sizeof(uint64_t) * c->chains, hipMemcpyDeviceToHost, c->st));
+  if (acc)
+    HIPCHK(c, hipMemcpyAsync(acc, c->acc.p, sizeof(uint64_t) * c->chains, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
+  TRY(need_rbm(c));
+  if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
+  if (n_sweeps < 1 || n_discard < 0) return fail(c, YQ_ERR_ARG, "n_sweeps >= 1, n_discard >= 0");
+  const long long ns = (long long)c->chains * n_sweeps;
+  if (ns > (1ll << 30)) return fail(c, YQ_ERR_ARG, "too many samples");
+  c->ns = (int)ns;
+  c->bprec = c->sprec;
+  const size_t rs = c->sprec == YQ_MIXED ? sizeof(float) : sizeof(double);
+  TRY(ensure(c, c->bsig, (size_t)ns * c->n));
+  TRY(ensure(c, c->btheta, rs * (size_t)ns * c->m));
+  SampleArgs a;
+  a.W64 = W64(c);
+  a.b64 = b64(c);
+  a.Wr = c->sprec == YQ_MIXED ? (const void*)c->W32.p : (const void*)W64(c);
+  a.ar = c->sprec == YQ_MIXED ? (const void*)c->a32.p : (const void*)a64(c);
+  a.sigma = c->sig_state.as<int8_t>();
+  a.ctr = c->ctr.as<uint64_t>();
+  a.acc = c->acc.as<uint64_t>();
+  a.out_sigma = c->bsig.as<int8_t>();
+  a.out_theta = c->btheta.p;
+  a.C = c->chains;
+  a.n = c->n;
+  a.m = c->m;
+  a.nprop = c->n * c->sweep_factor;
+  a.n_sweeps = n_sweeps;
+  a.n_discard = n_discard;
+  a.npad = (c->n + 15) & ~15;
+  a.kind = c->skind;
+  a.mixed = c->sprec == YQ_MIXED;
+  a.beta = 1.0;
+  a.seed = c->seed;
+  launch_sample(a, c->st);
+  LAUNCHCHK(c);
+  TRY(sync(c));
+  c->have_batch = true;
+  c->have_eloc = false;
+  c->have_sr = false;
+  if (n_samples) *n_samples = (int)ns;
+  return YQ_OK;
+}
+
+YQ_API int yq_samples_get(yq_ctx* c, int8_t* sigma, double* theta) {
+  TRY(need_rbm(c));
+  if (!c->have_batch) return fail(c, YQ_ERR_STATE, "no samples");
+  const size_t cnt = (size_t)c->ns * c->m;
+  if (sigma)
+    HIPCHK(c, hipMemcpyAsync(sigma, c->bsig.p, (size_t)c->ns * c->n, hipMemcpyDeviceToHost, c->st));
+  if (theta) {
+    if (c->bprec == YQ_MIXED) {
+      std::vector<float> f(cnt);
+      HIPCHK(c, hipMemcpyAsync(f.data(), c->btheta.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->st));
+      TRY(sync(c));
+      for (size_t q = 0; q < cnt; ++q) theta[q] = f[q];
+    } else {
+      HIPCHK(c, hipMemcpyAsync(theta, c->btheta.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->st));
+    }
+  }
+  return sync(c);
+}
+
+YQ_API int yq_samples_set(yq_ctx* c, int ns, const int8_t* sigma, int precision) {
+  TRY(need_rbm(c));
+  if (ns < 1 || !sigma) return fail(c, YQ_ERR_ARG, "need ns >= 1 and sigma");
+  if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
+  const size_t cnt = (size_t)ns * c->n;
+  for (size_t q = 0; q < cnt; ++q)
+    if (sigma[q] != 1 && sigma[q] != -1) return fail(c, YQ_ERR_ARG, "spins must be +1 or -1");
+  const size_t rs = precision == YQ_MIXED ? sizeof(float) : sizeof(double);
+  TRY(ensure(c, c->bsig, cnt));
+  TRY(ensure(c, c->btheta, rs * (size_t)ns * c->m));
+  HIPCHK(c, hipMemcpyAsync(c->bsig.p, sigma, cnt, hipMemcpyHostToDevice, c->st));
+  launch_theta_batch(W64(c), b64(c), c->bsig.as<int8_t>(), ns, c->n, c->m, c->btheta.p,
+                     precision == YQ_MIXED, c->st);
+  LAUNCHCHK(c);
+  c->ns = ns;
+  c->bprec = precision;
+  c->have_batch = true;
+  c->have_eloc = false;
+  c->have_sr = false;
+  return sync(c);
+}
+
+// ---------------------------------------------------------------- energy
+YQ_API int yq_eloc(yq_ctx* c, double* eloc, double* mean, double* var) {
+  TRY(need_rbm(c));
+  TRY(compute_eloc(c));
+  if (eloc)
+    HIPCHK(c, hipMemcpyAsync(eloc, c->eloc.p, sizeof(double) * c->ns, hipMemcpyDeviceToHost, c->st));
+  if (mean) *mean = c->e_mean;
+  if (var) *var = c->e_var;
+  return sync(c);
+}
+
+// ---------------------------------------------------------------- SR
+YQ_API int yq_sr_build(yq_ctx* c, int solver, int precision) {
+  TRY(need_rbm(c));
+  if (!c->have_batch) return fail(c, YQ_ERR_STATE, "no samples");
+  return sr_build(c, solver, precision);
+}
+
+YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {
+  TRY(need_rbm(c));
+  if (!out) return fail(c, YQ_ERR_ARG, "null output");
+  if (item == YQ_SR_ELOC) {
+    if (!c->have_eloc) return fail(c, YQ_ERR_STATE, "no local energies");
+    HIPCHK(c, hipMemcpyAsync(out, c->eloc.p, sizeof(double) * c->ns, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+  }
+  if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
+  const size_t dim = c->dim;
+  switch (item) {
+    case YQ_SR_F:
+      HIPCHK(c, hipMemcpyAsync(out, c->F.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->st));
+      break;
+    case YQ_SR_OMEAN:
+      HIPCHK(c, hipMemcpyAsync(out, c->omean.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->st));
+      break;
+    case YQ_SR_O:
+      if (c->sr_solver == YQ_SOLVE_CG_FACTORED)
+        return fail(c, YQ_ERR_STATE, "the factored solver does not materialise O");
+      HIPCHK(c, hipMemcpyAsync(out, c->O.p, sizeof(double) * dim * c->ns, hipMemcpyDeviceToHost, c->st));
+      break;
+    case YQ_SR_S:
+      if (c->sr_solver != YQ_SOLVE_EXPLICIT)
+        return fail(c, YQ_ERR_STATE, "S exists only for the explicit solver");
+      HIPCHK(c, hipMemcpyAsync(out, c->S.p, sizeof(double) * dim * dim, hipMemcpyDeviceToHost, c->st));
+      TRY(sync(c));
+      for (size_t j = 0; j < dim; ++j)  // mirror the lower triangle (column-major)
+        for (size_t i = 0; i < j; ++i) out[j * dim + i] = out[i * dim + j];
+      return YQ_OK;
+    default:
+      return fail(c, YQ_ERR_ARG, "unknown item");
+  }
+  return sync(c);
+}
+
+YQ_API int yq_sr_apply(yq_ctx* c, double shift, const double* v, double* out) {
+  TRY(need_rbm(c));
+  if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
+  if (!v || !out) return fail(c, YQ_ERR_ARG, "null vector");
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->vx, sizeof(double) * dim));
+  TRY(ensure(c, c->vAp, sizeof(double) * dim));
+  HIPCHK(c, hipMemcpyAsync(c->vx.p, v, sizeof(double) * dim, hipMemcpyHostToDevice, c->st));
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+    TRY(blas(c));
+    const double one = 1.0;
+    HIPCHK(c, hipMemcpyAsync(c->vAp.p, c->vx.p, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    BLASCHK(c, rocblas_dsymv(c->blas, rocblas_fill_lower, (rocblas_int)dim, &one,
+                             c->S.as<double>(), (rocblas_int)dim, c->vx.as<double>(), 1, &shift,
+                             c->vAp.as<double>(), 1));
+  } else if (c->sr_prec == YQ_MIXED) {
+    launch_mixed_apply64(c->mixed, shift, c->omean.as<double>(), c->vx.as<double>(),
+                         c->vAp.as<double>(), c->st);
+    LAUNCHCHK(c);
+  } else {
+    TRY(apply64(c, shift, c->vx.as<double>(), c->vAp.as<double>()));
+  }
+  HIPCHK(c, hipMemcpyAsync(out, c->vAp.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_sr_solve(yq_ctx* c, double shift, double tol, int max_iter, const double* rhs,
+                       double* x, double* resid, int* iters) {
+  TRY(need_rbm(c));
+  if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->vx, sizeof(double) * dim));
+  TRY(ensure(c, c->vb, sizeof(double) * std::max(dim, (size_t)c->ns)));
+  const double* b = c->F.as<double>();
+  if (rhs) {
+    HIPCHK(c, hipMemcpyAsync(c->vb.p, rhs, sizeof(double) * dim, hipMemcpyHostToDevice, c->st));
+    b = c->vb.as<double>();
+  }
+  int rc = solve(c, shift, tol, max_iter, b, c->vx.as<double>(), resid, iters);
+  if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+  if (x) HIPCHK(c, hipMemcpyAsync(x, c->vx.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  return rc;
+}
+
+// ---------------------------------------------------------------- profiling
+YQ_API int yq_prof_enable(yq_ctx* c, const char* labels) {
+  if (!c) return YQ_ERR_ARG;
+  TRY(sync(c));
+  for (ProfEntry& e : g_prof.entries) {
+    for (auto& p : e.pending) {
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+    for (hipEvent_t ev : e.open) (void)hipEventDestroy(ev);
+  }
+  g_prof = ProfState();
+  g_prof_any = false;
+  if (!labels || !*labels) return YQ_OK;
+  std::string s(labels);
+  size_t pos = 0;
+  while (pos <= s.size()) {
+    const size_t comma = s.find(',', pos);
+    const std::string l = s.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+    if (!l.empty()) {
+      g_prof.labels.push_back(l);
+      g_prof.entries.emplace_back();
+    }
+    if (comma == std::string::npos) break;
+    pos = comma + 1;
+  }
+  g_prof_any = !g_prof.labels.empty();
+  return YQ_OK;
+}
+
+YQ_API int yq_prof_read(yq_ctx* c, const char* label, double* total_ms, long long* launches) {
+  if (!c || !label) return YQ_ERR_ARG;
+  TRY(sync(c));
+  const int i = g_prof.find(label);
+  if (i < 0) return fail(c, YQ_ERR_ARG, std::string("label not enabled: ") + label);
+  ProfEntry& e = g_prof.entries[i];
+  for (auto& p : e.pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) e.total_ms += ms;
+    ++e.launches;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  e.pending.clear();
+  if (total_ms) *total_ms = e.total_ms;
+  if (launches) *launches = e.launches;
+  e.total_ms = 0.0;
+  e.launches = 0;
+  return YQ_OK;
+}
+
+// ---------------------------------------------------------------- runner
+YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
+  TRY(need_rbm(c));
+  if (!cfg || !out) return fail(c, YQ_ERR_ARG, "null cfg/out");
+  if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
+  hipEvent_t e0, e1, e2;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventCreate(&e2));
+  auto cleanup = [&]() {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+  };
+  int rc = YQ_OK;
+  do {
+    if (cfg->randomize && (rc = yq_sampler_randomize(c, cfg->nup)) != YQ_OK) break;
+    if ((rc = hipEventRecord(e0, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
+    int ns = 0;
+    if ((rc = yq_sample(c, cfg->n_sweeps, cfg->n_discard, &ns)) != YQ_OK) break;
+    if ((rc = hipEventRecord(e1, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
+    if ((rc = compute_eloc(c)) != YQ_OK) break;
+    if ((rc = sr_build(c, cfg->solver, cfg->precision)) != YQ_OK) break;
+    int it = 0;
+    double err = 0.0;
+    if ((rc = ensure(c, c->vx, sizeof(double) * c->dim)) != YQ_OK) break;
+    rc = solve(c, cfg->lambda, cfg->cg_tol, cfg->cg_max_iter, c->F.as<double>(),
+               c->vx.as<double>(), &err, &it);
+    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) break;
+    const int solve_rc = rc;
+    double vv;
+    if ((rc = dot(c, c->vx.as<double>(), c->vx.as<double>(), c->dim, &vv)) != YQ_OK) break;
+    const double nv = sqrt(vv);
+    double scale = -cfg->lr;
+    if (cfg->grad_clip != 0.0 && nv > cfg->grad_clip) scale *= cfg->grad_clip / nv;
+    launch_axpy(scale, c->vx.as<double>(), c->params.as<double>(), c->dim, c->st);
+    if ((rc = refresh_params(c)) != YQ_OK) break;
+    if ((rc = hipEventRecord(e2, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
+    if ((rc = sync(c)) != YQ_OK) break;
+    float t01 = 0, t12 = 0;
+    (void)hipEventElapsedTime(&t01, e0, e1);
+    (void)hipEventElapsedTime(&t12, e1, e2);
+    out->energy = c->e_mean;
+    out->energy_var = c->e_var;
+    out->update_norm = nv;
+    out->cg_err = err;
+    out->cg_iters = it;
+    out->n_samples = ns;
+    out->sample_ms = t01;
+    out->solve_ms = t12;
+    if (!std::isfinite(nv) || !std::isfinite(c->e_mean)) {
+      rc = fail(c, YQ_ERR_NAN, "non-finite energy or update");
+      break;
+    }
+    rc = solve_rc;
+  } while (0);
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"

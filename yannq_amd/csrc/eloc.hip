@@ -1,0 +1,181 @@
+// Local energies E_loc(sigma) = sum_b c_b(sigma) psi(sigma^b)/psi(sigma) + diag
+// for the TFI, XXZ and J1-J2 chains (Hamiltonians/TFIsing.hpp:28-39,
+// XXZ.hpp:37-49, J1J2.hpp:35-55) as batched flip-ratio kernels.
+//
+// Pass 1 (k_eloc_ratio): one thread per (sample, bond).  A block of 256
+// consecutive (sample, bond) pairs stages theta_s and logcosh(theta_s) of the
+// few samples it touches in LDS, then each thread walks the hidden units in
+// index order exactly like RBMStateObj::logRatio (RBMState.hpp:54-62 /
+// 82-91), reading W^T rows that are contiguous across the bonds of a sample.
+// Pass 2 (k_eloc_combine): one thread per sample accumulates the diagonal and
+// ratio terms in the functor's order, in double.
+#include "yq_internal.h"
+#include "yq_math.h"
+
+namespace yq {
+
+int num_bonds(const yq_ham_desc& h) { return h.kind == YQ_J1J2 ? 2 * h.n : h.n; }
+
+namespace {
+
+__device__ inline void bond_sites(int kind, int n, int b, int& k, int& l) {
+  if (kind == YQ_TFI) {
+    k = b;
+    l = -1;
+  } else if (kind == YQ_XXZ) {
+    k = b;
+    l = (b + 1) % n;
+  } else if (b < n) {
+    k = b;
+    l = (b + 1) % n;
+  } else {
+    k = b - n;
+    l = (b - n + 2) % n;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_eloc_ratio(const ElocArgs A, int span) {
+  extern __shared__ unsigned char lds_raw[];
+  R* th = reinterpret_cast<R*>(lds_raw);
+  R* lc = th + (size_t)span * A.m;
+  int8_t* sg = reinterpret_cast<int8_t*>(lc + (size_t)span * A.m);
+  const int n = A.n, m = A.m, B = A.B;
+  const size_t first = (size_t)blockIdx.x * 256;
+  const size_t total = (size_t)A.ns * B;
+  const int s_lo = (int)(first / B);
+  const size_t last = first + 255 < total ? first + 255 : total - 1;
+  const int s_hi = (int)(last / B);
+  const int cnt = s_hi - s_lo + 1;
+  const R* theta = static_cast<const R*>(A.theta);
+  for (int q = threadIdx.x; q < cnt * m; q += 256) {
+    const R t = theta[(size_t)s_lo * m + q];
+    th[q] = t;
+    lc[q] = Math<R>::logcosh(t);
+  }
+  for (int q = threadIdx.x; q < cnt * n; q += 256)
+    sg[q] = A.sigma[(size_t)s_lo * n + q];
+  __syncthreads();
+  const size_t idx = first + threadIdx.x;
+  if (idx >= total) return;
+  const int s = (int)(idx / B), b = (int)(idx % B);
+  const int ls = s - s_lo;
+  const int8_t* sgs = sg + (size_t)ls * n;
+  int k, l;
+  bond_sites(A.ham.kind, n, b, k, l);
+  if (l >= 0 && sgs[k] == sgs[l]) {  // prefactor (1 - zz) vanishes
+    A.ratio[idx] = 0.0;
+    return;
+  }
+  const R* WT = static_cast<const R*>(A.WT);
+  const R* ar = static_cast<const R*>(A.ar);
+  const int8_t sk = sgs[k];
+  const R twok = sk > 0 ? (R)2 : (R)-2;
+  R res = ((R)-2 * ar[k]) * (R)sk;
+  const R* ths = th + (size_t)ls * m;
+  const R* lcs = lc + (size_t)ls * m;
+  if (l < 0) {
+    for (int j = 0; j < m; ++j) {
+      const R t = ths[j] - twok * WT[(size_t)j * n + k];
+      res = res + (Math<R>::logcosh(t) - lcs[j]);
+    }
+  } else {
+    const int8_t sl = sgs[l];
+    const R twol = sl > 0 ? (R)2 : (R)-2;
+    res = res - ((R)2 * ar[l]) * (R)sl;
+    for (int j = 0; j < m; ++j) {
+      R t = ths[j] - twok * WT[(size_t)j * n + k];
+      t = t - twol * WT[(size_t)j * n + l];
+      res = res + (Math<R>::logcosh(t) - lcs[j]);
+    }
+  }
+  A.ratio[idx] = (double)Math<R>::exp(res);
+}
+
+__global__ void __launch_bounds__(256) k_eloc_combine(const ElocArgs A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.ns) return;
+  const int n = A.n;
+  const int8_t* sg = A.sigma + (size_t)s * n;
+  const double* ratio = A.ratio + (size_t)s * A.B;
+  const yq_ham_desc& H = A.ham;
+  double e = 0.0;
+  if (H.kind == YQ_TFI) {
+    for (int i = 0; i < n; ++i) {
+      e += H.J * (double)sg[i] * (double)sg[(i + 1) % n];
+      e += H.h * ratio[i];
+    }
+  } else if (H.kind == YQ_XXZ) {
+    const double sign = H.sign_rule ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) {
+      const int zz = (int)sg[i] * (int)sg[(i + 1) % n];
+      e += H.J * H.delta * zz;
+      e += sign * H.J * (1 - zz) * ratio[i];
+    }
+  } else {
+    const int sign = H.sign_rule ? -1 : 1;
+    for (int i = 0; i < n; ++i) {
+      const double yy = -(double)((int)sg[i] * (int)sg[(i + 1) % n]);
+      e += -H.J * yy;
+      e += H.J * sign * (1.0 + yy) * ratio[i];
+    }
+    for (int i = 0; i < n; ++i) {
+      const double yy = -(double)((int)sg[i] * (int)sg[(i + 2) % n]);
+      e += -H.J2 * yy;
+      e += H.J2 * (1.0 + yy) * ratio[n + i];
+    }
+  }
+  A.eloc[s] = e;
+}
+
+// mean and variance in one block, fixed reduction order
+__global__ void __launch_bounds__(1024) k_eloc_stats(const double* e, int ns,
+                                                     double* out) {
+  __shared__ double s1[1024], s2[1024];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < ns; i += 1024) {
+    a += e[i];
+    b += e[i] * e[i];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 512; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double mean = s1[0] / ns;
+    out[0] = mean;
+    out[1] = s2[0] / ns - mean * mean;
+  }
+}
+
+}  // namespace
+
+void launch_eloc(const ElocArgs& a, hipStream_t st) {
+  const size_t total = (size_t)a.ns * a.B;
+  const int span = 256 / a.B + 2;
+  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
+  const size_t lds = (size_t)span * a.m * 2 * rs + (size_t)span * a.n + 16;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  {
+    ProfGuard g("eloc_ratio", st);
+    if (a.use_float)
+      hipLaunchKernelGGL(k_eloc_ratio<float>, grid, block, lds, st, a, span);
+    else
+      hipLaunchKernelGGL(k_eloc_ratio<double>, grid, block, lds, st, a, span);
+  }
+  ProfGuard g("eloc_combine", st);
+  hipLaunchKernelGGL(k_eloc_combine, dim3((a.ns + 255) / 256), dim3(256), 0, st, a);
+}
+
+void launch_eloc_stats(const double* eloc, int ns, double* out,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_eloc_stats, dim3(1), dim3(1024), 0, st, eloc, ns, out);
+}
+
+}  // namespace yq

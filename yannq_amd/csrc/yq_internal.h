@@ -1,0 +1,127 @@
+// Internal declarations shared by the kernels and the C-ABI implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/yannq_amd.h"
+
+namespace yq {
+
+// Per-kernel HIP-event timing, enabled by label through yq_prof_enable.  A
+// ProfGuard placed around a launch records a start/stop event pair on the
+// launch's stream when its label is enabled, and costs one branch otherwise.
+bool prof_enabled(const char* label);
+void prof_begin(const char* label, hipStream_t st);
+void prof_end(const char* label, hipStream_t st);
+struct ProfGuard {
+  const char* label;
+  hipStream_t st;
+  bool on;
+  ProfGuard(const char* l, hipStream_t s) : label(l), st(s), on(prof_enabled(l)) {
+    if (on) prof_begin(label, st);
+  }
+  ~ProfGuard() {
+    if (on) prof_end(label, st);
+  }
+};
+
+constexpr int kWave = 64;
+constexpr int kMaxHidden = 512;  // theta registers per lane: 8 per 64 lanes
+constexpr int kMaxSites = 128;
+
+// ------------------------------------------------------------------ sampler
+struct SampleArgs {
+  const double* W64;  // m x n column-major, fp64 master
+  const double* b64;  // m
+  const void* Wr;     // working-precision W (double or float), m x n col-major
+  const void* ar;     // working-precision a (n)
+  int8_t* sigma;      // C x n chain state
+  uint64_t* ctr;      // C proposal counters
+  uint64_t* acc;      // C accepted-proposal counters
+  int8_t* out_sigma;  // n_sweeps*C x n
+  void* out_theta;    // n_sweeps*C x m (double for fp64, float for mixed)
+  int C, n, m, nprop, n_sweeps, n_discard, npad, kind, mixed;
+  double beta;
+  uint64_t seed;
+};
+void launch_sample(const SampleArgs& a, hipStream_t st);
+
+// theta = W sigma + b for a batch (fp64 sequential sum, optionally rounded to
+// float); out is double or float.
+void launch_theta_batch(const double* W64, const double* b64,
+                        const int8_t* sigma, int ns, int n, int m, void* out,
+                        int out_float, hipStream_t st);
+
+// ------------------------------------------------------------------ E_loc
+struct ElocArgs {
+  const int8_t* sigma;  // Ns x n
+  const void* theta;    // Ns x m (double or float)
+  const void* WT;       // m x n row-major (WT[j*n+i] = W_ji), double or float
+  const void* ar;       // n, double or float
+  double* ratio;        // Ns x B
+  double* eloc;         // Ns
+  int ns, n, m, B, use_float;
+  yq_ham_desc ham;
+};
+int num_bonds(const yq_ham_desc& h);
+void launch_eloc(const ElocArgs& a, hipStream_t st);
+// mean and variance (Energy.hpp:65-70) into out[0], out[1] on the device
+void launch_eloc_stats(const double* eloc, int ns, double* out,
+                       hipStream_t st);
+
+// ------------------------------------------------------------------ SR
+void launch_tanh(const void* theta, void* T, size_t count, int use_float,
+                 hipStream_t st);
+void launch_build_O(const int8_t* sigma, const double* T, int ns, int n, int m,
+                    int use_bias, double* O, hipStream_t st);
+// partial column sums of X (ns x dim row-major) over sample chunks
+int colsum_chunks(int ns);
+void launch_colsum(const double* X, int ns, int dim, double* partial,
+                   hipStream_t st);
+// out[c] = scale * sum_k partial[k][c]
+void launch_reduce_partials(const double* partial, int chunks, int dim,
+                            double scale, double* out, hipStream_t st);
+// O -= omean (in place) and partial[k][c] = sum_{s in chunk k} O_c[s][c] e[s]
+void launch_center_force(double* O, const double* omean, const double* e,
+                         int ns, int dim, double* partial, hipStream_t st);
+void launch_add_diag(double* M, int dim, double shift, hipStream_t st);
+void launch_copy_lower_shift(const double* S, double* M, int dim,
+                             double shift, hipStream_t st);
+
+// factored O (sigma, T), fp64: y = O v (uncentred), length ns
+void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
+                           int m, int use_bias, const double* v, double* y,
+                           hipStream_t st);
+// g = O^T r (uncentred), length dim; partial has colsum_chunks(ns)*dim doubles
+void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
+                           int m, int use_bias, const double* r, double* g,
+                           double* partial, hipStream_t st);
+// column mean of the factored O: omean = O^T 1 / ns
+void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
+                            int n, int m, int use_bias, double* omean,
+                            double* partial, hipStream_t st);
+
+// ------------------------------------------------------------------ vectors
+// deterministic dot product: partial sums per block; host finishes the sum
+int dot_blocks(size_t n);
+void launch_dot(const double* x, const double* y, size_t n, double* partial,
+                hipStream_t st);
+void launch_axpy(double alpha, const double* x, double* y, size_t n,
+                 hipStream_t st);  // y += alpha x
+void launch_xpby(const double* x, double beta, double* y, size_t n,
+                 hipStream_t st);  // y = x + beta y
+void launch_scale_add(double* y, double a, const double* x, double b, size_t n,
+                      hipStream_t st);  // y = a*y + b*x
+void launch_add_const(double* y, double c, size_t n, hipStream_t st);
+void launch_to_float(const double* x, float* y, size_t n, hipStream_t st);
+void launch_float_to_double(const float* x, double* y, size_t n, hipStream_t st);
+void launch_transpose(const double* W, int m, int n, double* WT,
+                      hipStream_t st);  // WT[j*n+i] = W[i*m+j]
+void launch_transpose_f(const double* W, int m, int n, float* WT,
+                        hipStream_t st);
+
+}  // namespace yq
