@@ -1,0 +1,214 @@
+"""SR steps/s of the VMC step (sample + E_loc + O + SR solve + update) on one
+MI355X: 1D transverse-field Ising chain, N=100, real RBM alpha=4 (40,500
+parameters), 16,384 samples per step.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+Timed path (value): mixed precision -- sampling and E_loc in float, O kept
+factored (spins and tanh(theta)), matrix-free CG whose products run as
+split-bf16 MFMA with float accumulation, FP64 iterative refinement to a 1e-8
+relative residual, FP64 update.  Also timed on the same start state: the plain
+FP64 path (explicit S = O_c^T O_c / Ns and Cholesky, the reference's default
+solver) and the FP64 factored CG.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, ALPHA = 100, 4
+M = N * ALPHA
+DIM = N * M + N + M
+CHAINS = 16384           # one sample per chain per step
+LAMBDA, LR, TOL = 1e-3, 0.01, 1e-8
+SEED = 20241203
+
+# algorithmic work per launch of the kernels that can dominate a step
+NS = CHAINS
+KERNEL_WORK = {
+    # (bound, amount per launch, unit of amount)
+    # forward product: reads T' (Ns x (m+1) float) and sigma' (Ns x (n+1) bf16), writes y
+    "fwd_mfma": ("hbm", NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4, "B"),
+    # backward product: reads T' and sigma' and y, writes the (n+1) x (m+1) partial grids
+    "bwd_mfma": ("hbm", NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4, "B"),
+    # sampler: one sweep of N proposals per chain; W column reads are L2-resident,
+    # HBM traffic = spins in/out + samples out (sigma int8, theta float)
+    "sample": ("hbm", NS * N * 3 + NS * M * 4, "B"),
+    # E_loc ratios: read theta (float) and spins, write ratios (double)
+    "eloc_ratio": ("hbm", NS * M * 4 + NS * N + NS * N * 8, "B"),
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def _ctx(yq, precision):
+    ctx = yq.Context(0)
+    ctx.rbm_create(N, M)
+    ctx.init_random(SEED, 0.01)
+    ctx.ham_set(yq.TFI, N, J=-1.0, h=-1.0)   # TFIsing(N, -1, -h) at h = 1
+    ctx.sampler_config(yq.LOCAL, CHAINS, precision=precision, seed=SEED)
+    return ctx
+
+
+def run_path(yq, precision, solver, warmup, steps, prof_label=None):
+    ctx = _ctx(yq, precision)
+    energies, outs = [], []
+    for i in range(warmup):
+        o = ctx.sr_step(n_sweeps=1, n_discard=20 if i == 0 else 0, solver=solver,
+                        precision=precision, lam=LAMBDA, cg_tol=TOL, cg_max_iter=50000,
+                        lr=LR, allow_unconverged=True)
+        energies.append(o["energy"])
+    if prof_label:
+        ctx._L.yq_prof_enable(ctx._h, prof_label.encode())
+    t0 = time.perf_counter()
+    for i in range(steps):
+        o = ctx.sr_step(n_sweeps=1, n_discard=0 if warmup else (20 if i == 0 else 0),
+                        solver=solver, precision=precision, lam=LAMBDA, cg_tol=TOL,
+                        cg_max_iter=50000, lr=LR, allow_unconverged=True)
+        outs.append(o)
+        energies.append(o["energy"])
+    t1 = time.perf_counter()   # every sr_step returns after its stream synchronised
+    prof = None
+    if prof_label:
+        import ctypes as C
+        tot, cnt = C.c_double(), C.c_longlong()
+        for lab in prof_label.split(","):
+            ctx._L.yq_prof_read(ctx._h, lab.encode(), C.byref(tot), C.byref(cnt))
+            prof = prof or {}
+            prof[lab] = (tot.value, cnt.value)
+        ctx._L.yq_prof_enable(ctx._h, None)
+    ctx.close()
+    return (t1 - t0), outs, energies, prof
+
+
+def cpu_baseline(iters_per_step, seconds=20.0):
+    """The oracle's threaded C++ restatement of the reference CG step
+    (SamplerMT + constructDelta/constructObs + SRMat::solveCG), timed on a
+    bounded sample of chains and scaled linearly in samples."""
+    import oracle as orc
+    threads = min(16, orc.lib().bl_threads())
+    rng = np.random.default_rng(SEED)
+    rbm = orc.RBM.random(N, M, 0.01, rng)
+    H = orc.ham(orc.TFI, N, J=-1.0, h=-1.0)
+    c = 2048
+    sig = orc.randomize(SEED, 0, np.arange(c), N)
+    t0 = time.perf_counter()
+    fin, ctr, acc, ss, st = orc.sample_replay(rbm, sig, seed=SEED, n_sweeps=1, n_discard=0,
+                                              threads=threads)
+    t_samp = time.perf_counter() - t0
+    import ctypes as Cc
+    el = np.empty(c)
+    t0 = time.perf_counter()
+    orc.lib().bl_eloc(Cc.byref(H), c, N, M, orc._p(rbm.W), orc._p(rbm.a), orc._p(ss),
+                      orc._p(st), orc._p(el), threads)
+    t_eloc = time.perf_counter() - t0
+    O = np.empty((c, DIM))
+    om, F = np.empty(DIM), np.empty(DIM)
+    t0 = time.perf_counter()
+    orc.lib().bl_sr_build(c, N, M, 1, orc._p(ss), orc._p(st), orc._p(el), orc._p(O),
+                          orc._p(om), orc._p(F), threads)
+    t_build = time.perf_counter() - t0
+    k = 20
+    x = np.empty(DIM)
+    it, err = Cc.c_int(), Cc.c_double()
+    t0 = time.perf_counter()
+    orc.lib().bl_cg_solve(c, DIM, orc._p(O), LAMBDA, orc._p(F), 1e-30, k, orc._p(x),
+                          Cc.byref(it), Cc.byref(err), threads)
+    t_cg_iter = (time.perf_counter() - t0) / max(it.value, 1)
+    scale = CHAINS / c
+    t_step = scale * (t_samp + t_eloc + t_build + t_cg_iter * iters_per_step)
+    return {
+        "value": 1.0 / t_step, "unit": "SR steps/s", "cores": threads, "kind": "port",
+        "sample": (f"{c} chains x 1 sweep, E_loc, O build and {it.value} CG applies on "
+                   f"{c} samples at N={N}, M={M}, scaled x{scale:.0f} to {CHAINS} samples "
+                   f"with {iters_per_step} CG iterations per step"),
+        "phase_s_at_full_size": {"sample": scale * t_samp, "eloc": scale * t_eloc,
+                                 "build_O_F": scale * t_build,
+                                 "cg": scale * t_cg_iter * iters_per_step},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import yannq_amd as yq
+
+    # pick the dominant kernel from a profiled warm-up of the timed path
+    labels = ",".join(KERNEL_WORK)
+    _, _, _, prof0 = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, max(args.warmup, 1), 2,
+                              prof_label=labels)
+    dominant = max(prof0, key=lambda k: prof0[k][0])
+
+    dt, outs, energies, prof = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, args.warmup,
+                                        args.steps, prof_label=dominant)
+    ms = 1e3 * dt / args.steps
+    value = args.steps / dt
+    iters = [o["cg_iters"] for o in outs]
+    tot_ms, launches = prof[dominant]
+    bound, amount, unit = KERNEL_WORK[dominant]
+    avg_s = tot_ms / 1e3 / max(launches, 1)
+    achieved = amount / avg_s / 1e9
+    roofline = {"bound": bound, "kernel": dominant, "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": launches,
+                "algorithmic_bytes_per_launch": amount}
+
+    extra = {}
+    if not args.no_fp64:
+        k = args.fp64_steps
+        dte, oute, ene, _ = run_path(yq, yq.FP64, yq.SOLVE_EXPLICIT, args.warmup, k)
+        dtc, outc, enc, _ = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, args.warmup, k)
+        extra = {
+            "fp64_explicit_steps_per_s": k / dte,
+            "fp64_factored_cg_steps_per_s": k / dtc,
+            "mixed_vs_fp64_explicit": value / (k / dte),
+            "mixed_vs_fp64_factored_cg": value / (k / dtc),
+            "fp64_cg_iters": [o["cg_iters"] for o in outc],
+            # same seeds and start state: energies of the first steps of each path
+            "energy_rel_diff_mixed_vs_fp64": float(np.max(np.abs(
+                np.array(energies[:len(ene)]) - np.array(ene)) / np.abs(np.array(ene)))),
+        }
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(int(np.median(iters)))
+    line = {
+        "metric": "SR steps/sec (sample+E_loc+O+SR solve), TFI N=100 RBM alpha=4, mixed vs FP64",
+        "value": value, "unit": "SR steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "1D TFI PBC N=100 h=1 (J=-1), real RBM alpha=4 (40500 params), "
+                              "16384 chains x 1 sweep = 16384 samples/step, persistent chains",
+                   "precision": "mixed: f32 sampling/E_loc, split-bf16x3 MFMA CG products "
+                                "(f32 accumulate), f64 refinement to 1e-8, f64 update",
+                   "lambda": LAMBDA, "lr": LR, "cg_tol": TOL},
+        "samples_per_s": value * CHAINS,
+        "cg_iters": iters,
+        "energy": energies[-1],
+        "phase_ms": {"sample": float(np.mean([o["sample_ms"] for o in outs])),
+                     "eloc_build_solve_update": float(np.mean([o["solve_ms"] for o in outs]))},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,590 @@
+// Header-only C++17 layer that rebuilds yannq's compile-time concepts --
+// Machine (RBM), Hamiltonian, Sweeper, Sampler (SamplerMT), SR (SRMat),
+// Optimizer and Runner (RunRBM) -- on top of the C ABI in yannq_amd.h, so
+// code written against the reference's template API drops onto the GPU.
+//
+// Member names, argument meaning and defaults follow the reference
+// (citations are paths under the yannq source tree).  Differences:
+//   * std::vector replaces Eigen vectors (Eigen is not a dependency);
+//   * C-ABI failures become yannq_amd::Error (a std::runtime_error) carrying
+//     the yq_status code -- the reference mostly fails silently;
+//   * random streams are counter-based Philox keyed by a seed, so
+//     initializeRandomEngine() only re-keys the streams.
+#pragma once
+
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <map>
+#include <memory>
+#include <ostream>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../yannq_amd.h"
+
+namespace yannq_amd {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+namespace detail {
+inline void check(yq_ctx* c, int rc, int allow = -1) {
+  if (rc != YQ_OK && rc != allow)
+    throw Error(rc, std::string("yannq_amd: ") + yq_last_error(c));
+}
+}  // namespace detail
+
+// One GPU context; every object below that lives on the device refers to one.
+class Device {
+ public:
+  explicit Device(int device = 0) {
+    const int rc = yq_ctx_create(device, &ctx_);
+    if (rc != YQ_OK) throw Error(rc, "yannq_amd: cannot create a context on the GPU");
+  }
+  ~Device() { yq_ctx_destroy(ctx_); }
+  Device(const Device&) = delete;
+  Device& operator=(const Device&) = delete;
+  yq_ctx* ctx() const { return ctx_; }
+
+ private:
+  yq_ctx* ctx_ = nullptr;
+};
+
+// ------------------------------------------------------------ Machine
+// Machines/RBM.hpp:22-407 (real double precision).
+template <typename T = double>
+class RBM {
+  static_assert(std::is_same<T, double>::value, "this build supports RBM<double>");
+
+ public:
+  using Scalar = T;
+  using RealScalar = T;
+  using Vector = std::vector<T>;
+  using DataT = std::tuple<std::vector<int>, Vector>;
+
+  RBM(Device& dev, uint32_t n, uint32_t m, bool useBias = true)
+      : dev_(&dev), n_(n), m_(m), useBias_(useBias) {
+    detail::check(ctx(), yq_rbm_create(ctx(), 1, (int)n, (int)m, useBias ? 1 : 0));
+  }
+  yq_ctx* ctx() const { return dev_->ctx(); }
+
+  uint32_t getN() const { return n_; }
+  uint32_t getM() const { return m_; }
+  uint32_t getDim() const { return useBias_ ? n_ * m_ + n_ + m_ : n_ * m_; }  // :124-130
+  bool useBias() const { return useBias_; }
+  std::string desc() const {  // :104-113
+    return "{\"name\": \"RBM\", \"useBias\": " + std::string(useBias_ ? "true" : "false") +
+           ", \"n\": " + std::to_string(n_) + ", \"m\": " + std::to_string(m_) + "}";
+  }
+
+  Vector getParams() const {  // :271-281
+    Vector v(getDim());
+    detail::check(ctx(), yq_rbm_get_params(ctx(), v.data()));
+    return v;
+  }
+  void setParams(const Vector& v) {  // :283-291
+    if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "setParams: wrong size");
+    detail::check(ctx(), yq_rbm_set_params(ctx(), v.data()));
+  }
+  void updateParams(const Vector& v) {  // :261-269
+    if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "updateParams: wrong size");
+    detail::check(ctx(), yq_rbm_update_params(ctx(), v.data()));
+  }
+  // :301-322; the engine only supplies the Philox key
+  template <class RandomEngine>
+  void initializeRandom(RandomEngine& re, T sigma = 1e-3) {
+    const uint64_t seed = ((uint64_t)re() << 32) ^ (uint64_t)re();
+    detail::check(ctx(), yq_rbm_init_random(ctx(), seed, sigma));
+  }
+  bool hasNaN() const {  // :293-296
+    int out = 0;
+    detail::check(ctx(), yq_rbm_has_nan(ctx(), &out));
+    return out != 0;
+  }
+  // W(j, i), A(i), B(j) readers (:205-216) through a parameter copy
+  T W(uint32_t j, uint32_t i) const { return getParams()[(size_t)i * m_ + j]; }
+  T A(uint32_t i) const { return useBias_ ? getParams()[(size_t)n_ * m_ + i] : T(0); }
+  T B(uint32_t j) const { return useBias_ ? getParams()[(size_t)n_ * m_ + n_ + j] : T(0); }
+
+ private:
+  Device* dev_;
+  uint32_t n_, m_;
+  bool useBias_;
+};
+
+// ------------------------------------------------------------ Hamiltonians
+// E_loc functors become descriptors; the ED column functors stay on the host.
+class TFIsing {  // Hamiltonians/TFIsing.hpp
+ public:
+  TFIsing(int n, double J, double h) : n_(n), J_(J), h_(h) {}
+  yq_ham_desc desc() const { return yq_ham_desc{YQ_TFI, n_, J_, h_, 0.0, 0.0, 0}; }
+  std::map<uint32_t, double> operator()(uint32_t col) const {  // :53-65
+    std::map<uint32_t, double> res;
+    for (int i = 0; i < n_; i++) {
+      const int s1 = (col >> i) & 1, s2 = (col >> ((i + 1) % n_)) & 1;
+      res[col ^ (1u << i)] += h_;
+      res[col] += J_ * (1 - 2 * s1) * (1 - 2 * s2);
+    }
+    return res;
+  }
+  std::string params() const {
+    return "{\"name\": \"TFIsing\", \"n\": " + std::to_string(n_) + ", \"J\": " +
+           std::to_string(J_) + ", \"h\": " + std::to_string(h_) + "}";
+  }
+
+ private:
+  int n_;
+  double J_, h_;
+};
+
+class XXZ {  // Hamiltonians/XXZ.hpp
+ public:
+  XXZ(int n, double J, double Delta, bool signRule = false)
+      : n_(n), J_(J), Delta_(Delta), sign_(signRule ? -1.0 : 1.0) {}
+  yq_ham_desc desc() const {
+    return yq_ham_desc{YQ_XXZ, n_, J_, 0.0, Delta_, 0.0, sign_ < 0 ? 1 : 0};
+  }
+  std::map<uint32_t, double> operator()(uint32_t col) const {  // :51-64
+    std::map<uint32_t, double> res;
+    for (int i = 0; i < n_; i++) {
+      const int b1 = (col >> i) & 1, b2 = (col >> ((i + 1) % n_)) & 1;
+      const int zz = (1 - 2 * b1) * (1 - 2 * b2);
+      res[col] += J_ * Delta_ * zz;
+      res[col ^ ((1u << i) | (1u << ((i + 1) % n_)))] += sign_ * J_ * (1 - zz);
+    }
+    return res;
+  }
+
+ private:
+  int n_;
+  double J_, Delta_, sign_;
+};
+
+class J1J2 {  // Hamiltonians/J1J2.hpp
+ public:
+  J1J2(int n, double J1, double J2, bool signRule)
+      : n_(n), J1_(J1), J2_(J2), sign_(signRule ? -1 : 1) {}
+  yq_ham_desc desc() const {
+    return yq_ham_desc{YQ_J1J2, n_, J1_, 0.0, 0.0, J2_, sign_ < 0 ? 1 : 0};
+  }
+  std::map<uint32_t, double> operator()(uint32_t col) const {  // :57-79
+    std::map<uint32_t, double> res;
+    for (int d = 1; d <= 2; ++d)
+      for (int i = 0; i < n_; i++) {
+        const int b1 = (col >> i) & 1, b2 = (col >> ((i + d) % n_)) & 1;
+        const int zz = (1 - 2 * b1) * (1 - 2 * b2);
+        const double Jd = d == 1 ? J1_ : J2_;
+        res[col ^ ((1u << i) | (1u << ((i + d) % n_)))] +=
+            Jd * (d == 1 ? sign_ : 1) * (1.0 - zz * 1.0);
+        res[col] += Jd * zz;
+      }
+    return res;
+  }
+
+ private:
+  int n_;
+  double J1_, J2_;
+  int sign_;
+};
+
+// ------------------------------------------------------------ Sweepers
+class LocalSweeper {  // Samplers/LocalSweeper.hpp:11-54
+ public:
+  explicit LocalSweeper(uint32_t n, uint32_t nSweep = 1) : n_(n), nSweep_(nSweep) {}
+  static constexpr int kind = YQ_LOCAL;
+  uint32_t nSweep() const { return nSweep_; }
+  std::string desc() const {
+    return "{\"name\": \"Local Sweeper\", \"num_sweep_per\": " + std::to_string(nSweep_) + "}";
+  }
+
+ private:
+  uint32_t n_, nSweep_;
+};
+
+class SwapSweeper {  // Samplers/SwapSweeper.hpp:8-57
+ public:
+  explicit SwapSweeper(uint32_t n, uint32_t nSweep = 1) : n_(n), nSweep_(nSweep) {}
+  static constexpr int kind = YQ_EXCHANGE;
+  uint32_t nSweep() const { return nSweep_; }
+  std::string desc() const {
+    return "{\"name\": \"Swap Sweeper\", \"num_sweeps_per\": " + std::to_string(nSweep_) + "}";
+  }
+
+ private:
+  uint32_t n_, nSweep_;
+};
+
+// ------------------------------------------------------------ Sampler
+// The batch of one sample() call; it stays on the device.  size() and
+// operator[] (SamplerMT.hpp:308-316's container) copy out on first access.
+template <class Machine>
+class SampleBatch {
+ public:
+  using DataT = typename Machine::DataT;
+  SampleBatch(const Machine& qs, int ns) : qs_(&qs), ns_(ns) {}
+  std::size_t size() const { return (std::size_t)ns_; }
+  DataT operator[](std::size_t idx) const {
+    fetch();
+    const int n = (int)qs_->getN(), m = (int)qs_->getM();
+    std::vector<int> s(sig_.begin() + idx * n, sig_.begin() + (idx + 1) * n);
+    typename Machine::Vector t(theta_.begin() + idx * m, theta_.begin() + (idx + 1) * m);
+    return DataT(std::move(s), std::move(t));
+  }
+
+ private:
+  void fetch() const {
+    if (!sig_.empty()) return;
+    sig8_.resize((size_t)ns_ * qs_->getN());
+    theta_.resize((size_t)ns_ * qs_->getM());
+    detail::check(qs_->ctx(), yq_samples_get(qs_->ctx(), sig8_.data(), theta_.data()));
+    sig_.assign(sig8_.begin(), sig8_.end());
+  }
+  const Machine* qs_;
+  int ns_;
+  mutable std::vector<int8_t> sig8_;
+  mutable std::vector<int> sig_;
+  mutable std::vector<double> theta_;
+};
+
+// Samplers/SamplerMT.hpp:15-205 with nTmps == 1.
+template <class Machine, class Sweeper>
+class SamplerMT {
+ public:
+  using Scalar = typename Machine::Scalar;
+  using RealScalar = typename Machine::RealScalar;
+
+  SamplerMT(const Machine& qs, uint32_t nTmps, uint32_t nChainsPer, Sweeper& sweeper,
+            uint64_t seed = 0x5eed, int precision = YQ_FP64)
+      : qs_(qs), nChainsPer_(nChainsPer), sweeper_(sweeper), seed_(seed),
+        precision_(precision) {
+    if (nTmps != 1) throw Error(YQ_ERR_UNSUPPORTED, "SamplerMT: only nTmps == 1");
+    configure();
+  }
+  std::string desc() const {
+    return "{\"name\": \"SamplerMT\", \"num_temps\": 1, \"num_chains_per_each_temp\": " +
+           std::to_string(nChainsPer_) + ", \"sweeper\": " + sweeper_.desc() + "}";
+  }
+  // :185-192 -- re-keys the counter-based streams (keeps the seed by default)
+  void initializeRandomEngine() { configure(); }
+  void initializeRandomEngine(uint64_t seed) {
+    seed_ = seed;
+    configure();
+  }
+  void randomizeSigma() { detail::check(ctx(), yq_sampler_randomize(ctx(), -1)); }  // :194-202
+  void randomizeSigma(int nup) { detail::check(ctx(), yq_sampler_randomize(ctx(), nup)); }
+  // :216-223: randomizer(re) -> configuration, called once per chain
+  template <class Randomizer>
+  void randomize(Randomizer&& randomizer) {
+    std::mt19937_64 re(seed_ ^ (++epoch_ * 0x9E3779B97F4A7C15ull));
+    const uint32_t n = qs_.getN();
+    std::vector<int8_t> all((size_t)nChainsPer_ * n);
+    for (uint32_t c = 0; c < nChainsPer_; ++c) {
+      const auto s = randomizer(re);
+      if ((uint32_t)s.size() != n) throw Error(YQ_ERR_ARG, "randomizer: wrong length");
+      for (uint32_t i = 0; i < n; ++i) all[(size_t)c * n + i] = (int8_t)s[i];
+    }
+    detail::check(ctx(), yq_sampler_set_sigma(ctx(), all.data()));
+  }
+  // :293-317
+  SampleBatch<Machine> sample(uint32_t nSweeps, uint32_t nDiscard) {
+    int ns = 0;
+    detail::check(ctx(), yq_sample(ctx(), (int)nSweeps, (int)nDiscard, &ns));
+    return SampleBatch<Machine>(qs_, ns);
+  }
+  int precision() const { return precision_; }
+
+ private:
+  yq_ctx* ctx() const { return qs_.ctx(); }
+  void configure() {
+    detail::check(ctx(), yq_sampler_config(ctx(), Sweeper::kind, 1, (int)nChainsPer_,
+                                           (int)sweeper_.nSweep(), precision_, seed_));
+  }
+  const Machine& qs_;
+  uint32_t nChainsPer_;
+  Sweeper& sweeper_;
+  uint64_t seed_;
+  int precision_;
+  uint64_t epoch_ = 0;
+};
+
+// ------------------------------------------------------------ SR
+// GroundState/SRMat.hpp:33-239.  The solver choice (explicit S + Cholesky,
+// CG on the materialised O, or CG on the factored O) and the precision are
+// fixed at construction.
+template <class Machine, class Hamiltonian>
+class SRMat {
+ public:
+  using Scalar = typename Machine::Scalar;
+  using RealScalar = typename Machine::RealScalar;
+  using Vector = std::vector<Scalar>;
+
+  SRMat(const Machine& qs, const Hamiltonian& ham, int solver = YQ_SOLVE_EXPLICIT,
+        int precision = YQ_FP64)
+      : qs_(qs), ham_(ham), solver_(solver), precision_(precision) {}
+
+  long rows() const { return (long)qs_.getDim(); }
+  long cols() const { return (long)qs_.getDim(); }
+
+  // :89-103
+  template <class SamplingResult>
+  void constructFromSamples(SamplingResult&& sr) {
+    (void)sr;  // the batch of the last sample() call is on the device
+    const yq_ham_desc d = ham_.desc();
+    detail::check(ctx(), yq_ham_set(ctx(), &d));
+    double mean = 0.0, var = 0.0;
+    detail::check(ctx(), yq_eloc(ctx(), nullptr, &mean, &var));
+    eloc_ = mean;
+    elocVar_ = var;
+    detail::check(ctx(), yq_sr_build(ctx(), solver_, precision_));
+  }
+  void setShift(RealScalar shift) { shift_ = shift; }
+  RealScalar getShift() const { return shift_; }
+  RealScalar eloc() const { return eloc_; }        // :151-154
+  RealScalar elocVar() const { return elocVar_; }  // :156-159
+  Vector energyGrad() const { return item(YQ_SR_F); }  // :161-168
+  Vector oloc() const { return item(YQ_SR_OMEAN); }   // :132-140
+  // :142-149, dim x dim column-major (explicit solver only)
+  std::vector<Scalar> corrMat() const {
+    const size_t d = qs_.getDim();
+    std::vector<Scalar> S(d * d);
+    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_S, S.data()));
+    return S;
+  }
+  Vector apply(const Vector& rhs) const {  // :170-189
+    Vector out(rhs.size());
+    detail::check(ctx(), yq_sr_apply(ctx(), shift_, rhs.data(), out.data()));
+    return out;
+  }
+  // :197-223.  Unconverged CG returns its iterate, as Eigen's solve() does.
+  Vector solveCG(RealScalar shift, RealScalar tol = 1e-4) { return solveCG(energyGrad(), shift, tol); }
+  Vector solveCG(const Vector& vec, double shift, double tol = 1e-4) {
+    setShift(shift);
+    Vector x(vec.size());
+    double res = 0.0;
+    int it = 0;
+    detail::check(ctx(), yq_sr_solve(ctx(), shift, tol, 2 * (int)vec.size(), vec.data(),
+                                     x.data(), &res, &it),
+                  YQ_ERR_NOT_CONVERGED);
+    lastIters_ = it;
+    lastResid_ = res;
+    return x;
+  }
+  Vector solveExact(RealScalar shift) { return solveExact(energyGrad(), shift); }  // :231-237
+  // RunRBM.hpp:194-203: Cholesky of S + shift I applied to an averaged gradient
+  Vector solveExact(const Vector& rhs, RealScalar shift) {
+    setShift(shift);
+    Vector x(qs_.getDim());
+    double res = 0.0;
+    int it = 0;
+    detail::check(ctx(), yq_sr_solve(ctx(), shift, 0.0, 0, rhs.data(), x.data(), &res, &it));
+    lastResid_ = res;
+    return x;
+  }
+  int lastIterations() const { return lastIters_; }
+  double lastResidual() const { return lastResid_; }
+  int solver() const { return solver_; }
+
+ private:
+  yq_ctx* ctx() const { return qs_.ctx(); }
+  Vector item(int which) const {
+    Vector v(qs_.getDim());
+    detail::check(ctx(), yq_sr_get(ctx(), which, v.data()));
+    return v;
+  }
+  const Machine& qs_;
+  const Hamiltonian& ham_;
+  int solver_, precision_;
+  RealScalar shift_ = 0.0, eloc_ = 0.0, elocVar_ = 0.0;
+  int lastIters_ = 0;
+  double lastResid_ = 0.0;
+};
+
+// ------------------------------------------------------------ Optimizers
+template <typename T>
+class Optimizer {  // Optimizers/Optimizer.hpp:10-22
+ public:
+  using Vector = std::vector<T>;
+  virtual std::string desc() const = 0;
+  virtual Vector getUpdate(const Vector& grad) = 0;
+  virtual ~Optimizer() {}
+};
+
+template <typename T>
+class SGD : public Optimizer<T> {  // Optimizers/SGD.hpp
+ public:
+  using typename Optimizer<T>::Vector;
+  explicit SGD(double alpha = 0.05, double p = 0.0) : alpha_(alpha), p_(p), t_(0) {}
+  std::string desc() const override {
+    return "{\"name\": \"SGD\", \"alhpa\": " + std::to_string(alpha_) + ", \"p\": " +
+           std::to_string(p_) + "}";
+  }
+  double eta() const { return std::max(alpha_ / std::pow((double)t_, p_), 1e-4); }
+  Vector getUpdate(const Vector& v) override {  // :48-54
+    ++t_;
+    const double e = eta();
+    Vector out(v.size());
+    for (size_t i = 0; i < v.size(); ++i) out[i] = -e * v[i];
+    return out;
+  }
+  int step() const { return t_; }
+
+ private:
+  double alpha_, p_;
+  int t_;
+};
+
+// Optimizers/OptimizerFactory.hpp:84-90: unknown names throw invalid_argument.
+template <typename T>
+std::unique_ptr<Optimizer<T>> createOptimizer(const std::string& name, double alpha = 0.05,
+                                              double p = 0.0) {
+  if (name == "SGD") return std::unique_ptr<Optimizer<T>>(new SGD<T>(alpha, p));
+  throw std::invalid_argument("Such an optimizer does not exist.");
+}
+
+// ------------------------------------------------------------ Runner
+// Runners/RunRBM.hpp:41-222 with AbstractRunner.hpp's defaults (:32-38).
+template <typename T = double>
+class RunRBM {
+ public:
+  using Machine = RBM<T>;
+  using Vector = std::vector<T>;
+
+  RunRBM(Device& dev, unsigned N, unsigned alpha, bool useBias, std::ostream& logger)
+      : logger_(logger), qs_(dev, N, alpha * N, useBias), re_(std::random_device{}()) {}
+
+  void setLambda(double ini, double decay = 1.0, double min = 0.0) {
+    lambdaIni_ = ini;
+    lambdaDecay_ = decay;
+    lambdaMin_ = min;
+  }
+  void setIterParams(int maxIter, int saveWfPer) {
+    maxIter_ = maxIter;
+    saveWfPer_ = saveWfPer;
+  }
+  void setSolverParams(bool useCG, double cgTol = 1e-4) {
+    useCG_ = useCG;
+    cgTol_ = cgTol;
+  }
+  // The first-moment averaging (beta1) runs on the host; the fisher average
+  // (beta2 > 0) needs a dense S per epoch and is not supported.
+  void setMomentum(double beta1 = 0.0, double beta2 = 0.0) {
+    if (beta2 != 0.0) throw Error(YQ_ERR_UNSUPPORTED, "setMomentum: beta2 must be 0");
+    beta1_ = beta1;
+  }
+  void setGradClip(double c) { gradClip_ = c; }
+  void setOptimizer(const std::string& name, double alpha = 0.05, double p = 0.0) {
+    opt_ = createOptimizer<T>(name, alpha, p);
+  }
+  void setSeed(uint64_t s) { re_.seed(s); }
+  void initializeRandom(double wIni = 1e-3) {  // AbstractRunner.hpp:96-101
+    machineInitialized_ = true;
+    logger_ << "Set initial weights randomly from N(0.0, " << wIni << "^2)" << std::endl;
+    qs_.initializeRandom(re_, wIni);
+  }
+  Machine& getQs() { return qs_; }
+  uint32_t getDim() const { return qs_.getDim(); }
+  template <class Sweeper>
+  SamplerMT<Machine, Sweeper> createSampler(Sweeper& sweeper, uint32_t nTmps,
+                                            uint32_t nChainsPer, uint64_t seed = 0x5eed,
+                                            int precision = YQ_FP64) const {
+    return SamplerMT<Machine, Sweeper>(qs_, nTmps, nChainsPer, sweeper, seed, precision);
+  }
+
+  // :122-221.  callback(ll, E, |v|, cgErr, smp_ms, slv_ms).  The solver is
+  // explicit S + Cholesky unless setSolverParams(true, .), which selects CG
+  // on the factored O (precision given by the sampler).
+  template <class Sampler, class Callback, class Randomizer, class Hamiltonian>
+  void run(Sampler& sampler, Callback&& callback, Randomizer&& randomizer, Hamiltonian&& ham,
+           int nSweeps = -1, int nSweepsDiscard = -1) {
+    using Clock = std::chrono::high_resolution_clock;
+    if (!machineInitialized_) initializeRandom();
+    if (!opt_) setOptimizer("SGD");
+    const int solver = useCG_ ? YQ_SOLVE_CG_FACTORED : YQ_SOLVE_EXPLICIT;
+    const int precision = useCG_ ? sampler.precision() : YQ_FP64;
+    SRMat<Machine, typename std::decay<Hamiltonian>::type> srm(qs_, ham, solver, precision);
+    sampler.initializeRandomEngine();
+    const int dim = (int)getDim();
+    if (nSweeps == -1) nSweeps = dim;
+    if (nSweepsDiscard == -1) nSweepsDiscard = int(0.1 * nSweeps);
+    Vector gradAvg(dim, 0.0);
+    int t = 0;
+    for (int ll = 0; ll <= maxIter_; ll++) {
+      logger_ << "Epochs: " << ll << "\t# of discard samples: " << nSweepsDiscard << std::endl;
+      sampler.randomize(randomizer);
+      const auto smp_start = Clock::now();
+      auto sr = sampler.sample(nSweeps, nSweepsDiscard);
+      const auto smp_dur = std::chrono::duration_cast<std::chrono::milliseconds>(
+                               Clock::now() - smp_start).count();
+      logger_ << "Number of samples: " << sr.size() << std::endl;
+      const auto slv_start = Clock::now();
+      srm.constructFromSamples(sr);
+      const double currE = srm.eloc();
+      const double lambda = std::max(lambdaIni_ * std::pow(lambdaDecay_, ll), lambdaMin_);
+      // ObsAvg with beta1 (RunRBM.hpp:13-36): bias-corrected moving average
+      const Vector g = srm.energyGrad();
+      ++t;
+      for (int i = 0; i < dim; ++i) gradAvg[i] = beta1_ * gradAvg[i] + (1.0 - beta1_) * g[i];
+      Vector grad(dim);
+      for (int i = 0; i < dim; ++i) grad[i] = gradAvg[i] / (1.0 - std::pow(beta1_, t));
+      Vector v;
+      double cgErr = 0.0;
+      if (useCG_) {
+        v = srm.solveCG(grad, lambda, cgTol_);
+        cgErr = srm.lastResidual() * norm(grad);  // |apply(v) - grad|
+      } else {
+        v = srm.solveExact(grad, lambda);
+      }
+      const double nv = norm(v);
+      if (gradClip_ != 0.0 && nv > gradClip_)
+        for (auto& x : v) x *= gradClip_ / nv;
+      const Vector optV = opt_->getUpdate(v);
+      const auto slv_dur = std::chrono::duration_cast<std::chrono::milliseconds>(
+                               Clock::now() - slv_start).count();
+      qs_.updateParams(optV);
+      callback(ll, currE, nv, cgErr, smp_dur, slv_dur);
+    }
+  }
+
+ private:
+  static double norm(const Vector& v) {
+    double s = 0.0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+  }
+  std::ostream& logger_;
+  Machine qs_;
+  std::mt19937_64 re_;
+  std::unique_ptr<Optimizer<T>> opt_;
+  bool machineInitialized_ = false, useCG_ = false;
+  double cgTol_ = 1e-4, beta1_ = 0.0, gradClip_ = 0.0;
+  double lambdaIni_ = 1e-3, lambdaDecay_ = 1.0, lambdaMin_ = 1e-4;
+  int maxIter_ = 2000, saveWfPer_ = 100;
+};
+
+// Utilities/Utility.hpp:64-88 random configurations for randomize()
+template <class RandomEngine>
+std::vector<int> randomSigma(int n, RandomEngine& re) {
+  std::uniform_int_distribution<> uid(0, 1);
+  std::vector<int> s(n);
+  for (int i = 0; i < n; i++) s[i] = -2 * uid(re) + 1;
+  return s;
+}
+template <class RandomEngine>
+std::vector<int> randomSigma(int n, int nup, RandomEngine& re) {
+  std::vector<int> s(n, -1);
+  for (int i = nup; i < n; i++) s[i] = 1;
+  std::shuffle(s.begin(), s.end(), re);
+  return s;
+}
+
+}  // namespace yannq_amd

@@ -21,10 +21,13 @@ def orc():
 
 
 def _gpu_available():
+    """True when the HIP runtime reports a device (asks the library itself)."""
     try:
-        import torch  # noqa: F401  (only to count devices without initialising HIP)
-        return torch.cuda.device_count() > 0
-    except Exception:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        cnt = ctypes.c_int(0)
+        return hip.hipGetDeviceCount(ctypes.byref(cnt)) == 0 and cnt.value > 0
+    except OSError:
         return False
 
 

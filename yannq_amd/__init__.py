@@ -84,6 +84,7 @@ def lib():
         "yq_samples_set": [p, i, p, i], "yq_eloc": [p, p, p, p],
         "yq_sr_build": [p, i, i], "yq_sr_get": [p, i, p], "yq_sr_apply": [p, d, p, p],
         "yq_sr_solve": [p, d, d, i, p, p, p, p], "yq_sr_step": [p, p, p],
+        "yq_prof_enable": [p, C.c_char_p], "yq_prof_read": [p, C.c_char_p, p, p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
