@@ -190,6 +190,49 @@ typedef struct {
  * update, all on the device. */
 YQ_API int yq_sr_step(yq_ctx* ctx, const yq_step_cfg* cfg, yq_step_out* out);
 
+/* ---- FeedForward machine: Machines/FeedForward.hpp --------------------- */
+/* The network of the C4 configuration: Conv1D(1 -> channels, kernel width,
+ * periodic, stride 1; layers/Conv1D.hpp) -> LnCosh -> FullyConnected(
+ * channels*n -> hidden; layers/FullyConnected.hpp) -> Tanh -> Mean
+ * (layers/Mean.hpp), all without bias (the layer defaults).  dtype 0 = float,
+ * 1 = double.  Parameters (doubles on the host) in FeedForward's order:
+ * [conv kernel, index ki + oc*kernel | FC weight W(i, o) at i + o*channels*n]. */
+typedef struct {
+  int n, channels, kernel, hidden;
+  int dtype;
+} yq_ff_desc;
+/* FeedForward() + addLayer(...) x5: FeedForward.hpp:76-82 */
+YQ_API int yq_ff_create(yq_ctx* ctx, const yq_ff_desc* desc);
+/* getDim: number of parameters */
+YQ_API int yq_ff_npar(const yq_ctx* ctx, int* npar);
+/* setParams / getParams / updateParams */
+YQ_API int yq_ff_set_params(yq_ctx* ctx, const double* params);
+YQ_API int yq_ff_get_params(yq_ctx* ctx, double* params);
+YQ_API int yq_ff_update_params(yq_ctx* ctx, const double* delta);
+/* initializeRandom(re, mode): FeedForward.hpp:43-73.  mode 0 LeCun, 1
+ * Xavier (Gaussian sigma = sqrt(2/(fanIn+fanOut)), the reference's), 2 He,
+ * 3 Xavier-uniform U(+-sqrt(6/(fanIn+fanOut))); Philox draws keyed by seed. */
+YQ_API int yq_ff_init_random(yq_ctx* ctx, uint64_t seed, int mode);
+/* forward(sigma) for ns configurations (FeedForward.hpp:152-168) */
+YQ_API int yq_ff_forward(yq_ctx* ctx, int ns, const int8_t* sigma,
+                         double* logpsi);
+/* backward(makeData(sigma)) for ns configurations: rows of d log psi / d
+ * params (ns x npar, FeedForward.hpp:186-205); logpsi may be null. */
+YQ_API int yq_ff_jacobian(yq_ctx* ctx, int ns, const int8_t* sigma,
+                          double* jac, double* logpsi);
+/* Device-pointer forms for framework interop (e.g. a torch.autograd
+ * Function).  Arrays are in the machine dtype on the context's device; the
+ * caller finishes its own work on them before the call, which returns after
+ * completion.  act (ns x (2*channels*n + hidden)) stores the activations in
+ * yq_ff_forward_dev when non-null; yq_ff_vjp_dev reads them when non-null and
+ * recomputes them otherwise.  grad = sum_s r_s d log psi_s / d params. */
+YQ_API int yq_ff_forward_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
+                             void* d_logpsi, void* d_act);
+YQ_API int yq_ff_jacobian_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
+                              void* d_jac, void* d_logpsi);
+YQ_API int yq_ff_vjp_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
+                         const void* d_act, const void* d_r, void* d_grad);
+
 /* ---- instrumentation (no reference counterpart) ------------------------- */
 /* Time every launch of the named kernels with HIP events on the launch's
  * stream.  labels: comma-separated kernel labels (e.g. "sample,bwd_mfma"),

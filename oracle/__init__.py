@@ -72,11 +72,14 @@ def lib():
             "rp_sample": (i, [p, i, p, p, p, p, p, p, p, i, i, p, p]),
             "rp_eloc": (None, [p, i, i, i, i, p, p, p, p, p]),
             "rp_randomize": (None, [u64, u64, i, p, i, i, p]),
+            "or_ff_npar": (i, [p]),
+            "or_ff_forward": (d, [p, p, p]),
+            "or_ff_jacobian": (d, [p, p, p, p]),
             "bl_threads": (i, []),
             "bl_sample": (i, [p, i, p, p, p, p, p, p, p, i, i, p, p, i]),
             "bl_eloc": (None, [p, i, i, i, p, p, p, p, p, i]),
             "bl_sr_build": (None, [i, i, i, i, p, p, p, p, p, p, i]),
-            "bl_cg_solve": (i, [i, i, p, d, p, d, i, p, p, p, p, i]),
+            "bl_cg_solve": (i, [i, i, p, d, p, d, i, p, p, p, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -297,3 +300,31 @@ def sample_replay(rbm, sigma, *, seed, n_sweeps, n_discard, kind=LOCAL, precisio
     if rc:
         raise ValueError("bad replay configuration")
     return sigma, ctr, acc, out_s, out_t
+
+
+# ---------------------------------------------------------------- FeedForward
+class FFDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("channels", C.c_int), ("kernel", C.c_int),
+                ("hidden", C.c_int)]
+
+
+def ff_npar(desc):
+    return lib().or_ff_npar(C.byref(desc))
+
+
+def ff_forward(desc, params, sigma):
+    params, sigma = f64(params), i8(np.atleast_2d(sigma))
+    L = lib()
+    return np.array([L.or_ff_forward(C.byref(desc), _p(params), _p(sigma[s]))
+                     for s in range(sigma.shape[0])])
+
+
+def ff_jacobian(desc, params, sigma):
+    """Rows d log psi / d params and the outputs log psi."""
+    params, sigma = f64(params), i8(np.atleast_2d(sigma))
+    L = lib()
+    J = np.empty((sigma.shape[0], ff_npar(desc)))
+    out = np.empty(sigma.shape[0])
+    for s in range(sigma.shape[0]):
+        out[s] = L.or_ff_jacobian(C.byref(desc), _p(params), _p(sigma[s]), _p(J[s]))
+    return J, out

@@ -347,6 +347,88 @@ void or_sgd_update(int dim, double alpha, double p, int t, const double* v,
   for (int i = 0; i < dim; ++i) out[i] = -eta * v[i];
 }
 
+// ============================================================ FeedForward
+int or_ff_npar(const or_ff_desc* d) {
+  return d->kernel * d->channels + d->channels * d->n * d->hidden;
+}
+
+namespace {
+struct FFAct {
+  std::vector<double> in, z, a1, u, a2;
+  double out;
+};
+// forward of every layer, keeping the activations (FeedForward::makeData,
+// FeedForward.hpp:170-184)
+void ff_forward_all(const or_ff_desc* d, const double* p, const int8_t* sigma, FFAct& A) {
+  const int n = d->n, C = d->channels, K = d->kernel, H = d->hidden, CN = C * n;
+  const double* ker = p;            // (ki + ic*K) + oc*(in*K), in = 1
+  const double* W = p + K * C;      // W(i, o) = W[i + o*CN]
+  A.in.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) A.in[i] = sgn(sigma[i]);
+  // Conv1D::forward, layers/Conv1D.hpp:144-171 (stride 1, no bias)
+  A.z.assign(CN, 0.0);
+  for (int oc = 0; oc < C; ++oc)
+    for (int r = 0; r < n; ++r)
+      for (int ki = 0; ki < K; ++ki)
+        A.z[r + oc * n] += ker[ki + oc * K] * A.in[(r + ki - K / 2 + n) % n];
+  // LnCosh, activations.hpp:83-87
+  A.a1.resize(CN);
+  for (int i = 0; i < CN; ++i) A.a1[i] = or_logcosh(A.z[i]);
+  // FullyConnected::forward, FullyConnected.hpp:150-157 (output = W^T x)
+  A.u.assign(H, 0.0);
+  for (int o = 0; o < H; ++o) {
+    double acc = 0.0;
+    for (int i = 0; i < CN; ++i) acc += W[i + (size_t)o * CN] * A.a1[i];
+    A.u[o] = acc;
+  }
+  // Tanh with constant 1, activations.hpp:127-132
+  A.a2.resize(H);
+  for (int o = 0; o < H; ++o) A.a2[o] = tanh(A.u[o]);
+  // Mean({H, 1}, axis 0), layers/Mean.hpp:92-103
+  double s = 0.0;
+  for (int o = 0; o < H; ++o) s += A.a2[o];
+  A.out = s / H;
+}
+}  // namespace
+
+double or_ff_forward(const or_ff_desc* d, const double* params, const int8_t* sigma) {
+  FFAct A;
+  ff_forward_all(d, params, sigma, A);
+  return A.out;
+}
+
+// FeedForward::backward, FeedForward.hpp:186-205, dout = 1, layers in reverse
+double or_ff_jacobian(const or_ff_desc* d, const double* p, const int8_t* sigma,
+                      double* der) {
+  const int n = d->n, C = d->channels, K = d->kernel, H = d->hidden, CN = C * n;
+  const double* W = p + K * C;
+  FFAct A;
+  ff_forward_all(d, p, sigma, A);
+  // Mean::backprop (Mean.hpp:106-125): every input gets dout / H
+  std::vector<double> d2(H, 1.0 / H);
+  // Tanh::ApplyJacobian (activations.hpp:139-144): G = F (1 - A^2)
+  for (int o = 0; o < H; ++o) d2[o] = d2[o] * (1. - A.a2[o] * A.a2[o]);
+  // FullyConnected::backprop (FullyConnected.hpp:160-179): dW = x dout^T, din = W dout
+  double* derW = der + K * C;
+  for (int o = 0; o < H; ++o)
+    for (int i = 0; i < CN; ++i) derW[i + (size_t)o * CN] = A.a1[i] * d2[o];
+  std::vector<double> d1(CN, 0.0);
+  for (int i = 0; i < CN; ++i) {
+    double acc = 0.0;
+    for (int o = 0; o < H; ++o) acc += W[i + (size_t)o * CN] * d2[o];
+    d1[i] = acc;
+  }
+  // LnCosh::ApplyJacobian (activations.hpp:93-97): G = F tanh(Z)
+  for (int i = 0; i < CN; ++i) d1[i] = d1[i] * tanh(A.z[i]);
+  // Conv1D::backprop weight part (layers/Conv1D.hpp:200-214)
+  for (int q = 0; q < K * C; ++q) der[q] = 0.0;
+  for (int oc = 0; oc < C; ++oc)
+    for (int r = 0; r < n; ++r)
+      for (int ki = 0; ki < K; ++ki)
+        der[ki + oc * K] += A.in[(r + ki - K / 2 + n) % n] * d1[r + oc * n];
+  return A.out;
+}
+
 // ============================================================ replay
 void rp_theta(int n, int m, const double* W, const double* b,
               const int8_t* sigma, double* theta) {

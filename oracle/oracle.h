@@ -61,6 +61,20 @@ int or_cg_solve(int ns, int dim, const double* Oc, const double* weights,
 void or_sgd_update(int dim, double alpha, double p, int t, const double* v,
                    double* out);
 
+// ---- FeedForward: Conv1D(1->C, K, periodic) -> LnCosh -> FC(C*n -> H) ->
+// Tanh -> Mean, no biases (the layer defaults).  Parameters [conv kernel
+// (K*C, index ki + oc*K) | FC weight (C*n x H column-major)].
+typedef struct {
+  int n, channels, kernel, hidden;
+} or_ff_desc;
+int or_ff_npar(const or_ff_desc* d);
+// log psi(sigma) = network output (FeedForward.hpp:152-168)
+double or_ff_forward(const or_ff_desc* d, const double* params,
+                     const int8_t* sigma);
+// d log psi / d params (FeedForward.hpp:186-205); returns log psi
+double or_ff_jacobian(const or_ff_desc* d, const double* params,
+                      const int8_t* sigma, double* der);
+
 // ---- replay of the kernels' arithmetic -------------------------------------
 typedef struct {
   int n, m, use_bias;

@@ -199,8 +199,7 @@ def test_error_paths(ctx, orc, yq):
         ctx.set_params(bad)
     assert e.value.code == yq.ERR_NAN
     ctx.ham_set(0, n)
-    sig = np.ones((5, n), np.int8)
-    sig[:, ::2] = -1
+    sig = np.random.default_rng(3).choice([-1, 1], size=(40, n)).astype(np.int8)
     ctx.samples_set(sig)
     ctx.sr_build(0)
     with pytest.raises(yq.YqError) as e:   # ns < dim: S is singular without a shift

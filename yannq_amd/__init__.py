@@ -40,6 +40,11 @@ class HamDesc(C.Structure):
                 ("delta", C.c_double), ("J2", C.c_double), ("sign_rule", C.c_int)]
 
 
+class FFDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("channels", C.c_int), ("kernel", C.c_int),
+                ("hidden", C.c_int), ("dtype", C.c_int)]
+
+
 class StepCfg(C.Structure):
     _fields_ = [("n_sweeps", C.c_int), ("n_discard", C.c_int), ("randomize", C.c_int),
                 ("nup", C.c_int), ("solver", C.c_int), ("precision", C.c_int),
@@ -85,6 +90,11 @@ def lib():
         "yq_sr_build": [p, i, i], "yq_sr_get": [p, i, p], "yq_sr_apply": [p, d, p, p],
         "yq_sr_solve": [p, d, d, i, p, p, p, p], "yq_sr_step": [p, p, p],
         "yq_prof_enable": [p, C.c_char_p], "yq_prof_read": [p, C.c_char_p, p, p],
+        "yq_ff_create": [p, p], "yq_ff_npar": [p, p], "yq_ff_set_params": [p, p],
+        "yq_ff_get_params": [p, p], "yq_ff_update_params": [p, p],
+        "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
+        "yq_ff_jacobian": [p, i, p, p, p], "yq_ff_forward_dev": [p, i, p, p, p],
+        "yq_ff_jacobian_dev": [p, i, p, p, p], "yq_ff_vjp_dev": [p, i, p, p, p, p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -241,6 +251,52 @@ class Context:
                                       C.byref(res), C.byref(it)),
                   allow=(ERR_NOT_CONVERGED,) if allow_unconverged else ())
         return x, res.value, it.value
+
+    # FeedForward machine
+    def ff_create(self, n, channels, kernel, hidden, dtype=np.float32):
+        d = FFDesc(n, channels, kernel, hidden, 1 if np.dtype(dtype) == np.float64 else 0)
+        self._chk(self._L.yq_ff_create(self._h, C.byref(d)))
+        k = C.c_int()
+        self._chk(self._L.yq_ff_npar(self._h, C.byref(k)))
+        self.ff_n, self.ff_npar = n, k.value
+        self.ff_dtype = np.dtype(np.float64 if d.dtype else np.float32)
+
+    def ff_set_params(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.ff_npar,)
+        self._chk(self._L.yq_ff_set_params(self._h, _ptr(v)))
+
+    def ff_get_params(self):
+        out = np.empty(self.ff_npar)
+        self._chk(self._L.yq_ff_get_params(self._h, _ptr(out)))
+        return out
+
+    def ff_init_random(self, seed, mode=3):
+        self._chk(self._L.yq_ff_init_random(self._h, seed, mode))
+
+    def ff_forward(self, sigma):
+        sigma = np.ascontiguousarray(sigma, dtype=np.int8)
+        out = np.empty(sigma.shape[0])
+        self._chk(self._L.yq_ff_forward(self._h, sigma.shape[0], _ptr(sigma), _ptr(out)))
+        return out
+
+    def ff_jacobian(self, sigma):
+        sigma = np.ascontiguousarray(sigma, dtype=np.int8)
+        J = np.empty((sigma.shape[0], self.ff_npar))
+        out = np.empty(sigma.shape[0])
+        self._chk(self._L.yq_ff_jacobian(self._h, sigma.shape[0], _ptr(sigma), _ptr(J),
+                                         _ptr(out)))
+        return J, out
+
+    # device-pointer forms (integers from e.g. torch.Tensor.data_ptr())
+    def ff_forward_dev(self, ns, d_sigma, d_out, d_act=None):
+        self._chk(self._L.yq_ff_forward_dev(self._h, ns, d_sigma, d_out, d_act))
+
+    def ff_jacobian_dev(self, ns, d_sigma, d_jac, d_out=None):
+        self._chk(self._L.yq_ff_jacobian_dev(self._h, ns, d_sigma, d_jac, d_out))
+
+    def ff_vjp_dev(self, ns, d_sigma, d_act, d_r, d_grad):
+        self._chk(self._L.yq_ff_vjp_dev(self._h, ns, d_sigma, d_act, d_r, d_grad))
 
     def sr_step(self, **kw):
         cfg = StepCfg(kw.get("n_sweeps", 1), kw.get("n_discard", 0), int(kw.get("randomize", 0)),

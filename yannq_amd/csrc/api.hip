@@ -106,10 +106,18 @@ struct yq_ctx {
   Buf vx, vr, vp, vAp, vy, vtmp, vb, dotp;
   yq::MixedWork mixed;
 
+  // FeedForward machine
+  bool have_ff = false;
+  yq_ff_desc ff{};
+  int ff_npar = 0;
+  std::vector<double> ff_host;  // fp64 master parameters
+  Buf ffp, ff_sig, ff_out, ff_jac, ff_part, ff_grad;
+
   yq_ctx() {
     for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
-                   &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp})
+                   &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp,
+                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad})
       bufs.push_back(b);
   }
 };
@@ -933,9 +941,9 @@ YQ_API int yq_sr_apply(yq_ctx* c, double shift, const double* v, double* out) {
                              c->S.as<double>(), (rocblas_int)dim, c->vx.as<double>(), 1, &shift,
                              c->vAp.as<double>(), 1));
   } else if (c->sr_prec == YQ_MIXED) {
-    launch_mixed_apply64(c->mixed, shift, c->omean.as<double>(), c->vx.as<double>(),
-                         c->vAp.as<double>(), c->st);
-    LAUNCHCHK(c);
+    const int rc = mixed_apply64(c->mixed, shift, c->omean.as<double>(), c->vx.as<double>(),
+                                 c->vAp.as<double>(), c->st, c->err);
+    if (rc) return rc;
   } else {
     TRY(apply64(c, shift, c->vx.as<double>(), c->vAp.as<double>()));
   }
@@ -960,6 +968,206 @@ YQ_API int yq_sr_solve(yq_ctx* c, double shift, double tol, int max_iter, const 
   if (x) HIPCHK(c, hipMemcpyAsync(x, c->vx.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->st));
   TRY(sync(c));
   return rc;
+}
+
+// ---------------------------------------------------------------- FeedForward
+}  // extern "C"
+
+namespace {
+int need_ff(yq_ctx* c) {
+  if (!c) return YQ_ERR_ARG;
+  if (!c->have_ff) return fail(c, YQ_ERR_STATE, "no FeedForward machine: call yq_ff_create");
+  return YQ_OK;
+}
+size_t ff_rs(const yq_ctx* c) { return c->ff.dtype == 1 ? sizeof(double) : sizeof(float); }
+// upload the fp64 master parameters in the machine dtype
+int ff_upload(yq_ctx* c) {
+  const size_t np = (size_t)c->ff_npar;
+  if (c->ff.dtype == 1) {
+    HIPCHK(c, hipMemcpyAsync(c->ffp.p, c->ff_host.data(), sizeof(double) * np,
+                             hipMemcpyHostToDevice, c->st));
+  } else {
+    std::vector<float> f(np);
+    for (size_t q = 0; q < np; ++q) f[q] = (float)c->ff_host[q];
+    HIPCHK(c, hipMemcpyAsync(c->ffp.p, f.data(), sizeof(float) * np, hipMemcpyHostToDevice, c->st));
+    TRY(sync(c));
+  }
+  return sync(c);
+}
+FFArgs ff_args(yq_ctx* c, int ns, const int8_t* d_sigma) {
+  FFArgs a{};
+  a.n = c->ff.n;
+  a.C = c->ff.channels;
+  a.K = c->ff.kernel;
+  a.H = c->ff.hidden;
+  a.ns = ns;
+  a.params = c->ffp.p;
+  a.sigma = d_sigma;
+  return a;
+}
+int ff_check_sigma(yq_ctx* c, int ns, const int8_t* sigma) {
+  if (ns < 1 || !sigma) return fail(c, YQ_ERR_ARG, "need ns >= 1 and sigma");
+  for (size_t q = 0; q < (size_t)ns * c->ff.n; ++q)
+    if (sigma[q] != 1 && sigma[q] != -1) return fail(c, YQ_ERR_ARG, "spins must be +1 or -1");
+  return YQ_OK;
+}
+// host double arrays <-> device machine-dtype arrays
+int ff_to_host(yq_ctx* c, const void* dev, size_t count, double* out) {
+  if (c->ff.dtype == 1) {
+    HIPCHK(c, hipMemcpyAsync(out, dev, sizeof(double) * count, hipMemcpyDeviceToHost, c->st));
+    return sync(c);
+  }
+  std::vector<float> f(count);
+  HIPCHK(c, hipMemcpyAsync(f.data(), dev, sizeof(float) * count, hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  for (size_t q = 0; q < count; ++q) out[q] = f[q];
+  return YQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+YQ_API int yq_ff_create(yq_ctx* c, const yq_ff_desc* d) {
+  if (!c) return YQ_ERR_ARG;
+  if (!d) return fail(c, YQ_ERR_ARG, "null descriptor");
+  if (d->n < 2 || d->n > 1024 || d->channels < 1 || d->kernel < 1 || d->kernel > d->n ||
+      d->hidden < 1 || (d->dtype != 0 && d->dtype != 1))
+    return fail(c, YQ_ERR_ARG, "invalid FeedForward shape or dtype");
+  const long long np = (long long)d->kernel * d->channels + (long long)d->channels * d->n * d->hidden;
+  if (np > 96 * 256) return fail(c, YQ_ERR_UNSUPPORTED, "FeedForward limited to 24576 parameters");
+  c->ff = *d;
+  c->ff_npar = (int)np;
+  c->ff_host.assign((size_t)np, 0.0);
+  FFArgs probe = ff_args(c, 1, nullptr);
+  if (ff_smem_bytes(probe, d->dtype) > 64 * 1024)
+    return fail(c, YQ_ERR_UNSUPPORTED, "FeedForward activations exceed the LDS budget");
+  TRY(ensure(c, c->ffp, ff_rs(c) * (size_t)np));
+  c->have_ff = true;
+  return ff_upload(c);
+}
+
+YQ_API int yq_ff_npar(const yq_ctx* c, int* npar) {
+  if (!c || !c->have_ff) return YQ_ERR_STATE;
+  if (npar) *npar = c->ff_npar;
+  return YQ_OK;
+}
+
+YQ_API int yq_ff_set_params(yq_ctx* c, const double* p) {
+  TRY(need_ff(c));
+  if (!p) return fail(c, YQ_ERR_ARG, "null params");
+  for (int q = 0; q < c->ff_npar; ++q)
+    if (!std::isfinite(p[q])) return fail(c, YQ_ERR_NAN, "non-finite parameter");
+  c->ff_host.assign(p, p + c->ff_npar);
+  return ff_upload(c);
+}
+
+YQ_API int yq_ff_get_params(yq_ctx* c, double* p) {
+  TRY(need_ff(c));
+  if (!p) return fail(c, YQ_ERR_ARG, "null output");
+  std::copy(c->ff_host.begin(), c->ff_host.end(), p);
+  return YQ_OK;
+}
+
+YQ_API int yq_ff_update_params(yq_ctx* c, const double* delta) {
+  TRY(need_ff(c));
+  if (!delta) return fail(c, YQ_ERR_ARG, "null delta");
+  for (int q = 0; q < c->ff_npar; ++q) c->ff_host[q] += delta[q];
+  return ff_upload(c);
+}
+
+YQ_API int yq_ff_init_random(yq_ctx* c, uint64_t seed, int mode) {
+  TRY(need_ff(c));
+  if (mode < 0 || mode > 3) return fail(c, YQ_ERR_ARG, "mode must be 0..3");
+  const yq_ff_desc& d = c->ff;
+  // layers with parameters, in order: (paramDim, fanIn, fanOut)
+  const int CN = d.channels * d.n;
+  const int layers[2][3] = {{d.kernel * d.channels, d.kernel, d.channels * d.kernel},
+                            {CN * d.hidden, CN, d.hidden}};
+  size_t q = 0;
+  for (const auto& L : layers) {
+    const double fi = L[1], fo = L[2];
+    const double s = mode == 0 ? sqrt(1.0 / fi) : mode == 1 ? sqrt(2.0 / (fi + fo))
+                   : mode == 2 ? sqrt(2.0 / fi) : sqrt(6.0 / (fi + fo));
+    for (int k = 0; k < L[0]; ++k, ++q) {
+      const u32x4 x = draw(seed, q / 2, 1u, kDomParams);
+      if (mode == 3) {
+        const double u = (q & 1) ? u01_double(x.z, x.w) : u01_double(x.x, x.y);
+        c->ff_host[q] = s * (2.0 * u - 1.0);
+      } else {
+        const double u1 = 1.0 - u01_double(x.x, x.y), u2 = u01_double(x.z, x.w);
+        const double rad = sqrt(-2.0 * log(u1));
+        c->ff_host[q] = s * ((q & 1) ? rad * sin(2.0 * M_PI * u2) : rad * cos(2.0 * M_PI * u2));
+      }
+    }
+  }
+  return ff_upload(c);
+}
+
+YQ_API int yq_ff_forward_dev(yq_ctx* c, int ns, const int8_t* d_sigma, void* d_logpsi,
+                             void* d_act) {
+  TRY(need_ff(c));
+  if (ns < 1 || !d_sigma || !d_logpsi) return fail(c, YQ_ERR_ARG, "need ns >= 1 and buffers");
+  FFArgs a = ff_args(c, ns, d_sigma);
+  a.out = d_logpsi;
+  a.act_out = d_act;
+  TRY(launch_ff(a, kFFForward, c->ff.dtype, c->st));
+  LAUNCHCHK(c);
+  return sync(c);
+}
+
+YQ_API int yq_ff_jacobian_dev(yq_ctx* c, int ns, const int8_t* d_sigma, void* d_jac,
+                              void* d_logpsi) {
+  TRY(need_ff(c));
+  if (ns < 1 || !d_sigma || !d_jac) return fail(c, YQ_ERR_ARG, "need ns >= 1 and buffers");
+  TRY(ensure(c, c->ff_out, ff_rs(c) * (size_t)ns));
+  FFArgs a = ff_args(c, ns, d_sigma);
+  a.jac = d_jac;
+  a.out = d_logpsi ? d_logpsi : c->ff_out.p;
+  TRY(launch_ff(a, kFFJacobian, c->ff.dtype, c->st));
+  LAUNCHCHK(c);
+  return sync(c);
+}
+
+YQ_API int yq_ff_vjp_dev(yq_ctx* c, int ns, const int8_t* d_sigma, const void* d_act,
+                         const void* d_r, void* d_grad) {
+  TRY(need_ff(c));
+  if (ns < 1 || !d_sigma || !d_r || !d_grad) return fail(c, YQ_ERR_ARG, "need ns >= 1 and buffers");
+  TRY(ensure(c, c->ff_part, ff_rs(c) * (size_t)ff_vjp_blocks(ns, c->ff.dtype) * c->ff_npar));
+  FFArgs a = ff_args(c, ns, d_sigma);
+  a.act_in = d_act;
+  a.r = d_r;
+  a.partial = c->ff_part.p;
+  a.grad = d_grad;
+  const int rc = launch_ff(a, kFFVjp, c->ff.dtype, c->st);
+  if (rc) return fail(c, rc, "FeedForward VJP shape unsupported");
+  LAUNCHCHK(c);
+  return sync(c);
+}
+
+YQ_API int yq_ff_forward(yq_ctx* c, int ns, const int8_t* sigma, double* logpsi) {
+  TRY(need_ff(c));
+  TRY(ff_check_sigma(c, ns, sigma));
+  if (!logpsi) return fail(c, YQ_ERR_ARG, "null output");
+  TRY(ensure(c, c->ff_sig, (size_t)ns * c->ff.n));
+  TRY(ensure(c, c->ff_out, ff_rs(c) * (size_t)ns));
+  HIPCHK(c, hipMemcpyAsync(c->ff_sig.p, sigma, (size_t)ns * c->ff.n, hipMemcpyHostToDevice, c->st));
+  TRY(yq_ff_forward_dev(c, ns, c->ff_sig.as<int8_t>(), c->ff_out.p, nullptr));
+  return ff_to_host(c, c->ff_out.p, (size_t)ns, logpsi);
+}
+
+YQ_API int yq_ff_jacobian(yq_ctx* c, int ns, const int8_t* sigma, double* jac, double* logpsi) {
+  TRY(need_ff(c));
+  TRY(ff_check_sigma(c, ns, sigma));
+  if (!jac) return fail(c, YQ_ERR_ARG, "null output");
+  const size_t cnt = (size_t)ns * c->ff_npar;
+  TRY(ensure(c, c->ff_sig, (size_t)ns * c->ff.n));
+  TRY(ensure(c, c->ff_jac, ff_rs(c) * cnt));
+  TRY(ensure(c, c->ff_grad, ff_rs(c) * (size_t)ns));
+  HIPCHK(c, hipMemcpyAsync(c->ff_sig.p, sigma, (size_t)ns * c->ff.n, hipMemcpyHostToDevice, c->st));
+  TRY(yq_ff_jacobian_dev(c, ns, c->ff_sig.as<int8_t>(), c->ff_jac.p, c->ff_grad.p));
+  TRY(ff_to_host(c, c->ff_jac.p, cnt, jac));
+  if (logpsi) TRY(ff_to_host(c, c->ff_grad.p, (size_t)ns, logpsi));
+  return YQ_OK;
 }
 
 // ---------------------------------------------------------------- profiling

@@ -1,5 +1,6 @@
 // Mixed-precision factored SR operator: (O_c^T O_c / Ns + shift) p with the
-// products O p and O^T y on bf16 MFMA (v_mfma_f32_16x16x32_bf16).
+// products O p and O^T y on bf16 MFMA (v_mfma_f32_16x16x32_bf16), and the
+// float conjugate-gradient loop that drives it with device-resident scalars.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -10,9 +11,10 @@ namespace yq {
 
 struct MixedWork {
   int ns = 0, n = 0, m = 0, use_bias = 1;
-  int Kp = 0;    // n+1 rounded up to 32 (MFMA k-steps of the forward product)
+  int Kp = 0;    // n+1 rounded up to 32 (k-steps of the forward product)
   int Np = 0;    // m+1 rounded up to 16 (column tiles)
-  int nsp = 0;   // ns rounded up to 512 (backward chunks)
+  int nsp = 0;   // ns rounded up to 512
+  int ncg = 0;   // column groups of the forward product
   size_t dim = 0;
   // operands prepared once per SR build
   uint16_t* sigA = nullptr;  // bf16 sigma' row-major  nsp x Kp
@@ -21,13 +23,15 @@ struct MixedWork {
   float* Tt = nullptr;       // T'^T                   Np x nsp
   // per-product scratch
   uint16_t* vsplit = nullptr;  // 3 x Np x Kp bf16 (hi, mid, lo of V')
+  float* ypart = nullptr;      // ncg x nsp forward partial sums
   float* y = nullptr;          // nsp
-  float* part = nullptr;       // chunks x Kp x Np
+  float* part = nullptr;       // chunks x Kp x Np backward partials
   float* om32 = nullptr;       // column mean in float (dim)
   // CG vectors (float) and device scalars (double)
   float *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
-  double* dpart = nullptr;     // dot partials
-  double* scal = nullptr;      // per-iteration scalars
+  double* dpart = nullptr;     // per-block partials of the fused dot products
+  unsigned* ticket = nullptr;  // last-block counter of the fused dot products
+  double* scal = nullptr;      // 4 scalars per CG iteration + scratch
   int scal_cap = 0;
   size_t cap_rows = 0, cap_dim = 0;
 };
@@ -36,18 +40,15 @@ struct MixedWork {
 // T32: ns x m float = tanh(theta)).
 int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns,
                   int n, int m, int use_bias, hipStream_t st, std::string& err);
-// out = (O_c^T O_c / Ns + shift) p for float vectors (dim); omean64 is the
-// FP64 column mean.  Used by the tests through the inner CG and directly.
-int mixed_apply(MixedWork& w, double shift, const float* p, float* out,
-                hipStream_t st, std::string& err);
+// out = (S + shift) v for FP64 vectors on the device, through the float /
+// split-bf16 operator (omean64: FP64 column mean of O).
+int mixed_apply64(MixedWork& w, double shift, const double* omean64,
+                  const double* v, double* out, hipStream_t st, std::string& err);
 // Inner CG in float with device scalars: solves (S + shift) x = b to the
-// relative residual tol (checked every few iterations); b and x are FP64.
+// relative residual tol (checked every 8 iterations); b and x are FP64.
 int mixed_cg(MixedWork& w, double shift, const double* omean64,
              const double* b, double* x, double tol, int max_iter, int* iters,
              hipStream_t st, std::string& err);
 void mixed_release(MixedWork& w);
-// float <-> bf16 helpers exposed for tests of the split
-void launch_mixed_apply64(MixedWork& w, double shift, const double* omean64,
-                          const double* v, double* out, hipStream_t st);
 
 }  // namespace yq

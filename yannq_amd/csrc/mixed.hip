@@ -2,15 +2,20 @@
 //
 // O's rows are sigma'_s (x) T'_s (sigma' = [sigma | 1], T' = [tanh theta | 1],
 // see sr.hip).  Spins are exactly representable in bf16, so every product
-// keeps full float accuracy by splitting only the float operand into three
-// bf16 pieces (x = hi + mid + lo to 24 bits) and issuing one MFMA per piece
-// with float accumulation:
-//   forward   U' = sigma' V'        (ns x Np, K = n+1 -> Kp)   then
-//             y_s = sum_j T'_sj U'_sj           (fused epilogue)
-//   backward  G' = sigma'^T diag(y) T'  (Kp x Np, K = ns, split in chunks)
+// keeps float accuracy by splitting only the float operand into three bf16
+// pieces (x = hi + mid + lo to 24 bits) and issuing one MFMA per piece with
+// float accumulation:
+//   forward   U' = sigma' V'        (nsp x Np, K = Kp)  with the fused
+//             epilogue y_s = sum_j T'_sj U'_sj, split over column groups
+//   backward  G' = sigma'^T diag(y) T'  (Kp x Np, K = nsp in chunks)
 // v_mfma_f32_16x16x32_bf16 operand layout used here: lane l holds
-// A[row l%16][k 8*(l/16) .. +7], B[k 8*(l/16) .. +7][col l%16] and returns
-// D[4*(l/16) + q][l%16], q = 0..3.
+// A[row l%16][k 8*(l/16) .. +7] and B[k 8*(l/16) .. +7][col l%16] and
+// returns D[4*(l/16) + q][l%16], q = 0..3.
+//
+// The CG loop needs six launches per iteration and no host round trip: the
+// vector updates carry their dot products, finished on the device by the
+// last block to arrive (per-block partials summed in block order, so the
+// result does not depend on scheduling).
 #include <math.h>
 
 #include "mixed.h"
@@ -23,9 +28,11 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kBwdChunk = 512;  // samples per backward chunk
-constexpr int kMaxKs = 5;       // Kp <= 160 (n <= 128)
+constexpr int kBwdChunk = 256;  // samples per backward chunk
+constexpr int kRowPad = 512;    // ns padding
+constexpr int kMaxKs = 5;       // Kp <= 160 (n <= 159)
 constexpr int kMaxIt = 10;      // Kp / 16
+constexpr int kColTiles = 4;    // column tiles per forward group
 
 __device__ __host__ inline uint16_t f2bf(float x) {
   uint32_t u;
@@ -51,10 +58,53 @@ __device__ inline bf16x8 ld8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, v);
 }
 
+// Sum v over the block (fixed tree), store the block partial, and let the
+// last block to finish sum all partials in block order into *dst.
+__device__ void block_dot_finish(double v, double* part, unsigned* ticket, double* dst) {
+  __shared__ double red[256];
+  __shared__ bool last;
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[blockIdx.x] = red[0];
+    __threadfence();
+    const unsigned prev = atomicAdd(ticket, 1u);
+    last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double s = 0.0;
+  for (unsigned i = t; i < gridDim.x; i += 256)
+    s += __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    *dst = red[0];
+    *ticket = 0u;
+  }
+}
+
+// index of parameter c on the (Kp x Np) grid of V' / G'
+__device__ inline size_t aug_index(size_t c, int n, int m, int Np) {
+  const size_t nm = (size_t)n * m;
+  if (c < nm) return (c / m) * Np + (c % m);
+  if (c < nm + n) return (c - nm) * Np + m;
+  return (size_t)n * Np + (c - nm - n);
+}
+
 // sigma' (bf16) row-major and transposed; zero outside ns x (n+1)
-__global__ void k_prep_sig(const int8_t* __restrict__ sigma, int ns, int n,
-                           int Kp, int nsp, uint16_t* __restrict__ A,
-                           uint16_t* __restrict__ AT) {
+__global__ void k_prep_sig(const int8_t* __restrict__ sigma, int ns, int n, int Kp, int nsp,
+                           uint16_t* __restrict__ A, uint16_t* __restrict__ AT) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (size_t)nsp * Kp) return;
   const int s = (int)(q / Kp), i = (int)(q % Kp);
@@ -69,8 +119,8 @@ __global__ void k_prep_sig(const int8_t* __restrict__ sigma, int ns, int n,
 }
 
 // T' row-major (nsp x Np) and transposed (Np x nsp)
-__global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np,
-                         int nsp, float* __restrict__ Trm, float* __restrict__ Tt) {
+__global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int nsp,
+                         float* __restrict__ Trm, float* __restrict__ Tt) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (size_t)nsp * Np) return;
   const int s = (int)(q / Np), j = (int)(q % Np);
@@ -83,33 +133,51 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np,
   Tt[(size_t)j * nsp + s] = v;
 }
 
-// V' (Kp x Np grid of the [W | a | b] vector p) split into three bf16 layers,
-// stored transposed: vs[(layer * Np + j) * Kp + k]
-__global__ void k_split_v(const float* __restrict__ p, int n, int m, int use_bias,
-                          int Kp, int Np, uint16_t* __restrict__ vs) {
+// p <- r + beta p on the parameter vector (beta = rr_now / rr_prev, or 0 when
+// rr_prev is null), V' = p on the (Kp x Np) grid split into three bf16 layers
+// stored transposed, vs[(layer * Np + j) * Kp + k]; *dst = <omean, p>.
+__global__ void __launch_bounds__(256)
+    k_split_v(const float* __restrict__ r, float* __restrict__ p, const double* rr_now,
+              const double* rr_prev, const float* __restrict__ om, int n, int m,
+              int use_bias, int Kp, int Np, uint16_t* __restrict__ vs, double* part,
+              unsigned* ticket, double* dst) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= Np * Kp) return;
-  const int j = q / Kp, k = q % Kp;
   const size_t nm = (size_t)n * m;
   float v = 0.0f;
-  if (k < n && j < m) v = p[(size_t)k * m + j];
-  else if (use_bias && k < n && j == m) v = p[nm + k];
-  else if (use_bias && k == n && j < m) v = p[nm + n + j];
-  uint16_t hi, mid, lo;
-  split3(v, hi, mid, lo);
-  vs[q] = hi;
-  vs[(size_t)Np * Kp + q] = mid;
-  vs[(size_t)2 * Np * Kp + q] = lo;
+  double dv = 0.0;
+  if (q < Np * Kp) {
+    const int j = q / Kp, k = q % Kp;
+    long c = -1;
+    if (k < n && j < m) c = (long)k * m + j;
+    else if (use_bias && k < n && j == m) c = (long)(nm + k);
+    else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
+    if (c >= 0) {
+      const float beta = rr_prev ? (float)(*rr_now / *rr_prev) : 0.0f;
+      v = rr_prev ? r[c] + beta * p[c] : r[c];
+      p[c] = v;
+      dv = (double)om[c] * (double)v;
+    }
+    uint16_t hi, mid, lo;
+    split3(v, hi, mid, lo);
+    vs[q] = hi;
+    vs[(size_t)Np * Kp + q] = mid;
+    vs[(size_t)2 * Np * Kp + q] = lo;
+  }
+  block_dot_finish(dv, part, ticket, dst);
 }
 
-// y_s = sum_j T'_sj sum_k sigma'_sk V'_kj ; one wave per 16 samples
+// ypart[cg][s] = sum over the column tiles of group cg of T'_sj U'_sj;
+// one wave per 16 samples, blockIdx.y = column group
 __global__ void __launch_bounds__(256)
     k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
                const float* __restrict__ Trm, int Kp, int Np, int nsp,
-               float* __restrict__ y) {
+               float* __restrict__ ypart) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int row0 = (blockIdx.x * 4 + wave) * 16;
   if (row0 >= nsp) return;
+  const int cg = blockIdx.y;
+  const int ct0 = cg * kColTiles;
+  const int ct1 = min(ct0 + kColTiles, Np / 16);
   const int nks = Kp / 32;
   const int li = l & 15, lg = l >> 4;
   bf16x8 afr[kMaxKs];
@@ -118,9 +186,12 @@ __global__ void __launch_bounds__(256)
     if (ks < nks) afr[ks] = ld8(sigA + (size_t)(row0 + li) * Kp + ks * 32 + lg * 8);
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
   const size_t layer = (size_t)Np * Kp;
-  for (int ct = 0; ct < Np / 16; ++ct) {
+  for (int ct = ct0; ct < ct1; ++ct) {
     floatx4 c = {0.f, 0.f, 0.f, 0.f};
     const uint16_t* vb = vs + (size_t)(ct * 16 + li) * Kp + lg * 8;
+    float tv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tv[q] = Trm[(size_t)(row0 + 4 * lg + q) * Np + ct * 16 + li];
 #pragma unroll
     for (int sp = 0; sp < 3; ++sp)
 #pragma unroll
@@ -129,8 +200,7 @@ __global__ void __launch_bounds__(256)
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[ks], ld8(vb + sp * layer + ks * 32),
                                                       c, 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      acc[q] += Trm[(size_t)(row0 + 4 * lg + q) * Np + ct * 16 + li] * c[q];
+    for (int q = 0; q < 4; ++q) acc[q] += tv[q] * c[q];
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -138,19 +208,20 @@ __global__ void __launch_bounds__(256)
     for (int s = 8; s >= 1; s >>= 1) acc[q] += __shfl_xor(acc[q], s);
   if (li == 0)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) y[row0 + 4 * lg + q] = acc[q];
+    for (int q = 0; q < 4; ++q) ypart[(size_t)cg * nsp + row0 + 4 * lg + q] = acc[q];
 }
 
-// y_s <- (y_s - <omean, p>) / ns for s < ns, 0 beyond
-__global__ void k_center_y(float* __restrict__ y, int ns, int nsp,
-                           const double* __restrict__ dotp) {
+// y_s = (sum_cg ypart - <omean, p>) / ns for s < ns, 0 beyond
+__global__ void k_center_y(const float* __restrict__ ypart, int ncg, int ns, int nsp,
+                           const double* __restrict__ dotp, float* __restrict__ y) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsp) return;
-  const float d = (float)dotp[0];
-  y[s] = s < ns ? (y[s] - d) / (float)ns : 0.0f;
+  float v = 0.0f;
+  for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
+  y[s] = s < ns ? (v - (float)dotp[0]) / (float)ns : 0.0f;
 }
 
-// partial[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj ; one wave per
+// part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj ; one wave per
 // (column tile, chunk); all Kp/16 row tiles accumulate in registers
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
@@ -166,6 +237,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int it = 0; it < kMaxIt; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
   const float* trow = Tt + (size_t)(ct * 16 + li) * nsp;
+#pragma unroll 2
   for (int ks = 0; ks < kBwdChunk / 32; ++ks) {
     const int s0 = kc * kBwdChunk + ks * 32 + lg * 8;
     u16x8 bh, bm, bl;
@@ -199,76 +271,56 @@ __global__ void __launch_bounds__(256)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
 
-// out[c] = G'[aug(c)] - omean_c G'[n][m] + shift p_c  (G' summed over chunks)
-__global__ void k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m,
-                             int use_bias, int Kp, int Np, const float* __restrict__ om,
-                             double shift, const float* __restrict__ p,
-                             float* __restrict__ out) {
+// out[c] = G'[aug(c)] - omean_c G'[n][m] + shift p_c (G' summed over chunks in
+// order); *dst = <p, out>
+__global__ void __launch_bounds__(256)
+    k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m, int use_bias,
+                 int Kp, int Np, const float* __restrict__ om, double shift,
+                 const float* __restrict__ p, float* __restrict__ out, double* dpart,
+                 unsigned* ticket, double* dst) {
   const size_t nm = (size_t)n * m;
   const size_t dim = use_bias ? nm + n + m : nm;
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= dim) return;
-  size_t a;
-  if (c < nm) a = (c / m) * Np + (c % m);
-  else if (c < nm + n) a = (c - nm) * Np + m;
-  else a = (size_t)n * Np + (c - nm - n);
-  const size_t corner = (size_t)n * Np + m;
-  const size_t stride = (size_t)Kp * Np;
-  float g = 0.0f, sy = 0.0f;
-  for (int k = 0; k < chunks; ++k) {
-    g += part[(size_t)k * stride + a];
-    sy += part[(size_t)k * stride + corner];
+  double dv = 0.0;
+  if (c < dim) {
+    const size_t a = aug_index(c, n, m, Np);
+    const size_t corner = (size_t)n * Np + m;
+    const size_t stride = (size_t)Kp * Np;
+    float g = 0.0f, sy = 0.0f;
+    for (int k = 0; k < chunks; ++k) {
+      g += part[(size_t)k * stride + a];
+      sy += part[(size_t)k * stride + corner];
+    }
+    const float o = (g - om[c] * sy) + (float)shift * p[c];
+    out[c] = o;
+    dv = (double)p[c] * (double)o;
   }
-  out[c] = (g - om[c] * sy) + (float)shift * p[c];
+  block_dot_finish(dv, dpart, ticket, dst);
 }
 
-// ---- vectors and device-side CG scalars ------------------------------------
-constexpr int kDotBlocks = 256;
+// x += alpha p, r -= alpha Ap with alpha = rr / pAp; *dst = <r, r>
+__global__ void __launch_bounds__(256)
+    k_cg_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+            const float* __restrict__ Ap, size_t n, const double* __restrict__ sc,
+            double* dpart, unsigned* ticket, double* dst) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double dv = 0.0;
+  if (i < n) {
+    const double pap = sc[1];
+    const float alpha = pap != 0.0 ? (float)(sc[0] / pap) : 0.0f;
+    x[i] += alpha * p[i];
+    const float rv = r[i] - alpha * Ap[i];
+    r[i] = rv;
+    dv = (double)rv * (double)rv;
+  }
+  block_dot_finish(dv, dpart, ticket, dst);
+}
 
 __global__ void __launch_bounds__(256)
-    k_dot32(const float* __restrict__ x, const float* __restrict__ y, size_t n,
-            double* __restrict__ partial) {
-  __shared__ double red[256];
-  double acc = 0.0;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kDotBlocks * 256)
-    acc += (double)x[i] * (double)y[i];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-__global__ void __launch_bounds__(256)
-    k_dot_finish(const double* __restrict__ partial, double* __restrict__ dst) {
-  __shared__ double red[256];
-  red[threadIdx.x] = partial[threadIdx.x];
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) dst[0] = red[0];
-}
-// sc layout per iteration t: sc[4t+0] = r.r (before), sc[4t+1] = p.Ap,
-// sc[4t+2] = r.r (after)
-__global__ void k_cg_xr(float* __restrict__ x, float* __restrict__ r,
-                        const float* __restrict__ p, const float* __restrict__ Ap,
-                        size_t n, const double* __restrict__ sc) {
+    k_dot32(const float* __restrict__ x, const float* __restrict__ y, size_t n, double* dpart,
+            unsigned* ticket, double* dst) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double pap = sc[1];
-  const float alpha = pap != 0.0 ? (float)(sc[0] / pap) : 0.0f;
-  x[i] += alpha * p[i];
-  r[i] -= alpha * Ap[i];
-}
-__global__ void k_cg_p(float* __restrict__ p, const float* __restrict__ r, size_t n,
-                       const double* __restrict__ sc) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float beta = sc[0] != 0.0 ? (float)(sc[2] / sc[0]) : 0.0f;
-  p[i] = r[i] + beta * p[i];
+  block_dot_finish(i < n ? (double)x[i] * (double)y[i] : 0.0, dpart, ticket, dst);
 }
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,9 +330,9 @@ __global__ void k_f2d(const float* __restrict__ a, double* __restrict__ b, size_
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (double)a[i];
 }
-__global__ void k_copy_f(const float* __restrict__ a, float* __restrict__ b, size_t n) {
+__global__ void k_zero_f(float* __restrict__ a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[i];
+  if (i < n) a[i] = 0.0f;
 }
 
 inline unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -303,17 +355,46 @@ int alloc(T*& p, size_t count, std::string& err) {
   return YQ_OK;
 }
 
-// dst[0] = x . y, stays on the device
-void dot32(MixedWork& w, const float* x, const float* y, size_t n, double* dst,
-           hipStream_t st) {
-  hipLaunchKernelGGL(k_dot32, dim3(kDotBlocks), dim3(256), 0, st, x, y, n, w.dpart);
-  hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(256), 0, st, w.dpart, dst);
+// (S + shift) p -> Ap with the products on MFMA; <omean, p> goes to dot_op and
+// <p, Ap> to dot_pap.  p is first updated to r + beta p (see k_split_v).
+void apply_launches(MixedWork& w, double shift, const float* r, const double* rr_now,
+                    const double* rr_prev, double* dot_op, double* dot_pap, hipStream_t st) {
+  const size_t grid_v = (size_t)w.Np * w.Kp;
+  hipLaunchKernelGGL(k_split_v, dim3(nb(grid_v)), dim3(256), 0, st, r, w.p, rr_now, rr_prev,
+                     w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np, w.vsplit, w.dpart, w.ticket,
+                     dot_op);
+  {
+    ProfGuard g("fwd_mfma", st);
+    hipLaunchKernelGGL(k_fwd_mfma, dim3(w.nsp / 64, w.ncg), dim3(256), 0, st, w.sigA, w.vsplit,
+                       w.Trm, w.Kp, w.Np, w.nsp, w.ypart);
+  }
+  hipLaunchKernelGGL(k_center_y, dim3(nb(w.nsp)), dim3(256), 0, st, w.ypart, w.ncg, w.ns, w.nsp,
+                     dot_op, w.y);
+  const int chunks = w.nsp / kBwdChunk;
+  {
+    ProfGuard g("bwd_mfma", st);
+    hipLaunchKernelGGL(k_bwd_mfma, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), 0, st, w.sigT,
+                       w.Tt, w.y, w.Kp, w.Np, w.nsp, w.part);
+  }
+  ProfGuard g("bwd_reduce", st);
+  hipLaunchKernelGGL(k_bwd_reduce, dim3(nb(w.dim)), dim3(256), 0, st, w.part, chunks, w.n, w.m,
+                     w.use_bias, w.Kp, w.Np, w.om32, shift, w.p, w.Ap, w.dpart, w.ticket,
+                     dot_pap);
+}
+
+int ensure_scal(MixedWork& w, int iters, std::string& err) {
+  const int need = iters + 4 > 16 ? iters + 4 : 16;
+  if (w.scal_cap >= need) return YQ_OK;
+  int rc = alloc(w.scal, (size_t)4 * need, err);
+  if (rc) return rc;
+  w.scal_cap = need;
+  return YQ_OK;
 }
 
 }  // namespace
 
-int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns,
-                  int n, int m, int use_bias, hipStream_t st, std::string& err) {
+int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, int n, int m,
+                  int use_bias, hipStream_t st, std::string& err) {
   if (n + 1 > 32 * kMaxKs) {
     err = "mixed solver supports n <= 159";
     return YQ_ERR_UNSUPPORTED;
@@ -324,89 +405,64 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns,
   w.use_bias = use_bias;
   w.Kp = rup(n + 1, 32);
   w.Np = rup(m + 1, 16);
-  w.nsp = rup(ns, kBwdChunk);
+  w.nsp = rup(ns, kRowPad);
+  w.ncg = (w.Np / 16 + kColTiles - 1) / kColTiles;
   w.dim = (size_t)n * m + (use_bias ? (size_t)n + m : 0);
   const size_t rows = (size_t)w.nsp;
-  if (rows * w.Np > w.cap_rows || w.dim > w.cap_dim) {
+  if (rows * w.Np > w.cap_rows || w.dim > w.cap_dim || !w.ticket) {
     const size_t chunks = rows / kBwdChunk;
+    const size_t dparts = nb(w.dim > (size_t)w.Np * w.Kp ? w.dim : (size_t)w.Np * w.Kp);
     int rc;
     if ((rc = alloc(w.sigA, rows * w.Kp, err)) || (rc = alloc(w.sigT, rows * w.Kp, err)) ||
         (rc = alloc(w.Trm, rows * w.Np, err)) || (rc = alloc(w.Tt, rows * w.Np, err)) ||
-        (rc = alloc(w.y, rows, err)) ||
+        (rc = alloc(w.ypart, rows * w.ncg, err)) || (rc = alloc(w.y, rows, err)) ||
         (rc = alloc(w.part, chunks * w.Kp * w.Np, err)) ||
         (rc = alloc(w.vsplit, (size_t)3 * w.Np * w.Kp, err)) ||
         (rc = alloc(w.om32, w.dim, err)) || (rc = alloc(w.x, w.dim, err)) ||
         (rc = alloc(w.r, w.dim, err)) || (rc = alloc(w.p, w.dim, err)) ||
-        (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.dpart, kDotBlocks, err)))
+        (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.dpart, dparts, err)) ||
+        (rc = alloc(w.ticket, 1, err)))
       return rc;
+    hipError_t e = hipMemsetAsync(w.ticket, 0, sizeof(unsigned), st);
+    if (e != hipSuccess) return hipfail(err, "mixed_prepare", e);
     w.cap_rows = rows * w.Np;
     w.cap_dim = w.dim;
   }
   hipLaunchKernelGGL(k_prep_sig, dim3(nb(rows * w.Kp)), dim3(256), 0, st, sigma, ns, n, w.Kp,
                      w.nsp, w.sigA, w.sigT);
-  hipLaunchKernelGGL(k_prep_T, dim3(nb(rows * w.Np)), dim3(256), 0, st, T32, ns, m, w.Np,
-                     w.nsp, w.Trm, w.Tt);
+  hipLaunchKernelGGL(k_prep_T, dim3(nb(rows * w.Np)), dim3(256), 0, st, T32, ns, m, w.Np, w.nsp,
+                     w.Trm, w.Tt);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_prepare", e);
 }
 
-int mixed_apply(MixedWork& w, double shift, const float* p, float* out, hipStream_t st,
-                std::string& err) {
-  if (w.scal_cap < 8) {
-    int rc = alloc(w.scal, 64, err);
-    if (rc) return rc;
-    w.scal_cap = 16;
-  }
-  hipLaunchKernelGGL(k_split_v, dim3(nb((size_t)w.Np * w.Kp)), dim3(256), 0, st, p, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.vsplit);
-  {
-    ProfGuard g("fwd_mfma", st);
-    hipLaunchKernelGGL(k_fwd_mfma, dim3((w.nsp / 16 + 3) / 4), dim3(256), 0, st, w.sigA,
-                       w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.y);
-  }
-  double* d = w.scal + w.scal_cap * 4 - 1;  // scratch slot for <omean, p>
-  dot32(w, w.om32, p, w.dim, d, st);
-  hipLaunchKernelGGL(k_center_y, dim3(nb(w.nsp)), dim3(256), 0, st, w.y, w.ns, w.nsp, d);
-  const int chunks = w.nsp / kBwdChunk;
-  {
-    ProfGuard g("bwd_mfma", st);
-    hipLaunchKernelGGL(k_bwd_mfma, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), 0, st,
-                       w.sigT, w.Tt, w.y, w.Kp, w.Np, w.nsp, w.part);
-  }
-  ProfGuard g("bwd_reduce", st);
-  hipLaunchKernelGGL(k_bwd_reduce, dim3(nb(w.dim)), dim3(256), 0, st, w.part, chunks, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.om32, shift, p, out);
+int mixed_apply64(MixedWork& w, double shift, const double* omean64, const double* v,
+                  double* out, hipStream_t st, std::string& err) {
+  int rc = ensure_scal(w, 0, err);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, omean64, w.om32, w.dim);
+  hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, v, w.r, w.dim);
+  double* s = w.scal + 4 * (w.scal_cap - 1);  // scratch slot
+  apply_launches(w, shift, w.r, nullptr, nullptr, s + 0, s + 1, st);
+  hipLaunchKernelGGL(k_f2d, dim3(nb(w.dim)), dim3(256), 0, st, w.Ap, out, w.dim);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_apply", e);
-}
-
-void launch_mixed_apply64(MixedWork& w, double shift, const double* omean64, const double* v,
-                          double* out, hipStream_t st) {
-  std::string err;
-  hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, omean64, w.om32, w.dim);
-  hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, v, w.p, w.dim);
-  (void)mixed_apply(w, shift, w.p, w.Ap, st, err);
-  hipLaunchKernelGGL(k_f2d, dim3(nb(w.dim)), dim3(256), 0, st, w.Ap, out, w.dim);
 }
 
 int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b, double* x,
              double tol, int max_iter, int* iters, hipStream_t st, std::string& err) {
   const size_t dim = w.dim;
-  const int need = max_iter + 2 > 16 ? max_iter + 2 : 16;
-  if (w.scal_cap < need) {
-    int rc = alloc(w.scal, (size_t)4 * need, err);
-    if (rc) return rc;
-    w.scal_cap = need;
-  }
+  int rc = ensure_scal(w, max_iter, err);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, omean64, w.om32, dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, b, w.r, dim);
-  hipLaunchKernelGGL(k_copy_f, dim3(nb(dim)), dim3(256), 0, st, w.r, w.p, dim);
-  hipError_t e = hipMemsetAsync(w.x, 0, sizeof(float) * dim, st);
-  if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
-  double* sc = w.scal;
-  dot32(w, w.r, w.r, dim, sc + 0, st);
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.x, dim);
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.p, dim);
+  double* sc = w.scal;  // slot t: [0] r.r at the start of iteration t, [1] p.Ap, [2] <omean, p>
+  hipLaunchKernelGGL(k_dot32, dim3(nb(dim)), dim3(256), 0, st, w.r, w.r, dim, w.dpart, w.ticket,
+                     sc);
   double bn2 = 0.0;
-  e = hipMemcpyAsync(&bn2, sc, sizeof(double), hipMemcpyDeviceToHost, st);
+  hipError_t e = hipMemcpyAsync(&bn2, sc, sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   *iters = 0;
@@ -420,22 +476,16 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b,
   bool done = false;
   while (!done && it < max_iter) {
     double* s = sc + 4 * it;
-    int rc = mixed_apply(w, shift, w.p, w.Ap, st, err);
-    if (rc) return rc;
-    dot32(w, w.p, w.Ap, dim, s + 1, st);
-    hipLaunchKernelGGL(k_cg_xr, dim3(nb(dim)), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dim, s);
-    dot32(w, w.r, w.r, dim, s + 2, st);
-    hipLaunchKernelGGL(k_cg_p, dim3(nb(dim)), dim3(256), 0, st, w.p, w.r, dim, s);
-    // next iteration's r.r before = this iteration's r.r after
-    e = hipMemcpyAsync(sc + 4 * (it + 1), s + 2, sizeof(double), hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
+    apply_launches(w, shift, w.r, s, it > 0 ? s - 4 : nullptr, s + 2, s + 1, st);
+    hipLaunchKernelGGL(k_cg_xr, dim3(nb(dim)), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dim, s,
+                       w.dpart, w.ticket, s + 4);
     ++it;
     if (it % check == 0 || it == max_iter) {
       double rn2 = 0.0;
-      e = hipMemcpyAsync(&rn2, s + 2, sizeof(double), hipMemcpyDeviceToHost, st);
+      e = hipMemcpyAsync(&rn2, sc + 4 * it, sizeof(double), hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
-      if (!(rn2 >= thr)) done = true;  // converged (or NaN: stop and let FP64 check)
+      if (!(rn2 >= thr)) done = true;  // converged (or NaN: the FP64 check decides)
     }
   }
   hipLaunchKernelGGL(k_f2d, dim3(nb(dim)), dim3(256), 0, st, w.x, x, dim);
@@ -447,8 +497,9 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b,
 
 void mixed_release(MixedWork& w) {
   for (void* p : {(void*)w.sigA, (void*)w.sigT, (void*)w.Trm, (void*)w.Tt, (void*)w.vsplit,
-                  (void*)w.y, (void*)w.part, (void*)w.om32, (void*)w.x, (void*)w.r, (void*)w.p,
-                  (void*)w.Ap, (void*)w.dpart, (void*)w.scal})
+                  (void*)w.ypart, (void*)w.y, (void*)w.part, (void*)w.om32, (void*)w.x,
+                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.dpart, (void*)w.ticket,
+                  (void*)w.scal})
     if (p) (void)hipFree(p);
   w = MixedWork();
 }

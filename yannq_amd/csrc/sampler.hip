@@ -158,15 +158,17 @@ template <typename R, int KIND>
 void launch_r(const SampleArgs& a, hipStream_t st) {
   const dim3 grid((a.C + 3) / 4), block(256);
   const size_t lds = (size_t)4 * a.npad;
-  const int rmax = (a.m + kWave - 1) / kWave;
-  if (rmax <= 1)
-    hipLaunchKernelGGL((k_sample<R, KIND, 1>), grid, block, lds, st, a);
-  else if (rmax <= 2)
-    hipLaunchKernelGGL((k_sample<R, KIND, 2>), grid, block, lds, st, a);
-  else if (rmax <= 4)
-    hipLaunchKernelGGL((k_sample<R, KIND, 4>), grid, block, lds, st, a);
-  else
-    hipLaunchKernelGGL((k_sample<R, KIND, 8>), grid, block, lds, st, a);
+  // exactly ceil(m/64) register slots, so no slot is computed for nothing
+  switch ((a.m + kWave - 1) / kWave) {
+    case 1: hipLaunchKernelGGL((k_sample<R, KIND, 1>), grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_sample<R, KIND, 2>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_sample<R, KIND, 3>), grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_sample<R, KIND, 4>), grid, block, lds, st, a); break;
+    case 5: hipLaunchKernelGGL((k_sample<R, KIND, 5>), grid, block, lds, st, a); break;
+    case 6: hipLaunchKernelGGL((k_sample<R, KIND, 6>), grid, block, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((k_sample<R, KIND, 7>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_sample<R, KIND, 8>), grid, block, lds, st, a); break;
+  }
 }
 
 // theta for a batch, one thread per (sample, hidden unit)

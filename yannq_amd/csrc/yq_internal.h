@@ -105,6 +105,24 @@ void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
                             int n, int m, int use_bias, double* omean,
                             double* partial, hipStream_t st);
 
+// ------------------------------------------------------------------ FeedForward
+struct FFArgs {
+  int n, C, K, H, ns;
+  const void* params;   // device, machine precision
+  const int8_t* sigma;  // ns x n
+  void* out;            // log psi (ns)
+  void* act_out;        // optional stored activations (ns x (2*C*n + H))
+  const void* act_in;   // optional activations for the VJP (null: recompute)
+  void* jac;            // ns x npar
+  const void* r;        // VJP weights (ns)
+  void* partial;        // VJP partials (ff_vjp_blocks x npar)
+  void* grad;           // VJP result (npar)
+};
+enum { kFFForward = 0, kFFJacobian = 1, kFFVjp = 2 };
+int ff_vjp_blocks(int ns, int use_double);
+size_t ff_smem_bytes(const FFArgs& a, int use_double);
+int launch_ff(const FFArgs& a, int mode, int use_double, hipStream_t st);
+
 // ------------------------------------------------------------------ vectors
 // deterministic dot product: partial sums per block; host finishes the sum
 int dot_blocks(size_t n);
