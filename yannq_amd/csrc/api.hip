@@ -259,9 +259,7 @@ int compute_eloc(yq_ctx* c) {
 }
 
 // ---------------------------------------------------------------- SR build
-size_t aug_partial_doubles(int ns, int n, int m) {
-  return (size_t)colsum_chunks(ns) * (n + 1) * (m + 1) + ns;
-}
+size_t aug_partial_doubles(int ns, int n, int m) { return factored_partial_doubles(ns, n, m); }
 
 int sr_build(yq_ctx* c, int solver, int precision) {
   if (!c->have_eloc) TRY(compute_eloc(c));
@@ -351,7 +349,7 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
   double* y = c->vy.as<double>();
   if (c->sr_solver == YQ_SOLVE_CG_FACTORED) {
     launch_factored_fwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
-                          c->use_bias, v, y, c->st);
+                          c->use_bias, v, y, c->partial.as<double>(), c->st);
     LAUNCHCHK(c);
     double ov;
     TRY(dot(c, c->omean.as<double>(), v, dim, &ov));

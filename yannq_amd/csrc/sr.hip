@@ -102,121 +102,128 @@ __global__ void k_add_diag(double* M, int dim, double shift) {
   if (i < dim) M[(size_t)i * dim + i] += shift;
 }
 
-// ---- factored products --------------------------------------------------
-constexpr int kFwdS = 8;  // samples per block in the forward product
+// ---- factored products on FP64 MFMA ---------------------------------------
+// v_mfma_f64_16x16x4f64: lane l holds A[row l%16][k l/16] and B[k l/16][col
+// l%16] and returns D[4*(l/16) + q][l%16], q = 0..3.  The augmented operands
+// are produced on the fly from the int8 spins, T and the parameter vector.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kF64Tiles = 4;                     // column tiles per forward group
+constexpr int kMaxIt64 = (kMaxSites + 1 + 15) / 16;
 
-// y_s = sum_{i,j} sigma'_si V'_ij T'_sj (uncentred), V' read from v in the
-// [W | a | b] layout.
-__global__ void __launch_bounds__(256)
-    k_factored_fwd64(const int8_t* __restrict__ sigma,
-                     const double* __restrict__ T, int ns, int n, int m,
-                     int use_bias, const double* __restrict__ v,
-                     double* __restrict__ y) {
-  __shared__ int8_t sg[kFwdS][kMaxSites];
-  __shared__ double red[kFwdS][256];
-  const int s0 = blockIdx.x * kFwdS;
-  const int cnt = min(kFwdS, ns - s0);
-  for (int q = threadIdx.x; q < cnt * n; q += 256)
-    sg[q / n][q % n] = sigma[(size_t)s0 * n + q];
-  __syncthreads();
-  const size_t nm = (size_t)n * m;
-  double acc[kFwdS];
-#pragma unroll
-  for (int t = 0; t < kFwdS; ++t) acc[t] = 0.0;
-  // columns j < m: u_sj = sum_i sigma_si V_ij (+ b_j), then times T_sj
-  for (int j = threadIdx.x; j < m; j += 256) {
-    double u[kFwdS];
-    const double bj = use_bias ? v[nm + n + j] : 0.0;
-#pragma unroll
-    for (int t = 0; t < kFwdS; ++t) u[t] = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const double w = v[(size_t)i * m + j];
-#pragma unroll
-      for (int t = 0; t < kFwdS; ++t) u[t] = sg[t][i] > 0 ? u[t] + w : u[t] - w;
-    }
-#pragma unroll
-    for (int t = 0; t < kFwdS; ++t)
-      if (t < cnt) acc[t] += T[(size_t)(s0 + t) * m + j] * (u[t] + bj);
-  }
-  // column j = m (the a block): sum_i sigma_si a_i
-  if (use_bias)
-    for (int i = threadIdx.x; i < n; i += 256) {
-      const double w = v[nm + i];
-#pragma unroll
-      for (int t = 0; t < kFwdS; ++t) acc[t] += sg[t][i] > 0 ? w : -w;
-    }
-#pragma unroll
-  for (int t = 0; t < kFwdS; ++t) red[t][threadIdx.x] = acc[t];
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int t = 0; t < kFwdS; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + w];
-    __syncthreads();
-  }
-  if ((int)threadIdx.x < cnt) y[s0 + threadIdx.x] = red[threadIdx.x][0];
+__device__ inline double sig_aug(const int8_t* __restrict__ sigma, int ns, int n, int s, int i) {
+  if (s >= ns || i > n) return 0.0;
+  if (i == n) return 1.0;
+  return sigma[(size_t)s * n + i] > 0 ? 1.0 : -1.0;
+}
+__device__ inline double t_aug(const double* __restrict__ T, int ns, int m, int s, int j) {
+  if (s >= ns || j > m) return 0.0;
+  if (j == m) return 1.0;
+  return T[(size_t)s * m + j];
+}
+__device__ inline double v_aug(const double* __restrict__ v, int n, int m, int use_bias, int k,
+                               int j) {
+  if (k < n && j < m) return v[(size_t)k * m + j];
+  if (!use_bias) return 0.0;
+  if (k < n && j == m) return v[(size_t)n * m + k];
+  if (k == n && j < m) return v[(size_t)n * m + n + j];
+  return 0.0;
 }
 
-// partial[k][i*(m+1)+j] = sum_{s in chunk k} sigma'_si r_s T'_sj on the
-// augmented grid.  Block: 64 columns j x 4 row groups; each thread owns rows
-// i = g, g+4, ... (at most kMaxSites/4 + 1 accumulators).
-constexpr int kBwdRows = kMaxSites / 4 + 1;
+// ypart[cg][s] = sum over the column tiles of group cg of T'_sj (sigma' V')_sj
 __global__ void __launch_bounds__(256)
-    k_factored_bwd64(const int8_t* __restrict__ sigma,
-                     const double* __restrict__ T, int ns, int n, int m,
-                     const double* __restrict__ r,
-                     double* __restrict__ partial) {
-  __shared__ int8_t sg[kChunk][kMaxSites];
-  __shared__ double rr[kChunk];
-  const int k = blockIdx.y;
-  const int s0 = k * kChunk, cnt = min(kChunk, ns - s0);
-  for (int q = threadIdx.x; q < cnt * n; q += 256)
-    sg[q / n][q % n] = sigma[(size_t)s0 * n + q];
-  for (int q = threadIdx.x; q < cnt; q += 256) rr[q] = r[s0 + q];
-  __syncthreads();
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int g = threadIdx.x >> 6;
-  if (j > m) return;
-  double acc[kBwdRows];
+    k_fwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
+                 int m, int use_bias, const double* __restrict__ v, int Kp, int Np, int nrows,
+                 double* __restrict__ ypart) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row0 = (blockIdx.x * 4 + wave) * 16;
+  if (row0 >= nrows) return;
+  const int li = l & 15, lg = l >> 4;
+  const int ct0 = blockIdx.y * kF64Tiles, ct1 = min(ct0 + kF64Tiles, Np / 16);
+  const int nk = Kp / 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int ct = ct0; ct < ct1; ++ct) {
+    doublex4 c = {0.0, 0.0, 0.0, 0.0};
+    const int j = ct * 16 + li;
+#pragma unroll 4
+    for (int q = 0; q < nk; ++q)
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, row0 + li, 4 * q + lg),
+                                               v_aug(v, n, m, use_bias, 4 * q + lg, j), c, 0, 0,
+                                               0);
 #pragma unroll
-  for (int q = 0; q < kBwdRows; ++q) acc[q] = 0.0;
-  for (int s = 0; s < cnt; ++s) {
-    const double tq = (j < m ? T[(size_t)(s0 + s) * m + j] : 1.0) * rr[s];
-#pragma unroll
-    for (int q = 0; q < kBwdRows; ++q) {
-      const int i = g + 4 * q;
-      if (i < n) acc[q] += sg[s][i] > 0 ? tq : -tq;
-      else if (i == n) acc[q] += tq;
-    }
+    for (int q = 0; q < 4; ++q) acc[q] += t_aug(T, ns, m, row0 + 4 * lg + q, j) * c[q];
   }
-  const size_t W1 = (size_t)(m + 1);
-  double* out = partial + (size_t)k * (n + 1) * W1;
 #pragma unroll
-  for (int q = 0; q < kBwdRows; ++q) {
-    const int i = g + 4 * q;
-    if (i <= n) out[(size_t)i * W1 + j] = acc[q];
-  }
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int s = 8; s >= 1; s >>= 1) acc[q] += __shfl_xor(acc[q], s);
+  if (li == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ypart[(size_t)blockIdx.y * nrows + row0 + 4 * lg + q] = acc[q];
 }
 
-// g[c] = scale * sum_k partial[k][aug(c)], mapping the augmented grid to the
-// [W | a | b] layout
-__global__ void k_reduce_aug(const double* __restrict__ partial, int chunks,
-                             int n, int m, int use_bias, double scale,
+__global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows, int ns,
+                            double* __restrict__ y) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  double v = 0.0;
+  for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nrows + s];
+  y[s] = v;
+}
+
+// part[kc][i][j] = sum_{s in chunk kc} sigma'_si r_s T'_sj ; one wave per
+// (column tile, chunk)
+__global__ void __launch_bounds__(256)
+    k_bwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
+                 int m, const double* __restrict__ r, int Kp, int Np,
+                 double* __restrict__ part) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int ct = blockIdx.x * 4 + wave;
+  if (ct >= Np / 16) return;
+  const int kc = blockIdx.y;
+  const int li = l & 15, lg = l >> 4;
+  const int nit = Kp / 16;
+  const int j = ct * 16 + li;
+  doublex4 acc[kMaxIt64];
+#pragma unroll
+  for (int it = 0; it < kMaxIt64; ++it) acc[it] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = kc * kChunk; k0 < kc * kChunk + kChunk; k0 += 4) {
+    const int s = k0 + lg;
+    const double b = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j);
+#pragma unroll
+    for (int it = 0; it < kMaxIt64; ++it)
+      if (it < nit)
+        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, s, it * 16 + li), b,
+                                                       acc[it], 0, 0, 0);
+  }
+  double* out = part + (size_t)kc * Kp * Np;
+#pragma unroll
+  for (int it = 0; it < kMaxIt64; ++it)
+    if (it < nit)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[(size_t)(it * 16 + 4 * lg + q) * Np + j] = acc[it][q];
+}
+
+// g[c] = scale * sum_k part[k][aug(c)] on the padded (Kp x Np) grid, mapped to
+// the [W | a | b] layout
+__global__ void k_reduce_aug(const double* __restrict__ part, int chunks, int n, int m,
+                             int use_bias, int Kp, int Np, double scale,
                              double* __restrict__ g) {
   const size_t nm = (size_t)n * m;
   const size_t dim = use_bias ? nm + n + m : nm;
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= dim) return;
   size_t a;
-  const size_t W1 = (size_t)(m + 1);
-  if (c < nm) a = (c / m) * W1 + (c % m);
-  else if (c < nm + n) a = (c - nm) * W1 + m;
-  else a = (size_t)n * W1 + (c - nm - n);
-  const size_t stride = (size_t)(n + 1) * W1;
+  if (c < nm) a = (c / m) * Np + (c % m);
+  else if (c < nm + n) a = (c - nm) * Np + m;
+  else a = (size_t)n * Np + (c - nm - n);
+  const size_t stride = (size_t)Kp * Np;
   double acc = 0.0;
-  for (int k = 0; k < chunks; ++k) acc += partial[(size_t)k * stride + a];
+  for (int k = 0; k < chunks; ++k) acc += part[(size_t)k * stride + a];
   g[c] = acc * scale;
 }
+
+inline int kp16(int n) { return (n + 1 + 15) / 16 * 16; }
+inline int np16(int m) { return (m + 1 + 15) / 16 * 16; }
 
 __global__ void k_fill(double* x, double v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -330,33 +337,41 @@ void launch_copy_lower_shift(const double* S, double* M, int dim,
                      shift);
 }
 
+size_t factored_partial_doubles(int ns, int n, int m) {
+  const size_t grid = (size_t)colsum_chunks(ns) * kp16(n) * np16(m);
+  const size_t fwd = (size_t)((np16(m) / 16 + kF64Tiles - 1) / kF64Tiles) * ((ns + 15) / 16 * 16);
+  return (grid > fwd ? grid : fwd) + ns;
+}
+
 void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* v, double* y,
-                           hipStream_t st) {
+                           double* scratch, hipStream_t st) {
+  const int Kp = kp16(n), Np = np16(m), nrows = (ns + 15) / 16 * 16;
+  const int ncg = (Np / 16 + kF64Tiles - 1) / kF64Tiles;
   ProfGuard g("factored_fwd64", st);
-  hipLaunchKernelGGL(k_factored_fwd64, dim3((ns + kFwdS - 1) / kFwdS), dim3(256), 0,
-                     st, sigma, T, ns, n, m, use_bias, v, y);
+  hipLaunchKernelGGL(k_fwd64_mfma, dim3((nrows / 16 + 3) / 4, ncg), dim3(256), 0, st, sigma, T,
+                     ns, n, m, use_bias, v, Kp, Np, nrows, scratch);
+  hipLaunchKernelGGL(k_sum_ypart, dim3(nblk(ns)), dim3(256), 0, st, scratch, ncg, nrows, ns, y);
 }
 
 void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* r, double* g,
                            double* partial, hipStream_t st) {
   const int chunks = colsum_chunks(ns);
+  const int Kp = kp16(n), Np = np16(m);
   ProfGuard pg("factored_bwd64", st);
-  hipLaunchKernelGGL(k_factored_bwd64, dim3((m + 1 + 63) / 64, chunks), dim3(256), 0,
-                     st, sigma, T, ns, n, m, r, partial);
+  hipLaunchKernelGGL(k_bwd64_mfma, dim3((Np / 16 + 3) / 4, chunks), dim3(256), 0, st, sigma, T,
+                     ns, n, m, r, Kp, Np, partial);
   const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
-  hipLaunchKernelGGL(k_reduce_aug, dim3(nblk(dim)), dim3(256), 0, st, partial, chunks,
-                     n, m, use_bias, 1.0, g);
+  hipLaunchKernelGGL(k_reduce_aug, dim3(nblk(dim)), dim3(256), 0, st, partial, chunks, n, m,
+                     use_bias, Kp, Np, 1.0, g);
 }
 
 void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
                             int n, int m, int use_bias, double* omean,
                             double* partial, hipStream_t st) {
-  // r = 1/ns written into the tail of the partial buffer's first chunk is
-  // avoided: use a dedicated fill of a scratch vector at the end of partial.
-  const int chunks = colsum_chunks(ns);
-  double* ones = partial + (size_t)chunks * (n + 1) * (m + 1);
+  // r = 1/ns lives in the tail of the partial buffer
+  double* ones = partial + factored_partial_doubles(ns, n, m) - ns;
   hipLaunchKernelGGL(k_fill, dim3(nblk(ns)), dim3(256), 0, st, ones, 1.0 / ns, ns);
   launch_factored_bwd64(sigma, T, ns, n, m, use_bias, ones, omean, partial, st);
 }

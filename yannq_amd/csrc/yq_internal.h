@@ -93,9 +93,12 @@ void launch_copy_lower_shift(const double* S, double* M, int dim,
                              double shift, hipStream_t st);
 
 // factored O (sigma, T), fp64: y = O v (uncentred), length ns
+// scratch: factored_partial_doubles(ns, n, m) doubles (shared with the
+// backward product's partials)
+size_t factored_partial_doubles(int ns, int n, int m);
 void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* v, double* y,
-                           hipStream_t st);
+                           double* scratch, hipStream_t st);
 // g = O^T r (uncentred), length dim; partial has colsum_chunks(ns)*dim doubles
 void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* r, double* g,
