@@ -7,13 +7,17 @@ parameters), 16,384 samples per step.
 Timed path (value): mixed precision -- sampling and E_loc in float, O kept
 factored (spins and tanh(theta)), matrix-free CG whose products run as
 split-bf16 MFMA with float accumulation, FP64 iterative refinement to a 1e-8
-relative residual, FP64 update.  Also timed on the same start state: the plain
-FP64 path (explicit S = O_c^T O_c / Ns and Cholesky, the reference's default
-solver) and the FP64 factored CG.  Prints one JSON line.
+relative residual, FP64 update.  The CG uses the Jacobi preconditioner and
+starts from the previous step's solution (--precond/--warm-start; both
+solve the same system to the same tolerance).  Also timed on the same start
+state: the plain FP64 path (explicit S = O_c^T O_c / Ns and Cholesky, the
+reference's default solver), the FP64 factored CG, and the mixed path
+without the solver options.  Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,20 +34,21 @@ DIM = N * M + N + M
 CHAINS = 16384           # one sample per chain per step
 LAMBDA, LR, TOL = 1e-3, 0.01, 1e-8
 SEED = 20241203
-
-# algorithmic work per launch of the kernels that can dominate a step
 NS = CHAINS
+KP, NP = 128, 416        # padded (n+1) and (m+1) of the MFMA grids
+
+# algorithmic HBM bytes per launch of the kernels that can dominate a step
 KERNEL_WORK = {
-    # (bound, amount per launch, unit of amount)
-    # forward product: reads T' (Ns x (m+1) float) and sigma' (Ns x (n+1) bf16), writes y
-    "fwd_mfma": ("hbm", NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4, "B"),
-    # backward product: reads T' and sigma' and y, writes the (n+1) x (m+1) partial grids
-    "bwd_mfma": ("hbm", NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4, "B"),
-    # sampler: one sweep of N proposals per chain; W column reads are L2-resident,
-    # HBM traffic = spins in/out + samples out (sigma int8, theta float)
-    "sample": ("hbm", NS * N * 3 + NS * M * 4, "B"),
-    # E_loc ratios: read theta (float) and spins, write ratios (double)
-    "eloc_ratio": ("hbm", NS * M * 4 + NS * N + NS * N * 8, "B"),
+    # forward product: T' (Ns x (m+1) float) and sigma' (Ns x (n+1) bf16) in, y partials out
+    "fwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4,
+    # backward product: T'^T and sigma'^T and y in, the (n+1) x (m+1) chunk partials out
+    "bwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4 + (NS // 256) * KP * NP * 4,
+    # chunk reduction: the partials in, one dim-vector out
+    "bwd_reduce": (NS // 256) * KP * NP * 4 + DIM * 4 * 3,
+    # sampler: spins in/out and the recorded samples (spins, float theta)
+    "sample": NS * N * 3 + NS * M * 4,
+    # E_loc ratios: theta (float) and spins in, ratios (double) out
+    "eloc_ratio": NS * M * 4 + NS * N + NS * N * 8,
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -57,38 +62,38 @@ def _ctx(yq, precision):
     return ctx
 
 
-def run_path(yq, precision, solver, warmup, steps, prof_label=None):
+def run_path(yq, precision, solver, warmup, steps, precond=0, warm_start=0, prof_label=None):
+    """warmup + steps SR epochs from the same seeds; returns the timed seconds,
+    the per-step outputs, all energies and the HIP-event kernel timings."""
     ctx = _ctx(yq, precision)
+    kw = dict(n_sweeps=1, solver=solver, precision=precision, lam=LAMBDA, cg_tol=TOL,
+              cg_max_iter=50000, lr=LR, precond=precond, warm_start=warm_start,
+              allow_unconverged=True)
     energies, outs = [], []
     for i in range(warmup):
-        o = ctx.sr_step(n_sweeps=1, n_discard=20 if i == 0 else 0, solver=solver,
-                        precision=precision, lam=LAMBDA, cg_tol=TOL, cg_max_iter=50000,
-                        lr=LR, allow_unconverged=True)
+        o = ctx.sr_step(n_discard=20 if i == 0 else 0, **kw)
         energies.append(o["energy"])
     if prof_label:
         ctx._L.yq_prof_enable(ctx._h, prof_label.encode())
     t0 = time.perf_counter()
     for i in range(steps):
-        o = ctx.sr_step(n_sweeps=1, n_discard=0 if warmup else (20 if i == 0 else 0),
-                        solver=solver, precision=precision, lam=LAMBDA, cg_tol=TOL,
-                        cg_max_iter=50000, lr=LR, allow_unconverged=True)
+        o = ctx.sr_step(n_discard=0 if (warmup or i) else 20, **kw)
         outs.append(o)
         energies.append(o["energy"])
     t1 = time.perf_counter()   # every sr_step returns after its stream synchronised
     prof = None
     if prof_label:
-        import ctypes as C
+        prof = {}
         tot, cnt = C.c_double(), C.c_longlong()
         for lab in prof_label.split(","):
             ctx._L.yq_prof_read(ctx._h, lab.encode(), C.byref(tot), C.byref(cnt))
-            prof = prof or {}
             prof[lab] = (tot.value, cnt.value)
         ctx._L.yq_prof_enable(ctx._h, None)
     ctx.close()
     return (t1 - t0), outs, energies, prof
 
 
-def cpu_baseline(iters_per_step, seconds=20.0):
+def cpu_baseline(iters_per_step):
     """The oracle's threaded C++ restatement of the reference CG step
     (SamplerMT + constructDelta/constructObs + SRMat::solveCG), timed on a
     bounded sample of chains and scaled linearly in samples."""
@@ -100,13 +105,12 @@ def cpu_baseline(iters_per_step, seconds=20.0):
     c = 2048
     sig = orc.randomize(SEED, 0, np.arange(c), N)
     t0 = time.perf_counter()
-    fin, ctr, acc, ss, st = orc.sample_replay(rbm, sig, seed=SEED, n_sweeps=1, n_discard=0,
-                                              threads=threads)
+    _, _, _, ss, st = orc.sample_replay(rbm, sig, seed=SEED, n_sweeps=1, n_discard=0,
+                                        threads=threads)
     t_samp = time.perf_counter() - t0
-    import ctypes as Cc
     el = np.empty(c)
     t0 = time.perf_counter()
-    orc.lib().bl_eloc(Cc.byref(H), c, N, M, orc._p(rbm.W), orc._p(rbm.a), orc._p(ss),
+    orc.lib().bl_eloc(C.byref(H), c, N, M, orc._p(rbm.W), orc._p(rbm.a), orc._p(ss),
                       orc._p(st), orc._p(el), threads)
     t_eloc = time.perf_counter() - t0
     O = np.empty((c, DIM))
@@ -115,12 +119,11 @@ def cpu_baseline(iters_per_step, seconds=20.0):
     orc.lib().bl_sr_build(c, N, M, 1, orc._p(ss), orc._p(st), orc._p(el), orc._p(O),
                           orc._p(om), orc._p(F), threads)
     t_build = time.perf_counter() - t0
-    k = 20
     x = np.empty(DIM)
-    it, err = Cc.c_int(), Cc.c_double()
+    it, err = C.c_int(), C.c_double()
     t0 = time.perf_counter()
-    orc.lib().bl_cg_solve(c, DIM, orc._p(O), LAMBDA, orc._p(F), 1e-30, k, orc._p(x),
-                          Cc.byref(it), Cc.byref(err), threads)
+    orc.lib().bl_cg_solve(c, DIM, orc._p(O), LAMBDA, orc._p(F), 1e-30, 20, orc._p(x),
+                          C.byref(it), C.byref(err), threads)
     t_cg_iter = (time.perf_counter() - t0) / max(it.value, 1)
     scale = CHAINS / c
     t_step = scale * (t_samp + t_eloc + t_build + t_cg_iter * iters_per_step)
@@ -128,7 +131,7 @@ def cpu_baseline(iters_per_step, seconds=20.0):
         "value": 1.0 / t_step, "unit": "SR steps/s", "cores": threads, "kind": "port",
         "sample": (f"{c} chains x 1 sweep, E_loc, O build and {it.value} CG applies on "
                    f"{c} samples at N={N}, M={M}, scaled x{scale:.0f} to {CHAINS} samples "
-                   f"with {iters_per_step} CG iterations per step"),
+                   f"with {iters_per_step} CG iterations per step (the GPU FP64 CG count)"),
         "phase_s_at_full_size": {"sample": scale * t_samp, "eloc": scale * t_eloc,
                                  "build_O_F": scale * t_build,
                                  "cg": scale * t_cg_iter * iters_per_step},
@@ -141,6 +144,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--precond", type=int, default=1)
+    ap.add_argument("--warm-start", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -149,54 +154,62 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import yannq_amd as yq
+    opts = dict(precond=args.precond, warm_start=args.warm_start)
 
     # pick the dominant kernel from a profiled warm-up of the timed path
-    labels = ",".join(KERNEL_WORK)
     _, _, _, prof0 = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, max(args.warmup, 1), 2,
-                              prof_label=labels)
+                              prof_label=",".join(KERNEL_WORK), **opts)
     dominant = max(prof0, key=lambda k: prof0[k][0])
 
     dt, outs, energies, prof = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, args.warmup,
-                                        args.steps, prof_label=dominant)
+                                        args.steps, prof_label=dominant, **opts)
     ms = 1e3 * dt / args.steps
     value = args.steps / dt
     iters = [o["cg_iters"] for o in outs]
     tot_ms, launches = prof[dominant]
-    bound, amount, unit = KERNEL_WORK[dominant]
+    amount = KERNEL_WORK[dominant]
     avg_s = tot_ms / 1e3 / max(launches, 1)
     achieved = amount / avg_s / 1e9
-    roofline = {"bound": bound, "kernel": dominant, "achieved": achieved,
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": launches,
-                "algorithmic_bytes_per_launch": amount}
+                "algorithmic_bytes_per_launch": amount,
+                "share_of_step": tot_ms / (1e3 * dt)}
 
     extra = {}
+    fp64_iters = iters
     if not args.no_fp64:
         k = args.fp64_steps
-        dte, oute, ene, _ = run_path(yq, yq.FP64, yq.SOLVE_EXPLICIT, args.warmup, k)
-        dtc, outc, enc, _ = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, args.warmup, k)
+        dte, _, ene, _ = run_path(yq, yq.FP64, yq.SOLVE_EXPLICIT, args.warmup, k)
+        dtc, outc, _, _ = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, args.warmup, k, **opts)
+        dtp, outp, _, _ = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, args.warmup, k)
+        fp64_iters = [o["cg_iters"] for o in outc]
         extra = {
             "fp64_explicit_steps_per_s": k / dte,
             "fp64_factored_cg_steps_per_s": k / dtc,
+            "mixed_plain_cg_steps_per_s": k / dtp,
+            "mixed_plain_cg_iters": [o["cg_iters"] for o in outp],
             "mixed_vs_fp64_explicit": value / (k / dte),
             "mixed_vs_fp64_factored_cg": value / (k / dtc),
-            "fp64_cg_iters": [o["cg_iters"] for o in outc],
+            "fp64_cg_iters": fp64_iters,
             # same seeds and start state: energies of the first steps of each path
             "energy_rel_diff_mixed_vs_fp64": float(np.max(np.abs(
                 np.array(energies[:len(ene)]) - np.array(ene)) / np.abs(np.array(ene)))),
         }
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(int(np.median(iters)))
+        cpu = cpu_baseline(int(np.median(fp64_iters)))
     line = {
         "metric": "SR steps/sec (sample+E_loc+O+SR solve), TFI N=100 RBM alpha=4, mixed vs FP64",
         "value": value, "unit": "SR steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "1D TFI PBC N=100 h=1 (J=-1), real RBM alpha=4 (40500 params), "
-                              "16384 chains x 1 sweep = 16384 samples/step, persistent chains",
+                               "16384 chains x 1 sweep = 16384 samples/step, persistent chains",
                    "precision": "mixed: f32 sampling/E_loc, split-bf16x3 MFMA CG products "
                                 "(f32 accumulate), f64 refinement to 1e-8, f64 update",
+                   "solver": "factored CG, Jacobi preconditioner=%d, warm start=%d"
+                             % (args.precond, args.warm_start),
                    "lambda": LAMBDA, "lr": LR, "cg_tol": TOL},
         "samples_per_s": value * CHAINS,
         "cg_iters": iters,

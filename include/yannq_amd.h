@@ -162,6 +162,14 @@ YQ_API int yq_sr_solve(yq_ctx* ctx, double shift, double tol, int max_iter,
                        const double* rhs, double* x, double* resid,
                        int* iters);
 
+/* Options of the iterative solvers (no reference counterpart: the reference
+ * runs Eigen's CG with IdentityPreconditioner from x = 0, which is the
+ * default here).  precond 1: Jacobi preconditioner 1 / (diag S + shift),
+ * diag S = Var(O_k) from the batch.  warm_start 1: start the next solve from
+ * the previous solution.  Both leave the solved system and its tolerance
+ * unchanged. */
+YQ_API int yq_sr_set_options(yq_ctx* ctx, int precond, int warm_start);
+
 /* ---- Runner: Runners/RunRBM.hpp:151-220 --------------------------------- */
 typedef struct {
   int n_sweeps;      /* sample(nSweeps, .) */
@@ -175,6 +183,8 @@ typedef struct {
   int cg_max_iter;
   double lr;         /* SGD step eta (already max(alpha/t^p, 1e-4)) */
   double grad_clip;  /* 0 = off (RunRBM.hpp:205-209) */
+  int precond;       /* CG preconditioner, see yq_sr_set_options */
+  int warm_start;    /* CG warm start, see yq_sr_set_options */
 } yq_step_cfg;
 typedef struct {
   double energy;     /* srm.eloc() */

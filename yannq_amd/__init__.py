@@ -49,7 +49,8 @@ class StepCfg(C.Structure):
     _fields_ = [("n_sweeps", C.c_int), ("n_discard", C.c_int), ("randomize", C.c_int),
                 ("nup", C.c_int), ("solver", C.c_int), ("precision", C.c_int),
                 ("lambda_", C.c_double), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int),
-                ("lr", C.c_double), ("grad_clip", C.c_double)]
+                ("lr", C.c_double), ("grad_clip", C.c_double), ("precond", C.c_int),
+                ("warm_start", C.c_int)]
 
 
 class StepOut(C.Structure):
@@ -90,6 +91,7 @@ def lib():
         "yq_sr_build": [p, i, i], "yq_sr_get": [p, i, p], "yq_sr_apply": [p, d, p, p],
         "yq_sr_solve": [p, d, d, i, p, p, p, p], "yq_sr_step": [p, p, p],
         "yq_prof_enable": [p, C.c_char_p], "yq_prof_read": [p, C.c_char_p, p, p],
+        "yq_sr_set_options": [p, i, i],
         "yq_ff_create": [p, p], "yq_ff_npar": [p, p], "yq_ff_set_params": [p, p],
         "yq_ff_get_params": [p, p], "yq_ff_update_params": [p, p],
         "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
@@ -298,11 +300,15 @@ class Context:
     def ff_vjp_dev(self, ns, d_sigma, d_act, d_r, d_grad):
         self._chk(self._L.yq_ff_vjp_dev(self._h, ns, d_sigma, d_act, d_r, d_grad))
 
+    def sr_options(self, precond=0, warm_start=0):
+        self._chk(self._L.yq_sr_set_options(self._h, precond, warm_start))
+
     def sr_step(self, **kw):
         cfg = StepCfg(kw.get("n_sweeps", 1), kw.get("n_discard", 0), int(kw.get("randomize", 0)),
                       kw.get("nup", -1), kw.get("solver", SOLVE_CG_FACTORED),
                       kw.get("precision", FP64), kw.get("lam", 1e-3), kw.get("cg_tol", 1e-8),
-                      kw.get("cg_max_iter", 20000), kw.get("lr", 0.01), kw.get("grad_clip", 0.0))
+                      kw.get("cg_max_iter", 20000), kw.get("lr", 0.01), kw.get("grad_clip", 0.0),
+                      int(kw.get("precond", 0)), int(kw.get("warm_start", 0)))
         out = StepOut()
         self._chk(self._L.yq_sr_step(self._h, C.byref(cfg), C.byref(out)),
                   allow=(ERR_NOT_CONVERGED,) if kw.get("allow_unconverged") else ())

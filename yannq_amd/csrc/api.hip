@@ -105,6 +105,10 @@ struct yq_ctx {
   Buf T64, T32, O, omean, F, S, M, partial, ecent, info;
   Buf vx, vr, vp, vAp, vy, vtmp, vb, dotp;
   yq::MixedWork mixed;
+  // solver options: Jacobi preconditioning, warm start from the last solution
+  int precond = 0, warm_start = 0;
+  Buf diagS, dinv64, vz, xprev;
+  bool have_xprev = false;
 
   // FeedForward machine
   bool have_ff = false;
@@ -117,7 +121,8 @@ struct yq_ctx {
     for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp,
-                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad})
+                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
+                   &xprev})
       bufs.push_back(b);
   }
 };
@@ -334,6 +339,12 @@ int sr_build(yq_ctx* c, int solver, int precision) {
                                (rocblas_int)dim, &beta, c->S.as<double>(), (rocblas_int)dim));
     }
   }
+  // diag(S) for the Jacobi preconditioner (cheap: one pass over T)
+  TRY(ensure(c, c->diagS, sizeof(double) * dim));
+  TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)(colsum_chunks(ns) + 1) * m)));
+  launch_diag_S(c->T64.as<double>(), ns, n, m, c->use_bias, c->omean.as<double>(),
+                c->diagS.as<double>(), c->vtmp.as<double>(), c->st);
+  LAUNCHCHK(c);
   TRY(sync(c));
   c->have_sr = true;
   c->sr_solver = solver;
@@ -383,40 +394,70 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
   return YQ_OK;
 }
 
-// Conjugate gradients from x = 0 (SRMat.hpp:197-223 through Eigen's
-// ConjugateGradient): stop when |r| < tol |b|.
+// Conjugate gradients (SRMat.hpp:197-223 through Eigen's ConjugateGradient):
+// stop when |r| < tol |b|.  x starts at 0, or at x on entry when x0 is set
+// (warm start); with c->precond the Jacobi preconditioner 1/(diag S + shift)
+// is applied (Eigen's IdentityPreconditioner when 0).
 int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int max_iter,
-         int* iters, double* err) {
+         int* iters, double* err, bool x0) {
   const size_t dim = c->dim;
   TRY(ensure(c, c->vr, sizeof(double) * dim));
   TRY(ensure(c, c->vp, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
+  TRY(ensure(c, c->vz, sizeof(double) * dim));
+  TRY(ensure(c, c->dinv64, sizeof(double) * dim));
   double *r = c->vr.as<double>(), *p = c->vp.as<double>(), *Ap = c->vAp.as<double>();
-  HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
-  HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(c, hipMemcpyAsync(p, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  double* z = c->vz.as<double>();
+  const double* dinv = nullptr;
+  if (c->precond) {
+    launch_jacobi_inv64(c->diagS.as<double>(), shift, c->dinv64.as<double>(), dim, c->st);
+    dinv = c->dinv64.as<double>();
+  }
   double bn2;
   TRY(dot(c, b, b, dim, &bn2));
   *iters = 0;
   *err = 0.0;
-  if (bn2 == 0.0) return YQ_OK;
+  if (bn2 == 0.0) {
+    HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+    return YQ_OK;
+  }
+  if (x0) {  // r = b - (S + shift) x
+    TRY(apply64(c, shift, x, Ap));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    launch_axpy(-1.0, Ap, r, dim, c->st);
+  } else {
+    HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  }
+  if (dinv) launch_mul(dinv, r, z, dim, c->st);
+  else HIPCHK(c, hipMemcpyAsync(z, r, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(p, z, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  LAUNCHCHK(c);
   const double thr = tol * tol * bn2;
-  double rn2 = bn2, absNew = bn2;
+  double rn2, rho;
+  TRY(dot(c, r, r, dim, &rn2));
+  TRY(dot(c, r, z, dim, &rho));
   int it = 0;
   while (rn2 >= thr && it < max_iter) {
     TRY(apply64(c, shift, p, Ap));
     double pAp;
     TRY(dot(c, p, Ap, dim, &pAp));
-    const double alpha = absNew / pAp;
+    const double alpha = rho / pAp;
     launch_axpy(alpha, p, x, dim, c->st);
     launch_axpy(-alpha, Ap, r, dim, c->st);
     LAUNCHCHK(c);
     TRY(dot(c, r, r, dim, &rn2));
     ++it;
     if (rn2 < thr) break;
-    const double beta = rn2 / absNew;
-    absNew = rn2;
-    launch_xpby(r, beta, p, dim, c->st);
+    double rho_new = rn2;
+    if (dinv) {
+      launch_mul(dinv, r, z, dim, c->st);
+      TRY(dot(c, r, z, dim, &rho_new));
+      launch_xpby(z, rho_new / rho, p, dim, c->st);
+    } else {
+      launch_xpby(r, rho_new / rho, p, dim, c->st);
+    }
+    rho = rho_new;
     LAUNCHCHK(c);
   }
   *iters = it;
@@ -475,25 +516,39 @@ int solve_explicit(yq_ctx* c, double shift, const double* b, double* x) {
 // FP64 iterative refinement around the mixed-precision inner CG:
 // x += CG_mixed((S+shift) d = r), r = b - (S+shift) x in FP64.
 int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
-                int max_iter, int* iters, double* err) {
+                int max_iter, int* iters, double* err, bool x0) {
   const size_t dim = c->dim;
   TRY(ensure(c, c->vr, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
   TRY(ensure(c, c->vp, sizeof(double) * dim));
   double *r = c->vr.as<double>(), *Ax = c->vAp.as<double>(), *d = c->vp.as<double>();
-  HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
-  HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
   double bn2;
   TRY(dot(c, b, b, dim, &bn2));
   *iters = 0;
   *err = 0.0;
-  if (bn2 == 0.0) return YQ_OK;
+  if (bn2 == 0.0) {
+    HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+    return YQ_OK;
+  }
   double rel = 1.0;
+  if (x0) {  // warm start: r = b - (S + shift) x
+    TRY(apply64(c, shift, x, Ax));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    launch_axpy(-1.0, Ax, r, dim, c->st);
+    LAUNCHCHK(c);
+    double rn2;
+    TRY(dot(c, r, r, dim, &rn2));
+    rel = sqrt(rn2 / bn2);
+  } else {
+    HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  }
   int total = 0;
-  for (int outer = 0; outer < 30 && total < max_iter; ++outer) {
+  for (int outer = 0; outer < 30 && total < max_iter && !(rel < tol); ++outer) {
     int it = 0;
     const double inner_tol = std::max(tol / rel, 1e-4) < 1.0 ? std::max(tol / rel, 1e-4) : 0.5;
-    int rc = mixed_cg(c->mixed, shift, c->omean.as<double>(), r, d, inner_tol,
+    int rc = mixed_cg(c->mixed, shift, c->omean.as<double>(),
+                      c->precond ? c->diagS.as<double>() : nullptr, r, d, inner_tol,
                       max_iter - total, &it, c->st, c->err);
     total += it;
     if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
@@ -524,17 +579,27 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
     rc = solve_explicit(c, shift, b_dev, x_dev);
     if (rc) return rc;
     TRY(residual64(c, shift, x_dev, b_dev, &err));
-  } else if (c->sr_prec == YQ_MIXED) {
-    if (tol <= 0 || max_iter <= 0) return fail(c, YQ_ERR_ARG, "tol and max_iter must be > 0");
-    rc = solve_mixed(c, shift, b_dev, x_dev, tol, max_iter, &it, &err);
-    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
   } else {
     if (tol <= 0 || max_iter <= 0) return fail(c, YQ_ERR_ARG, "tol and max_iter must be > 0");
-    rc = cg64(c, shift, b_dev, x_dev, tol, max_iter, &it, &err);
-    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
-    double e2;
-    TRY(residual64(c, shift, x_dev, b_dev, &e2));
-    err = e2;
+    // warm start from the previous iterative solution (same dimension)
+    const bool x0 = c->warm_start && c->have_xprev;
+    if (x0)
+      HIPCHK(c, hipMemcpyAsync(x_dev, c->xprev.p, sizeof(double) * c->dim,
+                               hipMemcpyDeviceToDevice, c->st));
+    if (c->sr_prec == YQ_MIXED) {
+      rc = solve_mixed(c, shift, b_dev, x_dev, tol, max_iter, &it, &err, x0);
+      if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+    } else {
+      rc = cg64(c, shift, b_dev, x_dev, tol, max_iter, &it, &err, x0);
+      if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+      double e2;
+      TRY(residual64(c, shift, x_dev, b_dev, &e2));
+      err = e2;
+    }
+    TRY(ensure(c, c->xprev, sizeof(double) * c->dim));
+    HIPCHK(c, hipMemcpyAsync(c->xprev.p, x_dev, sizeof(double) * c->dim,
+                             hipMemcpyDeviceToDevice, c->st));
+    c->have_xprev = true;
   }
   TRY(sync(c));
   if (resid) *resid = err;
@@ -618,6 +683,7 @@ YQ_API int yq_rbm_create(yq_ctx* c, int dtype, int n, int m, int use_bias) {
   c->have_rbm = true;
   c->have_batch = false;
   c->have_sampler = false;
+  c->have_xprev = false;
   TRY(refresh_params(c));
   return sync(c);
 }
@@ -1219,6 +1285,15 @@ YQ_API int yq_prof_read(yq_ctx* c, const char* label, double* total_ms, long lon
   return YQ_OK;
 }
 
+YQ_API int yq_sr_set_options(yq_ctx* c, int precond, int warm_start) {
+  if (!c) return YQ_ERR_ARG;
+  if (precond != 0 && precond != 1) return fail(c, YQ_ERR_ARG, "precond must be 0 or 1");
+  c->precond = precond;
+  c->warm_start = warm_start ? 1 : 0;
+  if (!c->warm_start) c->have_xprev = false;
+  return YQ_OK;
+}
+
 // ---------------------------------------------------------------- runner
 YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
   TRY(need_rbm(c));
@@ -1235,6 +1310,7 @@ YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
   };
   int rc = YQ_OK;
   do {
+    if ((rc = yq_sr_set_options(c, cfg->precond, cfg->warm_start)) != YQ_OK) break;
     if (cfg->randomize && (rc = yq_sampler_randomize(c, cfg->nup)) != YQ_OK) break;
     if ((rc = hipEventRecord(e0, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
     int ns = 0;

@@ -29,6 +29,7 @@ struct MixedWork {
   float* om32 = nullptr;       // column mean in float (dim)
   // CG vectors (float) and device scalars (double)
   float *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
+  float* dinv = nullptr;       // Jacobi preconditioner 1 / (diag S + shift)
   double* dpart = nullptr;     // per-block partials of the fused dot products
   unsigned* ticket = nullptr;  // last-block counter of the fused dot products
   double* scal = nullptr;      // 4 scalars per CG iteration + scratch
@@ -46,7 +47,8 @@ int mixed_apply64(MixedWork& w, double shift, const double* omean64,
                   const double* v, double* out, hipStream_t st, std::string& err);
 // Inner CG in float with device scalars: solves (S + shift) x = b to the
 // relative residual tol (checked every 8 iterations); b and x are FP64.
-int mixed_cg(MixedWork& w, double shift, const double* omean64,
+// diag64 (diag S in FP64) selects Jacobi preconditioning; null for none.
+int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* diag64,
              const double* b, double* x, double tol, int max_iter, int* iters,
              hipStream_t st, std::string& err);
 void mixed_release(MixedWork& w);

@@ -94,6 +94,54 @@ __device__ void block_dot_finish(double v, double* part, unsigned* ticket, doubl
   }
 }
 
+// Two dot products finished together (partials interleaved per block).
+__device__ void block_dot_finish2(double v1, double v2, double* part, unsigned* ticket,
+                                  double* dst1, double* dst2) {
+  __shared__ double r1[256], r2[256];
+  __shared__ bool last;
+  const int t = threadIdx.x;
+  r1[t] = v1;
+  r2[t] = v2;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w) {
+      r1[t] += r1[t + w];
+      r2[t] += r2[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = r2[0];
+    __threadfence();
+    const unsigned prev = atomicAdd(ticket, 1u);
+    last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double s1 = 0.0, s2 = 0.0;
+  for (unsigned i = t; i < gridDim.x; i += 256) {
+    s1 += __hip_atomic_load(part + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s2 += __hip_atomic_load(part + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  r1[t] = s1;
+  r2[t] = s2;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w) {
+      r1[t] += r1[t + w];
+      r2[t] += r2[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    *dst1 = r1[0];
+    *dst2 = r2[0];
+    *ticket = 0u;
+  }
+}
+
 // index of parameter c on the (Kp x Np) grid of V' / G'
 __device__ inline size_t aug_index(size_t c, int n, int m, int Np) {
   const size_t nm = (size_t)n * m;
@@ -133,13 +181,14 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
   Tt[(size_t)j * nsp + s] = v;
 }
 
-// p <- r + beta p on the parameter vector (beta = rr_now / rr_prev, or 0 when
-// rr_prev is null), V' = p on the (Kp x Np) grid split into three bf16 layers
-// stored transposed, vs[(layer * Np + j) * Kp + k]; *dst = <omean, p>.
+// p <- z + beta p with z = dinv * r (dinv null: z = r) and beta = rho_now /
+// rho_prev (0 when rho_prev is null); V' = p on the (Kp x Np) grid split into
+// three bf16 layers stored transposed, vs[(layer * Np + j) * Kp + k];
+// *dst = <omean, p>.
 __global__ void __launch_bounds__(256)
-    k_split_v(const float* __restrict__ r, float* __restrict__ p, const double* rr_now,
-              const double* rr_prev, const float* __restrict__ om, int n, int m,
-              int use_bias, int Kp, int Np, uint16_t* __restrict__ vs, double* part,
+    k_split_v(const float* __restrict__ r, const float* __restrict__ dinv, float* __restrict__ p,
+              const double* rr_now, const double* rr_prev, const float* __restrict__ om, int n,
+              int m, int use_bias, int Kp, int Np, uint16_t* __restrict__ vs, double* part,
               unsigned* ticket, double* dst) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nm = (size_t)n * m;
@@ -153,7 +202,8 @@ __global__ void __launch_bounds__(256)
     else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
     if (c >= 0) {
       const float beta = rr_prev ? (float)(*rr_now / *rr_prev) : 0.0f;
-      v = rr_prev ? r[c] + beta * p[c] : r[c];
+      const float z = dinv ? r[c] * dinv[c] : r[c];
+      v = rr_prev ? z + beta * p[c] : z;
       p[c] = v;
       dv = (double)om[c] * (double)v;
     }
@@ -166,102 +216,154 @@ __global__ void __launch_bounds__(256)
   block_dot_finish(dv, part, ticket, dst);
 }
 
-// ypart[cg][s] = sum over the column tiles of group cg of T'_sj U'_sj;
-// one wave per 16 samples, blockIdx.y = column group
+// ypart[cg][s] = sum over the column tiles of group cg of T'_sj U'_sj.
+// Block: 4 waves x 64 rows (4 row tiles per wave); blockIdx.y = column group
+// of kColTiles tiles whose three V' layers are staged once in LDS (rows padded
+// by kLdsPad elements against bank conflicts) and shared by the 4 waves.
+constexpr int kLdsPad = 8;
 __global__ void __launch_bounds__(256)
     k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
                const float* __restrict__ Trm, int Kp, int Np, int nsp,
                float* __restrict__ ypart) {
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int row0 = (blockIdx.x * 4 + wave) * 16;
-  if (row0 >= nsp) return;
+  extern __shared__ uint16_t bsh[];  // [3][kColTiles*16][Kp + kLdsPad]
+  const int tid = threadIdx.x;
   const int cg = blockIdx.y;
-  const int ct0 = cg * kColTiles;
-  const int ct1 = min(ct0 + kColTiles, Np / 16);
-  const int nks = Kp / 32;
+  const int col0 = cg * kColTiles * 16;
+  const int ncols = min(kColTiles * 16, Np - col0);
+  const int ldk = Kp + kLdsPad;
+  const int kchunks = Kp / 8;
+  for (int q = tid; q < 3 * ncols * kchunks; q += 256) {
+    const int sp = q / (ncols * kchunks);
+    const int rem = q % (ncols * kchunks);
+    const int col = rem / kchunks, kk = (rem % kchunks) * 8;
+    *reinterpret_cast<u16x8*>(bsh + (sp * kColTiles * 16 + col) * ldk + kk) =
+        *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp + kk);
+  }
+  __syncthreads();
+  const int wave = tid >> 6, l = tid & 63;
   const int li = l & 15, lg = l >> 4;
-  bf16x8 afr[kMaxKs];
+  const int rbase = blockIdx.x * 256 + wave * 64;
+  const int nks = Kp / 32;
+  bf16x8 afr[4][kMaxKs];
 #pragma unroll
-  for (int ks = 0; ks < kMaxKs; ++ks)
-    if (ks < nks) afr[ks] = ld8(sigA + (size_t)(row0 + li) * Kp + ks * 32 + lg * 8);
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const size_t layer = (size_t)Np * Kp;
-  for (int ct = ct0; ct < ct1; ++ct) {
-    floatx4 c = {0.f, 0.f, 0.f, 0.f};
-    const uint16_t* vb = vs + (size_t)(ct * 16 + li) * Kp + lg * 8;
-    float tv[4];
+  for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) tv[q] = Trm[(size_t)(row0 + 4 * lg + q) * Np + ct * 16 + li];
+    for (int ks = 0; ks < kMaxKs; ++ks)
+      if (ks < nks) afr[rt][ks] = ld8(sigA + (size_t)(rbase + rt * 16 + li) * Kp + ks * 32 + lg * 8);
+  float acc[4][4];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[rt][q] = 0.f;
+  for (int ct = 0; ct < ncols / 16; ++ct) {
+    floatx4 c[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) c[rt] = (floatx4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int sp = 0; sp < 3; ++sp)
 #pragma unroll
       for (int ks = 0; ks < kMaxKs; ++ks)
-        if (ks < nks)
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[ks], ld8(vb + sp * layer + ks * 32),
-                                                      c, 0, 0, 0);
+        if (ks < nks) {
+          const bf16x8 b =
+              ld8(bsh + (sp * kColTiles * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += tv[q] * c[q];
+          for (int rt = 0; rt < 4; ++rt)
+            c[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[rt][ks], b, c[rt], 0, 0, 0);
+        }
+    const int j = col0 + ct * 16 + li;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc[rt][q] += Trm[(size_t)(rbase + rt * 16 + 4 * lg + q) * Np + j] * c[rt][q];
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) acc[q] += __shfl_xor(acc[q], s);
-  if (li == 0)
+    for (int q = 0; q < 4; ++q) {
+      float v = acc[rt][q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ypart[(size_t)cg * nsp + row0 + 4 * lg + q] = acc[q];
+      for (int s = 8; s >= 1; s >>= 1) v += __shfl_xor(v, s);
+      if (li == 0) ypart[(size_t)cg * nsp + rbase + rt * 16 + 4 * lg + q] = v;
+    }
 }
 
-// y_s = (sum_cg ypart - <omean, p>) / ns for s < ns, 0 beyond
-__global__ void k_center_y(const float* __restrict__ ypart, int ncg, int ns, int nsp,
-                           const double* __restrict__ dotp, float* __restrict__ y) {
+// y_s = (sum_cg ypart - <omean, p>) / ns for s < ns, 0 beyond; *dst = sum_s y_s
+// (the centring correction of the backward product)
+__global__ void __launch_bounds__(256)
+    k_center_y(const float* __restrict__ ypart, int ncg, int ns, int nsp,
+               const double* __restrict__ dotp, float* __restrict__ y, double* dpart,
+               unsigned* ticket, double* dst) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsp) return;
-  float v = 0.0f;
-  for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
-  y[s] = s < ns ? (v - (float)dotp[0]) / (float)ns : 0.0f;
+  double dv = 0.0;
+  if (s < nsp) {
+    float v = 0.0f;
+    for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
+    const float yv = s < ns ? (v - (float)dotp[0]) / (float)ns : 0.0f;
+    y[s] = yv;
+    dv = yv;
+  }
+  block_dot_finish(dv, dpart, ticket, dst);
 }
 
-// part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj ; one wave per
-// (column tile, chunk); all Kp/16 row tiles accumulate in registers
+// part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj.  Block: 4 waves =
+// 4 column tiles of one chunk; the chunk's sigma'^T slab is staged in LDS in
+// two halves of 128 samples and shared by the waves; every wave keeps all
+// Kp/16 row tiles in registers.
+constexpr int kBwdHalf = 128;
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ y, int Kp, int Np, int nsp,
                float* __restrict__ part) {
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __shared__ uint16_t ash[160 * (kBwdHalf + kLdsPad)];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, l = tid & 63;
   const int ct = blockIdx.x * 4 + wave;
-  if (ct >= Np / 16) return;
+  const bool active = ct < Np / 16;
   const int kc = blockIdx.y;
   const int li = l & 15, lg = l >> 4;
   const int nit = Kp / 16;
+  const int lda = kBwdHalf + kLdsPad;
   floatx4 acc[kMaxIt];
 #pragma unroll
   for (int it = 0; it < kMaxIt; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
-  const float* trow = Tt + (size_t)(ct * 16 + li) * nsp;
-#pragma unroll 2
-  for (int ks = 0; ks < kBwdChunk / 32; ++ks) {
-    const int s0 = kc * kBwdChunk + ks * 32 + lg * 8;
-    u16x8 bh, bm, bl;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const float v = trow[s0 + e] * y[s0 + e];
-      uint16_t h, md, lo;
-      split3(v, h, md, lo);
-      bh[e] = h;
-      bm[e] = md;
-      bl[e] = lo;
+  const float* trow = Tt + (size_t)(active ? ct * 16 + li : 0) * nsp;
+  for (int half = 0; half < kBwdChunk / kBwdHalf; ++half) {
+    const int sbase = kc * kBwdChunk + half * kBwdHalf;
+    __syncthreads();
+    for (int q = tid; q < Kp * (kBwdHalf / 8); q += 256) {
+      const int i = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
+      *reinterpret_cast<u16x8*>(ash + i * lda + kk) =
+          *reinterpret_cast<const u16x8*>(sigT + (size_t)i * nsp + sbase + kk);
     }
-    const bf16x8 b0 = __builtin_bit_cast(bf16x8, bh);
-    const bf16x8 b1 = __builtin_bit_cast(bf16x8, bm);
-    const bf16x8 b2 = __builtin_bit_cast(bf16x8, bl);
+    __syncthreads();
+    if (!active) continue;
+    for (int ks = 0; ks < kBwdHalf / 32; ++ks) {
+      const int s0 = sbase + ks * 32 + lg * 8;
+      u16x8 bh, bm, bl;
 #pragma unroll
-    for (int it = 0; it < kMaxIt; ++it)
-      if (it < nit) {
-        const bf16x8 a = ld8(sigT + (size_t)(it * 16 + li) * nsp + s0);
-        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[it], 0, 0, 0);
-        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[it], 0, 0, 0);
-        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2, acc[it], 0, 0, 0);
+      for (int e = 0; e < 8; ++e) {
+        const float v = trow[s0 + e] * y[s0 + e];
+        uint16_t h, md, lo;
+        split3(v, h, md, lo);
+        bh[e] = h;
+        bm[e] = md;
+        bl[e] = lo;
       }
+      const bf16x8 b0 = __builtin_bit_cast(bf16x8, bh);
+      const bf16x8 b1 = __builtin_bit_cast(bf16x8, bm);
+      const bf16x8 b2 = __builtin_bit_cast(bf16x8, bl);
+#pragma unroll
+      for (int it = 0; it < kMaxIt; ++it)
+        if (it < nit) {
+          const bf16x8 a = ld8(ash + (it * 16 + li) * lda + ks * 32 + lg * 8);
+          acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[it], 0, 0, 0);
+          acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[it], 0, 0, 0);
+          acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2, acc[it], 0, 0, 0);
+        }
+    }
   }
+  if (!active) return;
   float* out = part + (size_t)kc * Kp * Np;
 #pragma unroll
   for (int it = 0; it < kMaxIt; ++it)
@@ -271,56 +373,64 @@ __global__ void __launch_bounds__(256)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
 
-// out[c] = G'[aug(c)] - omean_c G'[n][m] + shift p_c (G' summed over chunks in
-// order); *dst = <p, out>
+// out[c] = G'[aug(c)] - omean_c sum(y) + shift p_c, G' summed over the chunks
+// with eight interleaved partial sums combined by a fixed tree; *dst = <p, out>
 __global__ void __launch_bounds__(256)
     k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m, int use_bias,
                  int Kp, int Np, const float* __restrict__ om, double shift,
-                 const float* __restrict__ p, float* __restrict__ out, double* dpart,
-                 unsigned* ticket, double* dst) {
+                 const double* __restrict__ sum_y, const float* __restrict__ p,
+                 float* __restrict__ out, double* dpart, unsigned* ticket, double* dst) {
   const size_t nm = (size_t)n * m;
   const size_t dim = use_bias ? nm + n + m : nm;
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double dv = 0.0;
   if (c < dim) {
     const size_t a = aug_index(c, n, m, Np);
-    const size_t corner = (size_t)n * Np + m;
     const size_t stride = (size_t)Kp * Np;
-    float g = 0.0f, sy = 0.0f;
-    for (int k = 0; k < chunks; ++k) {
-      g += part[(size_t)k * stride + a];
-      sy += part[(size_t)k * stride + corner];
+    float g8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < chunks; k += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k + u < chunks) g8[u] += part[(size_t)(k + u) * stride + a];
     }
-    const float o = (g - om[c] * sy) + (float)shift * p[c];
+    const float g = ((g8[0] + g8[1]) + (g8[2] + g8[3])) + ((g8[4] + g8[5]) + (g8[6] + g8[7]));
+    const float o = (g - om[c] * (float)sum_y[0]) + (float)shift * p[c];
     out[c] = o;
     dv = (double)p[c] * (double)o;
   }
   block_dot_finish(dv, dpart, ticket, dst);
 }
 
-// x += alpha p, r -= alpha Ap with alpha = rr / pAp; *dst = <r, r>
+// x += alpha p, r -= alpha Ap with alpha = rho / pAp; next slot: rho' = <r, dinv r>
+// and <r, r>
 __global__ void __launch_bounds__(256)
     k_cg_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
-            const float* __restrict__ Ap, size_t n, const double* __restrict__ sc,
-            double* dpart, unsigned* ticket, double* dst) {
+            const float* __restrict__ Ap, const float* __restrict__ dinv, size_t n,
+            const double* __restrict__ sc, double* dpart, unsigned* ticket, double* next) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double dv = 0.0;
+  double rz = 0.0, rr = 0.0;
   if (i < n) {
     const double pap = sc[1];
     const float alpha = pap != 0.0 ? (float)(sc[0] / pap) : 0.0f;
     x[i] += alpha * p[i];
     const float rv = r[i] - alpha * Ap[i];
     r[i] = rv;
-    dv = (double)rv * (double)rv;
+    rr = (double)rv * (double)rv;
+    rz = dinv ? (double)rv * (double)(rv * dinv[i]) : rr;
   }
-  block_dot_finish(dv, dpart, ticket, dst);
+  block_dot_finish2(rz, rr, dpart, ticket, next + 0, next + 4);
 }
 
 __global__ void __launch_bounds__(256)
-    k_dot32(const float* __restrict__ x, const float* __restrict__ y, size_t n, double* dpart,
-            unsigned* ticket, double* dst) {
+    k_dot_rz(const float* __restrict__ r, const float* __restrict__ dinv, size_t n, double* dpart,
+             unsigned* ticket, double* slot) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  block_dot_finish(i < n ? (double)x[i] * (double)y[i] : 0.0, dpart, ticket, dst);
+  double rz = 0.0, rr = 0.0;
+  if (i < n) {
+    rr = (double)r[i] * (double)r[i];
+    rz = dinv ? (double)r[i] * (double)(r[i] * dinv[i]) : rr;
+  }
+  block_dot_finish2(rz, rr, dpart, ticket, slot + 0, slot + 4);
 }
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,6 +439,12 @@ __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_
 __global__ void k_f2d(const float* __restrict__ a, double* __restrict__ b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (double)a[i];
+}
+// Jacobi preconditioner 1 / (diag(S) + shift)
+__global__ void k_jacobi_inv(const double* __restrict__ diag, double shift, float* __restrict__ out,
+                             size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)(1.0 / (diag[i] + shift));
 }
 __global__ void k_zero_f(float* __restrict__ a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -355,21 +471,23 @@ int alloc(T*& p, size_t count, std::string& err) {
   return YQ_OK;
 }
 
-// (S + shift) p -> Ap with the products on MFMA; <omean, p> goes to dot_op and
-// <p, Ap> to dot_pap.  p is first updated to r + beta p (see k_split_v).
-void apply_launches(MixedWork& w, double shift, const float* r, const double* rr_now,
-                    const double* rr_prev, double* dot_op, double* dot_pap, hipStream_t st) {
+// (S + shift) p -> Ap with the products on MFMA; the slot s receives
+// s[2] = <omean, p>, s[3] = sum(y) and s[1] = <p, Ap>.  p is first updated to
+// r + beta p (see k_split_v).
+void apply_launches(MixedWork& w, double shift, const float* r, const float* dinv,
+                    const double* rr_now, const double* rr_prev, double* s, hipStream_t st) {
   const size_t grid_v = (size_t)w.Np * w.Kp;
-  hipLaunchKernelGGL(k_split_v, dim3(nb(grid_v)), dim3(256), 0, st, r, w.p, rr_now, rr_prev,
+  hipLaunchKernelGGL(k_split_v, dim3(nb(grid_v)), dim3(256), 0, st, r, dinv, w.p, rr_now, rr_prev,
                      w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np, w.vsplit, w.dpart, w.ticket,
-                     dot_op);
+                     s + 2);
   {
     ProfGuard g("fwd_mfma", st);
-    hipLaunchKernelGGL(k_fwd_mfma, dim3(w.nsp / 64, w.ncg), dim3(256), 0, st, w.sigA, w.vsplit,
-                       w.Trm, w.Kp, w.Np, w.nsp, w.ypart);
+    const size_t lds = sizeof(uint16_t) * 3 * kColTiles * 16 * (w.Kp + kLdsPad);
+    hipLaunchKernelGGL(k_fwd_mfma, dim3(w.nsp / 256, w.ncg), dim3(256), lds, st, w.sigA,
+                       w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart);
   }
   hipLaunchKernelGGL(k_center_y, dim3(nb(w.nsp)), dim3(256), 0, st, w.ypart, w.ncg, w.ns, w.nsp,
-                     dot_op, w.y);
+                     s + 2, w.y, w.dpart, w.ticket, s + 3);
   const int chunks = w.nsp / kBwdChunk;
   {
     ProfGuard g("bwd_mfma", st);
@@ -378,14 +496,15 @@ void apply_launches(MixedWork& w, double shift, const float* r, const double* rr
   }
   ProfGuard g("bwd_reduce", st);
   hipLaunchKernelGGL(k_bwd_reduce, dim3(nb(w.dim)), dim3(256), 0, st, w.part, chunks, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.om32, shift, w.p, w.Ap, w.dpart, w.ticket,
-                     dot_pap);
+                     w.use_bias, w.Kp, w.Np, w.om32, shift, s + 3, w.p, w.Ap, w.dpart, w.ticket,
+                     s + 1);
 }
 
+constexpr int kSlot = 8;  // doubles per CG iteration slot
 int ensure_scal(MixedWork& w, int iters, std::string& err) {
   const int need = iters + 4 > 16 ? iters + 4 : 16;
   if (w.scal_cap >= need) return YQ_OK;
-  int rc = alloc(w.scal, (size_t)4 * need, err);
+  int rc = alloc(w.scal, (size_t)kSlot * need, err);
   if (rc) return rc;
   w.scal_cap = need;
   return YQ_OK;
@@ -420,7 +539,8 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
         (rc = alloc(w.vsplit, (size_t)3 * w.Np * w.Kp, err)) ||
         (rc = alloc(w.om32, w.dim, err)) || (rc = alloc(w.x, w.dim, err)) ||
         (rc = alloc(w.r, w.dim, err)) || (rc = alloc(w.p, w.dim, err)) ||
-        (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.dpart, dparts, err)) ||
+        (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.dpart, 2 * dparts, err)) ||
+        (rc = alloc(w.dinv, w.dim, err)) ||
         (rc = alloc(w.ticket, 1, err)))
       return rc;
     hipError_t e = hipMemsetAsync(w.ticket, 0, sizeof(unsigned), st);
@@ -442,15 +562,16 @@ int mixed_apply64(MixedWork& w, double shift, const double* omean64, const doubl
   if (rc) return rc;
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, omean64, w.om32, w.dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, v, w.r, w.dim);
-  double* s = w.scal + 4 * (w.scal_cap - 1);  // scratch slot
-  apply_launches(w, shift, w.r, nullptr, nullptr, s + 0, s + 1, st);
+  double* s = w.scal + kSlot * (w.scal_cap - 1);  // scratch slot
+  apply_launches(w, shift, w.r, nullptr, nullptr, nullptr, s, st);
   hipLaunchKernelGGL(k_f2d, dim3(nb(w.dim)), dim3(256), 0, st, w.Ap, out, w.dim);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_apply", e);
 }
 
-int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b, double* x,
-             double tol, int max_iter, int* iters, hipStream_t st, std::string& err) {
+int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* diag64,
+             const double* b, double* x, double tol, int max_iter, int* iters, hipStream_t st,
+             std::string& err) {
   const size_t dim = w.dim;
   int rc = ensure_scal(w, max_iter, err);
   if (rc) return rc;
@@ -458,11 +579,17 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b,
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, b, w.r, dim);
   hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.x, dim);
   hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.p, dim);
-  double* sc = w.scal;  // slot t: [0] r.r at the start of iteration t, [1] p.Ap, [2] <omean, p>
-  hipLaunchKernelGGL(k_dot32, dim3(nb(dim)), dim3(256), 0, st, w.r, w.r, dim, w.dpart, w.ticket,
-                     sc);
+  const float* dinv = nullptr;
+  if (diag64) {
+    hipLaunchKernelGGL(k_jacobi_inv, dim3(nb(dim)), dim3(256), 0, st, diag64, shift, w.dinv, dim);
+    dinv = w.dinv;
+  }
+  // slot t: [0] rho = <r, z>, [1] <p, Ap>, [2] <omean, p>, [3] sum y, [4] <r, r>
+  double* sc = w.scal;
+  hipLaunchKernelGGL(k_dot_rz, dim3(nb(dim)), dim3(256), 0, st, w.r, dinv, dim, w.dpart,
+                     w.ticket, sc);
   double bn2 = 0.0;
-  hipError_t e = hipMemcpyAsync(&bn2, sc, sizeof(double), hipMemcpyDeviceToHost, st);
+  hipError_t e = hipMemcpyAsync(&bn2, sc + 4, sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   *iters = 0;
@@ -475,14 +602,14 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b,
   int it = 0;
   bool done = false;
   while (!done && it < max_iter) {
-    double* s = sc + 4 * it;
-    apply_launches(w, shift, w.r, s, it > 0 ? s - 4 : nullptr, s + 2, s + 1, st);
-    hipLaunchKernelGGL(k_cg_xr, dim3(nb(dim)), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dim, s,
-                       w.dpart, w.ticket, s + 4);
+    double* s = sc + kSlot * it;
+    apply_launches(w, shift, w.r, dinv, s, it > 0 ? s - kSlot : nullptr, s, st);
+    hipLaunchKernelGGL(k_cg_xr, dim3(nb(dim)), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dinv, dim,
+                       s, w.dpart, w.ticket, s + kSlot);
     ++it;
     if (it % check == 0 || it == max_iter) {
       double rn2 = 0.0;
-      e = hipMemcpyAsync(&rn2, sc + 4 * it, sizeof(double), hipMemcpyDeviceToHost, st);
+      e = hipMemcpyAsync(&rn2, sc + kSlot * it + 4, sizeof(double), hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
       if (!(rn2 >= thr)) done = true;  // converged (or NaN: the FP64 check decides)
@@ -498,7 +625,7 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* b,
 void mixed_release(MixedWork& w) {
   for (void* p : {(void*)w.sigA, (void*)w.sigT, (void*)w.Trm, (void*)w.Tt, (void*)w.vsplit,
                   (void*)w.ypart, (void*)w.y, (void*)w.part, (void*)w.om32, (void*)w.x,
-                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.dpart, (void*)w.ticket,
+                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.dpart, (void*)w.ticket, (void*)w.dinv,
                   (void*)w.scal})
     if (p) (void)hipFree(p);
   w = MixedWork();

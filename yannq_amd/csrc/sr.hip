@@ -225,6 +225,48 @@ __global__ void k_reduce_aug(const double* __restrict__ part, int chunks, int n,
 inline int kp16(int n) { return (n + 1 + 15) / 16 * 16; }
 inline int np16(int m) { return (m + 1 + 15) / 16 * 16; }
 
+// partial[k][j] = sum over the samples of chunk k of T[s][j]^2
+__global__ void k_colsum_sq(const double* __restrict__ T, int ns, int m,
+                            double* __restrict__ partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int k = blockIdx.y;
+  const int s0 = k * kChunk, s1 = min(ns, s0 + kChunk);
+  double acc = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    const double t = T[(size_t)s * m + j];
+    acc += t * t;
+  }
+  partial[(size_t)k * m + j] = acc;
+}
+
+// diag(S) of the factored O: Var(O_c) with O_W(i,j)^2 = T_j^2 and O_a^2 = 1
+__global__ void k_diag_S(const double* __restrict__ meanT2, const double* __restrict__ omean,
+                         int n, int m, int use_bias, double* __restrict__ diag) {
+  const size_t nm = (size_t)n * m;
+  const size_t dim = use_bias ? nm + n + m : nm;
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= dim) return;
+  const double mu = omean[c];
+  double e2;
+  if (c < nm) e2 = meanT2[c % m];
+  else if (c < nm + n) e2 = 1.0;
+  else e2 = meanT2[c - nm - n];
+  diag[c] = e2 - mu * mu;
+}
+
+__global__ void k_jacobi_inv64(const double* __restrict__ diag, double shift,
+                               double* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 1.0 / (diag[i] + shift);
+}
+
+__global__ void k_mul(const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] * b[i];
+}
+
 __global__ void k_fill(double* x, double v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;
@@ -374,6 +416,27 @@ void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
   double* ones = partial + factored_partial_doubles(ns, n, m) - ns;
   hipLaunchKernelGGL(k_fill, dim3(nblk(ns)), dim3(256), 0, st, ones, 1.0 / ns, ns);
   launch_factored_bwd64(sigma, T, ns, n, m, use_bias, ones, omean, partial, st);
+}
+
+void launch_diag_S(const double* T, int ns, int n, int m, int use_bias, const double* omean,
+                   double* diag, double* partial, hipStream_t st) {
+  const int chunks = colsum_chunks(ns);
+  hipLaunchKernelGGL(k_colsum_sq, dim3(nblk(m), chunks), dim3(256), 0, st, T, ns, m, partial);
+  double* meanT2 = partial + (size_t)chunks * m;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(m)), dim3(256), 0, st, partial, chunks, m,
+                     1.0 / ns, meanT2);
+  const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
+  hipLaunchKernelGGL(k_diag_S, dim3(nblk(dim)), dim3(256), 0, st, meanT2, omean, n, m, use_bias,
+                     diag);
+}
+
+void launch_jacobi_inv64(const double* diag, double shift, double* out, size_t n,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_jacobi_inv64, dim3(nblk(n)), dim3(256), 0, st, diag, shift, out, n);
+}
+
+void launch_mul(const double* a, const double* b, double* out, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_mul, dim3(nblk(n)), dim3(256), 0, st, a, b, out, n);
 }
 
 int dot_blocks(size_t n) {

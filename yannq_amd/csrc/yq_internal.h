@@ -108,6 +108,15 @@ void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
                             int n, int m, int use_bias, double* omean,
                             double* partial, hipStream_t st);
 
+// diag(S) = Var(O_c) per parameter from T = tanh(theta) and the column mean;
+// partial needs (colsum_chunks(ns) + 1) * m doubles
+void launch_diag_S(const double* T, int ns, int n, int m, int use_bias, const double* omean,
+                   double* diag, double* partial, hipStream_t st);
+void launch_jacobi_inv64(const double* diag, double shift, double* out, size_t n,
+                         hipStream_t st);  // out = 1 / (diag + shift)
+void launch_mul(const double* a, const double* b, double* out, size_t n,
+                hipStream_t st);  // out = a * b
+
 // ------------------------------------------------------------------ FeedForward
 struct FFArgs {
   int n, C, K, H, ns;
