@@ -17,6 +17,10 @@
  *     ordered on the context's HIP stream and calls return after it finished.
  *   - Real parameters are doubles in the layout [W (m x n, column-major) |
  *     a (n) | b (m)] of Machines/RBM.hpp:271-281 (W block only without bias).
+ *   - A complex machine (YQ_DTYPE_C128) uses the same layout with complex
+ *     entries stored as interleaved (re, im) doubles, as std::complex<double>
+ *     arrays are; so are its theta, local energies, O, S, F, column mean and
+ *     the vectors of yq_sr_apply / yq_sr_solve (2 doubles per entry).
  *   - Spins are int8 (+1 / -1), one row of n per configuration.
  */
 #ifndef YANNQ_AMD_H
@@ -45,6 +49,8 @@ typedef enum {
 } yq_status;
 
 typedef enum { YQ_TFI = 0, YQ_XXZ = 1, YQ_J1J2 = 2 } yq_ham_kind;
+/* machine scalar type: RBM<double> or RBM<std::complex<double>> */
+typedef enum { YQ_DTYPE_F64 = 1, YQ_DTYPE_C128 = 2 } yq_dtype;
 typedef enum { YQ_LOCAL = 0, YQ_EXCHANGE = 1 } yq_sweeper_kind;
 /* YQ_FP64: every operation in double.  YQ_MIXED: sampling and local energies
  * in float with theta refreshed from double once per sweep; the CG products
@@ -87,8 +93,10 @@ YQ_API int yq_ctx_destroy(yq_ctx* ctx);
 YQ_API const char* yq_last_error(const yq_ctx* ctx);
 
 /* ---- Machine: Machines/RBM.hpp ---------------------------------------- */
-/* RBM(n, m, useBias), all parameters zero: RBM.hpp:50-56.  Only real double
- * machines are supported by this build (dtype must be 1). */
+/* RBM(n, m, useBias), all parameters zero: RBM.hpp:50-56.  dtype is a
+ * yq_dtype.  Complex machines run the explicit and dense CG solvers in FP64
+ * (with YQ_MIXED only sampling and E_loc run in float); the factored solver
+ * is real-only. */
 YQ_API int yq_rbm_create(yq_ctx* ctx, int dtype, int n, int m, int use_bias);
 /* getN / getM / getDim: RBM.hpp:115-130 */
 YQ_API int yq_rbm_dims(const yq_ctx* ctx, int* n, int* m, int* dim);
@@ -97,9 +105,10 @@ YQ_API int yq_rbm_set_params(yq_ctx* ctx, const double* params);
 YQ_API int yq_rbm_get_params(yq_ctx* ctx, double* params);
 /* updateParams (params += delta): RBM.hpp:261-269 */
 YQ_API int yq_rbm_update_params(yq_ctx* ctx, const double* delta);
-/* initializeRandom(re, sigma): RBM.hpp:301-322, normals drawn in the order
- * a, b, W from a Philox stream keyed by seed (values differ from the
- * reference's std::normal_distribution by design). */
+/* initializeRandom(re, sigma): RBM.hpp:301-322 (complex: :327-349, re and
+ * im of each entry from consecutive draws), normals drawn in the order a, b,
+ * W from a Philox stream keyed by seed (values differ from the reference's
+ * std::normal_distribution by design). */
 YQ_API int yq_rbm_init_random(yq_ctx* ctx, uint64_t seed, double sigma);
 /* hasNaN: RBM.hpp:293-296 */
 YQ_API int yq_rbm_has_nan(yq_ctx* ctx, int* out);

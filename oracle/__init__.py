@@ -80,6 +80,21 @@ def lib():
             "bl_eloc": (None, [p, i, i, i, p, p, p, p, p, i]),
             "bl_sr_build": (None, [i, i, i, i, p, p, p, p, p, p, i]),
             "bl_cg_solve": (i, [i, i, p, d, p, d, i, p, p, p, i]),
+            "or_c_logcosh": (None, [d, d, p]),
+            "or_c_theta": (None, [i, i, p, p, p, p]),
+            "or_c_log_ratio1": (None, [i, i, p, p, p, p, i, p]),
+            "or_c_log_ratio2": (None, [i, i, p, p, p, p, i, i, p]),
+            "or_c_log_coeff": (None, [i, i, p, p, p, p]),
+            "or_c_logderiv": (None, [i, i, i, p, p, p]),
+            "or_c_eloc": (None, [p, i, i, p, p, p, p, p]),
+            "or_c_sr_build": (None, [i, i, i, i, p, p, p, p, p, p, p, p]),
+            "rp_c_sample": (i, [p, i, p, p, p, p, p, p, p, i, i, p, p]),
+            "rp_c_eloc": (None, [p, i, i, i, i, p, p, p, p, p]),
+            "rp_logcosh_re_d": (d, [d, d]),
+            "rp_logcosh_re_f": (C.c_float, [C.c_float, C.c_float]),
+            "rp_logcosh_c_d": (None, [d, d, p]),
+            "rp_logcosh_c_f": (None, [C.c_float, C.c_float, p]),
+            "rp_tanh_c_d": (None, [d, d, p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -328,3 +343,144 @@ def ff_jacobian(desc, params, sigma):
     for s in range(sigma.shape[0]):
         out[s] = L.or_ff_jacobian(C.byref(desc), _p(params), _p(sigma[s]), _p(J[s]))
     return J, out
+
+
+# ---------------------------------------------------------------- complex RBM
+def c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class CRBM:
+    """Complex RBM (RBM<std::complex<double>>) in the [W col-major | a | b]
+    layout; arrays are complex128 (interleaved re, im in memory)."""
+
+    def __init__(self, n, m, W=None, a=None, b=None, use_bias=True):
+        self.n, self.m, self.use_bias = n, m, use_bias
+        self.W = c128(np.zeros((n, m)) if W is None else W)
+        self.a = c128(np.zeros(n) if a is None else a)
+        self.b = c128(np.zeros(m) if b is None else b)
+
+    @property
+    def dim(self):
+        return self.n * self.m + (self.n + self.m if self.use_bias else 0)
+
+    @classmethod
+    def random(cls, n, m, scale, rng, use_bias=True):
+        def cn(*shape):
+            return rng.normal(0, scale, shape) + 1j * rng.normal(0, scale, shape)
+        a = cn(n) if use_bias else np.zeros(n)
+        b = cn(m) if use_bias else np.zeros(m)
+        return cls(n, m, cn(n, m), a, b, use_bias)
+
+    def params(self):
+        v = [self.W.reshape(-1)]
+        if self.use_bias:
+            v += [self.a, self.b]
+        return np.concatenate(v)
+
+    def set_params(self, v):
+        v = c128(v)
+        nm = self.n * self.m
+        self.W = c128(v[:nm].reshape(self.n, self.m))
+        if self.use_bias:
+            self.a = c128(v[nm:nm + self.n])
+            self.b = c128(v[nm + self.n:])
+
+
+def c_logcosh(z):
+    out = np.empty(2)
+    lib().or_c_logcosh(float(np.real(z)), float(np.imag(z)), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def c_theta(rbm, sigma):
+    sigma = i8(np.atleast_2d(sigma))
+    out = np.empty((sigma.shape[0], rbm.m), np.complex128)
+    L = lib()
+    for s in range(sigma.shape[0]):
+        L.or_c_theta(rbm.n, rbm.m, _p(rbm.W), _p(rbm.b), _p(sigma[s]), _p(out[s]))
+    return out
+
+
+def c_log_ratio1(rbm, sigma, th, k):
+    out = np.empty(2)
+    lib().or_c_log_ratio1(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(i8(sigma)), _p(c128(th)), k,
+                          _p(out))
+    return out[0] + 1j * out[1]
+
+
+def c_log_ratio2(rbm, sigma, th, k, l):
+    out = np.empty(2)
+    lib().or_c_log_ratio2(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(i8(sigma)), _p(c128(th)), k, l,
+                          _p(out))
+    return out[0] + 1j * out[1]
+
+
+def c_log_coeff(rbm, sigma, th=None):
+    sigma = i8(sigma)
+    th = c_theta(rbm, sigma)[0] if th is None else c128(th)
+    out = np.empty(2)
+    lib().or_c_log_coeff(rbm.n, rbm.m, _p(rbm.a), _p(sigma), _p(th), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def c_logderiv(rbm, sigma, th):
+    out = np.empty(rbm.dim, np.complex128)
+    lib().or_c_logderiv(rbm.n, rbm.m, int(rbm.use_bias), _p(i8(sigma)), _p(c128(th)), _p(out))
+    return out
+
+
+def c_eloc(H, rbm, sigma, th):
+    sigma, th = i8(np.atleast_2d(sigma)), c128(np.atleast_2d(th))
+    L = lib()
+    out = np.empty(2)
+    res = np.empty(sigma.shape[0], np.complex128)
+    for s in range(sigma.shape[0]):
+        L.or_c_eloc(C.byref(H), rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(sigma[s]), _p(th[s]),
+                    _p(out))
+        res[s] = out[0] + 1j * out[1]
+    return res
+
+
+def c_eloc_replay(H, rbm, sigma, th, precision=FP64):
+    """Complex E_loc with the kernels' arithmetic (theta rounded to float
+    for precision MIXED)."""
+    sigma, th = i8(np.atleast_2d(sigma)), c128(np.atleast_2d(th))
+    out = np.empty(sigma.shape[0], np.complex128)
+    lib().rp_c_eloc(C.byref(H), precision, sigma.shape[0], rbm.n, rbm.m, _p(rbm.W), _p(rbm.a),
+                    _p(sigma), _p(th), _p(out))
+    return out
+
+
+def c_sr_build(rbm, sigma, th, el, weights=None, with_S=True):
+    sigma, th, el = i8(sigma), c128(th), c128(el)
+    ns = sigma.shape[0]
+    O = np.empty((ns, rbm.dim), np.complex128)
+    om = np.empty(rbm.dim, np.complex128)
+    F = np.empty(rbm.dim, np.complex128)
+    S = np.empty((rbm.dim, rbm.dim), np.complex128) if with_S else None
+    w = f64(weights) if weights is not None else None
+    lib().or_c_sr_build(ns, rbm.n, rbm.m, int(rbm.use_bias), _p(sigma), _p(th), _p(el), _p(w),
+                        _p(O), _p(om), _p(F), _p(S))
+    return O, om, F, S
+
+
+def c_sample_replay(rbm, sigma, *, seed, n_sweeps, n_discard, kind=LOCAL, precision=FP64,
+                    lanes=64, sweep_factor=1, beta=1.0, streams=None, ctr=None, acc=None):
+    """Replay of the complex sampler kernel; theta samples are complex128."""
+    sigma = i8(sigma).copy()
+    Cn = sigma.shape[0]
+    streams = np.arange(Cn, dtype=np.uint32) if streams is None else \
+        np.ascontiguousarray(streams, dtype=np.uint32)
+    ctr = np.zeros(Cn, np.uint64) if ctr is None else np.array(ctr, dtype=np.uint64)
+    acc = np.zeros(Cn, np.uint64) if acc is None else np.array(acc, dtype=np.uint64)
+    cfg = RpCfg(rbm.n, rbm.m, int(rbm.use_bias), kind, precision, lanes, sweep_factor,
+                beta, seed)
+    out_s = np.empty((n_sweeps * Cn, rbm.n), dtype=np.int8)
+    out_t = np.empty((n_sweeps * Cn, rbm.m), np.complex128)
+    rc = lib().rp_c_sample(C.byref(cfg), Cn, _p(streams), _p(rbm.W), _p(rbm.a), _p(rbm.b),
+                           _p(sigma), _p(ctr), _p(acc), n_sweeps, n_discard, _p(out_s),
+                           _p(out_t))
+    if rc:
+        raise ValueError("bad replay configuration")
+    return sigma, ctr, acc, out_s, out_t

@@ -61,6 +61,27 @@ int or_cg_solve(int ns, int dim, const double* Oc, const double* weights,
 void or_sgd_update(int dim, double alpha, double p, int t, const double* v,
                    double* out);
 
+// ---- complex RBM (std::complex<double>, interleaved re/im arrays) ----------
+// Parameters in the RBM layout [W col-major | a | b] as complex numbers.
+void or_c_logcosh(double re, double im, double* out2);
+void or_c_theta(int n, int m, const double* W, const double* b, const int8_t* sigma,
+                double* theta);
+void or_c_log_ratio1(int n, int m, const double* W, const double* a, const int8_t* sigma,
+                     const double* theta, int k, double* out2);
+void or_c_log_ratio2(int n, int m, const double* W, const double* a, const int8_t* sigma,
+                     const double* theta, int k, int l, double* out2);
+void or_c_log_coeff(int n, int m, const double* a, const int8_t* sigma, const double* theta,
+                    double* out2);
+void or_c_logderiv(int n, int m, int use_bias, const int8_t* sigma, const double* theta,
+                   double* out);
+void or_c_eloc(const or_ham* H, int n, int m, const double* W, const double* a,
+               const int8_t* sigma, const double* theta, double* out2);
+// O (ns x dim complex), centred in place; omean; F = O_c^H W E; S = O_c^H W
+// O_c (if S); W = diag(weights) (null -> 1/ns, SRMat::constructFromSamples).
+void or_c_sr_build(int ns, int n, int m, int use_bias, const int8_t* sigma, const double* theta,
+                   const double* eloc, const double* weights, double* O, double* omean,
+                   double* F, double* S);
+
 // ---- FeedForward: Conv1D(1->C, K, periodic) -> LnCosh -> FC(C*n -> H) ->
 // Tanh -> Mean, no biases (the layer defaults).  Parameters [conv kernel
 // (K*C, index ki + oc*K) | FC weight (C*n x H column-major)].
@@ -106,6 +127,23 @@ void rp_eloc(const or_ham* H, int precision, int ns, int n, int m,
              const double* theta, double* eloc);
 void rp_randomize(uint64_t seed, uint64_t epoch, int C, const uint32_t* stream,
                   int n, int nup, int8_t* sigma);
+// Replay of the complex sampler: decisions on Re(log ratio) with the
+// replayable Re logcosh; W, a, b, out_theta complex interleaved (re, im).
+int rp_c_sample(const rp_cfg* cfg, int C, const uint32_t* stream, const double* W,
+                const double* a, const double* b, int8_t* sigma, uint64_t* ctr, uint64_t* acc,
+                int n_sweeps, int n_discard, int8_t* out_sigma, double* out_theta);
+// Complex local energies with the kernels' arithmetic (precision 0: fp64, 1:
+// fp32 theta and log-ratios); eloc complex interleaved.  The phase uses the
+// host atan2, so the device agrees to a few ulp, not bit for bit.
+void rp_c_eloc(const or_ham* H, int precision, int ns, int n, int m, const double* W,
+               const double* a, const int8_t* sigma, const double* theta, double* eloc);
+// The kernels' complex functions: Re log cosh (replayed bit for bit), the
+// full log cosh (out2 = re, im) and tanh of a complex argument.
+double rp_logcosh_re_d(double x, double y);
+float rp_logcosh_re_f(float x, float y);
+void rp_logcosh_c_d(double x, double y, double* out2);
+void rp_logcosh_c_f(float x, float y, float* out2);
+void rp_tanh_c_d(double x, double y, double* out2);
 
 // ---- multi-threaded CPU baseline --------------------------------------------
 int bl_threads(void);

@@ -24,6 +24,7 @@ LOCAL, EXCHANGE = 0, 1
 FP64, MIXED = 0, 1
 SOLVE_EXPLICIT, SOLVE_CG, SOLVE_CG_FACTORED = 0, 1, 2
 SR_F, SR_OMEAN, SR_O, SR_S, SR_ELOC = range(5)
+DTYPE_F64, DTYPE_C128 = 1, 2
 
 STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_STATE", 3: "ERR_HIP", 4: "ERR_NAN",
                 5: "ERR_NOT_SPD", 6: "ERR_NOT_CONVERGED", 7: "ERR_OOM", 8: "ERR_UNSUPPORTED"}
@@ -149,23 +150,37 @@ class Context:
         return rc
 
     # machine
-    def rbm_create(self, n, m, use_bias=True):
-        self._chk(self._L.yq_rbm_create(self._h, 1, n, m, int(use_bias)))
+    def rbm_create(self, n, m, use_bias=True, dtype=np.float64):
+        """dtype float64 (RBM<double>) or complex128 (RBM<complex<double>>);
+        a complex machine takes and returns complex128 arrays."""
+        self.dt = np.dtype(dtype)
+        code = DTYPE_C128 if self.dt == np.complex128 else DTYPE_F64
+        self._chk(self._L.yq_rbm_create(self._h, code, n, m, int(use_bias)))
         self.n, self.m = n, m
         self.dim = n * m + (n + m if use_bias else 0)
 
+    @property
+    def cplx(self):
+        return getattr(self, "dt", np.dtype(np.float64)) == np.complex128
+
+    def _vec(self, v):
+        return np.ascontiguousarray(v, dtype=np.complex128 if self.cplx else np.float64)
+
+    def _out(self, *shape):
+        return np.empty(shape, np.complex128 if self.cplx else np.float64)
+
     def set_params(self, v):
-        v = np.ascontiguousarray(v, dtype=np.float64)
+        v = self._vec(v)
         assert v.shape == (self.dim,)
         self._chk(self._L.yq_rbm_set_params(self._h, _ptr(v)))
 
     def get_params(self):
-        out = np.empty(self.dim)
+        out = self._out(self.dim)
         self._chk(self._L.yq_rbm_get_params(self._h, _ptr(out)))
         return out
 
     def update_params(self, d):
-        d = np.ascontiguousarray(d, dtype=np.float64)
+        d = self._vec(d)
         self._chk(self._L.yq_rbm_update_params(self._h, _ptr(d)))
 
     def init_random(self, seed, sigma):
@@ -209,7 +224,7 @@ class Context:
 
     def samples(self):
         s = np.empty((self.ns, self.n), np.int8)
-        t = np.empty((self.ns, self.m))
+        t = self._out(self.ns, self.m)
         self._chk(self._L.yq_samples_get(self._h, _ptr(s), _ptr(t)))
         return s, t
 
@@ -219,7 +234,7 @@ class Context:
         self.ns = sigma.shape[0]
 
     def eloc(self):
-        e = np.empty(self.ns)
+        e = self._out(self.ns)
         mean, var = C.c_double(), C.c_double()
         self._chk(self._L.yq_eloc(self._h, _ptr(e), C.byref(mean), C.byref(var)))
         return e, mean.value, var.value
@@ -231,7 +246,7 @@ class Context:
     def sr_get(self, item):
         size = {SR_F: self.dim, SR_OMEAN: self.dim, SR_O: self.ns * self.dim,
                 SR_S: self.dim * self.dim, SR_ELOC: self.ns}[item]
-        out = np.empty(size)
+        out = self._out(size)
         self._chk(self._L.yq_sr_get(self._h, item, _ptr(out)))
         if item == SR_O:
             return out.reshape(self.ns, self.dim)
@@ -240,15 +255,15 @@ class Context:
         return out
 
     def sr_apply(self, shift, v):
-        v = np.ascontiguousarray(v, dtype=np.float64)
-        out = np.empty(self.dim)
+        v = self._vec(v)
+        out = self._out(self.dim)
         self._chk(self._L.yq_sr_apply(self._h, shift, _ptr(v), _ptr(out)))
         return out
 
     def sr_solve(self, shift, tol=1e-10, max_iter=10000, rhs=None, allow_unconverged=False):
-        x = np.empty(self.dim)
+        x = self._out(self.dim)
         res, it = C.c_double(), C.c_int()
-        r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
+        r = None if rhs is None else self._vec(rhs)
         self._chk(self._L.yq_sr_solve(self._h, shift, tol, max_iter, _ptr(r), _ptr(x),
                                       C.byref(res), C.byref(it)),
                   allow=(ERR_NOT_CONVERGED,) if allow_unconverged else ())

@@ -77,11 +77,13 @@ struct yq_ctx {
   rocblas_handle blas = nullptr;
   std::vector<Buf*> bufs;
 
-  // machine
+  // machine (cplx: complex double parameters, interleaved re / im)
   bool have_rbm = false;
-  int n = 0, m = 0, use_bias = 1;
+  int n = 0, m = 0, use_bias = 1, cplx = 0;
   size_t dim = 0, nparam_alloc = 0;
   Buf params, WT64, W32, WT32, a32;
+  Buf cpd, cpf;  // complex parameter planes (double, float)
+  CplxPlanes planes{};
 
   // hamiltonian
   bool have_ham = false;
@@ -107,8 +109,9 @@ struct yq_ctx {
   yq::MixedWork mixed;
   // solver options: Jacobi preconditioning, warm start from the last solution
   int precond = 0, warm_start = 0;
-  Buf diagS, dinv64, vz, xprev;
+  Buf diagS, dinv64, vz, xprev, ones;
   bool have_xprev = false;
+  int ones_len = 0;
 
   // FeedForward machine
   bool have_ff = false;
@@ -122,7 +125,7 @@ struct yq_ctx {
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp,
                    &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
-                   &xprev})
+                   &xprev, &ones, &cpd, &cpf})
       bufs.push_back(b);
   }
 };
@@ -187,13 +190,23 @@ int sync(yq_ctx* c) {
   return YQ_OK;
 }
 
+// doubles per parameter (2 for a complex machine) and vector lengths in doubles
+size_t ncomp(const yq_ctx* c) { return c->cplx ? 2 : 1; }
+size_t vlen(const yq_ctx* c) { return c->dim * ncomp(c); }
 double* W64(yq_ctx* c) { return c->params.as<double>(); }
-double* a64(yq_ctx* c) { return c->params.as<double>() + (size_t)c->n * c->m; }
-double* b64(yq_ctx* c) { return c->params.as<double>() + (size_t)c->n * c->m + c->n; }
+double* a64(yq_ctx* c) { return c->params.as<double>() + ncomp(c) * c->n * c->m; }
+double* b64(yq_ctx* c) { return c->params.as<double>() + ncomp(c) * ((size_t)c->n * c->m + c->n); }
 
 // working copies derived from the fp64 master parameters
 int refresh_params(yq_ctx* c) {
   const int n = c->n, m = c->m;
+  if (c->cplx) {
+    launch_cplx_planes(W64(c), n, m, c->planes, c->st);
+    LAUNCHCHK(c);
+    c->have_eloc = false;
+    c->have_sr = false;
+    return YQ_OK;
+  }
   launch_transpose(W64(c), m, n, c->WT64.as<double>(), c->st);
   launch_to_float(W64(c), c->W32.as<float>(), (size_t)n * m, c->st);
   launch_transpose_f(W64(c), m, n, c->WT32.as<float>(), c->st);
@@ -231,10 +244,10 @@ int compute_eloc(yq_ctx* c) {
   if (!c->have_ham) return fail(c, YQ_ERR_STATE, "no Hamiltonian: call yq_ham_set");
   if (c->ham.n != c->n) return fail(c, YQ_ERR_ARG, "Hamiltonian size differs from the machine's n");
   const int B = num_bonds(c->ham);
-  TRY(ensure(c, c->ratio, sizeof(double) * (size_t)c->ns * B));
-  TRY(ensure(c, c->eloc, sizeof(double) * (size_t)c->ns));
+  TRY(ensure(c, c->ratio, sizeof(double) * ncomp(c) * (size_t)c->ns * B));
+  TRY(ensure(c, c->eloc, sizeof(double) * ncomp(c) * (size_t)c->ns));
   TRY(ensure(c, c->estat, sizeof(double) * 2));
-  ElocArgs a;
+  ElocArgs a{};
   a.sigma = c->bsig.as<int8_t>();
   a.theta = c->btheta.p;
   a.use_float = c->bprec == YQ_MIXED;
@@ -247,12 +260,25 @@ int compute_eloc(yq_ctx* c) {
   a.m = c->m;
   a.B = B;
   a.ham = c->ham;
-  const size_t rs = a.use_float ? 4 : 8;
-  const size_t lds = (size_t)(256 / B + 2) * c->m * 2 * rs + (size_t)(256 / B + 2) * c->n;
-  if (lds > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
-  launch_eloc(a, c->st);
-  LAUNCHCHK(c);
-  launch_eloc_stats(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+  if (c->cplx) {
+    const CplxPlanes& P = c->planes;
+    a.WT = a.use_float ? (const void*)P.WTr32 : (const void*)P.WTr64;
+    a.WTi = a.use_float ? (const void*)P.WTi32 : (const void*)P.WTi64;
+    a.ar = a.use_float ? (const void*)P.ar32 : (const void*)P.ar64;
+    a.ai = a.use_float ? (const void*)P.ai32 : (const void*)P.ai64;
+    if (eloc_c_lds_bytes(a) > 64 * 1024)
+      return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
+    launch_eloc_c(a, c->st);
+    LAUNCHCHK(c);
+    launch_eloc_stats_c(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+  } else {
+    const size_t rs = a.use_float ? 4 : 8;
+    const size_t lds = (size_t)(256 / B + 2) * c->m * 2 * rs + (size_t)(256 / B + 2) * c->n;
+    if (lds > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
+    launch_eloc(a, c->st);
+    LAUNCHCHK(c);
+    launch_eloc_stats(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+  }
   LAUNCHCHK(c);
   double h[2];
   HIPCHK(c, hipMemcpyAsync(h, c->estat.p, sizeof(h), hipMemcpyDeviceToHost, c->st));
@@ -266,8 +292,65 @@ int compute_eloc(yq_ctx* c) {
 // ---------------------------------------------------------------- SR build
 size_t aug_partial_doubles(int ns, int n, int m) { return factored_partial_doubles(ns, n, m); }
 
+// Complex machine: dense conj(O) (see complex.hip), FP64 SR whatever the
+// batch precision; S only for the explicit solver.
+int sr_build_c(yq_ctx* c, int solver) {
+  const int ns = c->ns, n = c->n, m = c->m;
+  const size_t dim = c->dim;
+  if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG)
+    return fail(c, YQ_ERR_UNSUPPORTED,
+                "complex machines support the explicit and dense CG solvers");
+  TRY(ensure(c, c->omean, sizeof(double) * 2 * dim));
+  TRY(ensure(c, c->F, sizeof(double) * 2 * dim));
+  TRY(ensure(c, c->T64, sizeof(double) * 2 * (size_t)ns * m));
+  TRY(ensure(c, c->O, sizeof(double) * 2 * (size_t)ns * dim));
+  const int chunks = colsum_chunks(ns);
+  TRY(ensure(c, c->partial, sizeof(double) * 2 * (size_t)chunks * dim));
+  launch_tanh_c(c->btheta.p, c->bprec == YQ_MIXED, c->T64.as<double>(), (size_t)ns * m, c->st);
+  double* Oh = c->O.as<double>();
+  launch_build_Oh(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias, Oh, c->st);
+  // column means of conj(O) (complex columns as pairs of doubles)
+  launch_colsum(Oh, ns, (int)(2 * dim), c->partial.as<double>(), c->st);
+  launch_reduce_partials(c->partial.as<double>(), chunks, (int)(2 * dim), 1.0 / ns,
+                         c->omean.as<double>(), c->st);
+  // centre and F = conj(O_c)^T E / ns = O_c^H E / ns
+  launch_center_force_c(Oh, c->omean.as<double>(), c->eloc.as<double>(), ns, (int)dim,
+                        c->partial.as<double>(), c->st);
+  launch_reduce_partials(c->partial.as<double>(), chunks, (int)(2 * dim), 1.0 / ns,
+                         c->F.as<double>(), c->st);
+  LAUNCHCHK(c);
+  if (solver == YQ_SOLVE_EXPLICIT) {
+    TRY(blas(c));
+    TRY(ensure(c, c->S, sizeof(double) * 2 * dim * dim));
+    const double alpha = 1.0 / ns, beta = 0.0;
+    ProfGuard g("zherk", c->st);
+    BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                             (rocblas_int)dim, ns, &alpha,
+                             reinterpret_cast<const rocblas_double_complex*>(Oh),
+                             (rocblas_int)dim, &beta,
+                             reinterpret_cast<rocblas_double_complex*>(c->S.p),
+                             (rocblas_int)dim));
+  }
+  // diag(S) = mean |O_c|^2, duplicated per (re, im) for the Jacobi option
+  TRY(ensure(c, c->diagS, sizeof(double) * 2 * dim));
+  TRY(ensure(c, c->vtmp, sizeof(double) * dim));
+  launch_absq_colsum(Oh, ns, (int)dim, c->partial.as<double>(), c->st);
+  launch_reduce_partials(c->partial.as<double>(), chunks, (int)dim, 1.0 / ns,
+                         c->vtmp.as<double>(), c->st);
+  launch_dup2(c->vtmp.as<double>(), c->diagS.as<double>(), dim, c->st);
+  LAUNCHCHK(c);
+  TRY(sync(c));
+  c->have_sr = true;
+  c->sr_solver = solver;
+  c->sr_prec = YQ_FP64;
+  return YQ_OK;
+}
+
 int sr_build(yq_ctx* c, int solver, int precision) {
   if (!c->have_eloc) TRY(compute_eloc(c));
+  if (precision != YQ_FP64 && precision != YQ_MIXED)
+    return fail(c, YQ_ERR_ARG, "unknown precision");
+  if (c->cplx) return sr_build_c(c, solver);
   const int ns = c->ns, n = c->n, m = c->m;
   const size_t dim = c->dim;
   if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG &&
@@ -358,7 +441,23 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
   const size_t dim = c->dim;
   TRY(ensure(c, c->vy, sizeof(double) * ns));
   double* y = c->vy.as<double>();
-  if (c->sr_solver == YQ_SOLVE_CG_FACTORED) {
+  if (c->cplx) {
+    // y = A^H v / ns = O_c v / ns, out = A y + shift v = O_c^H y + shift v
+    TRY(ensure(c, c->vy, sizeof(double) * 2 * ns));
+    TRY(blas(c));
+    const rocblas_double_complex inv(1.0 / ns, 0.0), one(1.0, 0.0), zero(0.0, 0.0),
+        sh(shift, 0.0);
+    const auto* A = reinterpret_cast<const rocblas_double_complex*>(c->O.p);
+    auto* yc = reinterpret_cast<rocblas_double_complex*>(c->vy.p);
+    BLASCHK(c, rocblas_zgemv(c->blas, rocblas_operation_conjugate_transpose, (rocblas_int)dim,
+                             ns, &inv, A, (rocblas_int)dim,
+                             reinterpret_cast<const rocblas_double_complex*>(v), 1, &zero, yc,
+                             1));
+    HIPCHK(c, hipMemcpyAsync(out, v, sizeof(double) * 2 * dim, hipMemcpyDeviceToDevice, c->st));
+    BLASCHK(c, rocblas_zgemv(c->blas, rocblas_operation_none, (rocblas_int)dim, ns, &one, A,
+                             (rocblas_int)dim, yc, 1, &sh,
+                             reinterpret_cast<rocblas_double_complex*>(out), 1));
+  } else if (c->sr_solver == YQ_SOLVE_CG_FACTORED) {
     launch_factored_fwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
                           c->use_bias, v, y, c->partial.as<double>(), c->st);
     LAUNCHCHK(c);
@@ -369,15 +468,18 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
     launch_factored_bwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
                           c->use_bias, y, out, c->partial.as<double>(), c->st);
     LAUNCHCHK(c);
-    TRY(ensure(c, c->vb, sizeof(double) * ns));
-    double sy;
-    {
-      // sum of y via dot with a ones vector kept in vb
-      std::vector<double> ones(ns, 1.0);
-      HIPCHK(c, hipMemcpyAsync(c->vb.p, ones.data(), sizeof(double) * ns,
+    // sum of y as a dot with a ones vector (its own buffer: vb may hold the
+    // caller's right-hand side)
+    if (c->ones_len < ns) {
+      TRY(ensure(c, c->ones, sizeof(double) * ns));
+      std::vector<double> h(ns, 1.0);
+      HIPCHK(c, hipMemcpyAsync(c->ones.p, h.data(), sizeof(double) * ns,
                                hipMemcpyHostToDevice, c->st));
-      TRY(dot(c, y, c->vb.as<double>(), ns, &sy));
+      TRY(sync(c));
+      c->ones_len = ns;
     }
+    double sy;
+    TRY(dot(c, y, c->ones.as<double>(), ns, &sy));
     launch_axpy(-sy, c->omean.as<double>(), out, dim, c->st);
     launch_axpy(shift, v, out, dim, c->st);
     LAUNCHCHK(c);
@@ -400,7 +502,9 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
 // is applied (Eigen's IdentityPreconditioner when 0).
 int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int max_iter,
          int* iters, double* err, bool x0) {
-  const size_t dim = c->dim;
+  // complex vectors run as 2*dim doubles: every dot here is Re(a^H b), which
+  // is what CG on a Hermitian operator needs (p^H A p and r^H z are real)
+  const size_t dim = vlen(c);
   TRY(ensure(c, c->vr, sizeof(double) * dim));
   TRY(ensure(c, c->vp, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
@@ -467,10 +571,19 @@ int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int ma
 
 // |(S + shift) x - b| / |b| in FP64
 int residual64(yq_ctx* c, double shift, const double* x, const double* b, double* out) {
-  const size_t dim = c->dim;
+  const size_t dim = vlen(c);
   TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)c->ns * c->m)));
   double* t = c->vtmp.as<double>();
-  if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->cplx) {
+    TRY(blas(c));
+    HIPCHK(c, hipMemcpyAsync(t, x, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    const rocblas_double_complex one(1.0, 0.0), sh(shift, 0.0);
+    BLASCHK(c, rocblas_zhemv(c->blas, rocblas_fill_lower, (rocblas_int)c->dim, &one,
+                             reinterpret_cast<const rocblas_double_complex*>(c->S.p),
+                             (rocblas_int)c->dim,
+                             reinterpret_cast<const rocblas_double_complex*>(x), 1, &sh,
+                             reinterpret_cast<rocblas_double_complex*>(t), 1));
+  } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
     TRY(blas(c));
     HIPCHK(c, hipMemcpyAsync(t, x, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
     const double one = 1.0;
@@ -488,7 +601,34 @@ int residual64(yq_ctx* c, double shift, const double* x, const double* b, double
   return YQ_OK;
 }
 
+int solve_explicit_c(yq_ctx* c, double shift, const double* b, double* x) {
+  const size_t dim = c->dim;
+  TRY(blas(c));
+  TRY(ensure(c, c->M, sizeof(double) * 2 * dim * dim));
+  TRY(ensure(c, c->info, sizeof(rocblas_int)));
+  launch_copy_shift_c(c->S.as<double>(), c->M.as<double>(), (int)dim, shift, c->st);
+  LAUNCHCHK(c);
+  auto* M = reinterpret_cast<rocblas_double_complex*>(c->M.p);
+  {
+    ProfGuard g("zpotrf", c->st);
+    BLASCHK(c, rocsolver_zpotrf(c->blas, rocblas_fill_lower, (rocblas_int)dim, M,
+                                (rocblas_int)dim, c->info.as<rocblas_int>()));
+  }
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  if (info != 0)
+    return fail(c, YQ_ERR_NOT_SPD,
+                "Cholesky of S + shift I failed at pivot " + std::to_string(info));
+  HIPCHK(c, hipMemcpyAsync(x, b, sizeof(double) * 2 * dim, hipMemcpyDeviceToDevice, c->st));
+  BLASCHK(c, rocsolver_zpotrs(c->blas, rocblas_fill_lower, (rocblas_int)dim, 1, M,
+                              (rocblas_int)dim, reinterpret_cast<rocblas_double_complex*>(x),
+                              (rocblas_int)dim));
+  return YQ_OK;
+}
+
 int solve_explicit(yq_ctx* c, double shift, const double* b, double* x) {
+  if (c->cplx) return solve_explicit_c(c, shift, b, x);
   const size_t dim = c->dim;
   TRY(blas(c));
   TRY(ensure(c, c->M, sizeof(double) * dim * dim));
@@ -584,7 +724,7 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
     // warm start from the previous iterative solution (same dimension)
     const bool x0 = c->warm_start && c->have_xprev;
     if (x0)
-      HIPCHK(c, hipMemcpyAsync(x_dev, c->xprev.p, sizeof(double) * c->dim,
+      HIPCHK(c, hipMemcpyAsync(x_dev, c->xprev.p, sizeof(double) * vlen(c),
                                hipMemcpyDeviceToDevice, c->st));
     if (c->sr_prec == YQ_MIXED) {
       rc = solve_mixed(c, shift, b_dev, x_dev, tol, max_iter, &it, &err, x0);
@@ -596,8 +736,8 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
       TRY(residual64(c, shift, x_dev, b_dev, &e2));
       err = e2;
     }
-    TRY(ensure(c, c->xprev, sizeof(double) * c->dim));
-    HIPCHK(c, hipMemcpyAsync(c->xprev.p, x_dev, sizeof(double) * c->dim,
+    TRY(ensure(c, c->xprev, sizeof(double) * vlen(c)));
+    HIPCHK(c, hipMemcpyAsync(c->xprev.p, x_dev, sizeof(double) * vlen(c),
                              hipMemcpyDeviceToDevice, c->st));
     c->have_xprev = true;
   }
@@ -666,19 +806,31 @@ YQ_API const char* yq_last_error(const yq_ctx* c) { return c ? c->err.c_str() : 
 // ---------------------------------------------------------------- machine
 YQ_API int yq_rbm_create(yq_ctx* c, int dtype, int n, int m, int use_bias) {
   if (!c) return YQ_ERR_ARG;
-  if (dtype != 1) return fail(c, YQ_ERR_UNSUPPORTED, "only real double machines (dtype 1)");
+  if (dtype != YQ_DTYPE_F64 && dtype != YQ_DTYPE_C128)
+    return fail(c, YQ_ERR_UNSUPPORTED, "dtype must be 1 (double) or 2 (complex double)");
   if (n < 2 || n > kMaxSites || m < 1 || m > kMaxHidden)
     return fail(c, YQ_ERR_ARG, "need 2 <= n <= 128 and 1 <= m <= 512");
   c->n = n;
   c->m = m;
   c->use_bias = use_bias ? 1 : 0;
+  c->cplx = dtype == YQ_DTYPE_C128;
   c->dim = (size_t)n * m + (use_bias ? (size_t)n + m : 0);
-  const size_t np = (size_t)n * m + n + m;
+  const size_t np = ((size_t)n * m + n + m) * ncomp(c);
   TRY(ensure(c, c->params, sizeof(double) * np));
-  TRY(ensure(c, c->WT64, sizeof(double) * (size_t)n * m));
-  TRY(ensure(c, c->W32, sizeof(float) * (size_t)n * m));
-  TRY(ensure(c, c->WT32, sizeof(float) * (size_t)n * m));
-  TRY(ensure(c, c->a32, sizeof(float) * (size_t)n));
+  if (c->cplx) {
+    const size_t nm = (size_t)n * m;
+    TRY(ensure(c, c->cpd, sizeof(double) * (4 * nm + 2 * n)));
+    TRY(ensure(c, c->cpf, sizeof(float) * (4 * nm + 2 * n)));
+    double* d = c->cpd.as<double>();
+    float* f = c->cpf.as<float>();
+    c->planes = CplxPlanes{d, d + nm, d + 2 * nm, d + 3 * nm, d + 4 * nm, d + 4 * nm + n,
+                           f, f + nm, f + 2 * nm, f + 3 * nm, f + 4 * nm, f + 4 * nm + n};
+  } else {
+    TRY(ensure(c, c->WT64, sizeof(double) * (size_t)n * m));
+    TRY(ensure(c, c->W32, sizeof(float) * (size_t)n * m));
+    TRY(ensure(c, c->WT32, sizeof(float) * (size_t)n * m));
+    TRY(ensure(c, c->a32, sizeof(float) * (size_t)n));
+  }
   HIPCHK(c, hipMemsetAsync(c->params.p, 0, sizeof(double) * np, c->st));
   c->have_rbm = true;
   c->have_batch = false;
@@ -700,8 +852,8 @@ YQ_API int yq_rbm_set_params(yq_ctx* c, const double* p) {
   TRY(need_rbm(c));
   if (!p) return fail(c, YQ_ERR_ARG, "null params");
   const size_t nm = (size_t)c->n * c->m;
-  std::vector<double> h(nm + c->n + c->m, 0.0);
-  memcpy(h.data(), p, sizeof(double) * c->dim);
+  std::vector<double> h((nm + c->n + c->m) * ncomp(c), 0.0);
+  memcpy(h.data(), p, sizeof(double) * vlen(c));
   for (double v : h)
     if (!std::isfinite(v)) return fail(c, YQ_ERR_NAN, "non-finite parameter");
   HIPCHK(c, hipMemcpyAsync(c->params.p, h.data(), sizeof(double) * h.size(),
@@ -713,16 +865,16 @@ YQ_API int yq_rbm_set_params(yq_ctx* c, const double* p) {
 YQ_API int yq_rbm_get_params(yq_ctx* c, double* p) {
   TRY(need_rbm(c));
   if (!p) return fail(c, YQ_ERR_ARG, "null output");
-  HIPCHK(c, hipMemcpyAsync(p, c->params.p, sizeof(double) * c->dim, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(p, c->params.p, sizeof(double) * vlen(c), hipMemcpyDeviceToHost, c->st));
   return sync(c);
 }
 
 YQ_API int yq_rbm_update_params(yq_ctx* c, const double* delta) {
   TRY(need_rbm(c));
   if (!delta) return fail(c, YQ_ERR_ARG, "null delta");
-  TRY(ensure(c, c->vtmp, sizeof(double) * c->dim));
-  HIPCHK(c, hipMemcpyAsync(c->vtmp.p, delta, sizeof(double) * c->dim, hipMemcpyHostToDevice, c->st));
-  launch_axpy(1.0, c->vtmp.as<double>(), c->params.as<double>(), c->dim, c->st);
+  TRY(ensure(c, c->vtmp, sizeof(double) * vlen(c)));
+  HIPCHK(c, hipMemcpyAsync(c->vtmp.p, delta, sizeof(double) * vlen(c), hipMemcpyHostToDevice, c->st));
+  launch_axpy(1.0, c->vtmp.as<double>(), c->params.as<double>(), vlen(c), c->st);
   LAUNCHCHK(c);
   TRY(refresh_params(c));
   return sync(c);
@@ -732,14 +884,19 @@ YQ_API int yq_rbm_init_random(yq_ctx* c, uint64_t seed, double sigma) {
   TRY(need_rbm(c));
   const int n = c->n, m = c->m;
   const size_t nm = (size_t)n * m;
-  std::vector<double> h(nm + n + m, 0.0);
-  // draw order of RBM::initializeRandom: a, b, then W column by column
+  const size_t nc = ncomp(c);
+  std::vector<double> h((nm + n + m) * nc, 0.0);
+  // draw order of RBM::initializeRandom: a, b, then W column by column; a
+  // complex entry takes (re, im) from two consecutive draws (RBM.hpp:327-349)
   std::vector<double*> order;
+  auto take = [&](size_t idx) {
+    for (size_t k = 0; k < nc; ++k) order.push_back(&h[idx * nc + k]);
+  };
   if (c->use_bias) {
-    for (int i = 0; i < n; ++i) order.push_back(&h[nm + i]);
-    for (int j = 0; j < m; ++j) order.push_back(&h[nm + n + j]);
+    for (int i = 0; i < n; ++i) take(nm + i);
+    for (int j = 0; j < m; ++j) take(nm + n + j);
   }
-  for (size_t q = 0; q < nm; ++q) order.push_back(&h[q]);
+  for (size_t q = 0; q < nm; ++q) take(q);
   for (size_t q = 0; q < order.size(); ++q) {
     const u32x4 x = draw(seed, q / 2, 0u, kDomParams);
     const double u1 = 1.0 - u01_double(x.x, x.y);
@@ -757,7 +914,7 @@ YQ_API int yq_rbm_init_random(yq_ctx* c, uint64_t seed, double sigma) {
 YQ_API int yq_rbm_has_nan(yq_ctx* c, int* out) {
   TRY(need_rbm(c));
   if (!out) return fail(c, YQ_ERR_ARG, "null output");
-  return has_nan(c, c->params.as<double>(), c->dim, out);
+  return has_nan(c, c->params.as<double>(), vlen(c), out);
 }
 
 // ---------------------------------------------------------------- Hamiltonian
@@ -862,12 +1019,19 @@ YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
   c->bprec = c->sprec;
   const size_t rs = c->sprec == YQ_MIXED ? sizeof(float) : sizeof(double);
   TRY(ensure(c, c->bsig, (size_t)ns * c->n));
-  TRY(ensure(c, c->btheta, rs * (size_t)ns * c->m));
-  SampleArgs a;
+  TRY(ensure(c, c->btheta, rs * ncomp(c) * (size_t)ns * c->m));
+  SampleArgs a{};
   a.W64 = W64(c);
   a.b64 = b64(c);
-  a.Wr = c->sprec == YQ_MIXED ? (const void*)c->W32.p : (const void*)W64(c);
-  a.ar = c->sprec == YQ_MIXED ? (const void*)c->a32.p : (const void*)a64(c);
+  const bool f32 = c->sprec == YQ_MIXED;
+  if (c->cplx) {
+    a.Wr = f32 ? (const void*)c->planes.Wr32 : (const void*)c->planes.Wr64;
+    a.Wi = f32 ? (const void*)c->planes.Wi32 : (const void*)c->planes.Wi64;
+    a.ar = f32 ? (const void*)c->planes.ar32 : (const void*)c->planes.ar64;
+  } else {
+    a.Wr = f32 ? (const void*)c->W32.p : (const void*)W64(c);
+    a.ar = f32 ? (const void*)c->a32.p : (const void*)a64(c);
+  }
   a.sigma = c->sig_state.as<int8_t>();
   a.ctr = c->ctr.as<uint64_t>();
   a.acc = c->acc.as<uint64_t>();
@@ -884,7 +1048,8 @@ YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
   a.mixed = c->sprec == YQ_MIXED;
   a.beta = 1.0;
   a.seed = c->seed;
-  launch_sample(a, c->st);
+  if (c->cplx) launch_sample_c(a, c->st);
+  else launch_sample(a, c->st);
   LAUNCHCHK(c);
   TRY(sync(c));
   c->have_batch = true;
@@ -897,7 +1062,7 @@ YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
 YQ_API int yq_samples_get(yq_ctx* c, int8_t* sigma, double* theta) {
   TRY(need_rbm(c));
   if (!c->have_batch) return fail(c, YQ_ERR_STATE, "no samples");
-  const size_t cnt = (size_t)c->ns * c->m;
+  const size_t cnt = (size_t)c->ns * c->m * ncomp(c);
   if (sigma)
     HIPCHK(c, hipMemcpyAsync(sigma, c->bsig.p, (size_t)c->ns * c->n, hipMemcpyDeviceToHost, c->st));
   if (theta) {
@@ -922,10 +1087,14 @@ YQ_API int yq_samples_set(yq_ctx* c, int ns, const int8_t* sigma, int precision)
     if (sigma[q] != 1 && sigma[q] != -1) return fail(c, YQ_ERR_ARG, "spins must be +1 or -1");
   const size_t rs = precision == YQ_MIXED ? sizeof(float) : sizeof(double);
   TRY(ensure(c, c->bsig, cnt));
-  TRY(ensure(c, c->btheta, rs * (size_t)ns * c->m));
+  TRY(ensure(c, c->btheta, rs * ncomp(c) * (size_t)ns * c->m));
   HIPCHK(c, hipMemcpyAsync(c->bsig.p, sigma, cnt, hipMemcpyHostToDevice, c->st));
-  launch_theta_batch(W64(c), b64(c), c->bsig.as<int8_t>(), ns, c->n, c->m, c->btheta.p,
-                     precision == YQ_MIXED, c->st);
+  if (c->cplx)
+    launch_theta_batch_c(W64(c), b64(c), c->bsig.as<int8_t>(), ns, c->n, c->m, c->btheta.p,
+                         precision == YQ_MIXED, c->st);
+  else
+    launch_theta_batch(W64(c), b64(c), c->bsig.as<int8_t>(), ns, c->n, c->m, c->btheta.p,
+                       precision == YQ_MIXED, c->st);
   LAUNCHCHK(c);
   c->ns = ns;
   c->bprec = precision;
@@ -940,7 +1109,8 @@ YQ_API int yq_eloc(yq_ctx* c, double* eloc, double* mean, double* var) {
   TRY(need_rbm(c));
   TRY(compute_eloc(c));
   if (eloc)
-    HIPCHK(c, hipMemcpyAsync(eloc, c->eloc.p, sizeof(double) * c->ns, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(eloc, c->eloc.p, sizeof(double) * ncomp(c) * c->ns,
+                             hipMemcpyDeviceToHost, c->st));
   if (mean) *mean = c->e_mean;
   if (var) *var = c->e_var;
   return sync(c);
@@ -958,11 +1128,49 @@ YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {
   if (!out) return fail(c, YQ_ERR_ARG, "null output");
   if (item == YQ_SR_ELOC) {
     if (!c->have_eloc) return fail(c, YQ_ERR_STATE, "no local energies");
-    HIPCHK(c, hipMemcpyAsync(out, c->eloc.p, sizeof(double) * c->ns, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(out, c->eloc.p, sizeof(double) * ncomp(c) * c->ns,
+                             hipMemcpyDeviceToHost, c->st));
     return sync(c);
   }
   if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
   const size_t dim = c->dim;
+  if (c->cplx) {
+    // the device holds conj(O_c), conj(omean) and the lower triangle of S
+    // (column-major); the host gets O_c, omean and the full S (row-major)
+    switch (item) {
+      case YQ_SR_F:
+        HIPCHK(c, hipMemcpyAsync(out, c->F.p, sizeof(double) * 2 * dim, hipMemcpyDeviceToHost,
+                                 c->st));
+        return sync(c);
+      case YQ_SR_OMEAN:
+      case YQ_SR_O: {
+        const size_t cnt = item == YQ_SR_O ? 2 * dim * c->ns : 2 * dim;
+        const Buf& b = item == YQ_SR_O ? c->O : c->omean;
+        HIPCHK(c, hipMemcpyAsync(out, b.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->st));
+        TRY(sync(c));
+        for (size_t q = 1; q < cnt; q += 2) out[q] = -out[q];
+        return YQ_OK;
+      }
+      case YQ_SR_S: {
+        if (c->sr_solver != YQ_SOLVE_EXPLICIT)
+          return fail(c, YQ_ERR_STATE, "S exists only for the explicit solver");
+        std::vector<double> h(2 * dim * dim);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->S.p, sizeof(double) * h.size(),
+                                 hipMemcpyDeviceToHost, c->st));
+        TRY(sync(c));
+        for (size_t i = 0; i < dim; ++i)
+          for (size_t j = 0; j < dim; ++j) {
+            const bool low = i >= j;
+            const size_t src = low ? i + j * dim : j + i * dim;
+            out[2 * (i * dim + j)] = h[2 * src];
+            out[2 * (i * dim + j) + 1] = low ? h[2 * src + 1] : -h[2 * src + 1];
+          }
+        return YQ_OK;
+      }
+      default:
+        return fail(c, YQ_ERR_ARG, "unknown item");
+    }
+  }
   switch (item) {
     case YQ_SR_F:
       HIPCHK(c, hipMemcpyAsync(out, c->F.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->st));
@@ -993,11 +1201,21 @@ YQ_API int yq_sr_apply(yq_ctx* c, double shift, const double* v, double* out) {
   TRY(need_rbm(c));
   if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
   if (!v || !out) return fail(c, YQ_ERR_ARG, "null vector");
-  const size_t dim = c->dim;
+  const size_t dim = vlen(c);
   TRY(ensure(c, c->vx, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
   HIPCHK(c, hipMemcpyAsync(c->vx.p, v, sizeof(double) * dim, hipMemcpyHostToDevice, c->st));
-  if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+  if (c->cplx && c->sr_solver == YQ_SOLVE_EXPLICIT) {
+    TRY(blas(c));
+    HIPCHK(c, hipMemcpyAsync(c->vAp.p, c->vx.p, sizeof(double) * dim, hipMemcpyDeviceToDevice,
+                             c->st));
+    const rocblas_double_complex one(1.0, 0.0), sh(shift, 0.0);
+    BLASCHK(c, rocblas_zhemv(c->blas, rocblas_fill_lower, (rocblas_int)c->dim, &one,
+                             reinterpret_cast<const rocblas_double_complex*>(c->S.p),
+                             (rocblas_int)c->dim,
+                             reinterpret_cast<const rocblas_double_complex*>(c->vx.p), 1, &sh,
+                             reinterpret_cast<rocblas_double_complex*>(c->vAp.p), 1));
+  } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
     TRY(blas(c));
     const double one = 1.0;
     HIPCHK(c, hipMemcpyAsync(c->vAp.p, c->vx.p, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
@@ -1019,7 +1237,7 @@ YQ_API int yq_sr_solve(yq_ctx* c, double shift, double tol, int max_iter, const 
                        double* x, double* resid, int* iters) {
   TRY(need_rbm(c));
   if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
-  const size_t dim = c->dim;
+  const size_t dim = vlen(c);
   TRY(ensure(c, c->vx, sizeof(double) * dim));
   TRY(ensure(c, c->vb, sizeof(double) * std::max(dim, (size_t)c->ns)));
   const double* b = c->F.as<double>();
@@ -1320,17 +1538,17 @@ YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
     if ((rc = sr_build(c, cfg->solver, cfg->precision)) != YQ_OK) break;
     int it = 0;
     double err = 0.0;
-    if ((rc = ensure(c, c->vx, sizeof(double) * c->dim)) != YQ_OK) break;
+    if ((rc = ensure(c, c->vx, sizeof(double) * vlen(c))) != YQ_OK) break;
     rc = solve(c, cfg->lambda, cfg->cg_tol, cfg->cg_max_iter, c->F.as<double>(),
                c->vx.as<double>(), &err, &it);
     if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) break;
     const int solve_rc = rc;
     double vv;
-    if ((rc = dot(c, c->vx.as<double>(), c->vx.as<double>(), c->dim, &vv)) != YQ_OK) break;
+    if ((rc = dot(c, c->vx.as<double>(), c->vx.as<double>(), vlen(c), &vv)) != YQ_OK) break;
     const double nv = sqrt(vv);
     double scale = -cfg->lr;
     if (cfg->grad_clip != 0.0 && nv > cfg->grad_clip) scale *= cfg->grad_clip / nv;
-    launch_axpy(scale, c->vx.as<double>(), c->params.as<double>(), c->dim, c->st);
+    launch_axpy(scale, c->vx.as<double>(), c->params.as<double>(), vlen(c), c->st);
     if ((rc = refresh_params(c)) != YQ_OK) break;
     if ((rc = hipEventRecord(e2, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
     if ((rc = sync(c)) != YQ_OK) break;

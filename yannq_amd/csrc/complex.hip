@@ -1,0 +1,600 @@
+// Complex RBM (RBM<std::complex<double>>) on gfx950: sampler, local
+// energies and the dense SR building blocks.
+//
+// Parameters are complex doubles interleaved (re, im) in the [W | a | b]
+// layout of Machines/RBM.hpp:271-281.  The kernels read them as planes: the
+// real and imaginary parts of W by visible unit (i*m + j) and transposed
+// (j*n + i), and Re a / Im a, in double and float (launch_cplx_planes).
+//
+// Sampler: one wavefront per chain as in sampler.hip, with Re and Im of
+// theta in VGPRs.  Acceptance needs only Re dlogpsi (RBMState.hpp:42-45,
+// logRatioRe), so each hidden unit costs one Re logcosh (yq_math.h
+// logcosh_re, replayed bit for bit on the CPU); the lane-strided sum and the
+// stride-32..1 butterfly are those of the real sampler.
+//
+// Local energies: one thread per (sample, bond) with the full complex
+// log-ratio, ratio = exp(Re) (cos Im + i sin Im).
+//
+// SR: O is held conjugated, Oh = conj(O) (ns x dim complex, row-major), so
+// that with Oh read as a dim x ns column-major matrix A:
+//   S   = A A^H / ns          (rocBLAS zherk, trans N)
+//   F   = A E_c / ns          (column sums below)
+//   O_c v = A^H v,  O_c^H y = A y   (zgemv C and N)
+// which are the adjoint products of SRMat.hpp:89-103, 142-149, 170-189.
+#include "yq_internal.h"
+#include "yq_math.h"
+
+namespace yq {
+namespace {
+
+constexpr int kChunkC = 256;  // samples per partial-sum chunk (as sr.hip)
+
+__global__ void k_cplx_planes(const double* __restrict__ p, int n, int m, CplxPlanes P) {
+  const size_t nm = (size_t)n * m;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nm) {
+    const size_t i = q / m, j = q % m;
+    const double wr = p[2 * q], wi = p[2 * q + 1];
+    P.Wr64[q] = wr;
+    P.Wi64[q] = wi;
+    P.Wr32[q] = (float)wr;
+    P.Wi32[q] = (float)wi;
+    P.WTr64[j * n + i] = wr;
+    P.WTi64[j * n + i] = wi;
+    P.WTr32[j * n + i] = (float)wr;
+    P.WTi32[j * n + i] = (float)wi;
+  } else if (q < nm + n) {
+    const size_t i = q - nm;
+    const double ar = p[2 * q], ai = p[2 * q + 1];
+    P.ar64[i] = ar;
+    P.ai64[i] = ai;
+    P.ar32[i] = (float)ar;
+    P.ai32[i] = (float)ai;
+  }
+}
+
+template <typename R>
+__device__ inline R lc_re(R x, R y);
+template <>
+__device__ inline double lc_re<double>(double x, double y) {
+  return logcosh_re_d(x, y);
+}
+template <>
+__device__ inline float lc_re<float>(float x, float y) {
+  return logcosh_re_f(x, y);
+}
+
+template <typename R, int KIND, int RMAX>
+__global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
+  extern __shared__ int8_t lds_sig[];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int chain = blockIdx.x * 4 + wave;
+  if (chain >= A.C) return;  // wave-uniform; no block barrier follows
+  const int n = A.n, m = A.m;
+  int8_t* sig = lds_sig + wave * A.npad;
+  for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  const R* Wr = static_cast<const R*>(A.Wr);
+  const R* Wi = static_cast<const R*>(A.Wi);
+  const R* ar = static_cast<const R*>(A.ar);
+  R tr[RMAX], ti[RMAX], lc[RMAX], ur[RMAX], ui[RMAX], lt[RMAX];
+
+  // theta_j = (sum_i +/-W_ji, index order, fp64, Re and Im apart) + b_j
+  auto refresh = [&]() {
+    double sr[RMAX], si[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) sr[r] = si[r] = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const bool up = sig[i] > 0;
+      const double* wc = A.W64 + 2 * (size_t)i * m;
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) {
+          const double wr = wc[2 * j], wi = wc[2 * j + 1];
+          sr[r] = up ? sr[r] + wr : sr[r] - wr;
+          si[r] = up ? si[r] + wi : si[r] - wi;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) {
+        tr[r] = (R)(sr[r] + A.b64[2 * j]);
+        ti[r] = (R)(si[r] + A.b64[2 * j + 1]);
+        lc[r] = lc_re<R>(tr[r], ti[r]);
+      } else {
+        tr[r] = ti[r] = lc[r] = (R)0;
+      }
+    }
+  };
+
+  uint64_t ev = A.ctr[chain];
+  uint64_t nacc = 0;
+  const R beta = (R)A.beta;
+  refresh();
+  bool fresh = true;
+  const int total = A.n_discard + A.n_sweeps;
+  for (int sw = 0; sw < total; ++sw) {
+    if (A.mixed && !fresh) refresh();
+    fresh = false;
+    for (int p = 0; p < A.nprop; ++p) {
+      const u32x4 x = draw(A.seed, ev++, (uint32_t)chain, kDomSweep);
+      const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
+      const int8_t sk = sig[k];
+      const R twok = sk > 0 ? (R)2 : (R)-2;
+      const R* wkr = Wr + (size_t)k * m;
+      const R* wki = Wi + (size_t)k * m;
+      R cst = ((R)-2 * ar[k]) * (R)sk;
+      int l = 0;
+      int8_t sl = 0;
+      R twol = (R)0;
+      const R* wlr = wkr;
+      const R* wli = wki;
+      if (KIND == 1) {
+        l = __builtin_amdgcn_readfirstlane((int)index_from(x.y, (uint32_t)n));
+        sl = sig[l];
+        if (sk == sl) continue;  // SwapSweeper.hpp:44-47, uniform branch
+        twol = sl > 0 ? (R)2 : (R)-2;
+        wlr = Wr + (size_t)l * m;
+        wli = Wi + (size_t)l * m;
+        cst = cst - ((R)2 * ar[l]) * (R)sl;
+      }
+      R part = (R)0;
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) {
+          R vr = tr[r] - twok * wkr[j];
+          R vi = ti[r] - twok * wki[j];
+          if (KIND == 1) {
+            vr = vr - twol * wlr[j];
+            vi = vi - twol * wli[j];
+          }
+          ur[r] = vr;
+          ui[r] = vi;
+          lt[r] = lc_re<R>(vr, vi);
+          part = part + (lt[r] - lc[r]);
+        }
+      }
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
+      const R delta = cst + part;
+      bool ok;
+      if (sizeof(R) == 4)
+        ok = accept_f((float)beta, (float)delta, u01_float(x.z));
+      else
+        ok = accept_d((double)beta, (double)delta, u01_double(x.z, x.w));
+      if (ok) {
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+          const int j = lane + kWave * r;
+          if (j < m) {
+            tr[r] = ur[r];
+            ti[r] = ui[r];
+            lc[r] = lt[r];
+          }
+        }
+        sig[k] = (int8_t)-sk;  // every lane stores the same byte
+        if (KIND == 1) sig[l] = (int8_t)-sl;
+        ++nacc;
+      }
+    }
+    if (sw >= A.n_discard) {
+      const size_t row = (size_t)(sw - A.n_discard) * A.C + chain;
+      for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+      if (A.mixed) {
+        refresh();
+        fresh = true;
+      }
+      R* ot = static_cast<R*>(A.out_theta) + 2 * row * m;
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) {
+          ot[2 * j] = tr[r];
+          ot[2 * j + 1] = ti[r];
+        }
+      }
+    }
+  }
+  for (int i = lane; i < n; i += kWave) A.sigma[(size_t)chain * n + i] = sig[i];
+  if (lane == 0) {
+    A.ctr[chain] = ev;
+    A.acc[chain] += nacc;
+  }
+}
+
+template <typename R, int KIND>
+void launch_rc(const SampleArgs& a, hipStream_t st) {
+  const dim3 grid((a.C + 3) / 4), block(256);
+  const size_t lds = (size_t)4 * a.npad;
+  switch ((a.m + kWave - 1) / kWave) {
+    case 1: hipLaunchKernelGGL((k_sample_c<R, KIND, 1>), grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_sample_c<R, KIND, 2>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_sample_c<R, KIND, 3>), grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_sample_c<R, KIND, 4>), grid, block, lds, st, a); break;
+    case 5: hipLaunchKernelGGL((k_sample_c<R, KIND, 5>), grid, block, lds, st, a); break;
+    case 6: hipLaunchKernelGGL((k_sample_c<R, KIND, 6>), grid, block, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((k_sample_c<R, KIND, 7>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_sample_c<R, KIND, 8>), grid, block, lds, st, a); break;
+  }
+}
+
+// theta for a batch, one thread per (sample, hidden unit); out interleaved
+template <typename O>
+__global__ void __launch_bounds__(256)
+    k_theta_batch_c(const double* __restrict__ W64, const double* __restrict__ b64,
+                    const int8_t* __restrict__ sigma, int ns, int n, int m, O* __restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)ns * m) return;
+  const int s = (int)(idx / m), j = (int)(idx % m);
+  const int8_t* sg = sigma + (size_t)s * n;
+  double sr = 0.0, si = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double wr = W64[2 * ((size_t)i * m + j)], wi = W64[2 * ((size_t)i * m + j) + 1];
+    sr = sg[i] > 0 ? sr + wr : sr - wr;
+    si = sg[i] > 0 ? si + wi : si - wi;
+  }
+  out[2 * idx] = (O)(sr + b64[2 * j]);
+  out[2 * idx + 1] = (O)(si + b64[2 * j + 1]);
+}
+
+// ---------------------------------------------------------------- E_loc
+__device__ inline void bond_sites_c(int kind, int n, int b, int& k, int& l) {
+  if (kind == YQ_TFI) {
+    k = b;
+    l = -1;
+  } else if (kind == YQ_XXZ || b < n) {
+    k = b;
+    l = (b + 1) % n;
+  } else {
+    k = b - n;
+    l = (b - n + 2) % n;
+  }
+}
+
+__device__ inline void lc_c(double x, double y, double* re, double* im) {
+  logcosh_c_d(x, y, re, im);
+}
+__device__ inline void lc_c(float x, float y, float* re, float* im) {
+  logcosh_c_f(x, y, re, im);
+}
+__device__ inline void sc_c(double x, double* s, double* c) { sincos_d(x, s, c); }
+__device__ inline void sc_c(float x, float* s, float* c) { sincos_f(x, s, c); }
+
+// One thread per (sample, bond): the complex log-ratio of RBMState.hpp:48-63
+// / 78-93 in the reference's j order; ratio = exp of it (complex double).
+template <typename R>
+__global__ void __launch_bounds__(256) k_eloc_ratio_c(const ElocArgs A, int span) {
+  extern __shared__ unsigned char lds_raw[];
+  R* thr = reinterpret_cast<R*>(lds_raw);
+  R* thi = thr + (size_t)span * A.m;
+  R* lcr = thi + (size_t)span * A.m;
+  R* lci = lcr + (size_t)span * A.m;
+  int8_t* sg = reinterpret_cast<int8_t*>(lci + (size_t)span * A.m);
+  const int n = A.n, m = A.m, B = A.B;
+  const size_t first = (size_t)blockIdx.x * 256;
+  const size_t total = (size_t)A.ns * B;
+  const int s_lo = (int)(first / B);
+  const size_t last = first + 255 < total ? first + 255 : total - 1;
+  const int s_hi = (int)(last / B);
+  const int cnt = s_hi - s_lo + 1;
+  const R* theta = static_cast<const R*>(A.theta);
+  for (int q = threadIdx.x; q < cnt * m; q += 256) {
+    const R x = theta[2 * ((size_t)s_lo * m + q)], y = theta[2 * ((size_t)s_lo * m + q) + 1];
+    thr[q] = x;
+    thi[q] = y;
+    lc_c(x, y, &lcr[q], &lci[q]);
+  }
+  for (int q = threadIdx.x; q < cnt * n; q += 256) sg[q] = A.sigma[(size_t)s_lo * n + q];
+  __syncthreads();
+  const size_t idx = first + threadIdx.x;
+  if (idx >= total) return;
+  const int s = (int)(idx / B), b = (int)(idx % B);
+  const int ls = s - s_lo;
+  const int8_t* sgs = sg + (size_t)ls * n;
+  int k, l;
+  bond_sites_c(A.ham.kind, n, b, k, l);
+  double2* ratio = reinterpret_cast<double2*>(A.ratio);
+  if (l >= 0 && sgs[k] == sgs[l]) {  // prefactor (1 - zz) vanishes
+    ratio[idx] = make_double2(0.0, 0.0);
+    return;
+  }
+  const R* WTr = static_cast<const R*>(A.WT);
+  const R* WTi = static_cast<const R*>(A.WTi);
+  const R* ar = static_cast<const R*>(A.ar);
+  const R* ai = static_cast<const R*>(A.ai);
+  const int8_t sk = sgs[k];
+  const R twok = sk > 0 ? (R)2 : (R)-2;
+  R rr = ((R)-2 * ar[k]) * (R)sk, ri = ((R)-2 * ai[k]) * (R)sk;
+  const R* tsr = thr + (size_t)ls * m;
+  const R* tsi = thi + (size_t)ls * m;
+  const R* lsr = lcr + (size_t)ls * m;
+  const R* lsi = lci + (size_t)ls * m;
+  int8_t sl = 0;
+  R twol = (R)0;
+  if (l >= 0) {
+    sl = sgs[l];
+    twol = sl > 0 ? (R)2 : (R)-2;
+    rr = rr - ((R)2 * ar[l]) * (R)sl;
+    ri = ri - ((R)2 * ai[l]) * (R)sl;
+  }
+  for (int j = 0; j < m; ++j) {
+    R x = tsr[j] - twok * WTr[(size_t)j * n + k];
+    R y = tsi[j] - twok * WTi[(size_t)j * n + k];
+    if (l >= 0) {
+      x = x - twol * WTr[(size_t)j * n + l];
+      y = y - twol * WTi[(size_t)j * n + l];
+    }
+    R lr, li;
+    lc_c(x, y, &lr, &li);
+    rr = rr + (lr - lsr[j]);
+    ri = ri + (li - lsi[j]);
+  }
+  R sn, cs;
+  sc_c(ri, &sn, &cs);
+  const R mag = Math<R>::exp(rr);
+  ratio[idx] = make_double2((double)(mag * cs), (double)(mag * sn));
+}
+
+// the functor's accumulation (TFIsing.hpp:28-39, XXZ.hpp:37-49,
+// J1J2.hpp:35-55) with complex ratios
+__global__ void __launch_bounds__(256) k_eloc_combine_c(const ElocArgs A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.ns) return;
+  const int n = A.n;
+  const int8_t* sg = A.sigma + (size_t)s * n;
+  const double2* ratio = reinterpret_cast<const double2*>(A.ratio) + (size_t)s * A.B;
+  const yq_ham_desc& H = A.ham;
+  double er = 0.0, ei = 0.0;
+  if (H.kind == YQ_TFI) {
+    for (int i = 0; i < n; ++i) {
+      er += H.J * (double)sg[i] * (double)sg[(i + 1) % n];
+      er += H.h * ratio[i].x;
+      ei += H.h * ratio[i].y;
+    }
+  } else if (H.kind == YQ_XXZ) {
+    const double sign = H.sign_rule ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) {
+      const int zz = (int)sg[i] * (int)sg[(i + 1) % n];
+      er += H.J * H.delta * zz;
+      const double c = sign * H.J * (1 - zz);
+      er += c * ratio[i].x;
+      ei += c * ratio[i].y;
+    }
+  } else {
+    const int sign = H.sign_rule ? -1 : 1;
+    for (int i = 0; i < n; ++i) {
+      const double yy = -(double)((int)sg[i] * (int)sg[(i + 1) % n]);
+      er += -H.J * yy;
+      const double c = H.J * sign * (1.0 + yy);
+      er += c * ratio[i].x;
+      ei += c * ratio[i].y;
+    }
+    for (int i = 0; i < n; ++i) {
+      const double yy = -(double)((int)sg[i] * (int)sg[(i + 2) % n]);
+      er += -H.J2 * yy;
+      const double c = H.J2 * (1.0 + yy);
+      er += c * ratio[n + i].x;
+      ei += c * ratio[n + i].y;
+    }
+  }
+  reinterpret_cast<double2*>(A.eloc)[s] = make_double2(er, ei);
+}
+
+// E = Re <E_loc>, Var = <(Re E_loc)^2> - E^2 (Energy.hpp:59-70), one block
+__global__ void __launch_bounds__(1024) k_eloc_stats_c(const double* e, int ns, double* out) {
+  __shared__ double s1[1024], s2[1024];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < ns; i += 1024) {
+    const double v = e[2 * (size_t)i];
+    a += v;
+    b += v * v;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 512; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double mean = s1[0] / ns;
+    out[0] = mean;
+    out[1] = s2[0] / ns - mean * mean;
+  }
+}
+
+// ---------------------------------------------------------------- SR
+template <typename I>
+__global__ void k_tanh_c(const I* __restrict__ th, double* __restrict__ T, size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cnt) return;
+  double re, im;
+  tanh_c_d((double)th[2 * q], (double)th[2 * q + 1], &re, &im);
+  T[2 * q] = re;
+  T[2 * q + 1] = im;
+}
+
+// Oh[s][c] = conj(O_sc) with O_s = logDeriv (RBM.hpp:390-406)
+__global__ void k_build_Oh(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns,
+                           int n, int m, int use_bias, double* __restrict__ Oh) {
+  const size_t nm = (size_t)n * m;
+  const size_t dim = use_bias ? nm + n + m : nm;
+  const size_t total = (size_t)ns * dim;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = q / dim, c = q % dim;
+    double re, im;
+    if (c < nm) {
+      const int i = (int)(c / m), j = (int)(c % m);
+      const double sg = sigma[s * n + i] > 0 ? 1.0 : -1.0;
+      re = sg * T[2 * (s * m + j)];
+      im = sg * T[2 * (s * m + j) + 1];
+    } else if (c < nm + n) {
+      re = sigma[s * n + (c - nm)] > 0 ? 1.0 : -1.0;
+      im = 0.0;
+    } else {
+      const size_t j = c - nm - n;
+      re = T[2 * (s * m + j)];
+      im = T[2 * (s * m + j) + 1];
+    }
+    Oh[2 * q] = re;
+    Oh[2 * q + 1] = -im;
+  }
+}
+
+// Oh -= omean_h (in place); partial[k][c] = sum_{s in chunk k} Oh_sc e_s
+__global__ void k_center_force_c(double* __restrict__ Oh, const double* __restrict__ om,
+                                 const double* __restrict__ e, int ns, int dim,
+                                 double* __restrict__ partial) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= dim) return;
+  const int k = blockIdx.y;
+  const int s0 = k * kChunkC, s1 = min(ns, s0 + kChunkC);
+  const double mr = om[2 * c], mi = om[2 * c + 1];
+  double fr = 0.0, fi = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    double* o = Oh + 2 * ((size_t)s * dim + c);
+    const double vr = o[0] - mr, vi = o[1] - mi;
+    o[0] = vr;
+    o[1] = vi;
+    const double er = e[2 * s], ei = e[2 * s + 1];
+    fr += vr * er - vi * ei;
+    fi += vr * ei + vi * er;
+  }
+  partial[2 * ((size_t)k * dim + c)] = fr;
+  partial[2 * ((size_t)k * dim + c) + 1] = fi;
+}
+
+// partial[k][c] = sum_{s in chunk k} |Oh_sc|^2
+__global__ void k_absq_colsum(const double* __restrict__ Oh, int ns, int dim,
+                              double* __restrict__ partial) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= dim) return;
+  const int k = blockIdx.y;
+  const int s0 = k * kChunkC, s1 = min(ns, s0 + kChunkC);
+  double acc = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    const double* o = Oh + 2 * ((size_t)s * dim + c);
+    acc += o[0] * o[0] + o[1] * o[1];
+  }
+  partial[(size_t)k * dim + c] = acc;
+}
+
+__global__ void k_dup2(const double* __restrict__ x, double* __restrict__ y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[2 * i] = y[2 * i + 1] = x[i];
+}
+
+// M = S + shift I for a column-major complex dim x dim matrix
+__global__ void k_copy_shift_c(const double* __restrict__ S, double* __restrict__ M, int dim,
+                               double shift) {
+  const size_t total = (size_t)dim * dim;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = q % dim, c = q / dim;
+    M[2 * q] = S[2 * q] + (r == c ? shift : 0.0);
+    M[2 * q + 1] = S[2 * q + 1];
+  }
+}
+
+inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void launch_cplx_planes(const double* p, int n, int m, const CplxPlanes& P, hipStream_t st) {
+  hipLaunchKernelGGL(k_cplx_planes, dim3(nblk((size_t)n * m + n)), dim3(256), 0, st, p, n, m, P);
+}
+
+void launch_sample_c(const SampleArgs& a, hipStream_t st) {
+  ProfGuard g("sample_c", st);
+  if (a.mixed) {
+    if (a.kind == 1) launch_rc<float, 1>(a, st);
+    else launch_rc<float, 0>(a, st);
+  } else {
+    if (a.kind == 1) launch_rc<double, 1>(a, st);
+    else launch_rc<double, 0>(a, st);
+  }
+}
+
+void launch_theta_batch_c(const double* W64, const double* b64, const int8_t* sigma, int ns,
+                          int n, int m, void* out, int out_float, hipStream_t st) {
+  const size_t total = (size_t)ns * m;
+  if (out_float)
+    hipLaunchKernelGGL(k_theta_batch_c<float>, dim3(nblk(total)), dim3(256), 0, st, W64, b64,
+                       sigma, ns, n, m, static_cast<float*>(out));
+  else
+    hipLaunchKernelGGL(k_theta_batch_c<double>, dim3(nblk(total)), dim3(256), 0, st, W64, b64,
+                       sigma, ns, n, m, static_cast<double*>(out));
+}
+
+size_t eloc_c_lds_bytes(const ElocArgs& a) {
+  const int span = 256 / a.B + 2;
+  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
+  return (size_t)span * a.m * 4 * rs + (size_t)span * a.n + 16;
+}
+
+void launch_eloc_c(const ElocArgs& a, hipStream_t st) {
+  const size_t total = (size_t)a.ns * a.B;
+  const int span = 256 / a.B + 2;
+  const size_t lds = eloc_c_lds_bytes(a);
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  {
+    ProfGuard g("eloc_ratio_c", st);
+    if (a.use_float)
+      hipLaunchKernelGGL(k_eloc_ratio_c<float>, grid, block, lds, st, a, span);
+    else
+      hipLaunchKernelGGL(k_eloc_ratio_c<double>, grid, block, lds, st, a, span);
+  }
+  hipLaunchKernelGGL(k_eloc_combine_c, dim3((a.ns + 255) / 256), dim3(256), 0, st, a);
+}
+
+void launch_eloc_stats_c(const double* eloc, int ns, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_eloc_stats_c, dim3(1), dim3(1024), 0, st, eloc, ns, out);
+}
+
+void launch_tanh_c(const void* theta, int in_float, double* T, size_t count, hipStream_t st) {
+  if (in_float)
+    hipLaunchKernelGGL(k_tanh_c<float>, dim3(nblk(count)), dim3(256), 0, st,
+                       static_cast<const float*>(theta), T, count);
+  else
+    hipLaunchKernelGGL(k_tanh_c<double>, dim3(nblk(count)), dim3(256), 0, st,
+                       static_cast<const double*>(theta), T, count);
+}
+
+void launch_build_Oh(const int8_t* sigma, const double* T, int ns, int n, int m, int use_bias,
+                     double* Oh, hipStream_t st) {
+  ProfGuard g("build_Oh", st);
+  hipLaunchKernelGGL(k_build_Oh, dim3(4096), dim3(256), 0, st, sigma, T, ns, n, m, use_bias, Oh);
+}
+
+void launch_center_force_c(double* Oh, const double* om, const double* e, int ns, int dim,
+                           double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_center_force_c, dim3(nblk(dim), (ns + kChunkC - 1) / kChunkC), dim3(256),
+                     0, st, Oh, om, e, ns, dim, partial);
+}
+
+void launch_absq_colsum(const double* Oh, int ns, int dim, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_absq_colsum, dim3(nblk(dim), (ns + kChunkC - 1) / kChunkC), dim3(256), 0,
+                     st, Oh, ns, dim, partial);
+}
+
+void launch_dup2(const double* x, double* y, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_dup2, dim3(nblk(n)), dim3(256), 0, st, x, y, n);
+}
+
+void launch_copy_shift_c(const double* S, double* M, int dim, double shift, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_shift_c, dim3(8192), dim3(256), 0, st, S, M, dim, shift);
+}
+
+}  // namespace yq

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 namespace yq {
 
@@ -24,17 +25,14 @@ struct MixedWork {
   // per-product scratch
   uint16_t* vsplit = nullptr;  // 3 x Np x Kp bf16 (hi, mid, lo of V')
   float* ypart = nullptr;      // ncg x nsp forward partial sums
-  float* y = nullptr;          // nsp
   float* part = nullptr;       // chunks x Kp x Np backward partials
   float* om32 = nullptr;       // column mean in float (dim)
   // CG vectors (float) and device scalars (double)
   float *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
   float* dinv = nullptr;       // Jacobi preconditioner 1 / (diag S + shift)
-  double* dpart = nullptr;     // per-block partials of the fused dot products
-  unsigned* ticket = nullptr;  // last-block counter of the fused dot products
-  double* scal = nullptr;      // 4 scalars per CG iteration + scratch
-  int scal_cap = 0;
-  size_t cap_rows = 0, cap_dim = 0;
+  double* red = nullptr;       // per-block partials of the fused dot products
+  std::vector<double> host_red;
+  size_t cap_rows = 0, cap_dim = 0, cap_red = 0;
 };
 
 // Build the bf16 / float operands from the batch (sigma: ns x n int8,

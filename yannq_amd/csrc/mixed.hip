@@ -12,10 +12,9 @@
 // A[row l%16][k 8*(l/16) .. +7] and B[k 8*(l/16) .. +7][col l%16] and
 // returns D[4*(l/16) + q][l%16], q = 0..3.
 //
-// The CG loop needs six launches per iteration and no host round trip: the
-// vector updates carry their dot products, finished on the device by the
-// last block to arrive (per-block partials summed in block order, so the
-// result does not depend on scheduling).
+// The CG loop needs five launches per iteration and no host round trip: the
+// vector updates carry their dot products as per-block partials, which the
+// next kernel that needs the value sums itself in a fixed order.
 #include <math.h>
 
 #include "mixed.h"
@@ -58,11 +57,13 @@ __device__ inline bf16x8 ld8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, v);
 }
 
-// Sum v over the block (fixed tree), store the block partial, and let the
-// last block to finish sum all partials in block order into *dst.
-__device__ void block_dot_finish(double v, double* part, unsigned* ticket, double* dst) {
+// Dot products travel between launches as per-block partial sums: the
+// producing kernel stores one double per block (fixed tree over the block),
+// and every wave of a consuming kernel sums the partials itself in a fixed
+// order (lane-strided sums, then an xor butterfly).  No fences, atomics or
+// counters are involved, and the value does not depend on scheduling.
+__device__ void block_partial(double v, double* out, int slot) {
   __shared__ double red[256];
-  __shared__ bool last;
   const int t = threadIdx.x;
   red[t] = v;
   __syncthreads();
@@ -70,35 +71,10 @@ __device__ void block_dot_finish(double v, double* part, unsigned* ticket, doubl
     if (t < w) red[t] += red[t + w];
     __syncthreads();
   }
-  if (t == 0) {
-    part[blockIdx.x] = red[0];
-    __threadfence();
-    const unsigned prev = atomicAdd(ticket, 1u);
-    last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double s = 0.0;
-  for (unsigned i = t; i < gridDim.x; i += 256)
-    s += __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  red[t] = s;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
-  }
-  if (t == 0) {
-    *dst = red[0];
-    *ticket = 0u;
-  }
+  if (t == 0) out[slot] = red[0];
 }
-
-// Two dot products finished together (partials interleaved per block).
-__device__ void block_dot_finish2(double v1, double v2, double* part, unsigned* ticket,
-                                  double* dst1, double* dst2) {
+__device__ void block_partial2(double v1, double v2, double* out1, double* out2) {
   __shared__ double r1[256], r2[256];
-  __shared__ bool last;
   const int t = threadIdx.x;
   r1[t] = v1;
   r2[t] = v2;
@@ -111,35 +87,18 @@ __device__ void block_dot_finish2(double v1, double v2, double* part, unsigned* 
     __syncthreads();
   }
   if (t == 0) {
-    part[2 * blockIdx.x] = r1[0];
-    part[2 * blockIdx.x + 1] = r2[0];
-    __threadfence();
-    const unsigned prev = atomicAdd(ticket, 1u);
-    last = prev == gridDim.x - 1;
+    out1[blockIdx.x] = r1[0];
+    out2[blockIdx.x] = r2[0];
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double s1 = 0.0, s2 = 0.0;
-  for (unsigned i = t; i < gridDim.x; i += 256) {
-    s1 += __hip_atomic_load(part + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s2 += __hip_atomic_load(part + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  r1[t] = s1;
-  r2[t] = s2;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) {
-      r1[t] += r1[t + w];
-      r2[t] += r2[t + w];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    *dst1 = r1[0];
-    *dst2 = r2[0];
-    *ticket = 0u;
-  }
+}
+// total of cnt partials, identical in every lane of every wave
+__device__ inline double psum(const double* __restrict__ part, int cnt) {
+  const int l = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = l; i < cnt; i += 64) s += part[i];
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w);
+  return s;
 }
 
 // index of parameter c on the (Kp x Np) grid of V' / G'
@@ -182,38 +141,48 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
 }
 
 // p <- z + beta p with z = dinv * r (dinv null: z = r) and beta = rho_now /
-// rho_prev (0 when rho_prev is null); V' = p on the (Kp x Np) grid split into
-// three bf16 layers stored transposed, vs[(layer * Np + j) * Kp + k];
-// *dst = <omean, p>.
+// rho_prev (rho_prev null: beta = 0), rho given as partials; V' = p on the
+// (Kp x Np) grid split into three bf16 layers stored column-major,
+// vs[(layer * Np + j) * Kp + k]; op = partials of <omean, p>.  Block: a 16 x
+// 16 tile of the grid, read along j (consecutive parameters) and written
+// along k through an LDS transpose, so both sides are coalesced.
 __global__ void __launch_bounds__(256)
     k_split_v(const float* __restrict__ r, const float* __restrict__ dinv, float* __restrict__ p,
-              const double* rr_now, const double* rr_prev, const float* __restrict__ om, int n,
-              int m, int use_bias, int Kp, int Np, uint16_t* __restrict__ vs, double* part,
-              unsigned* ticket, double* dst) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+              const double* __restrict__ rz_now, const double* __restrict__ rz_prev, int nrz,
+              const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
+              uint16_t* __restrict__ vs, double* __restrict__ op) {
+  __shared__ uint16_t tile[3][16][17];
+  const int t = threadIdx.x;
+  const int j0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   const size_t nm = (size_t)n * m;
-  float v = 0.0f;
-  double dv = 0.0;
-  if (q < Np * Kp) {
-    const int j = q / Kp, k = q % Kp;
+  float beta = 0.0f;
+  if (rz_prev) beta = (float)(psum(rz_now, nrz) / psum(rz_prev, nrz));
+  {
+    const int kk = t >> 4, jj = t & 15;
+    const int k = k0 + kk, j = j0 + jj;
     long c = -1;
     if (k < n && j < m) c = (long)k * m + j;
     else if (use_bias && k < n && j == m) c = (long)(nm + k);
     else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
+    float v = 0.0f;
+    double dv = 0.0;
     if (c >= 0) {
-      const float beta = rr_prev ? (float)(*rr_now / *rr_prev) : 0.0f;
       const float z = dinv ? r[c] * dinv[c] : r[c];
-      v = rr_prev ? z + beta * p[c] : z;
+      v = rz_prev ? z + beta * p[c] : z;
       p[c] = v;
       dv = (double)om[c] * (double)v;
     }
     uint16_t hi, mid, lo;
     split3(v, hi, mid, lo);
-    vs[q] = hi;
-    vs[(size_t)Np * Kp + q] = mid;
-    vs[(size_t)2 * Np * Kp + q] = lo;
+    tile[0][jj][kk] = hi;
+    tile[1][jj][kk] = mid;
+    tile[2][jj][kk] = lo;
+    block_partial(dv, op, blockIdx.y * gridDim.x + blockIdx.x);  // contains barriers
   }
-  block_dot_finish(dv, part, ticket, dst);
+  const int jj = t >> 4, kk = t & 15;
+#pragma unroll
+  for (int sp = 0; sp < 3; ++sp)
+    vs[((size_t)sp * Np + j0 + jj) * Kp + k0 + kk] = tile[sp][jj][kk];
 }
 
 // ypart[cg][s] = sum over the column tiles of group cg of T'_sj U'_sj.
@@ -288,34 +257,21 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// y_s = (sum_cg ypart - <omean, p>) / ns for s < ns, 0 beyond; *dst = sum_s y_s
-// (the centring correction of the backward product)
-__global__ void __launch_bounds__(256)
-    k_center_y(const float* __restrict__ ypart, int ncg, int ns, int nsp,
-               const double* __restrict__ dotp, float* __restrict__ y, double* dpart,
-               unsigned* ticket, double* dst) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  double dv = 0.0;
-  if (s < nsp) {
-    float v = 0.0f;
-    for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
-    const float yv = s < ns ? (v - (float)dotp[0]) / (float)ns : 0.0f;
-    y[s] = yv;
-    dv = yv;
-  }
-  block_dot_finish(dv, dpart, ticket, dst);
-}
-
-// part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj.  Block: 4 waves =
+// part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj with the centred
+// forward result y_s = (sum_cg ypart - <omean, p>) / ns (0 for padded rows),
+// formed in LDS for the chunk; blocks of column group 0 also store the chunk
+// sum of y (ys, the centring correction of the reduction).  Block: 4 waves =
 // 4 column tiles of one chunk; the chunk's sigma'^T slab is staged in LDS in
 // two halves of 128 samples and shared by the waves; every wave keeps all
 // Kp/16 row tiles in registers.
 constexpr int kBwdHalf = 128;
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
-               const float* __restrict__ y, int Kp, int Np, int nsp,
-               float* __restrict__ part) {
+               const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
+               int nop, int Kp, int Np, int nsp, float* __restrict__ part,
+               double* __restrict__ ys) {
   __shared__ uint16_t ash[160 * (kBwdHalf + kLdsPad)];
+  __shared__ float ysh[kBwdChunk];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
   const int ct = blockIdx.x * 4 + wave;
@@ -324,6 +280,15 @@ __global__ void __launch_bounds__(256)
   const int li = l & 15, lg = l >> 4;
   const int nit = Kp / 16;
   const int lda = kBwdHalf + kLdsPad;
+  {
+    const float ov = (float)psum(op, nop);
+    const int s = kc * kBwdChunk + tid;  // blockDim == kBwdChunk
+    float v = 0.0f;
+    for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
+    const float yv = s < ns ? (v - ov) / (float)ns : 0.0f;
+    ysh[tid] = yv;
+    if (blockIdx.x == 0) block_partial((double)yv, ys, kc);  // uniform per block
+  }
   floatx4 acc[kMaxIt];
 #pragma unroll
   for (int it = 0; it < kMaxIt; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
@@ -343,7 +308,7 @@ __global__ void __launch_bounds__(256)
       u16x8 bh, bm, bl;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const float v = trow[s0 + e] * y[s0 + e];
+        const float v = trow[s0 + e] * ysh[s0 - kc * kBwdChunk + e];
         uint16_t h, md, lo;
         split3(v, h, md, lo);
         bh[e] = h;
@@ -373,64 +338,70 @@ __global__ void __launch_bounds__(256)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
 
-// out[c] = G'[aug(c)] - omean_c sum(y) + shift p_c, G' summed over the chunks
-// with eight interleaved partial sums combined by a fixed tree; *dst = <p, out>
+// out[c] = G'[aug(c)] - omean_c sum(y) + shift p_c, G' summed over the
+// chunks.  Block: 64 parameters x 4 chunk groups (wave g sums chunks g, g+4,
+// ... in order; the four group sums are combined as (g0 + g1) + (g2 + g3));
+// pap = partials of <p, out>.
 __global__ void __launch_bounds__(256)
     k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m, int use_bias,
                  int Kp, int Np, const float* __restrict__ om, double shift,
-                 const double* __restrict__ sum_y, const float* __restrict__ p,
-                 float* __restrict__ out, double* dpart, unsigned* ticket, double* dst) {
+                 const double* __restrict__ ys, int nys, const float* __restrict__ p,
+                 float* __restrict__ out, double* __restrict__ pap) {
+  __shared__ float grp[4][64];
   const size_t nm = (size_t)n * m;
   const size_t dim = use_bias ? nm + n + m : nm;
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double dv = 0.0;
+  const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const size_t c = (size_t)blockIdx.x * 64 + l;
+  const float sy = (float)psum(ys, nys);
+  float acc = 0.f;
   if (c < dim) {
     const size_t a = aug_index(c, n, m, Np);
     const size_t stride = (size_t)Kp * Np;
-    float g8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < chunks; k += 8) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (k + u < chunks) g8[u] += part[(size_t)(k + u) * stride + a];
-    }
-    const float g = ((g8[0] + g8[1]) + (g8[2] + g8[3])) + ((g8[4] + g8[5]) + (g8[6] + g8[7]));
-    const float o = (g - om[c] * (float)sum_y[0]) + (float)shift * p[c];
+    for (int k = g; k < chunks; k += 4) acc += part[(size_t)k * stride + a];
+  }
+  grp[g][l] = acc;
+  __syncthreads();
+  double dv = 0.0;
+  if (g == 0 && c < dim) {
+    const float gs = (grp[0][l] + grp[1][l]) + (grp[2][l] + grp[3][l]);
+    const float o = (gs - om[c] * sy) + (float)shift * p[c];
     out[c] = o;
     dv = (double)p[c] * (double)o;
   }
-  block_dot_finish(dv, dpart, ticket, dst);
+  block_partial(dv, pap, blockIdx.x);
 }
 
-// x += alpha p, r -= alpha Ap with alpha = rho / pAp; next slot: rho' = <r, dinv r>
-// and <r, r>
+// x += alpha p, r -= alpha Ap with alpha = rho / <p, Ap> (both as partials);
+// rz_next, rr = partials of <r, dinv r> and <r, r> of the updated r
 __global__ void __launch_bounds__(256)
     k_cg_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
             const float* __restrict__ Ap, const float* __restrict__ dinv, size_t n,
-            const double* __restrict__ sc, double* dpart, unsigned* ticket, double* next) {
+            const double* __restrict__ rz_now, int nrz, const double* __restrict__ pap, int npap,
+            double* __restrict__ rz_next, double* __restrict__ rr_out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double pv = psum(pap, npap);
+  const float alpha = pv != 0.0 ? (float)(psum(rz_now, nrz) / pv) : 0.0f;
   double rz = 0.0, rr = 0.0;
   if (i < n) {
-    const double pap = sc[1];
-    const float alpha = pap != 0.0 ? (float)(sc[0] / pap) : 0.0f;
     x[i] += alpha * p[i];
     const float rv = r[i] - alpha * Ap[i];
     r[i] = rv;
     rr = (double)rv * (double)rv;
     rz = dinv ? (double)rv * (double)(rv * dinv[i]) : rr;
   }
-  block_dot_finish2(rz, rr, dpart, ticket, next + 0, next + 4);
+  block_partial2(rz, rr, rz_next, rr_out);
 }
 
 __global__ void __launch_bounds__(256)
-    k_dot_rz(const float* __restrict__ r, const float* __restrict__ dinv, size_t n, double* dpart,
-             unsigned* ticket, double* slot) {
+    k_dot_rz(const float* __restrict__ r, const float* __restrict__ dinv, size_t n,
+             double* __restrict__ rz_out, double* __restrict__ rr_out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
   if (i < n) {
     rr = (double)r[i] * (double)r[i];
     rz = dinv ? (double)r[i] * (double)(r[i] * dinv[i]) : rr;
   }
-  block_dot_finish2(rz, rr, dpart, ticket, slot + 0, slot + 4);
+  block_partial2(rz, rr, rz_out, rr_out);
 }
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -471,43 +442,57 @@ int alloc(T*& p, size_t count, std::string& err) {
   return YQ_OK;
 }
 
-// (S + shift) p -> Ap with the products on MFMA; the slot s receives
-// s[2] = <omean, p>, s[3] = sum(y) and s[1] = <p, Ap>.  p is first updated to
-// r + beta p (see k_split_v).
+// block counts of the partial-sum producers and their offsets in w.red
+struct RedLayout {
+  int nbd, nbv, nby, nbr;
+  size_t rz0, rz1, rr, op, ys, pap, total;
+};
+RedLayout red_layout(const MixedWork& w) {
+  RedLayout L;
+  L.nbd = (int)nb(w.dim);                     // elementwise kernels over dim
+  L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_split_v tiles
+  L.nby = w.nsp / kBwdChunk;                  // k_bwd_mfma chunks
+  L.nbr = (int)((w.dim + 63) / 64);           // k_bwd_reduce
+  L.rz0 = 0;
+  L.rz1 = L.rz0 + L.nbd;
+  L.rr = L.rz1 + L.nbd;
+  L.op = L.rr + L.nbd;
+  L.ys = L.op + L.nbv;
+  L.pap = L.ys + L.nby;
+  L.total = L.pap + L.nbr;
+  return L;
+}
+
+// (S + shift) p -> Ap with the products on MFMA, leaving the partials of
+// <p, Ap> in w.red.  p is first updated to z + beta p (see k_split_v) with
+// rho from the partials rz_now / rz_prev.
 void apply_launches(MixedWork& w, double shift, const float* r, const float* dinv,
-                    const double* rr_now, const double* rr_prev, double* s, hipStream_t st) {
-  const size_t grid_v = (size_t)w.Np * w.Kp;
-  hipLaunchKernelGGL(k_split_v, dim3(nb(grid_v)), dim3(256), 0, st, r, dinv, w.p, rr_now, rr_prev,
-                     w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np, w.vsplit, w.dpart, w.ticket,
-                     s + 2);
+                    const double* rz_now, const double* rz_prev, hipStream_t st) {
+  const RedLayout L = red_layout(w);
+  double* op = w.red + L.op;
+  double* ys = w.red + L.ys;
+  {
+    ProfGuard g("split_v", st);
+    hipLaunchKernelGGL(k_split_v, dim3(w.Np / 16, w.Kp / 16), dim3(256), 0, st, r, dinv, w.p,
+                       rz_now, rz_prev, L.nbd, w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np,
+                       w.vsplit, op);
+  }
   {
     ProfGuard g("fwd_mfma", st);
     const size_t lds = sizeof(uint16_t) * 3 * kColTiles * 16 * (w.Kp + kLdsPad);
     hipLaunchKernelGGL(k_fwd_mfma, dim3(w.nsp / 256, w.ncg), dim3(256), lds, st, w.sigA,
                        w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart);
   }
-  hipLaunchKernelGGL(k_center_y, dim3(nb(w.nsp)), dim3(256), 0, st, w.ypart, w.ncg, w.ns, w.nsp,
-                     s + 2, w.y, w.dpart, w.ticket, s + 3);
   const int chunks = w.nsp / kBwdChunk;
   {
     ProfGuard g("bwd_mfma", st);
     hipLaunchKernelGGL(k_bwd_mfma, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), 0, st, w.sigT,
-                       w.Tt, w.y, w.Kp, w.Np, w.nsp, w.part);
+                       w.Tt, w.ypart, w.ncg, w.ns, op, L.nbv, w.Kp, w.Np, w.nsp, w.part, ys);
   }
   ProfGuard g("bwd_reduce", st);
-  hipLaunchKernelGGL(k_bwd_reduce, dim3(nb(w.dim)), dim3(256), 0, st, w.part, chunks, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.om32, shift, s + 3, w.p, w.Ap, w.dpart, w.ticket,
-                     s + 1);
-}
-
-constexpr int kSlot = 8;  // doubles per CG iteration slot
-int ensure_scal(MixedWork& w, int iters, std::string& err) {
-  const int need = iters + 4 > 16 ? iters + 4 : 16;
-  if (w.scal_cap >= need) return YQ_OK;
-  int rc = alloc(w.scal, (size_t)kSlot * need, err);
-  if (rc) return rc;
-  w.scal_cap = need;
-  return YQ_OK;
+  hipLaunchKernelGGL(k_bwd_reduce, dim3(L.nbr), dim3(256), 0, st, w.part, chunks, w.n, w.m,
+                     w.use_bias, w.Kp, w.Np, w.om32, shift, ys, L.nby, w.p, w.Ap,
+                     w.red + L.pap);
 }
 
 }  // namespace
@@ -528,25 +513,23 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
   w.ncg = (w.Np / 16 + kColTiles - 1) / kColTiles;
   w.dim = (size_t)n * m + (use_bias ? (size_t)n + m : 0);
   const size_t rows = (size_t)w.nsp;
-  if (rows * w.Np > w.cap_rows || w.dim > w.cap_dim || !w.ticket) {
+  const size_t red_total = red_layout(w).total;
+  if (rows * w.Np > w.cap_rows || w.dim > w.cap_dim || red_total > w.cap_red) {
     const size_t chunks = rows / kBwdChunk;
-    const size_t dparts = nb(w.dim > (size_t)w.Np * w.Kp ? w.dim : (size_t)w.Np * w.Kp);
     int rc;
     if ((rc = alloc(w.sigA, rows * w.Kp, err)) || (rc = alloc(w.sigT, rows * w.Kp, err)) ||
         (rc = alloc(w.Trm, rows * w.Np, err)) || (rc = alloc(w.Tt, rows * w.Np, err)) ||
-        (rc = alloc(w.ypart, rows * w.ncg, err)) || (rc = alloc(w.y, rows, err)) ||
+        (rc = alloc(w.ypart, rows * w.ncg, err)) ||
         (rc = alloc(w.part, chunks * w.Kp * w.Np, err)) ||
         (rc = alloc(w.vsplit, (size_t)3 * w.Np * w.Kp, err)) ||
         (rc = alloc(w.om32, w.dim, err)) || (rc = alloc(w.x, w.dim, err)) ||
         (rc = alloc(w.r, w.dim, err)) || (rc = alloc(w.p, w.dim, err)) ||
-        (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.dpart, 2 * dparts, err)) ||
-        (rc = alloc(w.dinv, w.dim, err)) ||
-        (rc = alloc(w.ticket, 1, err)))
+        (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.red, red_total, err)) ||
+        (rc = alloc(w.dinv, w.dim, err)))
       return rc;
-    hipError_t e = hipMemsetAsync(w.ticket, 0, sizeof(unsigned), st);
-    if (e != hipSuccess) return hipfail(err, "mixed_prepare", e);
     w.cap_rows = rows * w.Np;
     w.cap_dim = w.dim;
+    w.cap_red = red_total;
   }
   hipLaunchKernelGGL(k_prep_sig, dim3(nb(rows * w.Kp)), dim3(256), 0, st, sigma, ns, n, w.Kp,
                      w.nsp, w.sigA, w.sigT);
@@ -558,23 +541,34 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
 
 int mixed_apply64(MixedWork& w, double shift, const double* omean64, const double* v,
                   double* out, hipStream_t st, std::string& err) {
-  int rc = ensure_scal(w, 0, err);
-  if (rc) return rc;
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, omean64, w.om32, w.dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, v, w.r, w.dim);
-  double* s = w.scal + kSlot * (w.scal_cap - 1);  // scratch slot
-  apply_launches(w, shift, w.r, nullptr, nullptr, nullptr, s, st);
+  apply_launches(w, shift, w.r, nullptr, nullptr, nullptr, st);
   hipLaunchKernelGGL(k_f2d, dim3(nb(w.dim)), dim3(256), 0, st, w.Ap, out, w.dim);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_apply", e);
 }
 
+namespace {
+// host copy of the <r, r> partials, summed in block order
+int host_rr(MixedWork& w, const RedLayout& L, hipStream_t st, double* out, std::string& err) {
+  if ((int)w.host_red.size() < L.nbd) w.host_red.resize(L.nbd);
+  hipError_t e = hipMemcpyAsync(w.host_red.data(), w.red + L.rr, sizeof(double) * L.nbd,
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
+  double s = 0.0;
+  for (int i = 0; i < L.nbd; ++i) s += w.host_red[i];
+  *out = s;
+  return YQ_OK;
+}
+}  // namespace
+
 int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* diag64,
              const double* b, double* x, double tol, int max_iter, int* iters, hipStream_t st,
              std::string& err) {
   const size_t dim = w.dim;
-  int rc = ensure_scal(w, max_iter, err);
-  if (rc) return rc;
+  const RedLayout L = red_layout(w);
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, omean64, w.om32, dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, b, w.r, dim);
   hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.x, dim);
@@ -584,14 +578,13 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
     hipLaunchKernelGGL(k_jacobi_inv, dim3(nb(dim)), dim3(256), 0, st, diag64, shift, w.dinv, dim);
     dinv = w.dinv;
   }
-  // slot t: [0] rho = <r, z>, [1] <p, Ap>, [2] <omean, p>, [3] sum y, [4] <r, r>
-  double* sc = w.scal;
-  hipLaunchKernelGGL(k_dot_rz, dim3(nb(dim)), dim3(256), 0, st, w.r, dinv, dim, w.dpart,
-                     w.ticket, sc);
+  // rho = <r, z> partials alternate between rz0 (even iterations) and rz1
+  double* rz[2] = {w.red + L.rz0, w.red + L.rz1};
+  hipLaunchKernelGGL(k_dot_rz, dim3(L.nbd), dim3(256), 0, st, w.r, dinv, dim, rz[0],
+                     w.red + L.rr);
   double bn2 = 0.0;
-  hipError_t e = hipMemcpyAsync(&bn2, sc + 4, sizeof(double), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
+  int rc = host_rr(w, L, st, &bn2, err);
+  if (rc) return rc;
   *iters = 0;
   if (bn2 == 0.0) {
     hipLaunchKernelGGL(k_f2d, dim3(nb(dim)), dim3(256), 0, st, w.x, x, dim);
@@ -601,17 +594,19 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   const int check = 8;
   int it = 0;
   bool done = false;
+  hipError_t e;
   while (!done && it < max_iter) {
-    double* s = sc + kSlot * it;
-    apply_launches(w, shift, w.r, dinv, s, it > 0 ? s - kSlot : nullptr, s, st);
-    hipLaunchKernelGGL(k_cg_xr, dim3(nb(dim)), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dinv, dim,
-                       s, w.dpart, w.ticket, s + kSlot);
+    const int cur = it & 1;
+    apply_launches(w, shift, w.r, dinv, rz[cur], it > 0 ? rz[cur ^ 1] : nullptr, st);
+    {
+      ProfGuard g("cg_xr", st);
+      hipLaunchKernelGGL(k_cg_xr, dim3(L.nbd), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dinv,
+                         dim, rz[cur], L.nbd, w.red + L.pap, L.nbr, rz[cur ^ 1], w.red + L.rr);
+    }
     ++it;
     if (it % check == 0 || it == max_iter) {
       double rn2 = 0.0;
-      e = hipMemcpyAsync(&rn2, sc + kSlot * it + 4, sizeof(double), hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
+      if ((rc = host_rr(w, L, st, &rn2, err))) return rc;
       if (!(rn2 >= thr)) done = true;  // converged (or NaN: the FP64 check decides)
     }
   }
@@ -624,9 +619,8 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
 
 void mixed_release(MixedWork& w) {
   for (void* p : {(void*)w.sigA, (void*)w.sigT, (void*)w.Trm, (void*)w.Tt, (void*)w.vsplit,
-                  (void*)w.ypart, (void*)w.y, (void*)w.part, (void*)w.om32, (void*)w.x,
-                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.dpart, (void*)w.ticket, (void*)w.dinv,
-                  (void*)w.scal})
+                  (void*)w.ypart, (void*)w.part, (void*)w.om32, (void*)w.x,
+                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.red, (void*)w.dinv})
     if (p) (void)hipFree(p);
   w = MixedWork();
 }

@@ -104,8 +104,9 @@ __global__ void k_add_diag(double* M, int dim, double shift) {
 
 // ---- factored products on FP64 MFMA ---------------------------------------
 // v_mfma_f64_16x16x4f64: lane l holds A[row l%16][k l/16] and B[k l/16][col
-// l%16] and returns D[4*(l/16) + q][l%16], q = 0..3.  The augmented operands
-// are produced on the fly from the int8 spins, T and the parameter vector.
+// l%16] and returns D[(l/16) + 4*q][l%16], q = 0..3 (rows interleaved across
+// the lane groups, unlike the f32-output forms).  The augmented operands are
+// produced on the fly from the int8 spins, T and the parameter vector.
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kF64Tiles = 4;                     // column tiles per forward group
 constexpr int kMaxIt64 = (kMaxSites + 1 + 15) / 16;
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(256)
                                                v_aug(v, n, m, use_bias, 4 * q + lg, j), c, 0, 0,
                                                0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += t_aug(T, ns, m, row0 + 4 * lg + q, j) * c[q];
+    for (int q = 0; q < 4; ++q) acc[q] += t_aug(T, ns, m, row0 + lg + 4 * q, j) * c[q];
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(256)
     for (int s = 8; s >= 1; s >>= 1) acc[q] += __shfl_xor(acc[q], s);
   if (li == 0)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ypart[(size_t)blockIdx.y * nrows + row0 + 4 * lg + q] = acc[q];
+    for (int q = 0; q < 4; ++q) ypart[(size_t)blockIdx.y * nrows + row0 + lg + 4 * q] = acc[q];
 }
 
 __global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows, int ns,
@@ -200,7 +201,7 @@ __global__ void __launch_bounds__(256)
   for (int it = 0; it < kMaxIt64; ++it)
     if (it < nit)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) out[(size_t)(it * 16 + 4 * lg + q) * Np + j] = acc[it][q];
+      for (int q = 0; q < 4; ++q) out[(size_t)(it * 16 + lg + 4 * q) * Np + j] = acc[it][q];
 }
 
 // g[c] = scale * sum_k part[k][aug(c)] on the padded (Kp x Np) grid, mapped to

@@ -38,7 +38,9 @@ struct SampleArgs {
   const double* W64;  // m x n column-major, fp64 master
   const double* b64;  // m
   const void* Wr;     // working-precision W (double or float), m x n col-major
-  const void* ar;     // working-precision a (n)
+                      // (complex machine: real part)
+  const void* Wi;     // complex machine: imaginary part of W, else unused
+  const void* ar;     // working-precision a (n) (complex machine: Re a)
   int8_t* sigma;      // C x n chain state
   uint64_t* ctr;      // C proposal counters
   uint64_t* acc;      // C accepted-proposal counters
@@ -61,9 +63,12 @@ struct ElocArgs {
   const int8_t* sigma;  // Ns x n
   const void* theta;    // Ns x m (double or float)
   const void* WT;       // m x n row-major (WT[j*n+i] = W_ji), double or float
-  const void* ar;       // n, double or float
-  double* ratio;        // Ns x B
-  double* eloc;         // Ns
+                        // (complex machine: real part)
+  const void* WTi;      // complex machine: imaginary part of WT
+  const void* ar;       // n, double or float (complex machine: Re a)
+  const void* ai;       // complex machine: Im a
+  double* ratio;        // Ns x B (complex machine: interleaved complex)
+  double* eloc;         // Ns (complex machine: interleaved complex)
   int ns, n, m, B, use_float;
   yq_ham_desc ham;
 };
@@ -116,6 +121,36 @@ void launch_jacobi_inv64(const double* diag, double shift, double* out, size_t n
                          hipStream_t st);  // out = 1 / (diag + shift)
 void launch_mul(const double* a, const double* b, double* out, size_t n,
                 hipStream_t st);  // out = a * b
+
+// ------------------------------------------------------------------ complex RBM
+// Complex parameters are interleaved (re, im) doubles in the [W | a | b]
+// layout; the kernels read these planes of them.
+struct CplxPlanes {
+  double *Wr64, *Wi64, *WTr64, *WTi64, *ar64, *ai64;  // W: i*m+j, WT: j*n+i
+  float *Wr32, *Wi32, *WTr32, *WTi32, *ar32, *ai32;
+};
+void launch_cplx_planes(const double* params, int n, int m, const CplxPlanes& P,
+                        hipStream_t st);
+// sampler with W64 / b64 interleaved, Wr / Wi / ar planes; theta out interleaved
+void launch_sample_c(const SampleArgs& a, hipStream_t st);
+void launch_theta_batch_c(const double* W64, const double* b64, const int8_t* sigma, int ns,
+                          int n, int m, void* out, int out_float, hipStream_t st);
+size_t eloc_c_lds_bytes(const ElocArgs& a);
+void launch_eloc_c(const ElocArgs& a, hipStream_t st);
+void launch_eloc_stats_c(const double* eloc, int ns, double* out, hipStream_t st);
+// T = tanh(theta), complex, from an interleaved double or float theta
+void launch_tanh_c(const void* theta, int in_float, double* T, size_t count, hipStream_t st);
+// Oh = conj(O), ns x dim complex row-major
+void launch_build_Oh(const int8_t* sigma, const double* T, int ns, int n, int m, int use_bias,
+                     double* Oh, hipStream_t st);
+// Oh -= omean_h; partial[k][c] = sum_{s in chunk k} Oh_sc e_s (complex)
+void launch_center_force_c(double* Oh, const double* omean_h, const double* e, int ns, int dim,
+                           double* partial, hipStream_t st);
+// partial[k][c] = sum_{s in chunk k} |Oh_sc|^2 (real)
+void launch_absq_colsum(const double* Oh, int ns, int dim, double* partial, hipStream_t st);
+void launch_dup2(const double* x, double* y, size_t n, hipStream_t st);  // y[2i] = y[2i+1] = x[i]
+// M = S + shift I, complex dim x dim
+void launch_copy_shift_c(const double* S, double* M, int dim, double shift, hipStream_t st);
 
 // ------------------------------------------------------------------ FeedForward
 struct FFArgs {
