@@ -72,7 +72,7 @@ typedef enum {
   YQ_SR_F = 0,      /* energy gradient F (dim) */
   YQ_SR_OMEAN = 1,  /* column mean of O (dim) */
   YQ_SR_O = 2,      /* centred O (Ns x dim, row-major); explicit/CG only */
-  YQ_SR_S = 3,      /* S (dim x dim); explicit only */
+  YQ_SR_S = 3,      /* S (dim x dim, column-major, full); explicit only */
   YQ_SR_ELOC = 4    /* local energies (Ns) */
 } yq_sr_item;
 

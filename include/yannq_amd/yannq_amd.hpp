@@ -14,6 +14,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <functional>
 #include <map>
 #include <memory>
@@ -42,6 +43,28 @@ inline void check(yq_ctx* c, int rc, int allow = -1) {
   if (rc != YQ_OK && rc != allow)
     throw Error(rc, std::string("yannq_amd: ") + yq_last_error(c));
 }
+// Utilities/type_traits.hpp: is_complex_type / remove_complex
+template <typename T>
+struct is_complex : std::false_type {};
+template <typename T>
+struct is_complex<std::complex<T>> : std::true_type {};
+template <typename T>
+struct real_of {
+  using type = T;
+};
+template <typename T>
+struct real_of<std::complex<T>> {
+  using type = T;
+};
+// the C ABI carries complex entries as interleaved (re, im) doubles
+template <typename T>
+double* dbl(T* p) {
+  return reinterpret_cast<double*>(p);
+}
+template <typename T>
+const double* dbl(const T* p) {
+  return reinterpret_cast<const double*>(p);
+}
 }  // namespace detail
 
 // One GPU context; every object below that lives on the device refers to one.
@@ -61,20 +84,22 @@ class Device {
 };
 
 // ------------------------------------------------------------ Machine
-// Machines/RBM.hpp:22-407 (real double precision).
+// Machines/RBM.hpp:22-407 for T = double or std::complex<double>.
 template <typename T = double>
 class RBM {
-  static_assert(std::is_same<T, double>::value, "this build supports RBM<double>");
+  static_assert(std::is_same<T, double>::value || std::is_same<T, std::complex<double>>::value,
+                "RBM<double> or RBM<std::complex<double>>");
 
  public:
   using Scalar = T;
-  using RealScalar = T;
+  using RealScalar = typename detail::real_of<T>::type;
   using Vector = std::vector<T>;
   using DataT = std::tuple<std::vector<int>, Vector>;
 
   RBM(Device& dev, uint32_t n, uint32_t m, bool useBias = true)
       : dev_(&dev), n_(n), m_(m), useBias_(useBias) {
-    detail::check(ctx(), yq_rbm_create(ctx(), 1, (int)n, (int)m, useBias ? 1 : 0));
+    const int dtype = detail::is_complex<T>::value ? YQ_DTYPE_C128 : YQ_DTYPE_F64;
+    detail::check(ctx(), yq_rbm_create(ctx(), dtype, (int)n, (int)m, useBias ? 1 : 0));
   }
   yq_ctx* ctx() const { return dev_->ctx(); }
 
@@ -89,20 +114,20 @@ class RBM {
 
   Vector getParams() const {  // :271-281
     Vector v(getDim());
-    detail::check(ctx(), yq_rbm_get_params(ctx(), v.data()));
+    detail::check(ctx(), yq_rbm_get_params(ctx(), detail::dbl(v.data())));
     return v;
   }
   void setParams(const Vector& v) {  // :283-291
     if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "setParams: wrong size");
-    detail::check(ctx(), yq_rbm_set_params(ctx(), v.data()));
+    detail::check(ctx(), yq_rbm_set_params(ctx(), detail::dbl(v.data())));
   }
   void updateParams(const Vector& v) {  // :261-269
     if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "updateParams: wrong size");
-    detail::check(ctx(), yq_rbm_update_params(ctx(), v.data()));
+    detail::check(ctx(), yq_rbm_update_params(ctx(), detail::dbl(v.data())));
   }
-  // :301-322; the engine only supplies the Philox key
+  // :301-322 (complex: :327-349); the engine only supplies the Philox key
   template <class RandomEngine>
-  void initializeRandom(RandomEngine& re, T sigma = 1e-3) {
+  void initializeRandom(RandomEngine& re, RealScalar sigma = 1e-3) {
     const uint64_t seed = ((uint64_t)re() << 32) ^ (uint64_t)re();
     detail::check(ctx(), yq_rbm_init_random(ctx(), seed, sigma));
   }
@@ -246,14 +271,14 @@ class SampleBatch {
     if (!sig_.empty()) return;
     sig8_.resize((size_t)ns_ * qs_->getN());
     theta_.resize((size_t)ns_ * qs_->getM());
-    detail::check(qs_->ctx(), yq_samples_get(qs_->ctx(), sig8_.data(), theta_.data()));
+    detail::check(qs_->ctx(), yq_samples_get(qs_->ctx(), sig8_.data(), detail::dbl(theta_.data())));
     sig_.assign(sig8_.begin(), sig8_.end());
   }
   const Machine* qs_;
   int ns_;
   mutable std::vector<int8_t> sig8_;
   mutable std::vector<int> sig_;
-  mutable std::vector<double> theta_;
+  mutable std::vector<typename Machine::Scalar> theta_;
 };
 
 // Samplers/SamplerMT.hpp:15-205 with nTmps == 1.
@@ -357,12 +382,13 @@ class SRMat {
   std::vector<Scalar> corrMat() const {
     const size_t d = qs_.getDim();
     std::vector<Scalar> S(d * d);
-    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_S, S.data()));
+    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_S, detail::dbl(S.data())));
     return S;
   }
   Vector apply(const Vector& rhs) const {  // :170-189
     Vector out(rhs.size());
-    detail::check(ctx(), yq_sr_apply(ctx(), shift_, rhs.data(), out.data()));
+    detail::check(ctx(), yq_sr_apply(ctx(), shift_, detail::dbl(rhs.data()),
+                                     detail::dbl(out.data())));
     return out;
   }
   // :197-223.  Unconverged CG returns its iterate, as Eigen's solve() does.
@@ -372,8 +398,8 @@ class SRMat {
     Vector x(vec.size());
     double res = 0.0;
     int it = 0;
-    detail::check(ctx(), yq_sr_solve(ctx(), shift, tol, 2 * (int)vec.size(), vec.data(),
-                                     x.data(), &res, &it),
+    detail::check(ctx(), yq_sr_solve(ctx(), shift, tol, 2 * (int)vec.size(),
+                                     detail::dbl(vec.data()), detail::dbl(x.data()), &res, &it),
                   YQ_ERR_NOT_CONVERGED);
     lastIters_ = it;
     lastResid_ = res;
@@ -386,7 +412,8 @@ class SRMat {
     Vector x(qs_.getDim());
     double res = 0.0;
     int it = 0;
-    detail::check(ctx(), yq_sr_solve(ctx(), shift, 0.0, 0, rhs.data(), x.data(), &res, &it));
+    detail::check(ctx(), yq_sr_solve(ctx(), shift, 0.0, 0, detail::dbl(rhs.data()),
+                                     detail::dbl(x.data()), &res, &it));
     lastResid_ = res;
     return x;
   }
@@ -398,7 +425,7 @@ class SRMat {
   yq_ctx* ctx() const { return qs_.ctx(); }
   Vector item(int which) const {
     Vector v(qs_.getDim());
-    detail::check(ctx(), yq_sr_get(ctx(), which, v.data()));
+    detail::check(ctx(), yq_sr_get(ctx(), which, detail::dbl(v.data())));
     return v;
   }
   const Machine& qs_;
@@ -516,7 +543,7 @@ class RunRBM {
     const int dim = (int)getDim();
     if (nSweeps == -1) nSweeps = dim;
     if (nSweepsDiscard == -1) nSweepsDiscard = int(0.1 * nSweeps);
-    Vector gradAvg(dim, 0.0);
+    Vector gradAvg(dim, T(0.0));
     int t = 0;
     for (int ll = 0; ll <= maxIter_; ll++) {
       logger_ << "Epochs: " << ll << "\t# of discard samples: " << nSweepsDiscard << std::endl;
@@ -558,7 +585,7 @@ class RunRBM {
  private:
   static double norm(const Vector& v) {
     double s = 0.0;
-    for (double x : v) s += x * x;
+    for (const T& x : v) s += std::norm(x);
     return std::sqrt(s);
   }
   std::ostream& logger_;

@@ -250,8 +250,8 @@ class Context:
         self._chk(self._L.yq_sr_get(self._h, item, _ptr(out)))
         if item == SR_O:
             return out.reshape(self.ns, self.dim)
-        if item == SR_S:
-            return out.reshape(self.dim, self.dim)
+        if item == SR_S:  # column-major on the ABI
+            return out.reshape(self.dim, self.dim).T.copy()
         return out
 
     def sr_apply(self, shift, v):

@@ -1136,7 +1136,7 @@ YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {
   const size_t dim = c->dim;
   if (c->cplx) {
     // the device holds conj(O_c), conj(omean) and the lower triangle of S
-    // (column-major); the host gets O_c, omean and the full S (row-major)
+    // (column-major); the host gets O_c, omean and the full S (column-major)
     switch (item) {
       case YQ_SR_F:
         HIPCHK(c, hipMemcpyAsync(out, c->F.p, sizeof(double) * 2 * dim, hipMemcpyDeviceToHost,
@@ -1158,12 +1158,12 @@ YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {
         HIPCHK(c, hipMemcpyAsync(h.data(), c->S.p, sizeof(double) * h.size(),
                                  hipMemcpyDeviceToHost, c->st));
         TRY(sync(c));
-        for (size_t i = 0; i < dim; ++i)
-          for (size_t j = 0; j < dim; ++j) {
+        for (size_t j = 0; j < dim; ++j)
+          for (size_t i = 0; i < dim; ++i) {  // S_ij = conj(S_ji) above the diagonal
             const bool low = i >= j;
             const size_t src = low ? i + j * dim : j + i * dim;
-            out[2 * (i * dim + j)] = h[2 * src];
-            out[2 * (i * dim + j) + 1] = low ? h[2 * src + 1] : -h[2 * src + 1];
+            out[2 * (i + j * dim)] = h[2 * src];
+            out[2 * (i + j * dim) + 1] = low ? h[2 * src + 1] : -h[2 * src + 1];
           }
         return YQ_OK;
       }
