@@ -186,9 +186,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // ypart[cg][s] = sum over the column tiles of group cg of T'_sj U'_sj.
-// Block: 4 waves x 64 rows (4 row tiles per wave); blockIdx.y = column group
-// of kColTiles tiles whose three V' layers are staged once in LDS (rows padded
-// by kLdsPad elements against bank conflicts) and shared by the 4 waves.
+// The product is formed transposed, U'^T = V'^T sigma'^T (A = V'^T from LDS,
+// B = sigma'^T: the 8 consecutive k of one sample), so a lane's four results
+// belong to one sample and four consecutive columns: the T' epilogue is one
+// 16-byte load and one accumulator per sample tile, folded over the four lane
+// groups at the end.  Block: 4 waves x 64 samples (4 sample tiles per wave);
+// blockIdx.y = column group of kColTiles tiles whose three V' layers are
+// staged once in LDS (rows padded by kLdsPad elements against bank
+// conflicts) and shared by the 4 waves.
 constexpr int kLdsPad = 8;
 __global__ void __launch_bounds__(256)
     k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
@@ -213,48 +218,44 @@ __global__ void __launch_bounds__(256)
   const int li = l & 15, lg = l >> 4;
   const int rbase = blockIdx.x * 256 + wave * 64;
   const int nks = Kp / 32;
-  bf16x8 afr[4][kMaxKs];
+  bf16x8 sfr[4][kMaxKs];
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+  for (int st = 0; st < 4; ++st)
 #pragma unroll
     for (int ks = 0; ks < kMaxKs; ++ks)
-      if (ks < nks) afr[rt][ks] = ld8(sigA + (size_t)(rbase + rt * 16 + li) * Kp + ks * 32 + lg * 8);
-  float acc[4][4];
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[rt][q] = 0.f;
+      if (ks < nks) sfr[st][ks] = ld8(sigA + (size_t)(rbase + st * 16 + li) * Kp + ks * 32 + lg * 8);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
   for (int ct = 0; ct < ncols / 16; ++ct) {
     floatx4 c[4];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt) c[rt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    for (int st = 0; st < 4; ++st) c[st] = (floatx4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int sp = 0; sp < 3; ++sp)
 #pragma unroll
       for (int ks = 0; ks < kMaxKs; ++ks)
         if (ks < nks) {
-          const bf16x8 b =
+          const bf16x8 v =
               ld8(bsh + (sp * kColTiles * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
 #pragma unroll
-          for (int rt = 0; rt < 4; ++rt)
-            c[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[rt][ks], b, c[rt], 0, 0, 0);
+          for (int st = 0; st < 4; ++st)
+            c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v, sfr[st][ks], c[st], 0, 0, 0);
         }
-    const int j = col0 + ct * 16 + li;
+    // c[st][q] = U'^T[col0 + ct*16 + 4*lg + q][rbase + st*16 + li]
+    const int j0 = col0 + ct * 16 + 4 * lg;
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        acc[rt][q] += Trm[(size_t)(rbase + rt * 16 + 4 * lg + q) * Np + j] * c[rt][q];
+    for (int st = 0; st < 4; ++st) {
+      const floatx4 t =
+          *reinterpret_cast<const floatx4*>(Trm + (size_t)(rbase + st * 16 + li) * Np + j0);
+      acc[st] += ((t[0] * c[st][0] + t[1] * c[st][1]) + (t[2] * c[st][2] + t[3] * c[st][3]));
+    }
   }
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      float v = acc[rt][q];
-#pragma unroll
-      for (int s = 8; s >= 1; s >>= 1) v += __shfl_xor(v, s);
-      if (li == 0) ypart[(size_t)cg * nsp + rbase + rt * 16 + 4 * lg + q] = v;
-    }
+  for (int st = 0; st < 4; ++st) {
+    float v = acc[st];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    if (lg == 0) ypart[(size_t)cg * nsp + rbase + st * 16 + li] = v;
+  }
 }
 
 // part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj with the centred
@@ -262,15 +263,16 @@ __global__ void __launch_bounds__(256)
 // formed in LDS for the chunk; blocks of column group 0 also store the chunk
 // sum of y (ys, the centring correction of the reduction).  Block: 4 waves =
 // 4 column tiles of one chunk; the chunk's sigma'^T slab is staged in LDS in
-// two halves of 128 samples and shared by the waves; every wave keeps all
-// Kp/16 row tiles in registers.
-constexpr int kBwdHalf = 128;
+// slices of kBwdHalf samples (sized by Kp, so the LDS per block stays small
+// and several blocks share a CU) and shared by the waves; every wave keeps
+// all Kp/16 row tiles in registers.
+constexpr int kBwdHalf = 64;  // samples per LDS slice of the sigma'^T slab
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
                int nop, int Kp, int Np, int nsp, float* __restrict__ part,
                double* __restrict__ ys) {
-  __shared__ uint16_t ash[160 * (kBwdHalf + kLdsPad)];
+  extern __shared__ uint16_t ash[];  // [Kp][kBwdHalf + kLdsPad]
   __shared__ float ysh[kBwdChunk];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
@@ -486,7 +488,8 @@ void apply_launches(MixedWork& w, double shift, const float* r, const float* din
   const int chunks = w.nsp / kBwdChunk;
   {
     ProfGuard g("bwd_mfma", st);
-    hipLaunchKernelGGL(k_bwd_mfma, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), 0, st, w.sigT,
+    const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdHalf + kLdsPad);
+    hipLaunchKernelGGL(k_bwd_mfma, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds, st, w.sigT,
                        w.Tt, w.ypart, w.ncg, w.ns, op, L.nbv, w.Kp, w.Np, w.nsp, w.part, ys);
   }
   ProfGuard g("bwd_reduce", st);
