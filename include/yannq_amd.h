@@ -178,6 +178,20 @@ YQ_API int yq_sr_solve(yq_ctx* ctx, double shift, double tol, int max_iter,
  * the previous solution.  Both leave the solved system and its tolerance
  * unchanged. */
 YQ_API int yq_sr_set_options(yq_ctx* ctx, int precond, int warm_start);
+/* Launch mode of the mixed-precision CG (no reference counterpart): 1, the
+ * default, replays chunks of 8 iterations as a captured HIP graph that stops
+ * itself on the device once converged; 0 issues the same kernels one by one.
+ * Both perform identical arithmetic and return identical results. */
+YQ_API int yq_set_graph_mode(yq_ctx* ctx, int use_graphs);
+/* Tile-shape autotuning of the mixed-precision products (no reference
+ * counterpart): 1 times every (forward column group, backward chunk) pair on
+ * the first operands of each shape and keeps the fastest for the process;
+ * 0, the default, uses fixed shapes.  The choice changes the order of the
+ * float sums inside the products, so results agree to float rounding. */
+YQ_API int yq_set_autotune(yq_ctx* ctx, int on);
+/* Fix the tile shapes instead (reproducible sums across processes):
+ * col_tiles in {2, 4, 8}, chunk in {128, 256, 512}; 0, 0 releases them. */
+YQ_API int yq_set_tiles(yq_ctx* ctx, int col_tiles, int chunk);
 
 /* ---- Runner: Runners/RunRBM.hpp:151-220 --------------------------------- */
 typedef struct {
@@ -208,6 +222,42 @@ typedef struct {
 /* One epoch body: [randomize], sample, constructFromSamples, solve, SGD
  * update, all on the device. */
 YQ_API int yq_sr_step(yq_ctx* ctx, const yq_step_cfg* cfg, yq_step_out* out);
+
+/* ---- Many machines in one context (no reference counterpart) -------------
+ * A set of independent RBMs (each with its own Hamiltonian, chains and Philox
+ * seed), as one would run many RunRBM jobs side by side.  Sampling is one
+ * persistent launch over every (machine, chain) pair, ordered by cost so the
+ * waves stay balanced across uneven sizes; local energies are one launch per
+ * pass for all machines; the SR solves run concurrently, one host worker
+ * thread and one HIP stream per machine.  Each machine behaves exactly like
+ * a single-machine context with the same parameters and seed (the same
+ * chains, samples and energies). */
+typedef struct {
+  int n, m, use_bias;  /* RBM(n, m, useBias), real double parameters */
+  int chains;          /* nChainsPer */
+  int sweep_factor;    /* the sweeper's nSweep */
+  uint64_t seed;       /* Philox key of this machine's chains */
+  yq_ham_desc ham;
+} yq_multi_desc;
+/* Create count machines; all share the sweeper kind and sampling precision. */
+YQ_API int yq_multi_create(yq_ctx* ctx, int count, const yq_multi_desc* descs, int sweeper,
+                           int precision);
+YQ_API int yq_multi_set_params(yq_ctx* ctx, int idx, const double* params);
+YQ_API int yq_multi_get_params(yq_ctx* ctx, int idx, double* params);
+/* initializeRandom for every machine; machine i is keyed by seed + i */
+YQ_API int yq_multi_init_random(yq_ctx* ctx, uint64_t seed, double sigma);
+/* sample(nSweeps, nDiscard) for every machine in one launch */
+YQ_API int yq_multi_sample(yq_ctx* ctx, int n_sweeps, int n_discard);
+YQ_API int yq_multi_samples_get(yq_ctx* ctx, int idx, int8_t* sigma, double* theta);
+YQ_API int yq_multi_sampler_state(yq_ctx* ctx, int idx, int8_t* sigma, uint64_t* ctr,
+                                  uint64_t* acc);
+/* E_loc of every machine's batch; mean[i], var[i] (either may be null) */
+YQ_API int yq_multi_eloc(yq_ctx* ctx, double* mean, double* var);
+/* One epoch body (see yq_sr_step) for every machine: out[count]; sampling
+ * and E_loc as above, then the per-machine SR on up to n_threads host
+ * threads (0: one per machine, at most 16). */
+YQ_API int yq_multi_sr_step(yq_ctx* ctx, const yq_step_cfg* cfg, yq_step_out* out,
+                            int n_threads);
 
 /* ---- FeedForward machine: Machines/FeedForward.hpp --------------------- */
 /* The network of the C4 configuration: Conv1D(1 -> channels, kernel width,

@@ -1,7 +1,9 @@
 """Solver options on the GPU: Jacobi-preconditioned CG and warm starts solve
 the same system as the Cholesky path (the oracle's or_llt_solve) to the
-requested tolerance, in fewer iterations; the FP64 factored products on
-MFMA f64 agree with the dense operator."""
+requested tolerance (the Jacobi iteration count is pinned against a numpy
+PCG; on these SR matrices it is higher than plain CG's); the FP64 factored
+products on MFMA f64 agree with the dense operator; the graph-captured mixed
+CG matches the eagerly launched one bit for bit."""
 import numpy as np
 import pytest
 
@@ -127,3 +129,74 @@ def test_sr_step_options(ctx):
         assert o["cg_err"] < 1e-8 and np.isfinite(o["energy"])
         its.append(o["cg_iters"])
     assert min(its) > 0
+
+
+def test_graph_chunks_match_eager(ctx, orc):
+    """The mixed CG replayed as captured 8-iteration graphs (stopping on the
+    device) and the same kernels issued one by one give bit-identical
+    solutions and iteration counts, over repeated solves with the FP64
+    refinement running between the graph replays, and honour an iteration
+    budget that is not a multiple of the chunk."""
+    n, m, ns = 40, 80, 3000
+    _batch(ctx, orc, n, m, ns, scale=0.05)
+    rng = np.random.default_rng(12)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig, precision=1)
+    ctx.sr_build(2, precision=1)
+    out = {False: [], True: []}
+    for mode in (False, True, False, True):
+        ctx.graph_mode(mode)
+        x, res, it = ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000)
+        assert res < 1e-8
+        out[mode].append((x, it))
+    for mode in (False, True):
+        ctx.graph_mode(mode)
+        x, res, it = ctx.sr_solve(1e-3, tol=1e-8, max_iter=13, allow_unconverged=True)
+        assert it == 13 and res > 1e-8
+        out[mode].append((x, it))
+    ctx.graph_mode(True)
+    for (xe, ie), (xg, ig) in zip(out[False], out[True]):
+        assert ie == ig
+        np.testing.assert_array_equal(xe, xg)
+    # repeated solves of the same system are identical too
+    np.testing.assert_array_equal(out[True][0][0], out[True][1][0])
+
+
+@pytest.mark.parametrize("n,m,ns", [(100, 400, 2048), (40, 80, 3000)])
+def test_tile_shapes_and_autotune(ctx, orc, n, m, ns):
+    """Every tile shape of the mixed products (forward column group, backward
+    chunk) computes the same operator to float accuracy, and the autotuned
+    solve converges to the same answer."""
+    ctx.rbm_create(n, m)
+    rng = np.random.default_rng(14)
+    rbm = orc.RBM.random(n, m, 0.05, rng)
+    ctx.set_params(rbm.params())
+    ctx.ham_set(0, n, J=-1.0, h=-1.0)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig, precision=1)
+    th = orc.theta_replay(rbm, sig).astype(np.float32).astype(np.float64)
+    T = np.tanh(th)
+    Ow = (sig[:, :, None] * T[:, None, :]).reshape(ns, -1)
+    Od = np.concatenate([Ow, sig.astype(float), T], axis=1)
+    Oc = Od - Od.mean(axis=0)
+    v = np.random.default_rng(4).normal(size=Od.shape[1])
+    ref = Oc.T @ (Oc @ v) / ns + 1e-3 * v
+    kp = (n + 1 + 31) // 32 * 32
+    for ct in (2, 4, 8):
+        if 2 * 3 * ct * 16 * (kp + 8) > 64 * 1024:
+            with pytest.raises(Exception):
+                ctx.tiles(ct, 256)
+                ctx.sr_build(2, precision=1)
+            continue
+        for chunk in (128, 256, 512):
+            ctx.tiles(ct, chunk)
+            ctx.sr_build(2, precision=1)
+            assert _rel(ctx.sr_apply(1e-3, v), ref) < 2e-6, (ct, chunk)
+    ctx.tiles(0, 0)
+    ctx.sr_build(2, precision=1)
+    x0, r0, _ = ctx.sr_solve(1e-3, tol=1e-8, max_iter=20000)
+    ctx.autotune(True)
+    ctx.sr_build(2, precision=1)
+    x1, r1, _ = ctx.sr_solve(1e-3, tol=1e-8, max_iter=20000)
+    ctx.autotune(False)
+    assert r0 < 1e-8 and r1 < 1e-8 and _rel(x1, x0) < 1e-5

@@ -41,6 +41,11 @@ class HamDesc(C.Structure):
                 ("delta", C.c_double), ("J2", C.c_double), ("sign_rule", C.c_int)]
 
 
+class MultiDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("use_bias", C.c_int), ("chains", C.c_int),
+                ("sweep_factor", C.c_int), ("seed", C.c_uint64), ("ham", HamDesc)]
+
+
 class FFDesc(C.Structure):
     _fields_ = [("n", C.c_int), ("channels", C.c_int), ("kernel", C.c_int),
                 ("hidden", C.c_int), ("dtype", C.c_int)]
@@ -93,6 +98,12 @@ def lib():
         "yq_sr_solve": [p, d, d, i, p, p, p, p], "yq_sr_step": [p, p, p],
         "yq_prof_enable": [p, C.c_char_p], "yq_prof_read": [p, C.c_char_p, p, p],
         "yq_sr_set_options": [p, i, i],
+        "yq_set_graph_mode": [p, i], "yq_set_autotune": [p, i], "yq_set_tiles": [p, i, i],
+        "yq_multi_create": [p, i, p, i, i], "yq_multi_set_params": [p, i, p],
+        "yq_multi_get_params": [p, i, p], "yq_multi_init_random": [p, C.c_uint64, d],
+        "yq_multi_sample": [p, i, i], "yq_multi_samples_get": [p, i, p, p],
+        "yq_multi_sampler_state": [p, i, p, p, p], "yq_multi_eloc": [p, p, p],
+        "yq_multi_sr_step": [p, p, p, i],
         "yq_ff_create": [p, p], "yq_ff_npar": [p, p], "yq_ff_set_params": [p, p],
         "yq_ff_get_params": [p, p], "yq_ff_update_params": [p, p],
         "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
@@ -315,8 +326,84 @@ class Context:
     def ff_vjp_dev(self, ns, d_sigma, d_act, d_r, d_grad):
         self._chk(self._L.yq_ff_vjp_dev(self._h, ns, d_sigma, d_act, d_r, d_grad))
 
+    # many machines in one context
+    def multi_create(self, machines, sweeper=LOCAL, precision=MIXED):
+        """machines: list of dicts n, m, chains, seed, ham=(kind, kwargs) and
+        optional use_bias (default True) and sweep_factor (default 1)."""
+        arr = (MultiDesc * len(machines))()
+        for q, mc in enumerate(machines):
+            kind, hk = mc["ham"]
+            hd = HamDesc(kind, mc["n"], hk.get("J", -1.0), hk.get("h", -1.0),
+                         hk.get("delta", 1.0), hk.get("J2", 0.0), hk.get("sign_rule", 0))
+            arr[q] = MultiDesc(mc["n"], mc["m"], int(mc.get("use_bias", True)), mc["chains"],
+                               mc.get("sweep_factor", 1), mc["seed"], hd)
+        self._chk(self._L.yq_multi_create(self._h, len(machines), arr, sweeper, precision))
+        self.multi = [dict(mc) for mc in machines]
+        for mc in self.multi:
+            mc["dim"] = mc["n"] * mc["m"] + (mc["n"] + mc["m"] if mc.get("use_bias", True) else 0)
+
+    def multi_set_params(self, idx, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.multi[idx]["dim"],)
+        self._chk(self._L.yq_multi_set_params(self._h, idx, _ptr(v)))
+
+    def multi_get_params(self, idx):
+        out = np.empty(self.multi[idx]["dim"])
+        self._chk(self._L.yq_multi_get_params(self._h, idx, _ptr(out)))
+        return out
+
+    def multi_init_random(self, seed, sigma):
+        self._chk(self._L.yq_multi_init_random(self._h, seed, sigma))
+
+    def multi_sample(self, n_sweeps, n_discard):
+        self._chk(self._L.yq_multi_sample(self._h, n_sweeps, n_discard))
+        self.multi_sweeps = n_sweeps
+
+    def multi_samples(self, idx):
+        mc = self.multi[idx]
+        ns = mc["chains"] * self.multi_sweeps
+        s = np.empty((ns, mc["n"]), np.int8)
+        t = np.empty((ns, mc["m"]))
+        self._chk(self._L.yq_multi_samples_get(self._h, idx, _ptr(s), _ptr(t)))
+        return s, t
+
+    def multi_state(self, idx):
+        mc = self.multi[idx]
+        s = np.empty((mc["chains"], mc["n"]), np.int8)
+        ctr = np.empty(mc["chains"], np.uint64)
+        acc = np.empty(mc["chains"], np.uint64)
+        self._chk(self._L.yq_multi_sampler_state(self._h, idx, _ptr(s), _ptr(ctr), _ptr(acc)))
+        return s, ctr, acc
+
+    def multi_eloc(self):
+        k = len(self.multi)
+        mean, var = np.empty(k), np.empty(k)
+        self._chk(self._L.yq_multi_eloc(self._h, _ptr(mean), _ptr(var)))
+        return mean, var
+
+    def multi_sr_step(self, n_threads=0, **kw):
+        cfg = StepCfg(kw.get("n_sweeps", 1), kw.get("n_discard", 0), int(kw.get("randomize", 0)),
+                      kw.get("nup", -1), kw.get("solver", SOLVE_CG_FACTORED),
+                      kw.get("precision", MIXED), kw.get("lam", 1e-3), kw.get("cg_tol", 1e-8),
+                      kw.get("cg_max_iter", 20000), kw.get("lr", 0.01), kw.get("grad_clip", 0.0),
+                      int(kw.get("precond", 0)), int(kw.get("warm_start", 0)))
+        outs = (StepOut * len(self.multi))()
+        self._chk(self._L.yq_multi_sr_step(self._h, C.byref(cfg), outs, n_threads),
+                  allow=(ERR_NOT_CONVERGED,) if kw.get("allow_unconverged") else ())
+        self.multi_sweeps = kw.get("n_sweeps", 1)
+        return [{f: getattr(o, f) for f, _ in StepOut._fields_} for o in outs]
+
     def sr_options(self, precond=0, warm_start=0):
         self._chk(self._L.yq_sr_set_options(self._h, precond, warm_start))
+
+    def graph_mode(self, use_graphs=True):
+        self._chk(self._L.yq_set_graph_mode(self._h, int(use_graphs)))
+
+    def autotune(self, on=True):
+        self._chk(self._L.yq_set_autotune(self._h, int(on)))
+
+    def tiles(self, col_tiles=0, chunk=0):
+        self._chk(self._L.yq_set_tiles(self._h, col_tiles, chunk))
 
     def sr_step(self, **kw):
         cfg = StepCfg(kw.get("n_sweeps", 1), kw.get("n_discard", 0), int(kw.get("randomize", 0)),

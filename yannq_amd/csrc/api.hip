@@ -7,7 +7,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mixed.h"
@@ -35,11 +38,13 @@ struct ProfState {
 };
 ProfState g_prof;
 bool g_prof_any = false;
+std::mutex g_prof_mu;  // worker threads of yq_multi_sr_step share the registry
 }  // namespace
 
 namespace yq {
 bool prof_enabled(const char* label) { return g_prof_any && g_prof.find(label) >= 0; }
 void prof_begin(const char* label, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
   ProfEntry& e = g_prof.entries[g_prof.find(label)];
   hipEvent_t ev;
   if (hipEventCreate(&ev) != hipSuccess) return;
@@ -47,6 +52,7 @@ void prof_begin(const char* label, hipStream_t st) {
   e.open.push_back(ev);
 }
 void prof_end(const char* label, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
   ProfEntry& e = g_prof.entries[g_prof.find(label)];
   if (e.open.empty()) return;
   hipEvent_t ev;
@@ -113,6 +119,11 @@ struct yq_ctx {
   bool have_xprev = false;
   int ones_len = 0;
 
+  // many machines (yq_multi_*): one sub-context per machine
+  std::vector<yq_ctx*> subs;
+  int multi_kind = 0, multi_prec = 0, m_nitems = 0, m_npad = 16;
+  Buf m_sargs, m_items, m_eargs, m_stats;
+
   // FeedForward machine
   bool have_ff = false;
   yq_ff_desc ff{};
@@ -125,7 +136,7 @@ struct yq_ctx {
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp,
                    &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
-                   &xprev, &ones, &cpd, &cpf})
+                   &xprev, &ones, &cpd, &cpf, &m_sargs, &m_items, &m_eargs, &m_stats})
       bufs.push_back(b);
   }
 };
@@ -239,7 +250,8 @@ int dot(yq_ctx* c, const double* x, const double* y, size_t n, double* out) {
 }
 
 // ---------------------------------------------------------------- E_loc
-int compute_eloc(yq_ctx* c) {
+// Arguments of the E_loc kernels for the current batch; allocates outputs.
+int eloc_args(yq_ctx* c, ElocArgs* out) {
   if (!c->have_batch) return fail(c, YQ_ERR_STATE, "no samples: call yq_sample or yq_samples_set");
   if (!c->have_ham) return fail(c, YQ_ERR_STATE, "no Hamiltonian: call yq_ham_set");
   if (c->ham.n != c->n) return fail(c, YQ_ERR_ARG, "Hamiltonian size differs from the machine's n");
@@ -247,7 +259,8 @@ int compute_eloc(yq_ctx* c) {
   TRY(ensure(c, c->ratio, sizeof(double) * ncomp(c) * (size_t)c->ns * B));
   TRY(ensure(c, c->eloc, sizeof(double) * ncomp(c) * (size_t)c->ns));
   TRY(ensure(c, c->estat, sizeof(double) * 2));
-  ElocArgs a{};
+  ElocArgs& a = *out;
+  a = ElocArgs{};
   a.sigma = c->bsig.as<int8_t>();
   a.theta = c->btheta.p;
   a.use_float = c->bprec == YQ_MIXED;
@@ -266,15 +279,21 @@ int compute_eloc(yq_ctx* c) {
     a.WTi = a.use_float ? (const void*)P.WTi32 : (const void*)P.WTi64;
     a.ar = a.use_float ? (const void*)P.ar32 : (const void*)P.ar64;
     a.ai = a.use_float ? (const void*)P.ai32 : (const void*)P.ai64;
+  }
+  return YQ_OK;
+}
+
+int compute_eloc(yq_ctx* c) {
+  ElocArgs a;
+  TRY(eloc_args(c, &a));
+  if (c->cplx) {
     if (eloc_c_lds_bytes(a) > 64 * 1024)
       return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
     launch_eloc_c(a, c->st);
     LAUNCHCHK(c);
     launch_eloc_stats_c(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
   } else {
-    const size_t rs = a.use_float ? 4 : 8;
-    const size_t lds = (size_t)(256 / B + 2) * c->m * 2 * rs + (size_t)(256 / B + 2) * c->n;
-    if (lds > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
+    if (eloc_lds_bytes(a) > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
     launch_eloc(a, c->st);
     LAUNCHCHK(c);
     launch_eloc_stats(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
@@ -790,6 +809,8 @@ YQ_API int yq_ctx_create(int device, yq_ctx** out) {
 
 YQ_API int yq_ctx_destroy(yq_ctx* c) {
   if (!c) return YQ_OK;
+  for (yq_ctx* sub : c->subs) yq_ctx_destroy(sub);
+  c->subs.clear();
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   mixed_release(c->mixed);
@@ -1009,7 +1030,11 @@ YQ_API int yq_sampler_get_state(yq_ctx* c, int8_t* sigma, uint64_t* ctr, uint64_
   return sync(c);
 }
 
-YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
+}  // extern "C"
+
+namespace {
+// Arguments of one sample() call; allocates the batch.  The caller launches.
+int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   TRY(need_rbm(c));
   if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
   if (n_sweeps < 1 || n_discard < 0) return fail(c, YQ_ERR_ARG, "n_sweeps >= 1, n_discard >= 0");
@@ -1020,7 +1045,8 @@ YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
   const size_t rs = c->sprec == YQ_MIXED ? sizeof(float) : sizeof(double);
   TRY(ensure(c, c->bsig, (size_t)ns * c->n));
   TRY(ensure(c, c->btheta, rs * ncomp(c) * (size_t)ns * c->m));
-  SampleArgs a{};
+  SampleArgs& a = *out;
+  a = SampleArgs{};
   a.W64 = W64(c);
   a.b64 = b64(c);
   const bool f32 = c->sprec == YQ_MIXED;
@@ -1048,14 +1074,26 @@ YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
   a.mixed = c->sprec == YQ_MIXED;
   a.beta = 1.0;
   a.seed = c->seed;
+  return YQ_OK;
+}
+void batch_ready(yq_ctx* c) {
+  c->have_batch = true;
+  c->have_eloc = false;
+  c->have_sr = false;
+}
+}  // namespace
+
+extern "C" {
+
+YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
+  SampleArgs a;
+  TRY(sample_args(c, n_sweeps, n_discard, &a));
   if (c->cplx) launch_sample_c(a, c->st);
   else launch_sample(a, c->st);
   LAUNCHCHK(c);
   TRY(sync(c));
-  c->have_batch = true;
-  c->have_eloc = false;
-  c->have_sr = false;
-  if (n_samples) *n_samples = (int)ns;
+  batch_ready(c);
+  if (n_samples) *n_samples = c->ns;
   return YQ_OK;
 }
 
@@ -1503,6 +1541,35 @@ YQ_API int yq_prof_read(yq_ctx* c, const char* label, double* total_ms, long lon
   return YQ_OK;
 }
 
+YQ_API int yq_set_graph_mode(yq_ctx* c, int use_graphs) {
+  if (!c) return YQ_ERR_ARG;
+  c->mixed.use_graphs = use_graphs != 0;
+  for (yq_ctx* sub : c->subs) sub->mixed.use_graphs = use_graphs != 0;
+  return YQ_OK;
+}
+
+YQ_API int yq_set_tiles(yq_ctx* c, int col_tiles, int chunk) {
+  if (!c) return YQ_ERR_ARG;
+  const bool off = col_tiles == 0 && chunk == 0;
+  if (!off && !((col_tiles == 2 || col_tiles == 4 || col_tiles == 8) &&
+                (chunk == 128 || chunk == 256 || chunk == 512)))
+    return fail(c, YQ_ERR_ARG, "tiles: col_tiles in {2, 4, 8}, chunk in {128, 256, 512}");
+  for (yq_ctx* x : c->subs) {
+    x->mixed.pin_ct = col_tiles;
+    x->mixed.pin_chunk = chunk;
+  }
+  c->mixed.pin_ct = col_tiles;
+  c->mixed.pin_chunk = chunk;
+  return YQ_OK;
+}
+
+YQ_API int yq_set_autotune(yq_ctx* c, int on) {
+  if (!c) return YQ_ERR_ARG;
+  c->mixed.autotune = on != 0;
+  for (yq_ctx* sub : c->subs) sub->mixed.autotune = on != 0;
+  return YQ_OK;
+}
+
 YQ_API int yq_sr_set_options(yq_ctx* c, int precond, int warm_start) {
   if (!c) return YQ_ERR_ARG;
   if (precond != 0 && precond != 1) return fail(c, YQ_ERR_ARG, "precond must be 0 or 1");
@@ -1513,6 +1580,33 @@ YQ_API int yq_sr_set_options(yq_ctx* c, int precond, int warm_start) {
 }
 
 // ---------------------------------------------------------------- runner
+}  // extern "C"
+
+namespace {
+// constructFromSamples + solve + SGD update of RunRBM.hpp:176-212 on the
+// current batch (E_loc already computed); *solve_rc keeps a
+// YQ_ERR_NOT_CONVERGED of the solve while the update still happens.
+int step_update(yq_ctx* c, const yq_step_cfg* cfg, double* nv, double* err, int* it,
+                int* solve_rc) {
+  TRY(sr_build(c, cfg->solver, cfg->precision));
+  TRY(ensure(c, c->vx, sizeof(double) * vlen(c)));
+  int rc = solve(c, cfg->lambda, cfg->cg_tol, cfg->cg_max_iter, c->F.as<double>(),
+                 c->vx.as<double>(), err, it);
+  if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+  *solve_rc = rc;
+  double vv;
+  TRY(dot(c, c->vx.as<double>(), c->vx.as<double>(), vlen(c), &vv));
+  *nv = sqrt(vv);
+  double scale = -cfg->lr;
+  if (cfg->grad_clip != 0.0 && *nv > cfg->grad_clip) scale *= cfg->grad_clip / *nv;
+  launch_axpy(scale, c->vx.as<double>(), c->params.as<double>(), vlen(c), c->st);
+  LAUNCHCHK(c);
+  return refresh_params(c);
+}
+}  // namespace
+
+extern "C" {
+
 YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
   TRY(need_rbm(c));
   if (!cfg || !out) return fail(c, YQ_ERR_ARG, "null cfg/out");
@@ -1535,21 +1629,9 @@ YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
     if ((rc = yq_sample(c, cfg->n_sweeps, cfg->n_discard, &ns)) != YQ_OK) break;
     if ((rc = hipEventRecord(e1, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
     if ((rc = compute_eloc(c)) != YQ_OK) break;
-    if ((rc = sr_build(c, cfg->solver, cfg->precision)) != YQ_OK) break;
-    int it = 0;
-    double err = 0.0;
-    if ((rc = ensure(c, c->vx, sizeof(double) * vlen(c))) != YQ_OK) break;
-    rc = solve(c, cfg->lambda, cfg->cg_tol, cfg->cg_max_iter, c->F.as<double>(),
-               c->vx.as<double>(), &err, &it);
-    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) break;
-    const int solve_rc = rc;
-    double vv;
-    if ((rc = dot(c, c->vx.as<double>(), c->vx.as<double>(), vlen(c), &vv)) != YQ_OK) break;
-    const double nv = sqrt(vv);
-    double scale = -cfg->lr;
-    if (cfg->grad_clip != 0.0 && nv > cfg->grad_clip) scale *= cfg->grad_clip / nv;
-    launch_axpy(scale, c->vx.as<double>(), c->params.as<double>(), vlen(c), c->st);
-    if ((rc = refresh_params(c)) != YQ_OK) break;
+    int it = 0, solve_rc = YQ_OK;
+    double err = 0.0, nv = 0.0;
+    if ((rc = step_update(c, cfg, &nv, &err, &it, &solve_rc)) != YQ_OK) break;
     if ((rc = hipEventRecord(e2, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP)) break;
     if ((rc = sync(c)) != YQ_OK) break;
     float t01 = 0, t12 = 0;
@@ -1571,6 +1653,254 @@ YQ_API int yq_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out) {
   } while (0);
   cleanup();
   return rc;
+}
+
+// ---------------------------------------------------------------- many machines
+}  // extern "C"
+
+namespace {
+int need_multi(yq_ctx* c) {
+  if (!c) return YQ_ERR_ARG;
+  if (c->subs.empty()) return fail(c, YQ_ERR_STATE, "call yq_multi_create first");
+  return YQ_OK;
+}
+// status of a sub-context call, with its message moved to the owner
+int from_sub(yq_ctx* c, yq_ctx* sub, int rc, int idx) {
+  if (rc == YQ_OK) return rc;
+  return fail(c, rc, "machine " + std::to_string(idx) + ": " + sub->err);
+}
+void drop_subs(yq_ctx* c) {
+  for (yq_ctx* sub : c->subs) yq_ctx_destroy(sub);
+  c->subs.clear();
+}
+// machines ordered by decreasing cost of a sweep (n * sweep factor * m)
+std::vector<int> cost_order(const yq_ctx* c) {
+  std::vector<int> ord(c->subs.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+  auto cost = [&](int i) {
+    const yq_ctx* s = c->subs[i];
+    return (long long)s->n * s->sweep_factor * s->m;
+  };
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  return ord;
+}
+int multi_sample(yq_ctx* c, int n_sweeps, int n_discard) {
+  const int count = (int)c->subs.size();
+  std::vector<SampleArgs> args(count);
+  for (int i = 0; i < count; ++i)
+    TRY(from_sub(c, c->subs[i], sample_args(c->subs[i], n_sweeps, n_discard, &args[i]), i));
+  TRY(ensure(c, c->m_sargs, sizeof(SampleArgs) * count));
+  HIPCHK(c, hipMemcpyAsync(c->m_sargs.p, args.data(), sizeof(SampleArgs) * count,
+                           hipMemcpyHostToDevice, c->st));
+  launch_sample_multi(c->m_sargs.as<SampleArgs>(), c->m_items.as<int2>(), c->m_nitems,
+                      c->m_npad, c->multi_kind, c->multi_prec == YQ_MIXED, 0, c->st);
+  LAUNCHCHK(c);
+  TRY(sync(c));
+  for (yq_ctx* sub : c->subs) batch_ready(sub);
+  return YQ_OK;
+}
+int multi_eloc(yq_ctx* c) {
+  const int count = (int)c->subs.size();
+  std::vector<ElocArgs> args(count);
+  size_t lds = 0;
+  int max_blocks = 0, max_ns = 0;
+  for (int i = 0; i < count; ++i) {
+    yq_ctx* sub = c->subs[i];
+    TRY(from_sub(c, sub, eloc_args(sub, &args[i]), i));
+    lds = std::max(lds, eloc_lds_bytes(args[i]));
+    const size_t pairs = (size_t)args[i].ns * args[i].B;
+    max_blocks = std::max(max_blocks, (int)((pairs + 255) / 256));
+    max_ns = std::max(max_ns, args[i].ns);
+  }
+  if (lds > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
+  TRY(ensure(c, c->m_eargs, sizeof(ElocArgs) * count));
+  TRY(ensure(c, c->m_stats, sizeof(double) * 2 * count));
+  HIPCHK(c, hipMemcpyAsync(c->m_eargs.p, args.data(), sizeof(ElocArgs) * count,
+                           hipMemcpyHostToDevice, c->st));
+  launch_eloc_multi(c->m_eargs.as<ElocArgs>(), count, max_blocks, max_ns, lds,
+                    c->multi_prec == YQ_MIXED, c->m_stats.as<double>(), c->st);
+  LAUNCHCHK(c);
+  std::vector<double> h(2 * count);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->m_stats.p, sizeof(double) * h.size(),
+                           hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  for (int i = 0; i < count; ++i) {
+    c->subs[i]->e_mean = h[2 * i];
+    c->subs[i]->e_var = h[2 * i + 1];
+    c->subs[i]->have_eloc = true;
+  }
+  return YQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+YQ_API int yq_multi_create(yq_ctx* c, int count, const yq_multi_desc* d, int sweeper,
+                           int precision) {
+  if (!c) return YQ_ERR_ARG;
+  if (count < 1 || !d) return fail(c, YQ_ERR_ARG, "need count >= 1 and descriptors");
+  if (sweeper != YQ_LOCAL && sweeper != YQ_EXCHANGE) return fail(c, YQ_ERR_ARG, "unknown sweeper");
+  if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
+  drop_subs(c);
+  for (int i = 0; i < count; ++i) {
+    yq_ctx* sub = nullptr;
+    int rc = yq_ctx_create(c->device, &sub);
+    if (rc != YQ_OK) {
+      drop_subs(c);
+      return fail(c, rc, "cannot create the context of machine " + std::to_string(i));
+    }
+    c->subs.push_back(sub);
+    sub->mixed.use_graphs = c->mixed.use_graphs;
+    sub->mixed.autotune = c->mixed.autotune;
+    rc = yq_rbm_create(sub, YQ_DTYPE_F64, d[i].n, d[i].m, d[i].use_bias);
+    if (rc == YQ_OK) rc = yq_ham_set(sub, &d[i].ham);
+    if (rc == YQ_OK)
+      rc = yq_sampler_config(sub, sweeper, 1, d[i].chains, d[i].sweep_factor, precision,
+                             d[i].seed);
+    if (rc != YQ_OK) {
+      rc = from_sub(c, sub, rc, i);
+      drop_subs(c);
+      return rc;
+    }
+  }
+  c->multi_kind = sweeper;
+  c->multi_prec = precision;
+  // (machine, chain) items, the costliest machines first
+  std::vector<int2> items;
+  int npad = 16;
+  for (int i : cost_order(c)) {
+    const yq_ctx* sub = c->subs[i];
+    npad = std::max(npad, (sub->n + 15) & ~15);
+    for (int ch = 0; ch < sub->chains; ++ch) items.push_back(make_int2(i, ch));
+  }
+  c->m_nitems = (int)items.size();
+  c->m_npad = npad;
+  TRY(ensure(c, c->m_items, sizeof(int2) * items.size()));
+  HIPCHK(c, hipMemcpyAsync(c->m_items.p, items.data(), sizeof(int2) * items.size(),
+                           hipMemcpyHostToDevice, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_multi_set_params(yq_ctx* c, int idx, const double* p) {
+  TRY(need_multi(c));
+  if (idx < 0 || idx >= (int)c->subs.size()) return fail(c, YQ_ERR_ARG, "machine index");
+  return from_sub(c, c->subs[idx], yq_rbm_set_params(c->subs[idx], p), idx);
+}
+
+YQ_API int yq_multi_get_params(yq_ctx* c, int idx, double* p) {
+  TRY(need_multi(c));
+  if (idx < 0 || idx >= (int)c->subs.size()) return fail(c, YQ_ERR_ARG, "machine index");
+  return from_sub(c, c->subs[idx], yq_rbm_get_params(c->subs[idx], p), idx);
+}
+
+YQ_API int yq_multi_init_random(yq_ctx* c, uint64_t seed, double sigma) {
+  TRY(need_multi(c));
+  for (size_t i = 0; i < c->subs.size(); ++i)
+    TRY(from_sub(c, c->subs[i], yq_rbm_init_random(c->subs[i], seed + i, sigma), (int)i));
+  return YQ_OK;
+}
+
+YQ_API int yq_multi_sample(yq_ctx* c, int n_sweeps, int n_discard) {
+  TRY(need_multi(c));
+  return multi_sample(c, n_sweeps, n_discard);
+}
+
+YQ_API int yq_multi_samples_get(yq_ctx* c, int idx, int8_t* sigma, double* theta) {
+  TRY(need_multi(c));
+  if (idx < 0 || idx >= (int)c->subs.size()) return fail(c, YQ_ERR_ARG, "machine index");
+  return from_sub(c, c->subs[idx], yq_samples_get(c->subs[idx], sigma, theta), idx);
+}
+
+YQ_API int yq_multi_sampler_state(yq_ctx* c, int idx, int8_t* sigma, uint64_t* ctr,
+                                  uint64_t* acc) {
+  TRY(need_multi(c));
+  if (idx < 0 || idx >= (int)c->subs.size()) return fail(c, YQ_ERR_ARG, "machine index");
+  return from_sub(c, c->subs[idx], yq_sampler_get_state(c->subs[idx], sigma, ctr, acc), idx);
+}
+
+YQ_API int yq_multi_eloc(yq_ctx* c, double* mean, double* var) {
+  TRY(need_multi(c));
+  TRY(multi_eloc(c));
+  for (size_t i = 0; i < c->subs.size(); ++i) {
+    if (mean) mean[i] = c->subs[i]->e_mean;
+    if (var) var[i] = c->subs[i]->e_var;
+  }
+  return YQ_OK;
+}
+
+YQ_API int yq_multi_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out,
+                            int n_threads) {
+  TRY(need_multi(c));
+  if (!cfg || !out) return fail(c, YQ_ERR_ARG, "null cfg/out");
+  const int count = (int)c->subs.size();
+  for (int i = 0; i < count; ++i) {
+    yq_ctx* sub = c->subs[i];
+    TRY(from_sub(c, sub, yq_sr_set_options(sub, cfg->precond, cfg->warm_start), i));
+    if (cfg->randomize) TRY(from_sub(c, sub, yq_sampler_randomize(sub, cfg->nup), i));
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int rc = hipEventRecord(e0, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP;
+  if (rc == YQ_OK) rc = multi_sample(c, cfg->n_sweeps, cfg->n_discard);
+  if (rc == YQ_OK) rc = hipEventRecord(e1, c->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP;
+  if (rc == YQ_OK) rc = sync(c);
+  float smp = 0.f;
+  if (rc == YQ_OK) (void)hipEventElapsedTime(&smp, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  TRY(rc);
+  TRY(multi_eloc(c));
+  // per-machine SR on worker threads, costliest machines first
+  const std::vector<int> ord = cost_order(c);
+  std::vector<int> rcs(count, YQ_OK);
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    (void)hipSetDevice(c->device);
+    for (int q = next++; q < count; q = next++) {
+      const int i = ord[q];
+      yq_ctx* sub = c->subs[i];
+      hipEvent_t s0 = nullptr, s1 = nullptr;
+      int r = hipEventCreate(&s0) == hipSuccess && hipEventCreate(&s1) == hipSuccess
+                  ? YQ_OK : YQ_ERR_HIP;
+      int it = 0, solve_rc = YQ_OK;
+      double err = 0.0, nv = 0.0;
+      if (r == YQ_OK) r = hipEventRecord(s0, sub->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP;
+      if (r == YQ_OK) r = step_update(sub, cfg, &nv, &err, &it, &solve_rc);
+      if (r == YQ_OK) r = hipEventRecord(s1, sub->st) == hipSuccess ? YQ_OK : YQ_ERR_HIP;
+      if (r == YQ_OK) r = sync(sub);
+      float slv = 0.f;
+      if (r == YQ_OK) (void)hipEventElapsedTime(&slv, s0, s1);
+      if (s0) (void)hipEventDestroy(s0);
+      if (s1) (void)hipEventDestroy(s1);
+      yq_step_out& o = out[i];
+      o.energy = sub->e_mean;
+      o.energy_var = sub->e_var;
+      o.update_norm = nv;
+      o.cg_err = err;
+      o.cg_iters = it;
+      o.n_samples = sub->ns;
+      o.sample_ms = smp;
+      o.solve_ms = slv;
+      if (r == YQ_OK && (!std::isfinite(nv) || !std::isfinite(sub->e_mean)))
+        r = fail(sub, YQ_ERR_NAN, "non-finite energy or update");
+      rcs[i] = r == YQ_OK ? solve_rc : r;
+    }
+  };
+  const int nt = std::max(1, std::min(count, n_threads > 0 ? n_threads : 16));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  (void)hipSetDevice(c->device);
+  int worst = YQ_OK;
+  for (int i = 0; i < count; ++i)
+    if (rcs[i] != YQ_OK && rcs[i] != YQ_ERR_NOT_CONVERGED)
+      return from_sub(c, c->subs[i], rcs[i], i);
+    else if (rcs[i] == YQ_ERR_NOT_CONVERGED)
+      worst = rcs[i];
+  if (worst != YQ_OK) return fail(c, worst, "an iterative solve reached max_iter");
+  return YQ_OK;
 }
 
 }  // extern "C"

@@ -34,14 +34,14 @@ __device__ inline void bond_sites(int kind, int n, int b, int& k, int& l) {
   }
 }
 
+// One 256-thread block of (sample, bond) pairs starting at pair `first`.
 template <typename R>
-__global__ void __launch_bounds__(256) k_eloc_ratio(const ElocArgs A, int span) {
-  extern __shared__ unsigned char lds_raw[];
+__device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
+                                 unsigned char* lds_raw) {
   R* th = reinterpret_cast<R*>(lds_raw);
   R* lc = th + (size_t)span * A.m;
   int8_t* sg = reinterpret_cast<int8_t*>(lc + (size_t)span * A.m);
   const int n = A.n, m = A.m, B = A.B;
-  const size_t first = (size_t)blockIdx.x * 256;
   const size_t total = (size_t)A.ns * B;
   const int s_lo = (int)(first / B);
   const size_t last = first + 255 < total ? first + 255 : total - 1;
@@ -92,9 +92,23 @@ __global__ void __launch_bounds__(256) k_eloc_ratio(const ElocArgs A, int span) 
   A.ratio[idx] = (double)Math<R>::exp(res);
 }
 
-__global__ void __launch_bounds__(256) k_eloc_combine(const ElocArgs A) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.ns) return;
+template <typename R>
+__global__ void __launch_bounds__(256) k_eloc_ratio(const ElocArgs A, int span) {
+  extern __shared__ unsigned char lds_raw[];
+  eloc_ratio_block<R>(A, span, (size_t)blockIdx.x * 256, lds_raw);
+}
+
+// many machines: blockIdx.y selects the machine (its own span of samples per block)
+template <typename R>
+__global__ void __launch_bounds__(256) k_eloc_ratio_multi(const ElocArgs* __restrict__ inst) {
+  extern __shared__ unsigned char lds_raw[];
+  const ElocArgs A = inst[blockIdx.y];
+  const size_t first = (size_t)blockIdx.x * 256;
+  if (first >= (size_t)A.ns * A.B) return;  // block-uniform
+  eloc_ratio_block<R>(A, 256 / A.B + 2, first, lds_raw);
+}
+
+__device__ void eloc_combine_one(const ElocArgs& A, int s) {
   const int n = A.n;
   const int8_t* sg = A.sigma + (size_t)s * n;
   const double* ratio = A.ratio + (size_t)s * A.B;
@@ -128,9 +142,18 @@ __global__ void __launch_bounds__(256) k_eloc_combine(const ElocArgs A) {
   A.eloc[s] = e;
 }
 
+__global__ void __launch_bounds__(256) k_eloc_combine(const ElocArgs A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < A.ns) eloc_combine_one(A, s);
+}
+__global__ void __launch_bounds__(256) k_eloc_combine_multi(const ElocArgs* __restrict__ inst) {
+  const ElocArgs A = inst[blockIdx.y];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < A.ns) eloc_combine_one(A, s);
+}
+
 // mean and variance in one block, fixed reduction order
-__global__ void __launch_bounds__(1024) k_eloc_stats(const double* e, int ns,
-                                                     double* out) {
+__device__ void eloc_stats_block(const double* e, int ns, double* out) {
   __shared__ double s1[1024], s2[1024];
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < ns; i += 1024) {
@@ -153,6 +176,15 @@ __global__ void __launch_bounds__(1024) k_eloc_stats(const double* e, int ns,
     out[1] = s2[0] / ns - mean * mean;
   }
 }
+__global__ void __launch_bounds__(1024) k_eloc_stats(const double* e, int ns, double* out) {
+  eloc_stats_block(e, ns, out);
+}
+// one block per machine: out[2 i], out[2 i + 1]
+__global__ void __launch_bounds__(1024) k_eloc_stats_multi(const ElocArgs* __restrict__ inst,
+                                                           double* out) {
+  const ElocArgs A = inst[blockIdx.x];
+  eloc_stats_block(A.eloc, A.ns, out + 2 * blockIdx.x);
+}
 
 }  // namespace
 
@@ -171,6 +203,27 @@ void launch_eloc(const ElocArgs& a, hipStream_t st) {
   }
   ProfGuard g("eloc_combine", st);
   hipLaunchKernelGGL(k_eloc_combine, dim3((a.ns + 255) / 256), dim3(256), 0, st, a);
+}
+
+size_t eloc_lds_bytes(const ElocArgs& a) {
+  const int span = 256 / a.B + 2;
+  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
+  return (size_t)span * a.m * 2 * rs + (size_t)span * a.n + 16;
+}
+
+void launch_eloc_multi(const ElocArgs* inst_dev, int count, int max_blocks, int max_ns,
+                       size_t lds, int use_float, double* stats, hipStream_t st) {
+  {
+    ProfGuard g("eloc_ratio_multi", st);
+    const dim3 grid((unsigned)max_blocks, (unsigned)count);
+    if (use_float)
+      hipLaunchKernelGGL(k_eloc_ratio_multi<float>, grid, dim3(256), lds, st, inst_dev);
+    else
+      hipLaunchKernelGGL(k_eloc_ratio_multi<double>, grid, dim3(256), lds, st, inst_dev);
+  }
+  hipLaunchKernelGGL(k_eloc_combine_multi, dim3((max_ns + 255) / 256, count), dim3(256), 0, st,
+                     inst_dev);
+  hipLaunchKernelGGL(k_eloc_stats_multi, dim3(count), dim3(1024), 0, st, inst_dev, stats);
 }
 
 void launch_eloc_stats(const double* eloc, int ns, double* out,

@@ -15,7 +15,7 @@ struct MixedWork {
   int Kp = 0;    // n+1 rounded up to 32 (k-steps of the forward product)
   int Np = 0;    // m+1 rounded up to 16 (column tiles)
   int nsp = 0;   // ns rounded up to 512
-  int ncg = 0;   // column groups of the forward product
+  int ncg = 0;   // column groups of the forward product (Np/16 over ct)
   size_t dim = 0;
   // operands prepared once per SR build
   uint16_t* sigA = nullptr;  // bf16 sigma' row-major  nsp x Kp
@@ -32,7 +32,22 @@ struct MixedWork {
   float* dinv = nullptr;       // Jacobi preconditioner 1 / (diag S + shift)
   double* red = nullptr;       // per-block partials of the fused dot products
   std::vector<double> host_red;
-  size_t cap_rows = 0, cap_dim = 0, cap_red = 0;
+  size_t cap_rows = 0, cap_kp = 0, cap_np = 0, cap_dim = 0, cap_red = 0;
+  // tile shapes of the products: forward column tiles per group (2, 4, 8) and
+  // backward samples per chunk (128, 256, 512); chosen per operand shape by
+  // the autotuner when enabled
+  int ct = 4, chunk = 256;
+  bool autotune = false;
+  int pin_ct = 0, pin_chunk = 0;  // shapes fixed by the caller (0: not fixed)
+  // CG control block (device), its pinned host mirror and poll events
+  double* dctl = nullptr;      // shift, stopping threshold
+  int* ictl = nullptr;         // stop flag, completed iterations, budget
+  int* host_ctl = nullptr;     // 2 slots of 4 ints
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  // captured CG chunks (index: Jacobi preconditioner off / on) and their shape
+  bool use_graphs = true;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  long gkey[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // Build the bf16 / float operands from the batch (sigma: ns x n int8,
@@ -50,5 +65,7 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
              const double* b, double* x, double tol, int max_iter, int* iters,
              hipStream_t st, std::string& err);
 void mixed_release(MixedWork& w);
+// drop the captured CG chunks (after a reallocation or a shape change)
+void mixed_drop_graphs(MixedWork& w);
 
 }  // namespace yq

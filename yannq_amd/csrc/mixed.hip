@@ -17,6 +17,10 @@
 // next kernel that needs the value sums itself in a fixed order.
 #include <math.h>
 
+#include <array>
+#include <map>
+#include <mutex>
+
 #include "mixed.h"
 #include "yq_internal.h"
 
@@ -27,11 +31,17 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kBwdChunk = 256;  // samples per backward chunk
+constexpr int kBwdChunkMin = 128;  // smallest backward chunk (sizes the partials)
 constexpr int kRowPad = 512;    // ns padding
 constexpr int kMaxKs = 5;       // Kp <= 160 (n <= 159)
 constexpr int kMaxIt = 10;      // Kp / 16
-constexpr int kColTiles = 4;    // column tiles per forward group
+constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
+constexpr int kChunkIters = 8;  // CG iterations per captured graph / host poll
+
+// device control block of the CG loop: ints [stop flag, completed iterations,
+// iteration budget] and doubles [shift, stopping threshold on <r, r>]
+enum { kCtlStop = 0, kCtlIters = 1, kCtlMaxIt = 2 };
+enum { kCtlShift = 0, kCtlThr = 1 };
 
 __device__ __host__ inline uint16_t f2bf(float x) {
   uint32_t u;
@@ -149,6 +159,7 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
 __global__ void __launch_bounds__(256)
     k_split_v(const float* __restrict__ r, const float* __restrict__ dinv, float* __restrict__ p,
               const double* __restrict__ rz_now, const double* __restrict__ rz_prev, int nrz,
+              const double* __restrict__ rr, const double* __restrict__ dctl, int* ictl,
               const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
               uint16_t* __restrict__ vs, double* __restrict__ op) {
   __shared__ uint16_t tile[3][16][17];
@@ -156,7 +167,17 @@ __global__ void __launch_bounds__(256)
   const int j0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   const size_t nm = (size_t)n * m;
   float beta = 0.0f;
-  if (rz_prev) beta = (float)(psum(rz_now, nrz) / psum(rz_prev, nrz));
+  if (rz_prev) {
+    // CG continuation predicate (uniform over the grid): stop once <r, r>
+    // falls below the threshold (or is NaN) or the iteration budget is spent;
+    // the later kernels of the iteration see the flag and return at once
+    const double rn2 = psum(rr, nrz);
+    if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
+      if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) ictl[kCtlStop] = 1;
+      return;
+    }
+    beta = (float)(psum(rz_now, nrz) / psum(rz_prev, nrz));
+  }
   {
     const int kk = t >> 4, jj = t & 15;
     const int k = k0 + kk, j = j0 + jj;
@@ -191,26 +212,28 @@ __global__ void __launch_bounds__(256)
 // belong to one sample and four consecutive columns: the T' epilogue is one
 // 16-byte load and one accumulator per sample tile, folded over the four lane
 // groups at the end.  Block: 4 waves x 64 samples (4 sample tiles per wave);
-// blockIdx.y = column group of kColTiles tiles whose three V' layers are
+// blockIdx.y = column group of CT tiles whose three V' layers are
 // staged once in LDS (rows padded by kLdsPad elements against bank
 // conflicts) and shared by the 4 waves.
 constexpr int kLdsPad = 8;
+template <int CT>
 __global__ void __launch_bounds__(256)
     k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
                const float* __restrict__ Trm, int Kp, int Np, int nsp,
-               float* __restrict__ ypart) {
-  extern __shared__ uint16_t bsh[];  // [3][kColTiles*16][Kp + kLdsPad]
+               float* __restrict__ ypart, const int* __restrict__ ictl) {
+  extern __shared__ uint16_t bsh[];  // [3][CT*16][Kp + kLdsPad]
+  if (ictl[kCtlStop]) return;
   const int tid = threadIdx.x;
   const int cg = blockIdx.y;
-  const int col0 = cg * kColTiles * 16;
-  const int ncols = min(kColTiles * 16, Np - col0);
+  const int col0 = cg * CT * 16;
+  const int ncols = min(CT * 16, Np - col0);
   const int ldk = Kp + kLdsPad;
   const int kchunks = Kp / 8;
   for (int q = tid; q < 3 * ncols * kchunks; q += 256) {
     const int sp = q / (ncols * kchunks);
     const int rem = q % (ncols * kchunks);
     const int col = rem / kchunks, kk = (rem % kchunks) * 8;
-    *reinterpret_cast<u16x8*>(bsh + (sp * kColTiles * 16 + col) * ldk + kk) =
+    *reinterpret_cast<u16x8*>(bsh + (sp * CT * 16 + col) * ldk + kk) =
         *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp + kk);
   }
   __syncthreads();
@@ -235,7 +258,7 @@ __global__ void __launch_bounds__(256)
       for (int ks = 0; ks < kMaxKs; ++ks)
         if (ks < nks) {
           const bf16x8 v =
-              ld8(bsh + (sp * kColTiles * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
+              ld8(bsh + (sp * CT * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
 #pragma unroll
           for (int st = 0; st < 4; ++st)
             c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v, sfr[st][ks], c[st], 0, 0, 0);
@@ -267,13 +290,15 @@ __global__ void __launch_bounds__(256)
 // and several blocks share a CU) and shared by the waves; every wave keeps
 // all Kp/16 row tiles in registers.
 constexpr int kBwdHalf = 64;  // samples per LDS slice of the sigma'^T slab
+template <int CHUNK>
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
                int nop, int Kp, int Np, int nsp, float* __restrict__ part,
-               double* __restrict__ ys) {
+               double* __restrict__ ys, const int* __restrict__ ictl) {
+  if (ictl[kCtlStop]) return;
   extern __shared__ uint16_t ash[];  // [Kp][kBwdHalf + kLdsPad]
-  __shared__ float ysh[kBwdChunk];
+  __shared__ float ysh[CHUNK];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
   const int ct = blockIdx.x * 4 + wave;
@@ -284,19 +309,23 @@ __global__ void __launch_bounds__(256)
   const int lda = kBwdHalf + kLdsPad;
   {
     const float ov = (float)psum(op, nop);
-    const int s = kc * kBwdChunk + tid;  // blockDim == kBwdChunk
-    float v = 0.0f;
-    for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
-    const float yv = s < ns ? (v - ov) / (float)ns : 0.0f;
-    ysh[tid] = yv;
-    if (blockIdx.x == 0) block_partial((double)yv, ys, kc);  // uniform per block
+    double dv = 0.0;
+    for (int q = tid; q < CHUNK; q += 256) {
+      const int s = kc * CHUNK + q;
+      float v = 0.0f;
+      for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
+      const float yv = s < ns ? (v - ov) / (float)ns : 0.0f;
+      ysh[q] = yv;
+      dv += (double)yv;
+    }
+    if (blockIdx.x == 0) block_partial(dv, ys, kc);  // uniform per block
   }
   floatx4 acc[kMaxIt];
 #pragma unroll
   for (int it = 0; it < kMaxIt; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
   const float* trow = Tt + (size_t)(active ? ct * 16 + li : 0) * nsp;
-  for (int half = 0; half < kBwdChunk / kBwdHalf; ++half) {
-    const int sbase = kc * kBwdChunk + half * kBwdHalf;
+  for (int half = 0; half < CHUNK / kBwdHalf; ++half) {
+    const int sbase = kc * CHUNK + half * kBwdHalf;
     __syncthreads();
     for (int q = tid; q < Kp * (kBwdHalf / 8); q += 256) {
       const int i = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
@@ -310,7 +339,7 @@ __global__ void __launch_bounds__(256)
       u16x8 bh, bm, bl;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const float v = trow[s0 + e] * ysh[s0 - kc * kBwdChunk + e];
+        const float v = trow[s0 + e] * ysh[s0 - kc * CHUNK + e];
         uint16_t h, md, lo;
         split3(v, h, md, lo);
         bh[e] = h;
@@ -346,10 +375,12 @@ __global__ void __launch_bounds__(256)
 // pap = partials of <p, out>.
 __global__ void __launch_bounds__(256)
     k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m, int use_bias,
-                 int Kp, int Np, const float* __restrict__ om, double shift,
+                 int Kp, int Np, const float* __restrict__ om, const double* __restrict__ dctl,
                  const double* __restrict__ ys, int nys, const float* __restrict__ p,
-                 float* __restrict__ out, double* __restrict__ pap) {
+                 float* __restrict__ out, double* __restrict__ pap, const int* __restrict__ ictl) {
   __shared__ float grp[4][64];
+  if (ictl[kCtlStop]) return;
+  const double shift = dctl[kCtlShift];
   const size_t nm = (size_t)n * m;
   const size_t dim = use_bias ? nm + n + m : nm;
   const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -379,8 +410,10 @@ __global__ void __launch_bounds__(256)
     k_cg_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
             const float* __restrict__ Ap, const float* __restrict__ dinv, size_t n,
             const double* __restrict__ rz_now, int nrz, const double* __restrict__ pap, int npap,
-            double* __restrict__ rz_next, double* __restrict__ rr_out) {
+            double* __restrict__ rz_next, double* __restrict__ rr_out, int* ictl) {
+  if (ictl[kCtlStop]) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ictl[kCtlIters] += 1;  // one writer per launch
   const double pv = psum(pap, npap);
   const float alpha = pv != 0.0 ? (float)(psum(rz_now, nrz) / pv) : 0.0f;
   double rz = 0.0, rr = 0.0;
@@ -419,6 +452,16 @@ __global__ void k_jacobi_inv(const double* __restrict__ diag, double shift, floa
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (float)(1.0 / (diag[i] + shift));
 }
+// control block of a CG run (a kernel, so the reset is also valid in a graph)
+__global__ void k_set_ctl(double* dctl, int* ictl, double shift, double thr, int max_it) {
+  if (threadIdx.x == 0) {
+    dctl[kCtlShift] = shift;
+    dctl[kCtlThr] = thr;
+    ictl[kCtlStop] = 0;
+    ictl[kCtlIters] = 0;
+    ictl[kCtlMaxIt] = max_it;
+  }
+}
 __global__ void k_zero_f(float* __restrict__ a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0f;
@@ -449,11 +492,12 @@ struct RedLayout {
   int nbd, nbv, nby, nbr;
   size_t rz0, rz1, rr, op, ys, pap, total;
 };
-RedLayout red_layout(const MixedWork& w) {
+RedLayout red_layout(const MixedWork& w, int chunk = 0) {
   RedLayout L;
+  if (chunk == 0) chunk = w.chunk;
   L.nbd = (int)nb(w.dim);                     // elementwise kernels over dim
   L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_split_v tiles
-  L.nby = w.nsp / kBwdChunk;                  // k_bwd_mfma chunks
+  L.nby = w.nsp / chunk;                      // k_bwd_mfma chunks
   L.nbr = (int)((w.dim + 63) / 64);           // k_bwd_reduce
   L.rz0 = 0;
   L.rz1 = L.rz0 + L.nbd;
@@ -466,39 +510,161 @@ RedLayout red_layout(const MixedWork& w) {
 }
 
 // (S + shift) p -> Ap with the products on MFMA, leaving the partials of
-// <p, Ap> in w.red.  p is first updated to z + beta p (see k_split_v) with
-// rho from the partials rz_now / rz_prev.
-void apply_launches(MixedWork& w, double shift, const float* r, const float* dinv,
-                    const double* rz_now, const double* rz_prev, hipStream_t st) {
+// <p, Ap> in w.red; shift and the CG predicate come from the control block.
+// p is first updated to z + beta p (see k_split_v) with rho from the
+// partials rz_now / rz_prev.
+void apply_launches(MixedWork& w, const float* r, const float* dinv, const double* rz_now,
+                    const double* rz_prev, hipStream_t st) {
   const RedLayout L = red_layout(w);
   double* op = w.red + L.op;
   double* ys = w.red + L.ys;
   {
     ProfGuard g("split_v", st);
     hipLaunchKernelGGL(k_split_v, dim3(w.Np / 16, w.Kp / 16), dim3(256), 0, st, r, dinv, w.p,
-                       rz_now, rz_prev, L.nbd, w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np,
-                       w.vsplit, op);
+                       rz_now, rz_prev, L.nbd, w.red + L.rr, w.dctl, w.ictl, w.om32, w.n, w.m,
+                       w.use_bias, w.Kp, w.Np, w.vsplit, op);
   }
   {
     ProfGuard g("fwd_mfma", st);
-    const size_t lds = sizeof(uint16_t) * 3 * kColTiles * 16 * (w.Kp + kLdsPad);
-    hipLaunchKernelGGL(k_fwd_mfma, dim3(w.nsp / 256, w.ncg), dim3(256), lds, st, w.sigA,
-                       w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart);
+    const size_t lds = sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad);
+    auto fwd = w.ct == 2 ? k_fwd_mfma<2> : w.ct == 8 ? k_fwd_mfma<8> : k_fwd_mfma<4>;
+    hipLaunchKernelGGL(fwd, dim3(w.nsp / 256, w.ncg), dim3(256), lds, st, w.sigA,
+                       w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl);
   }
-  const int chunks = w.nsp / kBwdChunk;
+  const int chunks = w.nsp / w.chunk;
   {
     ProfGuard g("bwd_mfma", st);
     const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdHalf + kLdsPad);
-    hipLaunchKernelGGL(k_bwd_mfma, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds, st, w.sigT,
-                       w.Tt, w.ypart, w.ncg, w.ns, op, L.nbv, w.Kp, w.Np, w.nsp, w.part, ys);
+    auto bwd = w.chunk == 128 ? k_bwd_mfma<128> : w.chunk == 512 ? k_bwd_mfma<512>
+                                                                 : k_bwd_mfma<256>;
+    hipLaunchKernelGGL(bwd, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds, st, w.sigT,
+                       w.Tt, w.ypart, w.ncg, w.ns, op, L.nbv, w.Kp, w.Np, w.nsp, w.part, ys,
+                       w.ictl);
   }
   ProfGuard g("bwd_reduce", st);
   hipLaunchKernelGGL(k_bwd_reduce, dim3(L.nbr), dim3(256), 0, st, w.part, chunks, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.om32, shift, ys, L.nby, w.p, w.Ap,
-                     w.red + L.pap);
+                     w.use_bias, w.Kp, w.Np, w.om32, w.dctl, ys, L.nby, w.p, w.Ap,
+                     w.red + L.pap, w.ictl);
+}
+
+// one CG iteration: the operator, then x, r and the next rho
+void iteration_launches(MixedWork& w, const float* dinv, int cur, bool first, hipStream_t st) {
+  const RedLayout L = red_layout(w);
+  double* rz[2] = {w.red + L.rz0, w.red + L.rz1};
+  apply_launches(w, w.r, dinv, rz[cur], first ? nullptr : rz[cur ^ 1], st);
+  ProfGuard g("cg_xr", st);
+  hipLaunchKernelGGL(k_cg_xr, dim3(L.nbd), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dinv, w.dim,
+                     rz[cur], L.nbd, w.red + L.pap, L.nbr, rz[cur ^ 1], w.red + L.rr, w.ictl);
+}
+
+// shape of the captured launches (a different shape needs a new capture)
+void graph_key(const MixedWork& w, long key[10]) {
+  const long k[10] = {w.ns, w.n, w.m, w.use_bias, w.Kp, w.Np, w.nsp, (long)w.dim, w.ct, w.chunk};
+  for (int i = 0; i < 10; ++i) key[i] = k[i];
+}
+
+// kChunkIters iterations (absolute iterations 1 + 8q + p use rho slot
+// (1 + p) & 1) captured once per operand shape and preconditioner setting
+int mixed_graph(MixedWork& w, const float* dinv, hipStream_t st, hipGraphExec_t* out,
+                std::string& err) {
+  long key[10];
+  graph_key(w, key);
+  bool same = true;
+  for (int i = 0; i < 10; ++i) same = same && key[i] == w.gkey[i];
+  if (!same) {
+    mixed_drop_graphs(w);
+    for (int i = 0; i < 10; ++i) w.gkey[i] = key[i];
+  }
+  const int slot = dinv ? 1 : 0;
+  if (!w.gexec[slot]) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return hipfail(err, "graph capture", e);
+    for (int p = 0; p < kChunkIters; ++p) iteration_launches(w, dinv, (1 + p) & 1, false, st);
+    e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) return hipfail(err, "graph capture", e);
+    e = hipGraphInstantiate(&w.gexec[slot], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      w.gexec[slot] = nullptr;
+      return hipfail(err, "graph instantiate", e);
+    }
+  }
+  *out = w.gexec[slot];
+  return YQ_OK;
+}
+
+// Tile-shape autotuner: times every (forward column tiles, backward chunk)
+// pair on the prepared operands (3 applies after a warm-up, HIP events) and
+// keeps the fastest per (Kp, Np, nsp) for the life of the process.  The
+// shapes change only the order of float sums, not what is computed.
+std::map<std::array<int, 3>, std::pair<int, int>> g_tuned;
+std::mutex g_tuned_mu;
+
+int tune(MixedWork& w, hipStream_t st, std::string& err) {
+  const std::array<int, 3> key{w.Kp, w.Np, w.nsp};
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      w.ct = it->second.first;
+      w.chunk = it->second.second;
+      return YQ_OK;
+    }
+  }
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(w.dim)), dim3(256), 0, st, w.r, w.dim);
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(w.dim)), dim3(256), 0, st, w.p, w.dim);
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(w.dim)), dim3(256), 0, st, w.om32, w.dim);
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, 1e-3, 0.0, 1);
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e != hipSuccess) return hipfail(err, "autotune", e);
+  float best = 1e30f;
+  int bct = 4, bch = 256;
+  for (int ct : {2, 4, 8}) {
+    if (sizeof(uint16_t) * 3 * ct * 16 * (w.Kp + kLdsPad) > 64 * 1024) continue;
+    for (int ch : {128, 256, 512}) {
+      w.ct = ct;
+      w.chunk = ch;
+      w.ncg = (w.Np / 16 + ct - 1) / ct;
+      apply_launches(w, w.r, nullptr, nullptr, nullptr, st);
+      e = hipEventRecord(a, st);
+      for (int rep = 0; rep < 3 && e == hipSuccess; ++rep)
+        apply_launches(w, w.r, nullptr, nullptr, nullptr, st);
+      if (e == hipSuccess) e = hipEventRecord(b, st);
+      if (e == hipSuccess) e = hipEventSynchronize(b);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e != hipSuccess) break;
+      if (ms < best) {
+        best = ms;
+        bct = ct;
+        bch = ch;
+      }
+    }
+    if (e != hipSuccess) break;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (e != hipSuccess) return hipfail(err, "autotune", e);
+  w.ct = bct;
+  w.chunk = bch;
+  std::lock_guard<std::mutex> lk(g_tuned_mu);
+  g_tuned[key] = {bct, bch};
+  return YQ_OK;
 }
 
 }  // namespace
+
+void mixed_drop_graphs(MixedWork& w) {
+  for (auto& g : w.gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
 
 int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, int n, int m,
                   int use_bias, hipStream_t st, std::string& err) {
@@ -513,24 +679,38 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
   w.Kp = rup(n + 1, 32);
   w.Np = rup(m + 1, 16);
   w.nsp = rup(ns, kRowPad);
-  w.ncg = (w.Np / 16 + kColTiles - 1) / kColTiles;
+  if (w.ct == 0) w.ct = 4;
+  if (w.chunk == 0) w.chunk = 256;
+  w.ncg = (w.Np / 16 + w.ct - 1) / w.ct;
   w.dim = (size_t)n * m + (use_bias ? (size_t)n + m : 0);
   const size_t rows = (size_t)w.nsp;
-  const size_t red_total = red_layout(w).total;
-  if (rows * w.Np > w.cap_rows || w.dim > w.cap_dim || red_total > w.cap_red) {
-    const size_t chunks = rows / kBwdChunk;
+  const size_t red_total = red_layout(w, kBwdChunkMin).total;  // the most partials
+  if (rows > w.cap_rows || (size_t)w.Kp > w.cap_kp || (size_t)w.Np > w.cap_np ||
+      w.dim > w.cap_dim || red_total > w.cap_red) {
+    const size_t chunks = rows / kBwdChunkMin;
+    const size_t ncg_max = (w.Np / 16 + kColTilesMin - 1) / kColTilesMin;
     int rc;
     if ((rc = alloc(w.sigA, rows * w.Kp, err)) || (rc = alloc(w.sigT, rows * w.Kp, err)) ||
         (rc = alloc(w.Trm, rows * w.Np, err)) || (rc = alloc(w.Tt, rows * w.Np, err)) ||
-        (rc = alloc(w.ypart, rows * w.ncg, err)) ||
+        (rc = alloc(w.ypart, rows * ncg_max, err)) ||
         (rc = alloc(w.part, chunks * w.Kp * w.Np, err)) ||
         (rc = alloc(w.vsplit, (size_t)3 * w.Np * w.Kp, err)) ||
         (rc = alloc(w.om32, w.dim, err)) || (rc = alloc(w.x, w.dim, err)) ||
         (rc = alloc(w.r, w.dim, err)) || (rc = alloc(w.p, w.dim, err)) ||
         (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.red, red_total, err)) ||
-        (rc = alloc(w.dinv, w.dim, err)))
+        (rc = alloc(w.dinv, w.dim, err)) || (rc = alloc(w.dctl, 4, err)) ||
+        (rc = alloc(w.ictl, 4, err)))
       return rc;
-    w.cap_rows = rows * w.Np;
+    if (!w.host_ctl) {
+      hipError_t e = hipHostMalloc((void**)&w.host_ctl, sizeof(int) * 8, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev[0], hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev[1], hipEventDisableTiming);
+      if (e != hipSuccess) return hipfail(err, "mixed_prepare", e);
+    }
+    mixed_drop_graphs(w);  // buffers moved: captured launches are stale
+    w.cap_rows = rows;
+    w.cap_kp = w.Kp;
+    w.cap_np = w.Np;
     w.cap_dim = w.dim;
     w.cap_red = red_total;
   }
@@ -538,15 +718,31 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
                      w.nsp, w.sigA, w.sigT);
   hipLaunchKernelGGL(k_prep_T, dim3(nb(rows * w.Np)), dim3(256), 0, st, T32, ns, m, w.Np, w.nsp,
                      w.Trm, w.Tt);
+  if (w.pin_ct && w.pin_chunk) {
+    w.ct = w.pin_ct;
+    w.chunk = w.pin_chunk;
+  } else if (w.autotune) {
+    const int rc = tune(w, st, err);
+    if (rc) return rc;
+  } else {
+    w.ct = 4;
+    w.chunk = 256;
+  }
+  if (sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad) > 64 * 1024) {
+    err = "forward column group too wide for this n (LDS)";
+    return YQ_ERR_UNSUPPORTED;
+  }
+  w.ncg = (w.Np / 16 + w.ct - 1) / w.ct;
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_prepare", e);
 }
 
 int mixed_apply64(MixedWork& w, double shift, const double* omean64, const double* v,
                   double* out, hipStream_t st, std::string& err) {
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, 0.0, 1);
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, omean64, w.om32, w.dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, v, w.r, w.dim);
-  apply_launches(w, shift, w.r, nullptr, nullptr, nullptr, st);
+  apply_launches(w, w.r, nullptr, nullptr, nullptr, st);
   hipLaunchKernelGGL(k_f2d, dim3(nb(w.dim)), dim3(256), 0, st, w.Ap, out, w.dim);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_apply", e);
@@ -594,38 +790,70 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
     return YQ_OK;
   }
   const double thr = tol * tol * bn2;
-  const int check = 8;
-  int it = 0;
-  bool done = false;
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, thr, max_iter);
+  // iteration 0 (no beta) eagerly, then chunks of kChunkIters iterations; an
+  // iteration after the predicate fired leaves everything untouched, so the
+  // chunk after the one that converged may already be queued
+  iteration_launches(w, dinv, 0, true, st);
   hipError_t e;
-  while (!done && it < max_iter) {
-    const int cur = it & 1;
-    apply_launches(w, shift, w.r, dinv, rz[cur], it > 0 ? rz[cur ^ 1] : nullptr, st);
-    {
-      ProfGuard g("cg_xr", st);
-      hipLaunchKernelGGL(k_cg_xr, dim3(L.nbd), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dinv,
-                         dim, rz[cur], L.nbd, w.red + L.pap, L.nbr, rz[cur ^ 1], w.red + L.rr);
+  const bool graphs = w.use_graphs && !prof_enabled("split_v") && !prof_enabled("fwd_mfma") &&
+                      !prof_enabled("bwd_mfma") && !prof_enabled("bwd_reduce") &&
+                      !prof_enabled("cg_xr");
+  hipGraphExec_t gx = nullptr;
+  if (graphs && (rc = mixed_graph(w, dinv, st, &gx, err))) return rc;
+  // chunk c copies the control block into host slot c & 1 and records
+  // event c & 1; the host keeps one chunk queued ahead of the one it polls
+  int launched = 1, issued = 0;
+  auto issue = [&]() -> hipError_t {
+    if (gx) {
+      hipError_t r = hipGraphLaunch(gx, st);
+      if (r != hipSuccess) return r;
+    } else {
+      for (int k = 0; k < kChunkIters; ++k)
+        iteration_launches(w, dinv, (launched + k) & 1, false, st);
     }
-    ++it;
-    if (it % check == 0 || it == max_iter) {
-      double rn2 = 0.0;
-      if ((rc = host_rr(w, L, st, &rn2, err))) return rc;
-      if (!(rn2 >= thr)) done = true;  // converged (or NaN: the FP64 check decides)
-    }
+    launched += kChunkIters;
+    hipError_t r = hipMemcpyAsync(w.host_ctl + 4 * (issued & 1), w.ictl, sizeof(int) * 4,
+                                  hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipEventRecord(w.ev[issued & 1], st);
+    ++issued;
+    return r;
+  };
+  if (launched < max_iter && (e = issue()) != hipSuccess) return hipfail(err, "mixed_cg", e);
+  for (int k = 0; k < issued; ++k) {
+    if (launched < max_iter && (e = issue()) != hipSuccess) return hipfail(err, "mixed_cg", e);
+    if ((e = hipEventSynchronize(w.ev[k & 1])) != hipSuccess) return hipfail(err, "mixed_cg", e);
+    const int* c = w.host_ctl + 4 * (k & 1);
+    if (c[kCtlStop] || c[kCtlIters] >= max_iter) break;
   }
+  int hc[4];
+  e = hipMemcpyAsync(hc, w.ictl, sizeof(hc), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
+  const int it = hc[kCtlIters];
+  double rn2 = 0.0;
+  if ((rc = host_rr(w, L, st, &rn2, err))) return rc;
+  const bool conv = !(rn2 >= thr);
   hipLaunchKernelGGL(k_f2d, dim3(nb(dim)), dim3(256), 0, st, w.x, x, dim);
   e = hipGetLastError();
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   *iters = it;
-  return done ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+  return conv ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
 void mixed_release(MixedWork& w) {
+  mixed_drop_graphs(w);
   for (void* p : {(void*)w.sigA, (void*)w.sigT, (void*)w.Trm, (void*)w.Tt, (void*)w.vsplit,
                   (void*)w.ypart, (void*)w.part, (void*)w.om32, (void*)w.x,
-                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.red, (void*)w.dinv})
+                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.red, (void*)w.dinv,
+                  (void*)w.dctl, (void*)w.ictl})
     if (p) (void)hipFree(p);
+  for (auto& e : w.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (w.host_ctl) (void)hipHostFree(w.host_ctl);
+  const bool g = w.use_graphs;
   w = MixedWork();
+  w.use_graphs = g;
 }
 
 }  // namespace yq

@@ -16,21 +16,20 @@
 // then n_sweeps sweeps each followed by recording every chain).
 // Arithmetic: yq_math.h; the log-ratio is the lane-strided sum followed by a
 // stride-32..1 tree, which the CPU replay reproduces exactly.
+#include <algorithm>
+
 #include "yq_internal.h"
 #include "yq_math.h"
 
 namespace yq {
 namespace {
 
+// One chain of one machine, run by one wavefront: sig is the wave's LDS spin
+// strip.  RMAX >= ceil(m/64); larger values only add predicated-off slots,
+// so the arithmetic (and the replay) does not depend on it.
 template <typename R, int KIND, int RMAX>
-__global__ void __launch_bounds__(256) k_sample(const SampleArgs A) {
-  extern __shared__ int8_t lds_sig[];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int chain = blockIdx.x * 4 + wave;
-  if (chain >= A.C) return;  // wave-uniform; no block barrier follows
+__device__ void run_chain(const SampleArgs& A, int chain, int8_t* sig, int lane) {
   const int n = A.n, m = A.m;
-  int8_t* sig = lds_sig + wave * A.npad;
   for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -152,6 +151,40 @@ __global__ void __launch_bounds__(256) k_sample(const SampleArgs A) {
     A.ctr[chain] = ev;
     A.acc[chain] += nacc;
   }
+  // the strip is reused by the wave's next chain
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R, int KIND, int RMAX>
+__global__ void __launch_bounds__(256) k_sample(const SampleArgs A) {
+  extern __shared__ int8_t lds_sig[];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int chain = blockIdx.x * 4 + wave;
+  if (chain >= A.C) return;  // wave-uniform; no block barrier follows
+  run_chain<R, KIND, RMAX>(A, chain, lds_sig + wave * A.npad, lane);
+}
+
+// Many machines in one persistent launch: every wave walks the list of
+// (machine, chain) items with a grid-sized stride.  The list is ordered by
+// decreasing cost (n * m * sweep factor), so the strided walk hands every
+// wave a similar amount of work; a chain's result does not depend on which
+// wave runs it.
+template <typename R, int KIND>
+__global__ void __launch_bounds__(256)
+    k_sample_multi(const SampleArgs* __restrict__ inst, const int2* __restrict__ items,
+                   int n_items, int npad) {
+  extern __shared__ int8_t lds_sig[];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * 4;
+  for (int q = blockIdx.x * 4 + wave; q < n_items; q += nw) {
+    const int2 it = items[q];
+    const SampleArgs A = inst[it.x];
+    run_chain<R, KIND, kMaxHidden / kWave>(A, it.y, lds_sig + wave * npad, lane);
+  }
 }
 
 template <typename R, int KIND>
@@ -190,6 +223,42 @@ __global__ void __launch_bounds__(256)
 }
 
 }  // namespace
+
+void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_items,
+                         int npad, int kind, int mixed, int blocks, hipStream_t st) {
+  ProfGuard g("sample_multi", st);
+  const size_t lds = (size_t)4 * npad;
+  if (blocks <= 0) {  // persistent grid: what the chip keeps resident, at most one wave per item
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (mixed)
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, kind == 1 ? (const void*)k_sample_multi<float, 1> : (const void*)k_sample_multi<float, 0>,
+          256, lds);
+    else
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, kind == 1 ? (const void*)k_sample_multi<double, 1> : (const void*)k_sample_multi<double, 0>,
+          256, lds);
+    blocks = std::max(1, cus * std::max(per_cu, 1));
+    blocks = std::min(blocks, (n_items + 3) / 4);
+  }
+  if (mixed) {
+    if (kind == 1)
+      hipLaunchKernelGGL((k_sample_multi<float, 1>), dim3(blocks), dim3(256), lds, st, inst_dev,
+                         items, n_items, npad);
+    else
+      hipLaunchKernelGGL((k_sample_multi<float, 0>), dim3(blocks), dim3(256), lds, st, inst_dev,
+                         items, n_items, npad);
+  } else {
+    if (kind == 1)
+      hipLaunchKernelGGL((k_sample_multi<double, 1>), dim3(blocks), dim3(256), lds, st, inst_dev,
+                         items, n_items, npad);
+    else
+      hipLaunchKernelGGL((k_sample_multi<double, 0>), dim3(blocks), dim3(256), lds, st, inst_dev,
+                         items, n_items, npad);
+  }
+}
 
 void launch_sample(const SampleArgs& a, hipStream_t st) {
   ProfGuard g("sample", st);

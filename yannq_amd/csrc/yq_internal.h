@@ -51,6 +51,11 @@ struct SampleArgs {
   uint64_t seed;
 };
 void launch_sample(const SampleArgs& a, hipStream_t st);
+// Many machines in one persistent launch: inst_dev holds one SampleArgs per
+// machine (device memory), items the (machine, chain) pairs in decreasing
+// cost order; all machines share the sweeper kind and precision.
+void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_items, int npad,
+                         int kind, int mixed, int blocks, hipStream_t st);
 
 // theta = W sigma + b for a batch (fp64 sequential sum, optionally rounded to
 // float); out is double or float.
@@ -74,6 +79,11 @@ struct ElocArgs {
 };
 int num_bonds(const yq_ham_desc& h);
 void launch_eloc(const ElocArgs& a, hipStream_t st);
+size_t eloc_lds_bytes(const ElocArgs& a);
+// Many machines in one launch per pass: inst_dev holds one ElocArgs per
+// machine; stats[2 i], stats[2 i + 1] = mean and variance of machine i.
+void launch_eloc_multi(const ElocArgs* inst_dev, int count, int max_blocks, int max_ns,
+                       size_t lds, int use_float, double* stats, hipStream_t st);
 // mean and variance (Energy.hpp:65-70) into out[0], out[1] on the device
 void launch_eloc_stats(const double* eloc, int ns, double* out,
                        hipStream_t st);
