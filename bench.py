@@ -56,8 +56,12 @@ KERNEL_WORK = {
 HBM_PEAK_GBS = 8000.0
 
 
+AUTOTUNE = False
+
+
 def _ctx(yq, precision):
     ctx = yq.Context(0)
+    ctx.autotune(AUTOTUNE)
     ctx.rbm_create(N, M)
     ctx.init_random(SEED, 0.01)
     ctx.ham_set(yq.TFI, N, J=-1.0, h=-1.0)   # TFIsing(N, -1, -h) at h = 1
@@ -149,6 +153,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--precond", type=int, default=0)
     ap.add_argument("--warm-start", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="time the mixed products' tile shapes per operand shape")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -157,6 +163,8 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import yannq_amd as yq
+    global AUTOTUNE
+    AUTOTUNE = bool(args.autotune)
     opts = dict(precond=args.precond, warm_start=args.warm_start)
 
     # pick the dominant kernel from a profiled warm-up of the timed path
@@ -218,8 +226,8 @@ def main():
                                "16384 chains x 1 sweep = 16384 samples/step, persistent chains",
                    "precision": "mixed: f32 sampling/E_loc, split-bf16x3 MFMA CG products "
                                 "(f32 accumulate), f64 refinement to 1e-8, f64 update",
-                   "solver": "factored CG, Jacobi preconditioner=%d, warm start=%d"
-                             % (args.precond, args.warm_start),
+                   "solver": "factored CG, Jacobi preconditioner=%d, warm start=%d, "
+                             "tile autotune=%d" % (args.precond, args.warm_start, args.autotune),
                    "lambda": LAMBDA, "lr": LR, "cg_tol": TOL},
         "samples_per_s": value * CHAINS,
         "cg_iters": iters,

@@ -290,6 +290,19 @@ __global__ void __launch_bounds__(256)
 // and several blocks share a CU) and shared by the waves; every wave keeps
 // all Kp/16 row tiles in registers.
 constexpr int kBwdHalf = 64;  // samples per LDS slice of the sigma'^T slab
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): hardware block b runs on XCD b % 8.  Renumbering so that XCD x takes
+// the consecutive logical blocks [x * per, (x + 1) * per) keeps the column
+// groups of one chunk on one XCD, where they share its sigma'^T slab in L2.
+// A bijection on [0, total); the tail past a multiple of 8 keeps its index.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int b, int total) {
+  const int per = total / kXcds;
+  if (b >= per * kXcds) return b;
+  return (b % kXcds) * per + b / kXcds;
+}
+
 template <int CHUNK>
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
@@ -301,9 +314,10 @@ __global__ void __launch_bounds__(256)
   __shared__ float ysh[CHUNK];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
-  const int ct = blockIdx.x * 4 + wave;
+  const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int cx = lb % gridDim.x, kc = lb / gridDim.x;
+  const int ct = cx * 4 + wave;
   const bool active = ct < Np / 16;
-  const int kc = blockIdx.y;
   const int li = l & 15, lg = l >> 4;
   const int nit = Kp / 16;
   const int lda = kBwdHalf + kLdsPad;
@@ -318,7 +332,7 @@ __global__ void __launch_bounds__(256)
       ysh[q] = yv;
       dv += (double)yv;
     }
-    if (blockIdx.x == 0) block_partial(dv, ys, kc);  // uniform per block
+    if (cx == 0) block_partial(dv, ys, kc);  // uniform per block
   }
   floatx4 acc[kMaxIt];
 #pragma unroll
