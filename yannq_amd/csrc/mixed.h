@@ -28,7 +28,8 @@ struct MixedWork {
   float* part = nullptr;       // chunks x Kp x Np backward partials
   float* om32 = nullptr;       // column mean in float (dim)
   // CG vectors (float) and device scalars (double)
-  float *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
+  // (x, r; direction p and s = (S + shift) p; u = M r and Ap = (S + shift) u)
+  float *x = nullptr, *r = nullptr, *p = nullptr, *s = nullptr, *u = nullptr, *Ap = nullptr;
   float* dinv = nullptr;       // Jacobi preconditioner 1 / (diag S + shift)
   double* red = nullptr;       // per-block partials of the fused dot products
   std::vector<double> host_red;
@@ -40,7 +41,7 @@ struct MixedWork {
   bool autotune = false;
   int pin_ct = 0, pin_chunk = 0;  // shapes fixed by the caller (0: not fixed)
   // CG control block (device), its pinned host mirror and poll events
-  double* dctl = nullptr;      // shift, stopping threshold
+  double* dctl = nullptr;      // shift, stopping threshold, alpha and gamma slots
   int* ictl = nullptr;         // stop flag, completed iterations, budget
   int* host_ctl = nullptr;     // 2 slots of 4 ints
   hipEvent_t ev[2] = {nullptr, nullptr};

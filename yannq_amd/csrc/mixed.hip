@@ -12,9 +12,10 @@
 // A[row l%16][k 8*(l/16) .. +7] and B[k 8*(l/16) .. +7][col l%16] and
 // returns D[4*(l/16) + q][l%16], q = 0..3.
 //
-// The CG loop needs five launches per iteration and no host round trip: the
-// vector updates carry their dot products as per-block partials, which the
-// next kernel that needs the value sums itself in a fixed order.
+// The CG loop (Chronopoulos-Gear form) needs four launches per iteration and
+// no host round trip: the vector update carries its dot products as per-block
+// partials, which the next kernel that needs the value sums itself in a fixed
+// order.
 #include <math.h>
 
 #include <array>
@@ -39,9 +40,10 @@ constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
 constexpr int kChunkIters = 8;  // CG iterations per captured graph / host poll
 
 // device control block of the CG loop: ints [stop flag, completed iterations,
-// iteration budget] and doubles [shift, stopping threshold on <r, r>]
+// iteration budget]
 enum { kCtlStop = 0, kCtlIters = 1, kCtlMaxIt = 2 };
-enum { kCtlShift = 0, kCtlThr = 1 };
+// and doubles [shift, stopping threshold on <r, r>, alpha x 2, gamma x 2]
+enum { kCtlShift = 0, kCtlThr = 1, kCtlAlpha = 2, kCtlGamma = 4, kCtlDoubles = 6 };
 
 __device__ __host__ inline uint16_t f2bf(float x) {
   uint32_t u;
@@ -101,14 +103,79 @@ __device__ void block_partial2(double v1, double v2, double* out1, double* out2)
     out2[blockIdx.x] = r2[0];
   }
 }
-// total of cnt partials, identical in every lane of every wave
+__device__ void block_partial3(double v1, double v2, double v3, double* out1, double* out2,
+                               double* out3, int slot) {
+  __shared__ double r1[256], r2[256], r3[256];
+  const int t = threadIdx.x;
+  r1[t] = v1;
+  r2[t] = v2;
+  r3[t] = v3;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w) {
+      r1[t] += r1[t + w];
+      r2[t] += r2[t + w];
+      r3[t] += r3[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out1[slot] = r1[0];
+    out2[slot] = r2[0];
+    out3[slot] = r3[0];
+  }
+}
+// total of cnt partials, identical in every lane of every wave: lane l adds
+// partials l, l + 64, ... in order (loaded 8 at a time so that the loads are
+// in flight together), then an xor butterfly over the wave
 __device__ inline double psum(const double* __restrict__ part, int cnt) {
   const int l = threadIdx.x & 63;
   double s = 0.0;
-  for (int i = l; i < cnt; i += 64) s += part[i];
+  for (int i0 = l; i0 < cnt; i0 += 512) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = i0 + 64 * q < cnt ? part[i0 + 64 * q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (i0 + 64 * q < cnt) s += v[q];
+  }
 #pragma unroll
   for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w);
   return s;
+}
+// three totals at once, each in the order of psum
+__device__ inline void psum3(const double* __restrict__ a, int na, const double* __restrict__ b,
+                             int nb, const double* __restrict__ c, int nc, double& sa, double& sb,
+                             double& sc) {
+  const int l = threadIdx.x & 63;
+  const int top = max(na, max(nb, nc));
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int i0 = l; i0 < top; i0 += 256) {
+    double va[4], vb[4], vc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 64 * q;
+      va[q] = i < na ? a[i] : 0.0;
+      vb[q] = i < nb ? b[i] : 0.0;
+      vc[q] = i < nc ? c[i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 64 * q;
+      if (i < na) x += va[q];
+      if (i < nb) y += vb[q];
+      if (i < nc) z += vc[q];
+    }
+  }
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    x += __shfl_xor(x, w);
+    y += __shfl_xor(y, w);
+    z += __shfl_xor(z, w);
+  }
+  sa = x;
+  sb = y;
+  sc = z;
 }
 
 // index of parameter c on the (Kp x Np) grid of V' / G'
@@ -150,33 +217,63 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
   Tt[(size_t)j * nsp + s] = v;
 }
 
-// p <- z + beta p with z = dinv * r (dinv null: z = r) and beta = rho_now /
-// rho_prev (rho_prev null: beta = 0), rho given as partials; V' = p on the
-// (Kp x Np) grid split into three bf16 layers stored column-major,
-// vs[(layer * Np + j) * Kp + k]; op = partials of <omean, p>.  Block: a 16 x
-// 16 tile of the grid, read along j (consecutive parameters) and written
-// along k through an LDS transpose, so both sides are coalesced.
+// One Chronopoulos-Gear CG step for (S + shift) x = b with preconditioner M
+// (M = diag(dinv), or the identity when dinv is null), fused with the bf16
+// split of the vector the next product needs.  With u = M r, w = (S + shift) u,
+// gamma = <r, u> and delta = <w, u> (as partials of the previous launches):
+//   mode 0 (start):  u = M r only
+//   mode 1 (first):  alpha = gamma / delta, p = u, s = w
+//   mode 2:          beta = gamma / gamma_prev,
+//                    alpha = gamma / (delta - beta gamma / alpha_prev),
+//                    p = u + beta p, s = w + beta s
+// then (modes 1, 2) x += alpha p, r -= alpha s, u = M r.  The new u is stored
+// and split into V' on the (Kp x Np) grid in three bf16 layers stored
+// column-major, vs[(layer * Np + j) * Kp + k]; the partials of <r, u>,
+// <r, r> and <omean, u> of the new vectors go to g_out, rr_out and op.  The
+// recurrence needs one product per iteration (on u) and no separate vector
+// kernel: the update, the split and the three dot products share this launch.
+// alpha and gamma of iteration `par` are kept in control-block slot par & 1
+// for the next iteration.  Block: a 16 x 16 tile of the grid, read along j
+// (consecutive parameters) and written along k through an LDS transpose, so
+// both sides are coalesced.
 __global__ void __launch_bounds__(256)
-    k_split_v(const float* __restrict__ r, const float* __restrict__ dinv, float* __restrict__ p,
-              const double* __restrict__ rz_now, const double* __restrict__ rz_prev, int nrz,
-              const double* __restrict__ rr, const double* __restrict__ dctl, int* ictl,
-              const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
-              uint16_t* __restrict__ vs, double* __restrict__ op) {
+    k_cg_update(float* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
+                float* __restrict__ s, float* __restrict__ u, const float* __restrict__ w,
+                const float* __restrict__ dinv, int mode, int par,
+                const double* __restrict__ g_in, const double* __restrict__ rr_in, int nv,
+                const double* __restrict__ dl, int ndl, double* __restrict__ g_out,
+                double* __restrict__ rr_out, double* dctl, int* ictl,
+                const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
+                uint16_t* __restrict__ vs, double* __restrict__ op) {
   __shared__ uint16_t tile[3][16][17];
   const int t = threadIdx.x;
   const int j0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   const size_t nm = (size_t)n * m;
-  float beta = 0.0f;
-  if (rz_prev) {
+  float alpha = 0.0f, beta = 0.0f;
+  if (mode > 0) {
+    double rn2, gam, del;
+    psum3(rr_in, nv, g_in, nv, dl, ndl, rn2, gam, del);
     // CG continuation predicate (uniform over the grid): stop once <r, r>
     // falls below the threshold (or is NaN) or the iteration budget is spent;
     // the later kernels of the iteration see the flag and return at once
-    const double rn2 = psum(rr, nrz);
     if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
       if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) ictl[kCtlStop] = 1;
       return;
     }
-    beta = (float)(psum(rz_now, nrz) / psum(rz_prev, nrz));
+    double a, bt = 0.0;
+    if (mode == 1) {
+      a = gam / del;
+    } else {
+      bt = gam / dctl[kCtlGamma + (par ^ 1)];
+      a = gam / (del - bt * gam / dctl[kCtlAlpha + (par ^ 1)]);
+    }
+    if (!__builtin_isfinite(a)) a = 0.0;  // breakdown (delta or its correction 0): no step
+    alpha = (float)a;
+    beta = (float)bt;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+      dctl[kCtlAlpha + par] = a;
+      dctl[kCtlGamma + par] = gam;
+    }
   }
   {
     const int kk = t >> 4, jj = t & 15;
@@ -186,19 +283,33 @@ __global__ void __launch_bounds__(256)
     else if (use_bias && k < n && j == m) c = (long)(nm + k);
     else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
     float v = 0.0f;
-    double dv = 0.0;
+    double d_om = 0.0, d_g = 0.0, d_rr = 0.0;
     if (c >= 0) {
-      const float z = dinv ? r[c] * dinv[c] : r[c];
-      v = rz_prev ? z + beta * p[c] : z;
-      p[c] = v;
-      dv = (double)om[c] * (double)v;
+      float rv = r[c];
+      const float di = dinv ? dinv[c] : 1.0f;
+      if (mode > 0) {
+        const float uo = u[c];
+        const float pv = mode == 1 ? uo : uo + beta * p[c];
+        const float sv = mode == 1 ? w[c] : w[c] + beta * s[c];
+        p[c] = pv;
+        s[c] = sv;
+        x[c] += alpha * pv;
+        rv = rv - alpha * sv;
+        r[c] = rv;
+      }
+      v = dinv ? rv * di : rv;
+      u[c] = v;
+      d_om = (double)om[c] * (double)v;
+      d_rr = (double)rv * (double)rv;
+      d_g = (double)rv * (double)v;
     }
     uint16_t hi, mid, lo;
     split3(v, hi, mid, lo);
     tile[0][jj][kk] = hi;
     tile[1][jj][kk] = mid;
     tile[2][jj][kk] = lo;
-    block_partial(dv, op, blockIdx.y * gridDim.x + blockIdx.x);  // contains barriers
+    block_partial3(d_om, d_g, d_rr, op, g_out, rr_out,
+                   blockIdx.y * gridDim.x + blockIdx.x);  // contains barriers
   }
   const int jj = t >> 4, kk = t & 15;
 #pragma unroll
@@ -229,26 +340,63 @@ __global__ void __launch_bounds__(256)
   const int ncols = min(CT * 16, Np - col0);
   const int ldk = Kp + kLdsPad;
   const int kchunks = Kp / 8;
-  for (int q = tid; q < 3 * ncols * kchunks; q += 256) {
-    const int sp = q / (ncols * kchunks);
-    const int rem = q % (ncols * kchunks);
-    const int col = rem / kchunks, kk = (rem % kchunks) * 8;
-    *reinterpret_cast<u16x8*>(bsh + (sp * CT * 16 + col) * ldk + kk) =
-        *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp + kk);
-  }
-  __syncthreads();
   const int wave = tid >> 6, l = tid & 63;
   const int li = l & 15, lg = l >> 4;
   const int rbase = blockIdx.x * 256 + wave * 64;
   const int nks = Kp / 32;
+  // The prologue's global loads overlap: the V' slab goes through registers
+  // into LDS in two passes (at most 16 16-byte pieces per thread in all, the
+  // LDS budget of the slab), and this wave's sigma' fragments are issued
+  // between the first pass's loads and its LDS stores.
+  constexpr int kStage = (3 * CT * 16 * 4 * kMaxKs + 255) / 256 < 16
+                             ? (3 * CT * 16 * 4 * kMaxKs + 255) / 256 : 16;
+  constexpr int kPass = (kStage + 1) / 2;
+  const int total = 3 * ncols * kchunks;
+  u16x8 stg[kPass];
+  auto stage_load = [&](int pass) {
+#pragma unroll
+    for (int i = 0; i < kPass; ++i) {
+      const int q = tid + 256 * (pass * kPass + i);
+      if (q < total) {
+        const int sp = q / (ncols * kchunks);
+        const int rem = q % (ncols * kchunks);
+        const int col = rem / kchunks, kk = (rem % kchunks) * 8;
+        stg[i] = *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp + kk);
+      }
+    }
+  };
+  auto stage_store = [&](int pass) {
+#pragma unroll
+    for (int i = 0; i < kPass; ++i) {
+      const int q = tid + 256 * (pass * kPass + i);
+      if (q < total) {
+        const int sp = q / (ncols * kchunks);
+        const int rem = q % (ncols * kchunks);
+        const int col = rem / kchunks, kk = (rem % kchunks) * 8;
+        *reinterpret_cast<u16x8*>(bsh + (sp * CT * 16 + col) * ldk + kk) = stg[i];
+      }
+    }
+  };
+  stage_load(0);
   bf16x8 sfr[4][kMaxKs];
 #pragma unroll
   for (int st = 0; st < 4; ++st)
 #pragma unroll
     for (int ks = 0; ks < kMaxKs; ++ks)
       if (ks < nks) sfr[st][ks] = ld8(sigA + (size_t)(rbase + st * 16 + li) * Kp + ks * 32 + lg * 8);
+  stage_store(0);
+  stage_load(1);
+  stage_store(1);
+  __syncthreads();
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int ct = 0; ct < ncols / 16; ++ct) {
+  const int nct = ncols / 16;
+  for (int ct = 0; ct < nct; ++ct) {
+    // this tile's T' is loaded ahead of its products
+    floatx4 t[4];
+    const int j0 = col0 + ct * 16 + 4 * lg;
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+      t[st] = *reinterpret_cast<const floatx4*>(Trm + (size_t)(rbase + st * 16 + li) * Np + j0);
     floatx4 c[4];
 #pragma unroll
     for (int st = 0; st < 4; ++st) c[st] = (floatx4){0.f, 0.f, 0.f, 0.f};
@@ -264,13 +412,10 @@ __global__ void __launch_bounds__(256)
             c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v, sfr[st][ks], c[st], 0, 0, 0);
         }
     // c[st][q] = U'^T[col0 + ct*16 + 4*lg + q][rbase + st*16 + li]
-    const int j0 = col0 + ct * 16 + 4 * lg;
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const floatx4 t =
-          *reinterpret_cast<const floatx4*>(Trm + (size_t)(rbase + st * 16 + li) * Np + j0);
-      acc[st] += ((t[0] * c[st][0] + t[1] * c[st][1]) + (t[2] * c[st][2] + t[3] * c[st][3]));
-    }
+    for (int st = 0; st < 4; ++st)
+      acc[st] += ((t[st][0] * c[st][0] + t[st][1] * c[st][1]) +
+                  (t[st][2] * c[st][2] + t[st][3] * c[st][3]));
   }
 #pragma unroll
   for (int st = 0; st < 4; ++st) {
@@ -327,7 +472,14 @@ __global__ void __launch_bounds__(256)
     for (int q = tid; q < CHUNK; q += 256) {
       const int s = kc * CHUNK + q;
       float v = 0.0f;
-      for (int g = 0; g < ncg; ++g) v += ypart[(size_t)g * nsp + s];
+      for (int g0 = 0; g0 < ncg; g0 += 8) {  // loads in flight together, sums in order
+        float y8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) y8[e] = g0 + e < ncg ? ypart[(size_t)(g0 + e) * nsp + s] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (g0 + e < ncg) v += y8[e];
+      }
       const float yv = s < ns ? (v - ov) / (float)ns : 0.0f;
       ysh[q] = yv;
       dv += (double)yv;
@@ -338,22 +490,59 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int it = 0; it < kMaxIt; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
   const float* trow = Tt + (size_t)(active ? ct * 16 + li : 0) * nsp;
-  for (int half = 0; half < CHUNK / kBwdHalf; ++half) {
+  // Software pipeline over the slices: the sigma'^T slab of slice h + 1 (into
+  // registers) and this lane's T' values of slice h + 1 are loaded while the
+  // products of slice h run.  At most 5 16-byte slab pieces per thread
+  // (Kp <= 160).
+  constexpr int kSlab = (32 * kMaxKs * (kBwdHalf / 8) + 255) / 256;
+  const int nslab = Kp * (kBwdHalf / 8);
+  u16x8 sg[kSlab];
+  floatx4 tv[4] = {};  // T'[ct*16 + li][sbase + ks*32 + lg*8 + 0..7], ks = 0, 1
+  auto load_slice = [&](int half) {
     const int sbase = kc * CHUNK + half * kBwdHalf;
-    __syncthreads();
-    for (int q = tid; q < Kp * (kBwdHalf / 8); q += 256) {
-      const int i = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
-      *reinterpret_cast<u16x8*>(ash + i * lda + kk) =
-          *reinterpret_cast<const u16x8*>(sigT + (size_t)i * nsp + sbase + kk);
+#pragma unroll
+    for (int i = 0; i < kSlab; ++i) {
+      const int q = tid + 256 * i;
+      if (q < nslab) {
+        const int row = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
+        sg[i] = *reinterpret_cast<const u16x8*>(sigT + (size_t)row * nsp + sbase + kk);
+      }
     }
+    if (active) {
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const float* tp = trow + sbase + ks * 32 + lg * 8;
+        tv[2 * ks] = *reinterpret_cast<const floatx4*>(tp);
+        tv[2 * ks + 1] = *reinterpret_cast<const floatx4*>(tp + 4);
+      }
+    }
+  };
+  load_slice(0);
+  constexpr int kHalves = CHUNK / kBwdHalf;
+  for (int half = 0; half < kHalves; ++half) {
+    const int sbase = kc * CHUNK + half * kBwdHalf;
+    __syncthreads();  // the previous slice's products are done with the LDS
+#pragma unroll
+    for (int i = 0; i < kSlab; ++i) {
+      const int q = tid + 256 * i;
+      if (q < nslab) {
+        const int row = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
+        *reinterpret_cast<u16x8*>(ash + row * lda + kk) = sg[i];
+      }
+    }
+    floatx4 tc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tc[i] = tv[i];
+    if (half + 1 < kHalves) load_slice(half + 1);
     __syncthreads();
     if (!active) continue;
+#pragma unroll
     for (int ks = 0; ks < kBwdHalf / 32; ++ks) {
       const int s0 = sbase + ks * 32 + lg * 8;
       u16x8 bh, bm, bl;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const float v = trow[s0 + e] * ysh[s0 - kc * CHUNK + e];
+        const float v = tc[2 * ks + (e >> 2)][e & 3] * ysh[s0 - kc * CHUNK + e];
         uint16_t h, md, lo;
         split3(v, h, md, lo);
         bh[e] = h;
@@ -383,17 +572,21 @@ __global__ void __launch_bounds__(256)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
 
-// out[c] = G'[aug(c)] - omean_c sum(y) + shift p_c, G' summed over the
-// chunks.  Block: 64 parameters x 4 chunk groups (wave g sums chunks g, g+4,
-// ... in order; the four group sums are combined as (g0 + g1) + (g2 + g3));
-// pap = partials of <p, out>.
+// out[c] = G'[aug(c)] - omean_c sum(y) + shift u_c, G' summed over the
+// chunks (u: the vector the product was formed for).  Block: 64 parameters x
+// 4 chunk groups (wave g sums chunks g, g+4, ... in order; the four group
+// sums are combined as (g0 + g1) + (g2 + g3)); pap = partials of <u, out>.
+// count: this product completes a CG iteration (one writer increments the
+// iteration counter).
 __global__ void __launch_bounds__(256)
     k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m, int use_bias,
                  int Kp, int Np, const float* __restrict__ om, const double* __restrict__ dctl,
                  const double* __restrict__ ys, int nys, const float* __restrict__ p,
-                 float* __restrict__ out, double* __restrict__ pap, const int* __restrict__ ictl) {
+                 float* __restrict__ out, double* __restrict__ pap, int* __restrict__ ictl,
+                 int count) {
   __shared__ float grp[4][64];
   if (ictl[kCtlStop]) return;
+  if (count && blockIdx.x == 0 && threadIdx.x == 0) ictl[kCtlIters] += 1;  // one writer
   const double shift = dctl[kCtlShift];
   const size_t nm = (size_t)n * m;
   const size_t dim = use_bias ? nm + n + m : nm;
@@ -404,7 +597,15 @@ __global__ void __launch_bounds__(256)
   if (c < dim) {
     const size_t a = aug_index(c, n, m, Np);
     const size_t stride = (size_t)Kp * Np;
-    for (int k = g; k < chunks; k += 4) acc += part[(size_t)k * stride + a];
+    for (int k0 = g; k0 < chunks; k0 += 32) {  // 8 loads in flight, sums in chunk order
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        v[q] = k0 + 4 * q < chunks ? part[(size_t)(k0 + 4 * q) * stride + a] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (k0 + 4 * q < chunks) acc += v[q];
+    }
   }
   grp[g][l] = acc;
   __syncthreads();
@@ -418,40 +619,6 @@ __global__ void __launch_bounds__(256)
   block_partial(dv, pap, blockIdx.x);
 }
 
-// x += alpha p, r -= alpha Ap with alpha = rho / <p, Ap> (both as partials);
-// rz_next, rr = partials of <r, dinv r> and <r, r> of the updated r
-__global__ void __launch_bounds__(256)
-    k_cg_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
-            const float* __restrict__ Ap, const float* __restrict__ dinv, size_t n,
-            const double* __restrict__ rz_now, int nrz, const double* __restrict__ pap, int npap,
-            double* __restrict__ rz_next, double* __restrict__ rr_out, int* ictl) {
-  if (ictl[kCtlStop]) return;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ictl[kCtlIters] += 1;  // one writer per launch
-  const double pv = psum(pap, npap);
-  const float alpha = pv != 0.0 ? (float)(psum(rz_now, nrz) / pv) : 0.0f;
-  double rz = 0.0, rr = 0.0;
-  if (i < n) {
-    x[i] += alpha * p[i];
-    const float rv = r[i] - alpha * Ap[i];
-    r[i] = rv;
-    rr = (double)rv * (double)rv;
-    rz = dinv ? (double)rv * (double)(rv * dinv[i]) : rr;
-  }
-  block_partial2(rz, rr, rz_next, rr_out);
-}
-
-__global__ void __launch_bounds__(256)
-    k_dot_rz(const float* __restrict__ r, const float* __restrict__ dinv, size_t n,
-             double* __restrict__ rz_out, double* __restrict__ rr_out) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double rz = 0.0, rr = 0.0;
-  if (i < n) {
-    rr = (double)r[i] * (double)r[i];
-    rz = dinv ? (double)r[i] * (double)(r[i] * dinv[i]) : rr;
-  }
-  block_partial2(rz, rr, rz_out, rr_out);
-}
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
@@ -501,43 +668,37 @@ int alloc(T*& p, size_t count, std::string& err) {
   return YQ_OK;
 }
 
-// block counts of the partial-sum producers and their offsets in w.red
+// block counts of the partial-sum producers and their offsets in w.red:
+// gamma = <r, u> and <r, r> alternate between two slots (iteration parity)
 struct RedLayout {
-  int nbd, nbv, nby, nbr;
-  size_t rz0, rz1, rr, op, ys, pap, total;
+  int nbv, nby, nbr;
+  size_t g[2], rr[2], op, ys, pap, total;
 };
 RedLayout red_layout(const MixedWork& w, int chunk = 0) {
   RedLayout L;
   if (chunk == 0) chunk = w.chunk;
-  L.nbd = (int)nb(w.dim);                     // elementwise kernels over dim
-  L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_split_v tiles
+  L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_cg_update tiles
   L.nby = w.nsp / chunk;                      // k_bwd_mfma chunks
   L.nbr = (int)((w.dim + 63) / 64);           // k_bwd_reduce
-  L.rz0 = 0;
-  L.rz1 = L.rz0 + L.nbd;
-  L.rr = L.rz1 + L.nbd;
-  L.op = L.rr + L.nbd;
+  L.g[0] = 0;
+  L.g[1] = L.g[0] + L.nbv;
+  L.rr[0] = L.g[1] + L.nbv;
+  L.rr[1] = L.rr[0] + L.nbv;
+  L.op = L.rr[1] + L.nbv;
   L.ys = L.op + L.nbv;
   L.pap = L.ys + L.nby;
   L.total = L.pap + L.nbr;
   return L;
 }
 
-// (S + shift) p -> Ap with the products on MFMA, leaving the partials of
-// <p, Ap> in w.red; shift and the CG predicate come from the control block.
-// p is first updated to z + beta p (see k_split_v) with rho from the
-// partials rz_now / rz_prev.
-void apply_launches(MixedWork& w, const float* r, const float* dinv, const double* rz_now,
-                    const double* rz_prev, hipStream_t st) {
+// w.Ap = (S + shift) u for the u last split into w.vsplit by k_cg_update,
+// with the products on MFMA, leaving the partials of <u, w.Ap> in w.red;
+// shift comes from the control block, and every kernel returns at once when
+// the stop flag is set.  count: the product ends a CG iteration.
+void product_launches(MixedWork& w, int count, hipStream_t st) {
   const RedLayout L = red_layout(w);
   double* op = w.red + L.op;
   double* ys = w.red + L.ys;
-  {
-    ProfGuard g("split_v", st);
-    hipLaunchKernelGGL(k_split_v, dim3(w.Np / 16, w.Kp / 16), dim3(256), 0, st, r, dinv, w.p,
-                       rz_now, rz_prev, L.nbd, w.red + L.rr, w.dctl, w.ictl, w.om32, w.n, w.m,
-                       w.use_bias, w.Kp, w.Np, w.vsplit, op);
-  }
   {
     ProfGuard g("fwd_mfma", st);
     const size_t lds = sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad);
@@ -557,18 +718,34 @@ void apply_launches(MixedWork& w, const float* r, const float* dinv, const doubl
   }
   ProfGuard g("bwd_reduce", st);
   hipLaunchKernelGGL(k_bwd_reduce, dim3(L.nbr), dim3(256), 0, st, w.part, chunks, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.om32, w.dctl, ys, L.nby, w.p, w.Ap,
-                     w.red + L.pap, w.ictl);
+                     w.use_bias, w.Kp, w.Np, w.om32, w.dctl, ys, L.nby, w.u, w.Ap,
+                     w.red + L.pap, w.ictl, count);
 }
 
-// one CG iteration: the operator, then x, r and the next rho
-void iteration_launches(MixedWork& w, const float* dinv, int cur, bool first, hipStream_t st) {
+// k_cg_update in the given mode for iteration parity par (mode 0: u = M r
+// into the gamma / <r, r> slot 0)
+void update_launch(MixedWork& w, const float* dinv, int mode, int par, hipStream_t st) {
   const RedLayout L = red_layout(w);
-  double* rz[2] = {w.red + L.rz0, w.red + L.rz1};
-  apply_launches(w, w.r, dinv, rz[cur], first ? nullptr : rz[cur ^ 1], st);
-  ProfGuard g("cg_xr", st);
-  hipLaunchKernelGGL(k_cg_xr, dim3(L.nbd), dim3(256), 0, st, w.x, w.r, w.p, w.Ap, dinv, w.dim,
-                     rz[cur], L.nbd, w.red + L.pap, L.nbr, rz[cur ^ 1], w.red + L.rr, w.ictl);
+  double* g_out = w.red + (mode == 0 ? L.g[0] : L.g[par ^ 1]);
+  double* rr_out = w.red + (mode == 0 ? L.rr[0] : L.rr[par ^ 1]);
+  ProfGuard g("cg_update", st);
+  hipLaunchKernelGGL(k_cg_update, dim3(w.Np / 16, w.Kp / 16), dim3(256), 0, st, w.x, w.r, w.p,
+                     w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par], L.nbv,
+                     w.red + L.pap, L.nbr, g_out, rr_out, w.dctl, w.ictl, w.om32, w.n, w.m,
+                     w.use_bias, w.Kp, w.Np, w.vsplit, w.red + L.op);
+}
+
+// (S + shift) v for v in w.r, without preconditioning: the split (mode 0)
+// and the product; the result lands in w.Ap
+void apply_launches(MixedWork& w, hipStream_t st) {
+  update_launch(w, nullptr, 0, 0, st);
+  product_launches(w, 0, st);
+}
+
+// one CG iteration (parity par): the update, then the product on the new u
+void iteration_launches(MixedWork& w, const float* dinv, int par, bool first, hipStream_t st) {
+  update_launch(w, dinv, first ? 1 : 2, par, st);
+  product_launches(w, 1, st);
 }
 
 // shape of the captured launches (a different shape needs a new capture)
@@ -577,7 +754,7 @@ void graph_key(const MixedWork& w, long key[10]) {
   for (int i = 0; i < 10; ++i) key[i] = k[i];
 }
 
-// kChunkIters iterations (absolute iterations 1 + 8q + p use rho slot
+// kChunkIters iterations (absolute iterations 1 + 8q + p have parity
 // (1 + p) & 1) captured once per operand shape and preconditioner setting
 int mixed_graph(MixedWork& w, const float* dinv, hipStream_t st, hipGraphExec_t* out,
                 std::string& err) {
@@ -642,10 +819,9 @@ int tune(MixedWork& w, hipStream_t st, std::string& err) {
       w.ct = ct;
       w.chunk = ch;
       w.ncg = (w.Np / 16 + ct - 1) / ct;
-      apply_launches(w, w.r, nullptr, nullptr, nullptr, st);
+      apply_launches(w, st);
       e = hipEventRecord(a, st);
-      for (int rep = 0; rep < 3 && e == hipSuccess; ++rep)
-        apply_launches(w, w.r, nullptr, nullptr, nullptr, st);
+      for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) apply_launches(w, st);
       if (e == hipSuccess) e = hipEventRecord(b, st);
       if (e == hipSuccess) e = hipEventSynchronize(b);
       float ms = 0.f;
@@ -711,8 +887,9 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
         (rc = alloc(w.vsplit, (size_t)3 * w.Np * w.Kp, err)) ||
         (rc = alloc(w.om32, w.dim, err)) || (rc = alloc(w.x, w.dim, err)) ||
         (rc = alloc(w.r, w.dim, err)) || (rc = alloc(w.p, w.dim, err)) ||
+        (rc = alloc(w.s, w.dim, err)) || (rc = alloc(w.u, w.dim, err)) ||
         (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.red, red_total, err)) ||
-        (rc = alloc(w.dinv, w.dim, err)) || (rc = alloc(w.dctl, 4, err)) ||
+        (rc = alloc(w.dinv, w.dim, err)) || (rc = alloc(w.dctl, kCtlDoubles, err)) ||
         (rc = alloc(w.ictl, 4, err)))
       return rc;
     if (!w.host_ctl) {
@@ -756,22 +933,23 @@ int mixed_apply64(MixedWork& w, double shift, const double* omean64, const doubl
   hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, 0.0, 1);
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, omean64, w.om32, w.dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(w.dim)), dim3(256), 0, st, v, w.r, w.dim);
-  apply_launches(w, w.r, nullptr, nullptr, nullptr, st);
+  apply_launches(w, st);
   hipLaunchKernelGGL(k_f2d, dim3(nb(w.dim)), dim3(256), 0, st, w.Ap, out, w.dim);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_apply", e);
 }
 
 namespace {
-// host copy of the <r, r> partials, summed in block order
-int host_rr(MixedWork& w, const RedLayout& L, hipStream_t st, double* out, std::string& err) {
-  if ((int)w.host_red.size() < L.nbd) w.host_red.resize(L.nbd);
-  hipError_t e = hipMemcpyAsync(w.host_red.data(), w.red + L.rr, sizeof(double) * L.nbd,
+// host copy of the <r, r> partials of parity slot par, summed in block order
+int host_rr(MixedWork& w, const RedLayout& L, int par, hipStream_t st, double* out,
+            std::string& err) {
+  if ((int)w.host_red.size() < L.nbv) w.host_red.resize(L.nbv);
+  hipError_t e = hipMemcpyAsync(w.host_red.data(), w.red + L.rr[par], sizeof(double) * L.nbv,
                                 hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   double s = 0.0;
-  for (int i = 0; i < L.nbd; ++i) s += w.host_red[i];
+  for (int i = 0; i < L.nbv; ++i) s += w.host_red[i];
   *out = s;
   return YQ_OK;
 }
@@ -785,18 +963,19 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, omean64, w.om32, dim);
   hipLaunchKernelGGL(k_d2f, dim3(nb(dim)), dim3(256), 0, st, b, w.r, dim);
   hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.x, dim);
-  hipLaunchKernelGGL(k_zero_f, dim3(nb(dim)), dim3(256), 0, st, w.p, dim);
   const float* dinv = nullptr;
   if (diag64) {
     hipLaunchKernelGGL(k_jacobi_inv, dim3(nb(dim)), dim3(256), 0, st, diag64, shift, w.dinv, dim);
     dinv = w.dinv;
   }
-  // rho = <r, z> partials alternate between rz0 (even iterations) and rz1
-  double* rz[2] = {w.red + L.rz0, w.red + L.rz1};
-  hipLaunchKernelGGL(k_dot_rz, dim3(L.nbd), dim3(256), 0, st, w.r, dinv, dim, rz[0],
-                     w.red + L.rr);
+  // start: u = M r with gamma and <r, r> in slot 0, and the first product
+  // w = (S + shift) u with its <u, w> partials (the control block is reset
+  // first so that the product kernels run)
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, 0.0, 1);
+  update_launch(w, dinv, 0, 0, st);
+  product_launches(w, 0, st);
   double bn2 = 0.0;
-  int rc = host_rr(w, L, st, &bn2, err);
+  int rc = host_rr(w, L, 0, st, &bn2, err);
   if (rc) return rc;
   *iters = 0;
   if (bn2 == 0.0) {
@@ -810,9 +989,8 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   // chunk after the one that converged may already be queued
   iteration_launches(w, dinv, 0, true, st);
   hipError_t e;
-  const bool graphs = w.use_graphs && !prof_enabled("split_v") && !prof_enabled("fwd_mfma") &&
-                      !prof_enabled("bwd_mfma") && !prof_enabled("bwd_reduce") &&
-                      !prof_enabled("cg_xr");
+  const bool graphs = w.use_graphs && !prof_enabled("cg_update") && !prof_enabled("fwd_mfma") &&
+                      !prof_enabled("bwd_mfma") && !prof_enabled("bwd_reduce");
   hipGraphExec_t gx = nullptr;
   if (graphs && (rc = mixed_graph(w, dinv, st, &gx, err))) return rc;
   // chunk c copies the control block into host slot c & 1 and records
@@ -845,8 +1023,9 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   const int it = hc[kCtlIters];
+  // after `it` updates the latest <r, r> partials sit in slot it & 1
   double rn2 = 0.0;
-  if ((rc = host_rr(w, L, st, &rn2, err))) return rc;
+  if ((rc = host_rr(w, L, it & 1, st, &rn2, err))) return rc;
   const bool conv = !(rn2 >= thr);
   hipLaunchKernelGGL(k_f2d, dim3(nb(dim)), dim3(256), 0, st, w.x, x, dim);
   e = hipGetLastError();
@@ -859,7 +1038,8 @@ void mixed_release(MixedWork& w) {
   mixed_drop_graphs(w);
   for (void* p : {(void*)w.sigA, (void*)w.sigT, (void*)w.Trm, (void*)w.Tt, (void*)w.vsplit,
                   (void*)w.ypart, (void*)w.part, (void*)w.om32, (void*)w.x,
-                  (void*)w.r, (void*)w.p, (void*)w.Ap, (void*)w.red, (void*)w.dinv,
+                  (void*)w.r, (void*)w.p, (void*)w.s, (void*)w.u, (void*)w.Ap, (void*)w.red,
+                  (void*)w.dinv,
                   (void*)w.dctl, (void*)w.ictl})
     if (p) (void)hipFree(p);
   for (auto& e : w.ev)
