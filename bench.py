@@ -46,8 +46,9 @@ KERNEL_WORK = {
     "fwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4,
     # backward product: T'^T and sigma'^T and y in, the (n+1) x (m+1) chunk partials out
     "bwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4 + (NS // 256) * KP * NP * 4,
-    # chunk reduction: the partials in, one dim-vector out
-    "bwd_reduce": (NS // 256) * KP * NP * 4 + DIM * 4 * 3,
+    # CG step: the chunk partials in; x, r, p, s, u, omean in; x, r, p, s, u and
+    # the three bf16 layers of V' out
+    "cg_step": (NS // 256) * KP * NP * 4 + DIM * 4 * 11 + 3 * KP * NP * 2,
     # sampler: spins in/out and the recorded samples (spins, float theta)
     "sample": NS * N * 3 + NS * M * 4,
     # E_loc ratios: theta (float) and spins in, ratios (double) out

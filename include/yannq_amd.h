@@ -180,8 +180,11 @@ YQ_API int yq_sr_solve(yq_ctx* ctx, double shift, double tol, int max_iter,
 YQ_API int yq_sr_set_options(yq_ctx* ctx, int precond, int warm_start);
 /* Launch mode of the mixed-precision CG (no reference counterpart): 1, the
  * default, replays chunks of 8 iterations as a captured HIP graph that stops
- * itself on the device once converged; 0 issues the same kernels one by one.
- * Both perform identical arithmetic and return identical results. */
+ * itself on the device once converged; an even number 2..256 does the same
+ * with chunks of that many iterations (the host polls the convergence flag
+ * once per chunk); 0 issues the same kernels one by one (polling every 8
+ * iterations).  All modes perform identical arithmetic and return identical
+ * results. */
 YQ_API int yq_set_graph_mode(yq_ctx* ctx, int use_graphs);
 /* Tile-shape autotuning of the mixed-precision products (no reference
  * counterpart): 1 times every (forward column group, backward chunk) pair on

@@ -397,6 +397,8 @@ class Context:
         self._chk(self._L.yq_sr_set_options(self._h, precond, warm_start))
 
     def graph_mode(self, use_graphs=True):
+        """True / 1: captured CG chunks of 8 iterations; an even int: chunks
+        of that many iterations; False / 0: eager launches."""
         self._chk(self._L.yq_set_graph_mode(self._h, int(use_graphs)))
 
     def autotune(self, on=True):

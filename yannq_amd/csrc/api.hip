@@ -1543,8 +1543,15 @@ YQ_API int yq_prof_read(yq_ctx* c, const char* label, double* total_ms, long lon
 
 YQ_API int yq_set_graph_mode(yq_ctx* c, int use_graphs) {
   if (!c) return YQ_ERR_ARG;
+  if (use_graphs < 0 || use_graphs > 256 || (use_graphs > 1 && use_graphs % 2))
+    return fail(c, YQ_ERR_ARG, "graph mode: 0, 1 or an even chunk length <= 256");
+  const int chunk = use_graphs > 1 ? use_graphs : 8;
   c->mixed.use_graphs = use_graphs != 0;
-  for (yq_ctx* sub : c->subs) sub->mixed.use_graphs = use_graphs != 0;
+  c->mixed.chunk_iters = chunk;
+  for (yq_ctx* sub : c->subs) {
+    sub->mixed.use_graphs = use_graphs != 0;
+    sub->mixed.chunk_iters = chunk;
+  }
   return YQ_OK;
 }
 
@@ -1751,6 +1758,7 @@ YQ_API int yq_multi_create(yq_ctx* c, int count, const yq_multi_desc* d, int swe
     }
     c->subs.push_back(sub);
     sub->mixed.use_graphs = c->mixed.use_graphs;
+    sub->mixed.chunk_iters = c->mixed.chunk_iters;
     sub->mixed.autotune = c->mixed.autotune;
     rc = yq_rbm_create(sub, YQ_DTYPE_F64, d[i].n, d[i].m, d[i].use_bias);
     if (rc == YQ_OK) rc = yq_ham_set(sub, &d[i].ham);

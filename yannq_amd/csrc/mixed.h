@@ -47,8 +47,9 @@ struct MixedWork {
   hipEvent_t ev[2] = {nullptr, nullptr};
   // captured CG chunks (index: Jacobi preconditioner off / on) and their shape
   bool use_graphs = true;
+  int chunk_iters = 8;  // CG iterations per captured graph and host poll (even)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  long gkey[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long gkey[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // Build the bf16 / float operands from the batch (sigma: ns x n int8,

@@ -12,8 +12,9 @@
 // A[row l%16][k 8*(l/16) .. +7] and B[k 8*(l/16) .. +7][col l%16] and
 // returns D[4*(l/16) + q][l%16], q = 0..3.
 //
-// The CG loop (Chronopoulos-Gear form) needs four launches per iteration and
-// no host round trip: the vector update carries its dot products as per-block
+// The CG loop (Chronopoulos-Gear form) needs three launches per iteration
+// (the fused reduction + vector update + split, and the two products) and no
+// host round trip: the kernels carry their dot products as per-block
 // partials, which the next kernel that needs the value sums itself in a fixed
 // order.
 #include <math.h>
@@ -37,7 +38,6 @@ constexpr int kRowPad = 512;    // ns padding
 constexpr int kMaxKs = 5;       // Kp <= 160 (n <= 159)
 constexpr int kMaxIt = 10;      // Kp / 16
 constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
-constexpr int kChunkIters = 8;  // CG iterations per captured graph / host poll
 
 // device control block of the CG loop: ints [stop flag, completed iterations,
 // iteration budget]
@@ -74,56 +74,30 @@ __device__ inline bf16x8 ld8(const uint16_t* p) {
 // and every wave of a consuming kernel sums the partials itself in a fixed
 // order (lane-strided sums, then an xor butterfly).  No fences, atomics or
 // counters are involved, and the value does not depend on scheduling.
-__device__ void block_partial(double v, double* out, int slot) {
-  __shared__ double red[256];
-  const int t = threadIdx.x;
-  red[t] = v;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
+// Fixed-order block sum of NV doubles over the first nwaves waves of the
+// block (xor butterfly over the lanes, then the waves in index order);
+// thread 0 stores total k to out[k][slot].  Every thread of the block must
+// call it (one barrier inside).
+template <int NV>
+__device__ void block_sums(const double (&v)[NV], double* const (&out)[NV], int slot,
+                           int nwaves) {
+  __shared__ double red[NV][16];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) x += __shfl_xor(x, w);
+    if (lane == 0 && wave < nwaves) red[k][wave] = x;
   }
-  if (t == 0) out[slot] = red[0];
-}
-__device__ void block_partial2(double v1, double v2, double* out1, double* out2) {
-  __shared__ double r1[256], r2[256];
-  const int t = threadIdx.x;
-  r1[t] = v1;
-  r2[t] = v2;
   __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) {
-      r1[t] += r1[t + w];
-      r2[t] += r2[t + w];
+  if (t == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double x = 0.0;
+      for (int w = 0; w < nwaves; ++w) x += red[k][w];
+      out[k][slot] = x;
     }
-    __syncthreads();
-  }
-  if (t == 0) {
-    out1[blockIdx.x] = r1[0];
-    out2[blockIdx.x] = r2[0];
-  }
-}
-__device__ void block_partial3(double v1, double v2, double v3, double* out1, double* out2,
-                               double* out3, int slot) {
-  __shared__ double r1[256], r2[256], r3[256];
-  const int t = threadIdx.x;
-  r1[t] = v1;
-  r2[t] = v2;
-  r3[t] = v3;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) {
-      r1[t] += r1[t + w];
-      r2[t] += r2[t + w];
-      r3[t] += r3[t + w];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    out1[slot] = r1[0];
-    out2[slot] = r2[0];
-    out3[slot] = r3[0];
-  }
 }
 // total of cnt partials, identical in every lane of every wave: lane l adds
 // partials l, l + 64, ... in order (loaded 8 at a time so that the loads are
@@ -217,106 +191,6 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
   Tt[(size_t)j * nsp + s] = v;
 }
 
-// One Chronopoulos-Gear CG step for (S + shift) x = b with preconditioner M
-// (M = diag(dinv), or the identity when dinv is null), fused with the bf16
-// split of the vector the next product needs.  With u = M r, w = (S + shift) u,
-// gamma = <r, u> and delta = <w, u> (as partials of the previous launches):
-//   mode 0 (start):  u = M r only
-//   mode 1 (first):  alpha = gamma / delta, p = u, s = w
-//   mode 2:          beta = gamma / gamma_prev,
-//                    alpha = gamma / (delta - beta gamma / alpha_prev),
-//                    p = u + beta p, s = w + beta s
-// then (modes 1, 2) x += alpha p, r -= alpha s, u = M r.  The new u is stored
-// and split into V' on the (Kp x Np) grid in three bf16 layers stored
-// column-major, vs[(layer * Np + j) * Kp + k]; the partials of <r, u>,
-// <r, r> and <omean, u> of the new vectors go to g_out, rr_out and op.  The
-// recurrence needs one product per iteration (on u) and no separate vector
-// kernel: the update, the split and the three dot products share this launch.
-// alpha and gamma of iteration `par` are kept in control-block slot par & 1
-// for the next iteration.  Block: a 16 x 16 tile of the grid, read along j
-// (consecutive parameters) and written along k through an LDS transpose, so
-// both sides are coalesced.
-__global__ void __launch_bounds__(256)
-    k_cg_update(float* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
-                float* __restrict__ s, float* __restrict__ u, const float* __restrict__ w,
-                const float* __restrict__ dinv, int mode, int par,
-                const double* __restrict__ g_in, const double* __restrict__ rr_in, int nv,
-                const double* __restrict__ dl, int ndl, double* __restrict__ g_out,
-                double* __restrict__ rr_out, double* dctl, int* ictl,
-                const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
-                uint16_t* __restrict__ vs, double* __restrict__ op) {
-  __shared__ uint16_t tile[3][16][17];
-  const int t = threadIdx.x;
-  const int j0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
-  const size_t nm = (size_t)n * m;
-  float alpha = 0.0f, beta = 0.0f;
-  if (mode > 0) {
-    double rn2, gam, del;
-    psum3(rr_in, nv, g_in, nv, dl, ndl, rn2, gam, del);
-    // CG continuation predicate (uniform over the grid): stop once <r, r>
-    // falls below the threshold (or is NaN) or the iteration budget is spent;
-    // the later kernels of the iteration see the flag and return at once
-    if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
-      if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) ictl[kCtlStop] = 1;
-      return;
-    }
-    double a, bt = 0.0;
-    if (mode == 1) {
-      a = gam / del;
-    } else {
-      bt = gam / dctl[kCtlGamma + (par ^ 1)];
-      a = gam / (del - bt * gam / dctl[kCtlAlpha + (par ^ 1)]);
-    }
-    if (!__builtin_isfinite(a)) a = 0.0;  // breakdown (delta or its correction 0): no step
-    alpha = (float)a;
-    beta = (float)bt;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
-      dctl[kCtlAlpha + par] = a;
-      dctl[kCtlGamma + par] = gam;
-    }
-  }
-  {
-    const int kk = t >> 4, jj = t & 15;
-    const int k = k0 + kk, j = j0 + jj;
-    long c = -1;
-    if (k < n && j < m) c = (long)k * m + j;
-    else if (use_bias && k < n && j == m) c = (long)(nm + k);
-    else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
-    float v = 0.0f;
-    double d_om = 0.0, d_g = 0.0, d_rr = 0.0;
-    if (c >= 0) {
-      float rv = r[c];
-      const float di = dinv ? dinv[c] : 1.0f;
-      if (mode > 0) {
-        const float uo = u[c];
-        const float pv = mode == 1 ? uo : uo + beta * p[c];
-        const float sv = mode == 1 ? w[c] : w[c] + beta * s[c];
-        p[c] = pv;
-        s[c] = sv;
-        x[c] += alpha * pv;
-        rv = rv - alpha * sv;
-        r[c] = rv;
-      }
-      v = dinv ? rv * di : rv;
-      u[c] = v;
-      d_om = (double)om[c] * (double)v;
-      d_rr = (double)rv * (double)rv;
-      d_g = (double)rv * (double)v;
-    }
-    uint16_t hi, mid, lo;
-    split3(v, hi, mid, lo);
-    tile[0][jj][kk] = hi;
-    tile[1][jj][kk] = mid;
-    tile[2][jj][kk] = lo;
-    block_partial3(d_om, d_g, d_rr, op, g_out, rr_out,
-                   blockIdx.y * gridDim.x + blockIdx.x);  // contains barriers
-  }
-  const int jj = t >> 4, kk = t & 15;
-#pragma unroll
-  for (int sp = 0; sp < 3; ++sp)
-    vs[((size_t)sp * Np + j0 + jj) * Kp + k0 + kk] = tile[sp][jj][kk];
-}
-
 // ypart[cg][s] = sum over the column tiles of group cg of T'_sj U'_sj.
 // The product is formed transposed, U'^T = V'^T sigma'^T (A = V'^T from LDS,
 // B = sigma'^T: the 8 consecutive k of one sample), so a lane's four results
@@ -325,16 +199,22 @@ __global__ void __launch_bounds__(256)
 // groups at the end.  Block: 4 waves x 64 samples (4 sample tiles per wave);
 // blockIdx.y = column group of CT tiles whose three V' layers are
 // staged once in LDS (rows padded by kLdsPad elements against bank
-// conflicts) and shared by the 4 waves.
+// conflicts) and shared by the 4 waves.  NKS = Kp / 32 k-steps (the sigma'
+// fragments live in registers).  The block is latency-bound, so every
+// global load is issued as early as the registers allow: the T' of up to 4
+// column tiles is requested before the products that need it.
 constexpr int kLdsPad = 8;
-template <int CT>
+template <int CT, int NKS>
 __global__ void __launch_bounds__(256)
     k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
                const float* __restrict__ Trm, int Kp, int Np, int nsp,
-               float* __restrict__ ypart, const int* __restrict__ ictl) {
+               float* __restrict__ ypart, int* __restrict__ ictl, int count) {
   extern __shared__ uint16_t bsh[];  // [3][CT*16][Kp + kLdsPad]
   if (ictl[kCtlStop]) return;
   const int tid = threadIdx.x;
+  // count: this product belongs to a CG iteration whose update ran (one
+  // writer; the next update reads the counter)
+  if (count && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) ictl[kCtlIters] += 1;
   const int cg = blockIdx.y;
   const int col0 = cg * CT * 16;
   const int ncols = min(CT * 16, Np - col0);
@@ -343,13 +223,12 @@ __global__ void __launch_bounds__(256)
   const int wave = tid >> 6, l = tid & 63;
   const int li = l & 15, lg = l >> 4;
   const int rbase = blockIdx.x * 256 + wave * 64;
-  const int nks = Kp / 32;
-  // The prologue's global loads overlap: the V' slab goes through registers
-  // into LDS in two passes (at most 16 16-byte pieces per thread in all, the
-  // LDS budget of the slab), and this wave's sigma' fragments are issued
-  // between the first pass's loads and its LDS stores.
-  constexpr int kStage = (3 * CT * 16 * 4 * kMaxKs + 255) / 256 < 16
-                             ? (3 * CT * 16 * 4 * kMaxKs + 255) / 256 : 16;
+  // The V' slab goes through registers into LDS in two passes (at most 16
+  // 16-byte pieces per thread in all, the LDS budget of the slab), and this
+  // wave's sigma' fragments are issued between the first pass's loads and its
+  // LDS stores.
+  constexpr int kStage = (3 * CT * 16 * 4 * NKS + 255) / 256 < 16
+                             ? (3 * CT * 16 * 4 * NKS + 255) / 256 : 16;
   constexpr int kPass = (kStage + 1) / 2;
   const int total = 3 * ncols * kchunks;
   u16x8 stg[kPass];
@@ -378,44 +257,58 @@ __global__ void __launch_bounds__(256)
     }
   };
   stage_load(0);
-  bf16x8 sfr[4][kMaxKs];
+  bf16x8 sfr[4][NKS];
 #pragma unroll
   for (int st = 0; st < 4; ++st)
 #pragma unroll
-    for (int ks = 0; ks < kMaxKs; ++ks)
-      if (ks < nks) sfr[st][ks] = ld8(sigA + (size_t)(rbase + st * 16 + li) * Kp + ks * 32 + lg * 8);
+    for (int ks = 0; ks < NKS; ++ks)
+      sfr[st][ks] = ld8(sigA + (size_t)(rbase + st * 16 + li) * Kp + ks * 32 + lg * 8);
   stage_store(0);
   stage_load(1);
   stage_store(1);
+  constexpr int PF = CT < 4 ? CT : 4;  // column tiles whose T' is in flight together
+  const int nct = ncols / 16;
+  floatx4 t[PF][4];
+  auto load_t = [&](int cb) {
+#pragma unroll
+    for (int q = 0; q < PF; ++q)
+      if (cb + q < nct) {
+        const int j0 = col0 + (cb + q) * 16 + 4 * lg;
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+          t[q][st] =
+              *reinterpret_cast<const floatx4*>(Trm + (size_t)(rbase + st * 16 + li) * Np + j0);
+      }
+  };
+  load_t(0);
   __syncthreads();
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const int nct = ncols / 16;
-  for (int ct = 0; ct < nct; ++ct) {
-    // this tile's T' is loaded ahead of its products
-    floatx4 t[4];
-    const int j0 = col0 + ct * 16 + 4 * lg;
+  for (int cb = 0; cb < nct; cb += PF) {
+    if (cb > 0) load_t(cb);
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
-      t[st] = *reinterpret_cast<const floatx4*>(Trm + (size_t)(rbase + st * 16 + li) * Np + j0);
-    floatx4 c[4];
+    for (int q = 0; q < PF; ++q) {
+      const int ct = cb + q;
+      if (ct < nct) {
+        floatx4 c[4];
 #pragma unroll
-    for (int st = 0; st < 4; ++st) c[st] = (floatx4){0.f, 0.f, 0.f, 0.f};
+        for (int st = 0; st < 4; ++st) c[st] = (floatx4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int sp = 0; sp < 3; ++sp)
+        for (int sp = 0; sp < 3; ++sp)
 #pragma unroll
-      for (int ks = 0; ks < kMaxKs; ++ks)
-        if (ks < nks) {
-          const bf16x8 v =
-              ld8(bsh + (sp * CT * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
+          for (int ks = 0; ks < NKS; ++ks) {
+            const bf16x8 v =
+                ld8(bsh + (sp * CT * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
 #pragma unroll
-          for (int st = 0; st < 4; ++st)
-            c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v, sfr[st][ks], c[st], 0, 0, 0);
-        }
-    // c[st][q] = U'^T[col0 + ct*16 + 4*lg + q][rbase + st*16 + li]
+            for (int st = 0; st < 4; ++st)
+              c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v, sfr[st][ks], c[st], 0, 0, 0);
+          }
+        // c[st][e] = U'^T[col0 + ct*16 + 4*lg + e][rbase + st*16 + li]
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
-      acc[st] += ((t[st][0] * c[st][0] + t[st][1] * c[st][1]) +
-                  (t[st][2] * c[st][2] + t[st][3] * c[st][3]));
+        for (int st = 0; st < 4; ++st)
+          acc[st] += ((t[q][st][0] * c[st][0] + t[q][st][1] * c[st][1]) +
+                      (t[q][st][2] * c[st][2] + t[q][st][3] * c[st][3]));
+      }
+    }
   }
 #pragma unroll
   for (int st = 0; st < 4; ++st) {
@@ -426,10 +319,26 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// forward kernel for (CT, Kp / 32)
+template <int CT>
+auto fwd_kernel_ks(int nks) {
+  switch (nks) {
+    case 1: return k_fwd_mfma<CT, 1>;
+    case 2: return k_fwd_mfma<CT, 2>;
+    case 3: return k_fwd_mfma<CT, 3>;
+    case 4: return k_fwd_mfma<CT, 4>;
+    default: return k_fwd_mfma<CT, 5>;
+  }
+}
+auto fwd_kernel(int ct, int nks) {
+  return ct == 2 ? fwd_kernel_ks<2>(nks) : ct == 8 ? fwd_kernel_ks<8>(nks) : fwd_kernel_ks<4>(nks);
+}
+
 // part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj with the centred
 // forward result y_s = (sum_cg ypart - <omean, p>) / ns (0 for padded rows),
 // formed in LDS for the chunk; blocks of column group 0 also store the chunk
-// sum of y (ys, the centring correction of the reduction).  Block: 4 waves =
+// sums of y (ys, the centring correction of the reduction) and of y^2 (yy:
+// <u, S u> = ns sum_s y_s^2, the CG curvature).  Block: 4 waves =
 // 4 column tiles of one chunk; the chunk's sigma'^T slab is staged in LDS in
 // slices of kBwdHalf samples (sized by Kp, so the LDS per block stays small
 // and several blocks share a CU) and shared by the waves; every wave keeps
@@ -453,7 +362,7 @@ __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
                int nop, int Kp, int Np, int nsp, float* __restrict__ part,
-               double* __restrict__ ys, const int* __restrict__ ictl) {
+               double* __restrict__ ys, double* __restrict__ yy, const int* __restrict__ ictl) {
   if (ictl[kCtlStop]) return;
   extern __shared__ uint16_t ash[];  // [Kp][kBwdHalf + kLdsPad]
   __shared__ float ysh[CHUNK];
@@ -468,7 +377,7 @@ __global__ void __launch_bounds__(256)
   const int lda = kBwdHalf + kLdsPad;
   {
     const float ov = (float)psum(op, nop);
-    double dv = 0.0;
+    double dv = 0.0, dv2 = 0.0;
     for (int q = tid; q < CHUNK; q += 256) {
       const int s = kc * CHUNK + q;
       float v = 0.0f;
@@ -483,8 +392,13 @@ __global__ void __launch_bounds__(256)
       const float yv = s < ns ? (v - ov) / (float)ns : 0.0f;
       ysh[q] = yv;
       dv += (double)yv;
+      dv2 += (double)yv * (double)yv;
     }
-    if (cx == 0) block_partial(dv, ys, kc);  // uniform per block
+    if (cx == 0) {  // uniform per block
+      const double v2[2] = {dv, dv2};
+      double* const o2[2] = {ys, yy};
+      block_sums<2>(v2, o2, kc, 4);
+    }
   }
   floatx4 acc[kMaxIt];
 #pragma unroll
@@ -572,51 +486,150 @@ __global__ void __launch_bounds__(256)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
 
-// out[c] = G'[aug(c)] - omean_c sum(y) + shift u_c, G' summed over the
-// chunks (u: the vector the product was formed for).  Block: 64 parameters x
-// 4 chunk groups (wave g sums chunks g, g+4, ... in order; the four group
-// sums are combined as (g0 + g1) + (g2 + g3)); pap = partials of <u, out>.
-// count: this product completes a CG iteration (one writer increments the
-// iteration counter).
-__global__ void __launch_bounds__(256)
-    k_bwd_reduce(const float* __restrict__ part, int chunks, int n, int m, int use_bias,
-                 int Kp, int Np, const float* __restrict__ om, const double* __restrict__ dctl,
-                 const double* __restrict__ ys, int nys, const float* __restrict__ p,
-                 float* __restrict__ out, double* __restrict__ pap, int* __restrict__ ictl,
-                 int count) {
-  __shared__ float grp[4][64];
-  if (ictl[kCtlStop]) return;
-  if (count && blockIdx.x == 0 && threadIdx.x == 0) ictl[kCtlIters] += 1;  // one writer
-  const double shift = dctl[kCtlShift];
+// One Chronopoulos-Gear CG step for (S + shift) x = b with preconditioner M
+// (M = diag(dinv), or the identity when dinv is null), fused with the
+// reduction of the previous product and the bf16 split of the vector the
+// next product needs.  With u = M r, w = (S + shift) u, gamma = <r, u> and
+// delta = <w, u> = ns sum_s y_s^2 + shift <u, u> (partials of earlier
+// launches):
+//   mode 0 (start):   u = M r
+//   mode 1 (first):   alpha = gamma / delta, p = u, s = w
+//   mode 2:           beta = gamma / gamma_prev,
+//                     alpha = gamma / (delta - beta gamma / alpha_prev),
+//                     p = u + beta p, s = w + beta s
+//   mode 3 (product): out = w only
+// then (modes 1, 2) x += alpha p, r -= alpha s, u = M r.  w is formed here
+// from the backward chunk partials, w_c = G'[aug(c)] - omean_c sum(y) +
+// shift u_c.  The new u is stored and split into V' on the (Kp x Np) grid in
+// three bf16 layers stored column-major, vs[(layer * Np + j) * Kp + k]; the
+// partials of <omean, u>, <r, u>, <r, r> and <u, u> of the new vectors go to
+// op, g_out, rr_out and uu_out.  The predicate and the step sizes read the
+// slots of parity par and write those of par ^ 1; alpha and gamma of this
+// iteration are kept in control-block slot par for the next one.
+// Block: 1024 threads on a 16 x 16 tile of the grid.  Four groups of 256
+// threads sum the chunk partials g, g + 4, ... of the tile (8 loads in flight
+// per thread; combined as (g0 + g1) + (g2 + g3)); group 0 then updates the
+// vectors, read along j (consecutive parameters), and writes V' along k
+// through an LDS transpose.
+__global__ void __launch_bounds__(1024)
+    k_cg_step(float* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
+              float* __restrict__ s, float* __restrict__ u, float* __restrict__ out,
+              const float* __restrict__ dinv, int mode, int par,
+              const double* __restrict__ g_in, const double* __restrict__ rr_in,
+              const double* __restrict__ uu_in, int nv, const double* __restrict__ ys,
+              const double* __restrict__ yy, int nys, int ns, double* __restrict__ g_out,
+              double* __restrict__ rr_out, double* __restrict__ uu_out, double* __restrict__ op,
+              double* dctl, int* ictl, const float* __restrict__ part, int chunks,
+              const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
+              uint16_t* __restrict__ vs) {
+  __shared__ float grp[3][256];
+  __shared__ uint16_t tile[3][16][17];
+  const int t = threadIdx.x, g = t >> 8, e = t & 255;
+  const int j0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
   const size_t nm = (size_t)n * m;
-  const size_t dim = use_bias ? nm + n + m : nm;
-  const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const size_t c = (size_t)blockIdx.x * 64 + l;
-  const float sy = (float)psum(ys, nys);
-  float acc = 0.f;
-  if (c < dim) {
-    const size_t a = aug_index(c, n, m, Np);
-    const size_t stride = (size_t)Kp * Np;
-    for (int k0 = g; k0 < chunks; k0 += 32) {  // 8 loads in flight, sums in chunk order
-      float v[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        v[q] = k0 + 4 * q < chunks ? part[(size_t)(k0 + 4 * q) * stride + a] : 0.0f;
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (k0 + 4 * q < chunks) acc += v[q];
+  const int kk = e >> 4, jj = e & 15;
+  const int k = k0 + kk, j = j0 + jj;
+  long c = -1;
+  if (k < n && j < m) c = (long)k * m + j;
+  else if (use_bias && k < n && j == m) c = (long)(nm + k);
+  else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
+  const double shift = dctl[kCtlShift];
+  float alpha = 0.0f, beta = 0.0f;
+  if (mode == 1 || mode == 2) {
+    double rn2, gam, uu;
+    psum3(rr_in, nv, g_in, nv, uu_in, nv, rn2, gam, uu);
+    // CG continuation predicate (uniform over the grid): stop once <r, r>
+    // falls below the threshold (or is NaN) or the iteration budget is spent;
+    // the later kernels of the iteration see the flag and return at once
+    if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
+      if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) ictl[kCtlStop] = 1;
+      return;
+    }
+    const double del = (double)ns * psum(yy, nys) + shift * uu;
+    double a, bt = 0.0;
+    if (mode == 1) {
+      a = gam / del;
+    } else {
+      bt = gam / dctl[kCtlGamma + (par ^ 1)];
+      a = gam / (del - bt * gam / dctl[kCtlAlpha + (par ^ 1)]);
+    }
+    if (!__builtin_isfinite(a)) a = 0.0;  // breakdown (delta or its correction 0): no step
+    alpha = (float)a;
+    beta = (float)bt;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+      dctl[kCtlAlpha + par] = a;
+      dctl[kCtlGamma + par] = gam;
     }
   }
-  grp[g][l] = acc;
-  __syncthreads();
-  double dv = 0.0;
-  if (g == 0 && c < dim) {
-    const float gs = (grp[0][l] + grp[1][l]) + (grp[2][l] + grp[3][l]);
-    const float o = (gs - om[c] * sy) + (float)shift * p[c];
-    out[c] = o;
-    dv = (double)p[c] * (double)o;
+  float wv = 0.0f;
+  if (mode != 0) {
+    float acc = 0.0f;
+    if (c >= 0) {
+      const size_t a = (size_t)k * Np + j;  // aug(c)
+      const size_t stride = (size_t)Kp * Np;
+      for (int q0 = g; q0 < chunks; q0 += 32) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          v[q] = q0 + 4 * q < chunks ? part[(size_t)(q0 + 4 * q) * stride + a] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (q0 + 4 * q < chunks) acc += v[q];
+      }
+    }
+    if (g > 0) grp[g - 1][e] = acc;
+    // sum(y) in every lane (psum shuffles across the wave, so no lane may be
+    // masked off here)
+    const float sy = (float)psum(ys, nys);
+    __syncthreads();
+    if (g == 0 && c >= 0) {
+      const float gs = (acc + grp[0][e]) + (grp[1][e] + grp[2][e]);
+      wv = (gs - om[c] * sy) + (float)shift * u[c];
+    }
+    if (mode == 3) {
+      if (g == 0 && c >= 0) out[c] = wv;
+      return;  // uniform per launch
+    }
   }
-  block_partial(dv, pap, blockIdx.x);
+  double d_om = 0.0, d_g = 0.0, d_rr = 0.0, d_uu = 0.0;
+  if (g == 0) {
+    float v = 0.0f;
+    if (c >= 0) {
+      float rv = r[c];
+      if (mode > 0) {
+        const float uo = u[c];
+        const float pv = mode == 1 ? uo : uo + beta * p[c];
+        const float sv = mode == 1 ? wv : wv + beta * s[c];
+        p[c] = pv;
+        s[c] = sv;
+        x[c] += alpha * pv;
+        rv = rv - alpha * sv;
+        r[c] = rv;
+      }
+      v = dinv ? rv * dinv[c] : rv;
+      u[c] = v;
+      d_om = (double)om[c] * (double)v;
+      d_rr = (double)rv * (double)rv;
+      d_g = (double)rv * (double)v;
+      d_uu = (double)v * (double)v;
+    }
+    uint16_t hi, mid, lo;
+    split3(v, hi, mid, lo);
+    tile[0][jj][kk] = hi;
+    tile[1][jj][kk] = mid;
+    tile[2][jj][kk] = lo;
+  }
+  {
+    const double v4[4] = {d_om, d_g, d_rr, d_uu};
+    double* const o4[4] = {op, g_out, rr_out, uu_out};
+    block_sums<4>(v4, o4, blockIdx.y * gridDim.x + blockIdx.x, 4);  // barrier inside
+  }
+  if (g == 0) {
+    const int tj = e >> 4, tk = e & 15;
+#pragma unroll
+    for (int sp = 0; sp < 3; ++sp)
+      vs[((size_t)sp * Np + j0 + tj) * Kp + k0 + tk] = tile[sp][tj][tk];
+  }
 }
 
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
@@ -669,109 +682,106 @@ int alloc(T*& p, size_t count, std::string& err) {
 }
 
 // block counts of the partial-sum producers and their offsets in w.red:
-// gamma = <r, u> and <r, r> alternate between two slots (iteration parity)
+// gamma = <r, u>, <r, r> and <u, u> alternate between two slots (iteration
+// parity)
 struct RedLayout {
-  int nbv, nby, nbr;
-  size_t g[2], rr[2], op, ys, pap, total;
+  int nbv, nby;
+  size_t g[2], rr[2], uu[2], op, ys, yy, total;
 };
 RedLayout red_layout(const MixedWork& w, int chunk = 0) {
   RedLayout L;
   if (chunk == 0) chunk = w.chunk;
-  L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_cg_update tiles
+  L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_cg_step tiles
   L.nby = w.nsp / chunk;                      // k_bwd_mfma chunks
-  L.nbr = (int)((w.dim + 63) / 64);           // k_bwd_reduce
-  L.g[0] = 0;
-  L.g[1] = L.g[0] + L.nbv;
-  L.rr[0] = L.g[1] + L.nbv;
-  L.rr[1] = L.rr[0] + L.nbv;
-  L.op = L.rr[1] + L.nbv;
+  size_t o = 0;
+  for (int q = 0; q < 2; ++q) {
+    L.g[q] = o;
+    L.rr[q] = o + L.nbv;
+    L.uu[q] = o + 2 * L.nbv;
+    o += 3 * L.nbv;
+  }
+  L.op = o;
   L.ys = L.op + L.nbv;
-  L.pap = L.ys + L.nby;
-  L.total = L.pap + L.nbr;
+  L.yy = L.ys + L.nby;
+  L.total = L.yy + L.nby;
   return L;
 }
 
-// w.Ap = (S + shift) u for the u last split into w.vsplit by k_cg_update,
-// with the products on MFMA, leaving the partials of <u, w.Ap> in w.red;
-// shift comes from the control block, and every kernel returns at once when
-// the stop flag is set.  count: the product ends a CG iteration.
+// the two products for the u last split into w.vsplit by k_cg_step, leaving
+// the backward chunk partials (reduced by the next k_cg_step); every kernel
+// returns at once when the stop flag is set.  count: the product belongs to a
+// CG iteration.
 void product_launches(MixedWork& w, int count, hipStream_t st) {
   const RedLayout L = red_layout(w);
-  double* op = w.red + L.op;
-  double* ys = w.red + L.ys;
   {
     ProfGuard g("fwd_mfma", st);
     const size_t lds = sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad);
-    auto fwd = w.ct == 2 ? k_fwd_mfma<2> : w.ct == 8 ? k_fwd_mfma<8> : k_fwd_mfma<4>;
-    hipLaunchKernelGGL(fwd, dim3(w.nsp / 256, w.ncg), dim3(256), lds, st, w.sigA,
-                       w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl);
+    hipLaunchKernelGGL(fwd_kernel(w.ct, w.Kp / 32), dim3(w.nsp / 256, w.ncg), dim3(256), lds, st,
+                       w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
   }
   const int chunks = w.nsp / w.chunk;
-  {
-    ProfGuard g("bwd_mfma", st);
-    const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdHalf + kLdsPad);
-    auto bwd = w.chunk == 128 ? k_bwd_mfma<128> : w.chunk == 512 ? k_bwd_mfma<512>
-                                                                 : k_bwd_mfma<256>;
-    hipLaunchKernelGGL(bwd, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds, st, w.sigT,
-                       w.Tt, w.ypart, w.ncg, w.ns, op, L.nbv, w.Kp, w.Np, w.nsp, w.part, ys,
-                       w.ictl);
-  }
-  ProfGuard g("bwd_reduce", st);
-  hipLaunchKernelGGL(k_bwd_reduce, dim3(L.nbr), dim3(256), 0, st, w.part, chunks, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.om32, w.dctl, ys, L.nby, w.u, w.Ap,
-                     w.red + L.pap, w.ictl, count);
+  ProfGuard g("bwd_mfma", st);
+  const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdHalf + kLdsPad);
+  auto bwd = w.chunk == 128 ? k_bwd_mfma<128> : w.chunk == 512 ? k_bwd_mfma<512>
+                                                               : k_bwd_mfma<256>;
+  hipLaunchKernelGGL(bwd, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds, st, w.sigT, w.Tt,
+                     w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp, w.part,
+                     w.red + L.ys, w.red + L.yy, w.ictl);
 }
 
-// k_cg_update in the given mode for iteration parity par (mode 0: u = M r
-// into the gamma / <r, r> slot 0)
-void update_launch(MixedWork& w, const float* dinv, int mode, int par, hipStream_t st) {
+// k_cg_step in the given mode for iteration parity par (mode 0 writes slot 0)
+void step_launch(MixedWork& w, const float* dinv, int mode, int par, hipStream_t st) {
   const RedLayout L = red_layout(w);
-  double* g_out = w.red + (mode == 0 ? L.g[0] : L.g[par ^ 1]);
-  double* rr_out = w.red + (mode == 0 ? L.rr[0] : L.rr[par ^ 1]);
-  ProfGuard g("cg_update", st);
-  hipLaunchKernelGGL(k_cg_update, dim3(w.Np / 16, w.Kp / 16), dim3(256), 0, st, w.x, w.r, w.p,
-                     w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par], L.nbv,
-                     w.red + L.pap, L.nbr, g_out, rr_out, w.dctl, w.ictl, w.om32, w.n, w.m,
-                     w.use_bias, w.Kp, w.Np, w.vsplit, w.red + L.op);
+  const int wr = mode == 0 ? 0 : par ^ 1;
+  ProfGuard g("cg_step", st);
+  hipLaunchKernelGGL(k_cg_step, dim3(w.Np / 16, w.Kp / 16), dim3(1024), 0, st, w.x, w.r, w.p,
+                     w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par],
+                     w.red + L.uu[par], L.nbv, w.red + L.ys, w.red + L.yy, L.nby, w.ns,
+                     w.red + L.g[wr], w.red + L.rr[wr], w.red + L.uu[wr], w.red + L.op, w.dctl,
+                     w.ictl, w.part, w.nsp / w.chunk, w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np,
+                     w.vsplit);
 }
 
-// (S + shift) v for v in w.r, without preconditioning: the split (mode 0)
-// and the product; the result lands in w.Ap
+// w.Ap = (S + shift) v for v in w.r, without preconditioning: the split
+// (mode 0), the products and the reduction (mode 3)
 void apply_launches(MixedWork& w, hipStream_t st) {
-  update_launch(w, nullptr, 0, 0, st);
+  step_launch(w, nullptr, 0, 0, st);
   product_launches(w, 0, st);
+  step_launch(w, nullptr, 3, 0, st);
 }
 
-// one CG iteration (parity par): the update, then the product on the new u
+// one CG iteration (parity par): the step, then the products on the new u
 void iteration_launches(MixedWork& w, const float* dinv, int par, bool first, hipStream_t st) {
-  update_launch(w, dinv, first ? 1 : 2, par, st);
+  step_launch(w, dinv, first ? 1 : 2, par, st);
   product_launches(w, 1, st);
 }
 
 // shape of the captured launches (a different shape needs a new capture)
-void graph_key(const MixedWork& w, long key[10]) {
-  const long k[10] = {w.ns, w.n, w.m, w.use_bias, w.Kp, w.Np, w.nsp, (long)w.dim, w.ct, w.chunk};
-  for (int i = 0; i < 10; ++i) key[i] = k[i];
+void graph_key(const MixedWork& w, long key[11]) {
+  const long k[11] = {w.ns, w.n,  w.m,  w.use_bias, w.Kp,         w.Np,
+                      w.nsp, (long)w.dim, w.ct, w.chunk, w.chunk_iters};
+  for (int i = 0; i < 11; ++i) key[i] = k[i];
 }
 
-// kChunkIters iterations (absolute iterations 1 + 8q + p have parity
-// (1 + p) & 1) captured once per operand shape and preconditioner setting
+// w.chunk_iters iterations (an even count: absolute iterations 1 + q chunk + p
+// have parity (1 + p) & 1) captured once per operand shape, chunk length and
+// preconditioner setting
 int mixed_graph(MixedWork& w, const float* dinv, hipStream_t st, hipGraphExec_t* out,
                 std::string& err) {
-  long key[10];
+  long key[11];
   graph_key(w, key);
   bool same = true;
-  for (int i = 0; i < 10; ++i) same = same && key[i] == w.gkey[i];
+  for (int i = 0; i < 11; ++i) same = same && key[i] == w.gkey[i];
   if (!same) {
     mixed_drop_graphs(w);
-    for (int i = 0; i < 10; ++i) w.gkey[i] = key[i];
+    for (int i = 0; i < 11; ++i) w.gkey[i] = key[i];
   }
   const int slot = dinv ? 1 : 0;
   if (!w.gexec[slot]) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hipfail(err, "graph capture", e);
-    for (int p = 0; p < kChunkIters; ++p) iteration_launches(w, dinv, (1 + p) & 1, false, st);
+    for (int p = 0; p < w.chunk_iters; ++p) iteration_launches(w, dinv, (1 + p) & 1, false, st);
     e = hipStreamEndCapture(st, &g);
     if (e != hipSuccess) return hipfail(err, "graph capture", e);
     e = hipGraphInstantiate(&w.gexec[slot], g, nullptr, nullptr, 0);
@@ -968,11 +978,10 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
     hipLaunchKernelGGL(k_jacobi_inv, dim3(nb(dim)), dim3(256), 0, st, diag64, shift, w.dinv, dim);
     dinv = w.dinv;
   }
-  // start: u = M r with gamma and <r, r> in slot 0, and the first product
-  // w = (S + shift) u with its <u, w> partials (the control block is reset
-  // first so that the product kernels run)
+  // start: u = M r with gamma, <r, r> and <u, u> in slot 0, and the products
+  // on u (the control block is reset first so that the product kernels run)
   hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, 0.0, 1);
-  update_launch(w, dinv, 0, 0, st);
+  step_launch(w, dinv, 0, 0, st);
   product_launches(w, 0, st);
   double bn2 = 0.0;
   int rc = host_rr(w, L, 0, st, &bn2, err);
@@ -984,13 +993,13 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   }
   const double thr = tol * tol * bn2;
   hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, thr, max_iter);
-  // iteration 0 (no beta) eagerly, then chunks of kChunkIters iterations; an
+  // iteration 0 (no beta) eagerly, then chunks of w.chunk_iters iterations; an
   // iteration after the predicate fired leaves everything untouched, so the
   // chunk after the one that converged may already be queued
   iteration_launches(w, dinv, 0, true, st);
   hipError_t e;
-  const bool graphs = w.use_graphs && !prof_enabled("cg_update") && !prof_enabled("fwd_mfma") &&
-                      !prof_enabled("bwd_mfma") && !prof_enabled("bwd_reduce");
+  const bool graphs = w.use_graphs && !prof_enabled("cg_step") && !prof_enabled("fwd_mfma") &&
+                      !prof_enabled("bwd_mfma");
   hipGraphExec_t gx = nullptr;
   if (graphs && (rc = mixed_graph(w, dinv, st, &gx, err))) return rc;
   // chunk c copies the control block into host slot c & 1 and records
@@ -1001,10 +1010,10 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
       hipError_t r = hipGraphLaunch(gx, st);
       if (r != hipSuccess) return r;
     } else {
-      for (int k = 0; k < kChunkIters; ++k)
+      for (int k = 0; k < w.chunk_iters; ++k)
         iteration_launches(w, dinv, (launched + k) & 1, false, st);
     }
-    launched += kChunkIters;
+    launched += w.chunk_iters;
     hipError_t r = hipMemcpyAsync(w.host_ctl + 4 * (issued & 1), w.ictl, sizeof(int) * 4,
                                   hipMemcpyDeviceToHost, st);
     if (r == hipSuccess) r = hipEventRecord(w.ev[issued & 1], st);
