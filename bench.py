@@ -46,15 +46,20 @@ KERNEL_WORK = {
     "fwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4,
     # backward product: T'^T and sigma'^T and y in, the (n+1) x (m+1) chunk partials out
     "bwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4 + (NS // 256) * KP * NP * 4,
-    # CG step: the chunk partials in; x, r, p, s, u, omean in; x, r, p, s, u and
-    # the three bf16 layers of V' out
-    "cg_step": (NS // 256) * KP * NP * 4 + DIM * 4 * 11 + 3 * KP * NP * 2,
+    # CG step: the chunk partials of the dim parameters in; x, r, p, s, u, omean
+    # in; x, r, p, s, u and the three bf16 layers of V' out
+    "cg_step": (NS // 256) * DIM * 4 + DIM * 4 * 11 + 3 * KP * NP * 2,
     # sampler: spins in/out and the recorded samples (spins, float theta)
     "sample": NS * N * 3 + NS * M * 4,
     # E_loc ratios: theta (float) and spins in, ratios (double) out
     "eloc_ratio": NS * M * 4 + NS * N + NS * N * 8,
 }
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) of this script, each counter scaled by what it reads
+# for a copy of known size (bench.py --copy-probe); written with the command
+# lines that produced it
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_tfi_n100_a4.json")
 
 
 AUTOTUNE = False
@@ -146,6 +151,32 @@ def cpu_baseline(iters_per_step):
     }
 
 
+def copy_probe():
+    """Calibration load for the PMC traffic counters: four device copies of
+    1 GiB each (1 GiB read and 1 GiB written per copy)."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_copy": 4 * n, "copies": 4}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC measurement, or
+    None when there is none for it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +189,12 @@ def main():
                     help="time the mixed products' tile shapes per operand shape")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the copy that calibrates the PMC traffic counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -189,7 +225,8 @@ def main():
     achieved = amount / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": launches,
+                "traffic": measured_traffic(dominant), "avg_launch_us": avg_s * 1e6,
+                "launches": launches,
                 "algorithmic_bytes_per_launch": amount,
                 "share_of_step": tot_ms / (1e3 * dtk),
                 "timing": "HIP events around each launch in a second pass over the same steps"}

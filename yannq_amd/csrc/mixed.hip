@@ -525,7 +525,11 @@ __global__ void __launch_bounds__(1024)
   __shared__ float grp[3][256];
   __shared__ uint16_t tile[3][16][17];
   const int t = threadIdx.x, g = t >> 8, e = t & 255;
-  const int j0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
+  // a tile row of the chunk partials is 64 bytes, half an L2 line: the
+  // XCD-aware numbering puts tiles (2i, 2i + 1) of a row on one XCD so that
+  // the line is fetched from HBM once
+  const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int j0 = (lb % gridDim.x) * 16, k0 = (lb / gridDim.x) * 16;
   const size_t nm = (size_t)n * m;
   const int kk = e >> 4, jj = e & 15;
   const int k = k0 + kk, j = j0 + jj;
@@ -542,7 +546,7 @@ __global__ void __launch_bounds__(1024)
     // falls below the threshold (or is NaN) or the iteration budget is spent;
     // the later kernels of the iteration see the flag and return at once
     if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
-      if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) ictl[kCtlStop] = 1;
+      if (lb == 0 && t == 0) ictl[kCtlStop] = 1;
       return;
     }
     const double del = (double)ns * psum(yy, nys) + shift * uu;
@@ -556,7 +560,7 @@ __global__ void __launch_bounds__(1024)
     if (!__builtin_isfinite(a)) a = 0.0;  // breakdown (delta or its correction 0): no step
     alpha = (float)a;
     beta = (float)bt;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+    if (lb == 0 && t == 0) {
       dctl[kCtlAlpha + par] = a;
       dctl[kCtlGamma + par] = gam;
     }
@@ -622,7 +626,7 @@ __global__ void __launch_bounds__(1024)
   {
     const double v4[4] = {d_om, d_g, d_rr, d_uu};
     double* const o4[4] = {op, g_out, rr_out, uu_out};
-    block_sums<4>(v4, o4, blockIdx.y * gridDim.x + blockIdx.x, 4);  // barrier inside
+    block_sums<4>(v4, o4, lb, 4);  // barrier inside
   }
   if (g == 0) {
     const int tj = e >> 4, tk = e & 15;
