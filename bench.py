@@ -229,7 +229,9 @@ def main():
                 "launches": launches,
                 "algorithmic_bytes_per_launch": amount,
                 "share_of_step": tot_ms / (1e3 * dtk),
-                "timing": "HIP events around each launch in a second pass over the same steps"}
+                "timing": "HIP events bound to each launch of the kernel (hipExtLaunchKernel) "
+                          "in a second pass over the same steps; the average includes the "
+                          "launches that return at once after the CG converged"}
 
     extra = {}
     fp64_iters = iters

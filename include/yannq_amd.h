@@ -26,6 +26,7 @@
 #ifndef YANNQ_AMD_H
 #define YANNQ_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -112,6 +113,38 @@ YQ_API int yq_rbm_update_params(yq_ctx* ctx, const double* delta);
 YQ_API int yq_rbm_init_random(yq_ctx* ctx, uint64_t seed, double sigma);
 /* hasNaN: RBM.hpp:293-296 */
 YQ_API int yq_rbm_has_nan(yq_ctx* ctx, int* out);
+
+/* ---- Checkpoints: Serializers/SerializeRBM.hpp + SerializeEigen.hpp ----- */
+/* The reference's weight files: cereal BinaryArchive of RBM<T>
+ * (SerializeRBM.hpp:21-79, Eigen matrices as SerializeEigen.hpp:10-35),
+ * written by RunRBM.hpp:155-165 as w%04d.dat through a std::unique_ptr
+ * (pointer_wrapper = 1: one leading "valid" byte) and read back by
+ * AbstractRunner.hpp:83-94; pointer_wrapper = 0 is the bare object form
+ * (TestSerializeRBM.cpp's second case).  The file does not name its scalar
+ * type, so the caller gives it: */
+typedef enum {
+  YQ_CKPT_F32 = 1,  /* RBM<float> */
+  YQ_CKPT_F64 = 2,  /* RBM<double> */
+  YQ_CKPT_C64 = 3,  /* RBM<std::complex<float>> */
+  YQ_CKPT_C128 = 4  /* RBM<std::complex<double>> */
+} yq_ckpt_elem;
+/* Bytes of the encoding (0 for invalid arguments).  Host only. */
+YQ_API size_t yq_ckpt_rbm_bytes(int elem, int n, int m, int use_bias, int pointer_wrapper);
+/* Encode parameters given in this ABI's layout (doubles; complex as (re, im)
+ * pairs; float files round to nearest).  Host only. */
+YQ_API int yq_ckpt_rbm_encode(int elem, int n, int m, int use_bias, const double* params,
+                              int pointer_wrapper, unsigned char* out, size_t size);
+/* Decode: n, m and use_bias always; params too when not null (params_len
+ * doubles available).  YQ_ERR_ARG when the bytes are not exactly such an
+ * encoding (wrong type, form, sizes or length).  Host only. */
+YQ_API int yq_ckpt_rbm_decode(int elem, const unsigned char* buf, size_t size,
+                              int pointer_wrapper, int* n, int* m, int* use_bias,
+                              double* params, size_t params_len);
+/* Write the context's machine to a file (elem must match real / complex). */
+YQ_API int yq_rbm_save(yq_ctx* ctx, const char* path, int elem, int pointer_wrapper);
+/* Read a file and make it the context's machine (yq_rbm_create with the
+ * file's n, m, useBias, then its parameters). */
+YQ_API int yq_rbm_load(yq_ctx* ctx, const char* path, int elem, int pointer_wrapper);
 
 /* ---- Hamiltonian: Hamiltonians/{TFIsing,XXZ,J1J2}.hpp ------------------ */
 /* A POD descriptor replaces the E_loc functor operator()(const State&):

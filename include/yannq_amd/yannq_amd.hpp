@@ -15,7 +15,10 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <fstream>
 #include <functional>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <ostream>
@@ -136,12 +139,39 @@ class RBM {
     detail::check(ctx(), yq_rbm_has_nan(ctx(), &out));
     return out != 0;
   }
+  // The reference's weight file (cereal binary archive of RBM<T> through a
+  // unique_ptr, Serializers/SerializeRBM.hpp:21-79), as RunRBM.hpp:155-165
+  // writes w%04d.dat.
+  void save(const std::string& path) const {
+    detail::check(ctx(), yq_rbm_save(ctx(), path.c_str(), ckptElem(), 1));
+  }
+  // Read such a file into this machine (AbstractRunner.hpp:83-94); its N, M
+  // and useBias must be this machine's.
+  void load(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw Error(YQ_ERR_ARG, "cannot open " + path);
+    const std::vector<unsigned char> buf((std::istreambuf_iterator<char>(in)),
+                                         std::istreambuf_iterator<char>());
+    int n = 0, m = 0, ub = 0;
+    Vector v(getDim());
+    const int rc = yq_ckpt_rbm_decode(ckptElem(), buf.data(), buf.size(), 1, &n, &m, &ub,
+                                      nullptr, 0);
+    if (rc != YQ_OK) throw Error(rc, path + ": not an RBM checkpoint of this scalar type");
+    if ((uint32_t)n != n_ || (uint32_t)m != m_ || (ub != 0) != useBias_)
+      throw Error(YQ_ERR_ARG, path + ": machine shape differs");
+    const size_t len = v.size() * (detail::is_complex<T>::value ? 2 : 1);
+    const int rc2 = yq_ckpt_rbm_decode(ckptElem(), buf.data(), buf.size(), 1, &n, &m, &ub,
+                                       detail::dbl(v.data()), len);
+    if (rc2 != YQ_OK) throw Error(rc2, path + ": malformed RBM checkpoint");
+    setParams(v);
+  }
   // W(j, i), A(i), B(j) readers (:205-216) through a parameter copy
   T W(uint32_t j, uint32_t i) const { return getParams()[(size_t)i * m_ + j]; }
   T A(uint32_t i) const { return useBias_ ? getParams()[(size_t)n_ * m_ + i] : T(0); }
   T B(uint32_t j) const { return useBias_ ? getParams()[(size_t)n_ * m_ + n_ + j] : T(0); }
 
  private:
+  static int ckptElem() { return detail::is_complex<T>::value ? YQ_CKPT_C128 : YQ_CKPT_F64; }
   Device* dev_;
   uint32_t n_, m_;
   bool useBias_;
@@ -513,6 +543,11 @@ class RunRBM {
     opt_ = createOptimizer<T>(name, alpha, p);
   }
   void setSeed(uint64_t s) { re_.seed(s); }
+  void initializeFrom(const std::string& filePath) {  // AbstractRunner.hpp:83-94
+    machineInitialized_ = true;
+    logger_ << "Loading initial weights from " << filePath << std::endl;
+    qs_.load(filePath);
+  }
   void initializeRandom(double wIni = 1e-3) {  // AbstractRunner.hpp:96-101
     machineInitialized_ = true;
     logger_ << "Set initial weights randomly from N(0.0, " << wIni << "^2)" << std::endl;
@@ -547,6 +582,11 @@ class RunRBM {
     int t = 0;
     for (int ll = 0; ll <= maxIter_; ll++) {
       logger_ << "Epochs: " << ll << "\t# of discard samples: " << nSweepsDiscard << std::endl;
+      if (saveWfPer_ != 0 && ll % saveWfPer_ == 0) {  // RunRBM.hpp:155-165
+        char fileName[30];
+        snprintf(fileName, sizeof(fileName), "w%04d.dat", ll);
+        qs_.save(fileName);
+      }
       sampler.randomize(randomizer);
       const auto smp_start = Clock::now();
       auto sr = sampler.sample(nSweeps, nSweepsDiscard);

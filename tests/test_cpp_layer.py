@@ -21,7 +21,7 @@ def _build(yq, tmp_path, name="run_tfi"):
     return exe
 
 
-@pytest.mark.parametrize("name", ["run_tfi", "run_xxz"])
+@pytest.mark.parametrize("name", ["run_tfi", "run_xxz", "run_ckpt"])
 def test_concept_layer_compiles(yq, tmp_path, name):
     assert os.path.exists(_build(yq, tmp_path, name))
 
@@ -47,3 +47,26 @@ def test_complex_runner_reaches_ground_state(yq, tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_runner_checkpoints(yq, tmp_path):
+    """w%04d.dat files written by RunRBM decode as the reference's format and
+    read back through initializeFrom."""
+    import numpy as np
+    from tests.conftest import _gpu_available
+    if not _gpu_available():
+        pytest.skip("no GPU visible")
+    exe = _build(yq, tmp_path, "run_ckpt")
+    work = tmp_path / "run"
+    work.mkdir()
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240, cwd=str(work))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n, m = 20, 20
+    sizes = {f: (work / f).stat().st_size for f in ("w0000.dat", "w0010.dat", "w0020.dat")}
+    assert set(sizes.values()) == {yq.lib().yq_ckpt_rbm_bytes(yq.CKPT_F64, n, m, 1, 1)}
+    decoded = [yq.ckpt_decode((work / f).read_bytes(), yq.CKPT_F64, True) for f in sorted(sizes)]
+    assert all(d[:3] == (n, m, True) for d in decoded)
+    # initializeRandom(0.01) at epoch 0, then SGD steps move the parameters
+    assert np.std(decoded[0][3]) < 0.05 and not np.array_equal(decoded[0][3], decoded[2][3])

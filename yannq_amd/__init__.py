@@ -109,11 +109,16 @@ def lib():
         "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
         "yq_ff_jacobian": [p, i, p, p, p], "yq_ff_forward_dev": [p, i, p, p, p],
         "yq_ff_jacobian_dev": [p, i, p, p, p], "yq_ff_vjp_dev": [p, i, p, p, p, p],
+        "yq_ckpt_rbm_encode": [i, i, i, i, p, i, p, C.c_size_t],
+        "yq_ckpt_rbm_decode": [i, p, C.c_size_t, i, p, p, p, p, C.c_size_t],
+        "yq_rbm_save": [p, C.c_char_p, i, i], "yq_rbm_load": [p, C.c_char_p, i, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = i
+    L.yq_ckpt_rbm_bytes.argtypes = [i, i, i, i, i]
+    L.yq_ckpt_rbm_bytes.restype = C.c_size_t
     L.yq_last_error.argtypes = [p]
     L.yq_last_error.restype = C.c_char_p
     _lib = L
@@ -122,6 +127,47 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# checkpoint element types (yq_ckpt_elem): RBM<float>, RBM<double>,
+# RBM<complex<float>>, RBM<complex<double>>
+CKPT_F32, CKPT_F64, CKPT_C64, CKPT_C128 = 1, 2, 3, 4
+
+
+def ckpt_encode(params, n, m, use_bias=True, elem=CKPT_F64, pointer=True) -> bytes:
+    """The reference's cereal binary encoding of an RBM (host only).  params:
+    the [W | a | b] layout; complex element types take a complex array."""
+    L = lib()
+    cplx = elem in (CKPT_C64, CKPT_C128)
+    v = np.ascontiguousarray(params, dtype=np.complex128 if cplx else np.float64)
+    nb = L.yq_ckpt_rbm_bytes(elem, n, m, int(use_bias), int(pointer))
+    dim = n * m + (n + m if use_bias else 0)
+    if nb == 0 or v.size != dim:
+        raise YqError(ERR_ARG, "ckpt_encode: bad sizes")
+    out = np.empty(nb, dtype=np.uint8)
+    rc = L.yq_ckpt_rbm_encode(elem, n, m, int(use_bias), _ptr(v), int(pointer), _ptr(out), nb)
+    if rc:
+        raise YqError(rc, "ckpt_encode failed")
+    return out.tobytes()
+
+
+def ckpt_decode(buf: bytes, elem=CKPT_F64, pointer=True):
+    """Inverse of ckpt_encode: (n, m, use_bias, params)."""
+    L = lib()
+    b = np.frombuffer(buf, dtype=np.uint8)
+    n, m, ub = C.c_int(), C.c_int(), C.c_int()
+    rc = L.yq_ckpt_rbm_decode(elem, _ptr(b), b.size, int(pointer), C.byref(n), C.byref(m),
+                              C.byref(ub), None, 0)
+    if rc:
+        raise YqError(rc, "not an RBM checkpoint of this element type and form")
+    cplx = elem in (CKPT_C64, CKPT_C128)
+    dim = n.value * m.value + (n.value + m.value if ub.value else 0)
+    v = np.empty(dim, dtype=np.complex128 if cplx else np.float64)
+    rc = L.yq_ckpt_rbm_decode(elem, _ptr(b), b.size, int(pointer), C.byref(n), C.byref(m),
+                              C.byref(ub), _ptr(v), v.size * (2 if cplx else 1))
+    if rc:
+        raise YqError(rc, "malformed RBM checkpoint")
+    return n.value, m.value, bool(ub.value), v
 
 
 class Context:
@@ -395,6 +441,21 @@ class Context:
 
     def sr_options(self, precond=0, warm_start=0):
         self._chk(self._L.yq_sr_set_options(self._h, precond, warm_start))
+
+    def rbm_save(self, path, elem=None, pointer=True):
+        """Write the machine as the reference's w%04d.dat files are written
+        (elem defaults to RBM<double> / RBM<complex<double>>)."""
+        if elem is None:
+            elem = CKPT_C128 if self.cplx else CKPT_F64
+        self._chk(self._L.yq_rbm_save(self._h, os.fsencode(path), elem, int(pointer)))
+
+    def rbm_load(self, path, elem=CKPT_F64, pointer=True):
+        """Make the machine stored in a reference checkpoint the context's."""
+        self._chk(self._L.yq_rbm_load(self._h, os.fsencode(path), elem, int(pointer)))
+        n, m, dim = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.yq_rbm_dims(self._h, C.byref(n), C.byref(m), C.byref(dim)))
+        self.dt = np.dtype(np.complex128 if elem in (CKPT_C64, CKPT_C128) else np.float64)
+        self.n, self.m, self.dim = n.value, m.value, dim.value
 
     def graph_mode(self, use_graphs=True):
         """True / 1: captured CG chunks of 8 iterations; an even int: chunks

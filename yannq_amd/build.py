@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyannq_amd.so")
-SOURCES = ["sampler.hip", "eloc.hip", "sr.hip", "mixed.hip", "ff.hip", "complex.hip", "api.hip"]
+SOURCES = ["sampler.hip", "eloc.hip", "sr.hip", "mixed.hip", "ff.hip", "complex.hip",
+           "checkpoint.hip", "api.hip"]
 ARCH = os.environ.get("YQ_OFFLOAD_ARCH", "gfx950")
 
 CFLAGS = [

@@ -51,6 +51,10 @@ void prof_begin(const char* label, hipStream_t st) {
   (void)hipEventRecord(ev, st);
   e.open.push_back(ev);
 }
+void prof_pair(const char* label, hipEvent_t start, hipEvent_t stop) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.entries[g_prof.find(label)].pending.push_back({start, stop});
+}
 void prof_end(const char* label, hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   ProfEntry& e = g_prof.entries[g_prof.find(label)];
@@ -936,6 +940,57 @@ YQ_API int yq_rbm_has_nan(yq_ctx* c, int* out) {
   TRY(need_rbm(c));
   if (!out) return fail(c, YQ_ERR_ARG, "null output");
   return has_nan(c, c->params.as<double>(), vlen(c), out);
+}
+
+// ---------------------------------------------------------------- checkpoints
+// RunRBM.hpp:155-165 (w%04d.dat through a unique_ptr) and
+// AbstractRunner.hpp:83-94 (initializeFrom); encoding in checkpoint.hip
+YQ_API int yq_rbm_save(yq_ctx* c, const char* path, int elem, int pointer_wrapper) {
+  TRY(need_rbm(c));
+  if (!path) return fail(c, YQ_ERR_ARG, "null path");
+  const bool cplx_elem = elem == YQ_CKPT_C64 || elem == YQ_CKPT_C128;
+  if (elem < YQ_CKPT_F32 || elem > YQ_CKPT_C128 || cplx_elem != (c->cplx != 0))
+    return fail(c, YQ_ERR_ARG, "checkpoint element type does not match the machine");
+  std::vector<double> h(vlen(c));
+  TRY(yq_rbm_get_params(c, h.data()));
+  const size_t bytes = yq_ckpt_rbm_bytes(elem, c->n, c->m, c->use_bias, pointer_wrapper);
+  std::vector<unsigned char> buf(bytes);
+  if (yq_ckpt_rbm_encode(elem, c->n, c->m, c->use_bias, h.data(), pointer_wrapper, buf.data(),
+                         bytes) != YQ_OK)
+    return fail(c, YQ_ERR_ARG, "checkpoint encoding failed");
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(c, YQ_ERR_ARG, std::string("cannot open ") + path + " for writing");
+  const size_t wr = fwrite(buf.data(), 1, bytes, f);
+  const int cl = fclose(f);
+  if (wr != bytes || cl != 0) return fail(c, YQ_ERR_ARG, std::string("short write to ") + path);
+  return YQ_OK;
+}
+
+YQ_API int yq_rbm_load(yq_ctx* c, const char* path, int elem, int pointer_wrapper) {
+  if (!c) return YQ_ERR_ARG;
+  if (!path) return fail(c, YQ_ERR_ARG, "null path");
+  if (elem < YQ_CKPT_F32 || elem > YQ_CKPT_C128)
+    return fail(c, YQ_ERR_ARG, "unknown checkpoint element type");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(c, YQ_ERR_ARG, std::string("cannot open ") + path);
+  std::vector<unsigned char> buf;
+  unsigned char chunk[1 << 16];
+  size_t got;
+  while ((got = fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+  fclose(f);
+  int n = 0, m = 0, ub = 0;
+  if (yq_ckpt_rbm_decode(elem, buf.data(), buf.size(), pointer_wrapper, &n, &m, &ub, nullptr,
+                         0) != YQ_OK)
+    return fail(c, YQ_ERR_ARG, std::string(path) +
+                                   ": not an RBM checkpoint of this element type and form");
+  const bool cplx_elem = elem == YQ_CKPT_C64 || elem == YQ_CKPT_C128;
+  const size_t dim = (size_t)n * m + (ub ? (size_t)n + m : 0);
+  std::vector<double> h(dim * (cplx_elem ? 2 : 1));
+  if (yq_ckpt_rbm_decode(elem, buf.data(), buf.size(), pointer_wrapper, &n, &m, &ub, h.data(),
+                         h.size()) != YQ_OK)
+    return fail(c, YQ_ERR_ARG, std::string(path) + ": malformed RBM checkpoint");
+  TRY(yq_rbm_create(c, cplx_elem ? YQ_DTYPE_C128 : YQ_DTYPE_F64, n, m, ub));
+  return yq_rbm_set_params(c, h.data());
 }
 
 // ---------------------------------------------------------------- Hamiltonian

@@ -340,10 +340,9 @@ auto fwd_kernel(int ct, int nks) {
 // sums of y (ys, the centring correction of the reduction) and of y^2 (yy:
 // <u, S u> = ns sum_s y_s^2, the CG curvature).  Block: 4 waves =
 // 4 column tiles of one chunk; the chunk's sigma'^T slab is staged in LDS in
-// slices of kBwdHalf samples (sized by Kp, so the LDS per block stays small
+// slices of SLICE samples (sized by Kp, so the LDS per block stays small
 // and several blocks share a CU) and shared by the waves; every wave keeps
 // all Kp/16 row tiles in registers.
-constexpr int kBwdHalf = 64;  // samples per LDS slice of the sigma'^T slab
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2): hardware block b runs on XCD b % 8.  Renumbering so that XCD x takes
@@ -357,14 +356,14 @@ __device__ inline int xcd_block(int b, int total) {
   return (b % kXcds) * per + b / kXcds;
 }
 
-template <int CHUNK>
+template <int CHUNK, int SLICE>
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
                int nop, int Kp, int Np, int nsp, float* __restrict__ part,
                double* __restrict__ ys, double* __restrict__ yy, const int* __restrict__ ictl) {
   if (ictl[kCtlStop]) return;
-  extern __shared__ uint16_t ash[];  // [Kp][kBwdHalf + kLdsPad]
+  extern __shared__ uint16_t ash[];  // [Kp][SLICE + kLdsPad]
   __shared__ float ysh[CHUNK];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
@@ -374,7 +373,7 @@ __global__ void __launch_bounds__(256)
   const bool active = ct < Np / 16;
   const int li = l & 15, lg = l >> 4;
   const int nit = Kp / 16;
-  const int lda = kBwdHalf + kLdsPad;
+  const int lda = SLICE + kLdsPad;
   {
     const float ov = (float)psum(op, nop);
     double dv = 0.0, dv2 = 0.0;
@@ -406,25 +405,25 @@ __global__ void __launch_bounds__(256)
   const float* trow = Tt + (size_t)(active ? ct * 16 + li : 0) * nsp;
   // Software pipeline over the slices: the sigma'^T slab of slice h + 1 (into
   // registers) and this lane's T' values of slice h + 1 are loaded while the
-  // products of slice h run.  At most 5 16-byte slab pieces per thread
-  // (Kp <= 160).
-  constexpr int kSlab = (32 * kMaxKs * (kBwdHalf / 8) + 255) / 256;
-  const int nslab = Kp * (kBwdHalf / 8);
+  // products of slice h run (Kp <= 160 bounds the slab pieces per thread).
+  constexpr int kSlab = (32 * kMaxKs * (SLICE / 8) + 255) / 256;
+  constexpr int kKs = SLICE / 32;  // k-steps of 32 samples per slice
+  const int nslab = Kp * (SLICE / 8);
   u16x8 sg[kSlab];
-  floatx4 tv[4] = {};  // T'[ct*16 + li][sbase + ks*32 + lg*8 + 0..7], ks = 0, 1
+  floatx4 tv[2 * kKs] = {};  // T'[ct*16 + li][sbase + ks*32 + lg*8 + 0..7]
   auto load_slice = [&](int half) {
-    const int sbase = kc * CHUNK + half * kBwdHalf;
+    const int sbase = kc * CHUNK + half * SLICE;
 #pragma unroll
     for (int i = 0; i < kSlab; ++i) {
       const int q = tid + 256 * i;
       if (q < nslab) {
-        const int row = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
+        const int row = q / (SLICE / 8), kk = (q % (SLICE / 8)) * 8;
         sg[i] = *reinterpret_cast<const u16x8*>(sigT + (size_t)row * nsp + sbase + kk);
       }
     }
     if (active) {
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
+      for (int ks = 0; ks < kKs; ++ks) {
         const float* tp = trow + sbase + ks * 32 + lg * 8;
         tv[2 * ks] = *reinterpret_cast<const floatx4*>(tp);
         tv[2 * ks + 1] = *reinterpret_cast<const floatx4*>(tp + 4);
@@ -432,26 +431,26 @@ __global__ void __launch_bounds__(256)
     }
   };
   load_slice(0);
-  constexpr int kHalves = CHUNK / kBwdHalf;
+  constexpr int kHalves = CHUNK / SLICE;
   for (int half = 0; half < kHalves; ++half) {
-    const int sbase = kc * CHUNK + half * kBwdHalf;
+    const int sbase = kc * CHUNK + half * SLICE;
     __syncthreads();  // the previous slice's products are done with the LDS
 #pragma unroll
     for (int i = 0; i < kSlab; ++i) {
       const int q = tid + 256 * i;
       if (q < nslab) {
-        const int row = q / (kBwdHalf / 8), kk = (q % (kBwdHalf / 8)) * 8;
+        const int row = q / (SLICE / 8), kk = (q % (SLICE / 8)) * 8;
         *reinterpret_cast<u16x8*>(ash + row * lda + kk) = sg[i];
       }
     }
-    floatx4 tc[4];
+    floatx4 tc[2 * kKs];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tc[i] = tv[i];
+    for (int i = 0; i < 2 * kKs; ++i) tc[i] = tv[i];
     if (half + 1 < kHalves) load_slice(half + 1);
     __syncthreads();
     if (!active) continue;
 #pragma unroll
-    for (int ks = 0; ks < kBwdHalf / 32; ++ks) {
+    for (int ks = 0; ks < kKs; ++ks) {
       const int s0 = sbase + ks * 32 + lg * 8;
       u16x8 bh, bm, bl;
 #pragma unroll
@@ -484,6 +483,15 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
+}
+
+// backward kernel for a chunk length; slices of 64 samples (longer slices,
+// with fewer latency steps but less overlap, measured slower at n = 100,
+// m = 400, 16384 samples)
+constexpr int kBwdSlice = 64;
+auto bwd_kernel(int chunk) {
+  return chunk == 128 ? k_bwd_mfma<128, kBwdSlice>
+                      : chunk == 512 ? k_bwd_mfma<512, kBwdSlice> : k_bwd_mfma<256, kBwdSlice>;
 }
 
 // One Chronopoulos-Gear CG step for (S + shift) x = b with preconditioner M
@@ -717,33 +725,25 @@ RedLayout red_layout(const MixedWork& w, int chunk = 0) {
 // CG iteration.
 void product_launches(MixedWork& w, int count, hipStream_t st) {
   const RedLayout L = red_layout(w);
-  {
-    ProfGuard g("fwd_mfma", st);
-    const size_t lds = sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad);
-    hipLaunchKernelGGL(fwd_kernel(w.ct, w.Kp / 32), dim3(w.nsp / 256, w.ncg), dim3(256), lds, st,
-                       w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
-  }
+  const size_t lds_f = sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad);
+  launch_prof("fwd_mfma", fwd_kernel(w.ct, w.Kp / 32), dim3(w.nsp / 256, w.ncg), dim3(256),
+              lds_f, st, w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
   const int chunks = w.nsp / w.chunk;
-  ProfGuard g("bwd_mfma", st);
-  const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdHalf + kLdsPad);
-  auto bwd = w.chunk == 128 ? k_bwd_mfma<128> : w.chunk == 512 ? k_bwd_mfma<512>
-                                                               : k_bwd_mfma<256>;
-  hipLaunchKernelGGL(bwd, dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds, st, w.sigT, w.Tt,
-                     w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp, w.part,
-                     w.red + L.ys, w.red + L.yy, w.ictl);
+  const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad);
+  launch_prof("bwd_mfma", bwd_kernel(w.chunk), dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds,
+              st, w.sigT, w.Tt, w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp,
+              w.part, w.red + L.ys, w.red + L.yy, w.ictl);
 }
 
 // k_cg_step in the given mode for iteration parity par (mode 0 writes slot 0)
 void step_launch(MixedWork& w, const float* dinv, int mode, int par, hipStream_t st) {
   const RedLayout L = red_layout(w);
   const int wr = mode == 0 ? 0 : par ^ 1;
-  ProfGuard g("cg_step", st);
-  hipLaunchKernelGGL(k_cg_step, dim3(w.Np / 16, w.Kp / 16), dim3(1024), 0, st, w.x, w.r, w.p,
-                     w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par],
-                     w.red + L.uu[par], L.nbv, w.red + L.ys, w.red + L.yy, L.nby, w.ns,
-                     w.red + L.g[wr], w.red + L.rr[wr], w.red + L.uu[wr], w.red + L.op, w.dctl,
-                     w.ictl, w.part, w.nsp / w.chunk, w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np,
-                     w.vsplit);
+  launch_prof("cg_step", k_cg_step, dim3(w.Np / 16, w.Kp / 16), dim3(1024), 0, st, w.x, w.r,
+              w.p, w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par],
+              w.red + L.uu[par], L.nbv, w.red + L.ys, w.red + L.yy, L.nby, w.ns,
+              w.red + L.g[wr], w.red + L.rr[wr], w.red + L.uu[wr], w.red + L.op, w.dctl, w.ictl,
+              w.part, w.nsp / w.chunk, w.om32, w.n, w.m, w.use_bias, w.Kp, w.Np, w.vsplit);
 }
 
 // w.Ap = (S + shift) v for v in w.r, without preconditioning: the split

@@ -1,6 +1,7 @@
 // Internal declarations shared by the kernels and the C-ABI implementation.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -28,6 +29,25 @@ struct ProfGuard {
     if (on) prof_end(label, st);
   }
 };
+// registers an event pair that brackets one launch
+void prof_pair(const char* label, hipEvent_t start, hipEvent_t stop);
+// A launch timed, when its label is enabled, by events bound to the kernel's
+// own dispatch (hipExtLaunchKernel: start and end of the kernel, without the
+// queue time between separately recorded events and the kernel).
+template <typename F, typename... Args>
+void launch_prof(const char* label, F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                 Args... args) {
+  if (prof_enabled(label)) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+      hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)lds, st, e0, e1, 0u, args...);
+      prof_pair(label, e0, e1);
+      return;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+  }
+  hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
 
 constexpr int kWave = 64;
 constexpr int kMaxHidden = 512;  // theta registers per lane: 8 per 64 lanes
