@@ -180,6 +180,25 @@ YQ_API int yq_samples_get(yq_ctx* ctx, int8_t* sigma, double* theta);
 YQ_API int yq_samples_set(yq_ctx* ctx, int ns, const int8_t* sigma,
                           int precision);
 
+/* ---- Exact sums: GroundState/SRMatExact.hpp + Samplers/ExactSampler.hpp -- */
+/* The batch becomes every configuration of a basis, in the reference's order
+ * -- Basis/BasisFull.hpp (nup < 0: the integers 0 .. 2^n - 1) or
+ * Basis/BasisJz.hpp (the integers with nup set bits, increasing) -- as spins
+ * sigma_i = 1 - 2 bit_i (toSigma, Utilities/Utility.hpp:91-99), weighted by
+ * |psi|^2 / sum |psi|^2 over the basis (getProbs, RBM.hpp:443-451).  The
+ * following yq_eloc returns the exact energy and variance and yq_sr_build
+ * (explicit or dense CG solver, FP64) the exact O mean, F and S of
+ * SRMatExact::constructExact / corrMat / energyGrad (SRMatExact.hpp:97-149).
+ * n <= 24; *basis_size = number of configurations. */
+YQ_API int yq_exact_batch(yq_ctx* ctx, int nup, int* basis_size);
+/* The weights of the current exact batch (basis_size doubles). */
+YQ_API int yq_exact_probs(yq_ctx* ctx, double* probs);
+/* ExactSampler::sample (ExactSampler.hpp:49-77): n_samples configurations
+ * drawn independently from |psi|^2 over the basis by inverse CDF (one Philox
+ * uniform per draw, keyed by seed), as an ordinary unweighted batch;
+ * indices (n_samples ints, may be null) receives the basis positions. */
+YQ_API int yq_exact_sample(yq_ctx* ctx, int nup, int n_samples, uint64_t seed, int* indices);
+
 /* ---- Energy: Observables/Energy.hpp + GroundState/utils.hpp:108-126 ---- */
 /* E_loc of every sample (eloc may be null), mean = Re <E_loc>,
  * var = <Re E_loc^2> - mean^2: Energy.hpp:59-70. */

@@ -508,6 +508,109 @@ std::unique_ptr<Optimizer<T>> createOptimizer(const std::string& name, double al
   throw std::invalid_argument("Such an optimizer does not exist.");
 }
 
+// ------------------------------------------------------------ exact sums
+// Basis/BasisFull.hpp and Basis/BasisJz.hpp: which configurations a sum runs
+// over (the device enumerates them in the same order).
+struct BasisFull {
+  explicit BasisFull(unsigned N) : N_(N) {}
+  int nup() const { return -1; }
+  unsigned N_;
+};
+struct BasisJz {
+  BasisJz(unsigned N, unsigned nup) : N_(N), nup_(nup) {}
+  int nup() const { return (int)nup_; }
+  unsigned N_, nup_;
+};
+
+// GroundState/SRMatExact.hpp:47-149: expectation values as weighted sums
+// over every basis configuration (|psi|^2 weights), on the device.
+template <class Machine, class Hamiltonian>
+class SRMatExact {
+ public:
+  using Scalar = typename Machine::Scalar;
+  using RealScalar = typename Machine::RealScalar;
+  using Vector = std::vector<Scalar>;
+
+  template <class Basis>
+  SRMatExact(const Machine& qs, Basis&& basis, const Hamiltonian& ham)
+      : qs_(qs), ham_(ham), nup_(basis.nup()) {}
+
+  void clear() { eloc_ = elocVar_ = 0.0; }
+  void constructExact() {  // :97-117
+    const yq_ham_desc d = ham_.desc();
+    detail::check(ctx(), yq_ham_set(ctx(), &d));
+    int B = 0;
+    detail::check(ctx(), yq_exact_batch(ctx(), nup_, &B));
+    double mean = 0.0, var = 0.0;
+    detail::check(ctx(), yq_eloc(ctx(), nullptr, &mean, &var));
+    eloc_ = mean;
+    elocVar_ = var;
+    detail::check(ctx(), yq_sr_build(ctx(), YQ_SOLVE_EXPLICIT, YQ_FP64));
+  }
+  RealScalar eloc() const { return eloc_; }
+  RealScalar elocVar() const { return elocVar_; }
+  Vector oloc() const { return item(YQ_SR_OMEAN); }      // :119-127
+  Vector energyGrad() const { return item(YQ_SR_F); }    // :136-144
+  std::vector<Scalar> corrMat() const {                   // :129-134, column-major
+    const size_t d = qs_.getDim();
+    std::vector<Scalar> S(d * d);
+    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_S, detail::dbl(S.data())));
+    return S;
+  }
+  Vector apply(const Vector& rhs) const {  // :146-151
+    Vector out(rhs.size());
+    detail::check(ctx(), yq_sr_apply(ctx(), 0.0, detail::dbl(rhs.data()),
+                                     detail::dbl(out.data())));
+    return out;
+  }
+  // RunRBMExact.hpp:57-66: LLT of corrMat + shift I applied to rhs
+  Vector solveExact(const Vector& rhs, RealScalar shift) const {
+    Vector x(qs_.getDim());
+    double res = 0.0;
+    int it = 0;
+    detail::check(ctx(), yq_sr_solve(ctx(), shift, 0.0, 0, detail::dbl(rhs.data()),
+                                     detail::dbl(x.data()), &res, &it));
+    return x;
+  }
+
+ private:
+  yq_ctx* ctx() const { return qs_.ctx(); }
+  Vector item(int which) const {
+    Vector v(qs_.getDim());
+    detail::check(ctx(), yq_sr_get(ctx(), which, detail::dbl(v.data())));
+    return v;
+  }
+  const Machine& qs_;
+  const Hamiltonian& ham_;
+  int nup_;
+  RealScalar eloc_ = 0.0, elocVar_ = 0.0;
+};
+
+// Samplers/ExactSampler.hpp:11-79: independent draws from |psi|^2
+template <class Machine>
+class ExactSampler {
+ public:
+  template <class Basis>
+  ExactSampler(const Machine& qs, Basis&& basis) : qs_(qs), nup_(basis.nup()) {}
+  void initializeRandomEngine() { seed_ = std::random_device{}(); }
+  void setSeed(uint64_t s) { seed_ = s; }
+  template <class Randomizer>
+  void randomize(Randomizer&&) const {}  // nothing to randomise
+  // sample(n_sweeps, .): n_sweeps draws; returns their basis positions (the
+  // configurations are the device batch, as SamplerMT's are)
+  std::vector<int> sample(uint32_t nSweeps, uint32_t /*nDiscard*/ = 0) {
+    std::vector<int> idx(nSweeps);
+    detail::check(qs_.ctx(), yq_exact_sample(qs_.ctx(), nup_, (int)nSweeps, seed_++, idx.data()));
+    return idx;
+  }
+  std::string desc() const { return "{\"name\": \"ExactSampler\"}"; }
+
+ private:
+  const Machine& qs_;
+  int nup_;
+  uint64_t seed_ = 0x5eed;
+};
+
 // ------------------------------------------------------------ Runner
 // Runners/RunRBM.hpp:41-222 with AbstractRunner.hpp's defaults (:32-38).
 template <typename T = double>
@@ -634,6 +737,74 @@ class RunRBM {
   std::unique_ptr<Optimizer<T>> opt_;
   bool machineInitialized_ = false, useCG_ = false;
   double cgTol_ = 1e-4, beta1_ = 0.0, gradClip_ = 0.0;
+  double lambdaIni_ = 1e-3, lambdaDecay_ = 1.0, lambdaMin_ = 1e-4;
+  int maxIter_ = 2000, saveWfPer_ = 100;
+};
+
+// Runners/RunRBMExact.hpp:8-82: SR with exact sums over a basis,
+// corrMat + lambda I by Cholesky, then the optimizer; callback(ll, E, |v|).
+template <typename T = double>
+class RunRBMExact {
+ public:
+  using Machine = RBM<T>;
+  using Vector = std::vector<T>;
+
+  RunRBMExact(Device& dev, unsigned N, unsigned alpha, bool useBias, std::ostream& logger)
+      : logger_(logger), qs_(dev, N, alpha * N, useBias), re_(std::random_device{}()) {}
+  void setLambda(double ini, double decay = 1.0, double min = 0.0) {
+    lambdaIni_ = ini;
+    lambdaDecay_ = decay;
+    lambdaMin_ = min;
+  }
+  void setIterParams(int maxIter, int saveWfPer) {
+    maxIter_ = maxIter;
+    saveWfPer_ = saveWfPer;
+  }
+  void setOptimizer(const std::string& name, double alpha = 0.05, double p = 0.0) {
+    opt_ = createOptimizer<T>(name, alpha, p);
+  }
+  void setSeed(uint64_t s) { re_.seed(s); }
+  void initializeRandom(double wIni = 1e-3) {
+    machineInitialized_ = true;
+    qs_.initializeRandom(re_, wIni);
+  }
+  void initializeFrom(const std::string& filePath) {
+    machineInitialized_ = true;
+    qs_.load(filePath);
+  }
+  Machine& getQs() { return qs_; }
+
+  template <class Callback, class Basis, class Hamiltonian>
+  void run(Callback&& callback, Basis&& basis, Hamiltonian&& ham) {
+    if (!machineInitialized_) initializeRandom();
+    if (!opt_) setOptimizer("SGD");
+    SRMatExact<Machine, typename std::decay<Hamiltonian>::type> srex(qs_, basis, ham);
+    for (int ll = 0; ll <= maxIter_; ll++) {
+      logger_ << "Epochs: " << ll << std::endl;
+      if (saveWfPer_ != 0 && ll % saveWfPer_ == 0) {
+        char fileName[30];
+        snprintf(fileName, sizeof(fileName), "w%04d.dat", ll);
+        qs_.save(fileName);
+      }
+      srex.clear();
+      srex.constructExact();
+      const double currE = srex.eloc();
+      const double lambda = std::max(lambdaIni_ * std::pow(lambdaDecay_, ll), lambdaMin_);
+      const Vector v = srex.solveExact(srex.energyGrad(), lambda);
+      const Vector optV = opt_->getUpdate(v);
+      qs_.updateParams(optV);
+      double nv = 0.0;
+      for (const T& x : v) nv += std::norm(x);
+      callback(ll, currE, std::sqrt(nv));
+    }
+  }
+
+ private:
+  std::ostream& logger_;
+  Machine qs_;
+  std::mt19937_64 re_;
+  std::unique_ptr<Optimizer<T>> opt_;
+  bool machineInitialized_ = false;
   double lambdaIni_ = 1e-3, lambdaDecay_ = 1.0, lambdaMin_ = 1e-4;
   int maxIter_ = 2000, saveWfPer_ = 100;
 };

@@ -21,7 +21,7 @@ def _build(yq, tmp_path, name="run_tfi"):
     return exe
 
 
-@pytest.mark.parametrize("name", ["run_tfi", "run_xxz", "run_ckpt"])
+@pytest.mark.parametrize("name", ["run_tfi", "run_xxz", "run_ckpt", "run_exact"])
 def test_concept_layer_compiles(yq, tmp_path, name):
     assert os.path.exists(_build(yq, tmp_path, name))
 
@@ -70,3 +70,15 @@ def test_runner_checkpoints(yq, tmp_path):
     assert all(d[:3] == (n, m, True) for d in decoded)
     # initializeRandom(0.01) at epoch 0, then SGD steps move the parameters
     assert np.std(decoded[0][3]) < 0.05 and not np.array_equal(decoded[0][3], decoded[2][3])
+
+
+@pytest.mark.gpu
+def test_exact_runner_reaches_ground_state(yq, tmp_path):
+    """RunRBMExact<complex<double>> over BasisFull as Examples/RBMTFIExact.cpp."""
+    from tests.conftest import _gpu_available
+    if not _gpu_available():
+        pytest.skip("no GPU visible")
+    exe = _build(yq, tmp_path, "run_exact")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -112,6 +112,8 @@ def lib():
         "yq_ckpt_rbm_encode": [i, i, i, i, p, i, p, C.c_size_t],
         "yq_ckpt_rbm_decode": [i, p, C.c_size_t, i, p, p, p, p, C.c_size_t],
         "yq_rbm_save": [p, C.c_char_p, i, i], "yq_rbm_load": [p, C.c_char_p, i, i],
+        "yq_exact_batch": [p, i, p], "yq_exact_probs": [p, p],
+        "yq_exact_sample": [p, i, i, u64, p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -289,6 +291,28 @@ class Context:
         sigma = np.ascontiguousarray(sigma, dtype=np.int8)
         self._chk(self._L.yq_samples_set(self._h, sigma.shape[0], _ptr(sigma), precision))
         self.ns = sigma.shape[0]
+
+    def exact_batch(self, nup=-1):
+        """Every configuration of the full basis (nup < 0) or of the sector
+        with nup set bits, weighted by |psi|^2 (SRMatExact); returns the
+        basis size."""
+        B = C.c_int()
+        self._chk(self._L.yq_exact_batch(self._h, nup, C.byref(B)))
+        self.ns = B.value
+        return B.value
+
+    def exact_probs(self):
+        out = np.empty(self.ns)
+        self._chk(self._L.yq_exact_probs(self._h, _ptr(out)))
+        return out
+
+    def exact_sample(self, n_samples, seed, nup=-1):
+        """ExactSampler: n_samples draws from |psi|^2; returns their basis
+        positions."""
+        idx = np.empty(n_samples, np.int32)
+        self._chk(self._L.yq_exact_sample(self._h, nup, n_samples, seed, _ptr(idx)))
+        self.ns = n_samples
+        return idx
 
     def eloc(self):
         e = self._out(self.ns)

@@ -105,10 +105,12 @@ struct yq_ctx {
   uint64_t seed = 0, rand_epoch = 0;
   Buf sig_state, ctr, acc;
 
-  // current batch
-  bool have_batch = false, have_eloc = false;
+  // current batch; weighted: an exact basis with |psi|^2 weights bw (row
+  // scales brs = sqrt(ns bw)), see exact.hip
+  bool have_batch = false, have_eloc = false, weighted = false;
   int ns = 0, bprec = 0;
   Buf bsig, btheta, ratio, eloc, estat;
+  Buf bw, brs, blogpsi, bbasis, bcdf, bpart, xsig;
   double e_mean = 0.0, e_var = 0.0;
 
   // SR
@@ -295,12 +297,20 @@ int compute_eloc(yq_ctx* c) {
       return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
     launch_eloc_c(a, c->st);
     LAUNCHCHK(c);
-    launch_eloc_stats_c(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+    if (c->weighted)
+      launch_eloc_stats_w(c->eloc.as<double>(), c->bw.as<double>(), c->ns, 1,
+                          c->estat.as<double>(), c->st);
+    else
+      launch_eloc_stats_c(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
   } else {
     if (eloc_lds_bytes(a) > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
     launch_eloc(a, c->st);
     LAUNCHCHK(c);
-    launch_eloc_stats(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+    if (c->weighted)
+      launch_eloc_stats_w(c->eloc.as<double>(), c->bw.as<double>(), c->ns, 0,
+                          c->estat.as<double>(), c->st);
+    else
+      launch_eloc_stats(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
   }
   LAUNCHCHK(c);
   double h[2];
@@ -314,6 +324,71 @@ int compute_eloc(yq_ctx* c) {
 
 // ---------------------------------------------------------------- SR build
 size_t aug_partial_doubles(int ns, int n, int m) { return factored_partial_doubles(ns, n, m); }
+
+// S = alpha A A^T (lower triangle; A^H for complex) with A = dim x ns
+// column-major.  rocBLAS SYRK/HERK parallelises over the output tiles, which
+// leaves most of the chip idle when dim is small and ns large (an exact basis
+// of 2^20 configurations); then the samples are split into P chunks, one
+// strided-batched SYRK/HERK forms the P partial products side by side, and a
+// fixed-order sum adds them (plus the remainder chunk).
+int syrk_samples(yq_ctx* c, const double* A, size_t dim, int ns, double alpha, double* S,
+                 bool cplx) {
+  TRY(blas(c));
+  const size_t nc = cplx ? 2 : 1;
+  const int kc = 16384;
+  int P = ns / kc;
+  const size_t cap = (size_t)2 << 30;  // bytes of partial products
+  while (P > 1 && (size_t)P * dim * dim * nc * sizeof(double) > cap) --P;
+  const double zero = 0.0, one = 1.0;
+  if (P < 4) {
+    ProfGuard g(cplx ? "zherk" : "dsyrk", c->st);
+    if (cplx)
+      BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                               (rocblas_int)dim, ns, &alpha,
+                               reinterpret_cast<const rocblas_double_complex*>(A),
+                               (rocblas_int)dim, &zero,
+                               reinterpret_cast<rocblas_double_complex*>(S), (rocblas_int)dim));
+    else
+      BLASCHK(c, rocblas_dsyrk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                               (rocblas_int)dim, ns, &alpha, A, (rocblas_int)dim, &zero, S,
+                               (rocblas_int)dim));
+    return YQ_OK;
+  }
+  TRY(ensure(c, c->M, sizeof(double) * nc * (size_t)P * dim * dim));
+  double* Pout = c->M.as<double>();
+  const rocblas_stride sa = (rocblas_stride)kc * dim, sc = (rocblas_stride)(dim * dim);
+  {
+    ProfGuard g(cplx ? "zherk" : "dsyrk", c->st);
+    if (cplx)
+      BLASCHK(c, rocblas_zherk_strided_batched(
+                     c->blas, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)dim, kc,
+                     &alpha, reinterpret_cast<const rocblas_double_complex*>(A),
+                     (rocblas_int)dim, sa, &zero,
+                     reinterpret_cast<rocblas_double_complex*>(Pout), (rocblas_int)dim, sc, P));
+    else
+      BLASCHK(c, rocblas_dsyrk_strided_batched(c->blas, rocblas_fill_lower,
+                                               rocblas_operation_none, (rocblas_int)dim, kc,
+                                               &alpha, A, (rocblas_int)dim, sa, &zero, Pout,
+                                               (rocblas_int)dim, sc, P));
+  }
+  launch_sum_lower(Pout, P, dim, (int)nc, S, c->st);
+  LAUNCHCHK(c);
+  const int rem = ns - P * kc;
+  if (rem > 0) {
+    const double* Ar = A + nc * (size_t)P * kc * dim;
+    if (cplx)
+      BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                               (rocblas_int)dim, rem, &alpha,
+                               reinterpret_cast<const rocblas_double_complex*>(Ar),
+                               (rocblas_int)dim, &one,
+                               reinterpret_cast<rocblas_double_complex*>(S), (rocblas_int)dim));
+    else
+      BLASCHK(c, rocblas_dsyrk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                               (rocblas_int)dim, rem, &alpha, Ar, (rocblas_int)dim, &one, S,
+                               (rocblas_int)dim));
+  }
+  return YQ_OK;
+}
 
 // Complex machine: dense conj(O) (see complex.hip), FP64 SR whatever the
 // batch precision; S only for the explicit solver.
@@ -332,27 +407,23 @@ int sr_build_c(yq_ctx* c, int solver) {
   launch_tanh_c(c->btheta.p, c->bprec == YQ_MIXED, c->T64.as<double>(), (size_t)ns * m, c->st);
   double* Oh = c->O.as<double>();
   launch_build_Oh(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias, Oh, c->st);
-  // column means of conj(O) (complex columns as pairs of doubles)
-  launch_colsum(Oh, ns, (int)(2 * dim), c->partial.as<double>(), c->st);
-  launch_reduce_partials(c->partial.as<double>(), chunks, (int)(2 * dim), 1.0 / ns,
+  // column means of conj(O) (complex columns as pairs of doubles); an exact
+  // weighted batch takes the |psi|^2 weights and scales the centred rows by
+  // sqrt(ns w_s), so that every product below (/ ns) is the weighted sum
+  const double* w = c->weighted ? c->bw.as<double>() : nullptr;
+  const double* rs = c->weighted ? c->brs.as<double>() : nullptr;
+  launch_colsum(Oh, ns, (int)(2 * dim), c->partial.as<double>(), c->st, w);
+  launch_reduce_partials(c->partial.as<double>(), chunks, (int)(2 * dim), w ? 1.0 : 1.0 / ns,
                          c->omean.as<double>(), c->st);
   // centre and F = conj(O_c)^T E / ns = O_c^H E / ns
   launch_center_force_c(Oh, c->omean.as<double>(), c->eloc.as<double>(), ns, (int)dim,
-                        c->partial.as<double>(), c->st);
+                        c->partial.as<double>(), c->st, rs);
   launch_reduce_partials(c->partial.as<double>(), chunks, (int)(2 * dim), 1.0 / ns,
                          c->F.as<double>(), c->st);
   LAUNCHCHK(c);
   if (solver == YQ_SOLVE_EXPLICIT) {
-    TRY(blas(c));
     TRY(ensure(c, c->S, sizeof(double) * 2 * dim * dim));
-    const double alpha = 1.0 / ns, beta = 0.0;
-    ProfGuard g("zherk", c->st);
-    BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                             (rocblas_int)dim, ns, &alpha,
-                             reinterpret_cast<const rocblas_double_complex*>(Oh),
-                             (rocblas_int)dim, &beta,
-                             reinterpret_cast<rocblas_double_complex*>(c->S.p),
-                             (rocblas_int)dim));
+    TRY(syrk_samples(c, Oh, dim, ns, 1.0 / ns, c->S.as<double>(), true));
   }
   // diag(S) = mean |O_c|^2, duplicated per (re, im) for the Jacobi option
   TRY(ensure(c, c->diagS, sizeof(double) * 2 * dim));
@@ -384,6 +455,9 @@ int sr_build(yq_ctx* c, int solver, int precision) {
   if (precision == YQ_MIXED && solver != YQ_SOLVE_CG_FACTORED)
     return fail(c, YQ_ERR_UNSUPPORTED,
                 "mixed precision is implemented for the factored CG solver only");
+  if (c->weighted && solver == YQ_SOLVE_CG_FACTORED)
+    return fail(c, YQ_ERR_UNSUPPORTED,
+                "an exact weighted batch uses the explicit or the dense CG solver");
   TRY(ensure(c, c->omean, sizeof(double) * dim));
   TRY(ensure(c, c->F, sizeof(double) * dim));
   TRY(ensure(c, c->T64, sizeof(double) * (size_t)ns * m));
@@ -426,30 +500,35 @@ int sr_build(yq_ctx* c, int solver, int precision) {
                                                        aug_partial_doubles(ns, n, m))));
     launch_build_O(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias,
                    c->O.as<double>(), c->st);
-    launch_colsum(c->O.as<double>(), ns, (int)dim, c->partial.as<double>(), c->st);
-    launch_reduce_partials(c->partial.as<double>(), chunks, (int)dim, 1.0 / ns,
+    // an exact weighted batch: weighted mean, rows of O_c scaled by sqrt(ns w_s)
+    const double* w = c->weighted ? c->bw.as<double>() : nullptr;
+    const double* rs = c->weighted ? c->brs.as<double>() : nullptr;
+    launch_colsum(c->O.as<double>(), ns, (int)dim, c->partial.as<double>(), c->st, w);
+    launch_reduce_partials(c->partial.as<double>(), chunks, (int)dim, w ? 1.0 : 1.0 / ns,
                            c->omean.as<double>(), c->st);
     launch_center_force(c->O.as<double>(), c->omean.as<double>(), c->eloc.as<double>(), ns,
-                        (int)dim, c->partial.as<double>(), c->st);
+                        (int)dim, c->partial.as<double>(), c->st, rs);
     launch_reduce_partials(c->partial.as<double>(), chunks, (int)dim, 1.0 / ns,
                            c->F.as<double>(), c->st);
     LAUNCHCHK(c);
     if (solver == YQ_SOLVE_EXPLICIT) {
-      TRY(blas(c));
-      TRY(ensure(c, c->S, sizeof(double) * dim * dim));
-      const double alpha = 1.0 / ns, beta = 0.0;
       // O_c is ns x dim row-major = dim x ns column-major: S = A A^T / ns
-      ProfGuard g("dsyrk", c->st);
-      BLASCHK(c, rocblas_dsyrk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)dim, ns, &alpha, c->O.as<double>(),
-                               (rocblas_int)dim, &beta, c->S.as<double>(), (rocblas_int)dim));
+      TRY(ensure(c, c->S, sizeof(double) * dim * dim));
+      TRY(syrk_samples(c, c->O.as<double>(), dim, ns, 1.0 / ns, c->S.as<double>(), false));
     }
   }
-  // diag(S) for the Jacobi preconditioner (cheap: one pass over T)
+  // diag(S) for the Jacobi preconditioner (cheap: one pass over T; for a
+  // weighted batch the column sums of the scaled O_c^2)
   TRY(ensure(c, c->diagS, sizeof(double) * dim));
   TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)(colsum_chunks(ns) + 1) * m)));
-  launch_diag_S(c->T64.as<double>(), ns, n, m, c->use_bias, c->omean.as<double>(),
-                c->diagS.as<double>(), c->vtmp.as<double>(), c->st);
+  if (c->weighted) {
+    launch_colsum(c->O.as<double>(), ns, (int)dim, c->partial.as<double>(), c->st, nullptr, 1);
+    launch_reduce_partials(c->partial.as<double>(), colsum_chunks(ns), (int)dim, 1.0 / ns,
+                           c->diagS.as<double>(), c->st);
+  } else {
+    launch_diag_S(c->T64.as<double>(), ns, n, m, c->use_bias, c->omean.as<double>(),
+                  c->diagS.as<double>(), c->vtmp.as<double>(), c->st);
+  }
   LAUNCHCHK(c);
   TRY(sync(c));
   c->have_sr = true;
@@ -1135,6 +1214,7 @@ void batch_ready(yq_ctx* c) {
   c->have_batch = true;
   c->have_eloc = false;
   c->have_sr = false;
+  c->weighted = false;
 }
 }  // namespace
 
@@ -1191,9 +1271,120 @@ YQ_API int yq_samples_set(yq_ctx* c, int ns, const int8_t* sigma, int precision)
   LAUNCHCHK(c);
   c->ns = ns;
   c->bprec = precision;
-  c->have_batch = true;
-  c->have_eloc = false;
-  c->have_sr = false;
+  batch_ready(c);
+  return sync(c);
+}
+
+// ---------------------------------------------------------------- exact sums
+namespace {
+// Basis/BasisFull.hpp (nup < 0: 0 .. 2^n - 1) or Basis/BasisJz.hpp (the
+// integers with nup set bits, increasing, by the same next-permutation step)
+int exact_basis(yq_ctx* c, int nup, std::vector<uint32_t>& out) {
+  const int n = c->n;
+  if (n > 24) return fail(c, YQ_ERR_UNSUPPORTED, "exact sums need n <= 24");
+  if (nup > n) return fail(c, YQ_ERR_ARG, "nup exceeds n");
+  out.clear();
+  if (nup < 0) {
+    out.resize((size_t)1 << n);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = (uint32_t)i;
+    return YQ_OK;
+  }
+  if (nup == 0) {
+    out.push_back(0u);
+    return YQ_OK;
+  }
+  uint32_t v = (1u << nup) - 1u;
+  const uint32_t last = v << (n - nup);
+  for (;;) {
+    out.push_back(v);
+    if (v == last) break;
+    const uint32_t t = v | (v - 1u);
+    v = (t + 1u) | (((~t & (0u - ~t)) - 1u) >> (__builtin_ctz(v) + 1));
+  }
+  return YQ_OK;
+}
+
+// the basis as the batch (FP64 theta) with its |psi|^2 weights
+int exact_prepare(yq_ctx* c, int nup) {
+  std::vector<uint32_t> basis;
+  TRY(exact_basis(c, nup, basis));
+  const int B = (int)basis.size();
+  const size_t nc = ncomp(c);
+  TRY(ensure(c, c->bbasis, sizeof(uint32_t) * B));
+  TRY(ensure(c, c->bsig, (size_t)B * c->n));
+  TRY(ensure(c, c->btheta, sizeof(double) * nc * (size_t)B * c->m));
+  TRY(ensure(c, c->blogpsi, sizeof(double) * nc * B));
+  TRY(ensure(c, c->bw, sizeof(double) * B));
+  TRY(ensure(c, c->brs, sizeof(double) * B));
+  TRY(ensure(c, c->bpart, sizeof(double) * 2 * exact_partials(B)));
+  HIPCHK(c, hipMemcpyAsync(c->bbasis.p, basis.data(), sizeof(uint32_t) * B,
+                           hipMemcpyHostToDevice, c->st));
+  launch_basis_spins(c->bbasis.as<uint32_t>(), B, c->n, c->bsig.as<int8_t>(), c->st);
+  if (c->cplx)
+    launch_theta_batch_c(W64(c), b64(c), c->bsig.as<int8_t>(), B, c->n, c->m, c->btheta.p, 0,
+                         c->st);
+  else
+    launch_theta_batch(W64(c), b64(c), c->bsig.as<int8_t>(), B, c->n, c->m, c->btheta.p, 0,
+                       c->st);
+  launch_exact_weights(c->bsig.as<int8_t>(), c->btheta.as<double>(),
+                       c->use_bias ? a64(c) : nullptr, B, c->n, c->m, (int)c->cplx,
+                       c->blogpsi.as<double>(), c->bw.as<double>(), c->brs.as<double>(),
+                       c->bpart.as<double>(), c->st);
+  LAUNCHCHK(c);
+  c->ns = B;
+  c->bprec = YQ_FP64;
+  batch_ready(c);
+  c->weighted = true;
+  return YQ_OK;
+}
+}  // namespace
+
+YQ_API int yq_exact_batch(yq_ctx* c, int nup, int* basis_size) {
+  TRY(need_rbm(c));
+  TRY(exact_prepare(c, nup));
+  if (basis_size) *basis_size = c->ns;
+  return sync(c);
+}
+
+YQ_API int yq_exact_probs(yq_ctx* c, double* probs) {
+  TRY(need_rbm(c));
+  if (!c->have_batch || !c->weighted) return fail(c, YQ_ERR_STATE, "no exact batch");
+  if (!probs) return fail(c, YQ_ERR_ARG, "null output");
+  HIPCHK(c, hipMemcpyAsync(probs, c->bw.p, sizeof(double) * c->ns, hipMemcpyDeviceToHost, c->st));
+  return sync(c);
+}
+
+YQ_API int yq_exact_sample(yq_ctx* c, int nup, int n_samples, uint64_t seed, int* indices) {
+  TRY(need_rbm(c));
+  if (n_samples < 1) return fail(c, YQ_ERR_ARG, "need n_samples >= 1");
+  TRY(exact_prepare(c, nup));
+  const int B = c->ns;
+  TRY(ensure(c, c->bcdf, sizeof(double) * B));
+  TRY(ensure(c, c->bpart, sizeof(double) * std::max(2 * exact_partials(B), exact_scan_tiles(B))));
+  TRY(ensure(c, c->xsig, (size_t)n_samples * c->n));
+  TRY(ensure(c, c->vtmp, sizeof(int) * (size_t)n_samples));
+  launch_exact_cdf(c->bw.as<double>(), B, c->bcdf.as<double>(), c->bpart.as<double>(), c->st);
+  launch_exact_draw(c->bcdf.as<double>(), B, c->bbasis.as<uint32_t>(), c->n, n_samples, seed,
+                    c->xsig.as<int8_t>(), c->vtmp.as<int>(), c->st);
+  LAUNCHCHK(c);
+  // the draws become an ordinary (unweighted) batch
+  const size_t cnt = (size_t)n_samples * c->n;
+  TRY(ensure(c, c->bsig, cnt));
+  TRY(ensure(c, c->btheta, sizeof(double) * ncomp(c) * (size_t)n_samples * c->m));
+  HIPCHK(c, hipMemcpyAsync(c->bsig.p, c->xsig.p, cnt, hipMemcpyDeviceToDevice, c->st));
+  if (c->cplx)
+    launch_theta_batch_c(W64(c), b64(c), c->bsig.as<int8_t>(), n_samples, c->n, c->m,
+                         c->btheta.p, 0, c->st);
+  else
+    launch_theta_batch(W64(c), b64(c), c->bsig.as<int8_t>(), n_samples, c->n, c->m,
+                       c->btheta.p, 0, c->st);
+  LAUNCHCHK(c);
+  if (indices)
+    HIPCHK(c, hipMemcpyAsync(indices, c->vtmp.p, sizeof(int) * n_samples, hipMemcpyDeviceToHost,
+                             c->st));
+  c->ns = n_samples;
+  c->bprec = YQ_FP64;
+  batch_ready(c);
   return sync(c);
 }
 

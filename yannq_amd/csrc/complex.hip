@@ -454,9 +454,11 @@ __global__ void k_build_Oh(const int8_t* __restrict__ sigma, const double* __res
 }
 
 // Oh -= omean_h (in place); partial[k][c] = sum_{s in chunk k} Oh_sc e_s
+// conj(O)_c = rs_s (conj(O) - om) (rs_s = 1 when rs is null) and partial
+// sums of rs_s conj(O)_c e_s
 __global__ void k_center_force_c(double* __restrict__ Oh, const double* __restrict__ om,
                                  const double* __restrict__ e, int ns, int dim,
-                                 double* __restrict__ partial) {
+                                 const double* __restrict__ rs, double* __restrict__ partial) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= dim) return;
   const int k = blockIdx.y;
@@ -465,10 +467,11 @@ __global__ void k_center_force_c(double* __restrict__ Oh, const double* __restri
   double fr = 0.0, fi = 0.0;
   for (int s = s0; s < s1; ++s) {
     double* o = Oh + 2 * ((size_t)s * dim + c);
-    const double vr = o[0] - mr, vi = o[1] - mi;
+    const double f = rs ? rs[s] : 1.0;
+    const double vr = (o[0] - mr) * f, vi = (o[1] - mi) * f;
     o[0] = vr;
     o[1] = vi;
-    const double er = e[2 * s], ei = e[2 * s + 1];
+    const double er = f * e[2 * s], ei = f * e[2 * s + 1];
     fr += vr * er - vi * ei;
     fi += vr * ei + vi * er;
   }
@@ -579,9 +582,9 @@ void launch_build_Oh(const int8_t* sigma, const double* T, int ns, int n, int m,
 }
 
 void launch_center_force_c(double* Oh, const double* om, const double* e, int ns, int dim,
-                           double* partial, hipStream_t st) {
+                           double* partial, hipStream_t st, const double* rs) {
   hipLaunchKernelGGL(k_center_force_c, dim3(nblk(dim), (ns + kChunkC - 1) / kChunkC), dim3(256),
-                     0, st, Oh, om, e, ns, dim, partial);
+                     0, st, Oh, om, e, ns, dim, rs, partial);
 }
 
 void launch_absq_colsum(const double* Oh, int ns, int dim, double* partial, hipStream_t st) {

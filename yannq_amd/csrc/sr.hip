@@ -11,6 +11,8 @@
 //   (O^T r)_ij = sum_s sigma'_si r_s T'_sj,
 // with V' the parameter vector laid out on the (n+1) x (m+1) grid.  All
 // reductions here use fixed orders, so results are run-to-run identical.
+#include <algorithm>
+
 #include "yq_internal.h"
 
 namespace yq {
@@ -46,15 +48,21 @@ __global__ void k_build_O(const int8_t* __restrict__ sigma,
   }
 }
 
-// partial[k][c] = sum over samples of chunk k of X[s][c], in sample order
+// partial[k][c] = sum over samples of chunk k of w_s X[s][c] (w_s = 1 when w
+// is null; X[s][c]^2 instead of X[s][c] when square), in sample order
 __global__ void k_colsum(const double* __restrict__ X, int ns, int dim,
+                         const double* __restrict__ w, int square,
                          double* __restrict__ partial) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= dim) return;
   const int k = blockIdx.y;
   const int s0 = k * kChunk, s1 = min(ns, s0 + kChunk);
   double acc = 0.0;
-  for (int s = s0; s < s1; ++s) acc += X[(size_t)s * dim + c];
+  for (int s = s0; s < s1; ++s) {
+    const double x = X[(size_t)s * dim + c];
+    const double v = square ? x * x : x;
+    acc += w ? w[s] * v : v;
+  }
   partial[(size_t)k * dim + c] = acc;
 }
 
@@ -68,9 +76,12 @@ __global__ void k_reduce_partials(const double* __restrict__ partial,
   out[c] = acc * scale;
 }
 
+// O_c = rs_s (O - omean) (rs_s = 1 when rs is null) and partial sums of
+// rs_s O_c e_s
 __global__ void k_center_force(double* __restrict__ O,
                                const double* __restrict__ omean,
                                const double* __restrict__ e, int ns, int dim,
+                               const double* __restrict__ rs,
                                double* __restrict__ partial) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= dim) return;
@@ -79,9 +90,10 @@ __global__ void k_center_force(double* __restrict__ O,
   const double mu = omean[c];
   double acc = 0.0;
   for (int s = s0; s < s1; ++s) {
-    const double v = O[(size_t)s * dim + c] - mu;
+    const double f = rs ? rs[s] : 1.0;
+    const double v = (O[(size_t)s * dim + c] - mu) * f;
     O[(size_t)s * dim + c] = v;
-    acc += v * e[s];
+    acc += v * (f * e[s]);
   }
   partial[(size_t)k * dim + c] = acc;
 }
@@ -94,6 +106,23 @@ __global__ void k_copy_lower_shift(const double* __restrict__ S,
        q += (size_t)gridDim.x * blockDim.x) {
     const size_t r = q % dim, c = q / dim;  // column-major
     M[q] = S[q] + (r == c ? shift : 0.0);
+  }
+}
+
+// S = sum_p P[p] over the lower triangle (column-major dim x dim, nc doubles
+// per entry), partial products in index order
+__global__ void k_sum_lower(const double* __restrict__ P, int np, size_t dim, int nc,
+                            double* __restrict__ S) {
+  const size_t total = dim * dim;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = q % dim, c = q / dim;
+    if (r < c) continue;
+    for (int k = 0; k < nc; ++k) {
+      double acc = 0.0;
+      for (int p = 0; p < np; ++p) acc += P[((size_t)p * total + q) * nc + k];
+      S[q * nc + k] = acc;
+    }
   }
 }
 
@@ -353,9 +382,9 @@ void launch_build_O(const int8_t* sigma, const double* T, int ns, int n, int m,
 int colsum_chunks(int ns) { return (ns + kChunk - 1) / kChunk; }
 
 void launch_colsum(const double* X, int ns, int dim, double* partial,
-                   hipStream_t st) {
+                   hipStream_t st, const double* w, int square) {
   hipLaunchKernelGGL(k_colsum, dim3(nblk(dim), colsum_chunks(ns)), dim3(256), 0, st,
-                     X, ns, dim, partial);
+                     X, ns, dim, w, square, partial);
 }
 
 void launch_reduce_partials(const double* partial, int chunks, int dim,
@@ -365,9 +394,15 @@ void launch_reduce_partials(const double* partial, int chunks, int dim,
 }
 
 void launch_center_force(double* O, const double* omean, const double* e,
-                         int ns, int dim, double* partial, hipStream_t st) {
+                         int ns, int dim, double* partial, hipStream_t st, const double* rs) {
   hipLaunchKernelGGL(k_center_force, dim3(nblk(dim), colsum_chunks(ns)), dim3(256),
-                     0, st, O, omean, e, ns, dim, partial);
+                     0, st, O, omean, e, ns, dim, rs, partial);
+}
+
+void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hipStream_t st) {
+  const size_t total = dim * dim;
+  const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_sum_lower, dim3(grid), dim3(256), 0, st, P, np, dim, nc, S);
 }
 
 void launch_add_diag(double* M, int dim, double shift, hipStream_t st) {

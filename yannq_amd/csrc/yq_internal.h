@@ -113,16 +113,19 @@ void launch_tanh(const void* theta, void* T, size_t count, int use_float,
                  hipStream_t st);
 void launch_build_O(const int8_t* sigma, const double* T, int ns, int n, int m,
                     int use_bias, double* O, hipStream_t st);
-// partial column sums of X (ns x dim row-major) over sample chunks
+// partial column sums of X (ns x dim row-major) over sample chunks, of
+// w_s X[s][c] when w is given, of X^2 when square
 int colsum_chunks(int ns);
 void launch_colsum(const double* X, int ns, int dim, double* partial,
-                   hipStream_t st);
+                   hipStream_t st, const double* w = nullptr, int square = 0);
 // out[c] = scale * sum_k partial[k][c]
 void launch_reduce_partials(const double* partial, int chunks, int dim,
                             double scale, double* out, hipStream_t st);
-// O -= omean (in place) and partial[k][c] = sum_{s in chunk k} O_c[s][c] e[s]
+// O_c = rs_s (O - omean) in place (rs_s = 1 without rs) and
+// partial[k][c] = sum_{s in chunk k} O_c[s][c] rs_s e[s]
 void launch_center_force(double* O, const double* omean, const double* e,
-                         int ns, int dim, double* partial, hipStream_t st);
+                         int ns, int dim, double* partial, hipStream_t st,
+                         const double* rs = nullptr);
 void launch_add_diag(double* M, int dim, double shift, hipStream_t st);
 void launch_copy_lower_shift(const double* S, double* M, int dim,
                              double shift, hipStream_t st);
@@ -173,9 +176,28 @@ void launch_tanh_c(const void* theta, int in_float, double* T, size_t count, hip
 // Oh = conj(O), ns x dim complex row-major
 void launch_build_Oh(const int8_t* sigma, const double* T, int ns, int n, int m, int use_bias,
                      double* Oh, hipStream_t st);
-// Oh -= omean_h; partial[k][c] = sum_{s in chunk k} Oh_sc e_s (complex)
+// Oh = rs_s (Oh - omean_h); partial[k][c] = sum_{s in chunk k} Oh_sc rs_s e_s
+// (complex; rs_s = 1 without rs)
 void launch_center_force_c(double* Oh, const double* omean_h, const double* e, int ns, int dim,
-                           double* partial, hipStream_t st);
+                           double* partial, hipStream_t st, const double* rs = nullptr);
+// S = sum of np column-major dim x dim partial products, lower triangle
+// (nc = 2: complex entries as pairs)
+void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hipStream_t st);
+// ---------------------------------------------------------------- exact sums
+// full-basis / sector configurations as spins, |psi|^2 weights (and their
+// row scales sqrt(B w)), weighted E_loc statistics, the CDF and the draws of
+// the exact sampler (exact.hip)
+int exact_partials(int B);
+int exact_scan_tiles(int B);
+void launch_basis_spins(const uint32_t* basis, int B, int n, int8_t* sigma, hipStream_t st);
+void launch_exact_weights(const int8_t* sigma, const double* theta, const double* a, int B, int n,
+                          int m, int cplx, double* logpsi, double* w, double* rs, double* part,
+                          hipStream_t st);
+void launch_eloc_stats_w(const double* e, const double* w, int ns, int cplx, double* out,
+                         hipStream_t st);
+void launch_exact_cdf(const double* w, int B, double* cdf, double* tile_sums, hipStream_t st);
+void launch_exact_draw(const double* cdf, int B, const uint32_t* basis, int n, int count,
+                       uint64_t seed, int8_t* sigma, int* idx, hipStream_t st);
 // partial[k][c] = sum_{s in chunk k} |Oh_sc|^2 (real)
 void launch_absq_colsum(const double* Oh, int ns, int dim, double* partial, hipStream_t st);
 void launch_dup2(const double* x, double* y, size_t n, hipStream_t st);  // y[2i] = y[2i+1] = x[i]

@@ -70,6 +70,7 @@ enum Domain : uint32_t {
   kDomRandomize = 1u,  // initial spin configurations
   kDomParams = 2u,     // parameter initialisation
   kDomUniform = 3u,    // uniform configurations for batch inputs
+  kDomExact = 4u,      // draws of the exact (inverse-CDF) sampler
 };
 
 YQ_HD inline u32x4 draw(uint64_t seed, uint64_t event, uint32_t stream,
