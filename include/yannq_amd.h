@@ -154,8 +154,12 @@ YQ_API int yq_ham_set(yq_ctx* ctx, const yq_ham_desc* desc);
 /* ---- Sampler: Samplers/SamplerMT.hpp + Local/SwapSweeper --------------- */
 /* SamplerMT(qs, nTmps, nChainsPer, sweeper): SamplerMT.hpp:157-173, with the
  * sweeper's nSweep (LocalSweeper.hpp:19, SwapSweeper.hpp:70) as
- * sweep_factor.  Only n_temps == 1 is supported.  seed keys the Philox
- * streams of every chain. */
+ * sweep_factor.  seed keys the Philox streams of every chain.  n_temps 1, or
+ * even and <= 16 (real machines): parallel tempering with nTmps x nChainsPer
+ * chain states, state g * n_temps + t at beta_t = (n_temps - t) / n_temps,
+ * replica swaps after every sweep (mixChains, SamplerMT.hpp:120-153) and one
+ * sample per chain group per sweep from the beta = 1 slot; state and counter
+ * arrays of the calls below then cover all n_temps x n_chains_per states. */
 YQ_API int yq_sampler_config(yq_ctx* ctx, int sweeper, int n_temps,
                              int n_chains_per, int sweep_factor, int precision,
                              uint64_t seed);

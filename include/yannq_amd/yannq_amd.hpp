@@ -320,15 +320,17 @@ class SamplerMT {
 
   SamplerMT(const Machine& qs, uint32_t nTmps, uint32_t nChainsPer, Sweeper& sweeper,
             uint64_t seed = 0x5eed, int precision = YQ_FP64)
-      : qs_(qs), nChainsPer_(nChainsPer), sweeper_(sweeper), seed_(seed),
+      : qs_(qs), nTmps_(nTmps), nChainsPer_(nChainsPer), sweeper_(sweeper), seed_(seed),
         precision_(precision) {
-    if (nTmps != 1) throw Error(YQ_ERR_UNSUPPORTED, "SamplerMT: only nTmps == 1");
-    configure();
+    configure();  // nTmps: 1, or even and <= 16 (parallel tempering, :44-60, :120-153)
   }
-  std::string desc() const {
-    return "{\"name\": \"SamplerMT\", \"num_temps\": 1, \"num_chains_per_each_temp\": " +
-           std::to_string(nChainsPer_) + ", \"sweeper\": " + sweeper_.desc() + "}";
+  std::string desc() const {  // :62-70
+    return "{\"name\": \"SamplerMT\", \"num_temps\": " + std::to_string(nTmps_) +
+           ", \"num_chains_per_each_temp\": " + std::to_string(nChainsPer_) +
+           ", \"sweeper\": " + sweeper_.desc() + "}";
   }
+  // betas_ (:48-51): beta of state idx
+  double beta(uint32_t idx) const { return double(nTmps_ - idx % nTmps_) / nTmps_; }
   // :185-192 -- re-keys the counter-based streams (keeps the seed by default)
   void initializeRandomEngine() { configure(); }
   void initializeRandomEngine(uint64_t seed) {
@@ -337,13 +339,14 @@ class SamplerMT {
   }
   void randomizeSigma() { detail::check(ctx(), yq_sampler_randomize(ctx(), -1)); }  // :194-202
   void randomizeSigma(int nup) { detail::check(ctx(), yq_sampler_randomize(ctx(), nup)); }
-  // :216-223: randomizer(re) -> configuration, called once per chain
+  // :216-223: randomizer(re) -> configuration, called once per chain state
   template <class Randomizer>
   void randomize(Randomizer&& randomizer) {
     std::mt19937_64 re(seed_ ^ (++epoch_ * 0x9E3779B97F4A7C15ull));
     const uint32_t n = qs_.getN();
-    std::vector<int8_t> all((size_t)nChainsPer_ * n);
-    for (uint32_t c = 0; c < nChainsPer_; ++c) {
+    const uint32_t states = nTmps_ * nChainsPer_;
+    std::vector<int8_t> all((size_t)states * n);
+    for (uint32_t c = 0; c < states; ++c) {
       const auto s = randomizer(re);
       if ((uint32_t)s.size() != n) throw Error(YQ_ERR_ARG, "randomizer: wrong length");
       for (uint32_t i = 0; i < n; ++i) all[(size_t)c * n + i] = (int8_t)s[i];
@@ -361,11 +364,11 @@ class SamplerMT {
  private:
   yq_ctx* ctx() const { return qs_.ctx(); }
   void configure() {
-    detail::check(ctx(), yq_sampler_config(ctx(), Sweeper::kind, 1, (int)nChainsPer_,
+    detail::check(ctx(), yq_sampler_config(ctx(), Sweeper::kind, (int)nTmps_, (int)nChainsPer_,
                                            (int)sweeper_.nSweep(), precision_, seed_));
   }
   const Machine& qs_;
-  uint32_t nChainsPer_;
+  uint32_t nTmps_, nChainsPer_;
   Sweeper& sweeper_;
   uint64_t seed_;
   int precision_;

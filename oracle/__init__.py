@@ -70,6 +70,7 @@ def lib():
             "rp_exp_f": (C.c_float, [C.c_float]),
             "rp_philox": (None, [u32, u32, u32, u32, u32, u32, p]),
             "rp_sample": (i, [p, i, p, p, p, p, p, p, p, i, i, p, p]),
+            "rp_sample_pt": (i, [p, i, i, p, p, p, p, p, p, p, i, i, p, p]),
             "rp_eloc": (None, [p, i, i, i, i, p, p, p, p, p]),
             "rp_randomize": (None, [u64, u64, i, p, i, i, p]),
             "or_ff_npar": (i, [p]),
@@ -312,6 +313,28 @@ def sample_replay(rbm, sigma, *, seed, n_sweeps, n_discard, kind=LOCAL, precisio
         rc = L.rp_sample(C.byref(cfg), Cn, _p(streams), _p(rbm.W), _p(rbm.a), _p(rbm.b),
                          _p(sigma), _p(ctr), _p(acc), n_sweeps, n_discard, _p(out_s),
                          _p(out_t))
+    if rc:
+        raise ValueError("bad replay configuration")
+    return sigma, ctr, acc, out_s, out_t
+
+
+def sample_replay_pt(rbm, sigma, *, n_temps, seed, n_sweeps, n_discard, kind=LOCAL,
+                     precision=FP64, lanes=64, sweep_factor=1, ctr=None, acc=None):
+    """Replay of the parallel-tempering kernel: sigma holds all n_temps x groups
+    chains (chain g * n_temps + t at beta (n_temps - t) / n_temps).  Returns
+    (sigma_final, ctr, acc, samples_sigma (n_sweeps*groups x n), samples_theta)."""
+    sigma = i8(sigma).copy()
+    Cn = sigma.shape[0]
+    groups = Cn // n_temps
+    streams = np.arange(Cn, dtype=np.uint32)
+    ctr = np.zeros(Cn, np.uint64) if ctr is None else np.array(ctr, dtype=np.uint64)
+    acc = np.zeros(Cn, np.uint64) if acc is None else np.array(acc, dtype=np.uint64)
+    cfg = RpCfg(rbm.n, rbm.m, int(rbm.use_bias), kind, precision, lanes, sweep_factor, 1.0, seed)
+    out_s = np.empty((n_sweeps * groups, rbm.n), dtype=np.int8)
+    out_t = np.empty((n_sweeps * groups, rbm.m))
+    rc = lib().rp_sample_pt(C.byref(cfg), n_temps, groups, _p(streams), _p(rbm.W), _p(rbm.a),
+                            _p(rbm.b), _p(sigma), _p(ctr), _p(acc), n_sweeps, n_discard,
+                            _p(out_s), _p(out_t))
     if rc:
         raise ValueError("bad replay configuration")
     return sigma, ctr, acc, out_s, out_t

@@ -549,6 +549,137 @@ void replay_chain(const rp_cfg* cfg, uint32_t stream, const double* W,
   }
 }
 
+// Parallel tempering (SamplerMT.hpp:44-60, 112-204 with nTmps = T > 1) with
+// the tempering kernel's arithmetic: the T chains of a group sweep at
+// beta_t = (T - t) / T, then swap states pairwise ((0,1), (2,3), ..., then
+// (1,2), ..., (T-3, T-2)) iff u < exp((beta_{t+1} - beta_t) 2 (lp_t - lp_{t+1}))
+// with lp = Re log psi as a lane-strided + tree sum and u drawn from the lower
+// chain's stream at its proposal counter (mix domain); slot 0 is recorded.
+template <typename R>
+struct PtChain {
+  int8_t* sig;
+  std::vector<R> th, lc;
+  uint64_t* ctr;
+  uint64_t* acc;
+  uint32_t stream;
+};
+
+template <typename R>
+void replay_pt_group(const rp_cfg* cfg, int T, int groups, int g, const uint32_t* stream,
+                     const double* W, const double* a, const double* b, int8_t* sigma,
+                     uint64_t* ctr, uint64_t* acc, int n_sweeps, int n_discard,
+                     int8_t* out_sigma, double* out_theta) {
+  const int n = cfg->n, m = cfg->m, L = cfg->lanes;
+  const bool mixed = cfg->precision == 1;
+  std::vector<R> Wr((size_t)n * m), ar(n), t(m), lt(m), part(L);
+  for (size_t q = 0; q < (size_t)n * m; ++q) Wr[q] = (R)W[q];
+  for (int i = 0; i < n; ++i) ar[i] = (R)a[i];
+  std::vector<double> th64(m);
+  std::vector<PtChain<R>> ch(T);
+  auto refresh = [&](PtChain<R>& c) {
+    rp_theta(n, m, W, b, c.sig, th64.data());
+    for (int j = 0; j < m; ++j) {
+      c.th[j] = (R)th64[j];
+      c.lc[j] = yq::Math<R>::logcosh(c.th[j]);
+    }
+  };
+  for (int q = 0; q < T; ++q) {
+    const int c = g * T + q;
+    ch[q].sig = sigma + (size_t)c * n;
+    ch[q].th.resize(m);
+    ch[q].lc.resize(m);
+    ch[q].ctr = ctr + c;
+    ch[q].acc = acc + c;
+    ch[q].stream = stream[c];
+    refresh(ch[q]);
+  }
+  const int nprop = n * cfg->sweep_factor;
+  std::vector<R> lp(T);
+  for (int sw = 0; sw < n_discard + n_sweeps; ++sw) {
+    for (int q = 0; q < T; ++q) {
+      PtChain<R>& c = ch[q];
+      if (mixed && sw > 0) refresh(c);
+      const R beta = (R)((double)(T - q) / T);
+      for (int p = 0; p < nprop; ++p) {
+        const yq::u32x4 x = yq::draw(cfg->seed, (*c.ctr)++, c.stream, yq::kDomSweep);
+        const int k = (int)yq::index_from(x.x, (uint32_t)n);
+        int l = -1;
+        R cst;
+        if (cfg->kind == 1) {
+          l = (int)yq::index_from(x.y, (uint32_t)n);
+          if (c.sig[k] == c.sig[l]) continue;
+          cst = ((R)-2 * ar[k]) * (R)c.sig[k] - ((R)2 * ar[l]) * (R)c.sig[l];
+        } else {
+          cst = ((R)-2 * ar[k]) * (R)c.sig[k];
+        }
+        for (int z = 0; z < L; ++z) part[z] = (R)0;
+        for (int j = 0; j < m; ++j) {
+          R tj = c.th[j] - (R)(2 * c.sig[k]) * Wr[(size_t)k * m + j];
+          if (l >= 0) tj = tj - (R)(2 * c.sig[l]) * Wr[(size_t)l * m + j];
+          t[j] = tj;
+          lt[j] = yq::Math<R>::logcosh(tj);
+          part[j % L] = part[j % L] + (lt[j] - c.lc[j]);
+        }
+        const R delta = cst + tree_sum(part, L);
+        const bool ok = mixed ? yq::accept_f((float)beta, (float)delta, yq::u01_float(x.z))
+                              : yq::accept_d((double)beta, (double)delta,
+                                             yq::u01_double(x.z, x.w));
+        if (ok) {
+          for (int j = 0; j < m; ++j) {
+            c.th[j] = t[j];
+            c.lc[j] = lt[j];
+          }
+          c.sig[k] = -c.sig[k];
+          if (l >= 0) c.sig[l] = -c.sig[l];
+          ++(*c.acc);
+        }
+      }
+      // lp = a . sigma + sum_j logcosh(theta_j), each as the lane-strided tree
+      for (int z = 0; z < L; ++z) part[z] = (R)0;
+      for (int i = 0; i < n; ++i) part[i % L] = part[i % L] + ar[i] * (R)c.sig[i];
+      const R sa = tree_sum(part, L);
+      for (int z = 0; z < L; ++z) part[z] = (R)0;
+      for (int j = 0; j < m; ++j) part[j % L] = part[j % L] + c.lc[j];
+      const R sl = tree_sum(part, L);
+      lp[q] = sa + sl;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int> d(T, 0);
+      for (int q = pass; q + 1 < T && (pass == 0 || q + 1 <= T - 2); q += 2) {
+        const yq::u32x4 x = yq::draw(cfg->seed, *ch[q].ctr, ch[q].stream, yq::kDomMix);
+        const double bt = (double)(T - q) / T;
+        const double dbeta = (double)(T - q - 1) / T - bt;
+        const R dl = lp[q] - lp[q + 1];
+        d[q] = mixed ? yq::accept_f((float)dbeta, (float)dl, yq::u01_float(x.z))
+                     : yq::accept_d(dbeta, (double)dl, yq::u01_double(x.z, x.w));
+      }
+      for (int q = 0; q + 1 < T; ++q) {
+        if (!d[q]) continue;
+        PtChain<R>& A = ch[q];
+        PtChain<R>& B = ch[q + 1];
+        for (int i = 0; i < n; ++i) std::swap(A.sig[i], B.sig[i]);
+        std::swap(A.th, B.th);
+        for (int j = 0; j < m; ++j) {
+          A.lc[j] = yq::Math<R>::logcosh(A.th[j]);
+          B.lc[j] = yq::Math<R>::logcosh(B.th[j]);
+        }
+        std::swap(lp[q], lp[q + 1]);
+      }
+    }
+    if (sw >= n_discard) {
+      const size_t row = (size_t)(sw - n_discard) * groups + g;
+      PtChain<R>& c = ch[0];
+      memcpy(out_sigma + row * n, c.sig, n);
+      if (mixed) {
+        refresh(c);
+        for (int j = 0; j < m; ++j) out_theta[row * m + j] = (double)c.th[j];
+      } else {
+        for (int j = 0; j < m; ++j) out_theta[row * m + j] = (double)c.th[j];
+      }
+    }
+  }
+}
+
 int sample_range(const rp_cfg* cfg, int C, const uint32_t* stream,
                  const double* W, const double* a, const double* b,
                  int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
@@ -638,6 +769,22 @@ int rp_sample(const rp_cfg* cfg, int C, const uint32_t* stream,
   if (cfg->lanes <= 0 || (cfg->lanes & (cfg->lanes - 1))) return 1;
   return sample_range(cfg, C, stream, W, a, b, sigma, ctr, acc, n_sweeps,
                       n_discard, out_sigma, out_theta, 0, C);
+}
+
+int rp_sample_pt(const rp_cfg* cfg, int T, int groups, const uint32_t* stream,
+                 const double* W, const double* a, const double* b, int8_t* sigma,
+                 uint64_t* ctr, uint64_t* acc, int n_sweeps, int n_discard,
+                 int8_t* out_sigma, double* out_theta) {
+  if (cfg->lanes <= 0 || (cfg->lanes & (cfg->lanes - 1)) || T < 2 || T % 2) return 1;
+  for (int g = 0; g < groups; ++g) {
+    if (cfg->precision == 1)
+      replay_pt_group<float>(cfg, T, groups, g, stream, W, a, b, sigma, ctr, acc, n_sweeps,
+                             n_discard, out_sigma, out_theta);
+    else
+      replay_pt_group<double>(cfg, T, groups, g, stream, W, a, b, sigma, ctr, acc, n_sweeps,
+                              n_discard, out_sigma, out_theta);
+  }
+  return 0;
 }
 
 void rp_eloc(const or_ham* H, int precision, int ns, int n, int m,

@@ -121,6 +121,13 @@ int rp_sample(const rp_cfg* cfg, int C, const uint32_t* stream,
               const double* W, const double* a, const double* b,
               int8_t* sigma, uint64_t* ctr, uint64_t* acc, int n_sweeps,
               int n_discard, int8_t* out_sigma, double* out_theta);
+// Parallel tempering replay: T temperatures per chain group, groups groups,
+// C = T * groups chains (sigma, ctr, acc, stream of all C); n_sweeps * groups
+// samples (the beta = 1 slot of every group) out.
+int rp_sample_pt(const rp_cfg* cfg, int T, int groups, const uint32_t* stream,
+                 const double* W, const double* a, const double* b, int8_t* sigma,
+                 uint64_t* ctr, uint64_t* acc, int n_sweeps, int n_discard,
+                 int8_t* out_sigma, double* out_theta);
 // Local energies with the kernel's arithmetic. precision 0: fp64, 1: fp32.
 void rp_eloc(const or_ham* H, int precision, int ns, int n, int m,
              const double* W, const double* a, const int8_t* sigma,

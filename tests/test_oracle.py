@@ -272,7 +272,7 @@ def test_randomize(orc):
     assert not np.array_equal(orc.randomize(11, 1, np.arange(4), 37), s1[:4])
 
 
-def _python_sweep(orc, rbm, sig, seed, stream, ctr, kind):
+def _python_sweep(orc, rbm, sig, seed, stream, ctr, kind, beta=1.0):
     """One sweep replayed with the libm restatement (or_log_ratio*) and the
     Philox draws; decisions must coincide with the kernel-arithmetic replay."""
     n = rbm.n
@@ -290,7 +290,7 @@ def _python_sweep(orc, rbm, sig, seed, stream, ctr, kind):
         else:
             d = orc.log_ratio1(rbm, sig, th, k)
         u = (((int(x[2]) >> 5) << 26) | (int(x[3]) >> 6)) / 2.0 ** 53
-        if u < min(1.0, math.exp(2 * d)):
+        if u < min(1.0, math.exp(beta * 2 * d)):
             sig[k] = -sig[k]
             if kind == orc.SWAP:
                 sig[l] = -sig[l]
@@ -345,3 +345,70 @@ def test_sampler_distribution_matches_psi2(orc):
         fin, _, _, ss2, st2 = orc.sample_replay(rbm, sig0, seed=77, n_sweeps=20, n_discard=0,
                                                 precision=precision)
         np.testing.assert_allclose(st2, orc.theta(rbm, ss2), atol=2e-7)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_tempering_replay_matches_restatement(orc, kind):
+    """Parallel tempering (SamplerMT.hpp:120-153): sweeps at beta_t = (T-t)/T
+    with the libm restatement, then the two swap passes with
+    p = exp((beta_{t+1} - beta_t) 2 (log psi_t - log psi_{t+1})) from
+    or_log_coeff and the lower chain's mix-domain Philox draw."""
+    rng = np.random.default_rng(18)
+    n, m, T, G = 12, 24, 4, 2
+    rbm = orc.RBM.random(n, m, 0.4, rng)
+    Cn = T * G
+    sig0 = orc.randomize(6, 0, np.arange(Cn), n, nup=n // 2 if kind else -1)
+    seed = 0x2468_ace0_1357
+    sweeps = 4
+    fin, ctr, acc, ss, st = orc.sample_replay_pt(rbm, sig0, n_temps=T, seed=seed,
+                                                 n_sweeps=sweeps, n_discard=0, kind=kind)
+    assert (ctr == sweeps * n).all()
+    swaps = 0
+    for g in range(G):
+        state = [sig0[g * T + t].copy() for t in range(T)]
+        for sw in range(sweeps):
+            for t in range(T):
+                c = g * T + t
+                state[t] = _python_sweep(orc, rbm, state[t], seed, c, sw * n, kind,
+                                         beta=(T - t) / T)
+            lp = [orc.log_coeff(rbm, state[t]) for t in range(T)]
+            for pas in (0, 1):
+                for t in range(pas, T - 1 - pas, 2):
+                    c = g * T + t
+                    ev = (sw + 1) * n
+                    x = orc.philox((ev & 0xffffffff, ev >> 32, c, 5),
+                                   (seed & 0xffffffff, seed >> 32))
+                    u = (((int(x[2]) >> 5) << 26) | (int(x[3]) >> 6)) / 2.0 ** 53
+                    dbeta = (T - t - 1) / T - (T - t) / T
+                    if u < min(1.0, math.exp(dbeta * 2 * (lp[t] - lp[t + 1]))):
+                        state[t], state[t + 1] = state[t + 1], state[t]
+                        lp[t], lp[t + 1] = lp[t + 1], lp[t]
+                        swaps += 1
+            np.testing.assert_array_equal(ss[sw * G + g], state[0])
+        for t in range(T):
+            np.testing.assert_array_equal(fin[g * T + t], state[t])
+    assert swaps > 0
+    np.testing.assert_allclose(st, orc.theta(rbm, ss), atol=1e-13)
+
+
+def test_tempering_distribution_matches_psi2(orc):
+    """The beta = 1 slot of a tempered chain samples |psi|^2 (chi^2, N = 6),
+    with swaps that move states between the temperatures."""
+    rng = np.random.default_rng(19)
+    n = 6
+    rbm = orc.RBM.random(n, 2 * n, 0.8, rng)
+    basis, th, logpsi = _psi_table(orc, rbm)
+    p = np.exp(2 * (logpsi - logpsi.max()))
+    p /= p.sum()
+    T, G = 4, 32
+    sig0 = orc.randomize(3, 0, np.arange(T * G), n)
+    _, _, _, ss, st = orc.sample_replay_pt(rbm, sig0, n_temps=T, seed=91, n_sweeps=400,
+                                           n_discard=20)
+    idx = (((1 - ss.astype(np.int64)) // 2) << np.arange(n)).sum(axis=1)
+    cnt = np.bincount(idx, minlength=1 << n)
+    N = cnt.sum()
+    chi2 = ((cnt - N * p) ** 2 / (N * p)).sum()
+    assert chi2 < 3 * 63, chi2
+    _, _, _, ss2, st2 = orc.sample_replay_pt(rbm, sig0, n_temps=T, seed=91, n_sweeps=20,
+                                             n_discard=0, precision=orc.MIXED)
+    np.testing.assert_allclose(st2, orc.theta(rbm, ss2), atol=2e-7)

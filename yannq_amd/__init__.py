@@ -258,7 +258,8 @@ class Context:
     def sampler_config(self, sweeper, chains, precision=FP64, seed=1, sweep_factor=1, n_temps=1):
         self._chk(self._L.yq_sampler_config(self._h, sweeper, n_temps, chains, sweep_factor,
                                             precision, seed))
-        self.chains = chains
+        self.chains = chains * n_temps   # chain states (n_temps per chain group)
+        self.n_temps = n_temps
 
     def randomize(self, nup=-1):
         self._chk(self._L.yq_sampler_randomize(self._h, nup))

@@ -102,6 +102,7 @@ struct yq_ctx {
   // sampler
   bool have_sampler = false;
   int skind = 0, chains = 0, sweep_factor = 1, sprec = 0;
+  int ntemps = 1;  // parallel tempering: chains = ntemps x chain groups
   uint64_t seed = 0, rand_epoch = 0;
   Buf sig_state, ctr, acc;
 
@@ -1092,11 +1093,17 @@ YQ_API int yq_sampler_config(yq_ctx* c, int sweeper, int n_temps, int n_chains_p
                              int sweep_factor, int precision, uint64_t seed) {
   TRY(need_rbm(c));
   if (sweeper != YQ_LOCAL && sweeper != YQ_EXCHANGE) return fail(c, YQ_ERR_ARG, "unknown sweeper");
-  if (n_temps != 1) return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering (n_temps > 1)");
+  // SamplerMT.hpp:47: nTmps even or 1; one block of nTmps wavefronts per chain group
+  if (n_temps < 1 || n_temps > 16 || (n_temps > 1 && n_temps % 2))
+    return fail(c, YQ_ERR_ARG, "n_temps must be 1 or even and <= 16");
+  if (n_temps > 1 && c->cplx)
+    return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering of complex machines");
   if (n_chains_per < 1 || sweep_factor < 1) return fail(c, YQ_ERR_ARG, "chains and sweep factor >= 1");
+  if ((long long)n_temps * n_chains_per > (1 << 24)) return fail(c, YQ_ERR_ARG, "too many chains");
   if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
   c->skind = sweeper;
-  c->chains = n_chains_per;
+  c->ntemps = n_temps;
+  c->chains = n_temps * n_chains_per;
   c->sweep_factor = sweep_factor;
   c->sprec = precision;
   c->seed = seed;
@@ -1172,7 +1179,8 @@ int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   TRY(need_rbm(c));
   if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
   if (n_sweeps < 1 || n_discard < 0) return fail(c, YQ_ERR_ARG, "n_sweeps >= 1, n_discard >= 0");
-  const long long ns = (long long)c->chains * n_sweeps;
+  // one sample per chain group and sweep (the beta = 1 slot, SamplerMT.hpp:163-171)
+  const long long ns = (long long)(c->chains / c->ntemps) * n_sweeps;
   if (ns > (1ll << 30)) return fail(c, YQ_ERR_ARG, "too many samples");
   c->ns = (int)ns;
   c->bprec = c->sprec;
@@ -1207,6 +1215,16 @@ int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   a.kind = c->skind;
   a.mixed = c->sprec == YQ_MIXED;
   a.beta = 1.0;
+  a.ntemps = c->ntemps;
+  if (a.ntemps > 1) {
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess || lds_max <= 0)
+      lds_max = 64 * 1024;
+    if (pt_lds_bytes(a) + 256 > (size_t)lds_max)
+      return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering: exchange area exceeds LDS");
+  }
   a.seed = c->seed;
   return YQ_OK;
 }

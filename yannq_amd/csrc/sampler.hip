@@ -24,137 +24,300 @@
 namespace yq {
 namespace {
 
+// theta_j = (sum_i +/-W_ji, index order, fp64) + b_j, then rounded to R, and
+// lc_j = logcosh(theta_j), for the hidden units of this lane
+template <typename R, int RMAX>
+__device__ __forceinline__ void refresh_theta(const SampleArgs& A, const int8_t* sig, int lane,
+                                              R (&th)[RMAX], R (&lc)[RMAX]) {
+  const int n = A.n, m = A.m;
+  double accd[RMAX];
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) accd[r] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const bool up = sig[i] > 0;
+    const double* wc = A.W64 + (size_t)i * m;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) {
+        const double w = wc[j];
+        accd[r] = up ? accd[r] + w : accd[r] - w;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) {
+    const int j = lane + kWave * r;
+    if (j < m) {
+      th[r] = (R)(accd[r] + A.b64[j]);
+      lc[r] = Math<R>::logcosh(th[r]);
+    } else {
+      th[r] = (R)0;
+      lc[r] = (R)0;
+    }
+  }
+}
+
+// One sweep (A.nprop proposals) of one chain at inverse temperature beta;
+// returns the accepted count.  ev is the chain's proposal counter.
+template <typename R, int KIND, int RMAX>
+__device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, int8_t* sig,
+                                                int lane, R (&th)[RMAX], R (&lc)[RMAX],
+                                                R (&tt)[RMAX], R (&lt)[RMAX], uint64_t& ev,
+                                                R beta) {
+  const int n = A.n, m = A.m;
+  const R* Wr = static_cast<const R*>(A.Wr);
+  const R* ar = static_cast<const R*>(A.ar);
+  uint64_t nacc = 0;
+  for (int p = 0; p < A.nprop; ++p) {
+    const u32x4 x = draw(A.seed, ev++, (uint32_t)chain, kDomSweep);
+    const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
+    const int8_t sk = sig[k];
+    const R twok = sk > 0 ? (R)2 : (R)-2;
+    const R* wk = Wr + (size_t)k * m;
+    R cst = ((R)-2 * ar[k]) * (R)sk;
+    int l = 0;
+    int8_t sl = 0;
+    R twol = (R)0;
+    const R* wl = wk;
+    if (KIND == 1) {
+      l = __builtin_amdgcn_readfirstlane((int)index_from(x.y, (uint32_t)n));
+      sl = sig[l];
+      if (sk == sl) continue;  // SwapSweeper.hpp:44-47, uniform branch
+      twol = sl > 0 ? (R)2 : (R)-2;
+      wl = Wr + (size_t)l * m;
+      cst = cst - ((R)2 * ar[l]) * (R)sl;
+    }
+    R part = (R)0;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) {
+        R t = th[r] - twok * wk[j];
+        if (KIND == 1) t = t - twol * wl[j];
+        tt[r] = t;
+        lt[r] = Math<R>::logcosh(t);
+        part = part + (lt[r] - lc[r]);
+      }
+    }
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
+    const R delta = cst + part;
+    bool ok;
+    if (sizeof(R) == 4)
+      ok = accept_f((float)beta, (float)delta, u01_float(x.z));
+    else
+      ok = accept_d((double)beta, (double)delta, u01_double(x.z, x.w));
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) {
+          th[r] = tt[r];
+          lc[r] = lt[r];
+        }
+      }
+      sig[k] = (int8_t)-sk;  // every lane stores the same byte
+      if (KIND == 1) sig[l] = (int8_t)-sl;
+      ++nacc;
+    }
+  }
+  return nacc;
+}
+
+// sample row `row`: spins and theta (mixed precision: theta refreshed from
+// the fp64 parameters first, fresh is then set)
+template <typename R, int RMAX>
+__device__ __forceinline__ void record_row(const SampleArgs& A, size_t row, const int8_t* sig,
+                                           int lane, R (&th)[RMAX], R (&lc)[RMAX],
+                                           bool& fresh) {
+  const int n = A.n, m = A.m;
+  for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+  if (A.mixed) {
+    refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+    fresh = true;
+  }
+  R* ot = static_cast<R*>(A.out_theta) + row * m;
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) {
+    const int j = lane + kWave * r;
+    if (j < m) ot[j] = th[r];
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One chain of one machine, run by one wavefront: sig is the wave's LDS spin
 // strip.  RMAX >= ceil(m/64); larger values only add predicated-off slots,
 // so the arithmetic (and the replay) does not depend on it.
 template <typename R, int KIND, int RMAX>
 __device__ void run_chain(const SampleArgs& A, int chain, int8_t* sig, int lane) {
-  const int n = A.n, m = A.m;
+  const int n = A.n;
   for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-  const R* Wr = static_cast<const R*>(A.Wr);
-  const R* ar = static_cast<const R*>(A.ar);
+  wave_sync();
   R th[RMAX], lc[RMAX], tt[RMAX], lt[RMAX];
-
-  // theta_j = (sum_i +/-W_ji, index order, fp64) + b_j, then rounded to R
-  auto refresh = [&]() {
-    double accd[RMAX];
-#pragma unroll
-    for (int r = 0; r < RMAX; ++r) accd[r] = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const bool up = sig[i] > 0;
-      const double* wc = A.W64 + (size_t)i * m;
-#pragma unroll
-      for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
-        if (j < m) {
-          const double w = wc[j];
-          accd[r] = up ? accd[r] + w : accd[r] - w;
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
-      if (j < m) {
-        th[r] = (R)(accd[r] + A.b64[j]);
-        lc[r] = Math<R>::logcosh(th[r]);
-      } else {
-        th[r] = (R)0;
-        lc[r] = (R)0;
-      }
-    }
-  };
-
   uint64_t ev = A.ctr[chain];
   uint64_t nacc = 0;
   const R beta = (R)A.beta;
-  refresh();
+  refresh_theta<R, RMAX>(A, sig, lane, th, lc);
   bool fresh = true;
   const int total = A.n_discard + A.n_sweeps;
   for (int sw = 0; sw < total; ++sw) {
-    if (A.mixed && !fresh) refresh();
+    if (A.mixed && !fresh) refresh_theta<R, RMAX>(A, sig, lane, th, lc);
     fresh = false;
-    for (int p = 0; p < A.nprop; ++p) {
-      const u32x4 x = draw(A.seed, ev++, (uint32_t)chain, kDomSweep);
-      const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
-      const int8_t sk = sig[k];
-      const R twok = sk > 0 ? (R)2 : (R)-2;
-      const R* wk = Wr + (size_t)k * m;
-      R cst = ((R)-2 * ar[k]) * (R)sk;
-      int l = 0;
-      int8_t sl = 0;
-      R twol = (R)0;
-      const R* wl = wk;
-      if (KIND == 1) {
-        l = __builtin_amdgcn_readfirstlane((int)index_from(x.y, (uint32_t)n));
-        sl = sig[l];
-        if (sk == sl) continue;  // SwapSweeper.hpp:44-47, uniform branch
-        twol = sl > 0 ? (R)2 : (R)-2;
-        wl = Wr + (size_t)l * m;
-        cst = cst - ((R)2 * ar[l]) * (R)sl;
-      }
-      R part = (R)0;
-#pragma unroll
-      for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
-        if (j < m) {
-          R t = th[r] - twok * wk[j];
-          if (KIND == 1) t = t - twol * wl[j];
-          tt[r] = t;
-          lt[r] = Math<R>::logcosh(t);
-          part = part + (lt[r] - lc[r]);
-        }
-      }
-#pragma unroll
-      for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
-      const R delta = cst + part;
-      bool ok;
-      if (sizeof(R) == 4)
-        ok = accept_f((float)beta, (float)delta, u01_float(x.z));
-      else
-        ok = accept_d((double)beta, (double)delta, u01_double(x.z, x.w));
-      if (ok) {
-#pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-          const int j = lane + kWave * r;
-          if (j < m) {
-            th[r] = tt[r];
-            lc[r] = lt[r];
-          }
-        }
-        sig[k] = (int8_t)-sk;  // every lane stores the same byte
-        if (KIND == 1) sig[l] = (int8_t)-sl;
-        ++nacc;
-      }
-    }
-    if (sw >= A.n_discard) {
-      const size_t row = (size_t)(sw - A.n_discard) * A.C + chain;
-      for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
-      if (A.mixed) {
-        refresh();
-        fresh = true;
-      }
-      R* ot = static_cast<R*>(A.out_theta) + row * m;
-#pragma unroll
-      for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
-        if (j < m) ot[j] = th[r];
-      }
-    }
+    nacc += sweep_chain<R, KIND, RMAX>(A, chain, sig, lane, th, lc, tt, lt, ev, beta);
+    if (sw >= A.n_discard)
+      record_row<R, RMAX>(A, (size_t)(sw - A.n_discard) * A.C + chain, sig, lane, th, lc, fresh);
   }
   for (int i = lane; i < n; i += kWave) A.sigma[(size_t)chain * n + i] = sig[i];
   if (lane == 0) {
     A.ctr[chain] = ev;
     A.acc[chain] += nacc;
   }
-  // the strip is reused by the wave's next chain
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_sync();  // the strip is reused by the wave's next chain
+}
+
+// Parallel tempering (SamplerMT.hpp:44-60, 112-204 with nTmps = T > 1): one
+// block per chain group g.  Chain g * T + t is temperature slot t at
+// beta_t = (T - t) / T with its own Philox stream and counter.  The block's
+// NW waves (the largest of 4, 2 dividing T) take slots w, w + NW, ... in turn;
+// a slot's state (spins and theta) lives in LDS storage idx[t], so a replica
+// swap is an exchange of two storage indices.  After every sweep of all slots
+// the swaps run as in mixChains (:120-153): pairs (0,1), (2,3), ..., then
+// (1,2), ..., (T-3, T-2); the pair (t, t+1) swaps iff
+// u < exp((beta_{t+1} - beta_t) 2 (lp_t - lp_{t+1})), lp = Re log psi =
+// a . sigma + sum_j logcosh(theta_j) (lane-strided sums and an xor butterfly),
+// u from slot t's stream at its proposal counter after the sweep (a separate
+// Philox domain).  Slot 0 (beta = 1) is recorded after every sweep past the
+// discard phase (appendData), row (sweep * C/T + g).
+template <typename R, int KIND, int RMAX>
+__global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
+  extern __shared__ unsigned char lds_raw[];
+  const int T = A.ntemps, n = A.n, m = A.m, npad = A.npad;
+  const int NW = (T % 4 == 0) ? 4 : 2;
+  // dynamic LDS (pt_lds_bytes): lp, counters, accept counts and storage index
+  // per slot, spin strips and theta storage
+  R* lpv = reinterpret_cast<R*>(lds_raw);                           // 16
+  uint64_t* evs = reinterpret_cast<uint64_t*>(lds_raw + 128);       // 16
+  uint64_t* nac = reinterpret_cast<uint64_t*>(lds_raw + 256);       // 16
+  int* idx = reinterpret_cast<int*>(lds_raw + 384);                 // 16
+  int8_t* strips = reinterpret_cast<int8_t*>(lds_raw + 448);        // T x npad
+  R* ths = reinterpret_cast<R*>(lds_raw + 448 + ((T * npad + 15) & ~15));  // T x m
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = blockIdx.x;
+  const int groups = A.C / T;
+  const R* ar = static_cast<const R*>(A.ar);
+  R th[RMAX], lc[RMAX], tt[RMAX], lt[RMAX];
+  // initial states: slot t in storage t, theta refreshed from fp64
+  for (int t = w; t < T; t += NW) {
+    const int chain = g * T + t;
+    int8_t* sig = strips + t * npad;
+    for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
+    wave_sync();
+    refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) ths[t * m + j] = th[r];
+    }
+    if (lane == 0) {
+      evs[t] = A.ctr[chain];
+      nac[t] = 0;
+      idx[t] = t;
+    }
+  }
+  __syncthreads();
+  const int total = A.n_discard + A.n_sweeps;
+  for (int sw = 0; sw < total; ++sw) {
+    for (int t = w; t < T; t += NW) {
+      const int chain = g * T + t;
+      const int st = idx[t];
+      int8_t* sig = strips + st * npad;
+      if (A.mixed && sw > 0) {
+        refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+      } else {
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+          const int j = lane + kWave * r;
+          th[r] = j < m ? ths[st * m + j] : (R)0;
+          lc[r] = j < m ? Math<R>::logcosh(th[r]) : (R)0;
+        }
+      }
+      uint64_t ev = evs[t];
+      const R beta = (R)((double)(T - t) / T);
+      const uint64_t acc = sweep_chain<R, KIND, RMAX>(A, chain, sig, lane, th, lc, tt, lt, ev,
+                                                      beta);
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) ths[st * m + j] = th[r];
+      }
+      // lp = a . sigma + sum_j logcosh(theta_j)
+      R sa = (R)0, sl = (R)0;
+      for (int i = lane; i < n; i += kWave) sa = sa + ar[i] * (R)sig[i];
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r)
+        if (lane + kWave * r < m) sl = sl + lc[r];
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) {
+        sa = sa + __shfl_xor(sa, s);
+        sl = sl + __shfl_xor(sl, s);
+      }
+      if (lane == 0) {
+        lpv[t] = sa + sl;
+        evs[t] = ev;
+        nac[t] += acc;
+      }
+      wave_sync();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // the two swap passes, pair by pair in index order
+      for (int pass = 0; pass < 2; ++pass)
+        for (int t = pass; t + 1 < T && (pass == 0 || t + 1 <= T - 2); t += 2) {
+          const u32x4 x = draw(A.seed, evs[t], (uint32_t)(g * T + t), kDomMix);
+          const double dbeta = (double)(T - t - 1) / T - (double)(T - t) / T;
+          const R dl = lpv[t] - lpv[t + 1];
+          const bool d = sizeof(R) == 4 ? accept_f((float)dbeta, (float)dl, u01_float(x.z))
+                                        : accept_d(dbeta, (double)dl, u01_double(x.z, x.w));
+          if (d) {
+            const int i0 = idx[t];
+            idx[t] = idx[t + 1];
+            idx[t + 1] = i0;
+            const R l0 = lpv[t];
+            lpv[t] = lpv[t + 1];
+            lpv[t + 1] = l0;
+          }
+        }
+    }
+    __syncthreads();
+    if (w == 0 && sw >= A.n_discard) {  // slot 0
+      const int st = idx[0];
+      const int8_t* sig = strips + st * npad;
+      const size_t row = (size_t)(sw - A.n_discard) * groups + g;
+      for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+      if (A.mixed) refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+      R* ot = static_cast<R*>(A.out_theta) + row * m;
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) ot[j] = A.mixed ? th[r] : ths[st * m + j];
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = w; t < T; t += NW) {
+    const int chain = g * T + t;
+    const int8_t* sig = strips + idx[t] * npad;
+    for (int i = lane; i < n; i += kWave) A.sigma[(size_t)chain * n + i] = sig[i];
+    if (lane == 0) {
+      A.ctr[chain] = evs[t];
+      A.acc[chain] += nac[t];
+    }
+  }
 }
 
 template <typename R, int KIND, int RMAX>
@@ -260,8 +423,39 @@ void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_it
   }
 }
 
+size_t pt_lds_bytes(const SampleArgs& a) {
+  const size_t rs = a.mixed ? sizeof(float) : sizeof(double);
+  return 448 + (((size_t)a.ntemps * a.npad + 15) & ~(size_t)15) + (size_t)a.ntemps * a.m * rs;
+}
+
+template <typename R, int KIND>
+void launch_pt(const SampleArgs& a, hipStream_t st) {
+  const dim3 grid(a.C / a.ntemps), block(kWave * (a.ntemps % 4 == 0 ? 4 : 2));
+  const size_t lds = pt_lds_bytes(a);
+  switch ((a.m + kWave - 1) / kWave) {
+    case 1: hipLaunchKernelGGL((k_sample_pt<R, KIND, 1>), grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_sample_pt<R, KIND, 2>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_sample_pt<R, KIND, 3>), grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_sample_pt<R, KIND, 4>), grid, block, lds, st, a); break;
+    case 5: hipLaunchKernelGGL((k_sample_pt<R, KIND, 5>), grid, block, lds, st, a); break;
+    case 6: hipLaunchKernelGGL((k_sample_pt<R, KIND, 6>), grid, block, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((k_sample_pt<R, KIND, 7>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_sample_pt<R, KIND, 8>), grid, block, lds, st, a); break;
+  }
+}
+
 void launch_sample(const SampleArgs& a, hipStream_t st) {
   ProfGuard g("sample", st);
+  if (a.ntemps > 1) {
+    if (a.mixed) {
+      if (a.kind == 1) launch_pt<float, 1>(a, st);
+      else launch_pt<float, 0>(a, st);
+    } else {
+      if (a.kind == 1) launch_pt<double, 1>(a, st);
+      else launch_pt<double, 0>(a, st);
+    }
+    return;
+  }
   if (a.mixed) {
     if (a.kind == 1) launch_r<float, 1>(a, st);
     else launch_r<float, 0>(a, st);

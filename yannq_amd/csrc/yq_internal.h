@@ -69,8 +69,11 @@ struct SampleArgs {
   int C, n, m, nprop, n_sweeps, n_discard, npad, kind, mixed;
   double beta;
   uint64_t seed;
+  int ntemps;  // parallel tempering: C = ntemps x chain groups (1: plain chains)
 };
 void launch_sample(const SampleArgs& a, hipStream_t st);
+// LDS of the parallel-tempering launch (spin strips, exchange areas)
+size_t pt_lds_bytes(const SampleArgs& a);
 // Many machines in one persistent launch: inst_dev holds one SampleArgs per
 // machine (device memory), items the (machine, chain) pairs in decreasing
 // cost order; all machines share the sweeper kind and precision.

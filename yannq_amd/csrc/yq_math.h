@@ -71,6 +71,7 @@ enum Domain : uint32_t {
   kDomParams = 2u,     // parameter initialisation
   kDomUniform = 3u,    // uniform configurations for batch inputs
   kDomExact = 4u,      // draws of the exact (inverse-CDF) sampler
+  kDomMix = 5u,        // replica swaps of parallel tempering
 };
 
 YQ_HD inline u32x4 draw(uint64_t seed, uint64_t event, uint32_t stream,
