@@ -361,6 +361,41 @@ YQ_API int yq_ff_jacobian_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
 YQ_API int yq_ff_vjp_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
                          const void* d_act, const void* d_r, void* d_grad);
 
+/* ---- AmplitudePhase machine: Machines/AmplitudePhase.hpp ---------------
+ * psi(sigma) = sqrt(RBM(sigma)) exp(i pi FF(sigma)) (:127-138): the context's
+ * real RBM (yq_rbm_create, YQ_DTYPE_F64) is the amplitude and its
+ * FeedForward machine (yq_ff_create, dtype 1, same n) the phase; the
+ * reference builds the same pair in its constructor (:37-40).  While enabled
+ * the context's calls act on psi:
+ *  - yq_sample draws from |psi|^2 = RBM(sigma), i.e. Metropolis on the RBM at
+ *    beta = 1/2 (logRatioRe = logRatio / 2, States/AmplitudePhaseState.hpp:
+ *    52-55); parallel tempering is not available;
+ *  - yq_exact_batch / yq_exact_sample weight by RBM(sigma) / sum;
+ *  - yq_eloc returns complex local energies (ns (re, im) pairs), the ratios
+ *    being exp(logRatio / 2 + i pi (FF(sigma') - FF(sigma))) (:57-120);
+ *  - yq_sr_build (YQ_SOLVE_EXPLICIT or YQ_SOLVE_CG, YQ_FP64) builds the real
+ *    SR system of the concatenated parameters [RBM | FF] (:64-77, 153-158):
+ *    rows O = [O_RBM / 2 | pi dFF/dp] (:42-58; the phase block multiplies i),
+ *    S = Re(<O^* O^T> - <O^*><O^T>), which is block-diagonal (the blocks are
+ *    GroundState/NGDExact.hpp:172-184's corrMatAmp and corrMatPhase), and
+ *    F = Re(<O^* E_loc> - <O^*><E_loc>) = [cov(O_amp, Re E_loc) |
+ *    cov(O_phase, Im E_loc)], half of NGDExact::energyGrad (:137-170);
+ *    yq_sr_get / yq_sr_apply / yq_sr_solve act on that system (vectors of
+ *    dim_amp + dim_phase doubles; the explicit solve factors the two blocks
+ *    separately, as RunAmplitudePhase.hpp:96-111 does);
+ *  - yq_sr_step updates both halves.
+ * yq_rbm_create or yq_ff_create disables it. */
+YQ_API int yq_ap_enable(yq_ctx* ctx, int on);
+/* getDimAmp / getDimPhase (:60-68) */
+YQ_API int yq_ap_dims(const yq_ctx* ctx, int* dim_amp, int* dim_phase);
+/* parameters [RBM | FF] (dim_amp + dim_phase doubles); updateParams :153-158 */
+YQ_API int yq_ap_get_params(yq_ctx* ctx, double* params);
+YQ_API int yq_ap_set_params(yq_ctx* ctx, const double* params);
+YQ_API int yq_ap_update_params(yq_ctx* ctx, const double* delta);
+/* log coeff(sigma) = (log RBM(sigma) / 2, pi FF(sigma)) per configuration:
+ * out holds ns (re, im) pairs (:127-138) */
+YQ_API int yq_ap_log_psi(yq_ctx* ctx, int ns, const int8_t* sigma, double* out);
+
 /* ---- instrumentation (no reference counterpart) ------------------------- */
 /* Time every launch of the named kernels with HIP events on the launch's
  * stream.  labels: comma-separated kernel labels (e.g. "sample,bwd_mfma"),

@@ -96,6 +96,11 @@ def lib():
             "rp_logcosh_c_d": (None, [d, d, p]),
             "rp_logcosh_c_f": (None, [C.c_float, C.c_float, p]),
             "rp_tanh_c_d": (None, [d, d, p]),
+            "or_ap_log_psi": (None, [i, i, p, p, p, p, p, p, p]),
+            "or_ap_log_ratio": (None, [i, i, p, p, p, p, p, p, i, i, p]),
+            "or_ap_eloc": (None, [p, i, i, p, p, p, p, p, p, p]),
+            "or_ap_logderiv": (None, [i, i, i, p, p, p, p, p]),
+            "or_ap_ngd_exact": (i, [p, i, i, i, p, p, p, p, i, p, p, p, p, p, p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -507,3 +512,79 @@ def c_sample_replay(rbm, sigma, *, seed, n_sweeps, n_discard, kind=LOCAL, precis
     if rc:
         raise ValueError("bad replay configuration")
     return sigma, ctr, acc, out_s, out_t
+
+
+# ---------------------------------------------------------------- AmplitudePhase
+# psi = sqrt(RBM) exp(i pi FF): a real RBM (oracle.RBM) as the amplitude and
+# the FeedForward network (FFDesc, params) as the phase.
+def ap_log_psi(rbm, desc, ffp, sigma):
+    """log coeff = log RBM coeff / 2 + i pi FF(sigma), per configuration."""
+    sigma, ffp = i8(np.atleast_2d(sigma)), f64(ffp)
+    L = lib()
+    out = np.empty((sigma.shape[0], 2))
+    for s in range(sigma.shape[0]):
+        L.or_ap_log_psi(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(rbm.b), C.byref(desc),
+                        _p(ffp), _p(sigma[s]), _p(out[s]))
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def ap_log_ratio(rbm, desc, ffp, sigma, th, k, l=-1):
+    sigma, th, ffp = i8(sigma), f64(th), f64(ffp)
+    out = np.empty(2)
+    lib().or_ap_log_ratio(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), C.byref(desc), _p(ffp),
+                          _p(sigma), _p(th), k, l, _p(out))
+    return out[0] + 1j * out[1]
+
+
+def ap_eloc(H, rbm, desc, ffp, sigma, th):
+    sigma, th, ffp = i8(np.atleast_2d(sigma)), f64(np.atleast_2d(th)), f64(ffp)
+    L = lib()
+    out = np.empty((sigma.shape[0], 2))
+    for s in range(sigma.shape[0]):
+        L.or_ap_eloc(C.byref(H), rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), C.byref(desc), _p(ffp),
+                     _p(sigma[s]), _p(th[s]), _p(out[s]))
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def ap_logderiv(rbm, desc, ffp, sigma, th):
+    """Real rows [RBM O / 2 | pi dFF] (the phase block multiplies i)."""
+    sigma, th, ffp = i8(np.atleast_2d(sigma)), f64(np.atleast_2d(th)), f64(ffp)
+    L = lib()
+    out = np.empty((sigma.shape[0], rbm.dim + ff_npar(desc)))
+    for s in range(sigma.shape[0]):
+        L.or_ap_logderiv(rbm.n, rbm.m, int(rbm.use_bias), C.byref(desc), _p(ffp),
+                         _p(sigma[s]), _p(th[s]), _p(out[s]))
+    return out
+
+
+def ap_ngd_exact(H, rbm, desc, ffp, basis, with_S=True):
+    """NGDExact over a sorted basis (integers, bit i set <=> sigma_i = -1)."""
+    basis = np.ascontiguousarray(basis, dtype=np.uint32)
+    ffp = f64(ffp)
+    dA, dP = rbm.dim, ff_npar(desc)
+    E = C.c_double()
+    oa, op, g = np.empty(dA), np.empty(dP), np.empty(dA + dP)
+    Sa = np.empty((dA, dA)) if with_S else None
+    Sp = np.empty((dP, dP)) if with_S else None
+    rc = lib().or_ap_ngd_exact(C.byref(H), rbm.n, rbm.m, int(rbm.use_bias), _p(f64(rbm.params())),
+                               C.byref(desc), _p(ffp), _p(basis), len(basis), C.byref(E),
+                               _p(oa), _p(op), _p(g), _p(Sa), _p(Sp))
+    if rc:
+        raise ValueError("Hamiltonian leaves the basis")
+    # column-major outputs: transpose back (the blocks are symmetric anyway)
+    return {"E": E.value, "oloc_amp": oa, "oloc_phase": op, "grad": g,
+            "S_amp": None if Sa is None else Sa.T.copy(),
+            "S_phase": None if Sp is None else Sp.T.copy()}
+
+
+def basis_jz(n, nup):
+    """BasisJz order: the integers below 2^n with nup set bits, increasing."""
+    from itertools import combinations
+    out = sorted(sum(1 << i for i in c) for c in combinations(range(n), nup))
+    return np.array(out, dtype=np.uint32)
+
+
+def basis_to_sigma(basis, n):
+    b = np.asarray(basis, dtype=np.uint64)[:, None]
+    bits = (b >> np.arange(n, dtype=np.uint64)[None, :]) & 1
+    return (1 - 2 * bits).astype(np.int8)

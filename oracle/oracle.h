@@ -96,6 +96,28 @@ double or_ff_forward(const or_ff_desc* d, const double* params,
 double or_ff_jacobian(const or_ff_desc* d, const double* params,
                       const int8_t* sigma, double* der);
 
+// ---- AmplitudePhase: psi = sqrt(RBM) exp(i pi FF) with a real RBM [W | a | b]
+// (a, b zero without bias) and the FeedForward network above.
+// out2 = (Re, Im) of log coeff (Machines/AmplitudePhase.hpp:127-138)
+void or_ap_log_psi(int n, int m, const double* W, const double* a, const double* b,
+                   const or_ff_desc* d, const double* ffp, const int8_t* sigma, double* out2);
+// log psi(sigma^{k,l}) - log psi(sigma) (l < 0: k only), AmplitudePhaseState.hpp:57-120
+void or_ap_log_ratio(int n, int m, const double* W, const double* a, const or_ff_desc* d,
+                     const double* ffp, const int8_t* sigma, const double* theta, int k, int l,
+                     double* out2);
+// complex local energy (re, im)
+void or_ap_eloc(const or_ham* H, int n, int m, const double* W, const double* a,
+                const or_ff_desc* d, const double* ffp, const int8_t* sigma,
+                const double* theta, double* out2);
+// real rows [RBM logDeriv / 2 | pi FF backward] (AmplitudePhase.hpp:42-58)
+void or_ap_logderiv(int n, int m, int use_bias, const or_ff_desc* d, const double* ffp,
+                    const int8_t* sigma, const double* theta, double* out);
+// NGDExact::constructExact + corrMatAmp / corrMatPhase (NGDExact.hpp:137-184)
+int or_ap_ngd_exact(const or_ham* H, int n, int m, int use_bias, const double* rbm_params,
+                    const or_ff_desc* d, const double* ffp, const uint32_t* basis, int B,
+                    double* E, double* olocAmp, double* olocPhase, double* grad, double* Samp,
+                    double* Sphase);
+
 // ---- replay of the kernels' arithmetic -------------------------------------
 typedef struct {
   int n, m, use_bias;

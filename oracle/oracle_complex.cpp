@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../yannq_amd/csrc/yq_math.h"
+#include "functors.h"
 #include "oracle.h"
 
 namespace {
@@ -97,33 +98,7 @@ void or_c_eloc(const or_ham* H, int n, int m, const double* W, const double* a,
     or_c_log_ratio2(n, m, W, a, s, theta, k, l, r);
     return std::exp(cd(r[0], r[1]));
   };
-  cd e = 0.0;
-  if (H->kind == OR_TFI) {
-    for (int i = 0; i < n; ++i) {
-      e += H->J * (double)s[i] * (double)s[(i + 1) % n];
-      e += H->h * ratio1(i);
-    }
-  } else if (H->kind == OR_XXZ) {
-    const double sign = H->sign_rule ? -1.0 : 1.0;
-    for (int i = 0; i < n; ++i) {
-      const int zz = (int)s[i] * (int)s[(i + 1) % n];
-      e += H->J * H->delta * zz;
-      e += sign * H->J * (1 - zz) * ratio2(i, (i + 1) % n);
-    }
-  } else {
-    const int sign = H->sign_rule ? -1 : 1;
-    for (int i = 0; i < n; ++i) {
-      const double yy = -(double)((int)s[i] * (int)s[(i + 1) % n]);
-      e += -H->J * yy;
-      e += H->J * sign * (1.0 + yy) * ratio2(i, (i + 1) % n);
-    }
-    for (int i = 0; i < n; ++i) {
-      const double yy = -(double)((int)s[i] * (int)s[(i + 2) % n]);
-      e += -H->J2 * yy;
-      e += H->J2 * (1.0 + yy) * ratio2(i, (i + 2) % n);
-    }
-  }
-  put(out2, 0, e);
+  put(out2, 0, oracle_detail::eloc_functor(H, s, ratio1, ratio2));
 }
 
 // GroundState/utils.hpp:13-51 and SRMat.hpp:89-103, 142-149 (adjoint products)

@@ -114,6 +114,9 @@ def lib():
         "yq_rbm_save": [p, C.c_char_p, i, i], "yq_rbm_load": [p, C.c_char_p, i, i],
         "yq_exact_batch": [p, i, p], "yq_exact_probs": [p, p],
         "yq_exact_sample": [p, i, i, u64, p],
+        "yq_ap_enable": [p, i], "yq_ap_dims": [p, p, p], "yq_ap_get_params": [p, p],
+        "yq_ap_set_params": [p, p], "yq_ap_update_params": [p, p],
+        "yq_ap_log_psi": [p, i, p, p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -185,6 +188,7 @@ class Context:
         self.n = self.m = self.dim = 0
         self.chains = 0
         self.ns = 0
+        self.ap = False
 
     def close(self):
         if getattr(self, "_h", None):
@@ -215,6 +219,7 @@ class Context:
         self.dt = np.dtype(dtype)
         code = DTYPE_C128 if self.dt == np.complex128 else DTYPE_F64
         self._chk(self._L.yq_rbm_create(self._h, code, n, m, int(use_bias)))
+        self.ap = False
         self.n, self.m = n, m
         self.dim = n * m + (n + m if use_bias else 0)
 
@@ -315,8 +320,16 @@ class Context:
         self.ns = n_samples
         return idx
 
+    @property
+    def sr_dim(self):
+        """Length of the SR system's vectors ([RBM | FF] for AmplitudePhase)."""
+        return self.dim + self.ff_npar if self.ap else self.dim
+
+    def _eout(self, count):
+        return np.empty(count, np.complex128 if (self.cplx or self.ap) else np.float64)
+
     def eloc(self):
-        e = self._out(self.ns)
+        e = self._eout(self.ns)
         mean, var = C.c_double(), C.c_double()
         self._chk(self._L.yq_eloc(self._h, _ptr(e), C.byref(mean), C.byref(var)))
         return e, mean.value, var.value
@@ -326,24 +339,28 @@ class Context:
         self._chk(self._L.yq_sr_build(self._h, solver, precision))
 
     def sr_get(self, item):
-        size = {SR_F: self.dim, SR_OMEAN: self.dim, SR_O: self.ns * self.dim,
-                SR_S: self.dim * self.dim, SR_ELOC: self.ns}[item]
+        D = self.sr_dim
+        if item == SR_ELOC:
+            out = self._eout(self.ns)
+            self._chk(self._L.yq_sr_get(self._h, item, _ptr(out)))
+            return out
+        size = {SR_F: D, SR_OMEAN: D, SR_O: self.ns * D, SR_S: D * D}[item]
         out = self._out(size)
         self._chk(self._L.yq_sr_get(self._h, item, _ptr(out)))
         if item == SR_O:
-            return out.reshape(self.ns, self.dim)
+            return out.reshape(self.ns, D)
         if item == SR_S:  # column-major on the ABI
-            return out.reshape(self.dim, self.dim).T.copy()
+            return out.reshape(D, D).T.copy()
         return out
 
     def sr_apply(self, shift, v):
         v = self._vec(v)
-        out = self._out(self.dim)
+        out = self._out(self.sr_dim)
         self._chk(self._L.yq_sr_apply(self._h, shift, _ptr(v), _ptr(out)))
         return out
 
     def sr_solve(self, shift, tol=1e-10, max_iter=10000, rhs=None, allow_unconverged=False):
-        x = self._out(self.dim)
+        x = self._out(self.sr_dim)
         res, it = C.c_double(), C.c_int()
         r = None if rhs is None else self._vec(rhs)
         self._chk(self._L.yq_sr_solve(self._h, shift, tol, max_iter, _ptr(r), _ptr(x),
@@ -355,6 +372,7 @@ class Context:
     def ff_create(self, n, channels, kernel, hidden, dtype=np.float32):
         d = FFDesc(n, channels, kernel, hidden, 1 if np.dtype(dtype) == np.float64 else 0)
         self._chk(self._L.yq_ff_create(self._h, C.byref(d)))
+        self.ap = False
         k = C.c_int()
         self._chk(self._L.yq_ff_npar(self._h, C.byref(k)))
         self.ff_n, self.ff_npar = n, k.value
@@ -396,6 +414,38 @@ class Context:
 
     def ff_vjp_dev(self, ns, d_sigma, d_act, d_r, d_grad):
         self._chk(self._L.yq_ff_vjp_dev(self._h, ns, d_sigma, d_act, d_r, d_grad))
+
+    # AmplitudePhase machine: this context's real RBM (amplitude) and its
+    # FeedForward network (phase, float64)
+    def ap_enable(self, on=True):
+        self._chk(self._L.yq_ap_enable(self._h, int(on)))
+        self.ap = bool(on)
+
+    def ap_dims(self):
+        a, p = C.c_int(), C.c_int()
+        self._chk(self._L.yq_ap_dims(self._h, C.byref(a), C.byref(p)))
+        return a.value, p.value
+
+    def ap_get_params(self):
+        out = np.empty(self.sr_dim)
+        self._chk(self._L.yq_ap_get_params(self._h, _ptr(out)))
+        return out
+
+    def ap_set_params(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.sr_dim,)
+        self._chk(self._L.yq_ap_set_params(self._h, _ptr(v)))
+
+    def ap_update_params(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        assert d.shape == (self.sr_dim,)
+        self._chk(self._L.yq_ap_update_params(self._h, _ptr(d)))
+
+    def ap_log_psi(self, sigma):
+        sigma = np.ascontiguousarray(np.atleast_2d(sigma), dtype=np.int8)
+        out = np.empty(sigma.shape[0], np.complex128)
+        self._chk(self._L.yq_ap_log_psi(self._h, sigma.shape[0], _ptr(sigma), _ptr(out)))
+        return out
 
     # many machines in one context
     def multi_create(self, machines, sweeper=LOCAL, precision=MIXED):

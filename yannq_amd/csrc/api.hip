@@ -138,12 +138,19 @@ struct yq_ctx {
   std::vector<double> ff_host;  // fp64 master parameters
   Buf ffp, ff_sig, ff_out, ff_jac, ff_part, ff_grad;
 
+  // AmplitudePhase machine (yq_ap_enable): the real RBM above is the
+  // amplitude, the FeedForward machine the phase; ap_ffs / ap_ffb hold the
+  // phase network's outputs on the batch and on its E_loc neighbours
+  bool ap = false;
+  Buf ap_ffs, ap_ffb;
+
   yq_ctx() {
     for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp,
                    &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
-                   &xprev, &ones, &cpd, &cpf, &m_sargs, &m_items, &m_eargs, &m_stats})
+                   &xprev, &ones, &cpd, &cpf, &m_sargs, &m_items, &m_eargs, &m_stats, &ap_ffs,
+                   &ap_ffb})
       bufs.push_back(b);
   }
 };
@@ -211,6 +218,11 @@ int sync(yq_ctx* c) {
 // doubles per parameter (2 for a complex machine) and vector lengths in doubles
 size_t ncomp(const yq_ctx* c) { return c->cplx ? 2 : 1; }
 size_t vlen(const yq_ctx* c) { return c->dim * ncomp(c); }
+// doubles per local energy (complex for a complex or an AmplitudePhase machine)
+size_t ecomp(const yq_ctx* c) { return (c->cplx || c->ap) ? 2 : 1; }
+// length of the SR system's vectors: the AmplitudePhase machine's parameters
+// are [RBM | FeedForward] (AmplitudePhase.hpp:64-77)
+size_t srlen(const yq_ctx* c) { return c->ap ? c->dim + (size_t)c->ff_npar : vlen(c); }
 double* W64(yq_ctx* c) { return c->params.as<double>(); }
 double* a64(yq_ctx* c) { return c->params.as<double>() + ncomp(c) * c->n * c->m; }
 double* b64(yq_ctx* c) { return c->params.as<double>() + ncomp(c) * ((size_t)c->n * c->m + c->n); }
@@ -256,6 +268,9 @@ int dot(yq_ctx* c, const double* x, const double* y, size_t n, double* out) {
   return YQ_OK;
 }
 
+// FeedForward kernel arguments for the context's network (defined below)
+FFArgs ff_args(yq_ctx* c, int ns, const int8_t* d_sigma);
+
 // ---------------------------------------------------------------- E_loc
 // Arguments of the E_loc kernels for the current batch; allocates outputs.
 int eloc_args(yq_ctx* c, ElocArgs* out) {
@@ -263,8 +278,8 @@ int eloc_args(yq_ctx* c, ElocArgs* out) {
   if (!c->have_ham) return fail(c, YQ_ERR_STATE, "no Hamiltonian: call yq_ham_set");
   if (c->ham.n != c->n) return fail(c, YQ_ERR_ARG, "Hamiltonian size differs from the machine's n");
   const int B = num_bonds(c->ham);
-  TRY(ensure(c, c->ratio, sizeof(double) * ncomp(c) * (size_t)c->ns * B));
-  TRY(ensure(c, c->eloc, sizeof(double) * ncomp(c) * (size_t)c->ns));
+  TRY(ensure(c, c->ratio, sizeof(double) * ecomp(c) * (size_t)c->ns * B));
+  TRY(ensure(c, c->eloc, sizeof(double) * ecomp(c) * (size_t)c->ns));
   TRY(ensure(c, c->estat, sizeof(double) * 2));
   ElocArgs& a = *out;
   a = ElocArgs{};
@@ -293,7 +308,34 @@ int eloc_args(yq_ctx* c, ElocArgs* out) {
 int compute_eloc(yq_ctx* c) {
   ElocArgs a;
   TRY(eloc_args(c, &a));
-  if (c->cplx) {
+  if (c->ap) {
+    // AmplitudePhaseState.hpp:57-120: the phase network on the batch and on
+    // every (sample, term) neighbour, then the RBM ratio pass folds
+    // exp(logRatio / 2 + i pi dFF) into complex ratios for the functors
+    if (eloc_lds_bytes(a) > 64 * 1024) return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
+    const size_t pairs = (size_t)c->ns * a.B;
+    if (pairs > (size_t)INT32_MAX) return fail(c, YQ_ERR_ARG, "too many (sample, term) pairs");
+    TRY(ensure(c, c->ap_ffs, sizeof(double) * c->ns));
+    TRY(ensure(c, c->ap_ffb, sizeof(double) * pairs));
+    FFArgs f = ff_args(c, c->ns, c->bsig.as<int8_t>());
+    f.out = c->ap_ffs.p;
+    TRY(launch_ff(f, kFFForward, 1, c->st));
+    f.ns = (int)pairs;
+    f.out = c->ap_ffb.p;
+    f.flip_B = a.B;
+    f.flip_kind = c->ham.kind;
+    TRY(launch_ff(f, kFFForward, 1, c->st));
+    a.ap_ffs = c->ap_ffs.as<double>();
+    a.ap_ffb = c->ap_ffb.as<double>();
+    launch_eloc_ratio(a, c->st);
+    launch_eloc_combine_c(a, c->st);
+    LAUNCHCHK(c);
+    if (c->weighted)
+      launch_eloc_stats_w(c->eloc.as<double>(), c->bw.as<double>(), c->ns, 1,
+                          c->estat.as<double>(), c->st);
+    else
+      launch_eloc_stats_c(c->eloc.as<double>(), c->ns, c->estat.as<double>(), c->st);
+  } else if (c->cplx) {
     if (eloc_c_lds_bytes(a) > 64 * 1024)
       return fail(c, YQ_ERR_UNSUPPORTED, "E_loc tile exceeds LDS");
     launch_eloc_c(a, c->st);
@@ -441,10 +483,97 @@ int sr_build_c(yq_ctx* c, int solver) {
   return YQ_OK;
 }
 
+// The SR system of a real machine as independent diagonal blocks: one block
+// for an RBM; for an AmplitudePhase machine the amplitude and the phase
+// parameters, whose metric Re S has no cross block (NGDExact.hpp:172-184:
+// with O = [O_amp | i O_phase] the cross terms of <O^* O^T> are imaginary).
+// Block b: parameters [off, off + d), centred rows O (ns x d row-major, i.e.
+// d x ns column-major) and the lower triangle of its S (d x d) at S.
+struct SrBlock {
+  size_t off, d;
+  double* O;
+  double* S;
+};
+std::vector<SrBlock> sr_blocks(yq_ctx* c) {
+  if (!c->ap) return {SrBlock{0, c->dim, c->O.as<double>(), c->S.as<double>()}};
+  const size_t dA = c->dim, dP = (size_t)c->ff_npar;
+  return {SrBlock{0, dA, c->O.as<double>(), c->S.as<double>()},
+          SrBlock{dA, dP, c->O.as<double>() + (size_t)c->ns * dA,
+                  c->S.p ? c->S.as<double>() + dA * dA : nullptr}};
+}
+
+// AmplitudePhase machine, FP64: rows [O_RBM / 2 | pi dFF] (AmplitudePhase.hpp:
+// 42-58; the phase block multiplies i), F = Re(<O^* E_loc> - <O^*><E_loc>) =
+// [cov(O_amp, Re E_loc) | cov(O_phase, Im E_loc)], S = Re(S) block-diagonal
+// (the blocks of NGDExact::corrMatAmp / corrMatPhase, NGDExact.hpp:172-184).
+int sr_build_ap(yq_ctx* c, int solver, int precision) {
+  if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG)
+    return fail(c, YQ_ERR_UNSUPPORTED,
+                "AmplitudePhase machines use the explicit or the dense CG solver");
+  if (precision != YQ_FP64)
+    return fail(c, YQ_ERR_UNSUPPORTED, "AmplitudePhase SR runs in FP64");
+  const int ns = c->ns, n = c->n, m = c->m;
+  const size_t dA = c->dim, dP = (size_t)c->ff_npar, D = dA + dP;
+  TRY(ensure(c, c->omean, sizeof(double) * D));
+  TRY(ensure(c, c->F, sizeof(double) * D));
+  TRY(ensure(c, c->diagS, sizeof(double) * D));
+  TRY(ensure(c, c->T64, sizeof(double) * (size_t)ns * m));
+  TRY(ensure(c, c->O, sizeof(double) * (size_t)ns * D));
+  TRY(ensure(c, c->ecent, sizeof(double) * 2 * (size_t)ns));
+  TRY(ensure(c, c->ff_out, sizeof(double) * (size_t)ns));
+  const int chunks = colsum_chunks(ns);
+  TRY(ensure(c, c->partial, sizeof(double) * (size_t)chunks * std::max(dA, dP)));
+  if (solver == YQ_SOLVE_EXPLICIT) TRY(ensure(c, c->S, sizeof(double) * (dA * dA + dP * dP)));
+  if (c->bprec == YQ_MIXED) {
+    TRY(ensure(c, c->vtmp, sizeof(double) * (size_t)ns * m));
+    launch_float_to_double(c->btheta.as<float>(), c->vtmp.as<double>(), (size_t)ns * m, c->st);
+    launch_tanh(c->vtmp.p, c->T64.p, (size_t)ns * m, 0, c->st);
+  } else {
+    launch_tanh(c->btheta.p, c->T64.p, (size_t)ns * m, 0, c->st);
+  }
+  const std::vector<SrBlock> blk = sr_blocks(c);
+  // logDerivAmp = RBM logDeriv / 2, logDerivPhase = pi FF backward
+  launch_build_O(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias, blk[0].O,
+                 c->st);
+  launch_scale_add(blk[0].O, 0.5, blk[0].O, 0.0, (size_t)ns * dA, c->st);
+  FFArgs f = ff_args(c, ns, c->bsig.as<int8_t>());
+  f.jac = blk[1].O;
+  f.out = c->ff_out.p;
+  TRY(launch_ff(f, kFFJacobian, 1, c->st));
+  launch_scale_add(blk[1].O, M_PI, blk[1].O, 0.0, (size_t)ns * dP, c->st);
+  double* er = c->ecent.as<double>();
+  launch_split_complex(c->eloc.as<double>(), ns, er, er + ns, c->st);
+  LAUNCHCHK(c);
+  const double* w = c->weighted ? c->bw.as<double>() : nullptr;
+  const double* rs = c->weighted ? c->brs.as<double>() : nullptr;
+  for (size_t b = 0; b < blk.size(); ++b) {
+    const int d = (int)blk[b].d;
+    double* part = c->partial.as<double>();
+    launch_colsum(blk[b].O, ns, d, part, c->st, w);
+    launch_reduce_partials(part, chunks, d, w ? 1.0 : 1.0 / ns, c->omean.as<double>() + blk[b].off,
+                           c->st);
+    launch_center_force(blk[b].O, c->omean.as<double>() + blk[b].off, er + b * ns, ns, d, part,
+                        c->st, rs);
+    launch_reduce_partials(part, chunks, d, 1.0 / ns, c->F.as<double>() + blk[b].off, c->st);
+    // diag(S) = sum of the (row-scaled) centred squares / ns
+    launch_colsum(blk[b].O, ns, d, part, c->st, nullptr, 1);
+    launch_reduce_partials(part, chunks, d, 1.0 / ns, c->diagS.as<double>() + blk[b].off, c->st);
+    LAUNCHCHK(c);
+    if (solver == YQ_SOLVE_EXPLICIT)
+      TRY(syrk_samples(c, blk[b].O, blk[b].d, ns, 1.0 / ns, blk[b].S, false));
+  }
+  TRY(sync(c));
+  c->have_sr = true;
+  c->sr_solver = solver;
+  c->sr_prec = YQ_FP64;
+  return YQ_OK;
+}
+
 int sr_build(yq_ctx* c, int solver, int precision) {
   if (!c->have_eloc) TRY(compute_eloc(c));
   if (precision != YQ_FP64 && precision != YQ_MIXED)
     return fail(c, YQ_ERR_ARG, "unknown precision");
+  if (c->ap) return sr_build_ap(c, solver, precision);
   if (c->cplx) return sr_build_c(c, solver);
   const int ns = c->ns, n = c->n, m = c->m;
   const size_t dim = c->dim;
@@ -544,7 +673,20 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
   const size_t dim = c->dim;
   TRY(ensure(c, c->vy, sizeof(double) * ns));
   double* y = c->vy.as<double>();
-  if (c->cplx) {
+  if (c->ap) {
+    // block by block: y = A_b^T v_b / ns, out_b = A_b y + shift v_b
+    TRY(blas(c));
+    const double one = 1.0, zero = 0.0, inv = 1.0 / ns;
+    for (const SrBlock& b : sr_blocks(c)) {
+      const rocblas_int d = (rocblas_int)b.d;
+      BLASCHK(c, rocblas_dgemv(c->blas, rocblas_operation_transpose, d, ns, &inv, b.O, d,
+                               v + b.off, 1, &zero, y, 1));
+      HIPCHK(c, hipMemcpyAsync(out + b.off, v + b.off, sizeof(double) * b.d,
+                               hipMemcpyDeviceToDevice, c->st));
+      BLASCHK(c, rocblas_dgemv(c->blas, rocblas_operation_none, d, ns, &one, b.O, d, y, 1,
+                               &shift, out + b.off, 1));
+    }
+  } else if (c->cplx) {
     // y = A^H v / ns = O_c v / ns, out = A y + shift v = O_c^H y + shift v
     TRY(ensure(c, c->vy, sizeof(double) * 2 * ns));
     TRY(blas(c));
@@ -607,7 +749,7 @@ int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int ma
          int* iters, double* err, bool x0) {
   // complex vectors run as 2*dim doubles: every dot here is Re(a^H b), which
   // is what CG on a Hermitian operator needs (p^H A p and r^H z are real)
-  const size_t dim = vlen(c);
+  const size_t dim = srlen(c);
   TRY(ensure(c, c->vr, sizeof(double) * dim));
   TRY(ensure(c, c->vp, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
@@ -672,9 +814,20 @@ int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int ma
   return rn2 < thr ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
+// out = (S + shift) x with the explicit real S, block by block (device pointers)
+int symv_blocks(yq_ctx* c, double shift, const double* x, double* out) {
+  TRY(blas(c));
+  const double one = 1.0;
+  HIPCHK(c, hipMemcpyAsync(out, x, sizeof(double) * srlen(c), hipMemcpyDeviceToDevice, c->st));
+  for (const SrBlock& b : sr_blocks(c))
+    BLASCHK(c, rocblas_dsymv(c->blas, rocblas_fill_lower, (rocblas_int)b.d, &one, b.S,
+                             (rocblas_int)b.d, x + b.off, 1, &shift, out + b.off, 1));
+  return YQ_OK;
+}
+
 // |(S + shift) x - b| / |b| in FP64
 int residual64(yq_ctx* c, double shift, const double* x, const double* b, double* out) {
-  const size_t dim = vlen(c);
+  const size_t dim = srlen(c);
   TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)c->ns * c->m)));
   double* t = c->vtmp.as<double>();
   if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->cplx) {
@@ -687,11 +840,7 @@ int residual64(yq_ctx* c, double shift, const double* x, const double* b, double
                              reinterpret_cast<const rocblas_double_complex*>(x), 1, &sh,
                              reinterpret_cast<rocblas_double_complex*>(t), 1));
   } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
-    TRY(blas(c));
-    HIPCHK(c, hipMemcpyAsync(t, x, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
-    const double one = 1.0;
-    BLASCHK(c, rocblas_dsymv(c->blas, rocblas_fill_lower, (rocblas_int)dim, &one,
-                             c->S.as<double>(), (rocblas_int)dim, x, 1, &shift, t, 1));
+    TRY(symv_blocks(c, shift, x, t));
   } else {
     TRY(apply64(c, shift, x, t));
   }
@@ -730,29 +879,40 @@ int solve_explicit_c(yq_ctx* c, double shift, const double* b, double* x) {
   return YQ_OK;
 }
 
+// Cholesky of S + shift I and its solve, block by block (one block unless
+// the machine is AmplitudePhase; its blocks are the separate amplitude and
+// phase solves of RunAmplitudePhase.hpp:96-111)
 int solve_explicit(yq_ctx* c, double shift, const double* b, double* x) {
   if (c->cplx) return solve_explicit_c(c, shift, b, x);
-  const size_t dim = c->dim;
+  const std::vector<SrBlock> blk = sr_blocks(c);
+  size_t msize = 0;
+  for (const SrBlock& k : blk) msize += k.d * k.d;
   TRY(blas(c));
-  TRY(ensure(c, c->M, sizeof(double) * dim * dim));
-  TRY(ensure(c, c->info, sizeof(rocblas_int)));
-  launch_copy_lower_shift(c->S.as<double>(), c->M.as<double>(), (int)dim, shift, c->st);
-  LAUNCHCHK(c);
-  {
-    ProfGuard g("dpotrf", c->st);
-    BLASCHK(c, rocsolver_dpotrf(c->blas, rocblas_fill_lower, (rocblas_int)dim,
-                                c->M.as<double>(), (rocblas_int)dim,
-                                c->info.as<rocblas_int>()));
+  TRY(ensure(c, c->M, sizeof(double) * msize));
+  TRY(ensure(c, c->info, sizeof(rocblas_int) * blk.size()));
+  HIPCHK(c, hipMemcpyAsync(x, b, sizeof(double) * srlen(c), hipMemcpyDeviceToDevice, c->st));
+  double* M = c->M.as<double>();
+  for (size_t q = 0; q < blk.size(); ++q) {
+    const rocblas_int d = (rocblas_int)blk[q].d;
+    launch_copy_lower_shift(blk[q].S, M, d, shift, c->st);
+    LAUNCHCHK(c);
+    {
+      ProfGuard g("dpotrf", c->st);
+      BLASCHK(c, rocsolver_dpotrf(c->blas, rocblas_fill_lower, d, M, d,
+                                  c->info.as<rocblas_int>() + q));
+    }
+    BLASCHK(c, rocsolver_dpotrs(c->blas, rocblas_fill_lower, d, 1, M, d, x + blk[q].off, d));
+    M += blk[q].d * blk[q].d;
   }
-  rocblas_int info = 0;
-  HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  std::vector<rocblas_int> info(blk.size(), 0);
+  HIPCHK(c, hipMemcpyAsync(info.data(), c->info.p, sizeof(rocblas_int) * blk.size(),
+                           hipMemcpyDeviceToHost, c->st));
   TRY(sync(c));
-  if (info != 0)
-    return fail(c, YQ_ERR_NOT_SPD,
-                "Cholesky of S + shift I failed at pivot " + std::to_string(info));
-  HIPCHK(c, hipMemcpyAsync(x, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
-  BLASCHK(c, rocsolver_dpotrs(c->blas, rocblas_fill_lower, (rocblas_int)dim, 1,
-                              c->M.as<double>(), (rocblas_int)dim, x, (rocblas_int)dim));
+  for (size_t q = 0; q < blk.size(); ++q)
+    if (info[q] != 0)
+      return fail(c, YQ_ERR_NOT_SPD,
+                  "Cholesky of S + shift I failed at pivot " +
+                      std::to_string(info[q] + (int)blk[q].off));
   return YQ_OK;
 }
 
@@ -827,7 +987,7 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
     // warm start from the previous iterative solution (same dimension)
     const bool x0 = c->warm_start && c->have_xprev;
     if (x0)
-      HIPCHK(c, hipMemcpyAsync(x_dev, c->xprev.p, sizeof(double) * vlen(c),
+      HIPCHK(c, hipMemcpyAsync(x_dev, c->xprev.p, sizeof(double) * srlen(c),
                                hipMemcpyDeviceToDevice, c->st));
     if (c->sr_prec == YQ_MIXED) {
       rc = solve_mixed(c, shift, b_dev, x_dev, tol, max_iter, &it, &err, x0);
@@ -839,8 +999,8 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
       TRY(residual64(c, shift, x_dev, b_dev, &e2));
       err = e2;
     }
-    TRY(ensure(c, c->xprev, sizeof(double) * vlen(c)));
-    HIPCHK(c, hipMemcpyAsync(c->xprev.p, x_dev, sizeof(double) * vlen(c),
+    TRY(ensure(c, c->xprev, sizeof(double) * srlen(c)));
+    HIPCHK(c, hipMemcpyAsync(c->xprev.p, x_dev, sizeof(double) * srlen(c),
                              hipMemcpyDeviceToDevice, c->st));
     c->have_xprev = true;
   }
@@ -919,6 +1079,7 @@ YQ_API int yq_rbm_create(yq_ctx* c, int dtype, int n, int m, int use_bias) {
   c->m = m;
   c->use_bias = use_bias ? 1 : 0;
   c->cplx = dtype == YQ_DTYPE_C128;
+  c->ap = false;
   c->dim = (size_t)n * m + (use_bias ? (size_t)n + m : 0);
   const size_t np = ((size_t)n * m + n + m) * ncomp(c);
   TRY(ensure(c, c->params, sizeof(double) * np));
@@ -1096,8 +1257,9 @@ YQ_API int yq_sampler_config(yq_ctx* c, int sweeper, int n_temps, int n_chains_p
   // SamplerMT.hpp:47: nTmps even or 1; one block of nTmps wavefronts per chain group
   if (n_temps < 1 || n_temps > 16 || (n_temps > 1 && n_temps % 2))
     return fail(c, YQ_ERR_ARG, "n_temps must be 1 or even and <= 16");
-  if (n_temps > 1 && c->cplx)
-    return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering of complex machines");
+  if (n_temps > 1 && (c->cplx || c->ap))
+    return fail(c, YQ_ERR_UNSUPPORTED,
+                "parallel tempering of complex and AmplitudePhase machines");
   if (n_chains_per < 1 || sweep_factor < 1) return fail(c, YQ_ERR_ARG, "chains and sweep factor >= 1");
   if ((long long)n_temps * n_chains_per > (1 << 24)) return fail(c, YQ_ERR_ARG, "too many chains");
   if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
@@ -1214,7 +1376,9 @@ int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   a.npad = (c->n + 15) & ~15;
   a.kind = c->skind;
   a.mixed = c->sprec == YQ_MIXED;
-  a.beta = 1.0;
+  // an AmplitudePhase machine samples |sqrt(RBM)|^2 = RBM: the RBM chains at
+  // beta = 1/2 (logRatioRe = logRatio / 2, AmplitudePhaseState.hpp:52-55)
+  a.beta = c->ap ? 0.5 : 1.0;
   a.ntemps = c->ntemps;
   if (a.ntemps > 1) {
     int dev = 0, lds_max = 0;
@@ -1347,7 +1511,7 @@ int exact_prepare(yq_ctx* c, int nup) {
   launch_exact_weights(c->bsig.as<int8_t>(), c->btheta.as<double>(),
                        c->use_bias ? a64(c) : nullptr, B, c->n, c->m, (int)c->cplx,
                        c->blogpsi.as<double>(), c->bw.as<double>(), c->brs.as<double>(),
-                       c->bpart.as<double>(), c->st);
+                       c->bpart.as<double>(), c->st, c->ap ? 1.0 : 2.0);
   LAUNCHCHK(c);
   c->ns = B;
   c->bprec = YQ_FP64;
@@ -1411,7 +1575,7 @@ YQ_API int yq_eloc(yq_ctx* c, double* eloc, double* mean, double* var) {
   TRY(need_rbm(c));
   TRY(compute_eloc(c));
   if (eloc)
-    HIPCHK(c, hipMemcpyAsync(eloc, c->eloc.p, sizeof(double) * ncomp(c) * c->ns,
+    HIPCHK(c, hipMemcpyAsync(eloc, c->eloc.p, sizeof(double) * ecomp(c) * c->ns,
                              hipMemcpyDeviceToHost, c->st));
   if (mean) *mean = c->e_mean;
   if (var) *var = c->e_var;
@@ -1430,12 +1594,54 @@ YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {
   if (!out) return fail(c, YQ_ERR_ARG, "null output");
   if (item == YQ_SR_ELOC) {
     if (!c->have_eloc) return fail(c, YQ_ERR_STATE, "no local energies");
-    HIPCHK(c, hipMemcpyAsync(out, c->eloc.p, sizeof(double) * ncomp(c) * c->ns,
+    HIPCHK(c, hipMemcpyAsync(out, c->eloc.p, sizeof(double) * ecomp(c) * c->ns,
                              hipMemcpyDeviceToHost, c->st));
     return sync(c);
   }
   if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
   const size_t dim = c->dim;
+  if (c->ap) {
+    // the device keeps the amplitude and phase blocks apart; the host gets
+    // rows [O_amp | O_phase] and the full (block-diagonal) S
+    const size_t D = srlen(c), ns = (size_t)c->ns;
+    const std::vector<SrBlock> blk = sr_blocks(c);
+    switch (item) {
+      case YQ_SR_F:
+      case YQ_SR_OMEAN:
+        HIPCHK(c, hipMemcpyAsync(out, (item == YQ_SR_F ? c->F : c->omean).p, sizeof(double) * D,
+                                 hipMemcpyDeviceToHost, c->st));
+        return sync(c);
+      case YQ_SR_O: {
+        std::vector<double> h(ns * D);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->O.p, sizeof(double) * h.size(),
+                                 hipMemcpyDeviceToHost, c->st));
+        TRY(sync(c));
+        for (const SrBlock& b : blk) {
+          const double* src = h.data() + ns * b.off;
+          for (size_t r = 0; r < ns; ++r)
+            std::copy(src + r * b.d, src + (r + 1) * b.d, out + r * D + b.off);
+        }
+        return YQ_OK;
+      }
+      case YQ_SR_S: {
+        if (c->sr_solver != YQ_SOLVE_EXPLICIT)
+          return fail(c, YQ_ERR_STATE, "S exists only for the explicit solver");
+        std::fill(out, out + D * D, 0.0);
+        for (const SrBlock& b : blk) {
+          std::vector<double> h(b.d * b.d);
+          HIPCHK(c, hipMemcpyAsync(h.data(), b.S, sizeof(double) * h.size(),
+                                   hipMemcpyDeviceToHost, c->st));
+          TRY(sync(c));
+          for (size_t j = 0; j < b.d; ++j)
+            for (size_t i = 0; i < b.d; ++i)  // lower triangle, mirrored
+              out[(b.off + j) * D + b.off + i] = i >= j ? h[j * b.d + i] : h[i * b.d + j];
+        }
+        return YQ_OK;
+      }
+      default:
+        return fail(c, YQ_ERR_ARG, "unknown item");
+    }
+  }
   if (c->cplx) {
     // the device holds conj(O_c), conj(omean) and the lower triangle of S
     // (column-major); the host gets O_c, omean and the full S (column-major)
@@ -1503,7 +1709,7 @@ YQ_API int yq_sr_apply(yq_ctx* c, double shift, const double* v, double* out) {
   TRY(need_rbm(c));
   if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
   if (!v || !out) return fail(c, YQ_ERR_ARG, "null vector");
-  const size_t dim = vlen(c);
+  const size_t dim = srlen(c);
   TRY(ensure(c, c->vx, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
   HIPCHK(c, hipMemcpyAsync(c->vx.p, v, sizeof(double) * dim, hipMemcpyHostToDevice, c->st));
@@ -1518,12 +1724,7 @@ YQ_API int yq_sr_apply(yq_ctx* c, double shift, const double* v, double* out) {
                              reinterpret_cast<const rocblas_double_complex*>(c->vx.p), 1, &sh,
                              reinterpret_cast<rocblas_double_complex*>(c->vAp.p), 1));
   } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
-    TRY(blas(c));
-    const double one = 1.0;
-    HIPCHK(c, hipMemcpyAsync(c->vAp.p, c->vx.p, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
-    BLASCHK(c, rocblas_dsymv(c->blas, rocblas_fill_lower, (rocblas_int)dim, &one,
-                             c->S.as<double>(), (rocblas_int)dim, c->vx.as<double>(), 1, &shift,
-                             c->vAp.as<double>(), 1));
+    TRY(symv_blocks(c, shift, c->vx.as<double>(), c->vAp.as<double>()));
   } else if (c->sr_prec == YQ_MIXED) {
     const int rc = mixed_apply64(c->mixed, shift, c->omean.as<double>(), c->vx.as<double>(),
                                  c->vAp.as<double>(), c->st, c->err);
@@ -1539,7 +1740,7 @@ YQ_API int yq_sr_solve(yq_ctx* c, double shift, double tol, int max_iter, const 
                        double* x, double* resid, int* iters) {
   TRY(need_rbm(c));
   if (!c->have_sr) return fail(c, YQ_ERR_STATE, "call yq_sr_build first");
-  const size_t dim = vlen(c);
+  const size_t dim = srlen(c);
   TRY(ensure(c, c->vx, sizeof(double) * dim));
   TRY(ensure(c, c->vb, sizeof(double) * std::max(dim, (size_t)c->ns)));
   const double* b = c->F.as<double>();
@@ -1567,6 +1768,10 @@ size_t ff_rs(const yq_ctx* c) { return c->ff.dtype == 1 ? sizeof(double) : sizeo
 // upload the fp64 master parameters in the machine dtype
 int ff_upload(yq_ctx* c) {
   const size_t np = (size_t)c->ff_npar;
+  if (c->ap) {  // the phase of an AmplitudePhase machine moved
+    c->have_eloc = false;
+    c->have_sr = false;
+  }
   if (c->ff.dtype == 1) {
     HIPCHK(c, hipMemcpyAsync(c->ffp.p, c->ff_host.data(), sizeof(double) * np,
                              hipMemcpyHostToDevice, c->st));
@@ -1619,6 +1824,7 @@ YQ_API int yq_ff_create(yq_ctx* c, const yq_ff_desc* d) {
     return fail(c, YQ_ERR_ARG, "invalid FeedForward shape or dtype");
   const long long np = (long long)d->kernel * d->channels + (long long)d->channels * d->n * d->hidden;
   if (np > 96 * 256) return fail(c, YQ_ERR_UNSUPPORTED, "FeedForward limited to 24576 parameters");
+  c->ap = false;
   c->ff = *d;
   c->ff_npar = (int)np;
   c->ff_host.assign((size_t)np, 0.0);
@@ -1754,6 +1960,103 @@ YQ_API int yq_ff_jacobian(yq_ctx* c, int ns, const int8_t* sigma, double* jac, d
   return YQ_OK;
 }
 
+// ---------------------------------------------------------------- AmplitudePhase
+}  // extern "C"
+
+namespace {
+int need_ap(yq_ctx* c) {
+  if (!c) return YQ_ERR_ARG;
+  if (!c->ap) return fail(c, YQ_ERR_STATE, "no AmplitudePhase machine: call yq_ap_enable");
+  return YQ_OK;
+}
+void ap_reset(yq_ctx* c) {  // the wave function changed meaning: no batch carries over
+  c->have_batch = false;
+  c->have_eloc = false;
+  c->have_sr = false;
+  c->have_xprev = false;
+}
+}  // namespace
+
+extern "C" {
+
+YQ_API int yq_ap_enable(yq_ctx* c, int on) {
+  if (!c) return YQ_ERR_ARG;
+  if (!on) {
+    if (c->ap) ap_reset(c);
+    c->ap = false;
+    return YQ_OK;
+  }
+  TRY(need_rbm(c));
+  if (c->cplx)
+    return fail(c, YQ_ERR_UNSUPPORTED, "the amplitude is a real RBM (YQ_DTYPE_F64)");
+  TRY(need_ff(c));
+  if (c->ff.dtype != 1)
+    return fail(c, YQ_ERR_UNSUPPORTED, "the phase network must be double (dtype 1)");
+  if (c->ff.n != c->n)
+    return fail(c, YQ_ERR_ARG, "amplitude and phase act on different numbers of spins");
+  if (c->have_sampler && c->ntemps > 1)
+    return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering of AmplitudePhase machines");
+  ap_reset(c);
+  c->ap = true;
+  return YQ_OK;
+}
+
+YQ_API int yq_ap_dims(const yq_ctx* c, int* dim_amp, int* dim_phase) {
+  if (!c || !c->ap) return YQ_ERR_STATE;
+  if (dim_amp) *dim_amp = (int)c->dim;
+  if (dim_phase) *dim_phase = c->ff_npar;
+  return YQ_OK;
+}
+
+YQ_API int yq_ap_get_params(yq_ctx* c, double* p) {
+  TRY(need_ap(c));
+  if (!p) return fail(c, YQ_ERR_ARG, "null output");
+  TRY(yq_rbm_get_params(c, p));
+  return yq_ff_get_params(c, p + c->dim);
+}
+
+YQ_API int yq_ap_set_params(yq_ctx* c, const double* p) {
+  TRY(need_ap(c));
+  if (!p) return fail(c, YQ_ERR_ARG, "null params");
+  for (size_t q = 0; q < srlen(c); ++q)
+    if (!std::isfinite(p[q])) return fail(c, YQ_ERR_NAN, "non-finite parameter");
+  TRY(yq_rbm_set_params(c, p));
+  return yq_ff_set_params(c, p + c->dim);
+}
+
+YQ_API int yq_ap_update_params(yq_ctx* c, const double* delta) {
+  TRY(need_ap(c));
+  if (!delta) return fail(c, YQ_ERR_ARG, "null delta");
+  TRY(yq_rbm_update_params(c, delta));
+  return yq_ff_update_params(c, delta + c->dim);
+}
+
+YQ_API int yq_ap_log_psi(yq_ctx* c, int ns, const int8_t* sigma, double* out) {
+  TRY(need_ap(c));
+  TRY(ff_check_sigma(c, ns, sigma));
+  if (!out) return fail(c, YQ_ERR_ARG, "null output");
+  const int n = c->n, m = c->m;
+  TRY(ensure(c, c->ff_sig, (size_t)ns * n));
+  TRY(ensure(c, c->vtmp, sizeof(double) * (size_t)ns * m));
+  TRY(ensure(c, c->ff_out, sizeof(double) * (size_t)ns));
+  TRY(ensure(c, c->ap_ffs, sizeof(double) * (size_t)ns));
+  TRY(ensure(c, c->vy, sizeof(double) * 2 * (size_t)ns));
+  HIPCHK(c, hipMemcpyAsync(c->ff_sig.p, sigma, (size_t)ns * n, hipMemcpyHostToDevice, c->st));
+  const int8_t* ds = c->ff_sig.as<int8_t>();
+  launch_theta_batch(W64(c), b64(c), ds, ns, n, m, c->vtmp.p, 0, c->st);
+  launch_log_psi(ds, c->vtmp.as<double>(), c->use_bias ? a64(c) : nullptr, ns, n, m, 0,
+                 c->ff_out.as<double>(), c->st);
+  FFArgs f = ff_args(c, ns, ds);
+  f.out = c->ap_ffs.p;
+  TRY(launch_ff(f, kFFForward, 1, c->st));
+  launch_ap_log_psi(c->ff_out.as<double>(), c->ap_ffs.as<double>(), ns, c->vy.as<double>(),
+                    c->st);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(out, c->vy.p, sizeof(double) * 2 * (size_t)ns, hipMemcpyDeviceToHost,
+                           c->st));
+  return sync(c);
+}
+
 // ---------------------------------------------------------------- profiling
 YQ_API int yq_prof_enable(yq_ctx* c, const char* labels) {
   if (!c) return YQ_ERR_ARG;
@@ -1860,18 +2163,27 @@ namespace {
 int step_update(yq_ctx* c, const yq_step_cfg* cfg, double* nv, double* err, int* it,
                 int* solve_rc) {
   TRY(sr_build(c, cfg->solver, cfg->precision));
-  TRY(ensure(c, c->vx, sizeof(double) * vlen(c)));
+  const size_t len = srlen(c);
+  TRY(ensure(c, c->vx, sizeof(double) * len));
   int rc = solve(c, cfg->lambda, cfg->cg_tol, cfg->cg_max_iter, c->F.as<double>(),
                  c->vx.as<double>(), err, it);
   if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
   *solve_rc = rc;
   double vv;
-  TRY(dot(c, c->vx.as<double>(), c->vx.as<double>(), vlen(c), &vv));
+  TRY(dot(c, c->vx.as<double>(), c->vx.as<double>(), len, &vv));
   *nv = sqrt(vv);
   double scale = -cfg->lr;
   if (cfg->grad_clip != 0.0 && *nv > cfg->grad_clip) scale *= cfg->grad_clip / *nv;
   launch_axpy(scale, c->vx.as<double>(), c->params.as<double>(), vlen(c), c->st);
   LAUNCHCHK(c);
+  if (c->ap) {  // AmplitudePhase::updateParams (:153-158): the tail moves the phase network
+    std::vector<double> tail((size_t)c->ff_npar);
+    HIPCHK(c, hipMemcpyAsync(tail.data(), c->vx.as<double>() + c->dim,
+                             sizeof(double) * tail.size(), hipMemcpyDeviceToHost, c->st));
+    TRY(sync(c));
+    for (size_t q = 0; q < tail.size(); ++q) c->ff_host[q] += scale * tail[q];
+    TRY(ff_upload(c));
+  }
   return refresh_params(c);
 }
 }  // namespace

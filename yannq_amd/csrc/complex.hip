@@ -246,19 +246,6 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------- E_loc
-__device__ inline void bond_sites_c(int kind, int n, int b, int& k, int& l) {
-  if (kind == YQ_TFI) {
-    k = b;
-    l = -1;
-  } else if (kind == YQ_XXZ || b < n) {
-    k = b;
-    l = (b + 1) % n;
-  } else {
-    k = b - n;
-    l = (b - n + 2) % n;
-  }
-}
-
 __device__ inline void lc_c(double x, double y, double* re, double* im) {
   logcosh_c_d(x, y, re, im);
 }
@@ -300,7 +287,7 @@ __global__ void __launch_bounds__(256) k_eloc_ratio_c(const ElocArgs A, int span
   const int ls = s - s_lo;
   const int8_t* sgs = sg + (size_t)ls * n;
   int k, l;
-  bond_sites_c(A.ham.kind, n, b, k, l);
+  bond_sites(A.ham.kind, n, b, k, l);
   double2* ratio = reinterpret_cast<double2*>(A.ratio);
   if (l >= 0 && sgs[k] == sgs[l]) {  // prefactor (1 - zz) vanishes
     ratio[idx] = make_double2(0.0, 0.0);
@@ -559,6 +546,11 @@ void launch_eloc_c(const ElocArgs& a, hipStream_t st) {
     else
       hipLaunchKernelGGL(k_eloc_ratio_c<double>, grid, block, lds, st, a, span);
   }
+  launch_eloc_combine_c(a, st);
+}
+
+void launch_eloc_combine_c(const ElocArgs& a, hipStream_t st) {
+  ProfGuard g("eloc_combine_c", st);
   hipLaunchKernelGGL(k_eloc_combine_c, dim3((a.ns + 255) / 256), dim3(256), 0, st, a);
 }
 

@@ -18,22 +18,6 @@ int num_bonds(const yq_ham_desc& h) { return h.kind == YQ_J1J2 ? 2 * h.n : h.n; 
 
 namespace {
 
-__device__ inline void bond_sites(int kind, int n, int b, int& k, int& l) {
-  if (kind == YQ_TFI) {
-    k = b;
-    l = -1;
-  } else if (kind == YQ_XXZ) {
-    k = b;
-    l = (b + 1) % n;
-  } else if (b < n) {
-    k = b;
-    l = (b + 1) % n;
-  } else {
-    k = b - n;
-    l = (b - n + 2) % n;
-  }
-}
-
 // One 256-thread block of (sample, bond) pairs starting at pair `first`.
 template <typename R>
 __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
@@ -64,7 +48,10 @@ __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
   int k, l;
   bond_sites(A.ham.kind, n, b, k, l);
   if (l >= 0 && sgs[k] == sgs[l]) {  // prefactor (1 - zz) vanishes
-    A.ratio[idx] = 0.0;
+    if (A.ap_ffs)
+      reinterpret_cast<double2*>(A.ratio)[idx] = make_double2(0.0, 0.0);
+    else
+      A.ratio[idx] = 0.0;
     return;
   }
   const R* WT = static_cast<const R*>(A.WT);
@@ -88,6 +75,15 @@ __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
       t = t - twol * WT[(size_t)j * n + l];
       res = res + (Math<R>::logcosh(t) - lcs[j]);
     }
+  }
+  if (A.ap_ffs) {
+    // AmplitudePhaseState.hpp:57-120: exp(logRatio / 2 + i pi (FF(flipped) - FF(sigma)))
+    const double mag = exp((double)res / 2.0);
+    const double ph = M_PI * (A.ap_ffb[idx] - A.ap_ffs[s]);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    reinterpret_cast<double2*>(A.ratio)[idx] = make_double2(mag * cs, mag * sn);
+    return;
   }
   A.ratio[idx] = (double)Math<R>::exp(res);
 }
@@ -188,19 +184,21 @@ __global__ void __launch_bounds__(1024) k_eloc_stats_multi(const ElocArgs* __res
 
 }  // namespace
 
-void launch_eloc(const ElocArgs& a, hipStream_t st) {
+void launch_eloc_ratio(const ElocArgs& a, hipStream_t st) {
   const size_t total = (size_t)a.ns * a.B;
   const int span = 256 / a.B + 2;
   const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
   const size_t lds = (size_t)span * a.m * 2 * rs + (size_t)span * a.n + 16;
   const dim3 grid((unsigned)((total + 255) / 256)), block(256);
-  {
-    ProfGuard g("eloc_ratio", st);
-    if (a.use_float)
-      hipLaunchKernelGGL(k_eloc_ratio<float>, grid, block, lds, st, a, span);
-    else
-      hipLaunchKernelGGL(k_eloc_ratio<double>, grid, block, lds, st, a, span);
-  }
+  ProfGuard g("eloc_ratio", st);
+  if (a.use_float)
+    hipLaunchKernelGGL(k_eloc_ratio<float>, grid, block, lds, st, a, span);
+  else
+    hipLaunchKernelGGL(k_eloc_ratio<double>, grid, block, lds, st, a, span);
+}
+
+void launch_eloc(const ElocArgs& a, hipStream_t st) {
+  launch_eloc_ratio(a, st);
   ProfGuard g("eloc_combine", st);
   hipLaunchKernelGGL(k_eloc_combine, dim3((a.ns + 255) / 256), dim3(256), 0, st, a);
 }

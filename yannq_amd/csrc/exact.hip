@@ -87,9 +87,11 @@ __global__ void __launch_bounds__(256) k_max_re(const double* __restrict__ lp, i
   const double r = block_reduce(v, true);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
-// w_s = exp(2 (Re log psi_s - max)); per-block sums of w
+// w_s = exp(pw (Re log psi_s - max)) (|psi|^2: pw = 2; an AmplitudePhase
+// machine's |sqrt(RBM)|^2 = RBM: pw = 1); per-block sums of w
 __global__ void __launch_bounds__(256) k_unnorm_weights(const double* __restrict__ lp, int B,
-                                                        int cplx, const double* __restrict__ mx,
+                                                        int cplx, double pw,
+                                                        const double* __restrict__ mx,
                                                         int nmx, double* __restrict__ w,
                                                         double* __restrict__ part) {
   __shared__ double top;
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(256) k_unnorm_weights(const double* __restrict
   __syncthreads();
   double acc = 0.0;
   for (int s = blockIdx.x * 256 + threadIdx.x; s < B; s += gridDim.x * 256) {
-    const double v = exp(2.0 * (lp[cplx ? 2 * (size_t)s : (size_t)s] - top));
+    const double v = exp(pw * (lp[cplx ? 2 * (size_t)s : (size_t)s] - top));
     w[s] = v;
     acc += v;
   }
@@ -230,14 +232,20 @@ void launch_basis_spins(const uint32_t* basis, int B, int n, int8_t* sigma, hipS
 
 void launch_exact_weights(const int8_t* sigma, const double* theta, const double* a, int B, int n,
                           int m, int cplx, double* logpsi, double* w, double* rs, double* part,
-                          hipStream_t st) {
+                          hipStream_t st, double pw) {
   hipLaunchKernelGGL(k_log_psi, dim3(blocks(B)), dim3(256), 0, st, sigma, theta, a, B, n, m, cplx,
                      logpsi);
   const int np = exact_partials(B);
   hipLaunchKernelGGL(k_max_re, dim3(np), dim3(256), 0, st, logpsi, B, cplx, part);
-  hipLaunchKernelGGL(k_unnorm_weights, dim3(np), dim3(256), 0, st, logpsi, B, cplx, part, np, w,
-                     part + np);
+  hipLaunchKernelGGL(k_unnorm_weights, dim3(np), dim3(256), 0, st, logpsi, B, cplx, pw, part, np,
+                     w, part + np);
   hipLaunchKernelGGL(k_normalize, dim3(np), dim3(256), 0, st, w, rs, B, part + np, np);
+}
+
+void launch_log_psi(const int8_t* sigma, const double* theta, const double* a, int B, int n, int m,
+                    int cplx, double* logpsi, hipStream_t st) {
+  hipLaunchKernelGGL(k_log_psi, dim3(blocks(B)), dim3(256), 0, st, sigma, theta, a, B, n, m, cplx,
+                     logpsi);
 }
 
 void launch_eloc_stats_w(const double* e, const double* w, int ns, int cplx, double* out,

@@ -60,8 +60,18 @@ __global__ void __launch_bounds__(256) k_ff(const FFArgs A) {
   for (int g = blockIdx.x; g < groups; g += gridDim.x) {
     const int s0 = g * kSB;
     const int cnt = min(kSB, A.ns - s0);
-    for (int q = tid; q < cnt * n; q += 256)
-      sg[q] = A.sigma[(size_t)s0 * n + q] > 0 ? (R)1 : (R)-1;
+    if (A.flip_B > 0) {
+      for (int q = tid; q < cnt * n; q += 256) {
+        const int cfg = s0 + q / n, i = q % n;
+        int k, l;
+        bond_sites(A.flip_kind, n, cfg % A.flip_B, k, l);
+        const bool up = A.sigma[(size_t)(cfg / A.flip_B) * n + i] > 0;
+        sg[q] = (up != (i == k || i == l)) ? (R)1 : (R)-1;
+      }
+    } else {
+      for (int q = tid; q < cnt * n; q += 256)
+        sg[q] = A.sigma[(size_t)s0 * n + q] > 0 ? (R)1 : (R)-1;
+    }
     if (MODE == kVjp)
       for (int q = tid; q < cnt; q += 256) rr[q] = static_cast<const R*>(A.r)[s0 + q];
     const R* act_in = static_cast<const R*>(A.act_in);

@@ -87,6 +87,22 @@ void launch_theta_batch(const double* W64, const double* b64,
                         int out_float, hipStream_t st);
 
 // ------------------------------------------------------------------ E_loc
+// Sites of off-diagonal term b in the order the E_loc functors visit them
+// (TFIsing.hpp:33-37 single flips; XXZ.hpp:37-49 and the first n terms of
+// J1J2.hpp:35-55 nearest-neighbour pairs; then J1J2's next-nearest pairs);
+// l = -1 for a single flip.
+__device__ inline void bond_sites(int kind, int n, int b, int& k, int& l) {
+  if (kind == YQ_TFI) {
+    k = b;
+    l = -1;
+  } else if (kind == YQ_XXZ || b < n) {
+    k = b;
+    l = (b + 1) % n;
+  } else {
+    k = b - n;
+    l = (b - n + 2) % n;
+  }
+}
 struct ElocArgs {
   const int8_t* sigma;  // Ns x n
   const void* theta;    // Ns x m (double or float)
@@ -99,9 +115,17 @@ struct ElocArgs {
   double* eloc;         // Ns (complex machine: interleaved complex)
   int ns, n, m, B, use_float;
   yq_ham_desc ham;
+  // AmplitudePhase machine (real RBM amplitude, FeedForward phase): the phase
+  // network's output for every sample (ns) and for the neighbour of every
+  // (sample, term) pair (ns x B); then ratio holds interleaved complex
+  // exp(logRatio / 2 + i pi (ap_ffb - ap_ffs)).  Null otherwise.
+  const double* ap_ffs;
+  const double* ap_ffb;
 };
 int num_bonds(const yq_ham_desc& h);
 void launch_eloc(const ElocArgs& a, hipStream_t st);
+// the ratio pass alone (ratios of every (sample, term) pair)
+void launch_eloc_ratio(const ElocArgs& a, hipStream_t st);
 size_t eloc_lds_bytes(const ElocArgs& a);
 // Many machines in one launch per pass: inst_dev holds one ElocArgs per
 // machine; stats[2 i], stats[2 i + 1] = mean and variance of machine i.
@@ -173,6 +197,8 @@ void launch_theta_batch_c(const double* W64, const double* b64, const int8_t* si
                           int n, int m, void* out, int out_float, hipStream_t st);
 size_t eloc_c_lds_bytes(const ElocArgs& a);
 void launch_eloc_c(const ElocArgs& a, hipStream_t st);
+// the functor accumulation over interleaved complex ratios (a.ratio) alone
+void launch_eloc_combine_c(const ElocArgs& a, hipStream_t st);
 void launch_eloc_stats_c(const double* eloc, int ns, double* out, hipStream_t st);
 // T = tanh(theta), complex, from an interleaved double or float theta
 void launch_tanh_c(const void* theta, int in_float, double* T, size_t count, hipStream_t st);
@@ -193,9 +219,13 @@ void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hi
 int exact_partials(int B);
 int exact_scan_tiles(int B);
 void launch_basis_spins(const uint32_t* basis, int B, int n, int8_t* sigma, hipStream_t st);
+// (weights |psi|^2 ~ exp(pw Re log psi), pw = 2; 1 for an AmplitudePhase amplitude)
 void launch_exact_weights(const int8_t* sigma, const double* theta, const double* a, int B, int n,
                           int m, int cplx, double* logpsi, double* w, double* rs, double* part,
-                          hipStream_t st);
+                          hipStream_t st, double pw = 2.0);
+// log psi of an RBM batch (a: visible bias or null; complex: (re, im) pairs)
+void launch_log_psi(const int8_t* sigma, const double* theta, const double* a, int B, int n, int m,
+                    int cplx, double* logpsi, hipStream_t st);
 void launch_eloc_stats_w(const double* e, const double* w, int ns, int cplx, double* out,
                          hipStream_t st);
 void launch_exact_cdf(const double* w, int B, double* cdf, double* tile_sums, hipStream_t st);
@@ -219,11 +249,21 @@ struct FFArgs {
   const void* r;        // VJP weights (ns)
   void* partial;        // VJP partials (ff_vjp_blocks x npar)
   void* grad;           // VJP result (npar)
+  // flip_B > 0: configuration q is the term-(q % flip_B) neighbour (see
+  // bond_sites, Hamiltonian kind flip_kind) of sigma row q / flip_B, so ns
+  // counts (sample, term) pairs
+  int flip_B, flip_kind;
 };
 enum { kFFForward = 0, kFFJacobian = 1, kFFVjp = 2 };
 int ff_vjp_blocks(int ns, int use_double);
 size_t ff_smem_bytes(const FFArgs& a, int use_double);
 int launch_ff(const FFArgs& a, int mode, int use_double, hipStream_t st);
+
+// ------------------------------------------------------------------ AmplitudePhase
+// out[s] = (lp[s] / 2, pi ff[s]): log coeff of psi = sqrt(RBM) exp(i pi FF)
+void launch_ap_log_psi(const double* lp, const double* ff, int ns, double* out, hipStream_t st);
+// re[s] = z[2 s], im[s] = z[2 s + 1]
+void launch_split_complex(const double* z, int ns, double* re, double* im, hipStream_t st);
 
 // ------------------------------------------------------------------ vectors
 // deterministic dot product: partial sums per block; host finishes the sum
