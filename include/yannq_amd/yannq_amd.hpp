@@ -12,6 +12,7 @@
 //     initializeRandomEngine() only re-keys the streams.
 #pragma once
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -810,6 +811,372 @@ class RunRBMExact {
   bool machineInitialized_ = false;
   double lambdaIni_ = 1e-3, lambdaDecay_ = 1.0, lambdaMin_ = 1e-4;
   int maxIter_ = 2000, saveWfPer_ = 100;
+};
+
+// ------------------------------------------------------------ FeedForward
+// Machines/FeedForward.hpp:16-205 for the network the device implements:
+// Conv1D(1 -> C, kernel K, stride 1) -> LnCosh -> FullyConnected(C*n -> H)
+// -> Tanh -> Mean({H, 1}, 0), without biases.  addLayer takes the reference's
+// layer constructor arguments (layers/Conv1D.hpp:57-61,
+// layers/FullyConnected.hpp:57-58, layers/Mean.hpp:21); the network is
+// created on the device when the Mean layer completes it, and any other
+// sequence throws.  Scalar is double.
+enum class InitializationMode { LeCun, Xavier, He };  // FeedForward.hpp:11-14
+
+struct Conv1D {
+  Conv1D(uint32_t inChannels, uint32_t outChannels, uint32_t kernelSize, uint32_t stride = 1,
+         bool useBias = false)
+      : in(inChannels), out(outChannels), kernel(kernelSize), stride(stride), useBias(useBias) {}
+  uint32_t in, out, kernel, stride;
+  bool useBias;
+};
+struct LnCosh {};
+struct Tanh {};
+struct FullyConnected {
+  FullyConnected(uint32_t inputDim, uint32_t outputDim, bool useBias = false)
+      : in(inputDim), out(outputDim), useBias(useBias) {}
+  uint32_t in, out;
+  bool useBias;
+};
+struct Mean {
+  Mean(const std::array<uint32_t, 2>& inputDim, uint32_t meanAxis)
+      : dim(inputDim), axis(meanAxis) {}
+  std::array<uint32_t, 2> dim;
+  uint32_t axis;
+};
+
+class FeedForward {
+ public:
+  using Scalar = double;
+  using Vector = std::vector<double>;
+
+  explicit FeedForward(Device& dev) : dev_(&dev) {}
+  FeedForward(FeedForward&&) = default;
+  yq_ctx* ctx() const { return dev_->ctx(); }
+  Device& device() const { return *dev_; }
+
+  template <class Layer, class... Ts>
+  void addLayer(Ts&&... args) {  // :76-82
+    push(Layer(std::forward<Ts>(args)...));
+  }
+  bool complete() const { return stage_ == 5; }
+  uint32_t getN() const { return n_; }
+  uint32_t getDim() const {  // :84-91
+    need();
+    int np = 0;
+    detail::check(ctx(), yq_ff_npar(ctx(), &np));
+    return (uint32_t)np;
+  }
+  std::string summary() const {  // :101-116
+    return "Conv1D(1, " + std::to_string(C_) + ", " + std::to_string(K_) +
+           ", 1)\nLnCosh\nFullyConnected(" + std::to_string(C_ * n_) + ", " +
+           std::to_string(H_) + ")\nTanh\nMean({" + std::to_string(H_) + ", 1}, 0)\n";
+  }
+  std::string desc() const {
+    return "{\"name\": \"FeedForward\", \"n\": " + std::to_string(n_) + ", \"channels\": " +
+           std::to_string(C_) + ", \"kernel\": " + std::to_string(K_) + ", \"hidden\": " +
+           std::to_string(H_) + "}";
+  }
+  Vector getParams() const {
+    Vector v(getDim());
+    detail::check(ctx(), yq_ff_get_params(ctx(), v.data()));
+    return v;
+  }
+  void setParams(const Vector& v) {
+    if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "setParams: wrong size");
+    detail::check(ctx(), yq_ff_set_params(ctx(), v.data()));
+  }
+  void updateParams(const Vector& v) {
+    if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "updateParams: wrong size");
+    detail::check(ctx(), yq_ff_update_params(ctx(), v.data()));
+  }
+  // :43-73 (fan-in / fan-out scaled Gaussians; Philox keyed from the engine)
+  template <class RandomEngine>
+  void initializeRandom(RandomEngine& re, InitializationMode mode) {
+    need();
+    const uint64_t seed = ((uint64_t)re() << 32) ^ (uint64_t)re();
+    const int m = mode == InitializationMode::LeCun ? 0 : mode == InitializationMode::Xavier ? 1 : 2;
+    detail::check(ctx(), yq_ff_init_random(ctx(), seed, m));
+  }
+  // :33-41: every parameter from N(0, sigma^2), drawn with the engine
+  template <class RandomEngine>
+  void initializeRandom(RandomEngine& re, double sigma) {
+    std::normal_distribution<double> nd(0.0, sigma);
+    Vector v(getDim());
+    for (auto& x : v) x = nd(re);
+    setParams(v);
+  }
+  // forward(sigma) (:152-168) and backward(makeData(sigma)) (:170-205)
+  double forward(const std::vector<int>& sigma) const {
+    const std::vector<int8_t> s = spins(sigma);
+    double out = 0.0;
+    detail::check(ctx(), yq_ff_forward(ctx(), 1, s.data(), &out));
+    return out;
+  }
+  Vector backward(const std::vector<int>& sigma) const {
+    const std::vector<int8_t> s = spins(sigma);
+    Vector der(getDim());
+    detail::check(ctx(), yq_ff_jacobian(ctx(), 1, s.data(), der.data(), nullptr));
+    return der;
+  }
+
+ private:
+  static void unsupported() {
+    throw Error(YQ_ERR_UNSUPPORTED,
+                "FeedForward: the device network is Conv1D(1 -> C, k, stride 1) -> LnCosh -> "
+                "FullyConnected(C*n -> H) -> Tanh -> Mean({H, 1}, 0), without biases");
+  }
+  void need() const {
+    if (!complete()) throw Error(YQ_ERR_STATE, "FeedForward: network incomplete");
+  }
+  std::vector<int8_t> spins(const std::vector<int>& sigma) const {
+    need();
+    if (sigma.size() != n_) throw Error(YQ_ERR_ARG, "FeedForward: wrong input size");
+    return std::vector<int8_t>(sigma.begin(), sigma.end());
+  }
+  void push(const Conv1D& l) {
+    if (stage_ != 0 || l.in != 1 || l.stride != 1 || l.useBias) unsupported();
+    C_ = l.out;
+    K_ = l.kernel;
+    stage_ = 1;
+  }
+  void push(const LnCosh&) {
+    if (stage_ != 1) unsupported();
+    stage_ = 2;
+  }
+  void push(const FullyConnected& l) {
+    if (stage_ != 2 || l.useBias || l.in % C_ != 0) unsupported();
+    n_ = l.in / C_;
+    H_ = l.out;
+    stage_ = 3;
+  }
+  void push(const Tanh&) {
+    if (stage_ != 3) unsupported();
+    stage_ = 4;
+  }
+  void push(const Mean& l) {
+    if (stage_ != 4 || l.dim[0] != H_ || l.dim[1] != 1 || l.axis != 0) unsupported();
+    const yq_ff_desc d{(int)n_, (int)C_, (int)K_, (int)H_, 1};
+    detail::check(ctx(), yq_ff_create(ctx(), &d));
+    stage_ = 5;
+  }
+  Device* dev_;
+  int stage_ = 0;
+  uint32_t n_ = 0, C_ = 0, K_ = 0, H_ = 0;
+};
+
+// ------------------------------------------------------------ AmplitudePhase
+// Machines/AmplitudePhase.hpp:9-178: psi = sqrt(RBM) exp(i pi FF) with a
+// real RBM(N, M, useBias) as the amplitude and the FeedForward network (on
+// the same Device) as the phase.  The device machine samples, measures and
+// solves for psi directly (yq_ap_enable); parameters are [RBM | FF].
+class AmplitudePhase {
+ public:
+  using Scalar = double;
+  using RealScalar = double;
+  using ComplexScalar = std::complex<double>;
+  using Vector = std::vector<double>;
+  using AmplitudeMachine = RBM<double>;
+
+  AmplitudePhase(Device& dev, uint32_t N, uint32_t M, bool useBias, FeedForward&& phase)
+      : N_(N), amplitude_(dev, N, M, useBias), phase_(std::move(phase)) {  // :37-40
+    if (&phase_.device() != &dev) throw Error(YQ_ERR_ARG, "AmplitudePhase: phase on another Device");
+    if (!phase_.complete() || phase_.getN() != N)
+      throw Error(YQ_ERR_ARG, "AmplitudePhase: the phase network must act on N spins");
+    detail::check(ctx(), yq_ap_enable(ctx(), 1));
+  }
+  yq_ctx* ctx() const { return amplitude_.ctx(); }
+
+  uint32_t getN() const { return N_; }
+  uint32_t getDimAmp() const { return amplitude_.getDim(); }  // :60-63
+  uint32_t getDimPhase() const { return phase_.getDim(); }    // :65-68
+  uint32_t getDim() const { return getDimAmp() + getDimPhase(); }
+  const AmplitudeMachine& amplitudeMachine() const { return amplitude_; }
+  const FeedForward& phaseMachine() const { return phase_; }
+
+  template <class RandomEngine>
+  void initializeAmplitudeRandom(RandomEngine& re, double sigma) {  // :85-89
+    amplitude_.initializeRandom(re, sigma);
+  }
+  template <class RandomEngine>
+  void initializeRandom(RandomEngine& re, double sigma) {  // :91-96
+    initializeAmplitudeRandom(re, sigma);
+    phase_.initializeRandom(re, InitializationMode::Xavier);
+  }
+  template <class RandomEngine>
+  void initializePhase(RandomEngine& re, InitializationMode mode = InitializationMode::LeCun) {
+    phase_.initializeRandom(re, mode);  // :99-103
+  }
+  Vector getParams() const {
+    Vector v(getDim());
+    detail::check(ctx(), yq_ap_get_params(ctx(), v.data()));
+    return v;
+  }
+  void updateParams(const Vector& m) {  // :153-158
+    if (m.size() != getDim()) throw Error(YQ_ERR_ARG, "updateParams: wrong size");
+    detail::check(ctx(), yq_ap_update_params(ctx(), m.data()));
+  }
+  // log coeff(sigma) = log RBM(sigma) / 2 + i pi FF(sigma); coeff (:127-138)
+  ComplexScalar logCoeff(const std::vector<int>& sigma) const {
+    if (sigma.size() != N_) throw Error(YQ_ERR_ARG, "logCoeff: wrong input size");
+    const std::vector<int8_t> s(sigma.begin(), sigma.end());
+    double out[2];
+    detail::check(ctx(), yq_ap_log_psi(ctx(), 1, s.data(), out));
+    return ComplexScalar(out[0], out[1]);
+  }
+  ComplexScalar coeff(const std::vector<int>& sigma) const { return std::exp(logCoeff(sigma)); }
+  double phaseForward(const std::vector<int>& sigma) const { return phase_.forward(sigma); }
+  std::string desc() const {  // :160-166
+    return "{\"amplitude\": " + amplitude_.desc() + ", \"phase\": " + phase_.desc() + "}";
+  }
+
+ private:
+  uint32_t N_;
+  AmplitudeMachine amplitude_;
+  FeedForward phase_;
+};
+
+// GroundState/NGDExact.hpp:17-204: exact energy, gradient and the amplitude
+// and phase blocks of the metric over a basis, from the device's weighted
+// basis batch.  energyGrad() keeps the reference's factor 2 (it is twice
+// the real part of SRMatExact's gradient).
+class NGDExact {
+ public:
+  using RealScalar = double;
+  using RealVector = std::vector<double>;
+  using RealMatrix = std::vector<double>;  // column-major
+
+  template <class Basis, class Hamiltonian>
+  NGDExact(const AmplitudePhase& qs, Basis&& basis, const Hamiltonian& ham)
+      : qs_(qs), nup_(basis.nup()), ham_(ham.desc()) {}
+
+  void constructExact() {  // :137-170
+    detail::check(ctx(), yq_ham_set(ctx(), &ham_));
+    int B = 0;
+    detail::check(ctx(), yq_exact_batch(ctx(), nup_, &B));
+    double var = 0.0;
+    detail::check(ctx(), yq_eloc(ctx(), nullptr, &energy_, &var));
+    detail::check(ctx(), yq_sr_build(ctx(), YQ_SOLVE_EXPLICIT, YQ_FP64));
+    const size_t D = qs_.getDim();
+    F_.assign(D, 0.0);
+    oloc_.assign(D, 0.0);
+    S_.assign(D * D, 0.0);
+    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_F, F_.data()));
+    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_OMEAN, oloc_.data()));
+    detail::check(ctx(), yq_sr_get(ctx(), YQ_SR_S, S_.data()));
+  }
+  RealScalar eloc() const { return energy_; }
+  RealScalar getEnergy() const { return energy_; }
+  RealVector olocAmp() const { return RealVector(oloc_.begin(), oloc_.begin() + dA()); }
+  RealVector olocPhase() const { return RealVector(oloc_.begin() + dA(), oloc_.end()); }
+  RealMatrix corrMatAmp() const { return block(0, dA()); }                      // :172-177
+  RealMatrix corrMatPhase() const { return block(dA(), qs_.getDim() - dA()); }  // :179-184
+  RealVector energyGrad() const {
+    RealVector g(F_);
+    for (auto& x : g) x *= 2.0;
+    return g;
+  }
+  // (block + lambda I) x = rhs for the amplitude (off 0) or phase block on
+  // the device; rhs and x span the whole parameter vector, the other block's
+  // entries of x are the solve of that block (the blocks are independent)
+  RealVector solve(const RealVector& rhs, double lambda) const {
+    RealVector x(rhs.size());
+    double res = 0.0;
+    int it = 0;
+    detail::check(ctx(), yq_sr_solve(ctx(), lambda, 0.0, 0, rhs.data(), x.data(), &res, &it));
+    return x;
+  }
+
+ private:
+  yq_ctx* ctx() const { return qs_.ctx(); }
+  size_t dA() const { return qs_.getDimAmp(); }
+  RealMatrix block(size_t off, size_t d) const {
+    const size_t D = qs_.getDim();
+    RealMatrix out(d * d);
+    for (size_t j = 0; j < d; ++j)
+      for (size_t i = 0; i < d; ++i) out[j * d + i] = S_[(off + j) * D + off + i];
+    return out;
+  }
+  const AmplitudePhase& qs_;
+  int nup_;
+  yq_ham_desc ham_;
+  double energy_ = 0.0;
+  RealVector F_, oloc_, S_;
+};
+
+// Runners/RunAmplitudePhase.hpp:10-127: NGDExact epochs with the amplitude
+// and phase blocks solved separately (each + lambda I), then the optimizer;
+// callback(ll, E, |grad|, |v|).  Weight files are not written (saveWfPer
+// must be 0: the reference's AmplitudePhase archive holds the phase
+// network's layer objects).
+enum class Mode { AMP, PHASE, ALL };
+
+class RunAmplitudePhaseExact {
+ public:
+  using Machine = AmplitudePhase;
+  using RealVector = std::vector<double>;
+
+  RunAmplitudePhaseExact(Device& dev, uint32_t N, uint32_t alpha, bool useBias,
+                         FeedForward&& phase, std::ostream& logger)
+      : logger_(logger), qs_(dev, N, N * alpha, useBias, std::move(phase)),
+        re_(std::random_device{}()) {}
+  void setLambda(double ini, double decay = 1.0, double min = 0.0) {
+    lambdaIni_ = ini;
+    lambdaDecay_ = decay;
+    lambdaMin_ = min;
+  }
+  void setIterParams(int maxIter, int saveWfPer) {
+    if (saveWfPer != 0)
+      throw Error(YQ_ERR_UNSUPPORTED, "RunAmplitudePhaseExact: weight files are not written");
+    maxIter_ = maxIter;
+  }
+  void setOptimizer(const std::string& name, double alpha = 0.05, double p = 0.0) {
+    opt_ = createOptimizer<double>(name, alpha, p);
+  }
+  void setSeed(uint64_t s) { re_.seed(s); }
+  void initializePhase() { qs_.initializePhase(re_, InitializationMode::Xavier); }  // :40-43
+  void initializeAmplitude(double sigma) { qs_.initializeAmplitudeRandom(re_, sigma); }
+  void initializeRandom(double sigma = 1e-3) {  // AbstractRunner.hpp:96-101
+    machineInitialized_ = true;
+    qs_.initializeRandom(re_, sigma);
+  }
+  Machine& getQs() { return qs_; }
+  uint32_t getDim() const { return qs_.getDim(); }
+
+  template <class Callback, class Basis, class Hamiltonian>
+  void run(Callback&& callback, Basis&& basis, Hamiltonian&& ham, Mode mode = Mode::ALL) {
+    if (!machineInitialized_) initializeRandom();
+    if (!opt_) setOptimizer("SGD");
+    const size_t dA = qs_.getDimAmp(), D = qs_.getDim();
+    NGDExact ngdex(qs_, basis, ham);
+    for (int ll = 0; ll <= maxIter_; ll++) {  // :72-121
+      logger_ << "Epochs: " << ll << std::endl;
+      ngdex.constructExact();
+      const double currE = ngdex.eloc();
+      const RealVector grad = ngdex.energyGrad();
+      const double lambda = std::max(lambdaIni_ * std::pow(lambdaDecay_, ll), lambdaMin_);
+      RealVector v = ngdex.solve(grad, lambda);  // both blocks, each with its own factor
+      if (mode == Mode::PHASE) std::fill(v.begin(), v.begin() + dA, 0.0);
+      if (mode == Mode::AMP) std::fill(v.begin() + dA, v.end(), 0.0);
+      const RealVector optV = opt_->getUpdate(v);
+      qs_.updateParams(optV);
+      double ng = 0.0, nv = 0.0;
+      for (size_t i = 0; i < D; ++i) {
+        ng += grad[i] * grad[i];
+        nv += v[i] * v[i];
+      }
+      callback(ll, currE, std::sqrt(ng), std::sqrt(nv));
+    }
+  }
+
+ private:
+  std::ostream& logger_;
+  Machine qs_;
+  std::mt19937_64 re_;
+  std::unique_ptr<Optimizer<double>> opt_;
+  bool machineInitialized_ = false;
+  double lambdaIni_ = 1e-3, lambdaDecay_ = 1.0, lambdaMin_ = 1e-4;
+  int maxIter_ = 2000;
 };
 
 // Utilities/Utility.hpp:64-88 random configurations for randomize()

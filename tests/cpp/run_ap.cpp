@@ -1,0 +1,78 @@
+// AmplitudePhase through the concept layer, written the way the reference
+// drives it: Examples/AmplitudePhase.cpp (RunAmplitudePhaseExact over
+// BasisJz(N, N/2) on the J1-J2 chain with the sign rule) and
+// Tests/TestAmplitudePhase.cpp (SRMatExact<AmplitudePhase> with
+// corrMat + lambda I and SGD(0.02)), with the device's phase network
+// Conv1D -> LnCosh -> FullyConnected -> Tanh -> Mean.
+//   run_ap ngd  E0   RunAmplitudePhaseExact; exit 0 if within 1e-4 of E0
+//   run_ap srex E0   SRMatExact loop; exit 0 if the energy falls and ends
+//                    within 1e-4 of E0 (relative)
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+#include "yannq_amd/yannq_amd.hpp"
+
+using namespace yannq_amd;
+
+static FeedForward phaseNetwork(Device& dev, int N) {
+  FeedForward ff(dev);
+  ff.addLayer<Conv1D>(1, 4, 5, 1, false);
+  ff.addLayer<LnCosh>();
+  ff.addLayer<FullyConnected>(4 * N, 8, false);
+  ff.addLayer<Tanh>();
+  ff.addLayer<Mean>(std::array<uint32_t, 2>{8, 1}, 0);
+  return ff;
+}
+
+int main(int argc, char** argv) {
+  if (argc != 3) {
+    std::printf("usage: %s ngd|srex E0\n", argv[0]);
+    return 2;
+  }
+  const double e0 = std::atof(argv[2]);
+  const int N = 10;
+  Device dev(0);
+  J1J2 ham(N, 1.0, 0.2, true);
+  double first = 0.0, last = 0.0;
+  if (std::strcmp(argv[1], "ngd") == 0) {
+    std::ostringstream log;
+    RunAmplitudePhaseExact runner(dev, N, 2, false, phaseNetwork(dev, N), log);
+    runner.setSeed(3);
+    runner.initializeRandom(0.01);
+    runner.setLambda(10.0, 0.9, 1e-3);
+    runner.setIterParams(300, 0);
+    runner.setOptimizer("SGD", 0.05, 0.0);
+    auto callback = [&](int ll, double E, double ng, double nv) {
+      if (ll == 0) first = E;
+      last = E;
+      if (ll % 50 == 0) std::printf("%d\t%.10f\t%.4e\t%.4e\n", ll, E, ng, nv);
+    };
+    runner.run(callback, BasisJz(N, N / 2), ham);
+    std::printf("final %.10f exact %.10f (start %.10f)\n", last, e0, first);
+    return std::fabs(last - e0) / std::fabs(e0) < 1e-4 ? 0 : 1;
+  }
+  // Tests/TestAmplitudePhase.cpp:42-76
+  std::mt19937_64 re(5);
+  FeedForward ff = phaseNetwork(dev, N);
+  ff.initializeRandom(re, InitializationMode::LeCun);
+  AmplitudePhase qs(dev, N, 4 * N, true, std::move(ff));
+  qs.initializeAmplitudeRandom(re, 0.01);
+  SGD<double> opt(0.02, 0.0);
+  const double lambda = 0.001;
+  SRMatExact<AmplitudePhase, J1J2> srex(qs, BasisJz(N, N / 2), ham);
+  for (int ll = 0; ll < 400; ll++) {
+    srex.constructExact();
+    const double energy = srex.eloc();
+    const auto s = srex.solveExact(srex.energyGrad(), lambda);
+    qs.updateParams(opt.getUpdate(s));
+    if (ll == 0) first = energy;
+    last = energy;
+    if (ll % 100 == 0) std::printf("%d\t%.10f\n", ll, energy);
+  }
+  std::printf("final %.10f exact %.10f (start %.10f)\n", last, e0, first);
+  return (last < first && std::fabs(last - e0) / std::fabs(e0) < 1e-4) ? 0 : 1;
+}

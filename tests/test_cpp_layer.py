@@ -21,7 +21,7 @@ def _build(yq, tmp_path, name="run_tfi"):
     return exe
 
 
-@pytest.mark.parametrize("name", ["run_tfi", "run_xxz", "run_ckpt", "run_exact"])
+@pytest.mark.parametrize("name", ["run_tfi", "run_xxz", "run_ckpt", "run_exact", "run_ap"])
 def test_concept_layer_compiles(yq, tmp_path, name):
     assert os.path.exists(_build(yq, tmp_path, name))
 
@@ -80,5 +80,26 @@ def test_exact_runner_reaches_ground_state(yq, tmp_path):
         pytest.skip("no GPU visible")
     exe = _build(yq, tmp_path, "run_exact")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["ngd", "srex"])
+def test_amplitude_phase_runners(yq, orc, tmp_path, mode):
+    """RunAmplitudePhaseExact and the SRMatExact<AmplitudePhase> loop on the
+    J1-J2 chain (N = 10, J2 = 0.2, sign rule) in the Sz = 0 sector, against
+    the sector's ground energy from exact diagonalisation."""
+    import numpy as np
+    import scipy.sparse.linalg as sla
+    from tests.conftest import _gpu_available
+    if not _gpu_available():
+        pytest.skip("no GPU visible")
+    n = 10
+    H = orc.ham_matrix(orc.ham(orc.J1J2, n, J=1.0, J2=0.2, sign_rule=1))
+    basis = orc.basis_jz(n, n // 2)
+    e0 = float(sla.eigsh(H[basis][:, basis].astype(float), k=1, which="SA")[0][0])
+    exe = _build(yq, tmp_path, "run_ap")
+    r = subprocess.run([exe, mode, repr(e0)], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
