@@ -178,6 +178,22 @@ def measured_traffic(kernel):
     return None if k is None else k["hbm_bytes_per_launch"]
 
 
+def cg_probe(yq):
+    """64 mixed CG iterations on 16,384 random configurations at N=100, M=400,
+    launched eagerly (no graphs), as a short load for counter passes."""
+    ctx = yq.Context(0)
+    ctx.graph_mode(0)
+    ctx.rbm_create(N, M)
+    ctx.init_random(3, 0.05)
+    ctx.ham_set(yq.TFI, N, J=-1.0, h=-1.0)
+    sig = np.random.default_rng(0).choice([-1, 1], size=(NS, N)).astype(np.int8)
+    ctx.samples_set(sig, precision=yq.MIXED)
+    ctx.sr_build(yq.SOLVE_CG_FACTORED, precision=yq.MIXED)
+    _, _, it = ctx.sr_solve(LAMBDA, tol=1e-30, max_iter=64, allow_unconverged=True)
+    print(json.dumps({"cg_probe_iterations": it}), flush=True)
+    ctx.close()
+
+
 def batched(yq, warmup, steps):
     """Many machines in one context: 32 TFI machines, N ~ U{16, 24, ..., 128},
     alpha ~ U{1, 2, 4}, h evenly spaced over [0.5, 1.5], 2,048 samples each,
@@ -286,6 +302,9 @@ def main():
                     help="only run the copy that calibrates the PMC traffic counters")
     ap.add_argument("--batched", action="store_true",
                     help="the 32-machine batched step against 32 sequential FP64 runs")
+    ap.add_argument("--cg-probe", action="store_true",
+                    help="only 64 eager mixed CG iterations at this shape (a short load for "
+                         "counter passes of the CG kernels)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -295,6 +314,9 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import yannq_amd as yq
+    if args.cg_probe:
+        cg_probe(yq)
+        return
     if args.batched:
         batched(yq, args.warmup, args.steps)
         return
@@ -330,6 +352,23 @@ def main():
                 "timing": "HIP events bound to each launch of the kernel (hipExtLaunchKernel) "
                           "in a second pass over the same steps; the average includes the "
                           "launches that return at once after the CG converged"}
+
+    # every kernel of the step from the profiled pass (2 steps): average launch,
+    # launches and time per step, algorithmic bytes and HBM fraction; the
+    # sampler and the E_loc ratios are VALU-bound (log-cosh differences)
+    per_kernel = {}
+    for lab, (t_ms, cnt) in prof0.items():
+        if cnt == 0:
+            continue
+        avg = t_ms / 1e3 / cnt
+        gbs = KERNEL_WORK[lab] / avg / 1e9
+        d = {"avg_launch_us": avg * 1e6, "launches_per_step": cnt / 2, "ms_per_step": t_ms / 2,
+             "algorithmic_bytes_per_launch": KERNEL_WORK[lab], "achieved_gbs": gbs,
+             "hbm_frac": gbs / HBM_PEAK_GBS,
+             "bound": "valu" if lab in ("sample", "eloc_ratio") else "hbm"}
+        if lab in ("sample", "eloc_ratio"):  # one sweep / one pass: Ns * n * m terms
+            d["logcosh_diff_terms_per_s"] = NS * N * M / avg
+        per_kernel[lab] = d
 
     extra = {}
     fp64_iters = iters
@@ -373,6 +412,7 @@ def main():
         "phase_ms": {"sample": float(np.mean([o["sample_ms"] for o in outs])),
                      "eloc_build_solve_update": float(np.mean([o["solve_ms"] for o in outs]))},
         "roofline": roofline,
+        "per_kernel": per_kernel,
         "cpu_baseline": cpu,
         **extra,
     }
