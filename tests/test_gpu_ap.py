@@ -136,6 +136,32 @@ def test_ap_eloc_and_sr_of_samples(ctx, orc, name):
     assert _rel(ctx.sr_apply(shift, v), S @ v + shift * v) < 1e-11
 
 
+def test_ap_mixed_batch(ctx, orc, yq):
+    """A float-theta batch (mixed sampling): E_loc and the SR rows in float
+    arithmetic for the amplitude ratios, against the FP64 oracle on the same
+    configurations (the stated mixed-precision error)."""
+    rng = np.random.default_rng(49)
+    n, m = 20, 40
+    rbm, desc, ffp = _setup(ctx, orc, n, m, rng, amp=0.2, phase=0.5)
+    H = _ham(ctx, orc, "j1j2", n)
+    ctx.sampler_config(1, 256, precision=yq.MIXED, seed=13)
+    ctx.randomize(n // 2)
+    ns = ctx.sample(4, 4)
+    sig, _ = ctx.samples()
+    th = orc.theta(rbm, sig)
+    el, mean, _ = ctx.eloc()
+    want = orc.ap_eloc(H, rbm, desc, ffp, sig, th)
+    assert _rel(el, want) < 2e-5
+    assert abs(mean - want.real.mean()) < 2e-5 * abs(mean)
+    ctx.sr_build(0)
+    rows = orc.ap_logderiv(rbm, desc, ffp, sig, th)
+    assert _rel(ctx.sr_get(2), rows - rows.mean(axis=0)) < 1e-5
+    dA = rbm.dim
+    Oc = rows - rows.mean(axis=0)
+    F = np.concatenate([Oc[:, :dA].T @ want.real, Oc[:, dA:].T @ want.imag]) / ns
+    assert _rel(ctx.sr_get(0), F) < 1e-4
+
+
 @pytest.mark.parametrize("name,nup,use_bias", [("tfi", -1, True), ("j1j2", 5, False),
                                                ("xxz", 5, True)])
 def test_ap_exact_matches_ngd_exact(ctx, orc, name, nup, use_bias):
