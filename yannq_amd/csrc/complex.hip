@@ -124,7 +124,13 @@ __global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
     if (A.mixed && !fresh) refresh();
     fresh = false;
     for (int p = 0; p < A.nprop; ++p) {
-      const u32x4 x = draw(A.seed, ev++, (uint32_t)chain, kDomSweep);
+      // wave-uniform counter and stream (lane 0's): Philox on the scalar unit
+      const uint64_t evu =
+          ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ev >> 32)) << 32) |
+          (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ev);
+      ev = evu + 1;
+      const u32x4 x = draw(A.seed, evu, __builtin_amdgcn_readfirstlane((uint32_t)chain),
+                           kDomSweep);
       const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
       const int8_t sk = sig[k];
       const R twok = sk > 0 ? (R)2 : (R)-2;

@@ -69,8 +69,15 @@ __device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, 
   const R* Wr = static_cast<const R*>(A.Wr);
   const R* ar = static_cast<const R*>(A.ar);
   uint64_t nacc = 0;
+  // the chain index and proposal counter are the same in every lane: read
+  // them from lane 0 so that the Philox block runs on the scalar unit
+  const uint32_t stream = __builtin_amdgcn_readfirstlane((uint32_t)chain);
   for (int p = 0; p < A.nprop; ++p) {
-    const u32x4 x = draw(A.seed, ev++, (uint32_t)chain, kDomSweep);
+    const uint64_t evu =
+        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ev >> 32)) << 32) |
+        (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ev);
+    ev = evu + 1;
+    const u32x4 x = draw(A.seed, evu, stream, kDomSweep);
     const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
     const int8_t sk = sig[k];
     const R twok = sk > 0 ? (R)2 : (R)-2;
