@@ -91,7 +91,9 @@ def run_path(yq, precision, solver, warmup, steps, precond=0, warm_start=0, prof
         ctx._L.yq_prof_enable(ctx._h, prof_label.encode())
     t0 = time.perf_counter()
     for i in range(steps):
+        ts = time.perf_counter()
         o = ctx.sr_step(n_discard=0 if (warmup or i) else 20, **kw)
+        o["wall_ms"] = 1e3 * (time.perf_counter() - ts)
         outs.append(o)
         energies.append(o["energy"])
     t1 = time.perf_counter()   # every sr_step returns after its stream synchronised
@@ -407,6 +409,7 @@ def main():
                              "tile autotune=%d" % (args.precond, args.warm_start, args.autotune),
                    "lambda": LAMBDA, "lr": LR, "cg_tol": TOL},
         "samples_per_s": value * CHAINS,
+        "median_step_ms": float(np.median([o["wall_ms"] for o in outs])),
         "cg_iters": iters,
         "energy": energies[-1],
         "phase_ms": {"sample": float(np.mean([o["sample_ms"] for o in outs])),
