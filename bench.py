@@ -180,6 +180,65 @@ def measured_traffic(kernel):
     return None if k is None else k["hbm_bytes_per_launch"]
 
 
+def precision_errors(yq):
+    """The mixed path's error against FP64 on identical configurations.
+    TFI N=100, alpha=4, 16,384 samples: E (mean and per-sample E_loc), F, the
+    operator S v (random v) and the SR update delta-theta ((S + 1e-3 I) x = F,
+    the refined mixed solve vs an FP64 solve to 1e-12).  Complex XXZ N=64,
+    alpha=2, 4096 chains x 4 sweeps (Sz = 0, exchange moves): E_loc mean and F
+    with float theta and float E_loc (the SR build is FP64 for complex
+    machines).  Prints one JSON line."""
+    def rel(a, b):
+        return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+    out = {}
+    ctx = yq.Context(0)
+    ctx.rbm_create(N, M)
+    ctx.init_random(SEED, 0.01)
+    ctx.ham_set(yq.TFI, N, J=-1.0, h=-1.0)
+    ctx.sampler_config(yq.LOCAL, CHAINS, precision=yq.FP64, seed=SEED)
+    ctx.sample(1, 20)
+    sig, _ = ctx.samples()
+    v = np.random.default_rng(SEED).normal(size=DIM)
+    res = {}
+    for prec in (yq.FP64, yq.MIXED):
+        ctx.samples_set(sig, precision=prec)
+        e, mean, _ = ctx.eloc()
+        ctx.sr_build(yq.SOLVE_CG_FACTORED, precision=prec)
+        F = ctx.sr_get(yq.SR_F)
+        Sv = ctx.sr_apply(0.0, v)
+        x, _, it = ctx.sr_solve(LAMBDA, tol=1e-12 if prec == yq.FP64 else TOL, max_iter=50000)
+        res[prec] = (e, mean, F, Sv, x, it)
+    a, b = res[yq.MIXED], res[yq.FP64]
+    out["tfi_n100_a4"] = {
+        "samples": int(sig.shape[0]),
+        "energy_rel_err": abs(a[1] - b[1]) / abs(b[1]), "eloc_rel_err": rel(a[0], b[0]),
+        "F_rel_err": rel(a[2], b[2]), "Sv_rel_err": rel(a[3], b[3]),
+        "dtheta_rel_err": rel(a[4], b[4]), "cg_iters": [b[5], a[5]]}
+    ctx.close()
+    ctx = yq.Context(0)
+    n2, m2 = 64, 128
+    ctx.rbm_create(n2, m2, dtype=np.complex128)
+    ctx.init_random(SEED, 0.01)
+    ctx.ham_set(yq.XXZ, n2, J=1.0, delta=1.0)
+    ctx.sampler_config(yq.EXCHANGE, 4096, precision=yq.FP64, seed=SEED)
+    ctx.randomize(n2 // 2)
+    ctx.sample(4, 20)
+    sig, _ = ctx.samples()
+    res = {}
+    for prec in (yq.FP64, yq.MIXED):
+        ctx.samples_set(sig, precision=prec)
+        e, mean, _ = ctx.eloc()
+        ctx.sr_build(yq.SOLVE_CG, precision=yq.FP64)
+        res[prec] = (e, mean, ctx.sr_get(yq.SR_F))
+    a, b = res[yq.MIXED], res[yq.FP64]
+    out["xxz_n64_a2_complex"] = {
+        "samples": int(sig.shape[0]),
+        "energy_rel_err": abs(a[1] - b[1]) / abs(b[1]), "eloc_rel_err": rel(a[0], b[0]),
+        "F_rel_err": rel(a[2], b[2])}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def cg_probe(yq):
     """64 mixed CG iterations on 16,384 random configurations at N=100, M=400,
     launched eagerly (no graphs), as a short load for counter passes."""
@@ -304,6 +363,8 @@ def main():
                     help="only run the copy that calibrates the PMC traffic counters")
     ap.add_argument("--batched", action="store_true",
                     help="the 32-machine batched step against 32 sequential FP64 runs")
+    ap.add_argument("--errors", action="store_true",
+                    help="the mixed path's error against FP64 on identical configurations")
     ap.add_argument("--cg-probe", action="store_true",
                     help="only 64 eager mixed CG iterations at this shape (a short load for "
                          "counter passes of the CG kernels)")
@@ -318,6 +379,9 @@ def main():
     import yannq_amd as yq
     if args.cg_probe:
         cg_probe(yq)
+        return
+    if args.errors:
+        precision_errors(yq)
         return
     if args.batched:
         batched(yq, args.warmup, args.steps)
