@@ -239,6 +239,39 @@ def precision_errors(yq):
     print(json.dumps(out), flush=True)
 
 
+def ff_bench(yq):
+    """The FeedForward per-sample Jacobian at its configuration: Conv1D(1 -> 8,
+    width 5) -> LnCosh -> FullyConnected(320 -> 64) -> Tanh -> Mean on N = 40
+    spins (20,520 parameters), float, a batch of 8192 Sz = 0 configurations;
+    kernel time from HIP events on the launch, achieved bytes (the Jacobian
+    written once, spins and parameters read) against the HBM peak.  Prints one
+    JSON line."""
+    n, C_, K, H, ns = 40, 8, 5, 64, 8192
+    ctx = yq.Context(0)
+    ctx.ff_create(n, C_, K, H, dtype=np.float32)
+    ctx.ff_init_random(SEED, 3)
+    rng = np.random.default_rng(SEED)
+    sig = np.ones((ns, n), np.int8)
+    for s in range(ns):
+        sig[s, rng.choice(n, n // 2, replace=False)] = -1
+    npar = ctx.ff_npar
+    ctx.ff_jacobian(sig)  # warm-up
+    ctx._L.yq_prof_enable(ctx._h, b"ff_jacobian")
+    for _ in range(5):
+        ctx.ff_jacobian(sig)
+    tot, cnt = C.c_double(), C.c_longlong()
+    ctx._L.yq_prof_read(ctx._h, b"ff_jacobian", C.byref(tot), C.byref(cnt))
+    ctx._L.yq_prof_enable(ctx._h, None)
+    avg = tot.value / 1e3 / cnt.value
+    nbytes = ns * npar * 4 + ns * n + npar * 4 + ns * 4
+    out = {"workload": "FeedForward Conv1D(1->8,k5)-LnCosh-FC(320->64)-Tanh-Mean, N=40, "
+                       f"{npar} params, float, batch {ns}",
+           "jacobian_us": avg * 1e6, "jacobian_bytes": nbytes,
+           "jacobian_gbs": nbytes / avg / 1e9, "hbm_frac": nbytes / avg / 1e9 / HBM_PEAK_GBS}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def cg_probe(yq):
     """64 mixed CG iterations on 16,384 random configurations at N=100, M=400,
     launched eagerly (no graphs), as a short load for counter passes."""
@@ -363,6 +396,8 @@ def main():
                     help="only run the copy that calibrates the PMC traffic counters")
     ap.add_argument("--batched", action="store_true",
                     help="the 32-machine batched step against 32 sequential FP64 runs")
+    ap.add_argument("--ff", action="store_true",
+                    help="the FeedForward Jacobian kernel at N=40, batch 8192")
     ap.add_argument("--errors", action="store_true",
                     help="the mixed path's error against FP64 on identical configurations")
     ap.add_argument("--cg-probe", action="store_true",
@@ -382,6 +417,9 @@ def main():
         return
     if args.errors:
         precision_errors(yq)
+        return
+    if args.ff:
+        ff_bench(yq)
         return
     if args.batched:
         batched(yq, args.warmup, args.steps)

@@ -1781,6 +1781,8 @@ int ff_upload(yq_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->ffp.p, f.data(), sizeof(float) * np, hipMemcpyHostToDevice, c->st));
     TRY(sync(c));
   }
+  launch_ff_transpose(ff_args(c, 1, nullptr), c->ff.dtype, c->st);
+  LAUNCHCHK(c);
   return sync(c);
 }
 FFArgs ff_args(yq_ctx* c, int ns, const int8_t* d_sigma) {
@@ -1791,6 +1793,8 @@ FFArgs ff_args(yq_ctx* c, int ns, const int8_t* d_sigma) {
   a.H = c->ff.hidden;
   a.ns = ns;
   a.params = c->ffp.p;
+  // the transposed FC weight follows the parameters in the same buffer
+  a.wt = static_cast<const char*>(c->ffp.p) + ff_rs(c) * (size_t)c->ff_npar;
   a.sigma = d_sigma;
   return a;
 }
@@ -1831,7 +1835,7 @@ YQ_API int yq_ff_create(yq_ctx* c, const yq_ff_desc* d) {
   FFArgs probe = ff_args(c, 1, nullptr);
   if (ff_smem_bytes(probe, d->dtype) > 64 * 1024)
     return fail(c, YQ_ERR_UNSUPPORTED, "FeedForward activations exceed the LDS budget");
-  TRY(ensure(c, c->ffp, ff_rs(c) * (size_t)np));
+  TRY(ensure(c, c->ffp, ff_rs(c) * ((size_t)np + (size_t)d->channels * d->n * d->hidden)));
   c->have_ff = true;
   return ff_upload(c);
 }

@@ -30,6 +30,13 @@ constexpr int sb() {
 
 enum { kFwd = 0, kJac = 1, kVjp = 2 };
 
+// periodic site index j mod n for j in [-n, 2n) (the kernel width is at
+// most n), without an integer division
+__device__ inline int wrap(int j, int n) {
+  j += j < 0 ? n : 0;
+  return j >= n ? j - n : j;
+}
+
 template <typename R>
 __device__ inline R tanh_r(R x) {
   return tanh(x);
@@ -94,19 +101,47 @@ __global__ void __launch_bounds__(256) k_ff(const FFArgs A) {
       for (int q = tid; q < cnt * CN; q += 256) {
         const int s = q / CN, c = q % CN, oc = c / n, r = c % n;
         R v = (R)0;
-        for (int ki = 0; ki < K; ++ki) v += ker[ki + oc * K] * sg[s * n + (r + ki - K / 2 + n) % n];
+        for (int ki = 0; ki < K; ++ki) v += ker[ki + oc * K] * sg[s * n + wrap(r + ki - K / 2, n)];
         z[q] = v;
         a1[q] = Math<R>::logcosh(v);
       }
       __syncthreads();
-      // FullyConnected::forward (W^T a1) then Tanh
-      for (int q = tid; q < cnt * H; q += 256) {
-        const int s = q / H, o = q % H;
-        const R* wo = W + (size_t)o * CN;
-        const R* x = a1 + s * CN;
-        R v = (R)0;
-        for (int i = 0; i < CN; ++i) v += wo[i] * x[i];
-        a2[q] = tanh_r(v);
+      // FullyConnected::forward (W^T a1) then Tanh.  Neighbouring threads take
+      // neighbouring outputs o and read the transposed weight WT[i][o] row by
+      // row (coalesced); each WT value serves all samples of the group.  When
+      // H divides the block, the P = 256 / H thread groups split i into P
+      // contiguous ranges whose partial sums (kept in dz, free until the
+      // backward pass) are added in range order.
+      const R* WT = static_cast<const R*>(A.wt);
+      const int P = 256 / H;
+      if (256 % H == 0 && CN >= 256) {
+        const int o = tid % H, pr = tid / H;
+        const int len = (CN + P - 1) / P, i0 = pr * len, i1 = min(CN, i0 + len);
+        R v[kSB];
+#pragma unroll
+        for (int s = 0; s < kSB; ++s) v[s] = (R)0;
+        for (int i = i0; i < i1; ++i) {
+          const R w = WT[(size_t)i * H + o];
+#pragma unroll
+          for (int s = 0; s < kSB; ++s) v[s] += w * a1[s * CN + i];
+        }
+#pragma unroll
+        for (int s = 0; s < kSB; ++s) dz[(pr * kSB + s) * H + o] = v[s];
+        __syncthreads();
+        for (int q = tid; q < cnt * H; q += 256) {
+          const int s = q / H, oo = q % H;
+          R t = (R)0;
+          for (int p2 = 0; p2 < P; ++p2) t += dz[(p2 * kSB + s) * H + oo];
+          a2[q] = tanh_r(t);
+        }
+      } else {
+        for (int q = tid; q < cnt * H; q += 256) {
+          const int s = q / H, o = q % H;
+          const R* x = a1 + s * CN;
+          R t = (R)0;
+          for (int i = 0; i < CN; ++i) t += WT[(size_t)i * H + o] * x[i];
+          a2[q] = tanh_r(t);
+        }
       }
       __syncthreads();
     }
@@ -140,31 +175,87 @@ __global__ void __launch_bounds__(256) k_ff(const FFArgs A) {
       static_cast<R*>(A.out)[s0 + tid] = s / (R)H;
     }
     __syncthreads();
-    // FullyConnected::backprop din = W d2, then LnCosh Jacobian tanh(z)
-    for (int q = tid; q < cnt * CN; q += 256) {
-      const int s = q / CN, i = q % CN;
-      R v = (R)0;
-      for (int o = 0; o < H; ++o) v += W[i + (size_t)o * CN] * d2[s * H + o];
-      dz[q] = v * tanh_r(z[q]);
+    // kJac: the FullyConnected rows a1 (x) d2 (all but K*C of every row's
+    // entries, the bulk of the kernel's HBM writes) go out first, so that
+    // their stores drain while the backward pass below computes
+    R* const jac = static_cast<R*>(A.jac);
+    const int KC = K * C;
+    if (MODE == kJac) {
+      const int o0 = tid / CN, i0 = tid % CN;
+      constexpr int VV = 16 / sizeof(R);
+      const bool vec = npar % VV == 0 && KC % VV == 0 && CN % VV == 0 &&
+                       reinterpret_cast<uintptr_t>(jac) % 16 == 0;
+      const int o0v = tid * VV / CN, i0v = tid * VV % CN;
+      for (int s = 0; s < cnt; ++s) {
+        // FullyConnected: a1 (x) d2 at KC + o * CN + i, with (o, i) advanced
+        // by the block stride instead of a division per element; 16-byte
+        // stores when the row layout allows them
+        const R* a1s = a1 + s * CN;
+        const R* d2s = d2 + s * H;
+        R* fc = jac + (size_t)(s0 + s) * npar + KC;
+        if (vec) {
+          constexpr int V = 16 / sizeof(R);
+          int o = o0v, i = i0v;
+          for (int q = tid * V; q < CN * H; q += 256 * V) {
+            const R dv = d2s[o];
+            if (V == 4) {
+              float4 t;
+              t.x = (float)(a1s[i] * dv);
+              t.y = (float)(a1s[i + 1] * dv);
+              t.z = (float)(a1s[i + 2] * dv);
+              t.w = (float)(a1s[i + 3] * dv);
+              *reinterpret_cast<float4*>(fc + q) = t;
+            } else {
+              double2 t;
+              t.x = (double)(a1s[i] * dv);
+              t.y = (double)(a1s[i + 1] * dv);
+              *reinterpret_cast<double2*>(fc + q) = t;
+            }
+            i += 256 * V;
+            while (i >= CN) {
+              i -= CN;
+              ++o;
+            }
+          }
+        } else {
+          int o = o0, i = i0;
+          for (int q = tid; q < CN * H; q += 256) {
+            fc[q] = a1s[i] * d2s[o];
+            i += 256;
+            while (i >= CN) {
+              i -= CN;
+              ++o;
+            }
+          }
+        }
+      }
+    }
+    // FullyConnected::backprop din = W d2, then LnCosh Jacobian tanh(z): one
+    // thread per input i for all samples of the group, so every W value is
+    // read once (coalesced over i) and the samples' sums run side by side
+    for (int i = tid; i < CN; i += 256) {
+      R v[kSB];
+#pragma unroll
+      for (int s = 0; s < kSB; ++s) v[s] = (R)0;
+      for (int o = 0; o < H; ++o) {
+        const R w = W[i + (size_t)o * CN];
+#pragma unroll
+        for (int s = 0; s < kSB; ++s) v[s] += w * d2[s * H + o];
+      }
+#pragma unroll
+      for (int s = 0; s < kSB; ++s)
+        if (s < cnt) dz[s * CN + i] = v[s] * tanh_r(z[s * CN + i]);
     }
     __syncthreads();
     if (MODE == kJac) {
-      R* jac = static_cast<R*>(A.jac);
-      for (int s = 0; s < cnt; ++s) {
-        R* row = jac + (size_t)(s0 + s) * npar;
-        for (int c = tid; c < npar; c += 256) {
-          R v;
-          if (c < K * C) {  // Conv1D::backprop weight part
-            const int ki = c % K, oc = c / K;
-            v = (R)0;
-            for (int r = 0; r < n; ++r)
-              v += sg[s * n + (r + ki - K / 2 + n) % n] * dz[s * CN + r + oc * n];
-          } else {  // FullyConnected: a1 (x) d2
-            const int q = c - K * C;
-            v = a1[s * CN + q % CN] * d2[s * H + q / CN];
-          }
-          row[c] = v;
-        }
+      // Conv1D::backprop weight part for every (sample, kernel entry) of the
+      // group, spread over the block, written straight into the rows
+      for (int q = tid; q < cnt * KC; q += 256) {
+        const int s = q / KC, c = q % KC, ki = c % K, oc = c / K;
+        R v = (R)0;
+        for (int r = 0; r < n; ++r)
+          v += sg[s * n + wrap(r + ki - K / 2, n)] * dz[s * CN + r + oc * n];
+        jac[(size_t)(s0 + s) * npar + c] = v;
       }
     } else {  // kVjp: acc[e] += sum_s r_s O_s[c], c = tid + 256 e
 #pragma unroll
@@ -178,7 +269,7 @@ __global__ void __launch_bounds__(256) k_ff(const FFArgs A) {
               const int ki = c % K, oc = c / K;
               o = (R)0;
               for (int r = 0; r < n; ++r)
-                o += sg[s * n + (r + ki - K / 2 + n) % n] * dz[s * CN + r + oc * n];
+                o += sg[s * n + wrap(r + ki - K / 2, n)] * dz[s * CN + r + oc * n];
             } else {
               const int q = c - K * C;
               o = a1[s * CN + q % CN] * d2[s * H + q / CN];
@@ -199,6 +290,12 @@ __global__ void __launch_bounds__(256) k_ff(const FFArgs A) {
       if (c < npar) part[c] = acc[e];
     }
   }
+}
+
+template <typename R>
+__global__ void k_ff_transpose(const R* __restrict__ W, int CN, int H, R* __restrict__ WT) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;  // q = i * H + o
+  if (q < CN * H) WT[q] = W[(size_t)(q % H) * CN + q / H];
 }
 
 template <typename R>
@@ -255,6 +352,19 @@ int ff_vjp_blocks(int ns, int use_double) {
 
 size_t ff_smem_bytes(const FFArgs& a, int use_double) {
   return use_double ? smem_bytes<double>(a) : smem_bytes<float>(a);
+}
+
+void launch_ff_transpose(const FFArgs& a, int use_double, hipStream_t st) {
+  const int CN = a.C * a.n, cnt = CN * a.H, KC = a.K * a.C;
+  const dim3 grid((cnt + 255) / 256);
+  if (use_double)
+    hipLaunchKernelGGL(k_ff_transpose<double>, grid, dim3(256), 0, st,
+                       static_cast<const double*>(a.params) + KC, CN, a.H,
+                       static_cast<double*>(const_cast<void*>(a.wt)));
+  else
+    hipLaunchKernelGGL(k_ff_transpose<float>, grid, dim3(256), 0, st,
+                       static_cast<const float*>(a.params) + KC, CN, a.H,
+                       static_cast<float*>(const_cast<void*>(a.wt)));
 }
 
 int launch_ff(const FFArgs& a, int mode, int use_double, hipStream_t st) {

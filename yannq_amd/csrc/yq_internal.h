@@ -241,6 +241,7 @@ void launch_copy_shift_c(const double* S, double* M, int dim, double shift, hipS
 struct FFArgs {
   int n, C, K, H, ns;
   const void* params;   // device, machine precision
+  const void* wt;       // the FC weight transposed, WT[i * H + o] = W(i, o) (forward reads)
   const int8_t* sigma;  // ns x n
   void* out;            // log psi (ns)
   void* act_out;        // optional stored activations (ns x (2*C*n + H))
@@ -258,6 +259,8 @@ enum { kFFForward = 0, kFFJacobian = 1, kFFVjp = 2 };
 int ff_vjp_blocks(int ns, int use_double);
 size_t ff_smem_bytes(const FFArgs& a, int use_double);
 int launch_ff(const FFArgs& a, int mode, int use_double, hipStream_t st);
+// a.wt from the FC weight of a.params (after every parameter upload)
+void launch_ff_transpose(const FFArgs& a, int use_double, hipStream_t st);
 
 // ------------------------------------------------------------------ AmplitudePhase
 // out[s] = (lp[s] / 2, pi ff[s]): log coeff of psi = sqrt(RBM) exp(i pi FF)
