@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256)
   for (int ct = ct0; ct < ct1; ++ct) {
     doublex4 c = {0.0, 0.0, 0.0, 0.0};
     const int j = ct * 16 + li;
-#pragma unroll 4
+#pragma unroll 8
     for (int q = 0; q < nk; ++q)
       c = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, row0 + li, 4 * q + lg),
                                                v_aug(v, n, m, use_bias, 4 * q + lg, j), c, 0, 0,
@@ -216,6 +216,9 @@ __global__ void __launch_bounds__(256)
   doublex4 acc[kMaxIt64];
 #pragma unroll
   for (int it = 0; it < kMaxIt64; ++it) acc[it] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  // unrolled so that several k-steps' loads are in flight together (the
+  // accumulation order over k is unchanged)
+#pragma unroll 8
   for (int k0 = kc * kChunk; k0 < kc * kChunk + kChunk; k0 += 4) {
     const int s = k0 + lg;
     const double b = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j);
