@@ -287,10 +287,15 @@ YQ_API int yq_sr_step(yq_ctx* ctx, const yq_step_cfg* cfg, yq_step_out* out);
  * seed), as one would run many RunRBM jobs side by side.  Sampling is one
  * persistent launch over every (machine, chain) pair, ordered by cost so the
  * waves stay balanced across uneven sizes; local energies are one launch per
- * pass for all machines; the SR solves run concurrently, one host worker
- * thread and one HIP stream per machine.  Each machine behaves exactly like
- * a single-machine context with the same parameters and seed (the same
- * chains, samples and energies). */
+ * pass for all machines.  With the mixed factored solver (and no warm
+ * start) the SR solves are grouped: every CG iteration is one forward
+ * product, one backward product and one CG step launch for all machines,
+ * each machine's blocks running its own tile shapes, with the FP64
+ * refinement of each machine on its own stream; otherwise the solves run
+ * concurrently, one host worker thread and one HIP stream per machine.  Each
+ * machine behaves exactly like a single-machine context with the same
+ * parameters and seed (the same chains, samples, energies and, grouped or
+ * not, the same SR solution bit for bit). */
 typedef struct {
   int n, m, use_bias;  /* RBM(n, m, useBias), real double parameters */
   int chains;          /* nChainsPer */
@@ -317,6 +322,10 @@ YQ_API int yq_multi_eloc(yq_ctx* ctx, double* mean, double* var);
  * threads (0: one per machine, at most 16). */
 YQ_API int yq_multi_sr_step(yq_ctx* ctx, const yq_step_cfg* cfg, yq_step_out* out,
                             int n_threads);
+/* Grouped SR solves in yq_multi_sr_step on (1, the default) or off (0: one
+ * CG per machine on its own stream).  In the grouped step every machine's
+ * solve_ms is the wall time of the whole SR phase (build, solve, update). */
+YQ_API int yq_multi_set_grouped(yq_ctx* ctx, int on);
 
 /* ---- FeedForward machine: Machines/FeedForward.hpp --------------------- */
 /* The network of the C4 configuration: Conv1D(1 -> channels, kernel width,

@@ -125,3 +125,39 @@ def test_c5_config(ctx):
     print(f"\nC5: 32 machines, {dims} parameters in total, step {1e3 * min(times):.1f} ms "
           f"(sampling {outs[0]['sample_ms']:.2f} ms for all machines in one launch), "
           f"CG iterations {sorted(o['cg_iters'] for o in outs)}")
+
+
+@pytest.mark.parametrize("case", ["default", "jacobi", "autotune", "eager"])
+def test_grouped_solve_matches_per_machine(yq, orc, case):
+    """The grouped SR solve (one forward, backward and CG-step launch per
+    iteration for all machines, each block running its machine's own tile
+    shapes) gives every machine the same solution, bit for bit, as the
+    per-machine solves: n from 16 to 128 (1 to 5 k-steps of 32), uneven chain counts,
+    with the Jacobi preconditioner, with autotuned tiles (other column-group
+    widths and chunks), and without graph capture."""
+    rng = np.random.default_rng(21)
+    shapes = [(120, 240), (16, 16), (100, 400), (37, 70), (64, 128), (128, 128)]
+    ms, rbms = _machines(orc, shapes, [512, 256, 1024, 384, 512, 768], rng, scale=0.02)
+    kw = dict(n_sweeps=1, solver=2, precision=1, lam=1e-3, cg_tol=1e-8, lr=0.02,
+              precond=1 if case == "jacobi" else 0)
+    res = []
+    for grouped in (True, False):
+        with yq.Context(0) as c:
+            c.multi_create(ms, sweeper=0, precision=1)
+            for q, rbm in enumerate(rbms):
+                c.multi_set_params(q, rbm.params())
+            c.multi_set_grouped(grouped)
+            if case == "autotune":
+                c.autotune(True)
+            if case == "eager":
+                c.graph_mode(0)
+            outs = [c.multi_sr_step(n_discard=10 if i == 0 else 0, **kw) for i in range(2)]
+            res.append((outs, [c.multi_get_params(q) for q in range(len(ms))]))
+    (og, pg), (op, pp) = res
+    for i in range(2):
+        for q in range(len(ms)):
+            for key in ("energy", "cg_iters", "cg_err", "update_norm"):
+                assert og[i][q][key] == op[i][q][key], (i, q, key)
+            assert og[i][q]["cg_err"] < 1e-8
+    for q in range(len(ms)):
+        np.testing.assert_array_equal(pg[q], pp[q])

@@ -103,7 +103,7 @@ def lib():
         "yq_multi_get_params": [p, i, p], "yq_multi_init_random": [p, C.c_uint64, d],
         "yq_multi_sample": [p, i, i], "yq_multi_samples_get": [p, i, p, p],
         "yq_multi_sampler_state": [p, i, p, p, p], "yq_multi_eloc": [p, p, p],
-        "yq_multi_sr_step": [p, p, p, i],
+        "yq_multi_sr_step": [p, p, p, i], "yq_multi_set_grouped": [p, i],
         "yq_ff_create": [p, p], "yq_ff_npar": [p, p], "yq_ff_set_params": [p, p],
         "yq_ff_get_params": [p, p], "yq_ff_update_params": [p, p],
         "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
@@ -501,6 +501,11 @@ class Context:
         mean, var = np.empty(k), np.empty(k)
         self._chk(self._L.yq_multi_eloc(self._h, _ptr(mean), _ptr(var)))
         return mean, var
+
+    def multi_set_grouped(self, on=True):
+        """Grouped SR solves in multi_sr_step (one launch per kernel and CG
+        iteration for all machines) on or off."""
+        self._chk(self._L.yq_multi_set_grouped(self._h, 1 if on else 0))
 
     def multi_sr_step(self, n_threads=0, **kw):
         cfg = StepCfg(kw.get("n_sweeps", 1), kw.get("n_discard", 0), int(kw.get("randomize", 0)),

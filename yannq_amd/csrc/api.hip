@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -129,6 +131,9 @@ struct yq_ctx {
   // many machines (yq_multi_*): one sub-context per machine
   std::vector<yq_ctx*> subs;
   int multi_kind = 0, multi_prec = 0, m_nitems = 0, m_npad = 16;
+  // the machines' inner CG in one set of launches per iteration (mixed SR)
+  yq::MixedGroup group;
+  bool grouped = true;
   Buf m_sargs, m_items, m_eargs, m_stats;
 
   // FeedForward machine
@@ -1057,6 +1062,7 @@ YQ_API int yq_ctx_destroy(yq_ctx* c) {
   c->subs.clear();
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
+  mixed_group_release(c->group);
   mixed_release(c->mixed);
   for (Buf* b : c->bufs)
     if (b->p) (void)hipFree(b->p);
@@ -2119,6 +2125,12 @@ YQ_API int yq_set_graph_mode(yq_ctx* c, int use_graphs) {
   const int chunk = use_graphs > 1 ? use_graphs : 8;
   c->mixed.use_graphs = use_graphs != 0;
   c->mixed.chunk_iters = chunk;
+  if (c->group.chunk_iters != chunk && c->group.gexec) {  // a new capture
+    (void)hipGraphExecDestroy(c->group.gexec);
+    c->group.gexec = nullptr;
+  }
+  c->group.use_graphs = use_graphs != 0;
+  c->group.chunk_iters = chunk;
   for (yq_ctx* sub : c->subs) {
     sub->mixed.use_graphs = use_graphs != 0;
     sub->mixed.chunk_iters = chunk;
@@ -2164,6 +2176,7 @@ namespace {
 // constructFromSamples + solve + SGD update of RunRBM.hpp:176-212 on the
 // current batch (E_loc already computed); *solve_rc keeps a
 // YQ_ERR_NOT_CONVERGED of the solve while the update still happens.
+int step_apply(yq_ctx* c, const yq_step_cfg* cfg, double* nv);
 int step_update(yq_ctx* c, const yq_step_cfg* cfg, double* nv, double* err, int* it,
                 int* solve_rc) {
   TRY(sr_build(c, cfg->solver, cfg->precision));
@@ -2173,6 +2186,12 @@ int step_update(yq_ctx* c, const yq_step_cfg* cfg, double* nv, double* err, int*
                  c->vx.as<double>(), err, it);
   if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
   *solve_rc = rc;
+  return step_apply(c, cfg, nv);
+}
+// params -= lr x for the solution x in vx (|x| clipped to grad_clip), then the
+// derived parameter copies; nv = |x|
+int step_apply(yq_ctx* c, const yq_step_cfg* cfg, double* nv) {
+  const size_t len = srlen(c);
   double vv;
   TRY(dot(c, c->vx.as<double>(), c->vx.as<double>(), len, &vv));
   *nv = sqrt(vv);
@@ -2416,6 +2435,193 @@ YQ_API int yq_multi_eloc(yq_ctx* c, double* mean, double* var) {
   return YQ_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// fn(i) for i in [0, count) on up to n_threads host threads (each with the
+// context's device current); the statuses land in rcs
+void parallel_for(yq_ctx* c, int count, int n_threads, std::vector<int>& rcs,
+                  const std::function<int(int)>& fn) {
+  rcs.assign(count, YQ_OK);
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    (void)hipSetDevice(c->device);
+    for (int i = next++; i < count; i = next++) rcs[i] = fn(i);
+  };
+  const int nt = std::max(1, std::min(count, n_threads > 0 ? n_threads : 16));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  (void)hipSetDevice(c->device);
+}
+// first failing status (machine index in idx), or YQ_OK
+int first_fail(const std::vector<int>& rcs, int* idx) {
+  for (size_t i = 0; i < rcs.size(); ++i)
+    if (rcs[i] != YQ_OK) {
+      *idx = (int)i;
+      return rcs[i];
+    }
+  return YQ_OK;
+}
+
+// The SR solve of every machine (real RBMs, mixed factored solver, cold
+// start): each machine runs the FP64 refinement of solve_mixed (the same
+// rounds, inner tolerances and budgets), and the float inner CG of all the
+// machines in a round runs grouped (mixed_cg_group), so each machine gets the
+// x that solve_mixed would give it.  Per machine: err, iters and rc (YQ_OK or
+// YQ_ERR_NOT_CONVERGED) in the outputs.
+int solve_mixed_group(yq_ctx* c, const yq_step_cfg* cfg, int n_threads,
+                      std::vector<double>& errs, std::vector<int>& iters,
+                      std::vector<int>& rcs) {
+  const int count = (int)c->subs.size();
+  const double shift = cfg->lambda, tol = cfg->cg_tol;
+  const int max_iter = cfg->cg_max_iter;
+  if (tol <= 0 || max_iter <= 0) return fail(c, YQ_ERR_ARG, "tol and max_iter must be > 0");
+  if (!(shift >= 0.0)) return fail(c, YQ_ERR_ARG, "shift must be >= 0");
+  std::vector<double> bn2(count, 0.0), rel(count, 1.0);
+  std::vector<int> total(count, 0);
+  std::vector<char> done(count, 0);
+  std::vector<int> st;
+  int bad = 0, rc;
+  // start: r = b, x = 0 (or x = 0 and done when b = 0)
+  parallel_for(c, count, n_threads, st, [&](int i) -> int {
+    yq_ctx* s = c->subs[i];
+    const size_t dim = s->dim;
+    TRY(ensure(s, s->vr, sizeof(double) * dim));
+    TRY(ensure(s, s->vAp, sizeof(double) * dim));
+    TRY(ensure(s, s->vp, sizeof(double) * dim));
+    const double* b = s->F.as<double>();
+    TRY(dot(s, b, b, dim, &bn2[i]));
+    HIPCHK(s, hipMemsetAsync(s->vx.p, 0, sizeof(double) * dim, s->st));
+    if (bn2[i] == 0.0) {
+      done[i] = 1;
+      rel[i] = 0.0;
+    } else {
+      HIPCHK(s, hipMemcpyAsync(s->vr.p, b, sizeof(double) * dim, hipMemcpyDeviceToDevice,
+                               s->st));
+    }
+    return sync(s);
+  });
+  if ((rc = first_fail(st, &bad)) != YQ_OK) return from_sub(c, c->subs[bad], rc, bad);
+  std::vector<yq::GroupJob> jobs(count);
+  for (int outer = 0; outer < 30; ++outer) {
+    bool any = false;
+    for (int i = 0; i < count; ++i) {
+      yq_ctx* s = c->subs[i];
+      yq::GroupJob& j = jobs[i];
+      j.w = &s->mixed;
+      j.omean64 = s->omean.as<double>();
+      j.diag64 = s->precond ? s->diagS.as<double>() : nullptr;
+      j.b = s->vr.as<double>();
+      j.x = s->vp.as<double>();
+      j.active = !done[i] && total[i] < max_iter && !(rel[i] < tol);
+      if (!j.active) continue;
+      any = true;
+      const double t = std::max(tol / rel[i], 1e-4);
+      j.tol = t < 1.0 ? t : 0.5;
+      j.max_iter = max_iter - total[i];
+    }
+    if (!any) break;
+    rc = mixed_cg_group(c->group, jobs, shift, c->st, c->err);
+    if (rc != YQ_OK) return fail(c, rc, c->err);
+    // x += d and the FP64 residual of each machine that iterated
+    parallel_for(c, count, n_threads, st, [&](int i) -> int {
+      const yq::GroupJob& j = jobs[i];
+      if (!j.active) return YQ_OK;
+      yq_ctx* s = c->subs[i];
+      const size_t dim = s->dim;
+      total[i] += j.iters;
+      double* x = s->vx.as<double>();
+      double* r = s->vr.as<double>();
+      double* Ax = s->vAp.as<double>();
+      launch_axpy(1.0, s->vp.as<double>(), x, dim, s->st);
+      LAUNCHCHK(s);
+      TRY(apply64(s, shift, x, Ax));
+      HIPCHK(s, hipMemcpyAsync(r, s->F.p, sizeof(double) * dim, hipMemcpyDeviceToDevice, s->st));
+      launch_axpy(-1.0, Ax, r, dim, s->st);
+      LAUNCHCHK(s);
+      double rn2;
+      TRY(dot(s, r, r, dim, &rn2));
+      rel[i] = sqrt(rn2 / bn2[i]);
+      if (rel[i] < tol) done[i] = 1;
+      return YQ_OK;
+    });
+    if ((rc = first_fail(st, &bad)) != YQ_OK) return from_sub(c, c->subs[bad], rc, bad);
+  }
+  errs.assign(count, 0.0);
+  iters.assign(count, 0);
+  rcs.assign(count, YQ_OK);
+  for (int i = 0; i < count; ++i) {
+    errs[i] = rel[i];
+    iters[i] = total[i];
+    rcs[i] = rel[i] < tol ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+  }
+  return YQ_OK;
+}
+
+// yq_multi_sr_step after sampling and E_loc with the grouped SR solve: build
+// (per machine, host threads), grouped solve, update (host threads)
+int multi_step_grouped(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out, int n_threads,
+                       float smp) {
+  const int count = (int)c->subs.size();
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> st;
+  int bad = 0, rc;
+  parallel_for(c, count, n_threads, st, [&](int i) -> int {
+    yq_ctx* s = c->subs[i];
+    TRY(sr_build(s, cfg->solver, cfg->precision));
+    TRY(ensure(s, s->vx, sizeof(double) * srlen(s)));
+    return sync(s);
+  });
+  if ((rc = first_fail(st, &bad)) != YQ_OK) return from_sub(c, c->subs[bad], rc, bad);
+  std::vector<double> errs;
+  std::vector<int> iters, srcs;
+  TRY(solve_mixed_group(c, cfg, n_threads, errs, iters, srcs));
+  std::vector<double> nvs(count, 0.0);
+  parallel_for(c, count, n_threads, st, [&](int i) -> int {
+    yq_ctx* s = c->subs[i];
+    // the solve's bookkeeping, as solve() keeps it
+    TRY(ensure(s, s->xprev, sizeof(double) * srlen(s)));
+    HIPCHK(s, hipMemcpyAsync(s->xprev.p, s->vx.p, sizeof(double) * srlen(s),
+                             hipMemcpyDeviceToDevice, s->st));
+    s->have_xprev = true;
+    TRY(step_apply(s, cfg, &nvs[i]));
+    TRY(sync(s));
+    if (!std::isfinite(nvs[i]) || !std::isfinite(s->e_mean))
+      return fail(s, YQ_ERR_NAN, "non-finite energy or update");
+    return YQ_OK;
+  });
+  if ((rc = first_fail(st, &bad)) != YQ_OK) return from_sub(c, c->subs[bad], rc, bad);
+  const float slv =
+      std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  int worst = YQ_OK;
+  for (int i = 0; i < count; ++i) {
+    yq_ctx* s = c->subs[i];
+    yq_step_out& o = out[i];
+    o.energy = s->e_mean;
+    o.energy_var = s->e_var;
+    o.update_norm = nvs[i];
+    o.cg_err = errs[i];
+    o.cg_iters = iters[i];
+    o.n_samples = s->ns;
+    o.sample_ms = smp;
+    o.solve_ms = slv;
+    if (srcs[i] == YQ_ERR_NOT_CONVERGED) worst = srcs[i];
+  }
+  if (worst != YQ_OK) return fail(c, worst, "an iterative solve reached max_iter");
+  return YQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+YQ_API int yq_multi_set_grouped(yq_ctx* c, int on) {
+  TRY(need_multi(c));
+  c->grouped = on != 0;
+  return YQ_OK;
+}
+
 YQ_API int yq_multi_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out,
                             int n_threads) {
   TRY(need_multi(c));
@@ -2439,6 +2645,9 @@ YQ_API int yq_multi_sr_step(yq_ctx* c, const yq_step_cfg* cfg, yq_step_out* out,
   (void)hipEventDestroy(e1);
   TRY(rc);
   TRY(multi_eloc(c));
+  if (c->grouped && cfg->solver == YQ_SOLVE_CG_FACTORED && cfg->precision == YQ_MIXED &&
+      !cfg->warm_start)
+    return multi_step_grouped(c, cfg, out, n_threads, smp);
   // per-machine SR on worker threads, costliest machines first
   const std::vector<int> ord = cost_order(c);
   std::vector<int> rcs(count, YQ_OK);

@@ -66,6 +66,47 @@ int mixed_apply64(MixedWork& w, double shift, const double* omean64,
 int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* diag64,
              const double* b, double* x, double tol, int max_iter, int* iters,
              hipStream_t st, std::string& err);
+// One machine's inner solve in a grouped call: (S + shift) x = b to the
+// relative residual tol within max_iter iterations, as mixed_cg does; jobs
+// that are not active are left alone.
+struct GroupJob {
+  MixedWork* w = nullptr;         // prepared by mixed_prepare
+  const double* omean64 = nullptr;
+  const double* diag64 = nullptr;  // Jacobi preconditioner, or null
+  const double* b = nullptr;
+  double* x = nullptr;
+  double tol = 0.0;
+  int max_iter = 0;
+  bool active = false;
+  int iters = 0;   // out: completed iterations
+  int rc = 0;      // out: YQ_OK or YQ_ERR_NOT_CONVERGED
+};
+// Device tables, poll buffers and the captured chunk of a grouped CG.
+struct MixedGroup {
+  int cap = 0, cap_blocks = 0;
+  void* desc = nullptr;       // per-machine descriptors (device)
+  void* host_desc = nullptr;  // pinned staging
+  int* btab = nullptr;        // block tables of the three grouped kernels (int2)
+  int* host_btab = nullptr;
+  int* ctl = nullptr;         // 2 slots x 4 control ints per machine
+  int* host_ctl = nullptr;
+  double* stat = nullptr;     // per machine: <b, b>, final <r, r>, iterations
+  double* host_stat = nullptr;
+  int nf = 0, nbw = 0, nc = 0;  // blocks of the forward, backward and step launches
+  size_t lds_f = 0, lds_b = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool use_graphs = true;
+  int chunk_iters = 8;
+  hipGraphExec_t gexec = nullptr;
+  long gkey[6] = {0, 0, 0, 0, 0, 0};
+};
+// The inner CG of every active job in one set of launches per iteration (a
+// grouped forward product, backward product and CG step whose grids
+// concatenate the machines' own); each machine computes exactly what
+// mixed_cg computes for it.  Synchronises st before returning.
+int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift,
+                   hipStream_t st, std::string& err);
+void mixed_group_release(MixedGroup& g);
 void mixed_release(MixedWork& w);
 // drop the captured CG chunks (after a reallocation or a shape change)
 void mixed_drop_graphs(MixedWork& w);

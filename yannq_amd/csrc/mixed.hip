@@ -19,7 +19,9 @@
 // order.
 #include <math.h>
 
+#include <algorithm>
 #include <array>
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -204,25 +206,27 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
 // global load is issued as early as the registers allow: the T' of up to 4
 // column tiles is requested before the products that need it.
 constexpr int kLdsPad = 8;
+// (bx, by): the block's place in the machine's own (nsp / 256) x ncg grid
 template <int CT, int NKS>
-__global__ void __launch_bounds__(256)
-    k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
-               const float* __restrict__ Trm, int Kp, int Np, int nsp,
-               float* __restrict__ ypart, int* __restrict__ ictl, int count) {
+__device__ __forceinline__ void fwd_body(int bx, int by, const uint16_t* __restrict__ sigA,
+                                         const uint16_t* __restrict__ vs,
+                                         const float* __restrict__ Trm, int Kp, int Np, int nsp,
+                                         float* __restrict__ ypart, int* __restrict__ ictl,
+                                         int count) {
   extern __shared__ uint16_t bsh[];  // [3][CT*16][Kp + kLdsPad]
   if (ictl[kCtlStop]) return;
   const int tid = threadIdx.x;
   // count: this product belongs to a CG iteration whose update ran (one
   // writer; the next update reads the counter)
-  if (count && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) ictl[kCtlIters] += 1;
-  const int cg = blockIdx.y;
+  if (count && bx == 0 && by == 0 && tid == 0) ictl[kCtlIters] += 1;
+  const int cg = by;
   const int col0 = cg * CT * 16;
   const int ncols = min(CT * 16, Np - col0);
   const int ldk = Kp + kLdsPad;
   const int kchunks = Kp / 8;
   const int wave = tid >> 6, l = tid & 63;
   const int li = l & 15, lg = l >> 4;
-  const int rbase = blockIdx.x * 256 + wave * 64;
+  const int rbase = bx * 256 + wave * 64;
   // The V' slab goes through registers into LDS in two passes (at most 16
   // 16-byte pieces per thread in all, the LDS budget of the slab), and this
   // wave's sigma' fragments are issued between the first pass's loads and its
@@ -318,6 +322,13 @@ __global__ void __launch_bounds__(256)
     if (lg == 0) ypart[(size_t)cg * nsp + rbase + st * 16 + li] = v;
   }
 }
+template <int CT, int NKS>
+__global__ void __launch_bounds__(256)
+    k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
+               const float* __restrict__ Trm, int Kp, int Np, int nsp,
+               float* __restrict__ ypart, int* __restrict__ ictl, int count) {
+  fwd_body<CT, NKS>(blockIdx.x, blockIdx.y, sigA, vs, Trm, Kp, Np, nsp, ypart, ictl, count);
+}
 
 // forward kernel for (CT, Kp / 32)
 template <int CT>
@@ -356,19 +367,24 @@ __device__ inline int xcd_block(int b, int total) {
   return (b % kXcds) * per + b / kXcds;
 }
 
+// (b, gx, total): the block's linear index in the machine's own
+// ((Np / 16 + 3) / 4) x chunks grid, the grid's x extent and its size
 template <int CHUNK, int SLICE>
-__global__ void __launch_bounds__(256)
-    k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
-               const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
-               int nop, int Kp, int Np, int nsp, float* __restrict__ part,
-               double* __restrict__ ys, double* __restrict__ yy, const int* __restrict__ ictl) {
+__device__ __forceinline__ void bwd_body(int b, int gx, int total,
+                                         const uint16_t* __restrict__ sigT,
+                                         const float* __restrict__ Tt,
+                                         const float* __restrict__ ypart, int ncg, int ns,
+                                         const double* __restrict__ op, int nop, int Kp, int Np,
+                                         int nsp, float* __restrict__ part,
+                                         double* __restrict__ ys, double* __restrict__ yy,
+                                         const int* __restrict__ ictl) {
   if (ictl[kCtlStop]) return;
   extern __shared__ uint16_t ash[];  // [Kp][SLICE + kLdsPad]
   __shared__ float ysh[CHUNK];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
-  const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-  const int cx = lb % gridDim.x, kc = lb / gridDim.x;
+  const int lb = xcd_block(b, total);
+  const int cx = lb % gx, kc = lb / gx;
   const int ct = cx * 4 + wave;
   const bool active = ct < Np / 16;
   const int li = l & 15, lg = l >> 4;
@@ -484,6 +500,15 @@ __global__ void __launch_bounds__(256)
       for (int q = 0; q < 4; ++q)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
+template <int CHUNK, int SLICE>
+__global__ void __launch_bounds__(256)
+    k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
+               const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
+               int nop, int Kp, int Np, int nsp, float* __restrict__ part,
+               double* __restrict__ ys, double* __restrict__ yy, const int* __restrict__ ictl) {
+  bwd_body<CHUNK, SLICE>(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.x * gridDim.y,
+                         sigT, Tt, ypart, ncg, ns, op, nop, Kp, Np, nsp, part, ys, yy, ictl);
+}
 
 // backward kernel for a chunk length; slices of 64 samples (longer slices,
 // with fewer latency steps but less overlap, measured slower at n = 100,
@@ -519,8 +544,9 @@ auto bwd_kernel(int chunk) {
 // per thread; combined as (g0 + g1) + (g2 + g3)); group 0 then updates the
 // vectors, read along j (consecutive parameters), and writes V' along k
 // through an LDS transpose.
-__global__ void __launch_bounds__(1024)
-    k_cg_step(float* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
+// (b, gx, total): the block's linear index in the machine's own
+// (Np / 16) x (Kp / 16) grid, the grid's x extent and its size
+__device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
               float* __restrict__ s, float* __restrict__ u, float* __restrict__ out,
               const float* __restrict__ dinv, int mode, int par,
               const double* __restrict__ g_in, const double* __restrict__ rr_in,
@@ -536,8 +562,8 @@ __global__ void __launch_bounds__(1024)
   // a tile row of the chunk partials is 64 bytes, half an L2 line: the
   // XCD-aware numbering puts tiles (2i, 2i + 1) of a row on one XCD so that
   // the line is fetched from HBM once
-  const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-  const int j0 = (lb % gridDim.x) * 16, k0 = (lb / gridDim.x) * 16;
+  const int lb = xcd_block(b, total);
+  const int j0 = (lb % gx) * 16, k0 = (lb / gx) * 16;
   const size_t nm = (size_t)n * m;
   const int kk = e >> 4, jj = e & 15;
   const int k = k0 + kk, j = j0 + jj;
@@ -642,6 +668,20 @@ __global__ void __launch_bounds__(1024)
     for (int sp = 0; sp < 3; ++sp)
       vs[((size_t)sp * Np + j0 + tj) * Kp + k0 + tk] = tile[sp][tj][tk];
   }
+}
+__global__ void __launch_bounds__(1024)
+    k_cg_step(float* __restrict__ x, float* __restrict__ r, float* __restrict__ p,
+              float* __restrict__ s, float* __restrict__ u, float* __restrict__ out,
+              const float* __restrict__ dinv, int mode, int par,
+              const double* __restrict__ g_in, const double* __restrict__ rr_in,
+              const double* __restrict__ uu_in, int nv, const double* __restrict__ ys,
+              const double* __restrict__ yy, int nys, int ns, double* __restrict__ g_out,
+              double* __restrict__ rr_out, double* __restrict__ uu_out, double* __restrict__ op,
+              double* dctl, int* ictl, const float* __restrict__ part, int chunks,
+              const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
+              uint16_t* __restrict__ vs) {
+  cg_step_body(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.x * gridDim.y,
+               x, r, p, s, u, out, dinv, mode, par, g_in, rr_in, uu_in, nv, ys, yy, nys, ns, g_out, rr_out, uu_out, op, dctl, ictl, part, chunks, om, n, m, use_bias, Kp, Np, vs);
 }
 
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
@@ -758,6 +798,166 @@ void apply_launches(MixedWork& w, hipStream_t st) {
 void iteration_launches(MixedWork& w, const float* dinv, int par, bool first, hipStream_t st) {
   step_launch(w, dinv, first ? 1 : 2, par, st);
   product_launches(w, 1, st);
+}
+
+// ---------------------------------------------------------------- grouped CG
+// The inner CG of several machines in one set of launches per iteration: each
+// grouped kernel's grid is the concatenation of the machines' own grids, a
+// block table maps a block to (machine, block of the machine's own grid), and
+// the block runs that machine's kernel body with the machine's own tile
+// shapes, so every machine's arithmetic is that of its own launches.  A
+// machine that is not iterating has its stop flag set and its blocks return
+// at once.
+struct GroupDesc {
+  const uint16_t* sigA;
+  const uint16_t* sigT;
+  const float* Trm;
+  const float* Tt;
+  const float* dinv;     // w.dinv when preconditioned, else null
+  const double* om64;    // FP64 column mean of O
+  const double* b64;     // right-hand side
+  const double* diag64;  // diag S (Jacobi), or null
+  double* x64;           // solution
+  uint16_t* vs;
+  float *ypart, *part, *om, *x, *r, *p, *s, *u, *Ap, *dinv_w;
+  double *red, *dctl;
+  int* ictl;
+  long long dim;
+  long long g[2], rr[2], uu[2], op, ys, yy;  // offsets into red (red_layout)
+  double tol;            // relative tolerance of this call
+  int ns, n, m, use_bias, Kp, Np, nsp, ncg, ct, chunk, nbv, nby;
+  int fwd_gx, bwd_gx, bwd_total, cg_gx, cg_total;
+  int in;                // takes part in this call
+  int max_it;            // iteration budget of this call
+};
+
+template <int CT>
+__device__ __forceinline__ void fwd_dispatch(int nks, int bx, int by, const GroupDesc& d,
+                                             int count) {
+  switch (nks) {
+    case 1: fwd_body<CT, 1>(bx, by, d.sigA, d.vs, d.Trm, d.Kp, d.Np, d.nsp, d.ypart, d.ictl, count); break;
+    case 2: fwd_body<CT, 2>(bx, by, d.sigA, d.vs, d.Trm, d.Kp, d.Np, d.nsp, d.ypart, d.ictl, count); break;
+    case 3: fwd_body<CT, 3>(bx, by, d.sigA, d.vs, d.Trm, d.Kp, d.Np, d.nsp, d.ypart, d.ictl, count); break;
+    case 4: fwd_body<CT, 4>(bx, by, d.sigA, d.vs, d.Trm, d.Kp, d.Np, d.nsp, d.ypart, d.ictl, count); break;
+    default: fwd_body<CT, 5>(bx, by, d.sigA, d.vs, d.Trm, d.Kp, d.Np, d.nsp, d.ypart, d.ictl, count); break;
+  }
+}
+// bt[block] = (machine, linear block of its (nsp / 256) x ncg grid)
+__global__ void __launch_bounds__(256)
+    k_fwd_group(const GroupDesc* __restrict__ gd, const int2* __restrict__ bt, int count) {
+  const int2 e = bt[blockIdx.x];
+  const GroupDesc& d = gd[e.x];
+  const int bx = e.y % d.fwd_gx, by = e.y / d.fwd_gx;
+  const int nks = d.Kp / 32;
+  if (d.ct == 2) fwd_dispatch<2>(nks, bx, by, d, count);
+  else if (d.ct == 8) fwd_dispatch<8>(nks, bx, by, d, count);
+  else fwd_dispatch<4>(nks, bx, by, d, count);
+}
+// bt[block] = (machine, linear block of its ((Np / 16 + 3) / 4) x chunks grid)
+__global__ void __launch_bounds__(256)
+    k_bwd_group(const GroupDesc* __restrict__ gd, const int2* __restrict__ bt) {
+  const int2 e = bt[blockIdx.x];
+  const GroupDesc& d = gd[e.x];
+#define YQ_BWD_ARGS                                                                            \
+  e.y, d.bwd_gx, d.bwd_total, d.sigT, d.Tt, d.ypart, d.ncg, d.ns, d.red + d.op, d.nbv, d.Kp,   \
+      d.Np, d.nsp, d.part, d.red + d.ys, d.red + d.yy, d.ictl
+  if (d.chunk == 128) bwd_body<128, kBwdSlice>(YQ_BWD_ARGS);
+  else if (d.chunk == 512) bwd_body<512, kBwdSlice>(YQ_BWD_ARGS);
+  else bwd_body<256, kBwdSlice>(YQ_BWD_ARGS);
+#undef YQ_BWD_ARGS
+}
+// bt[block] = (machine, linear block of its (Np / 16) x (Kp / 16) grid)
+__global__ void __launch_bounds__(1024)
+    k_cg_group(const GroupDesc* __restrict__ gd, const int2* __restrict__ bt, int mode, int par) {
+  const int2 e = bt[blockIdx.x];
+  const GroupDesc& d = gd[e.x];
+  if (d.ictl[kCtlStop]) return;  // not iterating (uniform per block)
+  const int wr = mode == 0 ? 0 : par ^ 1;
+  double* red = d.red;
+  cg_step_body(e.y, d.cg_gx, d.cg_total, d.x, d.r, d.p, d.s, d.u, d.Ap, d.dinv, mode, par,
+               red + d.g[par], red + d.rr[par], red + d.uu[par], d.nbv, red + d.ys, red + d.yy,
+               d.nby, d.ns, red + d.g[wr], red + d.rr[wr], red + d.uu[wr], red + d.op, d.dctl,
+               d.ictl, d.part, d.nsp / d.chunk, d.om, d.n, d.m, d.use_bias, d.Kp, d.Np, d.vs);
+}
+// start of a grouped call (blockIdx.y = machine): the float copies of the
+// column mean and of b, x = 0, the Jacobi inverse, and the control block
+// reset as for a product (machines not taking part: stop flag set)
+__global__ void k_group_start(const GroupDesc* __restrict__ gd, double shift) {
+  const GroupDesc& d = gd[blockIdx.y];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!d.in) {
+    if (lead) d.ictl[kCtlStop] = 1;
+    return;
+  }
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.dim) {
+    d.om[i] = (float)d.om64[i];
+    d.r[i] = (float)d.b64[i];
+    d.x[i] = 0.0f;
+    if (d.diag64) d.dinv_w[i] = (float)(1.0 / (d.diag64[i] + shift));
+  }
+  if (lead) {
+    d.dctl[kCtlShift] = shift;
+    d.dctl[kCtlThr] = 0.0;
+    d.ictl[kCtlStop] = 0;
+    d.ictl[kCtlIters] = 0;
+    d.ictl[kCtlMaxIt] = 1;
+  }
+}
+// sum of the <r, r> partials of slot sl in block order, as the host sums them
+__device__ inline double rr_total(const GroupDesc& d, int sl) {
+  const double* a = d.red + d.rr[sl];
+  double s = 0.0;
+  for (int i0 = 0; i0 < d.nbv; i0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = i0 + q < d.nbv ? a[i0 + q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (i0 + q < d.nbv) s += v[q];
+  }
+  return s;
+}
+// one thread per machine.  which 0, after the start products: stat[q] =
+// <b, b>, and the control block of the iterations (threshold tol^2 <b, b>,
+// budget; b = 0 stops at once with x = 0).  which 1, at the end: stat[count +
+// q] = <r, r> after the completed iterations, stat[2 count + q] = their count.
+__global__ void k_group_rr(const GroupDesc* __restrict__ gd, int count, int which,
+                           double* __restrict__ stat) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  const GroupDesc& d = gd[q];
+  if (!d.in) return;
+  if (which == 0) {
+    const double bn2 = rr_total(d, 0);
+    stat[q] = bn2;
+    if (bn2 == 0.0) {
+      d.ictl[kCtlStop] = 1;
+    } else {
+      d.dctl[kCtlThr] = d.tol * d.tol * bn2;
+      d.ictl[kCtlStop] = 0;
+      d.ictl[kCtlIters] = 0;
+      d.ictl[kCtlMaxIt] = d.max_it;
+    }
+  } else {
+    const int it = d.ictl[kCtlIters];
+    stat[count + q] = rr_total(d, it & 1);
+    stat[2 * count + q] = (double)it;
+  }
+}
+// the control blocks of all machines, 4 ints each
+__global__ void k_group_ctl(const GroupDesc* __restrict__ gd, int count, int* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[4 * q + i] = gd[q].ictl[i];
+}
+// x64 = x for the machines taking part (blockIdx.y = machine)
+__global__ void k_group_finish(const GroupDesc* __restrict__ gd) {
+  const GroupDesc& d = gd[blockIdx.y];
+  if (!d.in) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.dim) d.x64[i] = (double)d.x[i];
 }
 
 // shape of the captured launches (a different shape needs a new capture)
@@ -1045,6 +1245,258 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   *iters = it;
   return conv ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+}
+
+namespace {
+// the three grouped launches of one CG product / step
+void group_products(const MixedGroup& g, int count, hipStream_t st) {
+  const GroupDesc* gd = static_cast<const GroupDesc*>(g.desc);
+  const int2* bt = reinterpret_cast<const int2*>(g.btab);
+  hipLaunchKernelGGL(k_fwd_group, dim3(g.nf), dim3(256), g.lds_f, st, gd, bt, count);
+  hipLaunchKernelGGL(k_bwd_group, dim3(g.nbw), dim3(256), g.lds_b, st, gd, bt + g.nf);
+}
+void group_step(const MixedGroup& g, int mode, int par, hipStream_t st) {
+  const GroupDesc* gd = static_cast<const GroupDesc*>(g.desc);
+  const int2* bt = reinterpret_cast<const int2*>(g.btab);
+  hipLaunchKernelGGL(k_cg_group, dim3(g.nc), dim3(1024), 0, st, gd, bt + g.nf + g.nbw, mode,
+                     par);
+}
+void group_iteration(const MixedGroup& g, int par, bool first, hipStream_t st) {
+  group_step(g, first ? 1 : 2, par, st);
+  group_products(g, 1, st);
+}
+}  // namespace
+
+int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hipStream_t st,
+                   std::string& err) {
+  const int count = (int)jobs.size();
+  int maxit = 0;
+  for (const GroupJob& j : jobs)
+    if (j.active) maxit = std::max(maxit, j.max_iter);
+  for (GroupJob& j : jobs) {
+    j.iters = 0;
+    j.rc = YQ_OK;
+  }
+  if (count == 0 || maxit <= 0) return YQ_OK;
+  // descriptor and block tables (all machines, so the grids do not depend on
+  // which machines iterate in this call)
+  std::vector<GroupDesc> desc(count);
+  std::vector<int2> bf, bb, bc;
+  size_t lds_f = 0, lds_b = 0;
+  long long maxdim = 0;
+  for (int q = 0; q < count; ++q) {
+    const GroupJob& j = jobs[q];
+    MixedWork& w = *j.w;
+    if (w.ncg != (w.Np / 16 + w.ct - 1) / w.ct || w.Kp > 32 * kMaxKs || w.Kp % 32 != 0) {
+      err = "grouped CG: machine " + std::to_string(q) + " is not prepared";
+      return YQ_ERR_STATE;
+    }
+    const RedLayout L = red_layout(w);
+    GroupDesc d{};
+    d.sigA = w.sigA;
+    d.sigT = w.sigT;
+    d.Trm = w.Trm;
+    d.Tt = w.Tt;
+    d.dinv = j.diag64 ? w.dinv : nullptr;
+    d.om64 = j.omean64;
+    d.b64 = j.b;
+    d.diag64 = j.diag64;
+    d.x64 = j.x;
+    d.vs = w.vsplit;
+    d.ypart = w.ypart;
+    d.part = w.part;
+    d.om = w.om32;
+    d.x = w.x;
+    d.r = w.r;
+    d.p = w.p;
+    d.s = w.s;
+    d.u = w.u;
+    d.Ap = w.Ap;
+    d.dinv_w = w.dinv;
+    d.red = w.red;
+    d.dctl = w.dctl;
+    d.ictl = w.ictl;
+    d.dim = (long long)w.dim;
+    for (int k = 0; k < 2; ++k) {
+      d.g[k] = (long long)L.g[k];
+      d.rr[k] = (long long)L.rr[k];
+      d.uu[k] = (long long)L.uu[k];
+    }
+    d.op = (long long)L.op;
+    d.ys = (long long)L.ys;
+    d.yy = (long long)L.yy;
+    d.tol = j.tol;
+    d.ns = w.ns;
+    d.n = w.n;
+    d.m = w.m;
+    d.use_bias = w.use_bias;
+    d.Kp = w.Kp;
+    d.Np = w.Np;
+    d.nsp = w.nsp;
+    d.ncg = w.ncg;
+    d.ct = w.ct;
+    d.chunk = w.chunk;
+    d.nbv = L.nbv;
+    d.nby = L.nby;
+    d.fwd_gx = w.nsp / 256;
+    d.bwd_gx = (w.Np / 16 + 3) / 4;
+    d.bwd_total = d.bwd_gx * (w.nsp / w.chunk);
+    d.cg_gx = w.Np / 16;
+    d.cg_total = d.cg_gx * (w.Kp / 16);
+    d.in = j.active ? 1 : 0;
+    d.max_it = j.max_iter;
+    desc[q] = d;
+    for (int b = 0; b < d.fwd_gx * w.ncg; ++b) bf.push_back(make_int2(q, b));
+    for (int b = 0; b < d.bwd_total; ++b) bb.push_back(make_int2(q, b));
+    for (int b = 0; b < d.cg_total; ++b) bc.push_back(make_int2(q, b));
+    lds_f = std::max(lds_f, sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad));
+    lds_b = std::max(lds_b, sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad));
+    maxdim = std::max(maxdim, d.dim);
+  }
+  const int nblk = (int)(bf.size() + bb.size() + bc.size());
+  hipError_t e = hipSuccess;
+  if (count > g.cap || nblk > g.cap_blocks) {
+    mixed_group_release(g);
+    const int cap = std::max(count, 1), capb = std::max(nblk, 1);
+    void* p = nullptr;
+    if ((e = hipMalloc(&g.desc, sizeof(GroupDesc) * cap)) != hipSuccess ||
+        (e = hipHostMalloc(&g.host_desc, sizeof(GroupDesc) * cap, hipHostMallocDefault)) !=
+            hipSuccess ||
+        (e = hipMalloc(&p, sizeof(int2) * capb)) != hipSuccess ||
+        ((g.btab = static_cast<int*>(p)), false) ||
+        (e = hipHostMalloc(&p, sizeof(int2) * capb, hipHostMallocDefault)) != hipSuccess ||
+        ((g.host_btab = static_cast<int*>(p)), false) ||
+        (e = hipMalloc(&p, sizeof(int) * 8 * cap)) != hipSuccess ||
+        ((g.ctl = static_cast<int*>(p)), false) ||
+        (e = hipHostMalloc(&p, sizeof(int) * 8 * cap, hipHostMallocDefault)) != hipSuccess ||
+        ((g.host_ctl = static_cast<int*>(p)), false) ||
+        (e = hipMalloc(&p, sizeof(double) * 3 * cap)) != hipSuccess ||
+        ((g.stat = static_cast<double*>(p)), false) ||
+        (e = hipHostMalloc(&p, sizeof(double) * 3 * cap, hipHostMallocDefault)) != hipSuccess ||
+        ((g.host_stat = static_cast<double*>(p)), false) ||
+        (e = hipEventCreateWithFlags(&g.ev[0], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g.ev[1], hipEventDisableTiming)) != hipSuccess) {
+      mixed_group_release(g);
+      return hipfail(err, "grouped CG allocation", e);
+    }
+    g.cap = cap;
+    g.cap_blocks = capb;
+  }
+  // the staging buffers are free: the previous call ended with a synchronisation
+  std::memcpy(g.host_desc, static_cast<const void*>(desc.data()), sizeof(GroupDesc) * count);
+  int2* hb = reinterpret_cast<int2*>(g.host_btab);
+  std::copy(bf.begin(), bf.end(), hb);
+  std::copy(bb.begin(), bb.end(), hb + bf.size());
+  std::copy(bc.begin(), bc.end(), hb + bf.size() + bb.size());
+  const long key[6] = {count, (long)bf.size(), (long)bb.size(), (long)bc.size(), (long)lds_f,
+                       (long)lds_b};
+  bool same = g.gexec != nullptr;
+  for (int i = 0; i < 6; ++i) same = same && key[i] == g.gkey[i];
+  if (!same && g.gexec) {
+    (void)hipGraphExecDestroy(g.gexec);
+    g.gexec = nullptr;
+  }
+  for (int i = 0; i < 6; ++i) g.gkey[i] = key[i];
+  g.nf = (int)bf.size();
+  g.nbw = (int)bb.size();
+  g.nc = (int)bc.size();
+  g.lds_f = lds_f;
+  g.lds_b = lds_b;
+  GroupDesc* gd = static_cast<GroupDesc*>(g.desc);
+  if ((e = hipMemcpyAsync(g.desc, g.host_desc, sizeof(GroupDesc) * count,
+                          hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(g.btab, g.host_btab, sizeof(int2) * nblk, hipMemcpyHostToDevice,
+                          st)) != hipSuccess)
+    return hipfail(err, "grouped CG", e);
+  // start: u = M r and the products on u for every machine taking part, then
+  // each machine's <b, b>, threshold and budget on the device
+  hipLaunchKernelGGL(k_group_start, dim3(nb((size_t)maxdim), count), dim3(256), 0, st, gd,
+                     shift);
+  group_step(g, 0, 0, st);
+  group_products(g, 0, st);
+  hipLaunchKernelGGL(k_group_rr, dim3((count + 63) / 64), dim3(64), 0, st, gd, count, 0,
+                     g.stat);
+  // iteration 0 eagerly, then captured chunks of g.chunk_iters iterations
+  // polled one chunk behind, as in mixed_cg
+  group_iteration(g, 0, true, st);
+  const bool graphs = g.use_graphs && !prof_enabled("cg_step") && !prof_enabled("fwd_mfma") &&
+                      !prof_enabled("bwd_mfma");
+  if (graphs && !g.gexec) {
+    hipGraph_t gr = nullptr;
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return hipfail(err, "grouped CG capture", e);
+    for (int p = 0; p < g.chunk_iters; ++p) group_iteration(g, (1 + p) & 1, false, st);
+    e = hipStreamEndCapture(st, &gr);
+    if (e != hipSuccess) return hipfail(err, "grouped CG capture", e);
+    e = hipGraphInstantiate(&g.gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      g.gexec = nullptr;
+      return hipfail(err, "grouped CG instantiate", e);
+    }
+  }
+  int launched = 1, issued = 0;
+  auto issue = [&]() -> hipError_t {
+    if (graphs) {
+      hipError_t r = hipGraphLaunch(g.gexec, st);
+      if (r != hipSuccess) return r;
+    } else {
+      for (int k = 0; k < g.chunk_iters; ++k) group_iteration(g, (launched + k) & 1, false, st);
+    }
+    launched += g.chunk_iters;
+    const int slot = issued & 1;
+    hipLaunchKernelGGL(k_group_ctl, dim3((count + 63) / 64), dim3(64), 0, st, gd, count,
+                       g.ctl + 4 * count * slot);
+    hipError_t r = hipMemcpyAsync(g.host_ctl + 4 * count * slot, g.ctl + 4 * count * slot,
+                                  sizeof(int) * 4 * count, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipEventRecord(g.ev[slot], st);
+    ++issued;
+    return r;
+  };
+  if (launched < maxit && (e = issue()) != hipSuccess) return hipfail(err, "grouped CG", e);
+  for (int k = 0; k < issued; ++k) {
+    if (launched < maxit && (e = issue()) != hipSuccess) return hipfail(err, "grouped CG", e);
+    if ((e = hipEventSynchronize(g.ev[k & 1])) != hipSuccess) return hipfail(err, "grouped CG", e);
+    const int* c = g.host_ctl + 4 * count * (k & 1);
+    bool all = true;
+    for (int q = 0; q < count && all; ++q)
+      all = c[4 * q + kCtlStop] || c[4 * q + kCtlIters] >= jobs[q].max_iter;
+    if (all) break;
+  }
+  hipLaunchKernelGGL(k_group_rr, dim3((count + 63) / 64), dim3(64), 0, st, gd, count, 1,
+                     g.stat);
+  hipLaunchKernelGGL(k_group_finish, dim3(nb((size_t)maxdim), count), dim3(256), 0, st, gd);
+  e = hipMemcpyAsync(g.host_stat, g.stat, sizeof(double) * 3 * count, hipMemcpyDeviceToHost,
+                     st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) return hipfail(err, "grouped CG", e);
+  for (int q = 0; q < count; ++q) {
+    GroupJob& j = jobs[q];
+    if (!j.active) continue;
+    const double bn2 = g.host_stat[q];
+    if (bn2 == 0.0) continue;  // x = 0 after no iterations
+    const double thr = j.tol * j.tol * bn2;
+    const double rn2 = g.host_stat[count + q];
+    j.iters = (int)g.host_stat[2 * count + q];
+    j.rc = !(rn2 >= thr) ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+  }
+  return YQ_OK;
+}
+
+void mixed_group_release(MixedGroup& g) {
+  if (g.gexec) (void)hipGraphExecDestroy(g.gexec);
+  for (void* p : {g.desc, (void*)g.btab, (void*)g.ctl, (void*)g.stat})
+    if (p) (void)hipFree(p);
+  for (void* p : {g.host_desc, (void*)g.host_btab, (void*)g.host_ctl, (void*)g.host_stat})
+    if (p) (void)hipHostFree(p);
+  for (auto& e : g.ev)
+    if (e) (void)hipEventDestroy(e);
+  const bool use_graphs = g.use_graphs;
+  const int ci = g.chunk_iters;
+  g = MixedGroup();
+  g.use_graphs = use_graphs;
+  g.chunk_iters = ci;
 }
 
 void mixed_release(MixedWork& w) {
