@@ -93,12 +93,13 @@ struct MixedGroup {
   double* stat = nullptr;     // per machine: <b, b>, final <r, r>, iterations
   double* host_stat = nullptr;
   int nf = 0, nbw = 0, nc = 0;  // blocks of the forward, backward and step launches
+  int cg_threads = 1024;        // threads per step block
   size_t lds_f = 0, lds_b = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool use_graphs = true;
   int chunk_iters = 8;
   hipGraphExec_t gexec = nullptr;
-  long gkey[6] = {0, 0, 0, 0, 0, 0};
+  long gkey[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 // The inner CG of every active job in one set of launches per iteration (a
 // grouped forward product, backward product and CG step whose grids

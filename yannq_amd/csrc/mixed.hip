@@ -556,9 +556,10 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
               double* dctl, int* ictl, const float* __restrict__ part, int chunks,
               const float* __restrict__ om, int n, int m, int use_bias, int Kp, int Np,
               uint16_t* __restrict__ vs) {
-  __shared__ float grp[3][256];
+  __shared__ float grp[4][256];
   __shared__ uint16_t tile[3][16][17];
   const int t = threadIdx.x, g = t >> 8, e = t & 255;
+  const int G = blockDim.x >> 8;  // thread groups of 256 (1, 2 or 4)
   // a tile row of the chunk partials is 64 bytes, half an L2 line: the
   // XCD-aware numbering puts tiles (2i, 2i + 1) of a row on one XCD so that
   // the line is fetched from HBM once
@@ -573,9 +574,27 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
   else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
   const double shift = dctl[kCtlShift];
   float alpha = 0.0f, beta = 0.0f;
+  // The grid totals of the partials (<r, r>, <r, u>, <u, u>, sum y^2, sum y)
+  // are summed by wave 0 alone and shared through LDS: every wave would get
+  // the same values, and nv partials read by each of the 16 waves of every
+  // block would cost 16 x 3 x nv^2 doubles of L2 reads per launch.
+  __shared__ double tot[5];
+  if (mode != 0) {
+    if (t < 64) {
+      double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (mode == 1 || mode == 2) {
+        psum3(rr_in, nv, g_in, nv, uu_in, nv, v[0], v[1], v[2]);
+        v[3] = psum(yy, nys);
+      }
+      v[4] = psum(ys, nys);
+      if (t == 0)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) tot[q] = v[q];
+    }
+    __syncthreads();
+  }
   if (mode == 1 || mode == 2) {
-    double rn2, gam, uu;
-    psum3(rr_in, nv, g_in, nv, uu_in, nv, rn2, gam, uu);
+    const double rn2 = tot[0], gam = tot[1], uu = tot[2];
     // CG continuation predicate (uniform over the grid): stop once <r, r>
     // falls below the threshold (or is NaN) or the iteration budget is spent;
     // the later kernels of the iteration see the flag and return at once
@@ -583,7 +602,7 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
       if (lb == 0 && t == 0) ictl[kCtlStop] = 1;
       return;
     }
-    const double del = (double)ns * psum(yy, nys) + shift * uu;
+    const double del = (double)ns * tot[3] + shift * uu;
     double a, bt = 0.0;
     if (mode == 1) {
       a = gam / del;
@@ -601,10 +620,15 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
   }
   float wv = 0.0f;
   if (mode != 0) {
-    float acc = 0.0f;
-    if (c >= 0) {
+    // The chunk partials of this element fall into the residue classes
+    // r = 0..3 of the chunk index, each summed in chunk order and combined as
+    // (a0 + a1) + (a2 + a3); thread group g sums the classes r = g, g + G, ...
+    // (up to 8 chunks of each class in flight), so the value does not depend
+    // on G.
+    if (c >= 0 && G == 4) {  // one class per group
       const size_t a = (size_t)k * Np + j;  // aug(c)
       const size_t stride = (size_t)Kp * Np;
+      float acc = 0.0f;
       for (int q0 = g; q0 < chunks; q0 += 32) {
         float v[8];
 #pragma unroll
@@ -614,14 +638,34 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
         for (int q = 0; q < 8; ++q)
           if (q0 + 4 * q < chunks) acc += v[q];
       }
+      grp[g][e] = acc;
+    } else if (c >= 0) {
+      const size_t a = (size_t)k * Np + j;  // aug(c)
+      const size_t stride = (size_t)Kp * Np;
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int q0 = 0; q0 < chunks; q0 += 32) {
+        float v[4][8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int ch = q0 + r + 4 * q;
+            v[r][q] = (r & (G - 1)) == g && ch < chunks ? part[(size_t)ch * stride + a] : 0.0f;
+          }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if ((r & (G - 1)) == g && q0 + r + 4 * q < chunks) acc[r] += v[r][q];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if ((r & (G - 1)) == g) grp[r][e] = acc[r];
     }
-    if (g > 0) grp[g - 1][e] = acc;
-    // sum(y) in every lane (psum shuffles across the wave, so no lane may be
-    // masked off here)
-    const float sy = (float)psum(ys, nys);
+    const float sy = (float)tot[4];
     __syncthreads();
     if (g == 0 && c >= 0) {
-      const float gs = (acc + grp[0][e]) + (grp[1][e] + grp[2][e]);
+      const float gs = (grp[0][e] + grp[1][e]) + (grp[2][e] + grp[3][e]);
       wv = (gs - om[c] * sy) + (float)shift * u[c];
     }
     if (mode == 3) {
@@ -775,11 +819,16 @@ void product_launches(MixedWork& w, int count, hipStream_t st) {
               w.part, w.red + L.ys, w.red + L.yy, w.ictl);
 }
 
+// thread groups of 256 per k_cg_step block for a number of chunk partials
+// (any count gives the same values; fewer, smaller blocks when there are few)
+int cg_groups(int chunks) { return chunks > 16 ? 4 : chunks > 8 ? 2 : 1; }
+
 // k_cg_step in the given mode for iteration parity par (mode 0 writes slot 0)
 void step_launch(MixedWork& w, const float* dinv, int mode, int par, hipStream_t st) {
   const RedLayout L = red_layout(w);
   const int wr = mode == 0 ? 0 : par ^ 1;
-  launch_prof("cg_step", k_cg_step, dim3(w.Np / 16, w.Kp / 16), dim3(1024), 0, st, w.x, w.r,
+  launch_prof("cg_step", k_cg_step, dim3(w.Np / 16, w.Kp / 16),
+              dim3(256 * cg_groups(w.nsp / w.chunk)), 0, st, w.x, w.r,
               w.p, w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par],
               w.red + L.uu[par], L.nbv, w.red + L.ys, w.red + L.yy, L.nby, w.ns,
               w.red + L.g[wr], w.red + L.rr[wr], w.red + L.uu[wr], w.red + L.op, w.dctl, w.ictl,
@@ -1258,8 +1307,8 @@ void group_products(const MixedGroup& g, int count, hipStream_t st) {
 void group_step(const MixedGroup& g, int mode, int par, hipStream_t st) {
   const GroupDesc* gd = static_cast<const GroupDesc*>(g.desc);
   const int2* bt = reinterpret_cast<const int2*>(g.btab);
-  hipLaunchKernelGGL(k_cg_group, dim3(g.nc), dim3(1024), 0, st, gd, bt + g.nf + g.nbw, mode,
-                     par);
+  hipLaunchKernelGGL(k_cg_group, dim3(g.nc), dim3(g.cg_threads), 0, st, gd, bt + g.nf + g.nbw,
+                     mode, par);
 }
 void group_iteration(const MixedGroup& g, int par, bool first, hipStream_t st) {
   group_step(g, first ? 1 : 2, par, st);
@@ -1284,6 +1333,7 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
   std::vector<int2> bf, bb, bc;
   size_t lds_f = 0, lds_b = 0;
   long long maxdim = 0;
+  int cgg = 1;
   for (int q = 0; q < count; ++q) {
     const GroupJob& j = jobs[q];
     MixedWork& w = *j.w;
@@ -1352,6 +1402,7 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
     lds_f = std::max(lds_f, sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad));
     lds_b = std::max(lds_b, sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad));
     maxdim = std::max(maxdim, d.dim);
+    cgg = std::max(cgg, cg_groups(w.nsp / w.chunk));
   }
   const int nblk = (int)(bf.size() + bb.size() + bc.size());
   hipError_t e = hipSuccess;
@@ -1388,15 +1439,16 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
   std::copy(bf.begin(), bf.end(), hb);
   std::copy(bb.begin(), bb.end(), hb + bf.size());
   std::copy(bc.begin(), bc.end(), hb + bf.size() + bb.size());
-  const long key[6] = {count, (long)bf.size(), (long)bb.size(), (long)bc.size(), (long)lds_f,
-                       (long)lds_b};
+  const long key[7] = {count,       (long)bf.size(), (long)bb.size(), (long)bc.size(),
+                       (long)lds_f, (long)lds_b,     256L * cgg};
   bool same = g.gexec != nullptr;
-  for (int i = 0; i < 6; ++i) same = same && key[i] == g.gkey[i];
+  for (int i = 0; i < 7; ++i) same = same && key[i] == g.gkey[i];
   if (!same && g.gexec) {
     (void)hipGraphExecDestroy(g.gexec);
     g.gexec = nullptr;
   }
-  for (int i = 0; i < 6; ++i) g.gkey[i] = key[i];
+  for (int i = 0; i < 7; ++i) g.gkey[i] = key[i];
+  g.cg_threads = 256 * cgg;
   g.nf = (int)bf.size();
   g.nbw = (int)bb.size();
   g.nc = (int)bc.size();
