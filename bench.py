@@ -292,7 +292,8 @@ def batched(yq, warmup, steps):
     """Many machines in one context: 32 TFI machines, N ~ U{16, 24, ..., 128},
     alpha ~ U{1, 2, 4}, h evenly spaced over [0.5, 1.5], 2,048 samples each,
     float sampling and the mixed factored SR (one persistent sampler launch and
-    one E_loc launch for all machines, concurrent graph-captured CG), against
+    one E_loc launch for all machines, the CG iterations of all machines as one
+    set of graph-captured launches; also timed with one CG per machine), against
     32 sequential plain-FP64 GPU runs of the same machines (FP64 sampling and
     FP64 factored CG) from the same parameters and chain seeds.  Prints one
     JSON line; the energies of every machine and step are compared."""
@@ -306,10 +307,11 @@ def batched(yq, warmup, steps):
     kw = dict(n_sweeps=1, solver=yq.SOLVE_CG_FACTORED, lam=LAMBDA, cg_tol=TOL,
               cg_max_iter=50000, lr=LR, allow_unconverged=True)
 
-    def run_batched(sample_precision):
+    def run_batched(sample_precision, grouped=True):
         ctx = yq.Context(0)
         ctx.multi_create(ms, sweeper=yq.LOCAL, precision=sample_precision)
         ctx.multi_init_random(11, 0.01)
+        ctx.multi_set_grouped(grouped)
         p0 = [ctx.multi_get_params(q) for q in range(len(ms))]
         e_b, iters = [], []
         for i in range(warmup):
@@ -326,6 +328,10 @@ def batched(yq, warmup, steps):
         return dt, np.array(e_b).T, iters, p0
 
     dt_b, eb, iters, p0 = run_batched(yq.MIXED)
+    # the same step with one CG per machine (concurrent streams): the same
+    # energies bit for bit
+    dt_u, eu, _, _ = run_batched(yq.MIXED, grouped=False)
+    same_as_ungrouped = bool(np.array_equal(eb, eu))
     # the same batched launch with FP64 sampling: its chains follow the FP64
     # runs' chains, so the energies differ only through the mixed SR solve
     dt_b64, eb64, _, _ = run_batched(yq.FP64)
@@ -363,6 +369,8 @@ def batched(yq, warmup, steps):
                    "machines": [[mc["n"], mc["m"]] for mc in ms], "parameters": int(sum(dims)),
                    "solver": "mixed factored CG (split-bf16 MFMA) refined in FP64 to 1e-8",
                    "lambda": LAMBDA, "lr": LR},
+        "ungrouped_ms_per_step": 1e3 * dt_u / steps,
+        "grouped_equals_ungrouped": same_as_ungrouped,
         "sequential_fp64_ms_per_step": 1e3 * dt_s / steps,
         "speedup_vs_sequential_fp64": dt_s / dt_b,
         "energy_rel_diff_max": float(rel.max()),
