@@ -204,7 +204,7 @@ __global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows
 // (column tile, chunk)
 __global__ void __launch_bounds__(256)
     k_bwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
-                 int m, const double* __restrict__ r, int Kp, int Np,
+                 int m, const double* __restrict__ r, int Kp, int Np, int chunk,
                  double* __restrict__ part) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int ct = blockIdx.x * 4 + wave;
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256)
   // unrolled so that several k-steps' loads are in flight together (the
   // accumulation order over k is unchanged)
 #pragma unroll 8
-  for (int k0 = kc * kChunk; k0 < kc * kChunk + kChunk; k0 += 4) {
+  for (int k0 = kc * chunk; k0 < kc * chunk + chunk; k0 += 4) {
     const int s = k0 + lg;
     const double b = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j);
 #pragma unroll
@@ -383,6 +383,11 @@ void launch_build_O(const int8_t* sigma, const double* T, int ns, int n, int m,
 }
 
 int colsum_chunks(int ns) { return (ns + kChunk - 1) / kChunk; }
+// samples per chunk of the FP64 backward product: 64 for batches of up to
+// 4096 samples (four times the waves of 256-sample chunks; each wave's k-steps
+// are a chain of dependent MFMAs), else 256
+int bwd64_chunk(int ns) { return ns <= 4096 ? 64 : kChunk; }
+int bwd64_chunks(int ns) { return (ns + bwd64_chunk(ns) - 1) / bwd64_chunk(ns); }
 
 void launch_colsum(const double* X, int ns, int dim, double* partial,
                    hipStream_t st, const double* w, int square) {
@@ -419,7 +424,7 @@ void launch_copy_lower_shift(const double* S, double* M, int dim,
 }
 
 size_t factored_partial_doubles(int ns, int n, int m) {
-  const size_t grid = (size_t)colsum_chunks(ns) * kp16(n) * np16(m);
+  const size_t grid = (size_t)bwd64_chunks(ns) * kp16(n) * np16(m);
   const size_t fwd = (size_t)((np16(m) / 16 + kF64Tiles - 1) / kF64Tiles) * ((ns + 15) / 16 * 16);
   return (grid > fwd ? grid : fwd) + ns;
 }
@@ -438,11 +443,11 @@ void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
 void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* r, double* g,
                            double* partial, hipStream_t st) {
-  const int chunks = colsum_chunks(ns);
+  const int chunks = bwd64_chunks(ns);
   const int Kp = kp16(n), Np = np16(m);
   ProfGuard pg("factored_bwd64", st);
   hipLaunchKernelGGL(k_bwd64_mfma, dim3((Np / 16 + 3) / 4, chunks), dim3(256), 0, st, sigma, T,
-                     ns, n, m, r, Kp, Np, partial);
+                     ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial);
   const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
   hipLaunchKernelGGL(k_reduce_aug, dim3(nblk(dim)), dim3(256), 0, st, partial, chunks, n, m,
                      use_bias, Kp, Np, 1.0, g);
