@@ -125,6 +125,7 @@ struct yq_ctx {
   // solver options: Jacobi preconditioning, warm start from the last solution
   int precond = 0, warm_start = 0;
   Buf diagS, dinv64, vz, xprev, ones;
+  bool diag_ready = false;  // diagS holds diag(S) of the current build
   bool have_xprev = false;
   int ones_len = 0;
 
@@ -483,6 +484,7 @@ int sr_build_c(yq_ctx* c, int solver) {
   LAUNCHCHK(c);
   TRY(sync(c));
   c->have_sr = true;
+  c->diag_ready = true;  // formed above
   c->sr_solver = solver;
   c->sr_prec = YQ_FP64;
   return YQ_OK;
@@ -569,6 +571,7 @@ int sr_build_ap(yq_ctx* c, int solver, int precision) {
   }
   TRY(sync(c));
   c->have_sr = true;
+  c->diag_ready = true;  // formed above
   c->sr_solver = solver;
   c->sr_prec = YQ_FP64;
   return YQ_OK;
@@ -652,8 +655,21 @@ int sr_build(yq_ctx* c, int solver, int precision) {
       TRY(syrk_samples(c, c->O.as<double>(), dim, ns, 1.0 / ns, c->S.as<double>(), false));
     }
   }
-  // diag(S) for the Jacobi preconditioner (cheap: one pass over T; for a
-  // weighted batch the column sums of the scaled O_c^2)
+  c->diag_ready = false;  // diag(S) is formed when a preconditioned solve needs it
+  TRY(sync(c));
+  c->have_sr = true;
+  c->sr_solver = solver;
+  c->sr_prec = precision;
+  return YQ_OK;
+}
+
+// diag(S) of a real machine's build for the Jacobi preconditioner, formed on
+// first use (one pass over T; for a weighted batch the column sums of the
+// scaled O_c^2); the complex and AmplitudePhase builds form it themselves
+int ensure_diag(yq_ctx* c) {
+  if (c->diag_ready) return YQ_OK;
+  const int ns = c->ns, n = c->n, m = c->m;
+  const size_t dim = c->dim;
   TRY(ensure(c, c->diagS, sizeof(double) * dim));
   TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)(colsum_chunks(ns) + 1) * m)));
   if (c->weighted) {
@@ -665,10 +681,7 @@ int sr_build(yq_ctx* c, int solver, int precision) {
                   c->diagS.as<double>(), c->vtmp.as<double>(), c->st);
   }
   LAUNCHCHK(c);
-  TRY(sync(c));
-  c->have_sr = true;
-  c->sr_solver = solver;
-  c->sr_prec = precision;
+  c->diag_ready = true;
   return YQ_OK;
 }
 
@@ -764,6 +777,7 @@ int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int ma
   double* z = c->vz.as<double>();
   const double* dinv = nullptr;
   if (c->precond) {
+    TRY(ensure_diag(c));
     launch_jacobi_inv64(c->diagS.as<double>(), shift, c->dinv64.as<double>(), dim, c->st);
     dinv = c->dinv64.as<double>();
   }
@@ -951,6 +965,7 @@ int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
     HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
     HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
   }
+  if (c->precond) TRY(ensure_diag(c));
   int total = 0;
   for (int outer = 0; outer < 30 && total < max_iter && !(rel < tol); ++outer) {
     int it = 0;
@@ -2491,6 +2506,7 @@ int solve_mixed_group(yq_ctx* c, const yq_step_cfg* cfg, int n_threads,
     TRY(ensure(s, s->vr, sizeof(double) * dim));
     TRY(ensure(s, s->vAp, sizeof(double) * dim));
     TRY(ensure(s, s->vp, sizeof(double) * dim));
+    if (s->precond) TRY(ensure_diag(s));
     const double* b = s->F.as<double>();
     TRY(dot(s, b, b, dim, &bn2[i]));
     HIPCHK(s, hipMemsetAsync(s->vx.p, 0, sizeof(double) * dim, s->st));
