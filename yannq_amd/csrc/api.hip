@@ -121,6 +121,9 @@ struct yq_ctx {
   int sr_solver = 0, sr_prec = 0;
   Buf T64, T32, O, omean, F, S, M, partial, ecent, info;
   Buf vx, vr, vp, vAp, vy, vtmp, vb, dotp;
+  // device scalars: [0, 1] <omean, v> and sum y of the FP64 factored operator,
+  // [2, 3] <b, b> and <r, r> of the mixed refinement
+  Buf dscal;
   yq::MixedWork mixed;
   // solver options: Jacobi preconditioning, warm start from the last solution
   int precond = 0, warm_start = 0;
@@ -153,7 +156,7 @@ struct yq_ctx {
   yq_ctx() {
     for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
-                   &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp,
+                   &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp, &dscal,
                    &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
                    &xprev, &ones, &cpd, &cpf, &m_sargs, &m_items, &m_eargs, &m_stats, &ap_ffs,
                    &ap_ffb})
@@ -721,12 +724,17 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
                              (rocblas_int)dim, yc, 1, &sh,
                              reinterpret_cast<rocblas_double_complex*>(out), 1));
   } else if (c->sr_solver == YQ_SOLVE_CG_FACTORED) {
+    // the scalars <omean, v> and sum y stay on the device (summed in the order
+    // of a host loop over the dot partials), so the operator needs no round trip
+    TRY(ensure(c, c->dotp, sizeof(double) * 512));
+    TRY(ensure(c, c->dscal, sizeof(double) * 4));
+    double* ov = c->dscal.as<double>();
+    double* sy = ov + 1;
     launch_factored_fwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
                           c->use_bias, v, y, c->partial.as<double>(), c->st);
+    launch_dot_dev(c->omean.as<double>(), v, dim, c->dotp.as<double>(), ov, c->st);
     LAUNCHCHK(c);
-    double ov;
-    TRY(dot(c, c->omean.as<double>(), v, dim, &ov));
-    launch_add_const(y, -ov, ns, c->st);                 // O_c v
+    launch_add_const_p(y, ov, -1.0, ns, c->st);           // O_c v
     launch_scale_add(y, 1.0 / ns, y, 0.0, ns, c->st);     // / Ns
     launch_factored_bwd64(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, c->n, c->m,
                           c->use_bias, y, out, c->partial.as<double>(), c->st);
@@ -741,9 +749,8 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
       TRY(sync(c));
       c->ones_len = ns;
     }
-    double sy;
-    TRY(dot(c, y, c->ones.as<double>(), ns, &sy));
-    launch_axpy(-sy, c->omean.as<double>(), out, dim, c->st);
+    launch_dot_dev(y, c->ones.as<double>(), ns, c->dotp.as<double>(), sy, c->st);
+    launch_axpy_p(sy, -1.0, c->omean.as<double>(), out, dim, c->st);
     launch_axpy(shift, v, out, dim, c->st);
     LAUNCHCHK(c);
   } else if (c->sr_solver == YQ_SOLVE_CG || c->sr_solver == YQ_SOLVE_EXPLICIT) {
@@ -943,24 +950,41 @@ int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
   TRY(ensure(c, c->vr, sizeof(double) * dim));
   TRY(ensure(c, c->vAp, sizeof(double) * dim));
   TRY(ensure(c, c->vp, sizeof(double) * dim));
+  TRY(ensure(c, c->dotp, sizeof(double) * 512));
+  TRY(ensure(c, c->dscal, sizeof(double) * 4));
   double *r = c->vr.as<double>(), *Ax = c->vAp.as<double>(), *d = c->vp.as<double>();
-  double bn2;
-  TRY(dot(c, b, b, dim, &bn2));
+  // <b, b> and <r, r> are formed on the device (in the order of a host sum
+  // over the dot partials) and read together once per refinement round
+  double* bb = c->dscal.as<double>() + 2;
+  double* rr = bb + 1;
+  launch_dot_dev(b, b, dim, c->dotp.as<double>(), bb, c->st);
+  LAUNCHCHK(c);
+  double bn2 = 0.0;
+  auto read_rel = [&](double* rel) -> int {
+    launch_dot_dev(r, r, dim, c->dotp.as<double>(), rr, c->st);
+    LAUNCHCHK(c);
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    TRY(sync(c));
+    bn2 = h[0];
+    *rel = bn2 == 0.0 ? 0.0 : sqrt(h[1] / bn2);
+    return YQ_OK;
+  };
   *iters = 0;
   *err = 0.0;
-  if (bn2 == 0.0) {
-    HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
-    return YQ_OK;
-  }
+  // b = 0: x = 0 (the inner CG on r = b = 0 stops at once and leaves d = 0,
+  // so x stays 0 either way)
   double rel = 1.0;
   if (x0) {  // warm start: r = b - (S + shift) x
     TRY(apply64(c, shift, x, Ax));
     HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
     launch_axpy(-1.0, Ax, r, dim, c->st);
     LAUNCHCHK(c);
-    double rn2;
-    TRY(dot(c, r, r, dim, &rn2));
-    rel = sqrt(rn2 / bn2);
+    TRY(read_rel(&rel));
+    if (bn2 == 0.0) {
+      HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+      return YQ_OK;
+    }
   } else {
     HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
     HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
@@ -981,9 +1005,8 @@ int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
     HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
     launch_axpy(-1.0, Ax, r, dim, c->st);
     LAUNCHCHK(c);
-    double rn2;
-    TRY(dot(c, r, r, dim, &rn2));
-    rel = sqrt(rn2 / bn2);
+    TRY(read_rel(&rel));
+    if (bn2 == 0.0) break;
     if (rel < tol) break;
   }
   *iters = total;

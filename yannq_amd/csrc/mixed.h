@@ -32,7 +32,6 @@ struct MixedWork {
   float *x = nullptr, *r = nullptr, *p = nullptr, *s = nullptr, *u = nullptr, *Ap = nullptr;
   float* dinv = nullptr;       // Jacobi preconditioner 1 / (diag S + shift)
   double* red = nullptr;       // per-block partials of the fused dot products
-  std::vector<double> host_red;
   size_t cap_rows = 0, cap_kp = 0, cap_np = 0, cap_dim = 0, cap_red = 0;
   // tile shapes of the products: forward column tiles per group (2, 4, 8) and
   // backward samples per chunk (128, 256, 512); chosen per operand shape by
@@ -44,6 +43,7 @@ struct MixedWork {
   double* dctl = nullptr;      // shift, stopping threshold, alpha and gamma slots
   int* ictl = nullptr;         // stop flag, completed iterations, budget
   int* host_ctl = nullptr;     // 2 slots of 4 ints
+  int* host_ctl_dev = nullptr; // the same pinned memory as a device address
   hipEvent_t ev[2] = {nullptr, nullptr};
   // captured CG chunks (index: Jacobi preconditioner off / on) and their shape
   bool use_graphs = true;

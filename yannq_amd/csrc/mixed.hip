@@ -42,10 +42,14 @@ constexpr int kMaxIt = 10;      // Kp / 16
 constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
 
 // device control block of the CG loop: ints [stop flag, completed iterations,
-// iteration budget]
-enum { kCtlStop = 0, kCtlIters = 1, kCtlMaxIt = 2 };
-// and doubles [shift, stopping threshold on <r, r>, alpha x 2, gamma x 2]
-enum { kCtlShift = 0, kCtlThr = 1, kCtlAlpha = 2, kCtlGamma = 4, kCtlDoubles = 6 };
+// iteration budget, converged flag (<r, r> below the threshold when the loop
+// stopped), chunks published to the host]; the first four are what the host
+// polls
+enum { kCtlStop = 0, kCtlIters = 1, kCtlMaxIt = 2, kCtlConv = 3, kCtlChunk = 4, kCtlInts = 8 };
+constexpr int kSlotInts = 8;  // one host slot: 4 control ints, <r, r>, threshold
+// and doubles [shift, stopping threshold on <r, r>, alpha x 2, gamma x 2, the
+// latest <r, r> (what the host's convergence forecast reads)]
+enum { kCtlShift = 0, kCtlThr = 1, kCtlAlpha = 2, kCtlGamma = 4, kCtlRr = 6, kCtlDoubles = 7 };
 
 __device__ __host__ inline uint16_t f2bf(float x) {
   uint32_t u;
@@ -595,11 +599,15 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
   }
   if (mode == 1 || mode == 2) {
     const double rn2 = tot[0], gam = tot[1], uu = tot[2];
+    if (lb == 0 && t == 0) dctl[kCtlRr] = rn2;
     // CG continuation predicate (uniform over the grid): stop once <r, r>
     // falls below the threshold (or is NaN) or the iteration budget is spent;
     // the later kernels of the iteration see the flag and return at once
     if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
-      if (lb == 0 && t == 0) ictl[kCtlStop] = 1;
+      if (lb == 0 && t == 0) {
+        ictl[kCtlStop] = 1;
+        if (!(rn2 >= dctl[kCtlThr])) ictl[kCtlConv] = 1;
+      }
       return;
     }
     const double del = (double)ns * tot[3] + shift * uu;
@@ -750,6 +758,55 @@ __global__ void k_set_ctl(double* dctl, int* ictl, double shift, double thr, int
     ictl[kCtlStop] = 0;
     ictl[kCtlIters] = 0;
     ictl[kCtlMaxIt] = max_it;
+    ictl[kCtlConv] = 0;
+    ictl[kCtlChunk] = 0;
+  }
+}
+// Start of a CG run without a host round trip: <b, b> = the sum of the <r, r>
+// partials of slot 0 in block order (staged in LDS, added by one thread: the
+// order of a host loop), then the threshold tol^2 <b, b> and the budget;
+// b = 0 stops the run at once (x stays 0, reported as converged).
+__global__ void __launch_bounds__(256)
+    k_start_rr(const double* __restrict__ rr0, int nbv, double tol, int max_it, double* dctl,
+               int* ictl) {
+  __shared__ double sh[1024];
+  double bn2 = 0.0;
+  for (int base = 0; base < nbv; base += 1024) {
+    const int cnt = min(1024, nbv - base);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) sh[i] = rr0[base + i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; ++i) bn2 += sh[i];
+  }
+  if (threadIdx.x == 0) {
+    ictl[kCtlIters] = 0;
+    if (bn2 == 0.0) {
+      ictl[kCtlStop] = 1;
+      ictl[kCtlConv] = 1;
+    } else {
+      dctl[kCtlThr] = tol * tol * bn2;
+      ictl[kCtlStop] = 0;
+      ictl[kCtlConv] = 0;
+      ictl[kCtlMaxIt] = max_it;
+    }
+  }
+}
+// The control block's first four ints, the latest <r, r> and the threshold
+// into host slot (chunk count) & 1 of the pinned mirror (8 ints a slot: the
+// ints, then the two doubles): the last node of every chunk of iterations, so
+// the host reads the state after chunk k in slot k & 1 once the chunk's event
+// has completed (the event's completion makes the stores visible to the host).
+__global__ void k_publish(int* ictl, const double* __restrict__ dctl, int* host_ctl) {
+  if (threadIdx.x == 0) {
+    const int k = ictl[kCtlChunk];
+    ictl[kCtlChunk] = k + 1;
+    int* o = host_ctl + kSlotInts * (k & 1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = ictl[i];
+    double* od = reinterpret_cast<double*>(o + 4);
+    od[0] = dctl[kCtlRr];
+    od[1] = dctl[kCtlThr];
   }
 }
 __global__ void k_zero_f(float* __restrict__ a, size_t n) {
@@ -1035,6 +1092,7 @@ int mixed_graph(MixedWork& w, const float* dinv, hipStream_t st, hipGraphExec_t*
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hipfail(err, "graph capture", e);
     for (int p = 0; p < w.chunk_iters; ++p) iteration_launches(w, dinv, (1 + p) & 1, false, st);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev);
     e = hipStreamEndCapture(st, &g);
     if (e != hipSuccess) return hipfail(err, "graph capture", e);
     e = hipGraphInstantiate(&w.gexec[slot], g, nullptr, nullptr, 0);
@@ -1153,10 +1211,12 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
         (rc = alloc(w.s, w.dim, err)) || (rc = alloc(w.u, w.dim, err)) ||
         (rc = alloc(w.Ap, w.dim, err)) || (rc = alloc(w.red, red_total, err)) ||
         (rc = alloc(w.dinv, w.dim, err)) || (rc = alloc(w.dctl, kCtlDoubles, err)) ||
-        (rc = alloc(w.ictl, 4, err)))
+        (rc = alloc(w.ictl, kCtlInts, err)))
       return rc;
     if (!w.host_ctl) {
-      hipError_t e = hipHostMalloc((void**)&w.host_ctl, sizeof(int) * 8, hipHostMallocDefault);
+      hipError_t e = hipHostMalloc((void**)&w.host_ctl, sizeof(int) * 2 * kSlotInts,
+                                   hipHostMallocDefault);
+      if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&w.host_ctl_dev, w.host_ctl, 0);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev[0], hipEventDisableTiming);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev[1], hipEventDisableTiming);
       if (e != hipSuccess) return hipfail(err, "mixed_prepare", e);
@@ -1202,21 +1262,6 @@ int mixed_apply64(MixedWork& w, double shift, const double* omean64, const doubl
   return e == hipSuccess ? YQ_OK : hipfail(err, "mixed_apply", e);
 }
 
-namespace {
-// host copy of the <r, r> partials of parity slot par, summed in block order
-int host_rr(MixedWork& w, const RedLayout& L, int par, hipStream_t st, double* out,
-            std::string& err) {
-  if ((int)w.host_red.size() < L.nbv) w.host_red.resize(L.nbv);
-  hipError_t e = hipMemcpyAsync(w.host_red.data(), w.red + L.rr[par], sizeof(double) * L.nbv,
-                                hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
-  double s = 0.0;
-  for (int i = 0; i < L.nbv; ++i) s += w.host_red[i];
-  *out = s;
-  return YQ_OK;
-}
-}  // namespace
 
 int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* diag64,
              const double* b, double* x, double tol, int max_iter, int* iters, hipStream_t st,
@@ -1232,31 +1277,32 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
     dinv = w.dinv;
   }
   // start: u = M r with gamma, <r, r> and <u, u> in slot 0, and the products
-  // on u (the control block is reset first so that the product kernels run)
+  // on u (the control block is reset first so that the product kernels run);
+  // then the threshold from <b, b>, formed on the device (b = 0 stops the run)
   hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, 0.0, 1);
   step_launch(w, dinv, 0, 0, st);
   product_launches(w, 0, st);
-  double bn2 = 0.0;
-  int rc = host_rr(w, L, 0, st, &bn2, err);
-  if (rc) return rc;
+  hipLaunchKernelGGL(k_start_rr, dim3(1), dim3(256), 0, st, w.red + L.rr[0], L.nbv, tol,
+                     max_iter, w.dctl, w.ictl);
   *iters = 0;
-  if (bn2 == 0.0) {
-    hipLaunchKernelGGL(k_f2d, dim3(nb(dim)), dim3(256), 0, st, w.x, x, dim);
-    return YQ_OK;
-  }
-  const double thr = tol * tol * bn2;
-  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, w.dctl, w.ictl, shift, thr, max_iter);
   // iteration 0 (no beta) eagerly, then chunks of w.chunk_iters iterations; an
   // iteration after the predicate fired leaves everything untouched, so the
   // chunk after the one that converged may already be queued
   iteration_launches(w, dinv, 0, true, st);
   hipError_t e;
+  int rc;
   const bool graphs = w.use_graphs && !prof_enabled("cg_step") && !prof_enabled("fwd_mfma") &&
                       !prof_enabled("bwd_mfma");
   hipGraphExec_t gx = nullptr;
   if (graphs && (rc = mixed_graph(w, dinv, st, &gx, err))) return rc;
-  // chunk c copies the control block into host slot c & 1 and records
-  // event c & 1; the host keeps one chunk queued ahead of the one it polls
+  // chunk c ends with k_publish (control block, <r, r> and the threshold into
+  // host slot c & 1) and records event c & 1.  The host keeps one chunk
+  // queued ahead of the one it polls, except when the <r, r> of the last two
+  // polled chunks forecasts convergence inside the chunk already queued: a
+  // chunk queued after the stop only runs kernels that return at once
+  // (about 4.5 us each at N = 100, alpha = 4), a wrong forecast costs one
+  // wait.  The forecast changes only when chunks are queued, never what the
+  // iterations compute.
   int launched = 1, issued = 0;
   auto issue = [&]() -> hipError_t {
     if (gx) {
@@ -1265,35 +1311,55 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
     } else {
       for (int k = 0; k < w.chunk_iters; ++k)
         iteration_launches(w, dinv, (launched + k) & 1, false, st);
+      hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev);
     }
     launched += w.chunk_iters;
-    hipError_t r = hipMemcpyAsync(w.host_ctl + 4 * (issued & 1), w.ictl, sizeof(int) * 4,
-                                  hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipEventRecord(w.ev[issued & 1], st);
+    hipError_t r = hipEventRecord(w.ev[issued & 1], st);
     ++issued;
     return r;
   };
   if (launched < max_iter && (e = issue()) != hipSuccess) return hipfail(err, "mixed_cg", e);
-  for (int k = 0; k < issued; ++k) {
-    if (launched < max_iter && (e = issue()) != hipSuccess) return hipfail(err, "mixed_cg", e);
-    if ((e = hipEventSynchronize(w.ev[k & 1])) != hipSuccess) return hipfail(err, "mixed_cg", e);
-    const int* c = w.host_ctl + 4 * (k & 1);
-    if (c[kCtlStop] || c[kCtlIters] >= max_iter) break;
+  if (issued == 0) {  // a budget of one iteration: publish the state after it
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev);
+    if ((e = hipEventRecord(w.ev[0], st)) != hipSuccess) return hipfail(err, "mixed_cg", e);
+    issued = 1;
   }
-  int hc[4];
-  e = hipMemcpyAsync(hc, w.ictl, sizeof(hc), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
-  const int it = hc[kCtlIters];
-  // after `it` updates the latest <r, r> partials sit in slot it & 1
-  double rn2 = 0.0;
-  if ((rc = host_rr(w, L, it & 1, st, &rn2, err))) return rc;
-  const bool conv = !(rn2 >= thr);
+  // The state after the last polled chunk is final: a chunk is polled only
+  // after every earlier one, the loop leaves at the stop flag (after which no
+  // kernel changes the block) or after the last chunk issued.
+  int last[4] = {0, 0, 0, 0};
+  double lq_prev = 0.0, lq = 0.0;  // log(<r, r> / threshold) of the last two polls
+  int it_prev = -1, it_last = -1;
+  for (int k = 0; k < issued; ++k) {
+    // iterations still needed after the last poll, at the rate of the last
+    // two polls; the chunk being waited for covers w.chunk_iters of them
+    bool ahead = launched < max_iter;
+    if (ahead && k >= 2 && it_last > it_prev && lq < lq_prev && lq > 0.0) {
+      const double need = lq * (double)(it_last - it_prev) / (lq_prev - lq);
+      ahead = need > (double)w.chunk_iters;
+    }
+    if (ahead && (e = issue()) != hipSuccess) return hipfail(err, "mixed_cg", e);
+    if ((e = hipEventSynchronize(w.ev[k & 1])) != hipSuccess) return hipfail(err, "mixed_cg", e);
+    const int* c = w.host_ctl + kSlotInts * (k & 1);
+    for (int i = 0; i < 4; ++i) last[i] = c[i];
+    if (last[kCtlStop] || last[kCtlIters] >= max_iter) break;
+    const double* cd = reinterpret_cast<const double*>(c + 4);
+    lq_prev = lq;
+    it_prev = it_last;
+    lq = cd[0] > 0.0 && cd[1] > 0.0 ? log(cd[0] / cd[1]) : 0.0;
+    it_last = last[kCtlIters];
+    // not converged and nothing queued behind the polled chunk: queue one
+    if (k + 1 == issued && launched < max_iter && (e = issue()) != hipSuccess)
+      return hipfail(err, "mixed_cg", e);
+  }
   hipLaunchKernelGGL(k_f2d, dim3(nb(dim)), dim3(256), 0, st, w.x, x, dim);
   e = hipGetLastError();
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
-  *iters = it;
-  return conv ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+  *iters = last[kCtlIters];
+  // converged: the device's stop predicate saw <r, r> below the threshold
+  // (or b = 0); a run that spent its budget without that test reports
+  // YQ_ERR_NOT_CONVERGED
+  return last[kCtlConv] ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
 namespace {

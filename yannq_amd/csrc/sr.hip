@@ -342,6 +342,34 @@ __global__ void k_add_const(double* y, double c, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] += c;
 }
+// Device-resident scalars for operators that run without host round trips:
+// out = the sum of nb dot partials in block order (staged in LDS, added by one
+// thread: the order of the host loop that reads them back), then the same
+// updates as k_add_const and k_axpy with the scalar (times sign) read on the
+// device.
+__global__ void __launch_bounds__(256)
+    k_psum_scalar(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  __shared__ double sh[512];
+  double s = 0.0;
+  for (int base = 0; base < nb; base += 512) {
+    const int cnt = min(512, nb - base);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) sh[i] = partial[base + i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; ++i) s += sh[i];
+  }
+  if (threadIdx.x == 0) *out = s;
+}
+__global__ void k_add_const_p(double* y, const double* __restrict__ c, double sign, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += sign * *c;
+}
+__global__ void k_axpy_p(const double* __restrict__ a, double sign,
+                         const double* __restrict__ x, double* __restrict__ y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += (sign * *a) * x[i];
+}
 __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y,
                            size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,6 +519,18 @@ int dot_blocks(size_t n) {
 void launch_dot(const double* x, const double* y, size_t n, double* partial,
                 hipStream_t st) {
   hipLaunchKernelGGL(k_dot, dim3(dot_blocks(n)), dim3(256), 0, st, x, y, n, partial);
+}
+void launch_dot_dev(const double* x, const double* y, size_t n, double* partial, double* out,
+                    hipStream_t st) {
+  launch_dot(x, y, n, partial, st);
+  hipLaunchKernelGGL(k_psum_scalar, dim3(1), dim3(256), 0, st, partial, dot_blocks(n), out);
+}
+void launch_add_const_p(double* y, const double* c, double sign, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_const_p, dim3(nblk(n)), dim3(256), 0, st, y, c, sign, n);
+}
+void launch_axpy_p(const double* a, double sign, const double* x, double* y, size_t n,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_p, dim3(nblk(n)), dim3(256), 0, st, a, sign, x, y, n);
 }
 void launch_axpy(double alpha, const double* x, double* y, size_t n,
                  hipStream_t st) {

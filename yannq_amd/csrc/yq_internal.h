@@ -280,6 +280,13 @@ void launch_xpby(const double* x, double beta, double* y, size_t n,
 void launch_scale_add(double* y, double a, const double* x, double b, size_t n,
                       hipStream_t st);  // y = a*y + b*x
 void launch_add_const(double* y, double c, size_t n, hipStream_t st);
+// the same with the scalar on the device: *out = x . y summed in the host's
+// order (no round trip), y += sign * *c, y += (sign * *a) x
+void launch_dot_dev(const double* x, const double* y, size_t n, double* partial, double* out,
+                    hipStream_t st);
+void launch_add_const_p(double* y, const double* c, double sign, size_t n, hipStream_t st);
+void launch_axpy_p(const double* a, double sign, const double* x, double* y, size_t n,
+                   hipStream_t st);
 void launch_to_float(const double* x, float* y, size_t n, hipStream_t st);
 void launch_float_to_double(const float* x, double* y, size_t n, hipStream_t st);
 void launch_transpose(const double* W, int m, int n, double* WT,
