@@ -64,11 +64,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_tfi_n100_a4.json")
 
 
 AUTOTUNE = False
+GRAPH_CHUNK = 16  # CG iterations per captured graph of the mixed solve
 
 
 def _ctx(yq, precision):
     ctx = yq.Context(0)
     ctx.autotune(AUTOTUNE)
+    ctx.graph_mode(GRAPH_CHUNK)
     ctx.rbm_create(N, M)
     ctx.init_random(SEED, 0.01)
     ctx.ham_set(yq.TFI, N, J=-1.0, h=-1.0)   # TFIsing(N, -1, -h) at h = 1
@@ -398,6 +400,8 @@ def main():
     ap.add_argument("--warm-start", type=int, default=0)
     ap.add_argument("--autotune", type=int, default=0,
                     help="time the mixed products' tile shapes per operand shape")
+    ap.add_argument("--graph-chunk", type=int, default=16,
+                    help="CG iterations per captured HIP graph of the mixed solve (even)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -432,8 +436,9 @@ def main():
     if args.batched:
         batched(yq, args.warmup, args.steps)
         return
-    global AUTOTUNE
+    global AUTOTUNE, GRAPH_CHUNK
     AUTOTUNE = bool(args.autotune)
+    GRAPH_CHUNK = args.graph_chunk
     opts = dict(precond=args.precond, warm_start=args.warm_start)
 
     # pick the dominant kernel from a profiled warm-up of the timed path
@@ -516,7 +521,8 @@ def main():
                    "precision": "mixed: f32 sampling/E_loc, split-bf16x3 MFMA CG products "
                                 "(f32 accumulate), f64 refinement to 1e-8, f64 update",
                    "solver": "factored CG, Jacobi preconditioner=%d, warm start=%d, "
-                             "tile autotune=%d" % (args.precond, args.warm_start, args.autotune),
+                             "tile autotune=%d, graph chunk=%d iterations"
+                             % (args.precond, args.warm_start, args.autotune, args.graph_chunk),
                    "lambda": LAMBDA, "lr": LR, "cg_tol": TOL},
         "samples_per_s": value * CHAINS,
         "median_step_ms": float(np.median([o["wall_ms"] for o in outs])),
