@@ -162,6 +162,42 @@ def test_graph_chunks_match_eager(ctx, orc):
     np.testing.assert_array_equal(out[True][0][0], out[True][1][0])
 
 
+def test_mixed_solve_budgets_chunks_and_zero_rhs(ctx, orc):
+    """The host's queueing of captured chunks (one ahead, or none when the
+    published <r, r> forecasts convergence inside the queued chunk) never
+    changes what the solve computes: chunks of 8 and 16 iterations and eager
+    launches give bit-identical solutions and iteration counts, converged and
+    for budgets of 1 (no chunk at all), 2, 9 and 17 iterations; a zero
+    right-hand side returns x = 0 after no iteration with a zero residual."""
+    n, m, ns = 40, 80, 3000
+    _batch(ctx, orc, n, m, ns, scale=0.05)
+    rng = np.random.default_rng(21)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig, precision=1)
+    ctx.sr_build(2, precision=1)
+    dim = n * m + n + m
+    runs = {}
+    for mode in (0, 8, 16):
+        ctx.graph_mode(mode if mode != 8 else True)
+        x, res, it = ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000)
+        assert res < 1e-8 and it > 17
+        runs[(mode, None)] = (x, it)
+        for budget in (1, 2, 9, 17):
+            x, res, it = ctx.sr_solve(1e-3, tol=1e-8, max_iter=budget, allow_unconverged=True)
+            assert it == budget and res > 1e-8
+            runs[(mode, budget)] = (x, it)
+        x, res, it = ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000, rhs=np.zeros(dim))
+        assert it == 0 and res == 0.0
+        np.testing.assert_array_equal(x, np.zeros(dim))
+    ctx.graph_mode(True)
+    for budget in (None, 1, 2, 9, 17):
+        xe, ie = runs[(0, budget)]
+        for mode in (8, 16):
+            xg, ig = runs[(mode, budget)]
+            assert ig == ie, (mode, budget)
+            np.testing.assert_array_equal(xg, xe)
+
+
 @pytest.mark.parametrize("n,m,ns", [(100, 400, 2048), (40, 80, 3000)])
 def test_tile_shapes_and_autotune(ctx, orc, n, m, ns):
     """Every tile shape of the mixed products (forward column group, backward
