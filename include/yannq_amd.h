@@ -152,8 +152,8 @@ YQ_API int yq_rbm_load(yq_ctx* ctx, const char* path, int elem, int pointer_wrap
 YQ_API int yq_ham_set(yq_ctx* ctx, const yq_ham_desc* desc);
 
 /* ---- Sampler: Samplers/SamplerMT.hpp + Local/SwapSweeper --------------- */
-/* SamplerMT(qs, nTmps, nChainsPer, sweeper): SamplerMT.hpp:157-173, with the
- * sweeper's nSweep (LocalSweeper.hpp:19, SwapSweeper.hpp:70) as
+/* SamplerMT(qs, nTmps, nChainsPer, sweeper): SamplerMT.hpp:44-60, with the
+ * sweeper's nSweep (LocalSweeper.hpp:19, SwapSweeper.hpp:15) as
  * sweep_factor.  seed keys the Philox streams of every chain.  n_temps 1, or
  * even and <= 16 (real machines): parallel tempering with nTmps x nChainsPer
  * chain states, state g * n_temps + t at beta_t = (n_temps - t) / n_temps,
@@ -163,17 +163,17 @@ YQ_API int yq_ham_set(yq_ctx* ctx, const yq_ham_desc* desc);
 YQ_API int yq_sampler_config(yq_ctx* ctx, int sweeper, int n_temps,
                              int n_chains_per, int sweep_factor, int precision,
                              uint64_t seed);
-/* randomizeSigma() / randomizeSigma(nup): SamplerMT.hpp:194-211; nup < 0
+/* randomizeSigma() / randomizeSigma(nup): SamplerMT.hpp:81-98; nup < 0
  * draws unconstrained spins. */
 YQ_API int yq_sampler_randomize(yq_ctx* ctx, int nup);
 /* randomize(randomizer) with caller-chosen configurations (chains x n):
- * SamplerMT.hpp:216-223. */
+ * SamplerMT.hpp:103-110. */
 YQ_API int yq_sampler_set_sigma(yq_ctx* ctx, const int8_t* sigma);
 /* Chain state: spins (chains x n), per-chain proposal counters and accepted
  * proposal counts. */
 YQ_API int yq_sampler_get_state(yq_ctx* ctx, int8_t* sigma, uint64_t* ctr,
                                 uint64_t* acc);
-/* sample(nSweeps, nDiscard): SamplerMT.hpp:293-317.  Samples stay on the
+/* sample(nSweeps, nDiscard): SamplerMT.hpp:180-204.  Samples stay on the
  * device; *n_samples = n_chains_per * n_sweeps, ordered sweep-major. */
 YQ_API int yq_sample(yq_ctx* ctx, int n_sweeps, int n_discard, int* n_samples);
 /* Copy the current batch out: spins (Ns x n) and theta (Ns x m, doubles). */
@@ -205,7 +205,7 @@ YQ_API int yq_exact_sample(yq_ctx* ctx, int nup, int n_samples, uint64_t seed, i
 
 /* ---- Energy: Observables/Energy.hpp + GroundState/utils.hpp:108-126 ---- */
 /* E_loc of every sample (eloc may be null), mean = Re <E_loc>,
- * var = <Re E_loc^2> - mean^2: Energy.hpp:59-70. */
+ * var = <Re E_loc^2> - mean^2: Energy.hpp:53-58 (finIter). */
 YQ_API int yq_eloc(yq_ctx* ctx, double* eloc, double* mean, double* var);
 
 /* ---- SR: GroundState/SRMat.hpp ----------------------------------------- */
@@ -386,9 +386,9 @@ YQ_API int yq_ff_vjp_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
  *    SR system of the concatenated parameters [RBM | FF] (:64-77, 153-158):
  *    rows O = [O_RBM / 2 | pi dFF/dp] (:42-58; the phase block multiplies i),
  *    S = Re(<O^* O^T> - <O^*><O^T>), which is block-diagonal (the blocks are
- *    GroundState/NGDExact.hpp:172-184's corrMatAmp and corrMatPhase), and
+ *    GroundState/NGDExact.hpp:177-189's corrMatAmp and corrMatPhase), and
  *    F = Re(<O^* E_loc> - <O^*><E_loc>) = [cov(O_amp, Re E_loc) |
- *    cov(O_phase, Im E_loc)], half of NGDExact::energyGrad (:137-170);
+ *    cov(O_phase, Im E_loc)], half of NGDExact::energyGrad (NGDExact.hpp:139-145);
  *    yq_sr_get / yq_sr_apply / yq_sr_solve act on that system (vectors of
  *    dim_amp + dim_phase doubles; the explicit solve factors the two blocks
  *    separately, as RunAmplitudePhase.hpp:96-111 does);

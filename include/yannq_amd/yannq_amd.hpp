@@ -282,7 +282,7 @@ class SwapSweeper {  // Samplers/SwapSweeper.hpp:8-57
 
 // ------------------------------------------------------------ Sampler
 // The batch of one sample() call; it stays on the device.  size() and
-// operator[] (SamplerMT.hpp:308-316's container) copy out on first access.
+// operator[] (SamplerMT.hpp:195-203's container) copy out on first access.
 template <class Machine>
 class SampleBatch {
  public:
@@ -1104,7 +1104,7 @@ class NGDExact {
   RealVector F_, oloc_, S_;
 };
 
-// Runners/RunAmplitudePhase.hpp:10-127: NGDExact epochs with the amplitude
+// Runners/RunAmplitudePhase.hpp:13-122: NGDExact epochs with the amplitude
 // and phase blocks solved separately (each + lambda I), then the optimizer;
 // callback(ll, E, |grad|, |v|).  Weight files are not written (saveWfPer
 // must be 0: the reference's AmplitudePhase archive holds the phase

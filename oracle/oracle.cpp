@@ -796,7 +796,7 @@ void rp_eloc(const or_ham* H, int precision, int ns, int n, int m,
     eloc_range<double>(H, n, m, W, a, sigma, theta, eloc, 0, ns);
 }
 
-// Initial configurations (SamplerMT::randomizeSigma, SamplerMT.hpp:194-211,
+// Initial configurations (SamplerMT::randomizeSigma, SamplerMT.hpp:81-98,
 // Utility.hpp:64-88): uniform +/-1 spins, or a uniformly random arrangement
 // with exactly nup spins at -1 (Fisher-Yates shuffle driven by Philox).
 void rp_randomize(uint64_t seed, uint64_t epoch, int C, const uint32_t* stream,

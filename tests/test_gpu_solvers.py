@@ -236,3 +236,41 @@ def test_tile_shapes_and_autotune(ctx, orc, n, m, ns):
     x1, r1, _ = ctx.sr_solve(1e-3, tol=1e-8, max_iter=20000)
     ctx.autotune(False)
     assert r0 < 1e-8 and r1 < 1e-8 and _rel(x1, x0) < 1e-5
+
+
+def test_mixed_solve_budget_equal_to_converged_count_and_nan_rhs(ctx, orc):
+    """A budget equal to the iteration count of an unbounded solve converges
+    to the same solution bit for bit (the last inner run ends its budget on
+    the iteration that reaches the threshold, which is then tested on the
+    device), for eager launches and chunks of 8 and 16; a right-hand side with
+    a NaN or an infinity is reported as YQ_ERR_NAN by the mixed and the FP64
+    CG instead of a silent zero solution."""
+    import yannq_amd as yq
+    n, m, ns = 40, 80, 3000
+    _batch(ctx, orc, n, m, ns, scale=0.05)
+    rng = np.random.default_rng(33)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    dim = n * m + n + m
+    ctx.samples_set(sig, precision=1)
+    ctx.sr_build(2, precision=1)
+    for mode in (0, 8, 16):
+        ctx.graph_mode(mode if mode != 8 else True)
+        x, res, it = ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000)
+        assert res < 1e-8
+        xb, resb, itb = ctx.sr_solve(1e-3, tol=1e-8, max_iter=it)
+        assert itb == it and resb == res
+        np.testing.assert_array_equal(xb, x)
+        for bad in (np.nan, np.inf):
+            b = rng.normal(size=dim)
+            b[dim // 3] = bad
+            with pytest.raises(yq.YqError) as ei:
+                ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000, rhs=b)
+            assert ei.value.code == yq.ERR_NAN, str(ei.value)
+    ctx.graph_mode(True)
+    ctx.samples_set(sig, precision=0)
+    ctx.sr_build(2, precision=0)
+    b = rng.normal(size=dim)
+    b[7] = np.nan
+    with pytest.raises(yq.YqError) as ei:
+        ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000, rhs=b)
+    assert ei.value.code == yq.ERR_NAN, str(ei.value)

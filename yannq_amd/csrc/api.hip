@@ -495,7 +495,7 @@ int sr_build_c(yq_ctx* c, int solver) {
 
 // The SR system of a real machine as independent diagonal blocks: one block
 // for an RBM; for an AmplitudePhase machine the amplitude and the phase
-// parameters, whose metric Re S has no cross block (NGDExact.hpp:172-184:
+// parameters, whose metric Re S has no cross block (NGDExact.hpp:177-189:
 // with O = [O_amp | i O_phase] the cross terms of <O^* O^T> are imaginary).
 // Block b: parameters [off, off + d), centred rows O (ns x d row-major, i.e.
 // d x ns column-major) and the lower triangle of its S (d x d) at S.
@@ -515,7 +515,7 @@ std::vector<SrBlock> sr_blocks(yq_ctx* c) {
 // AmplitudePhase machine, FP64: rows [O_RBM / 2 | pi dFF] (AmplitudePhase.hpp:
 // 42-58; the phase block multiplies i), F = Re(<O^* E_loc> - <O^*><E_loc>) =
 // [cov(O_amp, Re E_loc) | cov(O_phase, Im E_loc)], S = Re(S) block-diagonal
-// (the blocks of NGDExact::corrMatAmp / corrMatPhase, NGDExact.hpp:172-184).
+// (the blocks of NGDExact::corrMatAmp / corrMatPhase, NGDExact.hpp:177-189).
 int sr_build_ap(yq_ctx* c, int solver, int precision) {
   if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG)
     return fail(c, YQ_ERR_UNSUPPORTED,
@@ -837,6 +837,8 @@ int cg64(yq_ctx* c, double shift, const double* b, double* x, double tol, int ma
   }
   *iters = it;
   *err = sqrt(rn2 / bn2);
+  if (!std::isfinite(*err))
+    return fail(c, YQ_ERR_NAN, "non-finite right-hand side or residual in the CG");
   return rn2 < thr ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
@@ -968,6 +970,8 @@ int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
     TRY(sync(c));
     bn2 = h[0];
     *rel = bn2 == 0.0 ? 0.0 : sqrt(h[1] / bn2);
+    if (!std::isfinite(*rel))
+      return fail(c, YQ_ERR_NAN, "non-finite right-hand side or residual in the mixed solve");
     return YQ_OK;
   };
   *iters = 0;
@@ -991,6 +995,7 @@ int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
   }
   if (c->precond) TRY(ensure_diag(c));
   int total = 0;
+  // (a NaN or infinite b also stops the inner CG at once with YQ_ERR_NAN)
   for (int outer = 0; outer < 30 && total < max_iter && !(rel < tol); ++outer) {
     int it = 0;
     const double inner_tol = std::max(tol / rel, 1e-4) < 1.0 ? std::max(tol / rel, 1e-4) : 0.5;
@@ -1298,7 +1303,7 @@ YQ_API int yq_sampler_config(yq_ctx* c, int sweeper, int n_temps, int n_chains_p
                              int sweep_factor, int precision, uint64_t seed) {
   TRY(need_rbm(c));
   if (sweeper != YQ_LOCAL && sweeper != YQ_EXCHANGE) return fail(c, YQ_ERR_ARG, "unknown sweeper");
-  // SamplerMT.hpp:47: nTmps even or 1; one block of nTmps wavefronts per chain group
+  // SamplerMT.hpp:48: nTmps even or 1; one block of nTmps wavefronts per chain group
   if (n_temps < 1 || n_temps > 16 || (n_temps > 1 && n_temps % 2))
     return fail(c, YQ_ERR_ARG, "n_temps must be 1 or even and <= 16");
   if (n_temps > 1 && (c->cplx || c->ap))
@@ -1385,7 +1390,7 @@ int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   TRY(need_rbm(c));
   if (!c->have_sampler) return fail(c, YQ_ERR_STATE, "call yq_sampler_config first");
   if (n_sweeps < 1 || n_discard < 0) return fail(c, YQ_ERR_ARG, "n_sweeps >= 1, n_discard >= 0");
-  // one sample per chain group and sweep (the beta = 1 slot, SamplerMT.hpp:163-171)
+  // one sample per chain group and sweep (the beta = 1 slot, SamplerMT.hpp:166-173)
   const long long ns = (long long)(c->chains / c->ntemps) * n_sweeps;
   if (ns > (1ll << 30)) return fail(c, YQ_ERR_ARG, "too many samples");
   c->ns = (int)ns;

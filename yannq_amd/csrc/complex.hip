@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_eloc_combine_c(const ElocArgs A) {
   reinterpret_cast<double2*>(A.eloc)[s] = make_double2(er, ei);
 }
 
-// E = Re <E_loc>, Var = <(Re E_loc)^2> - E^2 (Energy.hpp:59-70), one block
+// E = Re <E_loc>, Var = <(Re E_loc)^2> - E^2 (Energy.hpp:53-58), one block
 __global__ void __launch_bounds__(1024) k_eloc_stats_c(const double* e, int ns, double* out) {
   __shared__ double s1[1024], s2[1024];
   double a = 0.0, b = 0.0;

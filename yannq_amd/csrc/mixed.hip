@@ -42,10 +42,12 @@ constexpr int kMaxIt = 10;      // Kp / 16
 constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
 
 // device control block of the CG loop: ints [stop flag, completed iterations,
-// iteration budget, converged flag (<r, r> below the threshold when the loop
-// stopped), chunks published to the host]; the first four are what the host
+// iteration budget, outcome (kConvYes: <r, r> below the threshold when the
+// loop stopped, kConvNonFinite: <b, b> or <r, r> was NaN or infinite, 0:
+// neither), chunks published to the host]; the first four are what the host
 // polls
 enum { kCtlStop = 0, kCtlIters = 1, kCtlMaxIt = 2, kCtlConv = 3, kCtlChunk = 4, kCtlInts = 8 };
+enum { kConvYes = 1, kConvNonFinite = 2 };
 constexpr int kSlotInts = 8;  // one host slot: 4 control ints, <r, r>, threshold
 // and doubles [shift, stopping threshold on <r, r>, alpha x 2, gamma x 2, the
 // latest <r, r> (what the host's convergence forecast reads)]
@@ -601,12 +603,15 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
     const double rn2 = tot[0], gam = tot[1], uu = tot[2];
     if (lb == 0 && t == 0) dctl[kCtlRr] = rn2;
     // CG continuation predicate (uniform over the grid): stop once <r, r>
-    // falls below the threshold (or is NaN) or the iteration budget is spent;
-    // the later kernels of the iteration see the flag and return at once
-    if (ictl[kCtlStop] || !(rn2 >= dctl[kCtlThr]) || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
-      if (lb == 0 && t == 0) {
+    // falls below the threshold, is not finite, or the iteration budget is
+    // spent; the later kernels of the iteration see the flag and return at
+    // once
+    const bool fin = __builtin_isfinite(rn2);
+    const bool conv = fin && rn2 < dctl[kCtlThr];
+    if (ictl[kCtlStop] || conv || !fin || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
+      if (lb == 0 && t == 0 && !ictl[kCtlStop]) {
         ictl[kCtlStop] = 1;
-        if (!(rn2 >= dctl[kCtlThr])) ictl[kCtlConv] = 1;
+        ictl[kCtlConv] = conv ? kConvYes : fin ? 0 : kConvNonFinite;
       }
       return;
     }
@@ -765,7 +770,8 @@ __global__ void k_set_ctl(double* dctl, int* ictl, double shift, double thr, int
 // Start of a CG run without a host round trip: <b, b> = the sum of the <r, r>
 // partials of slot 0 in block order (staged in LDS, added by one thread: the
 // order of a host loop), then the threshold tol^2 <b, b> and the budget;
-// b = 0 stops the run at once (x stays 0, reported as converged).
+// b = 0 stops the run at once (x stays 0, reported as converged), a NaN or
+// infinite <b, b> too (reported as non-finite).
 __global__ void __launch_bounds__(256)
     k_start_rr(const double* __restrict__ rr0, int nbv, double tol, int max_it, double* dctl,
                int* ictl) {
@@ -781,9 +787,9 @@ __global__ void __launch_bounds__(256)
   }
   if (threadIdx.x == 0) {
     ictl[kCtlIters] = 0;
-    if (bn2 == 0.0) {
+    if (bn2 == 0.0 || !__builtin_isfinite(bn2)) {
       ictl[kCtlStop] = 1;
-      ictl[kCtlConv] = 1;
+      ictl[kCtlConv] = bn2 == 0.0 ? kConvYes : kConvNonFinite;
     } else {
       dctl[kCtlThr] = tol * tol * bn2;
       ictl[kCtlStop] = 0;
@@ -797,7 +803,22 @@ __global__ void __launch_bounds__(256)
 // ints, then the two doubles): the last node of every chunk of iterations, so
 // the host reads the state after chunk k in slot k & 1 once the chunk's event
 // has completed (the event's completion makes the stores visible to the host).
-__global__ void k_publish(int* ictl, const double* __restrict__ dctl, int* host_ctl) {
+// A run that spent its budget before any k_cg_step tested the residual of its
+// last update (the budget ends on a chunk boundary) is tested here, on the
+// <r, r> partials that update left (slot iters & 1, rr0 / rr1), summed in the
+// order k_cg_step sums them.  One wave.
+__global__ void __launch_bounds__(64)
+    k_publish(int* ictl, double* __restrict__ dctl, int* host_ctl,
+              const double* __restrict__ rr0, const double* __restrict__ rr1, int nbv) {
+  if (!ictl[kCtlStop] && ictl[kCtlIters] >= ictl[kCtlMaxIt]) {  // uniform
+    const double rn2 = psum((ictl[kCtlIters] & 1) ? rr1 : rr0, nbv);
+    if (threadIdx.x == 0) {
+      const bool fin = __builtin_isfinite(rn2);
+      dctl[kCtlRr] = rn2;
+      ictl[kCtlStop] = 1;
+      ictl[kCtlConv] = fin && rn2 < dctl[kCtlThr] ? kConvYes : fin ? 0 : kConvNonFinite;
+    }
+  }
   if (threadIdx.x == 0) {
     const int k = ictl[kCtlChunk];
     ictl[kCtlChunk] = k + 1;
@@ -858,6 +879,12 @@ RedLayout red_layout(const MixedWork& w, int chunk = 0) {
   L.yy = L.ys + L.nby;
   L.total = L.yy + L.nby;
   return L;
+}
+
+void publish_launch(MixedWork& w, hipStream_t st) {
+  const RedLayout L = red_layout(w);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev,
+                     w.red + L.rr[0], w.red + L.rr[1], L.nbv);
 }
 
 // the two products for the u last split into w.vsplit by k_cg_step, leaving
@@ -1037,7 +1064,7 @@ __global__ void k_group_rr(const GroupDesc* __restrict__ gd, int count, int whic
   if (which == 0) {
     const double bn2 = rr_total(d, 0);
     stat[q] = bn2;
-    if (bn2 == 0.0) {
+    if (bn2 == 0.0 || !__builtin_isfinite(bn2)) {
       d.ictl[kCtlStop] = 1;
     } else {
       d.dctl[kCtlThr] = d.tol * d.tol * bn2;
@@ -1092,7 +1119,7 @@ int mixed_graph(MixedWork& w, const float* dinv, hipStream_t st, hipGraphExec_t*
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hipfail(err, "graph capture", e);
     for (int p = 0; p < w.chunk_iters; ++p) iteration_launches(w, dinv, (1 + p) & 1, false, st);
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev);
+    publish_launch(w, st);
     e = hipStreamEndCapture(st, &g);
     if (e != hipSuccess) return hipfail(err, "graph capture", e);
     e = hipGraphInstantiate(&w.gexec[slot], g, nullptr, nullptr, 0);
@@ -1311,7 +1338,7 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
     } else {
       for (int k = 0; k < w.chunk_iters; ++k)
         iteration_launches(w, dinv, (launched + k) & 1, false, st);
-      hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev);
+      publish_launch(w, st);
     }
     launched += w.chunk_iters;
     hipError_t r = hipEventRecord(w.ev[issued & 1], st);
@@ -1320,7 +1347,7 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   };
   if (launched < max_iter && (e = issue()) != hipSuccess) return hipfail(err, "mixed_cg", e);
   if (issued == 0) {  // a budget of one iteration: publish the state after it
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, w.ictl, w.dctl, w.host_ctl_dev);
+    publish_launch(w, st);
     if ((e = hipEventRecord(w.ev[0], st)) != hipSuccess) return hipfail(err, "mixed_cg", e);
     issued = 1;
   }
@@ -1357,9 +1384,13 @@ int mixed_cg(MixedWork& w, double shift, const double* omean64, const double* di
   if (e != hipSuccess) return hipfail(err, "mixed_cg", e);
   *iters = last[kCtlIters];
   // converged: the device's stop predicate saw <r, r> below the threshold
-  // (or b = 0); a run that spent its budget without that test reports
-  // YQ_ERR_NOT_CONVERGED
-  return last[kCtlConv] ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+  // (or b = 0); a run that spent its budget with <r, r> above it reports
+  // YQ_ERR_NOT_CONVERGED, a NaN or infinite <b, b> or <r, r> YQ_ERR_NAN
+  if (last[kCtlConv] == kConvNonFinite) {
+    err = "non-finite residual in the mixed-precision CG";
+    return YQ_ERR_NAN;
+  }
+  return last[kCtlConv] == kConvYes ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
 namespace {
@@ -1594,10 +1625,14 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
     if (!j.active) continue;
     const double bn2 = g.host_stat[q];
     if (bn2 == 0.0) continue;  // x = 0 after no iterations
+    if (!std::isfinite(bn2)) {
+      j.rc = YQ_ERR_NAN;
+      continue;
+    }
     const double thr = j.tol * j.tol * bn2;
     const double rn2 = g.host_stat[count + q];
     j.iters = (int)g.host_stat[2 * count + q];
-    j.rc = !(rn2 >= thr) ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+    j.rc = !std::isfinite(rn2) ? YQ_ERR_NAN : rn2 < thr ? YQ_OK : YQ_ERR_NOT_CONVERGED;
   }
   return YQ_OK;
 }

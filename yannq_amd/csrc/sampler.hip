@@ -12,7 +12,7 @@
 // Semantics: LocalSweeper.hpp:32-53 (single flips, k ~ U{0..n-1},
 // p = min(1, exp(2 beta Re dlogpsi)), accept iff u < p) and
 // SwapSweeper.hpp:28-56 (k, l ~ U{0..n-1} independently, skip if parallel),
-// scheduled as SamplerMT::sample, SamplerMT.hpp:293-317 (n_discard sweeps,
+// scheduled as SamplerMT::sample, SamplerMT.hpp:180-204 (n_discard sweeps,
 // then n_sweeps sweeps each followed by recording every chain).
 // Arithmetic: yq_math.h; the log-ratio is the lane-strided sum followed by a
 // stride-32..1 tree, which the CPU replay reproduces exactly.
