@@ -9,14 +9,19 @@ Timed path (value): mixed precision -- sampling and E_loc in float, O kept
 factored (spins and tanh(theta)), matrix-free CG whose products run as
 split-bf16 MFMA with float accumulation, FP64 iterative refinement to a 1e-8
 relative residual, FP64 update.  The inner CG runs as captured HIP-graph
-chunks of 8 iterations that stop themselves on the device.  The CG starts from zero without a
+chunks that stop themselves on the device.  The CG starts from zero without a
 preconditioner, as the reference's solveCG does (--precond/--warm-start
-switch on the Jacobi preconditioner and warm starts; both solve the same
-system to the same tolerance, and on this workload both cost iterations).
-Also timed on the same start state: the plain FP64 path (explicit S =
-O_c^T O_c / Ns and Cholesky, the reference's default solver), the FP64
-factored CG, and the mixed path with both solver options.  Prints one JSON
-line.
+switch on the Jacobi preconditioner and warm starts).
+
+Every path below runs the same warm-up and timed steps from the same
+parameters and chain seeds, so their rates and CG iteration counts cover the
+same step window: the plain FP64 path (explicit S = O_c^T O_c / Ns and
+Cholesky, the reference's default solver), the FP64 factored CG (FP64
+sampling, FP64 MFMA products), and the mixed solver fed by FP64 sampling
+(its chains are those of the FP64 paths, so its energies differ from theirs
+only through the mixed solve).  Every timed solve must reach the 1e-8 FP64
+residual: a step that does not makes the run exit with status 1.  Prints one
+JSON line.
 """
 from __future__ import annotations
 
@@ -24,6 +29,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import platform
 import sys
 import time
 
@@ -41,26 +47,36 @@ SEED = 20241203
 NS = CHAINS
 KP, NP = 128, 416        # padded (n+1) and (m+1) of the MFMA grids
 
-# algorithmic HBM bytes per launch of the kernels that can dominate a step
+# Algorithmic HBM bytes per launch: the per-unit figures of the workload's
+# measurement table times the units one launch processes.  A product of the
+# factored operator (O_c u or O_c^T y) streams the spins as int8 (Ns x n) and
+# T = tanh(theta) as float (Ns x m) once, reads its dim- or Ns-long input and
+# writes its Ns- or dim-long result; the CG vector update reads and writes 11
+# dim-length float vectors and writes the three bf16 layers of the split
+# vector.  Chunk partial sums, bf16 spin copies and padding are this
+# implementation's own traffic: they appear in the measured `traffic`, not
+# here.
+FACTORS = NS * N + NS * M * 4
 KERNEL_WORK = {
-    # forward product: T' (Ns x (m+1) float) and sigma' (Ns x (n+1) bf16) in, y partials out
-    "fwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4,
-    # backward product: T'^T and sigma'^T and y in, the (n+1) x (m+1) chunk partials out
-    "bwd_mfma": NS * (M + 1) * 4 + NS * (N + 1) * 2 + NS * 4 + (NS // 256) * KP * NP * 4,
-    # CG step: the chunk partials of the dim parameters in; x, r, p, s, u, omean
-    # in; x, r, p, s, u and the three bf16 layers of V' out
-    "cg_step": (NS // 256) * DIM * 4 + DIM * 4 * 11 + 3 * KP * NP * 2,
+    "fwd_mfma": FACTORS + DIM * 4 + NS * 4,
+    "bwd_mfma": FACTORS + NS * 4 + DIM * 4,
+    "cg_step": 11 * DIM * 4 + 3 * KP * NP * 2,
     # sampler: spins in/out and the recorded samples (spins, float theta)
     "sample": NS * N * 3 + NS * M * 4,
     # E_loc ratios: theta (float) and spins in, ratios (double) out
     "eloc_ratio": NS * M * 4 + NS * N + NS * N * 8,
 }
+VALU_KERNELS = ("sample", "eloc_ratio")
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of this script, each counter scaled by what it reads
 # for a copy of known size (bench.py --copy-probe); written with the command
 # lines that produced it
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_tfi_n100_a4.json")
+# VALU utilisation of the sampler and E_loc kernels from a rocprofv3 SQ
+# counter pass (same provenance rules)
+VALU_FILE = os.path.join(ROOT, "profiles", "pmc_valu_sample_eloc_tfi_n100_a4.json")
 
 
 AUTOTUNE = False
@@ -78,13 +94,17 @@ def _ctx(yq, precision):
     return ctx
 
 
-def run_path(yq, precision, solver, warmup, steps, precond=0, warm_start=0, prof_label=None):
-    """warmup + steps SR epochs from the same seeds; returns the timed seconds,
-    the per-step outputs, all energies and the HIP-event kernel timings."""
-    ctx = _ctx(yq, precision)
-    kw = dict(n_sweeps=1, solver=solver, precision=precision, lam=LAMBDA, cg_tol=TOL,
-              cg_max_iter=50000, lr=LR, precond=precond, warm_start=warm_start,
-              allow_unconverged=True)
+def run_path(yq, sample_precision, solver, warmup, steps, solve_precision=None, precond=0,
+             warm_start=0, prof_label=None):
+    """warmup + steps SR epochs from the same parameters and chain seeds with
+    sampling in sample_precision and the solve in solve_precision (default:
+    the same); returns the timed seconds, the per-step outputs of the timed
+    steps, the energies of all steps and the HIP-event kernel timings."""
+    ctx = _ctx(yq, sample_precision)
+    kw = dict(n_sweeps=1, solver=solver,
+              precision=sample_precision if solve_precision is None else solve_precision,
+              lam=LAMBDA, cg_tol=TOL, cg_max_iter=50000, lr=LR, precond=precond,
+              warm_start=warm_start, allow_unconverged=True)
     energies, outs = [], []
     for i in range(warmup):
         o = ctx.sr_step(n_discard=20 if i == 0 else 0, **kw)
@@ -111,48 +131,70 @@ def run_path(yq, precision, solver, warmup, steps, precond=0, warm_start=0, prof
     return (t1 - t0), outs, energies, prof
 
 
-def cpu_baseline(iters_per_step):
-    """The oracle's threaded C++ restatement of the reference CG step
-    (SamplerMT + constructDelta/constructObs + SRMat::solveCG), timed on a
-    bounded sample of chains and scaled linearly in samples."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(params, steps=3, threads=16):
+    """The oracle's threaded C++ restatement of the reference CG step, timed
+    at full size: per step SamplerMT.sample(1, 0) over 16,384 chains, E_loc,
+    constructFromSamples with O materialised (16,384 x 40,500 doubles), CG on
+    O_c to the same 1e-8 relative residual, SGD update -- RunRBM's loop body.
+    Starts from the GPU run's initial parameters and randomised chains (no
+    thermalisation: every sweep costs the same); returns the median step."""
     import oracle as orc
-    threads = min(16, orc.lib().bl_threads())
-    rng = np.random.default_rng(SEED)
-    rbm = orc.RBM.random(N, M, 0.01, rng)
+    L = orc.lib()
+    rbm = orc.RBM(N, M)
+    rbm.set_params(params)
     H = orc.ham(orc.TFI, N, J=-1.0, h=-1.0)
-    c = 2048
-    sig = orc.randomize(SEED, 0, np.arange(c), N)
-    t0 = time.perf_counter()
-    _, _, _, ss, st = orc.sample_replay(rbm, sig, seed=SEED, n_sweeps=1, n_discard=0,
-                                        threads=threads)
-    t_samp = time.perf_counter() - t0
-    el = np.empty(c)
-    t0 = time.perf_counter()
-    orc.lib().bl_eloc(C.byref(H), c, N, M, orc._p(rbm.W), orc._p(rbm.a), orc._p(ss),
-                      orc._p(st), orc._p(el), threads)
-    t_eloc = time.perf_counter() - t0
-    O = np.empty((c, DIM))
-    om, F = np.empty(DIM), np.empty(DIM)
-    t0 = time.perf_counter()
-    orc.lib().bl_sr_build(c, N, M, 1, orc._p(ss), orc._p(st), orc._p(el), orc._p(O),
-                          orc._p(om), orc._p(F), threads)
-    t_build = time.perf_counter() - t0
-    x = np.empty(DIM)
+    streams = np.arange(CHAINS, dtype=np.uint32)
+    sig = orc.randomize(SEED, 0, streams, N)
+    ctr = np.zeros(CHAINS, np.uint64)
+    acc = np.zeros(CHAINS, np.uint64)
+    O = np.empty((CHAINS, DIM))
+    om, F, x, el = np.empty(DIM), np.empty(DIM), np.empty(DIM), np.empty(CHAINS)
     it, err = C.c_int(), C.c_double()
-    t0 = time.perf_counter()
-    orc.lib().bl_cg_solve(c, DIM, orc._p(O), LAMBDA, orc._p(F), 1e-30, 20, orc._p(x),
-                          C.byref(it), C.byref(err), threads)
-    t_cg_iter = (time.perf_counter() - t0) / max(it.value, 1)
-    scale = CHAINS / c
-    t_step = scale * (t_samp + t_eloc + t_build + t_cg_iter * iters_per_step)
+    times, phases, iters = [], [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        sig, ctr, acc, ss, st = orc.sample_replay(rbm, sig, seed=SEED, n_sweeps=1, n_discard=0,
+                                                  ctr=ctr, acc=acc, threads=threads)
+        t1 = time.perf_counter()
+        L.bl_eloc(C.byref(H), CHAINS, N, M, orc._p(rbm.W), orc._p(rbm.a), orc._p(ss),
+                  orc._p(st), orc._p(el), threads)
+        t2 = time.perf_counter()
+        L.bl_sr_build(CHAINS, N, M, 1, orc._p(ss), orc._p(st), orc._p(el), orc._p(O),
+                      orc._p(om), orc._p(F), threads)
+        t3 = time.perf_counter()
+        L.bl_cg_solve(CHAINS, DIM, orc._p(O), LAMBDA, orc._p(F), TOL, 50000, orc._p(x),
+                      C.byref(it), C.byref(err), threads)
+        rbm.set_params(rbm.params() - LR * x)
+        t4 = time.perf_counter()
+        times.append(t4 - t0)
+        phases.append([t1 - t0, t2 - t1, t3 - t2, t4 - t3])
+        iters.append(it.value)
+        print(f"# cpu baseline step: {t4 - t0:.1f} s, {it.value} CG iterations", file=sys.stderr,
+              flush=True)
+    k = int(np.argsort(times)[len(times) // 2])
     return {
-        "value": 1.0 / t_step, "unit": "SR steps/s", "cores": threads, "kind": "port",
-        "sample": (f"{c} chains x 1 sweep, E_loc, O build and {it.value} CG applies on "
-                   f"{c} samples at N={N}, M={M}, scaled x{scale:.0f} to {CHAINS} samples "
-                   f"with {iters_per_step} CG iterations per step (the GPU FP64 CG count)"),
-        "phase_s_at_full_size": {"sample": scale * t_samp, "eloc": scale * t_eloc,
-                                 "build_O_F": scale * t_build,
-                                 "cg": scale * t_cg_iter * iters_per_step},
+        "value": 1.0 / float(np.median(times)), "unit": "SR steps/s", "cores": threads,
+        "kind": "port",
+        "sample": (f"{steps} full steps timed, median taken: {CHAINS} chains x 1 sweep at "
+                   f"N={N}, M={M}, E_loc, O materialised ({CHAINS} x {DIM} doubles), centring "
+                   f"and F, CG on O_c to a {TOL:g} relative residual, SGD update; "
+                   f"{threads} threads"),
+        "step_s": times, "cg_iters": iters,
+        "median_step_phase_s": dict(zip(["sample", "eloc", "build_O_F", "cg_and_update"],
+                                        phases[k])),
+        "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                 "threads_used": threads},
     }
 
 
@@ -390,12 +432,28 @@ def batched(yq, warmup, steps):
     print(json.dumps(line))
 
 
+def measured_valu(kernel):
+    """VALU figures of a VALU-bound kernel from the committed SQ counter pass,
+    or None."""
+    try:
+        with open(VALU_FILE) as f:
+            return json.load(f).get("kernels", {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def _rel_diff(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    return np.abs(a - b) / np.abs(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--fp64-explicit-steps", type=int, default=-1,
+                    help="timed steps of the FP64 explicit path (default: --steps)")
     ap.add_argument("--precond", type=int, default=0)
     ap.add_argument("--warm-start", type=int, default=0)
     ap.add_argument("--autotune", type=int, default=0,
@@ -403,6 +461,7 @@ def main():
     ap.add_argument("--graph-chunk", type=int, default=16,
                     help="CG iterations per captured HIP graph of the mixed solve (even)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the copy that calibrates the PMC traffic counters")
@@ -440,22 +499,28 @@ def main():
     AUTOTUNE = bool(args.autotune)
     GRAPH_CHUNK = args.graph_chunk
     opts = dict(precond=args.precond, warm_start=args.warm_start)
+    W, K = args.warmup, args.steps
+
+    def log(msg):
+        print(f"# {msg}", file=sys.stderr, flush=True)
 
     # pick the dominant kernel from a profiled warm-up of the timed path
-    _, _, _, prof0 = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, max(args.warmup, 1), 2,
+    _, _, _, prof0 = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, max(W, 1), 2,
                               prof_label=",".join(KERNEL_WORK), **opts)
     dominant = max(prof0, key=lambda k: prof0[k][0])
+    log(f"kernel selection pass done: dominant kernel {dominant}")
 
     # timed pass: no per-kernel events, so the CG runs as captured graph chunks
-    dt, outs, energies, _ = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, args.warmup,
-                                     args.steps, **opts)
-    ms = 1e3 * dt / args.steps
-    value = args.steps / dt
+    dt, outs, energies, _ = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, W, K, **opts)
+    ms = 1e3 * dt / K
+    value = K / dt
     iters = [o["cg_iters"] for o in outs]
+    errs = [o["cg_err"] for o in outs]
+    log(f"mixed pass done: {value:.2f} SR steps/s, CG iterations {min(iters)}..{max(iters)}")
     # kernel timing pass: the same steps again with HIP events around every
     # launch of the dominant kernel (eager launches, on the kernel's stream)
-    dtk, _, _, prof = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, args.warmup,
-                               args.steps, prof_label=dominant, **opts)
+    dtk, _, _, prof = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, W, K, prof_label=dominant,
+                               **opts)
     tot_ms, launches = prof[dominant]
     amount = KERNEL_WORK[dominant]
     avg_s = tot_ms / 1e3 / max(launches, 1)
@@ -482,39 +547,90 @@ def main():
         d = {"avg_launch_us": avg * 1e6, "launches_per_step": cnt / 2, "ms_per_step": t_ms / 2,
              "algorithmic_bytes_per_launch": KERNEL_WORK[lab], "achieved_gbs": gbs,
              "hbm_frac": gbs / HBM_PEAK_GBS,
-             "bound": "valu" if lab in ("sample", "eloc_ratio") else "hbm"}
-        if lab in ("sample", "eloc_ratio"):  # one sweep / one pass: Ns * n * m terms
+             "bound": "valu" if lab in VALU_KERNELS else "hbm"}
+        if lab in VALU_KERNELS:  # one sweep / one pass: Ns * n * m terms
             d["logcosh_diff_terms_per_s"] = NS * N * M / avg
+            d["valu_counters"] = measured_valu(lab)
         per_kernel[lab] = d
 
     extra = {}
-    fp64_iters = iters
     if not args.no_fp64:
-        k = args.fp64_steps
-        dte, _, ene, _ = run_path(yq, yq.FP64, yq.SOLVE_EXPLICIT, args.warmup, k)
-        dtc, outc, _, _ = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, args.warmup, k, **opts)
-        dtp, outp, _, _ = run_path(yq, yq.MIXED, yq.SOLVE_CG_FACTORED, args.warmup, k,
-                                   precond=1, warm_start=1)
+        # the FP64 paths and the mixed solver on FP64 samples, over the same
+        # warm-up and timed steps (the same step window as the mixed pass)
+        dtc, outc, enc, profc = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, W, K,
+                                         prof_label="factored_fwd64,factored_bwd64", **opts)
+        log(f"FP64 factored CG pass done: {K / dtc:.2f} SR steps/s")
+        dtc2, _, _, _ = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, W, K, **opts)
+        dts, outs_s, ens, _ = run_path(yq, yq.FP64, yq.SOLVE_CG_FACTORED, W, K,
+                                       solve_precision=yq.MIXED, **opts)
+        log("mixed solver on FP64 samples done")
+        ke = K if args.fp64_explicit_steps < 0 else args.fp64_explicit_steps
+        dte, oute, ene, _ = run_path(yq, yq.FP64, yq.SOLVE_EXPLICIT, W, ke)
+        log(f"FP64 explicit pass done: {ke / dte:.3f} SR steps/s")
         fp64_iters = [o["cg_iters"] for o in outc]
+        errs += [o["cg_err"] for o in outc + outs_s + oute]
+        # per-launch FP64 products of the comparator (one label covers the MFMA
+        # product and its small reduction kernel) against their FP64-MFMA floor
+        comp = {}
+        for lab, (t_ms, cnt) in profc.items():
+            if cnt:
+                avg_us = 1e3 * t_ms / cnt
+                flop = 2.0 * NS * N * M
+                comp[lab] = {"avg_launch_us": avg_us, "launches": cnt,
+                             "algorithmic_flop": flop,
+                             "floor_us": flop / (FP64_MFMA_PEAK_TFLOPS * 1e6),
+                             "frac_of_fp64_mfma_peak": flop / (avg_us * 1e-6) /
+                             (FP64_MFMA_PEAK_TFLOPS * 1e12)}
+        ws = len(energies)
+        rel_f = _rel_diff(energies, enc)
+        rel_s = _rel_diff(ens, enc)
+        rel_e = _rel_diff(ene[:ws], enc[:len(ene)])
+        last = outs[-1]
+        last_c = outc[-1]
         extra = {
-            "fp64_explicit_steps_per_s": k / dte,
-            "fp64_factored_cg_steps_per_s": k / dtc,
-            "mixed_jacobi_warm_cg_steps_per_s": k / dtp,
-            "mixed_jacobi_warm_cg_iters": [o["cg_iters"] for o in outp],
-            "mixed_vs_fp64_explicit": value / (k / dte),
-            "mixed_vs_fp64_factored_cg": value / (k / dtc),
+            "fp64_factored_cg_steps_per_s": K / dtc,
+            "fp64_factored_cg_steps_per_s_repeat": K / dtc2,
+            "fp64_factored_cg_timing": "first figure from a pass with HIP events on the FP64 "
+                                       "products, the repeat without them",
+            "fp64_explicit_steps_per_s": ke / dte,
+            "fp64_explicit_timed_steps": ke,
+            "mixed_vs_fp64_factored_cg": value / (K / dtc2),
+            "mixed_vs_fp64_explicit": value / (ke / dte),
+            "ratio_note": "same warm-up and timed steps (from the same parameters and chain "
+                          "seeds) for every path; the FP64 comparator's MFMA products run at "
+                          "the fraction of the FP64 MFMA peak given in fp64_comparator_products",
             "fp64_cg_iters": fp64_iters,
-            # same seeds and start state: energies of the first steps of each path
-            "energy_rel_diff_mixed_vs_fp64": float(np.max(np.abs(
-                np.array(energies[:len(ene)]) - np.array(ene)) / np.abs(np.array(ene)))),
+            "fp64_comparator_products": comp,
+            "mixed_solver_fp64_sampling_steps_per_s": K / dts,
+            # energies of every step (warm-up and timed) of each path
+            "energy_rel_diff_mixed_vs_fp64_max": float(rel_f.max()),
+            "energy_rel_diff_mixed_vs_fp64_per_step": [float(v) for v in rel_f],
+            "energy_rel_diff_mixed_solver_vs_fp64_max": float(rel_s.max()),
+            "energy_rel_diff_fp64_explicit_vs_fp64_cg_max": float(rel_e.max()),
+            "energy_comparison": (
+                "mixed_vs_fp64: the timed mixed path (float sampling) against the FP64 "
+                "factored CG path, all %d steps; float theta changes some accept decisions, so "
+                "the two Markov chains and then the parameters drift apart stochastically. "
+                "mixed_solver_vs_fp64: the mixed solve fed by the FP64 path's own chains, all "
+                "%d steps: the difference the mixed arithmetic itself makes" % (ws, ws)),
+            "energy_last_step": {"mixed": energies[-1], "fp64": enc[-1],
+                                 "mixed_solver_fp64_sampling": ens[-1],
+                                 "mc_std_err_fp64": float(np.sqrt(last_c["energy_var"] / NS)),
+                                 "mc_std_err_mixed": float(np.sqrt(last["energy_var"] / NS))},
         }
+    err_max = float(max(errs))
+    converged = err_max < TOL
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(int(np.median(fp64_iters)))
+        p0 = None
+        c0 = _ctx(yq, yq.FP64)
+        p0 = c0.get_params()
+        c0.close()
+        cpu = cpu_baseline(p0, steps=args.cpu_steps)
     line = {
         "metric": "SR steps/sec (sample+E_loc+O+SR solve), TFI N=100 RBM alpha=4, mixed vs FP64",
-        "value": value, "unit": "SR steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "value": value, "unit": "SR steps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "1D TFI PBC N=100 h=1 (J=-1), real RBM alpha=4 (40500 params), "
                                "16384 chains x 1 sweep = 16384 samples/step, persistent chains",
@@ -527,6 +643,8 @@ def main():
         "samples_per_s": value * CHAINS,
         "median_step_ms": float(np.median([o["wall_ms"] for o in outs])),
         "cg_iters": iters,
+        "cg_err_max": err_max,
+        "cg_converged": converged,
         "energy": energies[-1],
         "phase_ms": {"sample": float(np.mean([o["sample_ms"] for o in outs])),
                      "eloc_build_solve_update": float(np.mean([o["solve_ms"] for o in outs]))},
@@ -535,7 +653,11 @@ def main():
         "cpu_baseline": cpu,
         **extra,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    if not converged:
+        print(f"bench.py: a timed solve ended above the {TOL:g} relative residual "
+              f"(max {err_max:.3g})", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":
