@@ -369,6 +369,17 @@ YQ_API int yq_ff_jacobian_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
                               void* d_jac, void* d_logpsi);
 YQ_API int yq_ff_vjp_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,
                          const void* d_act, const void* d_r, void* d_grad);
+/* yq_ff_vjp_dev without any device allocation.  No reference counterpart:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  The per-block gradient partials live in
+ * a fixed 16 MiB array that the library holds on every device it runs on
+ * until the process ends, so npar x element size may be at most 16 MiB
+ * (YQ_ERR_ARG beyond); with fewer blocks than yq_ff_vjp_dev uses when the
+ * array is the limit, the result agrees with it to rounding.  Every call on a
+ * device shares that array: calls that overlap on one GPU (several contexts
+ * or host threads) race. */
+YQ_API int yq_ff_vjp_dev_noalloc(yq_ctx* ctx, int ns, const int8_t* d_sigma,
+                                 const void* d_act, const void* d_r, void* d_grad);
 
 /* ---- AmplitudePhase machine: Machines/AmplitudePhase.hpp ---------------
  * psi(sigma) = sqrt(RBM(sigma)) exp(i pi FF(sigma)) (:127-138): the context's

@@ -109,6 +109,7 @@ def lib():
         "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
         "yq_ff_jacobian": [p, i, p, p, p], "yq_ff_forward_dev": [p, i, p, p, p],
         "yq_ff_jacobian_dev": [p, i, p, p, p], "yq_ff_vjp_dev": [p, i, p, p, p, p],
+        "yq_ff_vjp_dev_noalloc": [p, i, p, p, p, p],
         "yq_ckpt_rbm_encode": [i, i, i, i, p, i, p, C.c_size_t],
         "yq_ckpt_rbm_decode": [i, p, C.c_size_t, i, p, p, p, p, C.c_size_t],
         "yq_rbm_save": [p, C.c_char_p, i, i], "yq_rbm_load": [p, C.c_char_p, i, i],
@@ -414,6 +415,11 @@ class Context:
 
     def ff_vjp_dev(self, ns, d_sigma, d_act, d_r, d_grad):
         self._chk(self._L.yq_ff_vjp_dev(self._h, ns, d_sigma, d_act, d_r, d_grad))
+
+    def ff_vjp_dev_noalloc(self, ns, d_sigma, d_act, d_r, d_grad):
+        """yq_ff_vjp_dev with its partial sums in the library's fixed device
+        array (no device allocation; calls overlapping on one GPU race)."""
+        self._chk(self._L.yq_ff_vjp_dev_noalloc(self._h, ns, d_sigma, d_act, d_r, d_grad))
 
     # AmplitudePhase machine: this context's real RBM (amplitude) and its
     # FeedForward network (phase, float64)

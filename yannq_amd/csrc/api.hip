@@ -1089,6 +1089,10 @@ YQ_API int yq_ctx_create(int device, yq_ctx** out) {
   }
   if (device < 0 || device >= count) return YQ_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return YQ_ERR_HIP;
+  // the fixed scratch array of the allocation-free entry point: its address
+  // on this device, looked up once per device with the device current (a
+  // failure is reported by that entry point, which is its only user)
+  (void)ff_scratch_init(device);
   yq_ctx* c = new yq_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
@@ -1981,6 +1985,32 @@ YQ_API int yq_ff_vjp_dev(yq_ctx* c, int ns, const int8_t* d_sigma, const void* d
   a.r = d_r;
   a.partial = c->ff_part.p;
   a.grad = d_grad;
+  const int rc = launch_ff(a, kFFVjp, c->ff.dtype, c->st);
+  if (rc) return fail(c, rc, "FeedForward VJP shape unsupported");
+  LAUNCHCHK(c);
+  return sync(c);
+}
+
+YQ_API int yq_ff_vjp_dev_noalloc(yq_ctx* c, int ns, const int8_t* d_sigma, const void* d_act,
+                                 const void* d_r, void* d_grad) {
+  TRY(need_ff(c));
+  if (ns < 1 || !d_sigma || !d_r || !d_grad) return fail(c, YQ_ERR_ARG, "need ns >= 1 and buffers");
+  void* scr = ff_scratch(c->device);
+  if (!scr) return fail(c, YQ_ERR_HIP, "the fixed VJP scratch array has no address on this device");
+  // one block's partial row is npar values; the array holds as many rows as
+  // fit, and the grid-stride kernel covers any ns with that many blocks
+  const size_t row = ff_rs(c) * (size_t)c->ff_npar;
+  const size_t rows = kFFScratchBytes / row;
+  if (rows < 1)
+    return fail(c, YQ_ERR_ARG, "FeedForward parameters beyond the fixed VJP scratch (npar x "
+                               "element size must be at most 16 MiB)");
+  FFArgs a = ff_args(c, ns, d_sigma);
+  a.act_in = d_act;
+  a.r = d_r;
+  a.partial = scr;
+  a.grad = d_grad;
+  const int nb = ff_vjp_blocks(ns, c->ff.dtype);
+  a.vjp_blocks = (size_t)nb < rows ? nb : (int)rows;
   const int rc = launch_ff(a, kFFVjp, c->ff.dtype, c->st);
   if (rc) return fail(c, rc, "FeedForward VJP shape unsupported");
   LAUNCHCHK(c);

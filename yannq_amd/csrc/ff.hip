@@ -16,6 +16,9 @@
 //         read back when stored by kFwd and recomputed otherwise.
 // Parameter layout [conv kernel (index ki + oc*K) | FC weight W(i, o) at
 // i + o*C*n], FeedForward.hpp's concatenation of the layers' getParams().
+#include <mutex>
+#include <vector>
+
 #include "yq_internal.h"
 #include "yq_math.h"
 
@@ -326,7 +329,8 @@ int launch_r(const FFArgs& a, int mode, hipStream_t st) {
     ProfGuard g("ff_jacobian", st);
     hipLaunchKernelGGL((k_ff<R, kJac, 0>), dim3(groups), dim3(256), lds, st, a);
   } else {
-    const int blocks = groups < 512 ? groups : 512;
+    const int blocks = a.vjp_blocks > 0 ? (a.vjp_blocks < groups ? a.vjp_blocks : groups)
+                                        : (groups < 512 ? groups : 512);
     ProfGuard g("ff_vjp", st);
     const int e = (npar + 255) / 256;
     if (e <= 32)
@@ -343,7 +347,42 @@ int launch_r(const FFArgs& a, int mode, hipStream_t st) {
   return YQ_OK;
 }
 
+// Per-block VJP partials of yq_ff_vjp_dev_noalloc: a fixed array instead of a
+// context allocation, for callers that cannot allocate device memory.  Every
+// call on a device shares that device's copy; each launch writes every
+// partial it reads (rows 0 .. blocks-1, all npar entries) before the
+// reduction reads them, so nothing has to start at zero.
+static __device__ __align__(256) unsigned char scratch[kFFScratchBytes];
+
 }  // namespace
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device ordinal
+}  // namespace
+
+int ff_scratch_init(int device) {
+  if (device < 0) return YQ_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[(size_t)device]) return YQ_OK;
+  // the array's host-side name is not a device address: take the address the
+  // runtime gave this device's copy
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return YQ_ERR_HIP;
+  }
+  g_scratch_ptr[(size_t)device] = p;
+  return YQ_OK;
+}
+
+void* ff_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device]
+                                                              : nullptr;
+}
 
 int ff_vjp_blocks(int ns, int use_double) {
   const int groups = (ns + (use_double ? 3 : 7)) / (use_double ? 4 : 8);

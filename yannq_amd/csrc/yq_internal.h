@@ -250,6 +250,7 @@ struct FFArgs {
   const void* r;        // VJP weights (ns)
   void* partial;        // VJP partials (ff_vjp_blocks x npar)
   void* grad;           // VJP result (npar)
+  int vjp_blocks;       // VJP grid size (0: ff_vjp_blocks(ns))
   // flip_B > 0: configuration q is the term-(q % flip_B) neighbour (see
   // bond_sites, Hamiltonian kind flip_kind) of sigma row q / flip_B, so ns
   // counts (sample, term) pairs
@@ -257,6 +258,13 @@ struct FFArgs {
 };
 enum { kFFForward = 0, kFFJacobian = 1, kFFVjp = 2 };
 int ff_vjp_blocks(int ns, int use_double);
+// The fixed device array behind yq_ff_vjp_dev_noalloc (kFFScratchBytes per
+// device, held until the process ends): ff_scratch_init looks up its address
+// on the current device once (call it with that device current); ff_scratch
+// returns the address recorded for a device, or null.
+constexpr size_t kFFScratchBytes = (size_t)16 << 20;
+int ff_scratch_init(int device);
+void* ff_scratch(int device);
 size_t ff_smem_bytes(const FFArgs& a, int use_double);
 int launch_ff(const FFArgs& a, int mode, int use_double, hipStream_t st);
 // a.wt from the FC weight of a.params (after every parameter upload)
