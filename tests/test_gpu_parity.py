@@ -132,6 +132,35 @@ def test_sr_build_explicit_parity(ctx, orc, use_bias):
     assert _rel(ctx.sr_apply(lam, v), S @ v + lam * v) < 1e-12
 
 
+@pytest.mark.parametrize("n,m,ns,use_bias", [
+    (7, 13, 37, True),      # one partial tile, ragged samples
+    (9, 30, 517, False),    # three tile rows, no bias
+    (12, 24, 4001, True),   # ragged everything
+    (8, 8, 20000, True),    # one tile, many samples: split into sample chunks
+])
+def test_syrk_shapes(ctx, orc, n, m, ns, use_bias):
+    """The hand-written FP64 MFMA S = O_c^T O_c / Ns (formed from the spins,
+    tanh(theta) and the column mean) against the oracle's S at ragged sizes:
+    dims that are not multiples of the 128 tile, sample counts that are not
+    multiples of the 16-sample stage, without bias, and a tall batch whose
+    samples are split into chunks summed in order."""
+    rbm = _setup(ctx, orc, n, m, scale=0.2, use_bias=use_bias)
+    ctx.ham_set(0, n, J=-1.0, h=-1.0)
+    rng = np.random.default_rng(n * m + ns)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig)
+    ctx.sr_build(0)
+    th = orc.theta(rbm, sig)
+    el = orc.eloc(orc.ham(0, n), rbm, sig, th)
+    _, _, F, S = orc.sr_build(rbm, sig, th, el)
+    Sg = ctx.sr_get(3)
+    assert _rel(Sg, S) < 1e-12
+    np.testing.assert_array_equal(Sg, Sg.T)
+    lam = 1e-3
+    x, res, _ = ctx.sr_solve(lam)
+    assert _rel(x, orc.llt_solve(S, lam, F)) < 1e-9 and res < 1e-12
+
+
 @pytest.mark.parametrize("solver", [1, 2])
 def test_cg_solvers_match_cholesky(ctx, orc, solver):
     n, m, ns = 20, 40, 600

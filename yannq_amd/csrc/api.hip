@@ -377,68 +377,73 @@ int compute_eloc(yq_ctx* c) {
 // ---------------------------------------------------------------- SR build
 size_t aug_partial_doubles(int ns, int n, int m) { return factored_partial_doubles(ns, n, m); }
 
-// S = alpha A A^T (lower triangle; A^H for complex) with A = dim x ns
-// column-major.  rocBLAS SYRK/HERK parallelises over the output tiles, which
-// leaves most of the chip idle when dim is small and ns large (an exact basis
-// of 2^20 configurations); then the samples are split into P chunks, one
-// strided-batched SYRK/HERK forms the P partial products side by side, and a
+// Complex S = alpha A A^H (lower triangle) with A = conj(O_c) dim x ns
+// column-major, on rocBLAS HERK.  HERK parallelises over the output tiles,
+// which leaves most of the chip idle when dim is small and ns large (an exact
+// basis of 2^20 configurations); then the samples are split into P chunks,
+// one strided-batched HERK forms the P partial products side by side, and a
 // fixed-order sum adds them (plus the remainder chunk).
-int syrk_samples(yq_ctx* c, const double* A, size_t dim, int ns, double alpha, double* S,
-                 bool cplx) {
+int herk_samples(yq_ctx* c, const double* A, size_t dim, int ns, double alpha, double* S) {
   TRY(blas(c));
-  const size_t nc = cplx ? 2 : 1;
   const int kc = 16384;
   int P = ns / kc;
   const size_t cap = (size_t)2 << 30;  // bytes of partial products
-  while (P > 1 && (size_t)P * dim * dim * nc * sizeof(double) > cap) --P;
+  while (P > 1 && (size_t)P * dim * dim * 2 * sizeof(double) > cap) --P;
   const double zero = 0.0, one = 1.0;
+  auto* Sz = reinterpret_cast<rocblas_double_complex*>(S);
   if (P < 4) {
-    ProfGuard g(cplx ? "zherk" : "dsyrk", c->st);
-    if (cplx)
-      BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)dim, ns, &alpha,
-                               reinterpret_cast<const rocblas_double_complex*>(A),
-                               (rocblas_int)dim, &zero,
-                               reinterpret_cast<rocblas_double_complex*>(S), (rocblas_int)dim));
-    else
-      BLASCHK(c, rocblas_dsyrk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)dim, ns, &alpha, A, (rocblas_int)dim, &zero, S,
-                               (rocblas_int)dim));
+    ProfGuard g("zherk", c->st);
+    BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                             (rocblas_int)dim, ns, &alpha,
+                             reinterpret_cast<const rocblas_double_complex*>(A),
+                             (rocblas_int)dim, &zero, Sz, (rocblas_int)dim));
     return YQ_OK;
   }
-  TRY(ensure(c, c->M, sizeof(double) * nc * (size_t)P * dim * dim));
+  TRY(ensure(c, c->M, sizeof(double) * 2 * (size_t)P * dim * dim));
   double* Pout = c->M.as<double>();
   const rocblas_stride sa = (rocblas_stride)kc * dim, sc = (rocblas_stride)(dim * dim);
   {
-    ProfGuard g(cplx ? "zherk" : "dsyrk", c->st);
-    if (cplx)
-      BLASCHK(c, rocblas_zherk_strided_batched(
-                     c->blas, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)dim, kc,
-                     &alpha, reinterpret_cast<const rocblas_double_complex*>(A),
-                     (rocblas_int)dim, sa, &zero,
-                     reinterpret_cast<rocblas_double_complex*>(Pout), (rocblas_int)dim, sc, P));
-    else
-      BLASCHK(c, rocblas_dsyrk_strided_batched(c->blas, rocblas_fill_lower,
-                                               rocblas_operation_none, (rocblas_int)dim, kc,
-                                               &alpha, A, (rocblas_int)dim, sa, &zero, Pout,
-                                               (rocblas_int)dim, sc, P));
+    ProfGuard g("zherk", c->st);
+    BLASCHK(c, rocblas_zherk_strided_batched(
+                   c->blas, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)dim, kc,
+                   &alpha, reinterpret_cast<const rocblas_double_complex*>(A), (rocblas_int)dim,
+                   sa, &zero, reinterpret_cast<rocblas_double_complex*>(Pout), (rocblas_int)dim,
+                   sc, P));
   }
-  launch_sum_lower(Pout, P, dim, (int)nc, S, c->st);
+  launch_sum_lower(Pout, P, dim, 2, S, c->st);
   LAUNCHCHK(c);
   const int rem = ns - P * kc;
-  if (rem > 0) {
-    const double* Ar = A + nc * (size_t)P * kc * dim;
-    if (cplx)
-      BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)dim, rem, &alpha,
-                               reinterpret_cast<const rocblas_double_complex*>(Ar),
-                               (rocblas_int)dim, &one,
-                               reinterpret_cast<rocblas_double_complex*>(S), (rocblas_int)dim));
-    else
-      BLASCHK(c, rocblas_dsyrk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)dim, rem, &alpha, Ar, (rocblas_int)dim, &one, S,
-                               (rocblas_int)dim));
+  if (rem > 0)
+    BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
+                             (rocblas_int)dim, rem, &alpha,
+                             reinterpret_cast<const rocblas_double_complex*>(A + 2 * (size_t)P * kc * dim),
+                             (rocblas_int)dim, &one, Sz, (rocblas_int)dim));
+  return YQ_OK;
+}
+
+// S = alpha O_c^T O_c (lower triangle) of a real machine on the hand-written
+// FP64 MFMA SYRK (syrk.hip): from the factors (spins, T64, column mean and
+// the exact batch's row scales) when Odense is null, else from the stored
+// row-major centred O.  A batch with few output tiles and many samples (an
+// exact basis) is split into sample chunks whose partial products go to
+// c->M and are summed in chunk order.
+int syrk_real(yq_ctx* c, const double* Odense, size_t dim, int ns, double alpha, double* S) {
+  const size_t cap = ((size_t)2 << 30) / sizeof(double);  // partial products, doubles
+  const int P = syrk64_chunks(ns, dim, cap);
+  double* part = nullptr;
+  if (P > 1) {
+    TRY(ensure(c, c->M, sizeof(double) * (size_t)P * dim * dim));
+    part = c->M.as<double>();
   }
+  int rc;
+  if (Odense)
+    rc = launch_syrk64_dense(Odense, ns, dim, alpha, S, P, part, c->st);
+  else
+    rc = launch_syrk64_factored(c->bsig.as<int8_t>(), c->T64.as<double>(),
+                                c->omean.as<double>(), c->weighted ? c->brs.as<double>() : nullptr,
+                                ns, c->n, c->m, c->use_bias, alpha, S, P, part, c->st);
+  if (rc) return fail(c, rc, "SYRK arguments");
+  LAUNCHCHK(c);
   return YQ_OK;
 }
 
@@ -475,7 +480,7 @@ int sr_build_c(yq_ctx* c, int solver) {
   LAUNCHCHK(c);
   if (solver == YQ_SOLVE_EXPLICIT) {
     TRY(ensure(c, c->S, sizeof(double) * 2 * dim * dim));
-    TRY(syrk_samples(c, Oh, dim, ns, 1.0 / ns, c->S.as<double>(), true));
+    TRY(herk_samples(c, Oh, dim, ns, 1.0 / ns, c->S.as<double>()));
   }
   // diag(S) = mean |O_c|^2, duplicated per (re, im) for the Jacobi option
   TRY(ensure(c, c->diagS, sizeof(double) * 2 * dim));
@@ -570,7 +575,7 @@ int sr_build_ap(yq_ctx* c, int solver, int precision) {
     launch_reduce_partials(part, chunks, d, 1.0 / ns, c->diagS.as<double>() + blk[b].off, c->st);
     LAUNCHCHK(c);
     if (solver == YQ_SOLVE_EXPLICIT)
-      TRY(syrk_samples(c, blk[b].O, blk[b].d, ns, 1.0 / ns, blk[b].S, false));
+      TRY(syrk_real(c, blk[b].O, blk[b].d, ns, 1.0 / ns, blk[b].S));
   }
   TRY(sync(c));
   c->have_sr = true;
@@ -653,9 +658,9 @@ int sr_build(yq_ctx* c, int solver, int precision) {
                            c->F.as<double>(), c->st);
     LAUNCHCHK(c);
     if (solver == YQ_SOLVE_EXPLICIT) {
-      // O_c is ns x dim row-major = dim x ns column-major: S = A A^T / ns
+      // S = O_c^T O_c / ns formed from the factors (O_c is not read)
       TRY(ensure(c, c->S, sizeof(double) * dim * dim));
-      TRY(syrk_samples(c, c->O.as<double>(), dim, ns, 1.0 / ns, c->S.as<double>(), false));
+      TRY(syrk_real(c, nullptr, dim, ns, 1.0 / ns, c->S.as<double>()));
     }
   }
   c->diag_ready = false;  // diag(S) is formed when a preconditioned solve needs it

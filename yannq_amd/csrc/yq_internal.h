@@ -212,6 +212,20 @@ void launch_center_force_c(double* Oh, const double* omean_h, const double* e, i
 // S = sum of np column-major dim x dim partial products, lower triangle
 // (nc = 2: complex entries as pairs)
 void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hipStream_t st);
+// S = alpha O_c^T O_c, lower triangle of the dim x dim column-major S, on
+// FP64 MFMA (syrk.hip).  Factored: O_c[s][c] = (sigma'_{s,i(c)} T'_{s,j(c)} -
+// omean_c) rs_s (rs may be null) from the int8 spins and T = tanh(theta)
+// (ns x m); dense: O_c row-major ns x dim.  chunks > 1 splits the samples
+// into that many chunks whose partial products (in part, chunks x dim x dim
+// doubles) are summed in chunk order; syrk64_chunks picks the count for a
+// shape (1 unless the tiles are too few to fill the chip) within
+// part_cap_doubles.
+int syrk64_chunks(int ns, size_t dim, size_t part_cap_doubles);
+int launch_syrk64_factored(const int8_t* sigma, const double* T, const double* omean,
+                           const double* rs, int ns, int n, int m, int use_bias, double alpha,
+                           double* S, int chunks, double* part, hipStream_t st);
+int launch_syrk64_dense(const double* O, int ns, size_t dim, double alpha, double* S, int chunks,
+                        double* part, hipStream_t st);
 // ---------------------------------------------------------------- exact sums
 // full-basis / sector configurations as spins, |psi|^2 weights (and their
 // row scales sqrt(B w)), weighted E_loc statistics, the CDF and the draws of
