@@ -106,7 +106,7 @@ struct yq_ctx {
   int skind = 0, chains = 0, sweep_factor = 1, sprec = 0;
   int ntemps = 1;  // parallel tempering: chains = ntemps x chain groups
   uint64_t seed = 0, rand_epoch = 0;
-  Buf sig_state, ctr, acc;
+  Buf sig_state, ctr, acc, tbuf;
 
   // current batch; weighted: an exact basis with |psi|^2 weights bw (row
   // scales brs = sqrt(ns bw)), see exact.hip
@@ -154,7 +154,7 @@ struct yq_ctx {
   Buf ap_ffs, ap_ffb;
 
   yq_ctx() {
-    for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &bsig,
+    for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &tbuf, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp, &dscal,
                    &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
@@ -1463,6 +1463,11 @@ extern "C" {
 YQ_API int yq_sample(yq_ctx* c, int n_sweeps, int n_discard, int* n_samples) {
   SampleArgs a;
   TRY(sample_args(c, n_sweeps, n_discard, &a));
+  if (!c->cplx && c->ntemps == 1) {  // theta of the chains from the MFMA pass
+    const size_t rs = c->sprec == YQ_MIXED ? sizeof(float) : sizeof(double);
+    TRY(ensure(c, c->tbuf, rs * (size_t)c->chains * c->m));
+    a.theta_buf = c->tbuf.p;
+  }
   if (c->cplx) launch_sample_c(a, c->st);
   else launch_sample(a, c->st);
   LAUNCHCHK(c);

@@ -140,6 +140,9 @@ __device__ __forceinline__ void record_row(const SampleArgs& A, size_t row, cons
                                            bool& fresh) {
   const int n = A.n, m = A.m;
   for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+  // mixed precision with the MFMA theta pass: the pass that follows the
+  // launch records theta refreshed from FP64 (one sweep per launch)
+  if (A.mixed && A.theta_in) return;
   if (A.mixed) {
     refresh_theta<R, RMAX>(A, sig, lane, th, lc);
     fresh = true;
@@ -170,7 +173,17 @@ __device__ void run_chain(const SampleArgs& A, int chain, int8_t* sig, int lane)
   uint64_t ev = A.ctr[chain];
   uint64_t nacc = 0;
   const R beta = (R)A.beta;
-  refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+  if (A.theta_in) {  // from the MFMA theta pass (the same bits as refresh_theta)
+    const R* ti = static_cast<const R*>(A.theta_in) + (size_t)chain * A.m;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      th[r] = j < A.m ? ti[j] : (R)0;
+      lc[r] = j < A.m ? Math<R>::logcosh(th[r]) : (R)0;
+    }
+  } else {
+    refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+  }
   bool fresh = true;
   const int total = A.n_discard + A.n_sweeps;
   for (int sw = 0; sw < total; ++sw) {
@@ -374,22 +387,79 @@ void launch_r(const SampleArgs& a, hipStream_t st) {
   }
 }
 
-// theta for a batch, one thread per (sample, hidden unit)
+// theta = W sigma + b for `rows` configurations on v_mfma_f64_16x16x4f64
+// (lane l holds A[row l%16][k l/16], B[k l/16][col l%16], and returns
+// D[(l/16) + 4q][l%16]).  A = the spins (+/-1, 0 past n), B = W (W_ji at
+// W64[i m + j]), k = the spin index i.  Every product is exact and the f64
+// MFMA adds its four k terms in order, one rounding each, onto the running
+// sum, so the result is bit for bit the sequential FP64 sum over i of the
+// other theta paths (refresh_theta, the oracle's replay), then + b_j and the
+// rounding to O.  Block: 4 waves x 16 configurations, 64 hidden units (4
+// column tiles per wave); the loads of several k-steps are in flight
+// together, with clamped addresses instead of branches.
+typedef double double4v __attribute__((ext_vector_type(4)));
+
 template <typename O>
 __global__ void __launch_bounds__(256)
-    k_theta_batch(const double* __restrict__ W64, const double* __restrict__ b64,
-                  const int8_t* __restrict__ sigma, int ns, int n, int m,
-                  O* __restrict__ out) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)ns * m) return;
-  const int s = (int)(idx / m), j = (int)(idx % m);
-  const int8_t* sg = sigma + (size_t)s * n;
-  double acc = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double w = W64[(size_t)i * m + j];
-    acc = sg[i] > 0 ? acc + w : acc - w;
+    k_theta_mfma(const double* __restrict__ W64, const double* __restrict__ b64,
+                 const int8_t* __restrict__ sigma, int rows, int n, int m,
+                 O* __restrict__ out) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int li = l & 15, lg = l >> 4;
+  const int s0 = (blockIdx.x * 4 + wave) * 16;
+  const int j0 = blockIdx.y * 64;
+  if (s0 >= rows) return;  // wave-uniform; no barrier in this kernel
+  const int sa = min(s0 + li, rows - 1);  // this lane's A row (clamped)
+  const int8_t* srow = sigma + (size_t)sa * n;
+  int jc[4];
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) jc[ct] = min(j0 + ct * 16 + li, m - 1);
+  double4v acc[4];
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) acc[ct] = (double4v){0.0, 0.0, 0.0, 0.0};
+  const int ksteps = (n + 3) / 4;
+  constexpr int kU = 5;  // k-steps whose loads are issued together
+  for (int k0 = 0; k0 < ksteps; k0 += kU) {
+    double a[kU], b[kU][4];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = (k0 + u) * 4 + lg;
+      const int ic = min(i, n - 1);
+      const int8_t g = srow[ic];
+      a[u] = (i < n && k0 + u < ksteps) ? (g > 0 ? 1.0 : -1.0) : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) b[u][ct] = W64[(size_t)ic * m + jc[ct]];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (k0 + u < ksteps)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+          acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][ct], acc[ct], 0, 0, 0);
   }
-  out[idx] = (O)(acc + b64[j]);
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+    const int j = j0 + ct * 16 + li;
+    if (j >= m) continue;
+    const double bj = b64[j];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int s = s0 + lg + 4 * q;
+      if (s < rows) out[(size_t)s * m + j] = (O)(acc[ct][q] + bj);
+    }
+  }
+}
+
+void theta_pass(const double* W64, const double* b64, const int8_t* sigma, int rows, int n,
+                int m, void* out, int out_float, hipStream_t st) {
+  const dim3 grid((unsigned)((rows + 63) / 64), (unsigned)((m + 63) / 64)), block(256);
+  ProfGuard g("theta_mfma", st);
+  if (out_float)
+    hipLaunchKernelGGL(k_theta_mfma<float>, grid, block, 0, st, W64, b64, sigma, rows, n, m,
+                       static_cast<float*>(out));
+  else
+    hipLaunchKernelGGL(k_theta_mfma<double>, grid, block, 0, st, W64, b64, sigma, rows, n, m,
+                       static_cast<double*>(out));
 }
 
 }  // namespace
@@ -463,27 +533,60 @@ void launch_sample(const SampleArgs& a, hipStream_t st) {
     }
     return;
   }
-  if (a.mixed) {
-    if (a.kind == 1) launch_r<float, 1>(a, st);
-    else launch_r<float, 0>(a, st);
-  } else {
-    if (a.kind == 1) launch_r<double, 1>(a, st);
-    else launch_r<double, 0>(a, st);
+  auto run = [&](const SampleArgs& x) {
+    if (x.mixed) {
+      if (x.kind == 1) launch_r<float, 1>(x, st);
+      else launch_r<float, 0>(x, st);
+    } else {
+      if (x.kind == 1) launch_r<double, 1>(x, st);
+      else launch_r<double, 0>(x, st);
+    }
+  };
+  if (!a.theta_buf) {
+    run(a);
+    return;
+  }
+  const int fl = a.mixed ? 1 : 0;
+  const size_t rs = a.mixed ? sizeof(float) : sizeof(double);
+  // theta of the chains' current spins
+  theta_pass(a.W64, a.b64, a.sigma, a.C, a.n, a.m, a.theta_buf, fl, st);
+  if (!a.mixed) {  // FP64 chains keep their theta by rank-1 updates: one launch
+    SampleArgs x = a;
+    x.theta_in = a.theta_buf;
+    run(x);
+    return;
+  }
+  // float chains restart every sweep from theta refreshed from FP64: one
+  // launch per sweep, each followed by the theta pass of the spins it left
+  // (the recorded rows, or the chain state after a discard sweep), which is
+  // both the record and the next sweep's start
+  const void* tin = a.theta_buf;
+  for (int sw = 0; sw < a.n_discard + a.n_sweeps; ++sw) {
+    SampleArgs x = a;
+    x.theta_in = tin;
+    if (sw < a.n_discard) {
+      x.n_discard = 1;
+      x.n_sweeps = 0;
+      run(x);
+      theta_pass(a.W64, a.b64, a.sigma, a.C, a.n, a.m, a.theta_buf, fl, st);
+      tin = a.theta_buf;
+    } else {
+      const size_t r = (size_t)(sw - a.n_discard) * a.C;
+      x.n_discard = 0;
+      x.n_sweeps = 1;
+      x.out_sigma = a.out_sigma + r * a.n;
+      x.out_theta = static_cast<char*>(a.out_theta) + r * a.m * rs;
+      run(x);
+      theta_pass(a.W64, a.b64, x.out_sigma, a.C, a.n, a.m, x.out_theta, fl, st);
+      tin = x.out_theta;
+    }
   }
 }
 
 void launch_theta_batch(const double* W64, const double* b64,
                         const int8_t* sigma, int ns, int n, int m, void* out,
                         int out_float, hipStream_t st) {
-  const size_t total = (size_t)ns * m;
-  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
-  ProfGuard g("theta_batch", st);
-  if (out_float)
-    hipLaunchKernelGGL(k_theta_batch<float>, grid, block, 0, st, W64, b64,
-                       sigma, ns, n, m, static_cast<float*>(out));
-  else
-    hipLaunchKernelGGL(k_theta_batch<double>, grid, block, 0, st, W64, b64,
-                       sigma, ns, n, m, static_cast<double*>(out));
+  theta_pass(W64, b64, sigma, ns, n, m, out, out_float, st);
 }
 
 }  // namespace yq

@@ -70,6 +70,11 @@ struct SampleArgs {
   double beta;
   uint64_t seed;
   int ntemps;  // parallel tempering: C = ntemps x chain groups (1: plain chains)
+  // Real machine, ntemps == 1: theta of every chain's spins comes from the
+  // MFMA theta pass (theta_buf: C x m scratch in the working precision) and
+  // k_sample reads it instead of summing W itself; null: k_sample sums.
+  void* theta_buf;
+  const void* theta_in;  // set by launch_sample per launch
 };
 void launch_sample(const SampleArgs& a, hipStream_t st);
 // LDS of the parallel-tempering launch (spin strips, exchange areas)
@@ -80,8 +85,9 @@ size_t pt_lds_bytes(const SampleArgs& a);
 void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_items, int npad,
                          int kind, int mixed, int blocks, hipStream_t st);
 
-// theta = W sigma + b for a batch (fp64 sequential sum, optionally rounded to
-// float); out is double or float.
+// theta = W sigma + b for a batch on FP64 MFMA (the sum over the spins in
+// index order, exactly as a sequential FP64 loop adds it, then + b, optionally
+// rounded to float); out is double or float.
 void launch_theta_batch(const double* W64, const double* b64,
                         const int8_t* sigma, int ns, int n, int m, void* out,
                         int out_float, hipStream_t st);
