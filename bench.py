@@ -332,6 +332,29 @@ def cg_probe(yq):
     ctx.close()
 
 
+def syrk_probe(yq):
+    """One FP64 explicit SR build at this workload (16,384 samples after 20
+    thermalisation sweeps, S = O_c^T O_c / Ns on the FP64 MFMA SYRK) and its
+    Cholesky solve, as a short load for counter passes; prints the SYRK time
+    from HIP events on its launch."""
+    ctx = _ctx(yq, yq.FP64)
+    ctx.sample(1, 20)
+    ctx._L.yq_prof_enable(ctx._h, b"syrk64,dpotrf")
+    ctx.sr_build(yq.SOLVE_EXPLICIT)
+    _, res, _ = ctx.sr_solve(LAMBDA)
+    out = {"resid": res}
+    tot, cnt = C.c_double(), C.c_longlong()
+    for lab in ("syrk64", "dpotrf"):
+        ctx._L.yq_prof_read(ctx._h, lab.encode(), C.byref(tot), C.byref(cnt))
+        out[lab + "_ms"] = tot.value / max(cnt.value, 1)
+    flop = float(DIM) * DIM * NS   # the lower half: dim^2 Ns / 2 multiply-adds
+    out["syrk64_tflops"] = flop / out["syrk64_ms"] / 1e9
+    out["syrk64_frac_of_fp64_mfma_peak"] = out["syrk64_tflops"] / FP64_MFMA_PEAK_TFLOPS
+    ctx._L.yq_prof_enable(ctx._h, None)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def batched(yq, warmup, steps):
     """Many machines in one context: 32 TFI machines, N ~ U{16, 24, ..., 128},
     alpha ~ U{1, 2, 4}, h evenly spaced over [0.5, 1.5], 2,048 samples each,
@@ -471,6 +494,9 @@ def main():
                     help="the FeedForward Jacobian kernel at N=40, batch 8192")
     ap.add_argument("--errors", action="store_true",
                     help="the mixed path's error against FP64 on identical configurations")
+    ap.add_argument("--syrk-probe", action="store_true",
+                    help="only one FP64 explicit SR build and solve at this shape (a short "
+                         "load for counter passes of the SYRK)")
     ap.add_argument("--cg-probe", action="store_true",
                     help="only 64 eager mixed CG iterations at this shape (a short load for "
                          "counter passes of the CG kernels)")
@@ -485,6 +511,9 @@ def main():
     import yannq_amd as yq
     if args.cg_probe:
         cg_probe(yq)
+        return
+    if args.syrk_probe:
+        syrk_probe(yq)
         return
     if args.errors:
         precision_errors(yq)

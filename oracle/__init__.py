@@ -55,6 +55,7 @@ def lib():
             "or_theta": (None, [i, i, p, p, p, p]),
             "or_log_ratio1": (d, [i, i, p, p, p, p, i]),
             "or_log_ratio2": (d, [i, i, p, p, p, p, i, i]),
+            "or_log_ratio_vec": (d, [i, i, p, p, p, p, i, p]),
             "or_log_coeff": (d, [i, i, p, p, p]),
             "or_logderiv": (None, [i, i, i, p, p, p]),
             "or_eloc": (d, [p, i, i, p, p, p, p]),
@@ -85,6 +86,7 @@ def lib():
             "or_c_theta": (None, [i, i, p, p, p, p]),
             "or_c_log_ratio1": (None, [i, i, p, p, p, p, i, p]),
             "or_c_log_ratio2": (None, [i, i, p, p, p, p, i, i, p]),
+            "or_c_log_ratio_vec": (None, [i, i, p, p, p, p, i, p, p]),
             "or_c_log_coeff": (None, [i, i, p, p, p, p]),
             "or_c_logderiv": (None, [i, i, i, p, p, p]),
             "or_c_eloc": (None, [p, i, i, p, p, p, p, p]),
@@ -187,6 +189,14 @@ def log_ratio1(rbm, sigma, th, k):
 def log_ratio2(rbm, sigma, th, k, l):
     sigma, th = i8(sigma), f64(th)
     return lib().or_log_ratio2(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(sigma), _p(th), k, l)
+
+
+def log_ratio_vec(rbm, sigma, th, v):
+    """logRatio(vector): flip every listed site (RBMState.hpp:108-126)."""
+    sigma, th = i8(sigma), f64(th)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    return lib().or_log_ratio_vec(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(sigma), _p(th), len(v),
+                                  _p(v))
 
 
 def log_coeff(rbm, sigma, th=None):
@@ -434,6 +444,14 @@ def c_log_ratio1(rbm, sigma, th, k):
     out = np.empty(2)
     lib().or_c_log_ratio1(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(i8(sigma)), _p(c128(th)), k,
                           _p(out))
+    return out[0] + 1j * out[1]
+
+
+def c_log_ratio_vec(rbm, sigma, th, v):
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    out = np.empty(2)
+    lib().or_c_log_ratio_vec(rbm.n, rbm.m, _p(rbm.W), _p(rbm.a), _p(i8(sigma)), _p(c128(th)),
+                             len(v), _p(v), _p(out))
     return out[0] + 1j * out[1]
 
 

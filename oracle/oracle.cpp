@@ -118,6 +118,22 @@ double or_log_ratio2(int n, int m, const double* W, const double* a,
   return res;
 }
 
+// States/RBMState.hpp:108-126 (logRatio(vector)): the visible terms of
+// every listed site first, then per hidden unit theta_j minus the listed
+// columns in list order
+double or_log_ratio_vec(int n, int m, const double* W, const double* a, const int8_t* sigma,
+                        const double* theta, int k, const int* v) {
+  (void)n;
+  double res = 0.0;
+  for (int e = 0; e < k; ++e) res -= 2.0 * a[v[e]] * sgn(sigma[v[e]]);
+  for (int j = 0; j < m; ++j) {
+    double t = theta[j];
+    for (int e = 0; e < k; ++e) t -= 2.0 * sgn(sigma[v[e]]) * Wat(W, m, j, v[e]);
+    res += or_logcosh(t) - or_logcosh(theta[j]);
+  }
+  return res;
+}
+
 // Machines/RBM.hpp:367-378
 double or_log_coeff(int n, int m, const double* a, const int8_t* sigma,
                     const double* theta) {

@@ -41,6 +41,8 @@ double or_log_ratio2(int n, int m, const double* W, const double* a,
                      const int8_t* sigma, const double* theta, int k, int l);
 double or_log_coeff(int n, int m, const double* a, const int8_t* sigma,
                     const double* theta);
+double or_log_ratio_vec(int n, int m, const double* W, const double* a, const int8_t* sigma,
+                        const double* theta, int k, const int* v);
 void or_logderiv(int n, int m, int use_bias, const int8_t* sigma,
                  const double* theta, double* out);
 double or_eloc(const or_ham* H, int n, int m, const double* W, const double* a,
@@ -72,6 +74,8 @@ void or_c_log_ratio2(int n, int m, const double* W, const double* a, const int8_
                      const double* theta, int k, int l, double* out2);
 void or_c_log_coeff(int n, int m, const double* a, const int8_t* sigma, const double* theta,
                     double* out2);
+void or_c_log_ratio_vec(int n, int m, const double* W, const double* a, const int8_t* sigma,
+                        const double* theta, int k, const int* v, double* out2);
 void or_c_logderiv(int n, int m, int use_bias, const int8_t* sigma, const double* theta,
                    double* out);
 void or_c_eloc(const or_ham* H, int n, int m, const double* W, const double* a,

@@ -66,6 +66,20 @@ void or_c_log_ratio2(int n, int m, const double* W, const double* a, const int8_
   put(out2, 0, res);
 }
 
+// States/RBMState.hpp:108-126 (logRatio(vector)) for complex parameters
+void or_c_log_ratio_vec(int n, int m, const double* W, const double* a, const int8_t* sigma,
+                        const double* theta, int k, const int* v, double* out2) {
+  (void)n;
+  cd res = 0.0;
+  for (int e = 0; e < k; ++e) res -= 2.0 * at(a, v[e]) * sgn(sigma[v[e]]);
+  for (int j = 0; j < m; ++j) {
+    cd t = at(theta, j);
+    for (int e = 0; e < k; ++e) t -= 2.0 * sgn(sigma[v[e]]) * at(W, (size_t)v[e] * m + j);
+    res += logcosh_c(t) - logcosh_c(at(theta, j));
+  }
+  put(out2, 0, res);
+}
+
 // Machines/RBM.hpp:367-378
 void or_c_log_coeff(int n, int m, const double* a, const int8_t* sigma, const double* theta,
                     double* out2) {

@@ -109,6 +109,30 @@ HAMS = [
 ]
 
 
+def test_log_ratio_vector_brute_force(orc):
+    """logRatio(vector) (RBMState.hpp:108-126): flipping every listed site,
+    for lists of 1 to n distinct sites, against the brute-force psi table;
+    one- and two-site lists equal logRatio(k) and logRatio(k, l)."""
+    rng = np.random.default_rng(12)
+    n = 10
+    rbm = orc.RBM.random(n, 14, 0.7, rng)
+    basis, th, logpsi = _psi_table(orc, rbm)
+    for _ in range(25):
+        v = int(rng.integers(0, 1 << n))
+        for k in (1, 2, 3, 5, n):
+            sites = rng.choice(n, k, replace=False)
+            w = v
+            for s in sites:
+                w ^= 1 << int(s)
+            got = orc.log_ratio_vec(rbm, basis[v], th[v], sites)
+            assert abs(got - (logpsi[w] - logpsi[v])) < 1e-11
+        k, l = (int(x) for x in rng.choice(n, 2, replace=False))
+        assert orc.log_ratio_vec(rbm, basis[v], th[v], [k]) == \
+            orc.log_ratio1(rbm, basis[v], th[v], k)
+        assert orc.log_ratio_vec(rbm, basis[v], th[v], [k, l]) == \
+            orc.log_ratio2(rbm, basis[v], th[v], k, l)
+
+
 @pytest.mark.parametrize("name,mk", HAMS)
 def test_eloc_brute_force(orc, name, mk):
     rng = np.random.default_rng(4)

@@ -90,6 +90,24 @@ def _c_psi_table(orc, rbm):
     return basis, th, lp
 
 
+def test_complex_log_ratio_vector_brute_force(orc):
+    """Complex logRatio(vector) against the brute-force log psi table (mod
+    2 pi i), lists of 1 to n distinct sites."""
+    rng = np.random.default_rng(13)
+    n, m = 7, 9
+    rbm = orc.CRBM.random(n, m, 0.4, rng)
+    basis, th, lp = _c_psi_table(orc, rbm)
+    index = {tuple(s): q for q, s in enumerate(basis)}
+    for q in rng.choice(len(basis), 25, replace=False):
+        s = basis[q]
+        for k in (1, 2, 4, n):
+            sites = rng.choice(n, k, replace=False)
+            s2 = s.copy()
+            s2[sites] = -s2[sites]
+            want = lp[index[tuple(s2)]] - lp[q]
+            assert _close_log(orc.c_log_ratio_vec(rbm, s, th[q], sites), want, 1e-11)
+
+
 def test_complex_log_coeff_and_ratios_brute_force(orc):
     rng = np.random.default_rng(3)
     n, m = 7, 9
