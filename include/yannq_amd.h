@@ -176,6 +176,16 @@ YQ_API int yq_sampler_get_state(yq_ctx* ctx, int8_t* sigma, uint64_t* ctr,
 /* sample(nSweeps, nDiscard): SamplerMT.hpp:180-204.  Samples stay on the
  * device; *n_samples = n_chains_per * n_sweeps, ordered sweep-major. */
 YQ_API int yq_sample(yq_ctx* ctx, int n_sweeps, int n_discard, int* n_samples);
+/* State::logRatio(k), logRatio(k, l) and logRatio(vector) (States/
+ * RBMState.hpp:48-63, 78-93, 108-126) for ns configurations at once: out[s]
+ * = log psi(sigma_s with the sites listed in row s of sites flipped) -
+ * log psi(sigma_s), where row s holds up to k site indices (list order;
+ * entries < 0 are skipped, so rows may list fewer sites).  sigma is ns x n
+ * (+1 / -1), sites ns x k; out holds ns doubles, or ns (re, im) pairs for a
+ * complex machine (the imaginary part is the sum of principal logs, defined
+ * modulo 2 pi as the reference's).  FP64; independent of any batch. */
+YQ_API int yq_rbm_log_ratio(yq_ctx* ctx, int ns, const int8_t* sigma, int k,
+                            const int* sites, double* out);
 /* Copy the current batch out: spins (Ns x n) and theta (Ns x m, doubles). */
 YQ_API int yq_samples_get(yq_ctx* ctx, int8_t* sigma, double* theta);
 /* Replace the batch by caller configurations (Ns x n); theta = W sigma + b is

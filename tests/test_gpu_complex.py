@@ -182,3 +182,23 @@ def test_complex_xxz_reaches_ground_state(ctx, orc):
         es.append(o["energy"])
     e = np.mean(es[-20:])
     assert abs(e - e0) < 0.01 * abs(e0), (e, e0)
+
+
+def test_complex_log_ratio_vector(ctx, orc):
+    """Complex logRatio(vector) on the GPU against the oracle's restatement
+    (RBMState.hpp:108-126), modulo 2 pi i, C2 shape."""
+    n, m, ns, k = 64, 128, 200, 9
+    rbm = _setup(ctx, orc, n, m, scale=0.2)
+    rng = np.random.default_rng(42)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    sites = np.full((ns, k), -1, np.int32)
+    for s in range(ns):
+        cnt = 1 + s % k
+        sites[s, :cnt] = rng.choice(n, cnt, replace=False)
+    got = ctx.log_ratio(sig, sites)
+    th = orc.c_theta(rbm, sig)
+    for s in range(ns):
+        want = orc.c_log_ratio_vec(rbm, sig[s], th[s], sites[s][sites[s] >= 0])
+        d = got[s] - want
+        d = complex(d.real, (d.imag + np.pi) % (2 * np.pi) - np.pi)
+        assert abs(d) <= 1e-11 * max(1.0, abs(want)), (s, got[s], want)

@@ -263,3 +263,29 @@ def test_sr_steps_reach_ground_state(ctx, orc):
     tail = np.mean(es[-20:])
     assert tail < es[0]
     assert abs(tail - e0) / abs(e0) < 0.005, (tail, e0)
+
+
+def test_log_ratio_vector(ctx, orc):
+    """logRatio(k), (k, l) and (vector) on the GPU (yq_rbm_log_ratio) against
+    the oracle's restatements (RBMState.hpp:48-63, 78-93, 108-126), for rows
+    of 1 to n listed sites with skipped (-1) entries; C3 shape."""
+    n, m, ns, k = 100, 400, 300, 12
+    rbm = _setup(ctx, orc, n, m, scale=0.05)
+    rng = np.random.default_rng(41)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    sites = np.full((ns, k), -1, np.int32)
+    for s in range(ns):
+        cnt = 1 + s % k
+        sites[s, :cnt] = rng.choice(n, cnt, replace=False)
+    got = ctx.log_ratio(sig, sites)
+    th = orc.theta(rbm, sig)
+    for s in range(ns):
+        v = sites[s][sites[s] >= 0]
+        want = orc.log_ratio_vec(rbm, sig[s], th[s], v)
+        assert abs(got[s] - want) <= 1e-12 * max(1.0, abs(want)), (s, got[s], want)
+        if len(v) == 1:
+            assert abs(got[s] - orc.log_ratio1(rbm, sig[s], th[s], int(v[0]))) < 1e-12
+        if len(v) == 2:
+            assert abs(got[s] - orc.log_ratio2(rbm, sig[s], th[s], int(v[0]), int(v[1]))) < 1e-12
+    with pytest.raises(Exception):
+        ctx.log_ratio(sig[:1], np.array([[n]], np.int32))

@@ -109,6 +109,7 @@ def lib():
         "yq_ff_init_random": [p, u64, i], "yq_ff_forward": [p, i, p, p],
         "yq_ff_jacobian": [p, i, p, p, p], "yq_ff_forward_dev": [p, i, p, p, p],
         "yq_ff_jacobian_dev": [p, i, p, p, p], "yq_ff_vjp_dev": [p, i, p, p, p, p],
+        "yq_rbm_log_ratio": [p, i, p, i, p, p],
         "yq_ff_vjp_dev_noalloc": [p, i, p, p, p, p],
         "yq_ckpt_rbm_encode": [i, i, i, i, p, i, p, C.c_size_t],
         "yq_ckpt_rbm_decode": [i, p, C.c_size_t, i, p, p, p, p, C.c_size_t],
@@ -293,6 +294,16 @@ class Context:
         t = self._out(self.ns, self.m)
         self._chk(self._L.yq_samples_get(self._h, _ptr(s), _ptr(t)))
         return s, t
+
+    def log_ratio(self, sigma, sites):
+        """logRatio(vector) per row: sigma (ns x n), sites (ns x k, entries < 0
+        skipped); real or complex per the machine."""
+        sigma = np.ascontiguousarray(sigma, dtype=np.int8)
+        sites = np.ascontiguousarray(sites, dtype=np.int32)
+        ns, k = sites.shape
+        out = np.empty(ns, dtype=np.complex128 if self.cplx else np.float64)
+        self._chk(self._L.yq_rbm_log_ratio(self._h, ns, _ptr(sigma), k, _ptr(sites), _ptr(out)))
+        return out
 
     def samples_set(self, sigma, precision=FP64):
         sigma = np.ascontiguousarray(sigma, dtype=np.int8)

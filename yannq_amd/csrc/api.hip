@@ -106,7 +106,7 @@ struct yq_ctx {
   int skind = 0, chains = 0, sweep_factor = 1, sprec = 0;
   int ntemps = 1;  // parallel tempering: chains = ntemps x chain groups
   uint64_t seed = 0, rand_epoch = 0;
-  Buf sig_state, ctr, acc, tbuf;
+  Buf sig_state, ctr, acc, tbuf, lrbuf;
 
   // current batch; weighted: an exact basis with |psi|^2 weights bw (row
   // scales brs = sqrt(ns bw)), see exact.hip
@@ -154,7 +154,7 @@ struct yq_ctx {
   Buf ap_ffs, ap_ffb;
 
   yq_ctx() {
-    for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &tbuf, &bsig,
+    for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &tbuf, &lrbuf, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp, &dscal,
                    &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
@@ -1493,6 +1493,39 @@ YQ_API int yq_samples_get(yq_ctx* c, int8_t* sigma, double* theta) {
       HIPCHK(c, hipMemcpyAsync(theta, c->btheta.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->st));
     }
   }
+  return sync(c);
+}
+
+YQ_API int yq_rbm_log_ratio(yq_ctx* c, int ns, const int8_t* sigma, int k, const int* sites,
+                            double* out) {
+  TRY(need_rbm(c));
+  if (ns < 1 || k < 1 || !sigma || !sites || !out)
+    return fail(c, YQ_ERR_ARG, "need ns >= 1, k >= 1 and arrays");
+  const int n = c->n, m = c->m;
+  for (size_t q = 0; q < (size_t)ns * n; ++q)
+    if (sigma[q] != 1 && sigma[q] != -1) return fail(c, YQ_ERR_ARG, "spins must be +1 or -1");
+  for (size_t q = 0; q < (size_t)ns * k; ++q)
+    if (sites[q] >= n) return fail(c, YQ_ERR_ARG, "flip site out of range");
+  const size_t nc = ncomp(c);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_sig = up((size_t)ns * n), b_sites = up(sizeof(int) * (size_t)ns * k);
+  const size_t b_th = up(sizeof(double) * nc * (size_t)ns * m);
+  const size_t b_out = up(sizeof(double) * nc * (size_t)ns);
+  TRY(ensure(c, c->lrbuf, b_sig + b_sites + b_th + b_out));
+  char* base = static_cast<char*>(c->lrbuf.p);
+  int8_t* d_sig = reinterpret_cast<int8_t*>(base);
+  int* d_sites = reinterpret_cast<int*>(base + b_sig);
+  double* d_th = reinterpret_cast<double*>(base + b_sig + b_sites);
+  double* d_out = reinterpret_cast<double*>(base + b_sig + b_sites + b_th);
+  HIPCHK(c, hipMemcpyAsync(d_sig, sigma, (size_t)ns * n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(d_sites, sites, sizeof(int) * (size_t)ns * k, hipMemcpyHostToDevice,
+                           c->st));
+  if (c->cplx) launch_theta_batch_c(W64(c), b64(c), d_sig, ns, n, m, d_th, 0, c->st);
+  else launch_theta_batch(W64(c), b64(c), d_sig, ns, n, m, d_th, 0, c->st);
+  launch_log_ratio_vec(W64(c), a64(c), d_sig, d_th, d_sites, ns, n, m, k, c->cplx ? 1 : 0, d_out,
+                       c->st);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(out, d_out, sizeof(double) * nc * ns, hipMemcpyDeviceToHost, c->st));
   return sync(c);
 }
 

@@ -462,7 +462,89 @@ void theta_pass(const double* W64, const double* b64, const int8_t* sigma, int r
                        static_cast<double*>(out));
 }
 
+// logRatio(vector) of RBMState.hpp:108-126 for ns configurations, one wave
+// each: log psi(sigma with the listed sites flipped) - log psi(sigma).  Row s
+// of sites lists up to k sites (entries < 0 are skipped); theta (ns x m,
+// interleaved complex when CPLX) is W sigma + b of the unflipped
+// configuration.  Per hidden unit t_j = theta_j - sum over the list of
+// 2 sigma_e W_je (list order); the wave sums logcosh(t_j) - logcosh(theta_j)
+// lane-strided with an xor butterfly, then adds the visible terms
+// -2 a_e sigma_e in list order.
+template <bool CPLX>
+__global__ void __launch_bounds__(256)
+    k_log_ratio_vec(const double* __restrict__ W, const double* __restrict__ a,
+                    const int8_t* __restrict__ sigma, const double* __restrict__ theta,
+                    const int* __restrict__ sites, int ns, int n, int m, int k,
+                    double* __restrict__ out) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= ns) return;  // wave-uniform; no barrier in this kernel
+  const int8_t* sg = sigma + (size_t)s * n;
+  const int* v = sites + (size_t)s * k;
+  double pr = 0.0, pi = 0.0;
+  for (int j = lane; j < m; j += kWave) {
+    if (CPLX) {
+      const double tr0 = theta[2 * ((size_t)s * m + j)], ti0 = theta[2 * ((size_t)s * m + j) + 1];
+      double tr = tr0, ti = ti0;
+      for (int e = 0; e < k; ++e) {
+        const int q = v[e];
+        if (q < 0) continue;
+        const double two = sg[q] > 0 ? 2.0 : -2.0;
+        tr -= two * W[2 * ((size_t)q * m + j)];
+        ti -= two * W[2 * ((size_t)q * m + j) + 1];
+      }
+      double r1, i1, r0, i0;
+      logcosh_c_d(tr, ti, &r1, &i1);
+      logcosh_c_d(tr0, ti0, &r0, &i0);
+      pr += r1 - r0;
+      pi += i1 - i0;
+    } else {
+      const double t0 = theta[(size_t)s * m + j];
+      double t = t0;
+      for (int e = 0; e < k; ++e) {
+        const int q = v[e];
+        if (q < 0) continue;
+        t -= (sg[q] > 0 ? 2.0 : -2.0) * W[(size_t)q * m + j];
+      }
+      pr += logcosh_d(t) - logcosh_d(t0);
+    }
+  }
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    pr += __shfl_xor(pr, w);
+    if (CPLX) pi += __shfl_xor(pi, w);
+  }
+  if (lane == 0) {
+    double vr = 0.0, vi = 0.0;
+    for (int e = 0; e < k; ++e) {
+      const int q = v[e];
+      if (q < 0) continue;
+      const double two = sg[q] > 0 ? 2.0 : -2.0;
+      vr -= two * a[CPLX ? 2 * q : q];
+      if (CPLX) vi -= two * a[2 * q + 1];
+    }
+    if (CPLX) {
+      out[2 * s] = vr + pr;
+      out[2 * s + 1] = vi + pi;
+    } else {
+      out[s] = vr + pr;
+    }
+  }
+}
+
 }  // namespace
+
+void launch_log_ratio_vec(const double* W, const double* a, const int8_t* sigma,
+                          const double* theta, const int* sites, int ns, int n, int m, int k,
+                          int cplx, double* out, hipStream_t st) {
+  const dim3 grid((unsigned)((ns + 3) / 4)), block(256);
+  if (cplx)
+    hipLaunchKernelGGL(k_log_ratio_vec<true>, grid, block, 0, st, W, a, sigma, theta, sites, ns,
+                       n, m, k, out);
+  else
+    hipLaunchKernelGGL(k_log_ratio_vec<false>, grid, block, 0, st, W, a, sigma, theta, sites, ns,
+                       n, m, k, out);
+}
 
 void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_items,
                          int npad, int kind, int mixed, int blocks, hipStream_t st) {

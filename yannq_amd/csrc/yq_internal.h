@@ -88,6 +88,13 @@ void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_it
 // theta = W sigma + b for a batch on FP64 MFMA (the sum over the spins in
 // index order, exactly as a sequential FP64 loop adds it, then + b, optionally
 // rounded to float); out is double or float.
+// logRatio(vector) (RBMState.hpp:108-126) of ns configurations: out[s] (a
+// complex pair when cplx) for flipping the up to k sites listed in row s of
+// sites (entries < 0 skipped); W, a, theta in the machine's layout (complex
+// interleaved when cplx), theta = W sigma + b of the unflipped rows.
+void launch_log_ratio_vec(const double* W, const double* a, const int8_t* sigma,
+                          const double* theta, const int* sites, int ns, int n, int m, int k,
+                          int cplx, double* out, hipStream_t st);
 void launch_theta_batch(const double* W64, const double* b64,
                         const int8_t* sigma, int ns, int n, int m, void* out,
                         int out_float, hipStream_t st);
