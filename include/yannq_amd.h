@@ -223,6 +223,13 @@ YQ_API int yq_eloc(yq_ctx* ctx, double* eloc, double* mean, double* var);
  * (SRMat.hpp:89-103); with YQ_SOLVE_EXPLICIT also S (corrMat, :142-149).
  * Computes E_loc first if yq_eloc was not called on this batch. */
 YQ_API int yq_sr_build(yq_ctx* ctx, int solver, int precision);
+/* fisherAvg of RunRBM.hpp:143-144, 194-203 (ObsAvg<Matrix>, :13-36) on the
+ * device: after an explicit yq_sr_build, avg = beta2 avg + (1 - beta2) S and
+ * S := avg / (1 - beta2^t) for the t-th call since the last reset (reset =
+ * 1, or a change of shape, starts from avg = 0, t = 0), so the following
+ * yq_sr_get(YQ_SR_S) / yq_sr_apply / yq_sr_solve use the averaged matrix.
+ * The average lives in the context across builds (dim x dim doubles). */
+YQ_API int yq_sr_fisher_average(yq_ctx* ctx, double beta2, int reset);
 /* Copy one of the built items to the host. */
 YQ_API int yq_sr_get(yq_ctx* ctx, int item, double* out);
 /* apply: O_c^T (O_c v) / Ns + shift v, SRMat.hpp:170-189. */

@@ -451,6 +451,12 @@ class SRMat {
     lastResid_ = res;
     return x;
   }
+  // fisherAvg.update(corrMat()) followed by getAvg() (RunRBM.hpp:13-36,
+  // 197-198) on the device: S of this build becomes the bias-corrected
+  // moving average that the next solveExact factors (explicit solver)
+  void fisherAverage(double beta2, bool reset) {
+    detail::check(ctx(), yq_sr_fisher_average(ctx(), beta2, reset ? 1 : 0));
+  }
   int lastIterations() const { return lastIters_; }
   double lastResidual() const { return lastResid_; }
   int solver() const { return solver_; }
@@ -640,10 +646,10 @@ class RunRBM {
     cgTol_ = cgTol;
   }
   // The first-moment averaging (beta1) runs on the host; the fisher average
-  // (beta2 > 0) needs a dense S per epoch and is not supported.
+  // (beta2) on the device, on the explicit S of every epoch.
   void setMomentum(double beta1 = 0.0, double beta2 = 0.0) {
-    if (beta2 != 0.0) throw Error(YQ_ERR_UNSUPPORTED, "setMomentum: beta2 must be 0");
     beta1_ = beta1;
+    beta2_ = beta2;
   }
   void setGradClip(double c) { gradClip_ = c; }
   void setOptimizer(const std::string& name, double alpha = 0.05, double p = 0.0) {
@@ -670,16 +676,17 @@ class RunRBM {
   }
 
   // :122-221.  callback(ll, E, |v|, cgErr, smp_ms, slv_ms).  The solver is
-  // explicit S + Cholesky unless setSolverParams(true, .), which selects CG
-  // on the factored O (precision given by the sampler).
+  // explicit S + Cholesky unless setSolverParams(true, .) with beta2 = 0,
+  // which selects CG on the factored O (precision given by the sampler).
   template <class Sampler, class Callback, class Randomizer, class Hamiltonian>
   void run(Sampler& sampler, Callback&& callback, Randomizer&& randomizer, Hamiltonian&& ham,
            int nSweeps = -1, int nSweepsDiscard = -1) {
     using Clock = std::chrono::high_resolution_clock;
     if (!machineInitialized_) initializeRandom();
     if (!opt_) setOptimizer("SGD");
-    const int solver = useCG_ ? YQ_SOLVE_CG_FACTORED : YQ_SOLVE_EXPLICIT;
-    const int precision = useCG_ ? sampler.precision() : YQ_FP64;
+    const bool cg = useCG_ && beta2_ == 0.0;  // RunRBM.hpp:189
+    const int solver = cg ? YQ_SOLVE_CG_FACTORED : YQ_SOLVE_EXPLICIT;
+    const int precision = cg ? sampler.precision() : YQ_FP64;
     SRMat<Machine, typename std::decay<Hamiltonian>::type> srm(qs_, ham, solver, precision);
     sampler.initializeRandomEngine();
     const int dim = (int)getDim();
@@ -706,16 +713,23 @@ class RunRBM {
       const double lambda = std::max(lambdaIni_ * std::pow(lambdaDecay_, ll), lambdaMin_);
       // ObsAvg with beta1 (RunRBM.hpp:13-36): bias-corrected moving average
       const Vector g = srm.energyGrad();
-      ++t;
-      for (int i = 0; i < dim; ++i) gradAvg[i] = beta1_ * gradAvg[i] + (1.0 - beta1_) * g[i];
+      auto grad_update = [&]() {
+        ++t;
+        for (int i = 0; i < dim; ++i) gradAvg[i] = beta1_ * gradAvg[i] + (1.0 - beta1_) * g[i];
+      };
+      grad_update();
       Vector grad(dim);
       for (int i = 0; i < dim; ++i) grad[i] = gradAvg[i] / (1.0 - std::pow(beta1_, t));
       Vector v;
       double cgErr = 0.0;
-      if (useCG_) {
+      if (cg) {
         v = srm.solveCG(grad, lambda, cgTol_);
         cgErr = srm.lastResidual() * norm(grad);  // |apply(v) - grad|
       } else {
+        // :196-202: the gradient average is updated a second time (the
+        // solve uses the first), S enters fisherAvg (identity at beta2 = 0)
+        grad_update();
+        if (beta2_ != 0.0) srm.fisherAverage(beta2_, ll == 0);
         v = srm.solveExact(grad, lambda);
       }
       const double nv = norm(v);
@@ -740,7 +754,7 @@ class RunRBM {
   std::mt19937_64 re_;
   std::unique_ptr<Optimizer<T>> opt_;
   bool machineInitialized_ = false, useCG_ = false;
-  double cgTol_ = 1e-4, beta1_ = 0.0, gradClip_ = 0.0;
+  double cgTol_ = 1e-4, beta1_ = 0.0, beta2_ = 0.0, gradClip_ = 0.0;
   double lambdaIni_ = 1e-3, lambdaDecay_ = 1.0, lambdaMin_ = 1e-4;
   int maxIter_ = 2000, saveWfPer_ = 100;
 };

@@ -2,7 +2,8 @@
 // Examples/RBMTFI.cpp drives the reference: RunRBM + SamplerMT +
 // LocalSweeper + TFIsing(N, -1, -h).  Prints "final <E> <E_exact>" and exits
 // non-zero when the last 20 epochs average more than 0.5% above the exact
-// ground-state energy -2/sin(pi/2N).
+// ground-state energy -2/sin(pi/2N).  Modes: exact (Cholesky), cg (factored
+// CG), fisher (Cholesky of the fisher average, setMomentum(0.3, 0.6)).
 #include <cmath>
 #include <cstdio>
 #include <iostream>
@@ -15,7 +16,8 @@ using namespace yannq_amd;
 int main(int argc, char** argv) {
   const int N = 20, alpha = 1;
   const double h = 1.0;
-  const bool useCG = argc > 1 && std::string(argv[1]) == "cg";
+  const std::string mode = argc > 1 ? argv[1] : "exact";
+  const bool useCG = mode == "cg";
   Device dev(0);
   std::ostringstream log;
   RunRBM<double> runner(dev, N, alpha, true, log);
@@ -24,6 +26,7 @@ int main(int argc, char** argv) {
   runner.setIterParams(299, 0);
   runner.setOptimizer("SGD", 0.01, 0.0);
   runner.setSolverParams(useCG, 1e-8);
+  if (mode == "fisher") runner.setMomentum(0.3, 0.6);
   TFIsing ham(N, -1.0, -h);
   LocalSweeper sweeper(N);
   auto sampler = runner.createSampler(sweeper, 1, 16, 4321);

@@ -27,7 +27,7 @@ def test_concept_layer_compiles(yq, tmp_path, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["exact", "cg"])
+@pytest.mark.parametrize("mode", ["exact", "cg", "fisher"])
 def test_runner_reaches_ground_state(yq, tmp_path, mode):
     from tests.conftest import _gpu_available
     if not _gpu_available():

@@ -274,3 +274,36 @@ def test_mixed_solve_budget_equal_to_converged_count_and_nan_rhs(ctx, orc):
     with pytest.raises(yq.YqError) as ei:
         ctx.sr_solve(1e-3, tol=1e-8, max_iter=5000, rhs=b)
     assert ei.value.code == yq.ERR_NAN, str(ei.value)
+
+
+def test_fisher_average(ctx, orc):
+    """yq_sr_fisher_average is ObsAvg<Matrix> of RunRBM.hpp:13-36 on the
+    explicit S: after builds on two batches with beta2 = 0.7 the matrix that
+    S reads and the Cholesky solves is (0.7 (0.3 S1) + 0.3 S2) / (1 - 0.7^2);
+    a reset starts over."""
+    n, m, ns = 10, 20, 800
+    rbm = orc.RBM.random(n, m, 0.1, np.random.default_rng(3))
+    ctx.rbm_create(n, m)
+    ctx.set_params(rbm.params())
+    ctx.ham_set(0, n, J=-1.0, h=-1.0)
+    rng = np.random.default_rng(5)
+    Ss, Fs = [], []
+    b = 0.7
+    for q in range(2):
+        sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+        th = orc.theta(rbm, sig)
+        el = orc.eloc(orc.ham(0, n), rbm, sig, th)
+        _, _, F, S = orc.sr_build(rbm, sig, th, el)
+        Ss.append(S)
+        Fs.append(F)
+        ctx.samples_set(sig)
+        ctx.sr_build(0)
+        ctx.sr_fisher_average(b, reset=(q == 0))
+    want = (b * ((1 - b) * Ss[0]) + (1 - b) * Ss[1]) / (1 - b ** 2)
+    assert _rel(ctx.sr_get(3), want) < 1e-12
+    x, res, _ = ctx.sr_solve(1e-3, rhs=Fs[1])
+    assert _rel(x, orc.llt_solve(want, 1e-3, Fs[1])) < 1e-9 and res < 1e-12
+    ctx.sr_fisher_average(b, reset=True)   # a fresh average of the current S
+    assert _rel(ctx.sr_get(3), want) < 1e-12
+    with pytest.raises(Exception):
+        ctx.sr_fisher_average(1.0)

@@ -110,6 +110,7 @@ def lib():
         "yq_ff_jacobian": [p, i, p, p, p], "yq_ff_forward_dev": [p, i, p, p, p],
         "yq_ff_jacobian_dev": [p, i, p, p, p], "yq_ff_vjp_dev": [p, i, p, p, p, p],
         "yq_rbm_log_ratio": [p, i, p, i, p, p],
+        "yq_sr_fisher_average": [p, d, i],
         "yq_ff_vjp_dev_noalloc": [p, i, p, p, p, p],
         "yq_ckpt_rbm_encode": [i, i, i, i, p, i, p, C.c_size_t],
         "yq_ckpt_rbm_decode": [i, p, C.c_size_t, i, p, p, p, p, C.c_size_t],
@@ -349,6 +350,9 @@ class Context:
     # SR
     def sr_build(self, solver, precision=FP64):
         self._chk(self._L.yq_sr_build(self._h, solver, precision))
+
+    def sr_fisher_average(self, beta2, reset=False):
+        self._chk(self._L.yq_sr_fisher_average(self._h, beta2, int(reset)))
 
     def sr_get(self, item):
         D = self.sr_dim

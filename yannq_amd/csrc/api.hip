@@ -128,6 +128,10 @@ struct yq_ctx {
   // solver options: Jacobi preconditioning, warm start from the last solution
   int precond = 0, warm_start = 0;
   Buf diagS, dinv64, vz, xprev, ones;
+  // fisherAvg (yq_sr_fisher_average): the running average of S and its count
+  Buf savg;
+  int fisher_t = 0;
+  size_t fisher_len = 0;
   bool diag_ready = false;  // diagS holds diag(S) of the current build
   bool have_xprev = false;
   int ones_len = 0;
@@ -157,7 +161,7 @@ struct yq_ctx {
     for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &tbuf, &lrbuf, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp, &dscal,
-                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz,
+                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz, &savg,
                    &xprev, &ones, &cpd, &cpf, &m_sargs, &m_items, &m_eargs, &m_stats, &ap_ffs,
                    &ap_ffb})
       bufs.push_back(b);
@@ -1683,6 +1687,35 @@ YQ_API int yq_sr_build(yq_ctx* c, int solver, int precision) {
   TRY(need_rbm(c));
   if (!c->have_batch) return fail(c, YQ_ERR_STATE, "no samples");
   return sr_build(c, solver, precision);
+}
+
+YQ_API int yq_sr_fisher_average(yq_ctx* c, double beta2, int reset) {
+  TRY(need_rbm(c));
+  if (!c->have_sr || c->sr_solver != YQ_SOLVE_EXPLICIT)
+    return fail(c, YQ_ERR_STATE, "fisher average needs an explicit yq_sr_build");
+  if (!(beta2 >= 0.0 && beta2 < 1.0)) return fail(c, YQ_ERR_ARG, "need 0 <= beta2 < 1");
+  const int nc = c->cplx ? 2 : 1;
+  std::vector<SrBlock> blk;
+  if (c->cplx) blk = {SrBlock{0, c->dim, nullptr, c->S.as<double>()}};
+  else blk = sr_blocks(c);
+  size_t len = 0;
+  for (const SrBlock& b : blk) len += nc * b.d * b.d;
+  if (reset || len != c->fisher_len) {
+    TRY(ensure(c, c->savg, sizeof(double) * len));
+    HIPCHK(c, hipMemsetAsync(c->savg.p, 0, sizeof(double) * len, c->st));
+    c->fisher_t = 0;
+    c->fisher_len = len;
+  }
+  ++c->fisher_t;
+  const double corr = 1.0 - std::pow(beta2, c->fisher_t);
+  size_t off = 0;
+  for (const SrBlock& b : blk) {
+    launch_ema_lower(c->savg.as<double>() + off, b.S, b.d, nc, beta2, corr, c->st);
+    off += nc * b.d * b.d;
+  }
+  LAUNCHCHK(c);
+  c->diag_ready = false;
+  return sync(c);
 }
 
 YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {

@@ -109,6 +109,24 @@ __global__ void k_copy_lower_shift(const double* __restrict__ S,
   }
 }
 
+// ObsAvg<Matrix> (Runners/RunRBM.hpp:13-36) on the lower triangle of a
+// column-major dim x dim S (nc doubles per entry): avg = beta avg +
+// (1 - beta) S, then S = avg / corr with corr = 1 - beta^t (getAvg)
+__global__ void k_ema_lower(double* __restrict__ avg, double* __restrict__ S, size_t dim, int nc,
+                            double beta, double corr) {
+  const size_t total = dim * dim;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    if (q % dim < q / dim) continue;
+    for (int k = 0; k < nc; ++k) {
+      const size_t e = q * nc + k;
+      const double a = beta * avg[e] + (1.0 - beta) * S[e];
+      avg[e] = a;
+      S[e] = a / corr;
+    }
+  }
+}
+
 // S = sum_p P[p] over the lower triangle (column-major dim x dim, nc doubles
 // per entry), partial products in index order
 __global__ void k_sum_lower(const double* __restrict__ P, int np, size_t dim, int nc,
@@ -439,6 +457,13 @@ void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hi
   const size_t total = dim * dim;
   const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
   hipLaunchKernelGGL(k_sum_lower, dim3(grid), dim3(256), 0, st, P, np, dim, nc, S);
+}
+
+void launch_ema_lower(double* avg, double* S, size_t dim, int nc, double beta, double corr,
+                      hipStream_t st) {
+  const size_t total = dim * dim;
+  const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_ema_lower, dim3(grid), dim3(256), 0, st, avg, S, dim, nc, beta, corr);
 }
 
 void launch_add_diag(double* M, int dim, double shift, hipStream_t st) {

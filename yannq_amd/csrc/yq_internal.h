@@ -225,6 +225,9 @@ void launch_center_force_c(double* Oh, const double* omean_h, const double* e, i
 // S = sum of np column-major dim x dim partial products, lower triangle
 // (nc = 2: complex entries as pairs)
 void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hipStream_t st);
+// lower triangle: avg = beta avg + (1 - beta) S, S = avg / corr
+void launch_ema_lower(double* avg, double* S, size_t dim, int nc, double beta, double corr,
+                      hipStream_t st);
 // S = alpha O_c^T O_c, lower triangle of the dim x dim column-major S, on
 // FP64 MFMA (syrk.hip).  Factored: O_c[s][c] = (sigma'_{s,i(c)} T'_{s,j(c)} -
 // omean_c) rs_s (rs may be null) from the int8 spins and T = tanh(theta)
