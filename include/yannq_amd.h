@@ -58,7 +58,10 @@ typedef enum { YQ_LOCAL = 0, YQ_EXCHANGE = 1 } yq_sweeper_kind;
  * as split-bf16 MFMA with float accumulation; residual, refinement and update
  * in double. */
 typedef enum { YQ_FP64 = 0, YQ_MIXED = 1 } yq_precision;
-/* YQ_SOLVE_EXPLICIT: S = O_c^T O_c / Ns and a Cholesky solve.
+/* YQ_SOLVE_EXPLICIT: S = O_c^T O_c / Ns and a Cholesky solve; with YQ_MIXED
+ *   (real machines, unweighted samples) S comes from split-bf16 MFMA with
+ *   float accumulation and the FP32 Cholesky factor drives an FP64
+ *   iterative refinement against the FP64 operator.
  * YQ_SOLVE_CG: matrix-free conjugate gradients on O_c (materialised O in
  *   FP64, the reference's apply), no S.
  * YQ_SOLVE_CG_FACTORED: matrix-free CG on the factored O (sigma and
@@ -155,7 +158,7 @@ YQ_API int yq_ham_set(yq_ctx* ctx, const yq_ham_desc* desc);
 /* SamplerMT(qs, nTmps, nChainsPer, sweeper): SamplerMT.hpp:44-60, with the
  * sweeper's nSweep (LocalSweeper.hpp:19, SwapSweeper.hpp:15) as
  * sweep_factor.  seed keys the Philox streams of every chain.  n_temps 1, or
- * even and <= 16 (real machines): parallel tempering with nTmps x nChainsPer
+ * even and <= 16 (real and complex machines): parallel tempering with nTmps x nChainsPer
  * chain states, state g * n_temps + t at beta_t = (n_temps - t) / n_temps,
  * replica swaps after every sweep (mixChains, SamplerMT.hpp:120-153) and one
  * sample per chain group per sweep from the beta = 1 slot; state and counter
@@ -238,8 +241,10 @@ YQ_API int yq_sr_apply(yq_ctx* ctx, double shift, const double* v,
 /* Solve (S + shift I) x = rhs (rhs = F when null): solveExact
  * (SRMat.hpp:231-237, RunRBM.hpp:194-203) for the explicit solver,
  * solveCG (SRMat.hpp:197-223) otherwise.  resid = |(S+shift)x - rhs|/|rhs|,
- * iters = CG iterations (0 for Cholesky).  tol is the CG relative residual
- * (for YQ_MIXED factored CG: the outer FP64 refinement target). */
+ * iters = CG iterations (0 for an FP64 Cholesky, the refinement rounds for
+ * a mixed explicit solve).  tol is the CG relative residual (for YQ_MIXED:
+ * the outer FP64 refinement target, 1e-12 when tol <= 0 on the mixed
+ * explicit solve). */
 YQ_API int yq_sr_solve(yq_ctx* ctx, double shift, double tol, int max_iter,
                        const double* rhs, double* x, double* resid,
                        int* iters);

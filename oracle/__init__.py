@@ -92,6 +92,7 @@ def lib():
             "or_c_eloc": (None, [p, i, i, p, p, p, p, p]),
             "or_c_sr_build": (None, [i, i, i, i, p, p, p, p, p, p, p, p]),
             "rp_c_sample": (i, [p, i, p, p, p, p, p, p, p, i, i, p, p]),
+            "rp_c_sample_pt": (i, [p, i, i, p, p, p, p, p, p, p, i, i, p, p]),
             "rp_c_eloc": (None, [p, i, i, i, i, p, p, p, p, p]),
             "rp_logcosh_re_d": (d, [d, d]),
             "rp_logcosh_re_f": (C.c_float, [C.c_float, C.c_float]),
@@ -527,6 +528,27 @@ def c_sample_replay(rbm, sigma, *, seed, n_sweeps, n_discard, kind=LOCAL, precis
     rc = lib().rp_c_sample(C.byref(cfg), Cn, _p(streams), _p(rbm.W), _p(rbm.a), _p(rbm.b),
                            _p(sigma), _p(ctr), _p(acc), n_sweeps, n_discard, _p(out_s),
                            _p(out_t))
+    if rc:
+        raise ValueError("bad replay configuration")
+    return sigma, ctr, acc, out_s, out_t
+
+
+def c_sample_replay_pt(rbm, sigma, *, n_temps, seed, n_sweeps, n_discard, kind=LOCAL,
+                       precision=FP64, lanes=64, sweep_factor=1, ctr=None, acc=None):
+    """Replay of the complex parallel-tempering kernel (sample_replay_pt's
+    arrangement; theta samples complex128)."""
+    sigma = i8(sigma).copy()
+    Cn = sigma.shape[0]
+    groups = Cn // n_temps
+    streams = np.arange(Cn, dtype=np.uint32)
+    ctr = np.zeros(Cn, np.uint64) if ctr is None else np.array(ctr, dtype=np.uint64)
+    acc = np.zeros(Cn, np.uint64) if acc is None else np.array(acc, dtype=np.uint64)
+    cfg = RpCfg(rbm.n, rbm.m, int(rbm.use_bias), kind, precision, lanes, sweep_factor, 1.0, seed)
+    out_s = np.empty((n_sweeps * groups, rbm.n), dtype=np.int8)
+    out_t = np.empty((n_sweeps * groups, rbm.m), np.complex128)
+    rc = lib().rp_c_sample_pt(C.byref(cfg), n_temps, groups, _p(streams), _p(rbm.W),
+                              _p(rbm.a), _p(rbm.b), _p(sigma), _p(ctr), _p(acc), n_sweeps,
+                              n_discard, _p(out_s), _p(out_t))
     if rc:
         raise ValueError("bad replay configuration")
     return sigma, ctr, acc, out_s, out_t

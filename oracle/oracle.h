@@ -165,6 +165,12 @@ void rp_randomize(uint64_t seed, uint64_t epoch, int C, const uint32_t* stream,
 int rp_c_sample(const rp_cfg* cfg, int C, const uint32_t* stream, const double* W,
                 const double* a, const double* b, int8_t* sigma, uint64_t* ctr, uint64_t* acc,
                 int n_sweeps, int n_discard, int8_t* out_sigma, double* out_theta);
+// Parallel tempering replay of a complex machine (the arrangement of
+// rp_sample_pt, lp = Re a . sigma + sum_j Re logcosh(theta_j)).
+int rp_c_sample_pt(const rp_cfg* cfg, int T, int groups, const uint32_t* stream,
+                   const double* W, const double* a, const double* b, int8_t* sigma,
+                   uint64_t* ctr, uint64_t* acc, int n_sweeps, int n_discard,
+                   int8_t* out_sigma, double* out_theta);
 // Complex local energies with the kernels' arithmetic (precision 0: fp64, 1:
 // fp32 theta and log-ratios); eloc complex interleaved.  The phase uses the
 // host atan2, so the device agrees to a few ulp, not bit for bit.

@@ -376,6 +376,174 @@ extern "C" void rp_c_eloc(const or_ham* H, int precision, int ns, int n, int m, 
     c_eloc_all<double>(H, ns, n, m, W, a, sigma, theta, eloc);
 }
 
+namespace {
+// Parallel tempering of a complex machine with the kernel's arithmetic
+// (SamplerMT.hpp:44-60, 120-153, 180-204; the real machine's replay,
+// replay_pt_group in oracle.cpp, with complex theta): slots t = 0 .. T-1 of
+// chain group g at beta_t = (T - t) / T, sweeps with Re logcosh, then the
+// two swap passes on lp = Re a . sigma + sum_j Re logcosh(theta_j), each
+// sum as the lane-strided tree; slot 0 recorded after every sweep past the
+// discard phase.
+template <typename R>
+void replay_pt_group_c(const rp_cfg* cfg, int T, int groups, int g, const uint32_t* stream,
+                       const double* W, const double* a, const double* b, int8_t* sigma,
+                       uint64_t* ctr, uint64_t* acc, int n_sweeps, int n_discard,
+                       int8_t* out_sigma, double* out_theta) {
+  const int n = cfg->n, m = cfg->m, L = cfg->lanes;
+  const bool mixed = cfg->precision == 1;
+  std::vector<R> Wr((size_t)n * m), Wi((size_t)n * m), ar(n), ur(m), ui(m), lt(m), part(L);
+  for (size_t q = 0; q < (size_t)n * m; ++q) {
+    Wr[q] = (R)W[2 * q];
+    Wi[q] = (R)W[2 * q + 1];
+  }
+  for (int i = 0; i < n; ++i) ar[i] = (R)a[2 * i];
+  struct Slot {
+    int8_t* sig;
+    std::vector<R> tr, ti, lc;
+    uint64_t* ctr;
+    uint64_t* acc;
+    uint32_t stream;
+  };
+  std::vector<Slot> ch(T);
+  auto refresh = [&](Slot& c) {
+    for (int j = 0; j < m; ++j) {
+      double sr = 0.0, si = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double wr = W[2 * ((size_t)i * m + j)], wi = W[2 * ((size_t)i * m + j) + 1];
+        sr = c.sig[i] > 0 ? sr + wr : sr - wr;
+        si = c.sig[i] > 0 ? si + wi : si - wi;
+      }
+      c.tr[j] = (R)(sr + b[2 * j]);
+      c.ti[j] = (R)(si + b[2 * j + 1]);
+      c.lc[j] = lc_re<R>(c.tr[j], c.ti[j]);
+    }
+  };
+  auto tree = [&]() {
+    for (int s = L / 2; s >= 1; s /= 2)
+      for (int q = 0; q < s; ++q) part[q] = part[q] + part[q + s];
+    return part[0];
+  };
+  for (int q = 0; q < T; ++q) {
+    const int c = g * T + q;
+    ch[q].sig = sigma + (size_t)c * n;
+    ch[q].tr.resize(m);
+    ch[q].ti.resize(m);
+    ch[q].lc.resize(m);
+    ch[q].ctr = ctr + c;
+    ch[q].acc = acc + c;
+    ch[q].stream = stream[c];
+    refresh(ch[q]);
+  }
+  const int nprop = n * cfg->sweep_factor;
+  std::vector<R> lp(T);
+  for (int sw = 0; sw < n_discard + n_sweeps; ++sw) {
+    for (int q = 0; q < T; ++q) {
+      Slot& c = ch[q];
+      if (mixed && sw > 0) refresh(c);
+      const R beta = (R)((double)(T - q) / T);
+      for (int p = 0; p < nprop; ++p) {
+        const yq::u32x4 x = yq::draw(cfg->seed, (*c.ctr)++, c.stream, yq::kDomSweep);
+        const int k = (int)yq::index_from(x.x, (uint32_t)n);
+        int l = -1;
+        R cst;
+        if (cfg->kind == 1) {
+          l = (int)yq::index_from(x.y, (uint32_t)n);
+          if (c.sig[k] == c.sig[l]) continue;
+          cst = ((R)-2 * ar[k]) * (R)c.sig[k] - ((R)2 * ar[l]) * (R)c.sig[l];
+        } else {
+          cst = ((R)-2 * ar[k]) * (R)c.sig[k];
+        }
+        for (int z = 0; z < L; ++z) part[z] = (R)0;
+        for (int j = 0; j < m; ++j) {
+          const R tk = (R)(2 * c.sig[k]);
+          R vr = c.tr[j] - tk * Wr[(size_t)k * m + j];
+          R vi = c.ti[j] - tk * Wi[(size_t)k * m + j];
+          if (l >= 0) {
+            const R tl = (R)(2 * c.sig[l]);
+            vr = vr - tl * Wr[(size_t)l * m + j];
+            vi = vi - tl * Wi[(size_t)l * m + j];
+          }
+          ur[j] = vr;
+          ui[j] = vi;
+          lt[j] = lc_re<R>(vr, vi);
+          part[j % L] = part[j % L] + (lt[j] - c.lc[j]);
+        }
+        const R delta = cst + tree();
+        const bool ok = mixed ? yq::accept_f((float)beta, (float)delta, yq::u01_float(x.z))
+                              : yq::accept_d((double)beta, (double)delta,
+                                             yq::u01_double(x.z, x.w));
+        if (ok) {
+          for (int j = 0; j < m; ++j) {
+            c.tr[j] = ur[j];
+            c.ti[j] = ui[j];
+            c.lc[j] = lt[j];
+          }
+          c.sig[k] = -c.sig[k];
+          if (l >= 0) c.sig[l] = -c.sig[l];
+          ++(*c.acc);
+        }
+      }
+      for (int z = 0; z < L; ++z) part[z] = (R)0;
+      for (int i = 0; i < n; ++i) part[i % L] = part[i % L] + ar[i] * (R)c.sig[i];
+      const R sa = tree();
+      for (int z = 0; z < L; ++z) part[z] = (R)0;
+      for (int j = 0; j < m; ++j) part[j % L] = part[j % L] + c.lc[j];
+      const R sl = tree();
+      lp[q] = sa + sl;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int> d(T, 0);
+      for (int q = pass; q + 1 < T && (pass == 0 || q + 1 <= T - 2); q += 2) {
+        const yq::u32x4 x = yq::draw(cfg->seed, *ch[q].ctr, ch[q].stream, yq::kDomMix);
+        const double dbeta = (double)(T - q - 1) / T - (double)(T - q) / T;
+        const R dl = lp[q] - lp[q + 1];
+        d[q] = mixed ? yq::accept_f((float)dbeta, (float)dl, yq::u01_float(x.z))
+                     : yq::accept_d(dbeta, (double)dl, yq::u01_double(x.z, x.w));
+      }
+      for (int q = 0; q + 1 < T; ++q) {
+        if (!d[q]) continue;
+        Slot& A = ch[q];
+        Slot& B = ch[q + 1];
+        for (int i = 0; i < n; ++i) std::swap(A.sig[i], B.sig[i]);
+        std::swap(A.tr, B.tr);
+        std::swap(A.ti, B.ti);
+        for (int j = 0; j < m; ++j) {
+          A.lc[j] = lc_re<R>(A.tr[j], A.ti[j]);
+          B.lc[j] = lc_re<R>(B.tr[j], B.ti[j]);
+        }
+        std::swap(lp[q], lp[q + 1]);
+      }
+    }
+    if (sw >= n_discard) {
+      const size_t row = (size_t)(sw - n_discard) * groups + g;
+      Slot& c = ch[0];
+      memcpy(out_sigma + row * n, c.sig, n);
+      if (mixed) refresh(c);
+      for (int j = 0; j < m; ++j) {
+        out_theta[2 * (row * m + j)] = (double)c.tr[j];
+        out_theta[2 * (row * m + j) + 1] = (double)c.ti[j];
+      }
+    }
+  }
+}
+}  // namespace
+
+extern "C" int rp_c_sample_pt(const rp_cfg* cfg, int T, int groups, const uint32_t* stream,
+                              const double* W, const double* a, const double* b, int8_t* sigma,
+                              uint64_t* ctr, uint64_t* acc, int n_sweeps, int n_discard,
+                              int8_t* out_sigma, double* out_theta) {
+  if (cfg->lanes <= 0 || (cfg->lanes & (cfg->lanes - 1)) || T < 2 || T % 2) return 1;
+  for (int g = 0; g < groups; ++g) {
+    if (cfg->precision == 1)
+      replay_pt_group_c<float>(cfg, T, groups, g, stream, W, a, b, sigma, ctr, acc, n_sweeps,
+                               n_discard, out_sigma, out_theta);
+    else
+      replay_pt_group_c<double>(cfg, T, groups, g, stream, W, a, b, sigma, ctr, acc, n_sweeps,
+                                n_discard, out_sigma, out_theta);
+  }
+  return 0;
+}
+
 extern "C" int rp_c_sample(const rp_cfg* cfg, int C, const uint32_t* stream, const double* W,
                            const double* a, const double* b, int8_t* sigma, uint64_t* ctr,
                            uint64_t* acc, int n_sweeps, int n_discard, int8_t* out_sigma,

@@ -1,7 +1,9 @@
 // A complex RBM through the header-only concept layer, written the way
 // Examples/RBMXXZ.cpp drives the reference: RunRBM<std::complex<double>> +
 // SamplerMT + SwapSweeper + XXZ(N, 1, Delta, signRule = true), lambda
-// schedule setLambda(1.0, 0.9, 1e-4), 16 chains started in the Sz = 0 sector.
+// schedule setLambda(1.0, 0.9, 1e-4), parallel tempering over 16
+// temperatures (createSampler(sweeper, 16, .) as at Examples/RBMXXZ.cpp:81,
+// here 8 chain groups) started in the Sz = 0 sector.
 // N = 10, Delta = 1: prints "final <E> <E_exact>" and exits non-zero when the
 // last 20 epochs average more than 1% above the exact ground-state energy.
 #include <cmath>
@@ -27,7 +29,7 @@ int main() {
   runner.setSolverParams(false, 1e-3);
   XXZ ham(N, 1.0, 1.0, true);
   SwapSweeper sweeper(N);
-  auto sampler = runner.createSampler(sweeper, 1, 16, 77);
+  auto sampler = runner.createSampler(sweeper, 16, 8, 77);
   auto randomizer = [N](auto& re) { return randomSigma(N, N / 2, re); };
   double tail = 0.0;
   int cnt = 0;
@@ -38,7 +40,7 @@ int main() {
     }
     if (ll % 40 == 0) std::printf("%d\t%.8f\t%.4e\t%.2e\t%ld\t%ld\n", ll, E, nv, cgErr, smp, slv);
   };
-  runner.run(sampler, callback, randomizer, ham, 128, 20);
+  runner.run(sampler, callback, randomizer, ham, 256, 20);
   tail /= cnt;
   std::printf("final %.8f %.8f\n", tail, e0);
   // the machine is complex: its parameters come back as std::complex<double>

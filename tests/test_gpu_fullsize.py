@@ -113,3 +113,27 @@ def test_c2_complex_sampler_full_size(ctx, orc, prec):
         v = rng.normal(size=len(F)) + 1j * rng.normal(size=len(F))
         Sv = O.conj().T @ (O @ v) / O.shape[0]
         assert _rel(ctx.sr_apply(1e-3, v), Sv + 1e-3 * v) < 1e-11
+
+
+def test_c3_mixed_explicit(yq, ctx):
+    """Mixed explicit solve at full size: split-bf16 S ((S + lambda) v within
+    3e-5 of the FP64 factored operator: the pair products are exact to 2^-24,
+    the fp32 accumulation over 16,384 samples leaves about 1.6e-5) and FP32
+    Cholesky refined in FP64 to a 1e-10 residual of the exact operator."""
+    sig, p = _c3_samples(yq, ctx)
+    lam = 1e-3
+    rng = np.random.default_rng(6)
+    v = rng.normal(size=DIM)
+    with yq.Context(0) as f:
+        f.rbm_create(N, M)
+        f.set_params(p)
+        f.ham_set(yq.TFI, N, J=-1.0, h=-1.0)
+        f.samples_set(sig, precision=yq.FP64)
+        f.sr_build(yq.SOLVE_CG_FACTORED, precision=yq.FP64)
+        ref = f.sr_apply(lam, v)
+        ctx.samples_set(sig, precision=yq.FP64)   # FP64 E_loc and F, split-bf16 S
+        ctx.sr_build(yq.SOLVE_EXPLICIT, precision=yq.MIXED)
+        assert _rel(ctx.sr_apply(lam, v), ref) < 3e-5
+        x, res, rounds = ctx.sr_solve(lam, tol=1e-10)
+        assert res < 1e-10 and rounds >= 1
+        assert _rel(f.sr_apply(lam, x), f.sr_get(yq.SR_F)) < 1e-9

@@ -307,3 +307,21 @@ def test_fisher_average(ctx, orc):
     assert _rel(ctx.sr_get(3), want) < 1e-12
     with pytest.raises(Exception):
         ctx.sr_fisher_average(1.0)
+
+
+@pytest.mark.parametrize("n,m,ns", [(20, 40, 2000), (7, 13, 37), (12, 30, 5001)])
+def test_mixed_explicit_s_and_refined_solve(ctx, orc, n, m, ns):
+    """The split-bf16 S of the mixed explicit build (six hi/mid/lo products on
+    bf16 MFMA, fp32 accumulation) agrees with the oracle's FP64 S to float
+    accuracy, and the mixed explicit solve (FP32 Cholesky inside FP64
+    refinement with the FP64 operator) reaches the FP64 Cholesky solution."""
+    rbm, sig, O, om, F, S = _batch(ctx, orc, n, m, ns, scale=0.1)
+    ctx.samples_set(sig, precision=0)   # FP64 samples and E_loc, mixed S
+    ctx.sr_build(0, precision=1)
+    Sm = ctx.sr_get(3)
+    assert np.abs(Sm - S).max() <= 3e-6 * np.abs(S).max()
+    assert _rel(ctx.sr_get(0), F) < 1e-10
+    lam = 1e-3
+    x, res, rounds = ctx.sr_solve(lam, tol=1e-12)
+    assert res < 1e-12 and 1 <= rounds <= 30
+    assert _rel(x, orc.llt_solve(S, lam, F)) < 1e-9

@@ -284,10 +284,10 @@ def test_complex_sr_build_uniform_weights(orc):
     assert np.linalg.matrix_rank(S) <= ns - 1
 
 
-def _python_c_sweep(orc, rbm, sig, seed, stream, ctr, kind):
+def _python_c_sweep(orc, rbm, sig, seed, stream, ctr, kind, beta=1.0):
     """One sweep with the libm restatement (Re of or_c_log_ratio*) and the
     Philox draws (LocalSweeper.hpp:32-53 / SwapSweeper.hpp:28-56 acceptance
-    exp(2 Re dlogpsi))."""
+    exp(2 beta Re dlogpsi))."""
     n = rbm.n
     sig = sig.copy()
     for _ in range(n):
@@ -303,7 +303,7 @@ def _python_c_sweep(orc, rbm, sig, seed, stream, ctr, kind):
         else:
             d = orc.c_log_ratio1(rbm, sig, th, k).real
         u = (((int(x[2]) >> 5) << 26) | (int(x[3]) >> 6)) / 2.0 ** 53
-        if u < min(1.0, math.exp(2 * d)):
+        if u < min(1.0, math.exp(2 * beta * d)):
             sig[k] = -sig[k]
             if kind == orc.SWAP:
                 sig[l] = -sig[l]
@@ -359,3 +359,96 @@ def test_complex_sampler_distribution_matches_psi2(orc):
     chi2 = ((cnt[big] - N * p[big]) ** 2 / (N * p[big])).sum()
     assert big.sum() >= 12 and chi2 < 3 * (big.sum() - 1), chi2
     assert 0.5 * np.abs(cnt / N - p).sum() < 0.02
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_complex_tempering_replay_matches_restatement(orc, kind):
+    """Complex parallel tempering (SamplerMT.hpp:120-153): sweeps at
+    beta_t = (T-t)/T with the libm restatement, then the two swap passes with
+    p = exp((beta_{t+1} - beta_t) 2 Re(log psi_t - log psi_{t+1})) from numpy's
+    complex log cosh and the lower chain's mix-domain Philox draw."""
+    rng = np.random.default_rng(21)
+    n, m, T, G = 10, 20, 4, 2
+    rbm = orc.CRBM.random(n, m, 0.4, rng)
+    Cn = T * G
+    sig0 = orc.randomize(8, 0, np.arange(Cn), n, nup=n // 2 if kind else -1)
+    seed = 0x1357_2468_ace0
+    sweeps = 4
+
+    def lpsi(s):
+        th = s.astype(float) @ rbm.W + rbm.b
+        return (s.astype(float) @ rbm.a + np.log(np.cosh(th)).sum()).real
+
+    fin, ctr, acc, ss, st = orc.c_sample_replay_pt(rbm, sig0, n_temps=T, seed=seed,
+                                                   n_sweeps=sweeps, n_discard=0, kind=kind)
+    assert (ctr == sweeps * n).all()
+    swaps = 0
+    for g in range(G):
+        state = [sig0[g * T + t].copy() for t in range(T)]
+        for sw in range(sweeps):
+            for t in range(T):
+                state[t] = _python_c_sweep(orc, rbm, state[t], seed, g * T + t, sw * n, kind,
+                                           beta=(T - t) / T)
+            lp = [lpsi(state[t]) for t in range(T)]
+            for pas in (0, 1):
+                for t in range(pas, T - 1 - pas, 2):
+                    c = g * T + t
+                    ev = (sw + 1) * n
+                    x = orc.philox((ev & 0xffffffff, ev >> 32, c, 5),
+                                   (seed & 0xffffffff, seed >> 32))
+                    u = (((int(x[2]) >> 5) << 26) | (int(x[3]) >> 6)) / 2.0 ** 53
+                    dbeta = (T - t - 1) / T - (T - t) / T
+                    if u < min(1.0, math.exp(dbeta * 2 * (lp[t] - lp[t + 1]))):
+                        state[t], state[t + 1] = state[t + 1], state[t]
+                        lp[t], lp[t + 1] = lp[t + 1], lp[t]
+                        swaps += 1
+            np.testing.assert_array_equal(ss[sw * G + g], state[0])
+        for t in range(T):
+            np.testing.assert_array_equal(fin[g * T + t], state[t])
+    assert swaps > 0
+    np.testing.assert_allclose(st, orc.c_theta(rbm, ss), atol=1e-13)
+    _, _, _, ss2, st2 = orc.c_sample_replay_pt(rbm, sig0, n_temps=T, seed=seed, n_sweeps=4,
+                                               n_discard=0, kind=kind, precision=orc.MIXED)
+    np.testing.assert_allclose(st2, orc.c_theta(rbm, ss2), atol=2e-6)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_complex_tempering_real_parameters_equal_real_replay(orc, prec):
+    """With Im W = Im a = Im b = 0 the complex tempering replay makes the real
+    tempering replay's decisions: Re logcosh of a real argument is logcosh."""
+    rng = np.random.default_rng(22)
+    n, m, T, G = 12, 24, 4, 3
+    r = orc.RBM.random(n, m, 0.5, rng)
+    c = orc.CRBM(n, m, r.W.astype(complex), r.a.astype(complex), r.b.astype(complex))
+    sig0 = orc.randomize(9, 0, np.arange(T * G), n)
+    kw = dict(n_temps=T, seed=77, n_sweeps=5, n_discard=2, precision=prec)
+    a = orc.sample_replay_pt(r, sig0, **kw)
+    b = orc.c_sample_replay_pt(c, sig0, **kw)
+    for u, v in zip(a[:4], b[:4]):
+        np.testing.assert_array_equal(u, v)
+    np.testing.assert_array_equal(b[4].imag, 0)
+    np.testing.assert_allclose(b[4].real, a[4], rtol=0, atol=2e-7 if prec else 1e-15)
+
+
+def test_complex_tempering_distribution_matches_psi2(orc):
+    """The beta = 1 slot of tempered exchange-move chains samples |psi|^2 on
+    the Sz = 0 sector (N = 6), with swaps between the temperatures."""
+    rng = np.random.default_rng(23)
+    n = 6
+    rbm = orc.CRBM.random(n, 2 * n, 0.15, rng)
+    basis, th, lp = _c_psi_table(orc, rbm)
+    sector = basis.astype(int).sum(axis=1) == 0
+    p = np.exp(2 * (lp.real - lp.real.max())) * sector
+    p /= p.sum()
+    T, G = 4, 48
+    sig0 = orc.randomize(4, 0, np.arange(T * G), n, nup=n // 2)
+    _, _, _, ss, _ = orc.c_sample_replay_pt(rbm, sig0, n_temps=T, seed=93, n_sweeps=400,
+                                            n_discard=20, kind=orc.SWAP)
+    idx = (((1 - ss.astype(np.int64)) // 2) << np.arange(n)).sum(axis=1)
+    cnt = np.bincount(idx, minlength=1 << n)
+    assert cnt[~sector].sum() == 0
+    N = cnt.sum()
+    big = sector & (N * p >= 20)
+    chi2 = ((cnt[big] - N * p[big]) ** 2 / (N * p[big])).sum()
+    assert big.sum() >= 12 and chi2 < 3 * (big.sum() - 1), chi2
+    assert 0.5 * np.abs(cnt / N - p).sum() < 0.03

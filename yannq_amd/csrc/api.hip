@@ -602,9 +602,11 @@ int sr_build(yq_ctx* c, int solver, int precision) {
     return fail(c, YQ_ERR_ARG, "unknown solver");
   if (precision != YQ_FP64 && precision != YQ_MIXED)
     return fail(c, YQ_ERR_ARG, "unknown precision");
-  if (precision == YQ_MIXED && solver != YQ_SOLVE_CG_FACTORED)
+  if (precision == YQ_MIXED && solver == YQ_SOLVE_CG)
     return fail(c, YQ_ERR_UNSUPPORTED,
-                "mixed precision is implemented for the factored CG solver only");
+                "mixed precision is implemented for the factored CG and explicit solvers");
+  if (precision == YQ_MIXED && solver == YQ_SOLVE_EXPLICIT && c->weighted)
+    return fail(c, YQ_ERR_UNSUPPORTED, "an exact weighted batch builds S in FP64");
   if (c->weighted && solver == YQ_SOLVE_CG_FACTORED)
     return fail(c, YQ_ERR_UNSUPPORTED,
                 "an exact weighted batch uses the explicit or the dense CG solver");
@@ -662,9 +664,25 @@ int sr_build(yq_ctx* c, int solver, int precision) {
                            c->F.as<double>(), c->st);
     LAUNCHCHK(c);
     if (solver == YQ_SOLVE_EXPLICIT) {
-      // S = O_c^T O_c / ns formed from the factors (O_c is not read)
+      // S = O_c^T O_c / ns formed from the factors (O_c is not read): FP64
+      // MFMA, or split-bf16 MFMA for the mixed explicit solve
       TRY(ensure(c, c->S, sizeof(double) * dim * dim));
-      TRY(syrk_real(c, nullptr, dim, ns, 1.0 / ns, c->S.as<double>()));
+      if (precision == YQ_MIXED) {
+        const size_t cap = ((size_t)2 << 30) / sizeof(double);
+        const int P = syrk64_chunks(ns, dim, cap);
+        double* part = nullptr;
+        if (P > 1) {
+          TRY(ensure(c, c->M, sizeof(double) * (size_t)P * dim * dim));
+          part = c->M.as<double>();
+        }
+        const int rc = launch_syrk_bf16x3(c->bsig.as<int8_t>(), c->T64.as<double>(),
+                                          c->omean.as<double>(), nullptr, ns, n, m, c->use_bias,
+                                          1.0 / ns, c->S.as<double>(), P, part, c->st);
+        if (rc) return fail(c, rc, "split-bf16 SYRK arguments");
+        LAUNCHCHK(c);
+      } else {
+        TRY(syrk_real(c, nullptr, dim, ns, 1.0 / ns, c->S.as<double>()));
+      }
     }
   }
   c->diag_ready = false;  // diag(S) is formed when a preconditioned solve needs it
@@ -876,9 +894,9 @@ int residual64(yq_ctx* c, double shift, const double* x, const double* b, double
                              (rocblas_int)c->dim,
                              reinterpret_cast<const rocblas_double_complex*>(x), 1, &sh,
                              reinterpret_cast<rocblas_double_complex*>(t), 1));
-  } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+  } else if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->sr_prec != YQ_MIXED) {
     TRY(symv_blocks(c, shift, x, t));
-  } else {
+  } else {  // the FP64 operator on O (a mixed explicit S is not exact)
     TRY(apply64(c, shift, x, t));
   }
   launch_axpy(-1.0, b, t, dim, c->st);
@@ -951,6 +969,65 @@ int solve_explicit(yq_ctx* c, double shift, const double* b, double* x) {
                   "Cholesky of S + shift I failed at pivot " +
                       std::to_string(info[q] + (int)blk[q].off));
   return YQ_OK;
+}
+
+// Mixed explicit solve: Cholesky of the split-bf16 S + shift I in FP32
+// (rocSOLVER spotrf / spotrs) inside FP64 iterative refinement,
+// x += (S32 + shift)^-1 r, r = b - (S + shift) x with the FP64 operator on
+// the stored O, until |r| / |b| < tol (30 rounds at most).
+int solve_explicit_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
+                         int* rounds, double* err) {
+  const size_t dim = c->dim;
+  TRY(blas(c));
+  TRY(ensure(c, c->M, sizeof(float) * dim * dim));
+  TRY(ensure(c, c->info, sizeof(rocblas_int)));
+  TRY(ensure(c, c->vr, sizeof(double) * dim));
+  TRY(ensure(c, c->vp, sizeof(double) * dim));
+  TRY(ensure(c, c->vAp, sizeof(double) * dim));
+  TRY(ensure(c, c->vz, sizeof(double) * dim));
+  float* M = c->M.as<float>();
+  launch_lower_shift_f(c->S.as<double>(), M, dim, shift, c->st);
+  LAUNCHCHK(c);
+  {
+    ProfGuard g("spotrf", c->st);
+    BLASCHK(c, rocsolver_spotrf(c->blas, rocblas_fill_lower, (rocblas_int)dim, M,
+                                (rocblas_int)dim, c->info.as<rocblas_int>()));
+  }
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  if (info != 0)
+    return fail(c, YQ_ERR_NOT_SPD,
+                "FP32 Cholesky of S + shift I failed at pivot " + std::to_string(info));
+  double *r = c->vr.as<double>(), *d = c->vp.as<double>(), *Ax = c->vAp.as<double>();
+  float* rf = c->vz.as<float>();
+  double bn2;
+  TRY(dot(c, b, b, dim, &bn2));
+  HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * dim, c->st));
+  HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+  *rounds = 0;
+  *err = bn2 > 0.0 ? 1.0 : 0.0;
+  if (!std::isfinite(bn2)) return fail(c, YQ_ERR_NAN, "non-finite right-hand side");
+  for (int k = 0; k < 30 && bn2 > 0.0; ++k) {
+    launch_to_float(r, rf, dim, c->st);
+    LAUNCHCHK(c);
+    BLASCHK(c, rocsolver_spotrs(c->blas, rocblas_fill_lower, (rocblas_int)dim, 1, M,
+                                (rocblas_int)dim, rf, (rocblas_int)dim));
+    launch_float_to_double(rf, d, dim, c->st);
+    launch_axpy(1.0, d, x, dim, c->st);
+    LAUNCHCHK(c);
+    TRY(apply64(c, shift, x, Ax));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
+    launch_axpy(-1.0, Ax, r, dim, c->st);
+    LAUNCHCHK(c);
+    double rn2;
+    TRY(dot(c, r, r, dim, &rn2));
+    *rounds = k + 1;
+    *err = sqrt(rn2 / bn2);
+    if (!std::isfinite(*err)) return fail(c, YQ_ERR_NAN, "non-finite residual");
+    if (*err < tol) break;
+  }
+  return *err < tol ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
 // FP64 iterative refinement around the mixed-precision inner CG:
@@ -1035,7 +1112,10 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
   int it = 0;
   double err = 0.0;
   int rc = YQ_OK;
-  if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->sr_prec == YQ_MIXED && !c->cplx && !c->ap) {
+    rc = solve_explicit_mixed(c, shift, b_dev, x_dev, tol > 0 ? tol : 1e-12, &it, &err);
+    if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
+  } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
     rc = solve_explicit(c, shift, b_dev, x_dev);
     if (rc) return rc;
     TRY(residual64(c, shift, x_dev, b_dev, &err));
@@ -1319,9 +1399,8 @@ YQ_API int yq_sampler_config(yq_ctx* c, int sweeper, int n_temps, int n_chains_p
   // SamplerMT.hpp:48: nTmps even or 1; one block of nTmps wavefronts per chain group
   if (n_temps < 1 || n_temps > 16 || (n_temps > 1 && n_temps % 2))
     return fail(c, YQ_ERR_ARG, "n_temps must be 1 or even and <= 16");
-  if (n_temps > 1 && (c->cplx || c->ap))
-    return fail(c, YQ_ERR_UNSUPPORTED,
-                "parallel tempering of complex and AmplitudePhase machines");
+  if (n_temps > 1 && c->ap)
+    return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering of AmplitudePhase machines");
   if (n_chains_per < 1 || sweep_factor < 1) return fail(c, YQ_ERR_ARG, "chains and sweep factor >= 1");
   if ((long long)n_temps * n_chains_per > (1 << 24)) return fail(c, YQ_ERR_ARG, "too many chains");
   if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
@@ -1442,6 +1521,7 @@ int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   // beta = 1/2 (logRatioRe = logRatio / 2, AmplitudePhaseState.hpp:52-55)
   a.beta = c->ap ? 0.5 : 1.0;
   a.ntemps = c->ntemps;
+  a.cplx = c->cplx ? 1 : 0;
   if (a.ntemps > 1) {
     int dev = 0, lds_max = 0;
     if (hipGetDevice(&dev) != hipSuccess ||

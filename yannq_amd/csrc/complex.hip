@@ -64,6 +64,131 @@ __device__ inline float lc_re<float>(float x, float y) {
   return logcosh_re_f(x, y);
 }
 
+// theta_j = (sum_i +/-W_ji, index order, fp64, Re and Im apart) + b_j, then
+// rounded to R, and lc_j = Re logcosh(theta_j), for the hidden units of this
+// lane
+template <typename R, int RMAX>
+__device__ __forceinline__ void refresh_c(const SampleArgs& A, const int8_t* sig, int lane,
+                                          R (&tr)[RMAX], R (&ti)[RMAX], R (&lc)[RMAX]) {
+  const int n = A.n, m = A.m;
+  double sr[RMAX], si[RMAX];
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) sr[r] = si[r] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const bool up = sig[i] > 0;
+    const double* wc = A.W64 + 2 * (size_t)i * m;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) {
+        const double wr = wc[2 * j], wi = wc[2 * j + 1];
+        sr[r] = up ? sr[r] + wr : sr[r] - wr;
+        si[r] = up ? si[r] + wi : si[r] - wi;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) {
+    const int j = lane + kWave * r;
+    if (j < m) {
+      tr[r] = (R)(sr[r] + A.b64[2 * j]);
+      ti[r] = (R)(si[r] + A.b64[2 * j + 1]);
+      lc[r] = lc_re<R>(tr[r], ti[r]);
+    } else {
+      tr[r] = ti[r] = lc[r] = (R)0;
+    }
+  }
+}
+
+// One sweep (A.nprop proposals) of one chain at inverse temperature beta;
+// returns the accepted count.  ev is the chain's proposal counter.
+template <typename R, int KIND, int RMAX>
+__device__ __forceinline__ uint64_t sweep_chain_c(const SampleArgs& A, int chain, int8_t* sig,
+                                                  int lane, R (&tr)[RMAX], R (&ti)[RMAX],
+                                                  R (&lc)[RMAX], R (&ur)[RMAX], R (&ui)[RMAX],
+                                                  R (&lt)[RMAX], uint64_t& ev, R beta) {
+  const int n = A.n, m = A.m;
+  const R* Wr = static_cast<const R*>(A.Wr);
+  const R* Wi = static_cast<const R*>(A.Wi);
+  const R* ar = static_cast<const R*>(A.ar);
+  uint64_t nacc = 0;
+  for (int p = 0; p < A.nprop; ++p) {
+    // wave-uniform counter and stream (lane 0's): Philox on the scalar unit
+    const uint64_t evu =
+        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ev >> 32)) << 32) |
+        (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ev);
+    ev = evu + 1;
+    const u32x4 x = draw(A.seed, evu, __builtin_amdgcn_readfirstlane((uint32_t)chain),
+                         kDomSweep);
+    const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
+    const int8_t sk = sig[k];
+    const R twok = sk > 0 ? (R)2 : (R)-2;
+    const R* wkr = Wr + (size_t)k * m;
+    const R* wki = Wi + (size_t)k * m;
+    R cst = ((R)-2 * ar[k]) * (R)sk;
+    int l = 0;
+    int8_t sl = 0;
+    R twol = (R)0;
+    const R* wlr = wkr;
+    const R* wli = wki;
+    if (KIND == 1) {
+      l = __builtin_amdgcn_readfirstlane((int)index_from(x.y, (uint32_t)n));
+      sl = sig[l];
+      if (sk == sl) continue;  // SwapSweeper.hpp:44-47, uniform branch
+      twol = sl > 0 ? (R)2 : (R)-2;
+      wlr = Wr + (size_t)l * m;
+      wli = Wi + (size_t)l * m;
+      cst = cst - ((R)2 * ar[l]) * (R)sl;
+    }
+    R part = (R)0;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) {
+        R vr = tr[r] - twok * wkr[j];
+        R vi = ti[r] - twok * wki[j];
+        if (KIND == 1) {
+          vr = vr - twol * wlr[j];
+          vi = vi - twol * wli[j];
+        }
+        ur[r] = vr;
+        ui[r] = vi;
+        lt[r] = lc_re<R>(vr, vi);
+        part = part + (lt[r] - lc[r]);
+      }
+    }
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
+    const R delta = cst + part;
+    bool ok;
+    if (sizeof(R) == 4)
+      ok = accept_f((float)beta, (float)delta, u01_float(x.z));
+    else
+      ok = accept_d((double)beta, (double)delta, u01_double(x.z, x.w));
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) {
+          tr[r] = ur[r];
+          ti[r] = ui[r];
+          lc[r] = lt[r];
+        }
+      }
+      sig[k] = (int8_t)-sk;  // every lane stores the same byte
+      if (KIND == 1) sig[l] = (int8_t)-sl;
+      ++nacc;
+    }
+  }
+  return nacc;
+}
+
+__device__ __forceinline__ void wave_sync_c() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename R, int KIND, int RMAX>
 __global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
   extern __shared__ int8_t lds_sig[];
@@ -74,128 +199,23 @@ __global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
   const int n = A.n, m = A.m;
   int8_t* sig = lds_sig + wave * A.npad;
   for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-  const R* Wr = static_cast<const R*>(A.Wr);
-  const R* Wi = static_cast<const R*>(A.Wi);
-  const R* ar = static_cast<const R*>(A.ar);
+  wave_sync_c();
   R tr[RMAX], ti[RMAX], lc[RMAX], ur[RMAX], ui[RMAX], lt[RMAX];
-
-  // theta_j = (sum_i +/-W_ji, index order, fp64, Re and Im apart) + b_j
-  auto refresh = [&]() {
-    double sr[RMAX], si[RMAX];
-#pragma unroll
-    for (int r = 0; r < RMAX; ++r) sr[r] = si[r] = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const bool up = sig[i] > 0;
-      const double* wc = A.W64 + 2 * (size_t)i * m;
-#pragma unroll
-      for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
-        if (j < m) {
-          const double wr = wc[2 * j], wi = wc[2 * j + 1];
-          sr[r] = up ? sr[r] + wr : sr[r] - wr;
-          si[r] = up ? si[r] + wi : si[r] - wi;
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
-      if (j < m) {
-        tr[r] = (R)(sr[r] + A.b64[2 * j]);
-        ti[r] = (R)(si[r] + A.b64[2 * j + 1]);
-        lc[r] = lc_re<R>(tr[r], ti[r]);
-      } else {
-        tr[r] = ti[r] = lc[r] = (R)0;
-      }
-    }
-  };
-
   uint64_t ev = A.ctr[chain];
   uint64_t nacc = 0;
   const R beta = (R)A.beta;
-  refresh();
+  refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
   bool fresh = true;
   const int total = A.n_discard + A.n_sweeps;
   for (int sw = 0; sw < total; ++sw) {
-    if (A.mixed && !fresh) refresh();
+    if (A.mixed && !fresh) refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
     fresh = false;
-    for (int p = 0; p < A.nprop; ++p) {
-      // wave-uniform counter and stream (lane 0's): Philox on the scalar unit
-      const uint64_t evu =
-          ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ev >> 32)) << 32) |
-          (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ev);
-      ev = evu + 1;
-      const u32x4 x = draw(A.seed, evu, __builtin_amdgcn_readfirstlane((uint32_t)chain),
-                           kDomSweep);
-      const int k = __builtin_amdgcn_readfirstlane((int)index_from(x.x, (uint32_t)n));
-      const int8_t sk = sig[k];
-      const R twok = sk > 0 ? (R)2 : (R)-2;
-      const R* wkr = Wr + (size_t)k * m;
-      const R* wki = Wi + (size_t)k * m;
-      R cst = ((R)-2 * ar[k]) * (R)sk;
-      int l = 0;
-      int8_t sl = 0;
-      R twol = (R)0;
-      const R* wlr = wkr;
-      const R* wli = wki;
-      if (KIND == 1) {
-        l = __builtin_amdgcn_readfirstlane((int)index_from(x.y, (uint32_t)n));
-        sl = sig[l];
-        if (sk == sl) continue;  // SwapSweeper.hpp:44-47, uniform branch
-        twol = sl > 0 ? (R)2 : (R)-2;
-        wlr = Wr + (size_t)l * m;
-        wli = Wi + (size_t)l * m;
-        cst = cst - ((R)2 * ar[l]) * (R)sl;
-      }
-      R part = (R)0;
-#pragma unroll
-      for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
-        if (j < m) {
-          R vr = tr[r] - twok * wkr[j];
-          R vi = ti[r] - twok * wki[j];
-          if (KIND == 1) {
-            vr = vr - twol * wlr[j];
-            vi = vi - twol * wli[j];
-          }
-          ur[r] = vr;
-          ui[r] = vi;
-          lt[r] = lc_re<R>(vr, vi);
-          part = part + (lt[r] - lc[r]);
-        }
-      }
-#pragma unroll
-      for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
-      const R delta = cst + part;
-      bool ok;
-      if (sizeof(R) == 4)
-        ok = accept_f((float)beta, (float)delta, u01_float(x.z));
-      else
-        ok = accept_d((double)beta, (double)delta, u01_double(x.z, x.w));
-      if (ok) {
-#pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-          const int j = lane + kWave * r;
-          if (j < m) {
-            tr[r] = ur[r];
-            ti[r] = ui[r];
-            lc[r] = lt[r];
-          }
-        }
-        sig[k] = (int8_t)-sk;  // every lane stores the same byte
-        if (KIND == 1) sig[l] = (int8_t)-sl;
-        ++nacc;
-      }
-    }
+    nacc += sweep_chain_c<R, KIND, RMAX>(A, chain, sig, lane, tr, ti, lc, ur, ui, lt, ev, beta);
     if (sw >= A.n_discard) {
       const size_t row = (size_t)(sw - A.n_discard) * A.C + chain;
       for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
       if (A.mixed) {
-        refresh();
+        refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
         fresh = true;
       }
       R* ot = static_cast<R*>(A.out_theta) + 2 * row * m;
@@ -213,6 +233,160 @@ __global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
   if (lane == 0) {
     A.ctr[chain] = ev;
     A.acc[chain] += nacc;
+  }
+}
+
+// Parallel tempering of complex machines (SamplerMT.hpp:44-60, 112-204 with
+// nTmps = T > 1): the scheme of the real kernel (sampler.hip, k_sample_pt)
+// with complex theta storage.  One block per chain group g; chain g * T + t
+// is temperature slot t at beta_t = (T - t) / T; a slot's spins and theta
+// (Re and Im) live in LDS storage idx[t]; after every sweep of all slots the
+// swap passes of mixChains (:120-153) accept with
+// u < exp((beta_{t+1} - beta_t) 2 (lp_t - lp_{t+1})), lp = Re log psi =
+// Re a . sigma + sum_j Re logcosh(theta_j); slot 0 is recorded after every
+// sweep past the discard phase.
+template <typename R, int KIND, int RMAX>
+__global__ void __launch_bounds__(256) k_sample_pt_c(const SampleArgs A) {
+  extern __shared__ unsigned char lds_raw[];
+  const int T = A.ntemps, n = A.n, m = A.m, npad = A.npad;
+  const int NW = (T % 4 == 0) ? 4 : 2;
+  R* lpv = reinterpret_cast<R*>(lds_raw);                      // 16
+  uint64_t* evs = reinterpret_cast<uint64_t*>(lds_raw + 128);  // 16
+  uint64_t* nac = reinterpret_cast<uint64_t*>(lds_raw + 256);  // 16
+  int* idx = reinterpret_cast<int*>(lds_raw + 384);            // 16
+  int8_t* strips = reinterpret_cast<int8_t*>(lds_raw + 448);   // T x npad
+  R* thr = reinterpret_cast<R*>(lds_raw + 448 + ((T * npad + 15) & ~15));  // T x m
+  R* thi = thr + (size_t)T * m;                                            // T x m
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = blockIdx.x;
+  const int groups = A.C / T;
+  const R* ar = static_cast<const R*>(A.ar);
+  R tr[RMAX], ti[RMAX], lc[RMAX], ur[RMAX], ui[RMAX], lt[RMAX];
+  auto keep = [&](int st) {
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int j = lane + kWave * r;
+      if (j < m) {
+        thr[st * m + j] = tr[r];
+        thi[st * m + j] = ti[r];
+      }
+    }
+  };
+  for (int t = w; t < T; t += NW) {
+    const int chain = g * T + t;
+    int8_t* sig = strips + t * npad;
+    for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
+    wave_sync_c();
+    refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
+    keep(t);
+    if (lane == 0) {
+      evs[t] = A.ctr[chain];
+      nac[t] = 0;
+      idx[t] = t;
+    }
+  }
+  __syncthreads();
+  const int total = A.n_discard + A.n_sweeps;
+  for (int sw = 0; sw < total; ++sw) {
+    for (int t = w; t < T; t += NW) {
+      const int chain = g * T + t;
+      const int st = idx[t];
+      int8_t* sig = strips + st * npad;
+      if (A.mixed && sw > 0) {
+        refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
+      } else {
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+          const int j = lane + kWave * r;
+          tr[r] = j < m ? thr[st * m + j] : (R)0;
+          ti[r] = j < m ? thi[st * m + j] : (R)0;
+          lc[r] = j < m ? lc_re<R>(tr[r], ti[r]) : (R)0;
+        }
+      }
+      uint64_t ev = evs[t];
+      const R beta = (R)((double)(T - t) / T);
+      const uint64_t acc =
+          sweep_chain_c<R, KIND, RMAX>(A, chain, sig, lane, tr, ti, lc, ur, ui, lt, ev, beta);
+      keep(st);
+      R sa = (R)0, sl = (R)0;
+      for (int i = lane; i < n; i += kWave) sa = sa + ar[i] * (R)sig[i];
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r)
+        if (lane + kWave * r < m) sl = sl + lc[r];
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) {
+        sa = sa + __shfl_xor(sa, s);
+        sl = sl + __shfl_xor(sl, s);
+      }
+      if (lane == 0) {
+        lpv[t] = sa + sl;
+        evs[t] = ev;
+        nac[t] += acc;
+      }
+      wave_sync_c();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // the two swap passes, pair by pair in index order
+      for (int pass = 0; pass < 2; ++pass)
+        for (int t = pass; t + 1 < T && (pass == 0 || t + 1 <= T - 2); t += 2) {
+          const u32x4 x = draw(A.seed, evs[t], (uint32_t)(g * T + t), kDomMix);
+          const double dbeta = (double)(T - t - 1) / T - (double)(T - t) / T;
+          const R dl = lpv[t] - lpv[t + 1];
+          const bool d = sizeof(R) == 4 ? accept_f((float)dbeta, (float)dl, u01_float(x.z))
+                                        : accept_d(dbeta, (double)dl, u01_double(x.z, x.w));
+          if (d) {
+            const int i0 = idx[t];
+            idx[t] = idx[t + 1];
+            idx[t + 1] = i0;
+            const R l0 = lpv[t];
+            lpv[t] = lpv[t + 1];
+            lpv[t + 1] = l0;
+          }
+        }
+    }
+    __syncthreads();
+    if (w == 0 && sw >= A.n_discard) {  // slot 0
+      const int st = idx[0];
+      const int8_t* sig = strips + st * npad;
+      const size_t row = (size_t)(sw - A.n_discard) * groups + g;
+      for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+      if (A.mixed) refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
+      R* ot = static_cast<R*>(A.out_theta) + 2 * row * m;
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int j = lane + kWave * r;
+        if (j < m) {
+          ot[2 * j] = A.mixed ? tr[r] : thr[st * m + j];
+          ot[2 * j + 1] = A.mixed ? ti[r] : thi[st * m + j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = w; t < T; t += NW) {
+    const int chain = g * T + t;
+    const int8_t* sig = strips + idx[t] * npad;
+    for (int i = lane; i < n; i += kWave) A.sigma[(size_t)chain * n + i] = sig[i];
+    if (lane == 0) {
+      A.ctr[chain] = evs[t];
+      A.acc[chain] += nac[t];
+    }
+  }
+}
+
+template <typename R, int KIND>
+void launch_pt_c(const SampleArgs& a, hipStream_t st) {
+  const dim3 grid(a.C / a.ntemps), block(kWave * (a.ntemps % 4 == 0 ? 4 : 2));
+  const size_t lds = pt_lds_bytes(a);
+  switch ((a.m + kWave - 1) / kWave) {
+    case 1: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 1>), grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 2>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 3>), grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 4>), grid, block, lds, st, a); break;
+    case 5: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 5>), grid, block, lds, st, a); break;
+    case 6: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 6>), grid, block, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 7>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_sample_pt_c<R, KIND, 8>), grid, block, lds, st, a); break;
   }
 }
 
@@ -514,6 +688,16 @@ void launch_cplx_planes(const double* p, int n, int m, const CplxPlanes& P, hipS
 
 void launch_sample_c(const SampleArgs& a, hipStream_t st) {
   ProfGuard g("sample_c", st);
+  if (a.ntemps > 1) {
+    if (a.mixed) {
+      if (a.kind == 1) launch_pt_c<float, 1>(a, st);
+      else launch_pt_c<float, 0>(a, st);
+    } else {
+      if (a.kind == 1) launch_pt_c<double, 1>(a, st);
+      else launch_pt_c<double, 0>(a, st);
+    }
+    return;
+  }
   if (a.mixed) {
     if (a.kind == 1) launch_rc<float, 1>(a, st);
     else launch_rc<float, 0>(a, st);

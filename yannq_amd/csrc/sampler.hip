@@ -583,7 +583,8 @@ void launch_sample_multi(const SampleArgs* inst_dev, const int2* items, int n_it
 }
 
 size_t pt_lds_bytes(const SampleArgs& a) {
-  const size_t rs = a.mixed ? sizeof(float) : sizeof(double);
+  // complex machines keep Re and Im of theta
+  const size_t rs = (a.mixed ? sizeof(float) : sizeof(double)) * (a.cplx ? 2 : 1);
   return 448 + (((size_t)a.ntemps * a.npad + 15) & ~(size_t)15) + (size_t)a.ntemps * a.m * rs;
 }
 

@@ -127,6 +127,17 @@ __global__ void k_ema_lower(double* __restrict__ avg, double* __restrict__ S, si
   }
 }
 
+// M = float(S) + shift I on the lower triangle (column-major dim x dim)
+__global__ void k_lower_shift_f(const double* __restrict__ S, float* __restrict__ M, size_t dim,
+                                double shift) {
+  const size_t total = dim * dim;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = q % dim, c = q / dim;
+    if (r >= c) M[q] = (float)(S[q] + (r == c ? shift : 0.0));
+  }
+}
+
 // S = sum_p P[p] over the lower triangle (column-major dim x dim, nc doubles
 // per entry), partial products in index order
 __global__ void k_sum_lower(const double* __restrict__ P, int np, size_t dim, int nc,
@@ -464,6 +475,12 @@ void launch_ema_lower(double* avg, double* S, size_t dim, int nc, double beta, d
   const size_t total = dim * dim;
   const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
   hipLaunchKernelGGL(k_ema_lower, dim3(grid), dim3(256), 0, st, avg, S, dim, nc, beta, corr);
+}
+
+void launch_lower_shift_f(const double* S, float* M, size_t dim, double shift, hipStream_t st) {
+  const size_t total = dim * dim;
+  const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_lower_shift_f, dim3(grid), dim3(256), 0, st, S, M, dim, shift);
 }
 
 void launch_add_diag(double* M, int dim, double shift, hipStream_t st) {

@@ -221,6 +221,153 @@ __global__ void __launch_bounds__(256, 2) k_syrk64_mfma(const SyrkArgs a) {
     }
 }
 
+// ---- split-bf16 S (mixed precision) ---------------------------------------
+// O_c values rounded to float and split x = h + m + l into three bf16 (24
+// significant bits); each product keeps the six pair terms down to 2^-24 of
+// the leading one, hh + hm + mh + hl + lh + mm, on v_mfma_f32_16x16x32_bf16
+// with fp32 accumulation (lane l holds A[row l%16][k 8(l/16) .. +7] and
+// B[k 8(l/16) .. +7][col l%16] and returns D[4(l/16) + q][l%16]); S is
+// stored in double.  Tile 128 x 128 as above, 32 samples per stage, the
+// three layers of both slabs in LDS as [layer][column][sample] so that a
+// lane's eight consecutive samples are one 16-byte read.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+constexpr int kBK3 = 32;             // samples per stage
+constexpr int kLd3 = kBK3 + 8;       // LDS row stride (bf16)
+constexpr int kPer3 = kBK3 * kT / 256;  // slab elements per thread and stage (16)
+
+__device__ inline uint16_t to_bf16(float x) {
+  uint32_t u = __builtin_bit_cast(uint32_t, x);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+template <bool RS>
+__global__ void __launch_bounds__(256, 2) k_syrk_bf16x3(const SyrkArgs a) {
+  __shared__ uint16_t sh[2][3][kT][kLd3];  // [J, I slab][hi, mid, lo][column][sample]
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int li = l & 15, lg = l >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int s0 = blockIdx.y * a.kchunk;
+  const int s1 = min(a.ns, s0 + a.kchunk);
+  // this thread stages column cl of both slabs for 16 consecutive samples
+  const int cl = tid & (kT - 1), sb = (tid >> 7) * kPer3;
+  const Col cj = make_col<false>(a, (long long)bj * kT + cl);
+  const Col ci = make_col<false>(a, (long long)bi * kT + cl);
+  double tj[kPer3], ti[kPer3], rsv[kPer3];
+  int8_t gj[kPer3], gi[kPer3];
+  int k_loaded = s0;
+  auto load = [&](int k0) {
+    k_loaded = k0;
+#pragma unroll
+    for (int p = 0; p < kPer3; ++p) {
+      const int s = min(k0 + sb + p, s1 - 1);
+      tj[p] = cj.tp[(size_t)s * a.m];
+      ti[p] = ci.tp[(size_t)s * a.m];
+      gj[p] = cj.sp[(size_t)s * a.n];
+      gi[p] = ci.sp[(size_t)s * a.n];
+      if (RS) rsv[p] = a.rs[s];
+    }
+  };
+  auto value = [&](const Col& q, double t, int8_t g, double r, bool in) {
+    double v = q.use_t ? t : 1.0;
+    v = (q.use_s && g < 0) ? -v : v;
+    v = v - q.om;
+    if (RS) v = v * r;
+    return in && q.live ? (float)v : 0.0f;
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      u16x8 hj[3], hi_[3];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int p = half * 8 + e;
+        const bool in = k_loaded + sb + p < s1;
+        const float x = value(cj, tj[p], gj[p], RS ? rsv[p] : 1.0, in);
+        const float y = value(ci, ti[p], gi[p], RS ? rsv[p] : 1.0, in);
+        const uint16_t xh = to_bf16(x), yh = to_bf16(y);
+        const float xr = x - from_bf16(xh), yr = y - from_bf16(yh);
+        const uint16_t xm = to_bf16(xr), ym = to_bf16(yr);
+        hj[0][e] = xh;
+        hj[1][e] = xm;
+        hj[2][e] = to_bf16(xr - from_bf16(xm));
+        hi_[0][e] = yh;
+        hi_[1][e] = ym;
+        hi_[2][e] = to_bf16(yr - from_bf16(ym));
+      }
+#pragma unroll
+      for (int ly = 0; ly < 3; ++ly) {
+        *reinterpret_cast<u16x8*>(&sh[0][ly][cl][sb + half * 8]) = hj[ly];
+        *reinterpret_cast<u16x8*>(&sh[1][ly][cl][sb + half * 8]) = hi_[ly];
+      }
+    }
+  };
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
+  const int stages = (s1 - s0 + kBK3 - 1) / kBK3;
+  load(s0);
+  for (int st = 0; st < stages; ++st) {
+    __syncthreads();  // the previous stage's MFMAs are done with the slabs
+    store();
+    __syncthreads();
+    if (st + 1 < stages) load(s0 + (st + 1) * kBK3);
+    bf16x8 av[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int ly = 0; ly < 3; ++ly)
+        av[r][ly] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u16x8*>(&sh[0][ly][wr * 64 + r * 16 + li][lg * 8]));
+    // each stage's 32 samples start from zero, smallest pair products first,
+    // and join the running sum through a round-to-nearest add: chaining the
+    // MFMA accumulator over every stage lets its rounding build up with the
+    // sample count (1e-5 of max|S| at 5,000 samples)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      bf16x8 bv[3];
+#pragma unroll
+      for (int ly = 0; ly < 3; ++ly)
+        bv[ly] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u16x8*>(&sh[1][ly][wc * 64 + c * 16 + li][lg * 8]));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        floatx4 t = (floatx4){0.f, 0.f, 0.f, 0.f};
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r][0], bv[2], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r][2], bv[0], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r][1], bv[1], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r][0], bv[1], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r][1], bv[0], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r][0], bv[0], t, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[r][c][q] = acc[r][c][q] + t[q];
+      }
+    }
+  }
+  // D[4 lg + q][li] = sum_s O_c[s][j] O_c[s][i] with j the row (A = J slab):
+  // for fixed q the 16 lanes of a group hold 16 consecutive i of one j
+  const long long dim = a.dim;
+  double* out = a.S + (size_t)blockIdx.y * a.pstride;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const long long i = (long long)bi * kT + wc * 64 + c * 16 + li;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long long j = (long long)bj * kT + wr * 64 + r * 16 + 4 * lg + q;
+        if (i < dim && j <= i) out[(size_t)j * dim + i] = a.alpha * (double)acc[r][c][q];
+      }
+    }
+}
+
 SyrkArgs base_args(int ns, size_t dim, double alpha, double* S, int chunks, double* part) {
   SyrkArgs a{};
   a.ns = ns;
@@ -273,6 +420,29 @@ int launch_syrk64_factored(const int8_t* sigma, const double* T, const double* o
   a.m = m;
   a.use_bias = use_bias;
   launch(a, chunks, S, st);
+  return YQ_OK;
+}
+
+int launch_syrk_bf16x3(const int8_t* sigma, const double* T, const double* omean,
+                       const double* rs, int ns, int n, int m, int use_bias, double alpha,
+                       double* S, int chunks, double* part, hipStream_t st) {
+  const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
+  if (ns < 1 || dim < 1 || chunks < 1 || (chunks > 1 && !part)) return YQ_ERR_ARG;
+  if (rs) return YQ_ERR_UNSUPPORTED;  // weighted (exact) batches use the FP64 SYRK
+  SyrkArgs a = base_args(ns, dim, alpha, S, chunks, part);
+  a.sigma = sigma;
+  a.T = T;
+  a.omean = omean;
+  a.n = n;
+  a.m = m;
+  a.use_bias = use_bias;
+  const long long tiles = (long long)a.nb * (a.nb + 1) / 2;
+  {
+    ProfGuard g("syrk_bf16x3", st);
+    const dim3 grid((unsigned)tiles, (unsigned)chunks), block(256);
+    hipLaunchKernelGGL((k_syrk_bf16x3<false>), grid, block, 0, st, a);
+  }
+  if (chunks > 1) launch_sum_lower(a.S, chunks, dim, 1, S, st);
   return YQ_OK;
 }
 

@@ -75,6 +75,7 @@ struct SampleArgs {
   // k_sample reads it instead of summing W itself; null: k_sample sums.
   void* theta_buf;
   const void* theta_in;  // set by launch_sample per launch
+  int cplx;              // complex machine (Wr / Wi planes, interleaved theta)
 };
 void launch_sample(const SampleArgs& a, hipStream_t st);
 // LDS of the parallel-tempering launch (spin strips, exchange areas)
@@ -225,6 +226,8 @@ void launch_center_force_c(double* Oh, const double* omean_h, const double* e, i
 // S = sum of np column-major dim x dim partial products, lower triangle
 // (nc = 2: complex entries as pairs)
 void launch_sum_lower(const double* P, int np, size_t dim, int nc, double* S, hipStream_t st);
+// lower triangle: M = float(S) + shift I
+void launch_lower_shift_f(const double* S, float* M, size_t dim, double shift, hipStream_t st);
 // lower triangle: avg = beta avg + (1 - beta) S, S = avg / corr
 void launch_ema_lower(double* avg, double* S, size_t dim, int nc, double beta, double corr,
                       hipStream_t st);
@@ -240,6 +243,12 @@ int syrk64_chunks(int ns, size_t dim, size_t part_cap_doubles);
 int launch_syrk64_factored(const int8_t* sigma, const double* T, const double* omean,
                            const double* rs, int ns, int n, int m, int use_bias, double alpha,
                            double* S, int chunks, double* part, hipStream_t st);
+// The same S in split-bf16 (mixed precision): O_c rounded to float, three
+// bf16 pieces, six pair products per sample on bf16 MFMA with fp32
+// accumulation, stored in double.  rs must be null (YQ_ERR_UNSUPPORTED).
+int launch_syrk_bf16x3(const int8_t* sigma, const double* T, const double* omean,
+                       const double* rs, int ns, int n, int m, int use_bias, double alpha,
+                       double* S, int chunks, double* part, hipStream_t st);
 int launch_syrk64_dense(const double* O, int ns, size_t dim, double alpha, double* S, int chunks,
                         double* part, hipStream_t st);
 // ---------------------------------------------------------------- exact sums
