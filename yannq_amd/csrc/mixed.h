@@ -18,8 +18,8 @@ struct MixedWork {
   int ncg = 0;   // column groups of the forward product (Np/16 over ct)
   size_t dim = 0;
   // operands prepared once per SR build
-  uint16_t* sigA = nullptr;  // bf16 sigma' row-major  nsp x Kp
-  uint16_t* sigT = nullptr;  // bf16 sigma'^T          Kp x nsp
+  uint8_t* sigA = nullptr;   // sigma' bits row-major  nsp x 32 bytes (set: -1)
+  uint8_t* sigT = nullptr;   // sigma'^T bits          Kp x nsp / 8 bytes
   float* Trm = nullptr;      // T' row-major           nsp x Np
   float* Tt = nullptr;       // T'^T                   Np x nsp
   // per-product scratch

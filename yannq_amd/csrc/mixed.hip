@@ -38,7 +38,6 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 constexpr int kBwdChunkMin = 128;  // smallest backward chunk (sizes the partials)
 constexpr int kRowPad = 512;    // ns padding
 constexpr int kMaxKs = 5;       // Kp <= 160 (n <= 159)
-constexpr int kMaxIt = 10;      // Kp / 16
 constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
 
 // device control block of the CG loop: ints [stop flag, completed iterations,
@@ -65,12 +64,19 @@ __device__ __host__ inline float bf2f(uint16_t h) {
   memcpy(&x, &u, 4);
   return x;
 }
+// x = hi + mid + lo exactly: each piece takes the next 8 significant bits
+// (truncation: the top 16 bits of a float are a bf16, and each remainder is
+// exact -- x has 24 significant bits, hi, mid and lo 8 each).  Five VALU
+// operations a value, against some sixteen for round-to-nearest pieces,
+// which would be no more exact.
 __device__ inline void split3(float x, uint16_t& hi, uint16_t& mid, uint16_t& lo) {
-  hi = f2bf(x);
-  const float r1 = x - bf2f(hi);
-  mid = f2bf(r1);
-  const float r2 = r1 - bf2f(mid);
-  lo = f2bf(r2);
+  const uint32_t u = __float_as_uint(x);
+  const float r1 = x - __uint_as_float(u & 0xFFFF0000u);
+  const uint32_t u1 = __float_as_uint(r1);
+  const float r2 = r1 - __uint_as_float(u1 & 0xFFFF0000u);
+  hi = (uint16_t)(u >> 16);
+  mid = (uint16_t)(u1 >> 16);
+  lo = (uint16_t)(__float_as_uint(r2) >> 16);
 }
 __device__ inline bf16x8 ld8(const uint16_t* p) {
   const u16x8 v = *reinterpret_cast<const u16x8*>(p);
@@ -125,6 +131,35 @@ __device__ inline double psum(const double* __restrict__ part, int cnt) {
   for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w);
   return s;
 }
+// psum in two parts, so that other loads can be in flight with the partials:
+// psum_issue loads this lane's first 8 (partials l, l + 64, ..., l + 448),
+// psum_finish adds them and the rest in psum's order
+// (cnt >= 1; the loads are unconditional at clamped indices, the unused ones
+// are left out of the sum)
+__device__ inline void psum_issue(const double* __restrict__ part, int cnt, double (&v)[8]) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = part[min(l + 64 * q, cnt - 1)];
+}
+__device__ inline double psum_finish(const double* __restrict__ part, int cnt,
+                                     const double (&v)[8]) {
+  const int l = threadIdx.x & 63;
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (l + 64 * q < cnt) s += v[q];
+  for (int i0 = l + 512; i0 < cnt; i0 += 512) {
+    double w[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w[q] = i0 + 64 * q < cnt ? part[i0 + 64 * q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (i0 + 64 * q < cnt) s += w[q];
+  }
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w);
+  return s;
+}
 // three totals at once, each in the order of psum
 __device__ inline void psum3(const double* __restrict__ a, int na, const double* __restrict__ b,
                              int nb, const double* __restrict__ c, int nc, double& sa, double& sb,
@@ -168,20 +203,54 @@ __device__ inline size_t aug_index(size_t c, int n, int m, int Np) {
   return (size_t)n * Np + (c - nm - n);
 }
 
-// sigma' (bf16) row-major and transposed; zero outside ns x (n+1)
-__global__ void k_prep_sig(const int8_t* __restrict__ sigma, int ns, int n, int Kp, int nsp,
-                           uint16_t* __restrict__ A, uint16_t* __restrict__ AT) {
+// sigma' as bits, a set bit standing for -1 and a clear one for +1: row-major
+// (nsp rows of kSigRow bytes; bit k of row s is bit k % 8 of byte k / 8) and
+// transposed (Kp rows of nsp / 8 bytes; bit s of row k is bit s % 8 of byte
+// s / 8).  Column n (the bias) is +1, and so are the padding columns k > n
+// and the padding rows s >= ns: the rows k > n of V' are zero and those of G'
+// are never read, and a padded sample has T' = 0 and y = 0, so their signs do
+// not matter.  A kernel reads 1 bit per spin instead of a bf16 and expands
+// the bits into bf16 +-1 in registers (expand8).
+constexpr int kSigRow = 32;  // bytes per row-major bit row (Kp <= 256)
+__global__ void k_prep_sig_rows(const int8_t* __restrict__ sigma, int ns, int n, int nsp,
+                                uint8_t* __restrict__ A) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (size_t)nsp * Kp) return;
-  const int s = (int)(q / Kp), i = (int)(q % Kp);
-  float v = 0.0f;
-  if (s < ns) {
-    if (i < n) v = sigma[(size_t)s * n + i] > 0 ? 1.0f : -1.0f;
-    else if (i == n) v = 1.0f;
+  if (q >= (size_t)nsp * kSigRow) return;
+  const int s = (int)(q / kSigRow), b = (int)(q % kSigRow);
+  unsigned v = 0;
+  if (s < ns)
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * b + e;
+      if (k < n && sigma[(size_t)s * n + k] < 0) v |= 1u << e;
+    }
+  A[q] = (uint8_t)v;
+}
+__global__ void k_prep_sig_cols(const int8_t* __restrict__ sigma, int ns, int n, int Kp, int nsp,
+                                uint8_t* __restrict__ T) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rb = nsp / 8;
+  if (q >= (size_t)Kp * rb) return;
+  const int k = (int)(q / rb), sb = (int)(q % rb);
+  unsigned v = 0;
+  if (k < n)
+    for (int e = 0; e < 8; ++e) {
+      const int s = 8 * sb + e;
+      if (s < ns && sigma[(size_t)s * n + k] < 0) v |= 1u << e;
+    }
+  T[q] = (uint8_t)v;
+}
+// 8 spins (bits 0..7 of b, set = -1) as 8 bf16 (+1 = 0x3F80, -1 = 0xBF80):
+// bf16 pair p of the result takes bits 2p (low half) and 2p + 1 (high half)
+// into the sign positions 15 and 31
+__device__ inline u16x8 expand8(uint32_t b) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 w;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const uint32_t x = b >> (2 * p);
+    w[p] = 0x3F803F80u | ((x & 1u) << 15) | ((x & 2u) << 30);
   }
-  const uint16_t h = f2bf(v);
-  A[q] = h;
-  AT[(size_t)i * nsp + s] = h;
+  return __builtin_bit_cast(u16x8, w);
 }
 
 // T' row-major (nsp x Np) and transposed (Np x nsp)
@@ -214,17 +283,13 @@ __global__ void k_prep_T(const float* __restrict__ T, int ns, int m, int Np, int
 constexpr int kLdsPad = 8;
 // (bx, by): the block's place in the machine's own (nsp / 256) x ncg grid
 template <int CT, int NKS>
-__device__ __forceinline__ void fwd_body(int bx, int by, const uint16_t* __restrict__ sigA,
+__device__ __forceinline__ void fwd_body(int bx, int by, const uint8_t* __restrict__ sigA,
                                          const uint16_t* __restrict__ vs,
                                          const float* __restrict__ Trm, int Kp, int Np, int nsp,
                                          float* __restrict__ ypart, int* __restrict__ ictl,
                                          int count) {
   extern __shared__ uint16_t bsh[];  // [3][CT*16][Kp + kLdsPad]
-  if (ictl[kCtlStop]) return;
   const int tid = threadIdx.x;
-  // count: this product belongs to a CG iteration whose update ran (one
-  // writer; the next update reads the counter)
-  if (count && bx == 0 && by == 0 && tid == 0) ictl[kCtlIters] += 1;
   const int cg = by;
   const int col0 = cg * CT * 16;
   const int ncols = min(CT * 16, Np - col0);
@@ -233,49 +298,56 @@ __device__ __forceinline__ void fwd_body(int bx, int by, const uint16_t* __restr
   const int wave = tid >> 6, l = tid & 63;
   const int li = l & 15, lg = l >> 4;
   const int rbase = bx * 256 + wave * 64;
-  // The V' slab goes through registers into LDS in two passes (at most 16
-  // 16-byte pieces per thread in all, the LDS budget of the slab), and this
-  // wave's sigma' fragments are issued between the first pass's loads and its
-  // LDS stores.
+  // The V' slab goes through registers into LDS (at most 16 16-byte pieces
+  // per thread, the LDS budget of the slab).  All of the block's loads are
+  // independent of one another: the slab, this wave's sigma' fragments and
+  // the T' of the first column tiles are issued together before the first
+  // wait (every block of a launch is resident at once, so the registers this
+  // takes cost no concurrency).
   constexpr int kStage = (3 * CT * 16 * 4 * NKS + 255) / 256 < 16
                              ? (3 * CT * 16 * 4 * NKS + 255) / 256 : 16;
-  constexpr int kPass = (kStage + 1) / 2;
   const int total = 3 * ncols * kchunks;
-  u16x8 stg[kPass];
-  auto stage_load = [&](int pass) {
-#pragma unroll
-    for (int i = 0; i < kPass; ++i) {
-      const int q = tid + 256 * (pass * kPass + i);
-      if (q < total) {
-        const int sp = q / (ncols * kchunks);
-        const int rem = q % (ncols * kchunks);
-        const int col = rem / kchunks, kk = (rem % kchunks) * 8;
-        stg[i] = *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp + kk);
-      }
+  // piece q = tid + 256 i is (layer sp, column col, k chunk kc8) with
+  // q = (sp * ncols + col) * kchunks + kc8; the decomposition advances by 256
+  // per piece with one division at the start
+  const int d_col = 256 / kchunks, d_kc = 256 % kchunks;
+  int sp0 = tid / (ncols * kchunks);
+  int col0_ = (tid / kchunks) % ncols, kc0 = tid % kchunks;
+  auto advance = [&](int& sp, int& col, int& kc8) {
+    kc8 += d_kc;
+    col += d_col;
+    if (kc8 >= kchunks) {
+      kc8 -= kchunks;
+      ++col;
+    }
+    while (col >= ncols) {
+      col -= ncols;
+      ++sp;
     }
   };
-  auto stage_store = [&](int pass) {
+  u16x8 stg[kStage];
+  {
+    int sp = sp0, col = col0_, kc8 = kc0;
 #pragma unroll
-    for (int i = 0; i < kPass; ++i) {
-      const int q = tid + 256 * (pass * kPass + i);
-      if (q < total) {
-        const int sp = q / (ncols * kchunks);
-        const int rem = q % (ncols * kchunks);
-        const int col = rem / kchunks, kk = (rem % kchunks) * 8;
-        *reinterpret_cast<u16x8*>(bsh + (sp * CT * 16 + col) * ldk + kk) = stg[i];
-      }
+    for (int i = 0; i < kStage; ++i) {
+      if (tid + 256 * i < total)
+        stg[i] = *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp +
+                                                 kc8 * 8);
+      advance(sp, col, kc8);
     }
-  };
-  stage_load(0);
-  bf16x8 sfr[4][NKS];
+  }
+  // this lane's spin row of each sample tile: the k-step ks fragment is
+  // byte 4 ks + lg of the row (bits of k = 32 ks + 8 lg .. + 7), i.e. byte
+  // lg of dword ks
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  constexpr int kSegs = (NKS + 3) / 4;  // 16-byte pieces of the row in use
+  u32x4 sbits[4][kSegs];
 #pragma unroll
   for (int st = 0; st < 4; ++st)
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks)
-      sfr[st][ks] = ld8(sigA + (size_t)(rbase + st * 16 + li) * Kp + ks * 32 + lg * 8);
-  stage_store(0);
-  stage_load(1);
-  stage_store(1);
+    for (int h = 0; h < kSegs; ++h)
+      sbits[st][h] = *reinterpret_cast<const u32x4*>(
+          sigA + (size_t)(rbase + st * 16 + li) * kSigRow + 16 * h);
   constexpr int PF = CT < 4 ? CT : 4;  // column tiles whose T' is in flight together
   const int nct = ncols / 16;
   floatx4 t[PF][4];
@@ -291,7 +363,31 @@ __device__ __forceinline__ void fwd_body(int bx, int by, const uint16_t* __restr
       }
   };
   load_t(0);
+  // The stop flag is tested once the data loads are in flight (the buffers
+  // of a stopped run stay valid, so the loads are harmless); the empty asm
+  // keeps the compiler from sinking the loads below the test.
+  asm volatile("" ::: "memory");
+  if (ictl[kCtlStop]) return;
+  // count: this product belongs to a CG iteration whose update ran (one
+  // writer; the next update reads the counter)
+  if (count && bx == 0 && by == 0 && tid == 0) ictl[kCtlIters] += 1;
+  {
+    int sp = sp0, col = col0_, kc8 = kc0;
+#pragma unroll
+    for (int i = 0; i < kStage; ++i) {
+      if (tid + 256 * i < total)
+        *reinterpret_cast<u16x8*>(bsh + (sp * CT * 16 + col) * ldk + kc8 * 8) = stg[i];
+      advance(sp, col, kc8);
+    }
+  }
   __syncthreads();
+  bf16x8 sfr[4][NKS];
+#pragma unroll
+  for (int st = 0; st < 4; ++st)
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+      sfr[st][ks] = __builtin_bit_cast(
+          bf16x8, expand8((sbits[st][ks >> 2][ks & 3] >> (8 * lg)) & 0xFFu));
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
   for (int cb = 0; cb < nct; cb += PF) {
     if (cb > 0) load_t(cb);
@@ -302,16 +398,23 @@ __device__ __forceinline__ void fwd_body(int bx, int by, const uint16_t* __restr
         floatx4 c[4];
 #pragma unroll
         for (int st = 0; st < 4; ++st) c[st] = (floatx4){0.f, 0.f, 0.f, 0.f};
+        // the column tile's V' operands are read from LDS together, then used
+        // (read next to their products, every read would wait on its own)
+        bf16x8 vv[3][NKS];
 #pragma unroll
         for (int sp = 0; sp < 3; ++sp)
 #pragma unroll
-          for (int ks = 0; ks < NKS; ++ks) {
-            const bf16x8 v =
-                ld8(bsh + (sp * CT * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
+          for (int ks = 0; ks < NKS; ++ks)
+            vv[sp][ks] = ld8(bsh + (sp * CT * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp)
+#pragma unroll
+          for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
             for (int st = 0; st < 4; ++st)
-              c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v, sfr[st][ks], c[st], 0, 0, 0);
-          }
+              c[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv[sp][ks], sfr[st][ks], c[st], 0,
+                                                              0, 0);
         // c[st][e] = U'^T[col0 + ct*16 + 4*lg + e][rbase + st*16 + li]
 #pragma unroll
         for (int st = 0; st < 4; ++st)
@@ -330,7 +433,7 @@ __device__ __forceinline__ void fwd_body(int bx, int by, const uint16_t* __restr
 }
 template <int CT, int NKS>
 __global__ void __launch_bounds__(256)
-    k_fwd_mfma(const uint16_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
+    k_fwd_mfma(const uint8_t* __restrict__ sigA, const uint16_t* __restrict__ vs,
                const float* __restrict__ Trm, int Kp, int Np, int nsp,
                float* __restrict__ ypart, int* __restrict__ ictl, int count) {
   fwd_body<CT, NKS>(blockIdx.x, blockIdx.y, sigA, vs, Trm, Kp, Np, nsp, ypart, ictl, count);
@@ -375,16 +478,18 @@ __device__ inline int xcd_block(int b, int total) {
 
 // (b, gx, total): the block's linear index in the machine's own
 // ((Np / 16 + 3) / 4) x chunks grid, the grid's x extent and its size
-template <int CHUNK, int SLICE>
+// NIT = Kp / 16 row tiles (a template parameter, so that the row-tile loop
+// unrolls without branches and the LDS reads of its A operands are issued
+// together)
+template <int CHUNK, int SLICE, int NIT>
 __device__ __forceinline__ void bwd_body(int b, int gx, int total,
-                                         const uint16_t* __restrict__ sigT,
+                                         const uint8_t* __restrict__ sigT,
                                          const float* __restrict__ Tt,
                                          const float* __restrict__ ypart, int ncg, int ns,
                                          const double* __restrict__ op, int nop, int Kp, int Np,
                                          int nsp, float* __restrict__ part,
                                          double* __restrict__ ys, double* __restrict__ yy,
                                          const int* __restrict__ ictl) {
-  if (ictl[kCtlStop]) return;
   extern __shared__ uint16_t ash[];  // [Kp][SLICE + kLdsPad]
   __shared__ float ysh[CHUNK];
   const int tid = threadIdx.x;
@@ -394,15 +499,74 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
   const int ct = cx * 4 + wave;
   const bool active = ct < Np / 16;
   const int li = l & 15, lg = l >> 4;
-  const int nit = Kp / 16;
   const int lda = SLICE + kLdsPad;
+  const float* trow = Tt + (size_t)(active ? ct * 16 + li : 0) * nsp;
+  // Software pipeline over the slices: the sigma'^T bits (32 samples of one
+  // row per word) and this lane's T' values of the next kDepth slices are in
+  // flight while the products of a slice run.  The bits are expanded to bf16
+  // on their way into LDS (Kp <= 160 bounds the words per thread).
+  constexpr int kWpr = SLICE / 32;  // words of a row per slice
+  constexpr int kSlab = (32 * kMaxKs * kWpr + 255) / 256;
+  constexpr int kKs = SLICE / 32;  // k-steps of 32 samples per slice
+  constexpr int kHalves = CHUNK / SLICE;
+  constexpr int kDepth = kHalves < 2 ? kHalves : 2;  // slices in flight
+  const int nslab = Kp * kWpr;
+  const int rbytes = nsp / 8;  // bytes of a transposed bit row
+  uint32_t sg[kDepth][kSlab];
+  floatx4 tv[kDepth][2 * kKs];  // T'[ct*16 + li][sbase + ks*32 + lg*8 + 0..7]
+  // (slot: a compile-time index once the slice loop is unrolled)
+  auto load_slice = [&](int half, int slot) {
+    const int sbase = kc * CHUNK + half * SLICE;
+#pragma unroll
+    for (int i = 0; i < kSlab; ++i) {
+      const int q = tid + 256 * i;
+      if (q < nslab) {
+        const int row = q / kWpr, wd = q % kWpr;
+        sg[slot][i] = *reinterpret_cast<const uint32_t*>(sigT + (size_t)row * rbytes +
+                                                         sbase / 8 + 4 * wd);
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < kKs; ++ks) {  // (an inactive wave reads row 0: in bounds, unused)
+      const float* tp = trow + sbase + ks * 32 + lg * 8;
+      tv[slot][2 * ks] = *reinterpret_cast<const floatx4*>(tp);
+      tv[slot][2 * ks + 1] = *reinterpret_cast<const floatx4*>(tp + 4);
+    }
+  };
+  // Every independent load is issued before the first wait: the first
+  // kDepth slices, the first 8 column groups' forward partials of this
+  // thread's samples and the <omean, u> partials; then the stop flag.
+#pragma unroll
+  for (int h = 0; h < kDepth; ++h) load_slice(h, h);
+  constexpr int kYr = (CHUNK + 255) / 256;  // samples of the chunk per thread
+  // (clamped indices instead of conditions: a conditional load whose value
+  // is selected at once makes the wave wait for it)
+  float yl[kYr][8];
+#pragma unroll
+  for (int r = 0; r < kYr; ++r) {
+    const int q = min(tid + 256 * r, CHUNK - 1);
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      yl[r][e] = ypart[(size_t)min(e, ncg - 1) * nsp + kc * CHUNK + q];
+  }
+  double opv[8];
+  psum_issue(op, nop, opv);
+  // the stop flag is tested after the y values (a test before them would let
+  // the compiler move these loads behind it)
+  const int stop = ictl[kCtlStop];
   {
-    const float ov = (float)psum(op, nop);
+    const float ov = (float)psum_finish(op, nop, opv);
     double dv = 0.0, dv2 = 0.0;
-    for (int q = tid; q < CHUNK; q += 256) {
+#pragma unroll
+    for (int r = 0; r < kYr; ++r) {
+      const int q = tid + 256 * r;
+      if (q >= CHUNK) break;
       const int s = kc * CHUNK + q;
       float v = 0.0f;
-      for (int g0 = 0; g0 < ncg; g0 += 8) {  // loads in flight together, sums in order
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e < ncg) v += yl[r][e];
+      for (int g0 = 8; g0 < ncg; g0 += 8) {  // loads in flight together, sums in order
         float y8[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) y8[e] = g0 + e < ncg ? ypart[(size_t)(g0 + e) * nsp + s] : 0.0f;
@@ -415,65 +579,48 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
       dv += (double)yv;
       dv2 += (double)yv * (double)yv;
     }
+    if (stop) return;  // uniform per launch, before any barrier or global store
     if (cx == 0) {  // uniform per block
       const double v2[2] = {dv, dv2};
       double* const o2[2] = {ys, yy};
       block_sums<2>(v2, o2, kc, 4);
     }
   }
-  floatx4 acc[kMaxIt];
+  floatx4 acc[NIT];
 #pragma unroll
-  for (int it = 0; it < kMaxIt; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
-  const float* trow = Tt + (size_t)(active ? ct * 16 + li : 0) * nsp;
-  // Software pipeline over the slices: the sigma'^T slab of slice h + 1 (into
-  // registers) and this lane's T' values of slice h + 1 are loaded while the
-  // products of slice h run (Kp <= 160 bounds the slab pieces per thread).
-  constexpr int kSlab = (32 * kMaxKs * (SLICE / 8) + 255) / 256;
-  constexpr int kKs = SLICE / 32;  // k-steps of 32 samples per slice
-  const int nslab = Kp * (SLICE / 8);
-  u16x8 sg[kSlab];
-  floatx4 tv[2 * kKs] = {};  // T'[ct*16 + li][sbase + ks*32 + lg*8 + 0..7]
-  auto load_slice = [&](int half) {
-    const int sbase = kc * CHUNK + half * SLICE;
+  for (int it = 0; it < NIT; ++it) acc[it] = (floatx4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < kSlab; ++i) {
-      const int q = tid + 256 * i;
-      if (q < nslab) {
-        const int row = q / (SLICE / 8), kk = (q % (SLICE / 8)) * 8;
-        sg[i] = *reinterpret_cast<const u16x8*>(sigT + (size_t)row * nsp + sbase + kk);
-      }
-    }
-    if (active) {
-#pragma unroll
-      for (int ks = 0; ks < kKs; ++ks) {
-        const float* tp = trow + sbase + ks * 32 + lg * 8;
-        tv[2 * ks] = *reinterpret_cast<const floatx4*>(tp);
-        tv[2 * ks + 1] = *reinterpret_cast<const floatx4*>(tp + 4);
-      }
-    }
-  };
-  load_slice(0);
-  constexpr int kHalves = CHUNK / SLICE;
   for (int half = 0; half < kHalves; ++half) {
     const int sbase = kc * CHUNK + half * SLICE;
+    const int slot = half % kDepth;
     __syncthreads();  // the previous slice's products are done with the LDS
 #pragma unroll
     for (int i = 0; i < kSlab; ++i) {
       const int q = tid + 256 * i;
       if (q < nslab) {
-        const int row = q / (SLICE / 8), kk = (q % (SLICE / 8)) * 8;
-        *reinterpret_cast<u16x8*>(ash + row * lda + kk) = sg[i];
+        const int row = q / kWpr, wd = q % kWpr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          *reinterpret_cast<u16x8*>(ash + row * lda + 32 * wd + 8 * j) =
+              expand8((sg[slot][i] >> (8 * j)) & 0xFFu);
       }
     }
     floatx4 tc[2 * kKs];
 #pragma unroll
-    for (int i = 0; i < 2 * kKs; ++i) tc[i] = tv[i];
-    if (half + 1 < kHalves) load_slice(half + 1);
+    for (int i = 0; i < 2 * kKs; ++i) tc[i] = tv[slot][i];
+    if (half + kDepth < kHalves) load_slice(half + kDepth, slot);
     __syncthreads();
     if (!active) continue;
 #pragma unroll
     for (int ks = 0; ks < kKs; ++ks) {
       const int s0 = sbase + ks * 32 + lg * 8;
+      // the k-step's A operands are read from LDS first, so that their
+      // latency passes under the split below (read one at a time next to
+      // their products, each would wait on its own)
+      bf16x8 av[NIT];
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) av[it] = ld8(ash + (it * 16 + li) * lda + ks * 32 + lg * 8);
+      __builtin_amdgcn_sched_barrier(0);  // (the scheduler would sink them again)
       u16x8 bh, bm, bl;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -488,31 +635,28 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
       const bf16x8 b1 = __builtin_bit_cast(bf16x8, bm);
       const bf16x8 b2 = __builtin_bit_cast(bf16x8, bl);
 #pragma unroll
-      for (int it = 0; it < kMaxIt; ++it)
-        if (it < nit) {
-          const bf16x8 a = ld8(ash + (it * 16 + li) * lda + ks * 32 + lg * 8);
-          acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[it], 0, 0, 0);
-          acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[it], 0, 0, 0);
-          acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2, acc[it], 0, 0, 0);
-        }
+      for (int it = 0; it < NIT; ++it) {
+        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[it], b0, acc[it], 0, 0, 0);
+        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[it], b1, acc[it], 0, 0, 0);
+        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[it], b2, acc[it], 0, 0, 0);
+      }
     }
   }
   if (!active) return;
   float* out = part + (size_t)kc * Kp * Np;
 #pragma unroll
-  for (int it = 0; it < kMaxIt; ++it)
-    if (it < nit)
+  for (int it = 0; it < NIT; ++it)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
 }
-template <int CHUNK, int SLICE>
+template <int CHUNK, int SLICE, int NIT>
 __global__ void __launch_bounds__(256)
-    k_bwd_mfma(const uint16_t* __restrict__ sigT, const float* __restrict__ Tt,
+    k_bwd_mfma(const uint8_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
                int nop, int Kp, int Np, int nsp, float* __restrict__ part,
                double* __restrict__ ys, double* __restrict__ yy, const int* __restrict__ ictl) {
-  bwd_body<CHUNK, SLICE>(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.x * gridDim.y,
+  bwd_body<CHUNK, SLICE, NIT>(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.x * gridDim.y,
                          sigT, Tt, ypart, ncg, ns, op, nop, Kp, Np, nsp, part, ys, yy, ictl);
 }
 
@@ -520,9 +664,20 @@ __global__ void __launch_bounds__(256)
 // with fewer latency steps but less overlap, measured slower at n = 100,
 // m = 400, 16384 samples)
 constexpr int kBwdSlice = 64;
-auto bwd_kernel(int chunk) {
-  return chunk == 128 ? k_bwd_mfma<128, kBwdSlice>
-                      : chunk == 512 ? k_bwd_mfma<512, kBwdSlice> : k_bwd_mfma<256, kBwdSlice>;
+template <int CHUNK>
+auto bwd_kernel_it(int nit) {
+  switch (nit) {
+    case 2: return k_bwd_mfma<CHUNK, kBwdSlice, 2>;
+    case 4: return k_bwd_mfma<CHUNK, kBwdSlice, 4>;
+    case 6: return k_bwd_mfma<CHUNK, kBwdSlice, 6>;
+    case 8: return k_bwd_mfma<CHUNK, kBwdSlice, 8>;
+    default: return k_bwd_mfma<CHUNK, kBwdSlice, 10>;
+  }
+}
+// (Kp is a multiple of 32 up to 160: nit = Kp / 16 is 2, 4, 6, 8 or 10)
+auto bwd_kernel(int chunk, int nit) {
+  return chunk == 128 ? bwd_kernel_it<128>(nit)
+                      : chunk == 512 ? bwd_kernel_it<512>(nit) : bwd_kernel_it<256>(nit);
 }
 
 // One Chronopoulos-Gear CG step for (S + shift) x = b with preconditioner M
@@ -578,7 +733,39 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
   if (k < n && j < m) c = (long)k * m + j;
   else if (use_bias && k < n && j == m) c = (long)(nm + k);
   else if (use_bias && k == n && j < m) c = (long)(nm + n + j);
+  // the control block, read before anything waits (only block 0 writes it,
+  // into slots that no block reads in this launch)
   const double shift = dctl[kCtlShift];
+  const double thr = dctl[kCtlThr];
+  const double gam_prev = dctl[kCtlGamma + (par ^ 1)];
+  const double alpha_prev = dctl[kCtlAlpha + (par ^ 1)];
+  const int stop0 = ictl[kCtlStop];
+  const int iters = ictl[kCtlIters];
+  const int max_it = ictl[kCtlMaxIt];
+  // Every independent load is issued before the first wait: this thread's
+  // chunk partials (up to 16, G == 4) and, in group 0, its vector elements;
+  // the values are used in the order of the code below.
+  constexpr int kPre = 16;
+  float pre[kPre];
+  const size_t pstride = (size_t)Kp * Np;
+  const size_t paug = (size_t)k * Np + j;  // aug(c)
+  if (mode != 0 && c >= 0 && G == 4) {  // clamped chunk index: unconditional loads
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) pre[q] = part[(size_t)min(g + 4 * q, chunks - 1) * pstride + paug];
+  }
+  float r_c = 0.0f, u_c = 0.0f, p_c = 0.0f, s_c = 0.0f, x_c = 0.0f, om_c = 0.0f, di_c = 1.0f;
+  if (g == 0 && c >= 0) {
+    r_c = r[c];
+    u_c = u[c];
+    om_c = om[c];
+    if (mode == 2) {
+      p_c = p[c];
+      s_c = s[c];
+    }
+    if (mode == 1 || mode == 2) x_c = x[c];
+    if (dinv) di_c = dinv[c];
+  }
+  asm volatile("" ::: "memory");  // the loads stay above the first wait
   float alpha = 0.0f, beta = 0.0f;
   // The grid totals of the partials (<r, r>, <r, u>, <u, u>, sum y^2, sum y)
   // are summed by wave 0 alone and shared through LDS: every wave would get
@@ -607,9 +794,9 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
     // spent; the later kernels of the iteration see the flag and return at
     // once
     const bool fin = __builtin_isfinite(rn2);
-    const bool conv = fin && rn2 < dctl[kCtlThr];
-    if (ictl[kCtlStop] || conv || !fin || ictl[kCtlIters] >= ictl[kCtlMaxIt]) {
-      if (lb == 0 && t == 0 && !ictl[kCtlStop]) {
+    const bool conv = fin && rn2 < thr;
+    if (stop0 || conv || !fin || iters >= max_it) {
+      if (lb == 0 && t == 0 && !stop0) {
         ictl[kCtlStop] = 1;
         ictl[kCtlConv] = conv ? kConvYes : fin ? 0 : kConvNonFinite;
       }
@@ -620,8 +807,8 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
     if (mode == 1) {
       a = gam / del;
     } else {
-      bt = gam / dctl[kCtlGamma + (par ^ 1)];
-      a = gam / (del - bt * gam / dctl[kCtlAlpha + (par ^ 1)]);
+      bt = gam / gam_prev;
+      a = gam / (del - bt * gam / alpha_prev);
     }
     if (!__builtin_isfinite(a)) a = 0.0;  // breakdown (delta or its correction 0): no step
     alpha = (float)a;
@@ -639,10 +826,15 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
     // (up to 8 chunks of each class in flight), so the value does not depend
     // on G.
     if (c >= 0 && G == 4) {  // one class per group
-      const size_t a = (size_t)k * Np + j;  // aug(c)
-      const size_t stride = (size_t)Kp * Np;
-      float acc = 0.0f;
-      for (int q0 = g; q0 < chunks; q0 += 32) {
+      const size_t a = paug;
+      const size_t stride = pstride;
+      // (starts from the first partial rather than 0 + it, which would make
+      // the compiler wait for that load where it is issued)
+      float acc = pre[0];
+#pragma unroll
+      for (int q = 1; q < kPre; ++q)
+        if (g + 4 * q < chunks) acc += pre[q];
+      for (int q0 = g + 4 * kPre; q0 < chunks; q0 += 32) {
         float v[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q)
@@ -679,7 +871,7 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
     __syncthreads();
     if (g == 0 && c >= 0) {
       const float gs = (grp[0][e] + grp[1][e]) + (grp[2][e] + grp[3][e]);
-      wv = (gs - om[c] * sy) + (float)shift * u[c];
+      wv = (gs - om_c * sy) + (float)shift * u_c;
     }
     if (mode == 3) {
       if (g == 0 && c >= 0) out[c] = wv;
@@ -690,20 +882,20 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
   if (g == 0) {
     float v = 0.0f;
     if (c >= 0) {
-      float rv = r[c];
+      float rv = r_c;
       if (mode > 0) {
-        const float uo = u[c];
-        const float pv = mode == 1 ? uo : uo + beta * p[c];
-        const float sv = mode == 1 ? wv : wv + beta * s[c];
+        const float uo = u_c;
+        const float pv = mode == 1 ? uo : uo + beta * p_c;
+        const float sv = mode == 1 ? wv : wv + beta * s_c;
         p[c] = pv;
         s[c] = sv;
-        x[c] += alpha * pv;
+        x[c] = x_c + alpha * pv;
         rv = rv - alpha * sv;
         r[c] = rv;
       }
-      v = dinv ? rv * dinv[c] : rv;
+      v = dinv ? rv * di_c : rv;
       u[c] = v;
-      d_om = (double)om[c] * (double)v;
+      d_om = (double)om_c * (double)v;
       d_rr = (double)rv * (double)rv;
       d_g = (double)rv * (double)v;
       d_uu = (double)v * (double)v;
@@ -898,7 +1090,7 @@ void product_launches(MixedWork& w, int count, hipStream_t st) {
               lds_f, st, w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
   const int chunks = w.nsp / w.chunk;
   const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad);
-  launch_prof("bwd_mfma", bwd_kernel(w.chunk), dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds,
+  launch_prof("bwd_mfma", bwd_kernel(w.chunk, w.Kp / 16), dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds,
               st, w.sigT, w.Tt, w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp,
               w.part, w.red + L.ys, w.red + L.yy, w.ictl);
 }
@@ -942,8 +1134,8 @@ void iteration_launches(MixedWork& w, const float* dinv, int par, bool first, hi
 // machine that is not iterating has its stop flag set and its blocks return
 // at once.
 struct GroupDesc {
-  const uint16_t* sigA;
-  const uint16_t* sigT;
+  const uint8_t* sigA;
+  const uint8_t* sigT;
   const float* Trm;
   const float* Tt;
   const float* dinv;     // w.dinv when preconditioned, else null
@@ -994,9 +1186,22 @@ __global__ void __launch_bounds__(256)
 #define YQ_BWD_ARGS                                                                            \
   e.y, d.bwd_gx, d.bwd_total, d.sigT, d.Tt, d.ypart, d.ncg, d.ns, d.red + d.op, d.nbv, d.Kp,   \
       d.Np, d.nsp, d.part, d.red + d.ys, d.red + d.yy, d.ictl
-  if (d.chunk == 128) bwd_body<128, kBwdSlice>(YQ_BWD_ARGS);
-  else if (d.chunk == 512) bwd_body<512, kBwdSlice>(YQ_BWD_ARGS);
-  else bwd_body<256, kBwdSlice>(YQ_BWD_ARGS);
+#define YQ_BWD_NIT(CH)                                                                         \
+  switch (d.Kp / 16) {                                                                         \
+    case 2: bwd_body<CH, kBwdSlice, 2>(YQ_BWD_ARGS); break;                                    \
+    case 4: bwd_body<CH, kBwdSlice, 4>(YQ_BWD_ARGS); break;                                    \
+    case 6: bwd_body<CH, kBwdSlice, 6>(YQ_BWD_ARGS); break;                                    \
+    case 8: bwd_body<CH, kBwdSlice, 8>(YQ_BWD_ARGS); break;                                    \
+    default: bwd_body<CH, kBwdSlice, 10>(YQ_BWD_ARGS); break;                                  \
+  }
+  if (d.chunk == 128) {
+    YQ_BWD_NIT(128)
+  } else if (d.chunk == 512) {
+    YQ_BWD_NIT(512)
+  } else {
+    YQ_BWD_NIT(256)
+  }
+#undef YQ_BWD_NIT
 #undef YQ_BWD_ARGS
 }
 // bt[block] = (machine, linear block of its (Np / 16) x (Kp / 16) grid)
@@ -1228,7 +1433,7 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
     const size_t chunks = rows / kBwdChunkMin;
     const size_t ncg_max = (w.Np / 16 + kColTilesMin - 1) / kColTilesMin;
     int rc;
-    if ((rc = alloc(w.sigA, rows * w.Kp, err)) || (rc = alloc(w.sigT, rows * w.Kp, err)) ||
+    if ((rc = alloc(w.sigA, rows * kSigRow, err)) || (rc = alloc(w.sigT, rows / 8 * w.Kp, err)) ||
         (rc = alloc(w.Trm, rows * w.Np, err)) || (rc = alloc(w.Tt, rows * w.Np, err)) ||
         (rc = alloc(w.ypart, rows * ncg_max, err)) ||
         (rc = alloc(w.part, chunks * w.Kp * w.Np, err)) ||
@@ -1255,8 +1460,10 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
     w.cap_dim = w.dim;
     w.cap_red = red_total;
   }
-  hipLaunchKernelGGL(k_prep_sig, dim3(nb(rows * w.Kp)), dim3(256), 0, st, sigma, ns, n, w.Kp,
-                     w.nsp, w.sigA, w.sigT);
+  hipLaunchKernelGGL(k_prep_sig_rows, dim3(nb(rows * kSigRow)), dim3(256), 0, st, sigma, ns, n,
+                     w.nsp, w.sigA);
+  hipLaunchKernelGGL(k_prep_sig_cols, dim3(nb(rows / 8 * w.Kp)), dim3(256), 0, st, sigma, ns, n,
+                     w.Kp, w.nsp, w.sigT);
   hipLaunchKernelGGL(k_prep_T, dim3(nb(rows * w.Np)), dim3(256), 0, st, T32, ns, m, w.Np, w.nsp,
                      w.Trm, w.Tt);
   if (w.pin_ct && w.pin_chunk) {
