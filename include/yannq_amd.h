@@ -362,6 +362,46 @@ typedef struct {
 } yq_ff_desc;
 /* FeedForward() + addLayer(...) x5: FeedForward.hpp:76-82 */
 YQ_API int yq_ff_create(yq_ctx* ctx, const yq_ff_desc* desc);
+
+/* A general layer stack (FeedForward.hpp:76-82, any sequence of addLayer
+ * calls), one yq_ff_layer per addLayer:
+ *  - YQ_LAYER_CONV1D  Conv1D(a = inChannels, b = outChannels, c = kernelSize,
+ *    d = stride, use_bias) (layers/Conv1D.hpp:56-64, forward :144-171,
+ *    backprop :173-225): periodic, input and output channel-major;
+ *  - YQ_LAYER_FC      FullyConnected(a = inputDim, b = outputDim, use_bias)
+ *    (layers/FullyConnected.hpp:57-58, forward :150-157, backprop :160-179);
+ *  - YQ_LAYER_ACT     ActivationLayer<activation a> with its constructor
+ *    argument param (ActivationLayer.hpp:42-136, activations.hpp: Tanh's
+ *    constant, LeakyReLU's negative slope, LeakyHardTanh's outside slope,
+ *    Cos's a; ignored by the others);
+ *  - YQ_LAYER_MEAN    Mean({a, b}, c = axis) (layers/Mean.hpp:21-25, forward
+ *    :92-103, backprop :106-125): the input as an a x b column-major matrix.
+ * n is the input size (the spins); the last layer must leave one value
+ * (FeedForward.hpp:165).  Parameters: the layers' getParams() concatenated
+ * (Conv1D: kernel (ki + ic*kernelSize) + oc*inChannels*kernelSize, then the
+ * bias; FullyConnected: weight i + o*inputDim, then the bias).  dtype 0 =
+ * float, 1 = double.  The C4 stack (Conv1D(1, C, K) -> LnCosh ->
+ * FullyConnected(C*n, H) -> Tanh -> Mean({H, 1}, 0), no biases) runs on the
+ * kernels of yq_ff_create; every other stack on the general layer kernel.
+ * Every yq_ff_* call below serves both. */
+typedef enum { YQ_LAYER_CONV1D = 1, YQ_LAYER_FC = 2, YQ_LAYER_ACT = 3, YQ_LAYER_MEAN = 4 } yq_layer_kind;
+typedef enum {
+  YQ_ACT_IDENTITY = 0, YQ_ACT_LNCOSH = 1, YQ_ACT_TANH = 2, YQ_ACT_WEAKTANH = 3,
+  YQ_ACT_SIGMOID = 4, YQ_ACT_RELU = 5, YQ_ACT_LEAKYRELU = 6, YQ_ACT_HARDTANH = 7,
+  YQ_ACT_SOFTSHRINK = 8, YQ_ACT_LEAKYHARDTANH = 9, YQ_ACT_SOFTSIGN = 10, YQ_ACT_COS = 11
+} yq_activation;
+typedef struct {
+  int kind;
+  int a, b, c, d;
+  int use_bias;
+  double param;
+} yq_ff_layer;
+YQ_API int yq_ff_create_stack(yq_ctx* ctx, int n, int dtype, int n_layers,
+                              const yq_ff_layer* layers);
+/* Values per configuration that yq_ff_forward_dev stores in act (and
+ * yq_ff_vjp_dev reads): 2*channels*n + hidden for the C4 network, else the
+ * input and every layer's output (FeedForward::makeData, :170-184). */
+YQ_API int yq_ff_act_len(const yq_ctx* ctx, int* len);
 /* getDim: number of parameters */
 YQ_API int yq_ff_npar(const yq_ctx* ctx, int* npar);
 /* setParams / getParams / updateParams */
@@ -382,7 +422,7 @@ YQ_API int yq_ff_jacobian(yq_ctx* ctx, int ns, const int8_t* sigma,
 /* Device-pointer forms for framework interop (e.g. a torch.autograd
  * Function).  Arrays are in the machine dtype on the context's device; the
  * caller finishes its own work on them before the call, which returns after
- * completion.  act (ns x (2*channels*n + hidden)) stores the activations in
+ * completion.  act (ns x yq_ff_act_len values) stores the activations in
  * yq_ff_forward_dev when non-null; yq_ff_vjp_dev reads them when non-null and
  * recomputes them otherwise.  grad = sum_s r_s d log psi_s / d params. */
 YQ_API int yq_ff_forward_dev(yq_ctx* ctx, int ns, const int8_t* d_sigma,

@@ -77,6 +77,7 @@ def lib():
             "or_ff_npar": (i, [p]),
             "or_ff_forward": (d, [p, p, p]),
             "or_ff_jacobian": (d, [p, p, p, p]),
+            "or_ffs_check": (i, [i, i, p]),
             "bl_threads": (i, []),
             "bl_sample": (i, [p, i, p, p, p, p, p, p, p, i, i, p, p, i]),
             "bl_eloc": (None, [p, i, i, i, p, p, p, p, p, i]),
@@ -357,9 +358,35 @@ def sample_replay_pt(rbm, sigma, *, n_temps, seed, n_sweeps, n_discard, kind=LOC
 
 
 # ---------------------------------------------------------------- FeedForward
+class FFLayer(C.Structure):
+    """One layer of a general stack: kind 1 Conv1D(a in, b out, c kernel,
+    d stride), 2 FullyConnected(a in, b out), 3 activation a with param,
+    4 Mean({a, b}, axis c); the numbering of yq_ff_layer."""
+    _fields_ = [("kind", C.c_int), ("a", C.c_int), ("b", C.c_int), ("c", C.c_int),
+                ("d", C.c_int), ("use_bias", C.c_int), ("param", C.c_double)]
+
+
 class FFDesc(C.Structure):
     _fields_ = [("n", C.c_int), ("channels", C.c_int), ("kernel", C.c_int),
-                ("hidden", C.c_int)]
+                ("hidden", C.c_int), ("nl", C.c_int), ("layers", C.POINTER(FFLayer))]
+
+
+CONV1D, FC, ACT, MEAN = 1, 2, 3, 4
+(A_IDENTITY, A_LNCOSH, A_TANH, A_WEAKTANH, A_SIGMOID, A_RELU, A_LEAKYRELU, A_HARDTANH,
+ A_SOFTSHRINK, A_LEAKYHARDTANH, A_SOFTSIGN, A_COS) = range(12)
+
+
+def ff_stack(n, layers):
+    """FFDesc of a general stack on n inputs; layers are tuples
+    (kind, a, b, c, d, use_bias, param).  The layer array lives as long as
+    the descriptor."""
+    arr = (FFLayer * len(layers))(*[FFLayer(*[int(v) for v in L[:6]], float(L[6]))
+                                    for L in layers])
+    d = FFDesc(n, 0, 0, 0, len(layers), arr)
+    d._keep = arr
+    if lib().or_ffs_check(n, len(layers), C.byref(arr)):
+        raise ValueError("layer shapes do not chain to one output")
+    return d
 
 
 def ff_npar(desc):

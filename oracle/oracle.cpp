@@ -365,6 +365,7 @@ void or_sgd_update(int dim, double alpha, double p, int t, const double* v,
 
 // ============================================================ FeedForward
 int or_ff_npar(const or_ff_desc* d) {
+  if (d->nl > 0) return or_ffs_npar(d->nl, d->layers);
   return d->kernel * d->channels + d->channels * d->n * d->hidden;
 }
 
@@ -408,6 +409,7 @@ void ff_forward_all(const or_ff_desc* d, const double* p, const int8_t* sigma, F
 }  // namespace
 
 double or_ff_forward(const or_ff_desc* d, const double* params, const int8_t* sigma) {
+  if (d->nl > 0) return or_ffs_forward(d->n, d->nl, d->layers, params, sigma);
   FFAct A;
   ff_forward_all(d, params, sigma, A);
   return A.out;
@@ -416,6 +418,7 @@ double or_ff_forward(const or_ff_desc* d, const double* params, const int8_t* si
 // FeedForward::backward, FeedForward.hpp:186-205, dout = 1, layers in reverse
 double or_ff_jacobian(const or_ff_desc* d, const double* p, const int8_t* sigma,
                       double* der) {
+  if (d->nl > 0) return or_ffs_jacobian(d->n, d->nl, d->layers, p, sigma, der);
   const int n = d->n, C = d->channels, K = d->kernel, H = d->hidden, CN = C * n;
   const double* W = p + K * C;
   FFAct A;

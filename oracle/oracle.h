@@ -89,9 +89,29 @@ void or_c_sr_build(int ns, int n, int m, int use_bias, const int8_t* sigma, cons
 // ---- FeedForward: Conv1D(1->C, K, periodic) -> LnCosh -> FC(C*n -> H) ->
 // Tanh -> Mean, no biases (the layer defaults).  Parameters [conv kernel
 // (K*C, index ki + oc*K) | FC weight (C*n x H column-major)].
+// A general layer stack (nl > 0: channels, kernel and hidden are unused;
+// or_ff_npar / or_ff_forward / or_ff_jacobian and the AmplitudePhase
+// functions below then run the stack of oracle_ffs.cpp).
+typedef struct {
+  int kind;  // 1 Conv1D(a in, b out, c kernel, d stride), 2 FullyConnected(a in, b out),
+             // 3 activation a (param: Tanh / Cos constant, LeakyReLU / LeakyHardTanh slope),
+             // 4 Mean({a, b}, axis c)
+  int a, b, c, d;
+  int use_bias;
+  double param;
+} or_ff_layer;
 typedef struct {
   int n, channels, kernel, hidden;
+  int nl;
+  const or_ff_layer* layers;
 } or_ff_desc;
+// general stack on n inputs: 0 when the shapes chain to one output
+int or_ffs_check(int n, int nl, const or_ff_layer* layers);
+int or_ffs_npar(int nl, const or_ff_layer* layers);
+double or_ffs_forward(int n, int nl, const or_ff_layer* layers, const double* params,
+                      const int8_t* sigma);
+double or_ffs_jacobian(int n, int nl, const or_ff_layer* layers, const double* params,
+                       const int8_t* sigma, double* der);
 int or_ff_npar(const or_ff_desc* d);
 // log psi(sigma) = network output (FeedForward.hpp:152-168)
 double or_ff_forward(const or_ff_desc* d, const double* params,

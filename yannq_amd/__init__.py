@@ -51,6 +51,18 @@ class FFDesc(C.Structure):
                 ("hidden", C.c_int), ("dtype", C.c_int)]
 
 
+class FFLayer(C.Structure):
+    """yq_ff_layer: kind CONV1D(a in, b out, c kernel, d stride), FC(a in,
+    b out), ACT (a = activation, param), MEAN({a, b}, axis c)."""
+    _fields_ = [("kind", C.c_int), ("a", C.c_int), ("b", C.c_int), ("c", C.c_int),
+                ("d", C.c_int), ("use_bias", C.c_int), ("param", C.c_double)]
+
+
+LAYER_CONV1D, LAYER_FC, LAYER_ACT, LAYER_MEAN = 1, 2, 3, 4
+(ACT_IDENTITY, ACT_LNCOSH, ACT_TANH, ACT_WEAKTANH, ACT_SIGMOID, ACT_RELU, ACT_LEAKYRELU,
+ ACT_HARDTANH, ACT_SOFTSHRINK, ACT_LEAKYHARDTANH, ACT_SOFTSIGN, ACT_COS) = range(12)
+
+
 class StepCfg(C.Structure):
     _fields_ = [("n_sweeps", C.c_int), ("n_discard", C.c_int), ("randomize", C.c_int),
                 ("nup", C.c_int), ("solver", C.c_int), ("precision", C.c_int),
@@ -112,6 +124,7 @@ def lib():
         "yq_rbm_log_ratio": [p, i, p, i, p, p],
         "yq_sr_fisher_average": [p, d, i],
         "yq_ff_vjp_dev_noalloc": [p, i, p, p, p, p],
+        "yq_ff_create_stack": [p, i, i, i, p], "yq_ff_act_len": [p, p],
         "yq_ckpt_rbm_encode": [i, i, i, i, p, i, p, C.c_size_t],
         "yq_ckpt_rbm_decode": [i, p, C.c_size_t, i, p, p, p, p, C.c_size_t],
         "yq_rbm_save": [p, C.c_char_p, i, i], "yq_rbm_load": [p, C.c_char_p, i, i],
@@ -393,6 +406,24 @@ class Context:
         self._chk(self._L.yq_ff_npar(self._h, C.byref(k)))
         self.ff_n, self.ff_npar = n, k.value
         self.ff_dtype = np.dtype(np.float64 if d.dtype else np.float32)
+
+    def ff_create_stack(self, n, layers, dtype=np.float64):
+        """A general layer stack: layers are tuples (kind, a, b, c, d,
+        use_bias, param) in addLayer order."""
+        arr = (FFLayer * len(layers))(*[FFLayer(*[int(v) for v in L[:6]], float(L[6]))
+                                        for L in layers])
+        dt = 1 if np.dtype(dtype) == np.float64 else 0
+        self._chk(self._L.yq_ff_create_stack(self._h, n, dt, len(layers), C.byref(arr)))
+        self.ap = False
+        k = C.c_int()
+        self._chk(self._L.yq_ff_npar(self._h, C.byref(k)))
+        self.ff_n, self.ff_npar = n, k.value
+        self.ff_dtype = np.dtype(np.float64 if dt else np.float32)
+
+    def ff_act_len(self):
+        k = C.c_int()
+        self._chk(self._L.yq_ff_act_len(self._h, C.byref(k)))
+        return k.value
 
     def ff_set_params(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)

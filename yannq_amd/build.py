@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyannq_amd.so")
 SOURCES = ["sampler.hip", "eloc.hip", "sr.hip", "mixed.hip", "ff.hip", "complex.hip",
-           "checkpoint.hip", "exact.hip", "ap.hip", "syrk.hip", "api.hip"]
+           "checkpoint.hip", "exact.hip", "ap.hip", "syrk.hip", "ffstack.hip", "api.hip"]
 ARCH = os.environ.get("YQ_OFFLOAD_ARCH", "gfx950")
 
 CFLAGS = [

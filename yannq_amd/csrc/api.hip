@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <functional>
@@ -150,6 +151,13 @@ struct yq_ctx {
   int ff_npar = 0;
   std::vector<double> ff_host;  // fp64 master parameters
   Buf ffp, ff_sig, ff_out, ff_jac, ff_part, ff_grad;
+  // general layer stack (yq_ff_create_stack off the C4 pattern): the layer
+  // table (host copy and device copy), the activation strip length, the
+  // widest layer and the samples per block of the general kernel
+  bool ff_stack = false;
+  std::vector<FFLayerDev> ff_ldev;
+  Buf ff_ltab;
+  int ff_act_len = 0, ff_dmax = 0, ff_sb = 1;
 
   // AmplitudePhase machine (yq_ap_enable): the real RBM above is the
   // amplitude, the FeedForward machine the phase; ap_ffs / ap_ffb hold the
@@ -161,7 +169,7 @@ struct yq_ctx {
     for (Buf* b : {&params, &WT64, &W32, &WT32, &a32, &sig_state, &ctr, &acc, &tbuf, &lrbuf, &bsig,
                    &btheta, &ratio, &eloc, &estat, &T64, &T32, &O, &omean, &F, &S, &M,
                    &partial, &ecent, &info, &vx, &vr, &vp, &vAp, &vy, &vtmp, &vb, &dotp, &dscal,
-                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &diagS, &dinv64, &vz, &savg,
+                   &ffp, &ff_sig, &ff_out, &ff_jac, &ff_part, &ff_grad, &ff_ltab, &diagS, &dinv64, &vz, &savg,
                    &xprev, &ones, &cpd, &cpf, &m_sargs, &m_items, &m_eargs, &m_stats, &ap_ffs,
                    &ap_ffb})
       bufs.push_back(b);
@@ -1990,7 +1998,7 @@ int ff_upload(yq_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->ffp.p, f.data(), sizeof(float) * np, hipMemcpyHostToDevice, c->st));
     TRY(sync(c));
   }
-  launch_ff_transpose(ff_args(c, 1, nullptr), c->ff.dtype, c->st);
+  if (!c->ff_stack) launch_ff_transpose(ff_args(c, 1, nullptr), c->ff.dtype, c->st);
   LAUNCHCHK(c);
   return sync(c);
 }
@@ -2002,10 +2010,44 @@ FFArgs ff_args(yq_ctx* c, int ns, const int8_t* d_sigma) {
   a.H = c->ff.hidden;
   a.ns = ns;
   a.params = c->ffp.p;
-  // the transposed FC weight follows the parameters in the same buffer
-  a.wt = static_cast<const char*>(c->ffp.p) + ff_rs(c) * (size_t)c->ff_npar;
   a.sigma = d_sigma;
+  if (c->ff_stack) {
+    a.layers = c->ff_ltab.as<FFLayerDev>();
+    a.nl = (int)c->ff_ldev.size();
+    a.act_len = c->ff_act_len;
+    a.dmax = c->ff_dmax;
+    a.npar = c->ff_npar;
+    a.sb = c->ff_sb;
+  } else {
+    // the transposed FC weight follows the parameters in the same buffer
+    a.wt = static_cast<const char*>(c->ffp.p) + ff_rs(c) * (size_t)c->ff_npar;
+  }
   return a;
+}
+// VJP partial rows (one per block) for ns configurations
+int ff_vjp_rows(const yq_ctx* c, int ns) {
+  if (!c->ff_stack) return ff_vjp_blocks(ns, c->ff.dtype);
+  const int groups = (ns + c->ff_sb - 1) / c->ff_sb;
+  return groups < 512 ? groups : 512;
+}
+// (paramDim, fanIn, fanOut) of the layers with parameters, in order
+// (FeedForward.hpp:43-73; Conv1D.hpp:227-234, FullyConnected.hpp:181-188)
+std::vector<std::array<long long, 3>> ff_param_layers(const yq_ctx* c) {
+  std::vector<std::array<long long, 3>> v;
+  if (c->ff_stack) {
+    for (const FFLayerDev& L : c->ff_ldev) {
+      if (L.kind == YQ_LAYER_CONV1D)
+        v.push_back({L.npar, (long long)L.ic * L.K, (long long)L.oc * L.K});
+      else if (L.kind == YQ_LAYER_FC)
+        v.push_back({L.npar, L.ic, L.oc});
+    }
+    return v;
+  }
+  const yq_ff_desc& d = c->ff;
+  const long long CN = (long long)d.channels * d.n;
+  v.push_back({(long long)d.kernel * d.channels, d.kernel, (long long)d.channels * d.kernel});
+  v.push_back({CN * d.hidden, CN, d.hidden});
+  return v;
 }
 int ff_check_sigma(yq_ctx* c, int ns, const int8_t* sigma) {
   if (ns < 1 || !sigma) return fail(c, YQ_ERR_ARG, "need ns >= 1 and sigma");
@@ -2039,6 +2081,8 @@ YQ_API int yq_ff_create(yq_ctx* c, const yq_ff_desc* d) {
   if (np > 96 * 256) return fail(c, YQ_ERR_UNSUPPORTED, "FeedForward limited to 24576 parameters");
   c->ap = false;
   c->ff = *d;
+  c->ff_stack = false;
+  c->ff_ldev.clear();
   c->ff_npar = (int)np;
   c->ff_host.assign((size_t)np, 0.0);
   FFArgs probe = ff_args(c, 1, nullptr);
@@ -2047,6 +2091,121 @@ YQ_API int yq_ff_create(yq_ctx* c, const yq_ff_desc* d) {
   TRY(ensure(c, c->ffp, ff_rs(c) * ((size_t)np + (size_t)d->channels * d->n * d->hidden)));
   c->have_ff = true;
   return ff_upload(c);
+}
+
+YQ_API int yq_ff_create_stack(yq_ctx* c, int n, int dtype, int nl, const yq_ff_layer* ly) {
+  if (!c) return YQ_ERR_ARG;
+  if (!ly || nl < 1 || nl > 256) return fail(c, YQ_ERR_ARG, "need 1 to 256 layers");
+  if (n < 1 || n > (1 << 20) || (dtype != 0 && dtype != 1))
+    return fail(c, YQ_ERR_ARG, "invalid input size or dtype");
+  // the running width through FeedForward::makeData (:170-184), each layer's
+  // outputDim (Conv1D.hpp:103-105, FullyConnected.hpp, ActivationLayer.hpp:
+  // 98-101, Mean.hpp:71-75)
+  std::vector<FFLayerDev> t((size_t)nl);
+  long long width = n, off = n, poff = 0, dmax = n;
+  for (int i = 0; i < nl; ++i) {
+    const yq_ff_layer& l = ly[i];
+    FFLayerDev& L = t[(size_t)i];
+    L = FFLayerDev{};
+    L.kind = l.kind;
+    L.in_dim = (int)width;
+    L.in_off = (int)(off - width);
+    L.p_off = (int)poff;
+    long long out = 0, np = 0;
+    if (l.kind == YQ_LAYER_CONV1D) {
+      if (l.a < 1 || l.b < 1 || l.c < 1 || l.d < 1 || width % l.a)
+        return fail(c, YQ_ERR_ARG, "Conv1D: invalid channels, kernel or stride for its input");
+      const long long inSize = width / l.a;
+      if (l.c / 2 > inSize || inSize / l.d < 1)
+        return fail(c, YQ_ERR_ARG, "Conv1D: kernel or stride too large for its input");
+      L.ic = l.a, L.oc = l.b, L.K = l.c, L.stride = l.d, L.bias = l.use_bias ? 1 : 0;
+      out = (inSize / l.d) * l.b;
+      np = (long long)l.a * l.b * l.c + (l.use_bias ? l.b : 0);
+    } else if (l.kind == YQ_LAYER_FC) {
+      if (l.a != width || l.b < 1)
+        return fail(c, YQ_ERR_ARG, "FullyConnected: inputDim must equal the width before it");
+      L.ic = l.a, L.oc = l.b, L.bias = l.use_bias ? 1 : 0;
+      out = l.b;
+      np = (long long)l.a * l.b + (l.use_bias ? l.b : 0);
+    } else if (l.kind == YQ_LAYER_ACT) {
+      if (l.a < YQ_ACT_IDENTITY || l.a > YQ_ACT_COS)
+        return fail(c, YQ_ERR_ARG, "unknown activation");
+      if ((l.a == YQ_ACT_TANH || l.a == YQ_ACT_COS) && !(l.param != 0.0 && std::isfinite(l.param)))
+        return fail(c, YQ_ERR_ARG, "Tanh / Cos need a finite non-zero constant");
+      if (!std::isfinite(l.param)) return fail(c, YQ_ERR_ARG, "non-finite activation parameter");
+      L.act = l.a;
+      L.prm = l.param;
+      out = width;
+    } else if (l.kind == YQ_LAYER_MEAN) {
+      if (l.a < 1 || l.b < 1 || (long long)l.a * l.b != width || (l.c != 0 && l.c != 1))
+        return fail(c, YQ_ERR_ARG, "Mean: input dims must multiply to the width, axis 0 or 1");
+      L.rows = l.a, L.cols = l.b, L.axis = l.c;
+      out = l.c == 0 ? l.b : l.a;
+    } else {
+      return fail(c, YQ_ERR_ARG, "unknown layer kind");
+    }
+    L.out_dim = (int)out;
+    L.out_off = (int)off;
+    L.npar = (int)np;
+    width = out;
+    off += out;
+    poff += np;
+    dmax = std::max(dmax, out);
+    if (off > (1 << 24) || poff > (1LL << 30)) return fail(c, YQ_ERR_ARG, "network too large");
+  }
+  if (width != 1) return fail(c, YQ_ERR_ARG, "the last layer must leave one value");
+  if (poff < 1) return fail(c, YQ_ERR_ARG, "the network has no parameters");
+  // The C4 stack runs on its own kernels (ff.hip) when they can hold it.
+  const bool c4 = nl == 5 && ly[0].kind == YQ_LAYER_CONV1D && ly[0].a == 1 && ly[0].d == 1 &&
+                  !ly[0].use_bias && ly[1].kind == YQ_LAYER_ACT && ly[1].a == YQ_ACT_LNCOSH &&
+                  ly[2].kind == YQ_LAYER_FC && !ly[2].use_bias && ly[3].kind == YQ_LAYER_ACT &&
+                  ly[3].a == YQ_ACT_TANH && ly[3].param == 1.0 && ly[4].kind == YQ_LAYER_MEAN &&
+                  ly[4].b == 1 && ly[4].c == 0 && ly[0].c <= n && n >= 2 && n <= 1024;
+  if (c4) {
+    const yq_ff_desc d{n, ly[0].b, ly[0].c, ly[2].b, dtype};
+    FFArgs probe{};
+    probe.n = d.n;
+    probe.C = d.channels;
+    probe.K = d.kernel;
+    probe.H = d.hidden;
+    if (poff <= 96 * 256 && ff_smem_bytes(probe, dtype) <= 64 * 1024) return yq_ff_create(c, &d);
+  }
+  const size_t rs = dtype == 1 ? sizeof(double) : sizeof(float);
+  const size_t per = ffs_lds_bytes((int)off, (int)dmax, 1, dtype);
+  int sb = (int)std::min<size_t>(8, (64 * 1024) / per);
+  if (sb < 1) {
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess || lds_max <= 0)
+      lds_max = 64 * 1024;
+    if (per > (size_t)lds_max)
+      return fail(c, YQ_ERR_UNSUPPORTED, "FeedForward activations of one sample exceed the LDS");
+    sb = 1;
+  }
+  c->ap = false;
+  c->have_ff = false;
+  c->ff = yq_ff_desc{n, 0, 0, 0, dtype};
+  c->ff_stack = true;
+  c->ff_ldev = t;
+  c->ff_act_len = (int)off;
+  c->ff_dmax = (int)dmax;
+  c->ff_sb = sb;
+  c->ff_npar = (int)poff;
+  c->ff_host.assign((size_t)poff, 0.0);
+  TRY(ensure(c, c->ff_ltab, sizeof(FFLayerDev) * t.size()));
+  HIPCHK(c, hipMemcpyAsync(c->ff_ltab.p, c->ff_ldev.data(), sizeof(FFLayerDev) * t.size(),
+                           hipMemcpyHostToDevice, c->st));
+  TRY(ensure(c, c->ffp, rs * (size_t)poff));
+  c->have_ff = true;
+  return ff_upload(c);
+}
+
+YQ_API int yq_ff_act_len(const yq_ctx* c, int* len) {
+  if (!c || !c->have_ff) return YQ_ERR_STATE;
+  if (len)
+    *len = c->ff_stack ? c->ff_act_len : 2 * c->ff.channels * c->ff.n + c->ff.hidden;
+  return YQ_OK;
 }
 
 YQ_API int yq_ff_npar(const yq_ctx* c, int* npar) {
@@ -2081,17 +2240,14 @@ YQ_API int yq_ff_update_params(yq_ctx* c, const double* delta) {
 YQ_API int yq_ff_init_random(yq_ctx* c, uint64_t seed, int mode) {
   TRY(need_ff(c));
   if (mode < 0 || mode > 3) return fail(c, YQ_ERR_ARG, "mode must be 0..3");
-  const yq_ff_desc& d = c->ff;
   // layers with parameters, in order: (paramDim, fanIn, fanOut)
-  const int CN = d.channels * d.n;
-  const int layers[2][3] = {{d.kernel * d.channels, d.kernel, d.channels * d.kernel},
-                            {CN * d.hidden, CN, d.hidden}};
+  const std::vector<std::array<long long, 3>> layers = ff_param_layers(c);
   size_t q = 0;
   for (const auto& L : layers) {
     const double fi = L[1], fo = L[2];
     const double s = mode == 0 ? sqrt(1.0 / fi) : mode == 1 ? sqrt(2.0 / (fi + fo))
                    : mode == 2 ? sqrt(2.0 / fi) : sqrt(6.0 / (fi + fo));
-    for (int k = 0; k < L[0]; ++k, ++q) {
+    for (long long k = 0; k < L[0]; ++k, ++q) {
       const u32x4 x = draw(seed, q / 2, 1u, kDomParams);
       if (mode == 3) {
         const double u = (q & 1) ? u01_double(x.z, x.w) : u01_double(x.x, x.y);
@@ -2135,7 +2291,7 @@ YQ_API int yq_ff_vjp_dev(yq_ctx* c, int ns, const int8_t* d_sigma, const void* d
                          const void* d_r, void* d_grad) {
   TRY(need_ff(c));
   if (ns < 1 || !d_sigma || !d_r || !d_grad) return fail(c, YQ_ERR_ARG, "need ns >= 1 and buffers");
-  TRY(ensure(c, c->ff_part, ff_rs(c) * (size_t)ff_vjp_blocks(ns, c->ff.dtype) * c->ff_npar));
+  TRY(ensure(c, c->ff_part, ff_rs(c) * (size_t)ff_vjp_rows(c, ns) * c->ff_npar));
   FFArgs a = ff_args(c, ns, d_sigma);
   a.act_in = d_act;
   a.r = d_r;
@@ -2165,7 +2321,7 @@ YQ_API int yq_ff_vjp_dev_noalloc(yq_ctx* c, int ns, const int8_t* d_sigma, const
   a.r = d_r;
   a.partial = scr;
   a.grad = d_grad;
-  const int nb = ff_vjp_blocks(ns, c->ff.dtype);
+  const int nb = ff_vjp_rows(c, ns);
   a.vjp_blocks = (size_t)nb < rows ? nb : (int)rows;
   const int rc = launch_ff(a, kFFVjp, c->ff.dtype, c->st);
   if (rc) return fail(c, rc, "FeedForward VJP shape unsupported");

@@ -407,6 +407,7 @@ void launch_ff_transpose(const FFArgs& a, int use_double, hipStream_t st) {
 }
 
 int launch_ff(const FFArgs& a, int mode, int use_double, hipStream_t st) {
+  if (a.layers) return launch_ffs(a, mode, use_double, st);
   return use_double ? launch_r<double>(a, mode, st) : launch_r<float>(a, mode, st);
 }
 

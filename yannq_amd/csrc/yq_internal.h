@@ -294,7 +294,29 @@ struct FFArgs {
   // bond_sites, Hamiltonian kind flip_kind) of sigma row q / flip_B, so ns
   // counts (sample, term) pairs
   int flip_B, flip_kind;
+  // General layer stack (layers non-null): the layer table on the device,
+  // the per-sample activation strip (the input and every layer's output, in
+  // order; act_out / act_in hold ns strips of act_len values), the largest
+  // layer width, the parameter count and the samples per block.
+  const struct FFLayerDev* layers;
+  int nl, act_len, dmax, npar, sb;
 };
+// One layer of a general FeedForward stack, as the kernels read it: the
+// yq_ff_layer fields resolved against the running width (in_dim, out_dim),
+// with the offsets of its input and output in the activation strip and of
+// its parameters in the parameter vector (npar = weights + bias).
+struct FFLayerDev {
+  int kind, act;
+  int in_dim, out_dim, in_off, out_off;
+  int p_off, npar;
+  int ic, oc, K, stride, bias;  // Conv1D; FullyConnected uses ic = in, oc = out
+  int rows, cols, axis;         // Mean
+  double prm;                   // activation parameter
+};
+// LDS of the general kernel for sb samples a block
+size_t ffs_lds_bytes(int act_len, int dmax, int sb, int use_double);
+// the general kernel (FFArgs with layers set); launch_ff dispatches to it
+int launch_ffs(const FFArgs& a, int mode, int use_double, hipStream_t st);
 enum { kFFForward = 0, kFFJacobian = 1, kFFVjp = 2 };
 int ff_vjp_blocks(int ns, int use_double);
 // The fixed device array behind yq_ff_vjp_dev_noalloc (kFFScratchBytes per

@@ -1,7 +1,9 @@
 """torch.autograd wrapper of the FeedForward machine's HIP kernels.
 
 ``FFMachine(ctx_device, n, channels, kernel, hidden, dtype)`` holds the
-network of Machines/FeedForward.hpp used in the C4 configuration and exposes
+network of Machines/FeedForward.hpp used in the C4 configuration (or, with
+``layers``, any stack of Conv1D / FullyConnected / activation / Mean layers)
+and exposes
 ``logpsi(sigma)`` whose backward pass is the hand-written per-sample reverse
 sweep (``yq_ff_vjp_dev``), so ``torch.autograd.gradcheck`` and finite
 differences can be run against it.  The forward stores the layer
@@ -50,16 +52,21 @@ class _LogPsi(torch.autograd.Function):
 
 class FFMachine:
     """Conv1D(1->channels, kernel) -> LnCosh -> FC(channels*n -> hidden) ->
-    Tanh -> Mean on the GPU; parameters live in ``self.params``."""
+    Tanh -> Mean on the GPU, or with ``layers`` (tuples (kind, a, b, c, d,
+    use_bias, param) of yq_ff_layer, in addLayer order) any layer stack;
+    parameters live in ``self.params``."""
 
-    def __init__(self, n, channels, kernel, hidden, dtype=torch.float32, device=0,
-                 act_budget=1 << 30, seed=None, init_mode=3, ctx=None):
+    def __init__(self, n, channels=None, kernel=None, hidden=None, dtype=torch.float32,
+                 device=0, act_budget=1 << 30, seed=None, init_mode=3, ctx=None, layers=None):
         self.ctx = ctx if ctx is not None else Context(device)
         self.dtype = dtype
-        self.ctx.ff_create(n, channels, kernel, hidden,
-                           np.float64 if dtype == torch.float64 else np.float32)
+        npdt = np.float64 if dtype == torch.float64 else np.float32
+        if layers is not None:
+            self.ctx.ff_create_stack(n, layers, npdt)
+        else:
+            self.ctx.ff_create(n, channels, kernel, hidden, npdt)
         self.n, self.npar = n, self.ctx.ff_npar
-        self.act_width = 2 * channels * n + hidden
+        self.act_width = self.ctx.ff_act_len()
         self.act_budget = act_budget
         self.last_stored = None
         if seed is not None:
