@@ -828,13 +828,14 @@ class RunRBMExact {
 };
 
 // ------------------------------------------------------------ FeedForward
-// Machines/FeedForward.hpp:16-205 for the network the device implements:
-// Conv1D(1 -> C, kernel K, stride 1) -> LnCosh -> FullyConnected(C*n -> H)
-// -> Tanh -> Mean({H, 1}, 0), without biases.  addLayer takes the reference's
-// layer constructor arguments (layers/Conv1D.hpp:57-61,
-// layers/FullyConnected.hpp:57-58, layers/Mean.hpp:21); the network is
-// created on the device when the Mean layer completes it, and any other
-// sequence throws.  Scalar is double.
+// Machines/FeedForward.hpp:16-205: any sequence of addLayer<L>(args...) with
+// the reference's layer constructor arguments (layers/Conv1D.hpp:56-64,
+// layers/FullyConnected.hpp:57-58, layers/Mean.hpp:21, ActivationLayer.hpp:
+// 138-172 with activations.hpp's constructors).  The layers are kept on the
+// host; the device network is made for an input of N spins by bind(N), or
+// by the first forward / backward (N = the configuration's length) or the
+// AmplitudePhase that takes the network.  Parameters set or drawn before
+// that are kept and uploaded then.  Scalar is double.
 enum class InitializationMode { LeCun, Xavier, He };  // FeedForward.hpp:11-14
 
 struct Conv1D {
@@ -844,8 +845,6 @@ struct Conv1D {
   uint32_t in, out, kernel, stride;
   bool useBias;
 };
-struct LnCosh {};
-struct Tanh {};
 struct FullyConnected {
   FullyConnected(uint32_t inputDim, uint32_t outputDim, bool useBias = false)
       : in(inputDim), out(outputDim), useBias(useBias) {}
@@ -857,6 +856,35 @@ struct Mean {
       : dim(inputDim), axis(meanAxis) {}
   std::array<uint32_t, 2> dim;
   uint32_t axis;
+};
+// activation layers (ActivationLayer.hpp:138-172); the constructor argument
+// and its default are activations.hpp's
+namespace detail {
+template <int A>
+struct Activation {
+  explicit Activation(double p = 0.0) : param(p) {}
+  double param;
+};
+}  // namespace detail
+using Identity = detail::Activation<YQ_ACT_IDENTITY>;
+using LnCosh = detail::Activation<YQ_ACT_LNCOSH>;
+struct Tanh : detail::Activation<YQ_ACT_TANH> {  // constant * tanh(z / constant)
+  explicit Tanh(double constant = 1.0) : Activation(constant) {}
+};
+using WeakTanh = detail::Activation<YQ_ACT_WEAKTANH>;
+using Sigmoid = detail::Activation<YQ_ACT_SIGMOID>;
+using ReLU = detail::Activation<YQ_ACT_RELU>;
+struct LeakyReLU : detail::Activation<YQ_ACT_LEAKYRELU> {
+  explicit LeakyReLU(double negativeSlope = 0.1) : Activation(negativeSlope) {}
+};
+using HardTanh = detail::Activation<YQ_ACT_HARDTANH>;
+using SoftShrink = detail::Activation<YQ_ACT_SOFTSHRINK>;
+struct LeakyHardTanh : detail::Activation<YQ_ACT_LEAKYHARDTANH> {
+  explicit LeakyHardTanh(double outsideSlope = 0.01) : Activation(outsideSlope) {}
+};
+using SoftSign = detail::Activation<YQ_ACT_SOFTSIGN>;
+struct Cos : detail::Activation<YQ_ACT_COS> {  // a cos(pi z / a)
+  explicit Cos(double a = 1.0) : Activation(a) {}
 };
 
 class FeedForward {
@@ -872,45 +900,96 @@ class FeedForward {
   template <class Layer, class... Ts>
   void addLayer(Ts&&... args) {  // :76-82
     push(Layer(std::forward<Ts>(args)...));
+    if (n_) bind(n_);  // the device network follows the layer list
   }
-  bool complete() const { return stage_ == 5; }
+  bool complete() const { return !layers_.empty(); }
   uint32_t getN() const { return n_; }
-  uint32_t getDim() const {  // :84-91
-    need();
-    int np = 0;
-    detail::check(ctx(), yq_ff_npar(ctx(), &np));
+  uint32_t getDim() const {  // :84-91 (the sum of the layers' paramDim)
+    long long np = 0;
+    for (const yq_ff_layer& l : layers_) np += paramDim(l);
     return (uint32_t)np;
   }
-  std::string summary() const {  // :101-116
-    return "Conv1D(1, " + std::to_string(C_) + ", " + std::to_string(K_) +
-           ", 1)\nLnCosh\nFullyConnected(" + std::to_string(C_ * n_) + ", " +
-           std::to_string(H_) + ")\nTanh\nMean({" + std::to_string(H_) + ", 1}, 0)\n";
+  // the device network for an input of N spins (made again when N changes)
+  void bind(uint32_t N) {
+    if (layers_.empty()) throw Error(YQ_ERR_STATE, "FeedForward: no layers");
+    Vector keep = bound() ? getParams() : pending_;
+    detail::check(ctx(), yq_ff_create_stack(ctx(), (int)N, 1, (int)layers_.size(),
+                                            layers_.data()));
+    n_ = N;
+    if (keep.size() == getDim()) detail::check(ctx(), yq_ff_set_params(ctx(), keep.data()));
+    pending_.clear();
   }
-  std::string desc() const {
-    return "{\"name\": \"FeedForward\", \"n\": " + std::to_string(n_) + ", \"channels\": " +
-           std::to_string(C_) + ", \"kernel\": " + std::to_string(K_) + ", \"hidden\": " +
-           std::to_string(H_) + "}";
+  std::string summary() const {  // :101-116
+    std::string s = "Layer name\tNumber of params\n";
+    for (const yq_ff_layer& l : layers_)
+      s += std::string(layerName(l)) + "\t" + std::to_string(paramDim(l)) + "\n";
+    return s;
+  }
+  std::string desc() const {  // :118-126, the layers' desc() (Conv1D.hpp:236-246 etc.)
+    std::string s = "[";
+    for (size_t i = 0; i < layers_.size(); ++i) {
+      const yq_ff_layer& l = layers_[i];
+      s += i ? ", {" : "{";
+      s += "\"name\": \"" + std::string(layerName(l)) + "\"";
+      if (l.kind == YQ_LAYER_CONV1D)
+        s += ", \"use_bias\": " + std::string(l.use_bias ? "true" : "false") +
+             ", \"input_channels\": " + std::to_string(l.a) + ", \"output_channels\": " +
+             std::to_string(l.b) + ", \"kernel_size\": " + std::to_string(l.c) +
+             ", \"stride\": " + std::to_string(l.d);
+      else if (l.kind == YQ_LAYER_FC)
+        s += ", \"use_bias\": " + std::string(l.use_bias ? "true" : "false") +
+             ", \"input_dim\": " + std::to_string(l.a) + ", \"output_dim\": " +
+             std::to_string(l.b);
+      else if (l.kind == YQ_LAYER_MEAN)
+        s += ", \"input_dim\": [" + std::to_string(l.a) + ", " + std::to_string(l.b) +
+             "], \"mean_axis\": " + std::to_string(l.c);
+      else
+        s += ", \"activation\": \"" + std::string(activationName(l.a)) + "\"";
+      s += "}";
+    }
+    return s + "]";
   }
   Vector getParams() const {
+    if (!bound()) return pending_.empty() ? Vector(getDim(), 0.0) : pending_;
     Vector v(getDim());
     detail::check(ctx(), yq_ff_get_params(ctx(), v.data()));
     return v;
   }
   void setParams(const Vector& v) {
     if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "setParams: wrong size");
+    if (!bound()) {
+      pending_ = v;
+      return;
+    }
     detail::check(ctx(), yq_ff_set_params(ctx(), v.data()));
   }
   void updateParams(const Vector& v) {
     if (v.size() != getDim()) throw Error(YQ_ERR_ARG, "updateParams: wrong size");
+    if (!bound()) {
+      Vector p = getParams();
+      for (size_t i = 0; i < p.size(); ++i) p[i] += v[i];
+      pending_ = p;
+      return;
+    }
     detail::check(ctx(), yq_ff_update_params(ctx(), v.data()));
   }
-  // :43-73 (fan-in / fan-out scaled Gaussians; Philox keyed from the engine)
+  // :43-73: layer by layer, paramDim draws of N(0, sigma(fanIn, fanOut)^2)
+  // from the engine (randomVector, Utilities/Utility.hpp:126-138)
   template <class RandomEngine>
   void initializeRandom(RandomEngine& re, InitializationMode mode) {
-    need();
-    const uint64_t seed = ((uint64_t)re() << 32) ^ (uint64_t)re();
-    const int m = mode == InitializationMode::LeCun ? 0 : mode == InitializationMode::Xavier ? 1 : 2;
-    detail::check(ctx(), yq_ff_init_random(ctx(), seed, m));
+    Vector v;
+    v.reserve(getDim());
+    for (const yq_ff_layer& l : layers_) {
+      if (paramDim(l) == 0) continue;
+      const double fi = l.kind == YQ_LAYER_CONV1D ? (double)l.a * l.c : (double)l.a;
+      const double fo = l.kind == YQ_LAYER_CONV1D ? (double)l.b * l.c : (double)l.b;
+      const double sigma = mode == InitializationMode::LeCun   ? std::sqrt(1.0 / fi)
+                           : mode == InitializationMode::Xavier ? std::sqrt(2.0 / (fi + fo))
+                                                                : std::sqrt(2.0 / fi);
+      std::normal_distribution<double> nd(0.0, sigma);
+      for (long long k = 0; k < paramDim(l); ++k) v.push_back(nd(re));
+    }
+    setParams(v);
   }
   // :33-41: every parameter from N(0, sigma^2), drawn with the engine
   template <class RandomEngine>
@@ -921,13 +1000,13 @@ class FeedForward {
     setParams(v);
   }
   // forward(sigma) (:152-168) and backward(makeData(sigma)) (:170-205)
-  double forward(const std::vector<int>& sigma) const {
+  double forward(const std::vector<int>& sigma) {
     const std::vector<int8_t> s = spins(sigma);
     double out = 0.0;
     detail::check(ctx(), yq_ff_forward(ctx(), 1, s.data(), &out));
     return out;
   }
-  Vector backward(const std::vector<int>& sigma) const {
+  Vector backward(const std::vector<int>& sigma) {
     const std::vector<int8_t> s = spins(sigma);
     Vector der(getDim());
     detail::check(ctx(), yq_ff_jacobian(ctx(), 1, s.data(), der.data(), nullptr));
@@ -935,48 +1014,55 @@ class FeedForward {
   }
 
  private:
-  static void unsupported() {
-    throw Error(YQ_ERR_UNSUPPORTED,
-                "FeedForward: the device network is Conv1D(1 -> C, k, stride 1) -> LnCosh -> "
-                "FullyConnected(C*n -> H) -> Tanh -> Mean({H, 1}, 0), without biases");
+  bool bound() const { return n_ != 0; }
+  static long long paramDim(const yq_ff_layer& l) {
+    if (l.kind == YQ_LAYER_CONV1D) return (long long)l.a * l.b * l.c + (l.use_bias ? l.b : 0);
+    if (l.kind == YQ_LAYER_FC) return (long long)l.a * l.b + (l.use_bias ? l.b : 0);
+    return 0;
   }
-  void need() const {
-    if (!complete()) throw Error(YQ_ERR_STATE, "FeedForward: network incomplete");
+  static const char* layerName(const yq_ff_layer& l) {
+    switch (l.kind) {
+      case YQ_LAYER_CONV1D: return "Convolutional 1D Layer";
+      case YQ_LAYER_FC: return "Fully Connected Layer";
+      case YQ_LAYER_MEAN: return "Mean layer";
+      default: return "Activation Layer";
+    }
   }
-  std::vector<int8_t> spins(const std::vector<int>& sigma) const {
-    need();
-    if (sigma.size() != n_) throw Error(YQ_ERR_ARG, "FeedForward: wrong input size");
+  static const char* activationName(int a) {
+    static const char* const names[] = {"Identity", "LnCosh", "Tanh", "WeakTanh",
+                                        "Sigmoid", "ReLU", "LeakyReLU", "HardTanh",
+                                        "SoftShrink", "LeakyHardTanh", "SoftSign", "Cos"};
+    return a >= 0 && a < 12 ? names[a] : "?";
+  }
+  std::vector<int8_t> spins(const std::vector<int>& sigma) {
+    if (sigma.empty()) throw Error(YQ_ERR_ARG, "FeedForward: empty input");
+    if (sigma.size() != n_) bind((uint32_t)sigma.size());
     return std::vector<int8_t>(sigma.begin(), sigma.end());
   }
-  void push(const Conv1D& l) {
-    if (stage_ != 0 || l.in != 1 || l.stride != 1 || l.useBias) unsupported();
-    C_ = l.out;
-    K_ = l.kernel;
-    stage_ = 1;
+  void add(int kind, int a, int b, int c, int d, bool bias, double param) {
+    yq_ff_layer l{};
+    l.kind = kind, l.a = a, l.b = b, l.c = c, l.d = d, l.use_bias = bias ? 1 : 0;
+    l.param = param;
+    layers_.push_back(l);
+    pending_.clear();
   }
-  void push(const LnCosh&) {
-    if (stage_ != 1) unsupported();
-    stage_ = 2;
+  void push(const Conv1D& l) {
+    add(YQ_LAYER_CONV1D, (int)l.in, (int)l.out, (int)l.kernel, (int)l.stride, l.useBias, 0.0);
   }
   void push(const FullyConnected& l) {
-    if (stage_ != 2 || l.useBias || l.in % C_ != 0) unsupported();
-    n_ = l.in / C_;
-    H_ = l.out;
-    stage_ = 3;
-  }
-  void push(const Tanh&) {
-    if (stage_ != 3) unsupported();
-    stage_ = 4;
+    add(YQ_LAYER_FC, (int)l.in, (int)l.out, 0, 0, l.useBias, 0.0);
   }
   void push(const Mean& l) {
-    if (stage_ != 4 || l.dim[0] != H_ || l.dim[1] != 1 || l.axis != 0) unsupported();
-    const yq_ff_desc d{(int)n_, (int)C_, (int)K_, (int)H_, 1};
-    detail::check(ctx(), yq_ff_create(ctx(), &d));
-    stage_ = 5;
+    add(YQ_LAYER_MEAN, (int)l.dim[0], (int)l.dim[1], (int)l.axis, 0, false, 0.0);
+  }
+  template <int A>
+  void push(const detail::Activation<A>& l) {
+    add(YQ_LAYER_ACT, A, 0, 0, 0, false, l.param);
   }
   Device* dev_;
-  int stage_ = 0;
-  uint32_t n_ = 0, C_ = 0, K_ = 0, H_ = 0;
+  std::vector<yq_ff_layer> layers_;
+  Vector pending_;  // parameters given before the device network exists
+  uint32_t n_ = 0;
 };
 
 // ------------------------------------------------------------ AmplitudePhase
@@ -995,8 +1081,8 @@ class AmplitudePhase {
   AmplitudePhase(Device& dev, uint32_t N, uint32_t M, bool useBias, FeedForward&& phase)
       : N_(N), amplitude_(dev, N, M, useBias), phase_(std::move(phase)) {  // :37-40
     if (&phase_.device() != &dev) throw Error(YQ_ERR_ARG, "AmplitudePhase: phase on another Device");
-    if (!phase_.complete() || phase_.getN() != N)
-      throw Error(YQ_ERR_ARG, "AmplitudePhase: the phase network must act on N spins");
+    if (!phase_.complete()) throw Error(YQ_ERR_ARG, "AmplitudePhase: the phase network is empty");
+    phase_.bind(N);  // the phase network acts on the N spins
     detail::check(ctx(), yq_ap_enable(ctx(), 1));
   }
   yq_ctx* ctx() const { return amplitude_.ctx(); }
@@ -1039,7 +1125,7 @@ class AmplitudePhase {
     return ComplexScalar(out[0], out[1]);
   }
   ComplexScalar coeff(const std::vector<int>& sigma) const { return std::exp(logCoeff(sigma)); }
-  double phaseForward(const std::vector<int>& sigma) const { return phase_.forward(sigma); }
+  double phaseForward(const std::vector<int>& sigma) { return phase_.forward(sigma); }
   std::string desc() const {  // :160-166
     return "{\"amplitude\": " + amplitude_.desc() + ", \"phase\": " + phase_.desc() + "}";
   }

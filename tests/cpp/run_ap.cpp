@@ -2,9 +2,12 @@
 // drives it: Examples/AmplitudePhase.cpp (RunAmplitudePhaseExact over
 // BasisJz(N, N/2) on the J1-J2 chain with the sign rule) and
 // Tests/TestAmplitudePhase.cpp (SRMatExact<AmplitudePhase> with
-// corrMat + lambda I and SGD(0.02)), with the device's phase network
-// Conv1D -> LnCosh -> FullyConnected -> Tanh -> Mean.
+// corrMat + lambda I and SGD(0.02)), with the phase network Conv1D ->
+// LnCosh -> FullyConnected -> Tanh -> Mean (or the example's own).
 //   run_ap ngd  E0   RunAmplitudePhaseExact; exit 0 if within 1e-4 of E0
+//   run_ap example E0  the same with Examples/AmplitudePhase.cpp's own phase
+//                    network (six Conv1D + LeakyHardTanh, FullyConnected,
+//                    SoftSign)
 //   run_ap srex E0   SRMatExact loop; exit 0 if the energy falls and ends
 //                    within 1e-4 of E0 (relative)
 #include <array>
@@ -28,6 +31,27 @@ static FeedForward phaseNetwork(Device& dev, int N) {
   return ff;
 }
 
+// Examples/AmplitudePhase.cpp:42-55, the reference example's own network
+static FeedForward exampleNetwork(Device& dev, int N) {
+  FeedForward ff(dev);
+  const int kernel_size = 5;
+  ff.addLayer<Conv1D>(1, 12, kernel_size, 1, false);
+  ff.addLayer<LeakyHardTanh>(0.01);
+  ff.addLayer<Conv1D>(12, 10, kernel_size, 1, false);
+  ff.addLayer<LeakyHardTanh>(0.01);
+  ff.addLayer<Conv1D>(10, 8, kernel_size, 1, false);
+  ff.addLayer<LeakyHardTanh>(0.01);
+  ff.addLayer<Conv1D>(8, 6, kernel_size, 1, false);
+  ff.addLayer<LeakyHardTanh>(0.01);
+  ff.addLayer<Conv1D>(6, 4, kernel_size, 1, false);
+  ff.addLayer<LeakyHardTanh>(0.01);
+  ff.addLayer<Conv1D>(4, 2, kernel_size, 1, false);
+  ff.addLayer<LeakyHardTanh>(0.01);
+  ff.addLayer<FullyConnected>(2 * N, 1, false);
+  ff.addLayer<SoftSign>();
+  return ff;
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) {
     std::printf("usage: %s ngd|srex E0\n", argv[0]);
@@ -38,9 +62,12 @@ int main(int argc, char** argv) {
   Device dev(0);
   J1J2 ham(N, 1.0, 0.2, true);
   double first = 0.0, last = 0.0;
-  if (std::strcmp(argv[1], "ngd") == 0) {
+  const bool example = std::strcmp(argv[1], "example") == 0;
+  if (std::strcmp(argv[1], "ngd") == 0 || example) {
     std::ostringstream log;
-    RunAmplitudePhaseExact runner(dev, N, 2, false, phaseNetwork(dev, N), log);
+    FeedForward ff = example ? exampleNetwork(dev, N) : phaseNetwork(dev, N);
+    if (example) std::printf("%s%s\n", ff.summary().c_str(), ff.desc().c_str());
+    RunAmplitudePhaseExact runner(dev, N, 2, false, std::move(ff), log);
     runner.setSeed(3);
     runner.initializeRandom(0.01);
     runner.setLambda(10.0, 0.9, 1e-3);

@@ -85,11 +85,12 @@ def test_exact_runner_reaches_ground_state(yq, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["ngd", "srex"])
+@pytest.mark.parametrize("mode", ["ngd", "srex", "example"])
 def test_amplitude_phase_runners(yq, orc, tmp_path, mode):
     """RunAmplitudePhaseExact and the SRMatExact<AmplitudePhase> loop on the
     J1-J2 chain (N = 10, J2 = 0.2, sign rule) in the Sz = 0 sector, against
-    the sector's ground energy from exact diagonalisation."""
+    the sector's ground energy from exact diagonalisation; "example" runs
+    RunAmplitudePhaseExact with Examples/AmplitudePhase.cpp's phase network."""
     import numpy as np
     import scipy.sparse.linalg as sla
     from tests.conftest import _gpu_available
