@@ -98,9 +98,11 @@ YQ_API const char* yq_last_error(const yq_ctx* ctx);
 
 /* ---- Machine: Machines/RBM.hpp ---------------------------------------- */
 /* RBM(n, m, useBias), all parameters zero: RBM.hpp:50-56.  dtype is a
- * yq_dtype.  Complex machines run the explicit and dense CG solvers in FP64
- * (with YQ_MIXED only sampling and E_loc run in float); the factored solver
- * is real-only. */
+ * yq_dtype; 2 <= n <= 4096 and 1 <= m <= 8192.  Complex machines run the
+ * explicit and dense CG solvers in FP64 (with YQ_MIXED only sampling and
+ * E_loc run in float); the factored solver is real-only, and its mixed form
+ * needs n <= 159.  Parallel tempering and the many-machine calls need
+ * m <= 512. */
 YQ_API int yq_rbm_create(yq_ctx* ctx, int dtype, int n, int m, int use_bias);
 /* getN / getM / getDim: RBM.hpp:115-130 */
 YQ_API int yq_rbm_dims(const yq_ctx* ctx, int* n, int* m, int* dim);
@@ -166,6 +168,13 @@ YQ_API int yq_ham_set(yq_ctx* ctx, const yq_ham_desc* desc);
 YQ_API int yq_sampler_config(yq_ctx* ctx, int sweeper, int n_temps,
                              int n_chains_per, int sweep_factor, int precision,
                              uint64_t seed);
+/* Threads whose partial sums form one chain's log-ratio for a machine with
+ * m hidden units: 64 up to m = 512, else 64 times the smallest power of two
+ * that leaves at most 8 hidden units per thread.  Hidden unit j is summed by
+ * thread j % lanes in increasing j, then the threads are combined by the tree
+ * with strides lanes/2, ..., 1 (what a CPU replay of the accept decisions
+ * needs).  No counterpart in the reference, whose reduction is sequential. */
+YQ_API int yq_chain_lanes(int m);
 /* randomizeSigma() / randomizeSigma(nup): SamplerMT.hpp:81-98; nup < 0
  * draws unconstrained spins. */
 YQ_API int yq_sampler_randomize(yq_ctx* ctx, int nup);

@@ -32,3 +32,14 @@ def test_no_gpu_is_an_error_not_a_crash(yq):
         pytest.skip("a GPU is visible")
     h = C.c_void_p()
     assert yq.lib().yq_ctx_create(0, C.byref(h)) != yq.OK
+
+
+def test_chain_lanes_rule(yq):
+    """One wave up to 512 hidden units, then the smallest power-of-two group
+    with at most 8 units per thread (a host-side rule, no GPU call)."""
+    L = yq.lib()
+    want = {1: 64, 400: 64, 512: 64, 513: 128, 1024: 128, 1025: 256, 2048: 256,
+            2100: 512, 4096: 512, 4097: 1024, 8192: 1024}
+    for m, g in want.items():
+        assert L.yq_chain_lanes(m) == g, m
+        assert g * 8 >= m and (g == 64 or g * 4 < m)

@@ -132,7 +132,7 @@ def lib():
         "yq_exact_sample": [p, i, i, u64, p],
         "yq_ap_enable": [p, i], "yq_ap_dims": [p, p, p], "yq_ap_get_params": [p, p],
         "yq_ap_set_params": [p, p], "yq_ap_update_params": [p, p],
-        "yq_ap_log_psi": [p, i, p, p],
+        "yq_ap_log_psi": [p, i, p, p], "yq_chain_lanes": [i],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -1224,7 +1224,7 @@ YQ_API int yq_rbm_create(yq_ctx* c, int dtype, int n, int m, int use_bias) {
   if (dtype != YQ_DTYPE_F64 && dtype != YQ_DTYPE_C128)
     return fail(c, YQ_ERR_UNSUPPORTED, "dtype must be 1 (double) or 2 (complex double)");
   if (n < 2 || n > kMaxSites || m < 1 || m > kMaxHidden)
-    return fail(c, YQ_ERR_ARG, "need 2 <= n <= 128 and 1 <= m <= 512");
+    return fail(c, YQ_ERR_ARG, "need 2 <= n <= 4096 and 1 <= m <= 8192");
   c->n = n;
   c->m = m;
   c->use_bias = use_bias ? 1 : 0;
@@ -1255,6 +1255,8 @@ YQ_API int yq_rbm_create(yq_ctx* c, int dtype, int n, int m, int use_bias) {
   TRY(refresh_params(c));
   return sync(c);
 }
+
+YQ_API int yq_chain_lanes(int m) { return chain_group(m < 1 ? 1 : m); }
 
 YQ_API int yq_rbm_dims(const yq_ctx* c, int* n, int* m, int* dim) {
   if (!c || !c->have_rbm) return YQ_ERR_STATE;
@@ -1531,6 +1533,8 @@ int sample_args(yq_ctx* c, int n_sweeps, int n_discard, SampleArgs* out) {
   a.ntemps = c->ntemps;
   a.cplx = c->cplx ? 1 : 0;
   if (a.ntemps > 1) {
+    if (c->m > kWaveHidden)
+      return fail(c, YQ_ERR_UNSUPPORTED, "parallel tempering keeps one wave per slot: m <= 512");
     int dev = 0, lds_max = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
@@ -2698,7 +2702,7 @@ int multi_eloc(yq_ctx* c) {
   for (int i = 0; i < count; ++i) {
     yq_ctx* sub = c->subs[i];
     TRY(from_sub(c, sub, eloc_args(sub, &args[i]), i));
-    lds = std::max(lds, eloc_lds_bytes(args[i]));
+    lds = std::max(lds, eloc_stage_lds_bytes(args[i]));
     const size_t pairs = (size_t)args[i].ns * args[i].B;
     max_blocks = std::max(max_blocks, (int)((pairs + 255) / 256));
     max_ns = std::max(max_ns, args[i].ns);
@@ -2732,6 +2736,11 @@ YQ_API int yq_multi_create(yq_ctx* c, int count, const yq_multi_desc* d, int swe
   if (count < 1 || !d) return fail(c, YQ_ERR_ARG, "need count >= 1 and descriptors");
   if (sweeper != YQ_LOCAL && sweeper != YQ_EXCHANGE) return fail(c, YQ_ERR_ARG, "unknown sweeper");
   if (precision != YQ_FP64 && precision != YQ_MIXED) return fail(c, YQ_ERR_ARG, "unknown precision");
+  for (int i = 0; i < count; ++i)
+    if (d[i].m > kWaveHidden)
+      return fail(c, YQ_ERR_UNSUPPORTED,
+                  "the many-machine launch keeps one wave per chain: m <= 512 (machine " +
+                      std::to_string(i) + ")");
   drop_subs(c);
   for (int i = 0; i < count; ++i) {
     yq_ctx* sub = nullptr;

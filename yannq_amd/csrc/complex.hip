@@ -65,10 +65,10 @@ __device__ inline float lc_re<float>(float x, float y) {
 }
 
 // theta_j = (sum_i +/-W_ji, index order, fp64, Re and Im apart) + b_j, then
-// rounded to R, and lc_j = Re logcosh(theta_j), for the hidden units of this
-// lane
-template <typename R, int RMAX>
-__device__ __forceinline__ void refresh_c(const SampleArgs& A, const int8_t* sig, int lane,
+// rounded to R, and lc_j = Re logcosh(theta_j), for the hidden units of
+// thread gl of a G-thread chain group
+template <typename R, int RMAX, int G = kWave>
+__device__ __forceinline__ void refresh_c(const SampleArgs& A, const int8_t* sig, int gl,
                                           R (&tr)[RMAX], R (&ti)[RMAX], R (&lc)[RMAX]) {
   const int n = A.n, m = A.m;
   double sr[RMAX], si[RMAX];
@@ -79,7 +79,7 @@ __device__ __forceinline__ void refresh_c(const SampleArgs& A, const int8_t* sig
     const double* wc = A.W64 + 2 * (size_t)i * m;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       if (j < m) {
         const double wr = wc[2 * j], wi = wc[2 * j + 1];
         sr[r] = up ? sr[r] + wr : sr[r] - wr;
@@ -89,7 +89,7 @@ __device__ __forceinline__ void refresh_c(const SampleArgs& A, const int8_t* sig
   }
 #pragma unroll
   for (int r = 0; r < RMAX; ++r) {
-    const int j = lane + kWave * r;
+    const int j = gl + G * r;
     if (j < m) {
       tr[r] = (R)(sr[r] + A.b64[2 * j]);
       ti[r] = (R)(si[r] + A.b64[2 * j + 1]);
@@ -101,12 +101,15 @@ __device__ __forceinline__ void refresh_c(const SampleArgs& A, const int8_t* sig
 }
 
 // One sweep (A.nprop proposals) of one chain at inverse temperature beta;
-// returns the accepted count.  ev is the chain's proposal counter.
-template <typename R, int KIND, int RMAX>
+// returns the accepted count.  ev is the chain's proposal counter; gl, G,
+// red and par as in the real sweep (sampler.hip): a G > 64 group is the
+// whole block, every wave stores each accepted flip itself.
+template <typename R, int KIND, int RMAX, int G = kWave>
 __device__ __forceinline__ uint64_t sweep_chain_c(const SampleArgs& A, int chain, int8_t* sig,
-                                                  int lane, R (&tr)[RMAX], R (&ti)[RMAX],
+                                                  int gl, R (&tr)[RMAX], R (&ti)[RMAX],
                                                   R (&lc)[RMAX], R (&ur)[RMAX], R (&ui)[RMAX],
-                                                  R (&lt)[RMAX], uint64_t& ev, R beta) {
+                                                  R (&lt)[RMAX], uint64_t& ev, R beta,
+                                                  R* red = nullptr, int* par = nullptr) {
   const int n = A.n, m = A.m;
   const R* Wr = static_cast<const R*>(A.Wr);
   const R* Wi = static_cast<const R*>(A.Wi);
@@ -143,7 +146,7 @@ __device__ __forceinline__ uint64_t sweep_chain_c(const SampleArgs& A, int chain
     R part = (R)0;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       if (j < m) {
         R vr = tr[r] - twok * wkr[j];
         R vi = ti[r] - twok * wki[j];
@@ -157,8 +160,12 @@ __device__ __forceinline__ uint64_t sweep_chain_c(const SampleArgs& A, int chain
         part = part + (lt[r] - lc[r]);
       }
     }
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
+    if constexpr (G > kWave) {
+      part = group_sum<R, G>(part, red, *par);
+    } else {
+      int p0 = 0;
+      part = group_sum<R, G>(part, nullptr, p0);
+    }
     const R delta = cst + part;
     bool ok;
     if (sizeof(R) == 4)
@@ -168,14 +175,14 @@ __device__ __forceinline__ uint64_t sweep_chain_c(const SampleArgs& A, int chain
     if (ok) {
 #pragma unroll
       for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
+        const int j = gl + G * r;
         if (j < m) {
           tr[r] = ur[r];
           ti[r] = ui[r];
           lc[r] = lt[r];
         }
       }
-      sig[k] = (int8_t)-sk;  // every lane stores the same byte
+      sig[k] = (int8_t)-sk;  // every thread stores the same byte
       if (KIND == 1) sig[l] = (int8_t)-sl;
       ++nacc;
     }
@@ -189,39 +196,42 @@ __device__ __forceinline__ void wave_sync_c() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename R, int KIND, int RMAX>
-__global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
-  extern __shared__ int8_t lds_sig[];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int chain = blockIdx.x * 4 + wave;
-  if (chain >= A.C) return;  // wave-uniform; no block barrier follows
+// One chain run by a G-thread chain group (one wavefront, or the whole block
+// when G > 64): sig is the group's LDS spin strip, gl the thread's index in
+// the group, red group_sum's LDS exchange.
+template <typename R, int KIND, int RMAX, int G = kWave>
+__device__ void run_chain_c(const SampleArgs& A, int chain, int8_t* sig, int gl,
+                            R* red = nullptr) {
   const int n = A.n, m = A.m;
-  int8_t* sig = lds_sig + wave * A.npad;
-  for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
-  wave_sync_c();
+  for (int i = gl; i < n; i += G) sig[i] = A.sigma[(size_t)chain * n + i];
+  if (G > kWave)
+    __syncthreads();
+  else
+    wave_sync_c();
+  int par = 0;
   R tr[RMAX], ti[RMAX], lc[RMAX], ur[RMAX], ui[RMAX], lt[RMAX];
   uint64_t ev = A.ctr[chain];
   uint64_t nacc = 0;
   const R beta = (R)A.beta;
-  refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
+  refresh_c<R, RMAX, G>(A, sig, gl, tr, ti, lc);
   bool fresh = true;
   const int total = A.n_discard + A.n_sweeps;
   for (int sw = 0; sw < total; ++sw) {
-    if (A.mixed && !fresh) refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
+    if (A.mixed && !fresh) refresh_c<R, RMAX, G>(A, sig, gl, tr, ti, lc);
     fresh = false;
-    nacc += sweep_chain_c<R, KIND, RMAX>(A, chain, sig, lane, tr, ti, lc, ur, ui, lt, ev, beta);
+    nacc += sweep_chain_c<R, KIND, RMAX, G>(A, chain, sig, gl, tr, ti, lc, ur, ui, lt, ev, beta,
+                                            red, &par);
     if (sw >= A.n_discard) {
       const size_t row = (size_t)(sw - A.n_discard) * A.C + chain;
-      for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+      for (int i = gl; i < n; i += G) A.out_sigma[row * n + i] = sig[i];
       if (A.mixed) {
-        refresh_c<R, RMAX>(A, sig, lane, tr, ti, lc);
+        refresh_c<R, RMAX, G>(A, sig, gl, tr, ti, lc);
         fresh = true;
       }
       R* ot = static_cast<R*>(A.out_theta) + 2 * row * m;
 #pragma unroll
       for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
+        const int j = gl + G * r;
         if (j < m) {
           ot[2 * j] = tr[r];
           ot[2 * j + 1] = ti[r];
@@ -229,11 +239,31 @@ __global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
       }
     }
   }
-  for (int i = lane; i < n; i += kWave) A.sigma[(size_t)chain * n + i] = sig[i];
-  if (lane == 0) {
+  for (int i = gl; i < n; i += G) A.sigma[(size_t)chain * n + i] = sig[i];
+  if (gl == 0) {
     A.ctr[chain] = ev;
     A.acc[chain] += nacc;
   }
+}
+
+template <typename R, int KIND, int RMAX>
+__global__ void __launch_bounds__(256) k_sample_c(const SampleArgs A) {
+  extern __shared__ int8_t lds_sig[];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int chain = blockIdx.x * 4 + wave;
+  if (chain >= A.C) return;  // wave-uniform; no block barrier follows
+  run_chain_c<R, KIND, RMAX>(A, chain, lds_sig + wave * A.npad, lane);
+}
+
+// m > 512: one chain per block of G threads (LDS: the spin strip, then
+// group_sum's exchange of 2 G values)
+template <typename R, int KIND, int RMAX, int G>
+__global__ void __launch_bounds__(G) k_sample_c_wide(const SampleArgs A) {
+  extern __shared__ __align__(16) unsigned char lds_w[];
+  R* red = reinterpret_cast<R*>(lds_w + ((A.npad + 15) & ~15));
+  run_chain_c<R, KIND, RMAX, G>(A, blockIdx.x, reinterpret_cast<int8_t*>(lds_w), threadIdx.x,
+                                red);
 }
 
 // Parallel tempering of complex machines (SamplerMT.hpp:44-60, 112-204 with
@@ -258,6 +288,8 @@ __global__ void __launch_bounds__(256) k_sample_pt_c(const SampleArgs A) {
   R* thr = reinterpret_cast<R*>(lds_raw + 448 + ((T * npad + 15) & ~15));  // T x m
   R* thi = thr + (size_t)T * m;                                            // T x m
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int G = kWave;  // one wave per temperature slot
+  const int gl = lane;
   const int g = blockIdx.x;
   const int groups = A.C / T;
   const R* ar = static_cast<const R*>(A.ar);
@@ -265,7 +297,7 @@ __global__ void __launch_bounds__(256) k_sample_pt_c(const SampleArgs A) {
   auto keep = [&](int st) {
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       if (j < m) {
         thr[st * m + j] = tr[r];
         thi[st * m + j] = ti[r];
@@ -297,7 +329,7 @@ __global__ void __launch_bounds__(256) k_sample_pt_c(const SampleArgs A) {
       } else {
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) {
-          const int j = lane + kWave * r;
+          const int j = gl + G * r;
           tr[r] = j < m ? thr[st * m + j] : (R)0;
           ti[r] = j < m ? thi[st * m + j] : (R)0;
           lc[r] = j < m ? lc_re<R>(tr[r], ti[r]) : (R)0;
@@ -354,7 +386,7 @@ __global__ void __launch_bounds__(256) k_sample_pt_c(const SampleArgs A) {
       R* ot = static_cast<R*>(A.out_theta) + 2 * row * m;
 #pragma unroll
       for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
+        const int j = gl + G * r;
         if (j < m) {
           ot[2 * j] = A.mixed ? tr[r] : thr[st * m + j];
           ot[2 * j + 1] = A.mixed ? ti[r] : thi[st * m + j];
@@ -390,8 +422,24 @@ void launch_pt_c(const SampleArgs& a, hipStream_t st) {
   }
 }
 
+template <typename R, int KIND, int G>
+void launch_rc_wide(const SampleArgs& a, hipStream_t st) {
+  const size_t lds = (size_t)((a.npad + 15) & ~15) + 2 * G * sizeof(R);
+  if (a.m <= 6 * G)
+    hipLaunchKernelGGL((k_sample_c_wide<R, KIND, 6, G>), dim3(a.C), dim3(G), lds, st, a);
+  else
+    hipLaunchKernelGGL((k_sample_c_wide<R, KIND, 8, G>), dim3(a.C), dim3(G), lds, st, a);
+}
+
 template <typename R, int KIND>
 void launch_rc(const SampleArgs& a, hipStream_t st) {
+  switch (chain_group(a.m)) {
+    case kWave: break;
+    case 128: launch_rc_wide<R, KIND, 128>(a, st); return;
+    case 256: launch_rc_wide<R, KIND, 256>(a, st); return;
+    case 512: launch_rc_wide<R, KIND, 512>(a, st); return;
+    default: launch_rc_wide<R, KIND, 1024>(a, st); return;
+  }
   const dim3 grid((a.C + 3) / 4), block(256);
   const size_t lds = (size_t)4 * a.npad;
   switch ((a.m + kWave - 1) / kWave) {
@@ -437,14 +485,18 @@ __device__ inline void sc_c(float x, float* s, float* c) { sincos_f(x, s, c); }
 
 // One thread per (sample, bond): the complex log-ratio of RBMState.hpp:48-63
 // / 78-93 in the reference's j order; ratio = exp of it (complex double).
-template <typename R>
+// STAGED: theta and logcosh(theta) of the block's samples in LDS; else (large
+// m) theta is read from global memory and logcosh(theta_j) recomputed (the
+// same bits).
+template <typename R, bool STAGED>
 __global__ void __launch_bounds__(256) k_eloc_ratio_c(const ElocArgs A, int span) {
   extern __shared__ unsigned char lds_raw[];
+  const size_t sm = STAGED ? (size_t)span * A.m : 0;
   R* thr = reinterpret_cast<R*>(lds_raw);
-  R* thi = thr + (size_t)span * A.m;
-  R* lcr = thi + (size_t)span * A.m;
-  R* lci = lcr + (size_t)span * A.m;
-  int8_t* sg = reinterpret_cast<int8_t*>(lci + (size_t)span * A.m);
+  R* thi = thr + sm;
+  R* lcr = thi + sm;
+  R* lci = lcr + sm;
+  int8_t* sg = reinterpret_cast<int8_t*>(lci + sm);
   const int n = A.n, m = A.m, B = A.B;
   const size_t first = (size_t)blockIdx.x * 256;
   const size_t total = (size_t)A.ns * B;
@@ -453,12 +505,13 @@ __global__ void __launch_bounds__(256) k_eloc_ratio_c(const ElocArgs A, int span
   const int s_hi = (int)(last / B);
   const int cnt = s_hi - s_lo + 1;
   const R* theta = static_cast<const R*>(A.theta);
-  for (int q = threadIdx.x; q < cnt * m; q += 256) {
-    const R x = theta[2 * ((size_t)s_lo * m + q)], y = theta[2 * ((size_t)s_lo * m + q) + 1];
-    thr[q] = x;
-    thi[q] = y;
-    lc_c(x, y, &lcr[q], &lci[q]);
-  }
+  if (STAGED)
+    for (int q = threadIdx.x; q < cnt * m; q += 256) {
+      const R x = theta[2 * ((size_t)s_lo * m + q)], y = theta[2 * ((size_t)s_lo * m + q) + 1];
+      thr[q] = x;
+      thi[q] = y;
+      lc_c(x, y, &lcr[q], &lci[q]);
+    }
   for (int q = threadIdx.x; q < cnt * n; q += 256) sg[q] = A.sigma[(size_t)s_lo * n + q];
   __syncthreads();
   const size_t idx = first + threadIdx.x;
@@ -492,17 +545,26 @@ __global__ void __launch_bounds__(256) k_eloc_ratio_c(const ElocArgs A, int span
     rr = rr - ((R)2 * ar[l]) * (R)sl;
     ri = ri - ((R)2 * ai[l]) * (R)sl;
   }
+  const R* tg = theta + 2 * (size_t)s * m;  // unstaged: this sample's theta
   for (int j = 0; j < m; ++j) {
-    R x = tsr[j] - twok * WTr[(size_t)j * n + k];
-    R y = tsi[j] - twok * WTi[(size_t)j * n + k];
+    const R t0r = STAGED ? tsr[j] : tg[2 * j], t0i = STAGED ? tsi[j] : tg[2 * j + 1];
+    R x = t0r - twok * WTr[(size_t)j * n + k];
+    R y = t0i - twok * WTi[(size_t)j * n + k];
     if (l >= 0) {
       x = x - twol * WTr[(size_t)j * n + l];
       y = y - twol * WTi[(size_t)j * n + l];
     }
     R lr, li;
     lc_c(x, y, &lr, &li);
-    rr = rr + (lr - lsr[j]);
-    ri = ri + (li - lsi[j]);
+    R l0r, l0i;
+    if (STAGED) {
+      l0r = lsr[j];
+      l0i = lsi[j];
+    } else {
+      lc_c(t0r, t0i, &l0r, &l0i);
+    }
+    rr = rr + (lr - l0r);
+    ri = ri + (li - l0i);
   }
   R sn, cs;
   sc_c(ri, &sn, &cs);
@@ -718,10 +780,19 @@ void launch_theta_batch_c(const double* W64, const double* b64, const int8_t* si
                        sigma, ns, n, m, static_cast<double*>(out));
 }
 
+// theta staging (Re, Im and their logcosh) of a block's samples fits the
+// LDS budget
+static bool eloc_c_staged(const ElocArgs& a) {
+  const int span = 256 / a.B + 2;
+  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
+  return (size_t)span * a.m * 4 * rs <= kElocStageBytes;
+}
+
 size_t eloc_c_lds_bytes(const ElocArgs& a) {
   const int span = 256 / a.B + 2;
   const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
-  return (size_t)span * a.m * 4 * rs + (size_t)span * a.n + 16;
+  const size_t stage = eloc_c_staged(a) ? (size_t)span * a.m * 4 * rs : 0;
+  return stage + (size_t)span * a.n + 16;
 }
 
 void launch_eloc_c(const ElocArgs& a, hipStream_t st) {
@@ -731,10 +802,14 @@ void launch_eloc_c(const ElocArgs& a, hipStream_t st) {
   const dim3 grid((unsigned)((total + 255) / 256)), block(256);
   {
     ProfGuard g("eloc_ratio_c", st);
-    if (a.use_float)
-      hipLaunchKernelGGL(k_eloc_ratio_c<float>, grid, block, lds, st, a, span);
-    else
-      hipLaunchKernelGGL(k_eloc_ratio_c<double>, grid, block, lds, st, a, span);
+    const bool staged = eloc_c_staged(a);
+    if (a.use_float) {
+      if (staged) hipLaunchKernelGGL((k_eloc_ratio_c<float, true>), grid, block, lds, st, a, span);
+      else hipLaunchKernelGGL((k_eloc_ratio_c<float, false>), grid, block, lds, st, a, span);
+    } else {
+      if (staged) hipLaunchKernelGGL((k_eloc_ratio_c<double, true>), grid, block, lds, st, a, span);
+      else hipLaunchKernelGGL((k_eloc_ratio_c<double, false>), grid, block, lds, st, a, span);
+    }
   }
   launch_eloc_combine_c(a, st);
 }

@@ -7,6 +7,10 @@
 // few samples it touches in LDS, then each thread walks the hidden units in
 // index order exactly like RBMStateObj::logRatio (RBMState.hpp:54-62 /
 // 82-91), reading W^T rows that are contiguous across the bonds of a sample.
+// When the staged theta of a block's samples would not fit in 48 KB of LDS
+// (large m), the unstaged form reads theta from global memory and recomputes
+// logcosh(theta_j) next to logcosh(t_j): the same operations on the same
+// values, so the same bits.
 // Pass 2 (k_eloc_combine): one thread per sample accumulates the diagonal and
 // ratio terms in the functor's order, in double.
 #include "yq_internal.h"
@@ -19,12 +23,14 @@ int num_bonds(const yq_ham_desc& h) { return h.kind == YQ_J1J2 ? 2 * h.n : h.n; 
 namespace {
 
 // One 256-thread block of (sample, bond) pairs starting at pair `first`.
-template <typename R>
+// STAGED: theta and logcosh(theta) of the block's samples in LDS; else only
+// the spins are staged.
+template <typename R, bool STAGED = true>
 __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
                                  unsigned char* lds_raw) {
   R* th = reinterpret_cast<R*>(lds_raw);
-  R* lc = th + (size_t)span * A.m;
-  int8_t* sg = reinterpret_cast<int8_t*>(lc + (size_t)span * A.m);
+  R* lc = th + (STAGED ? (size_t)span * A.m : 0);
+  int8_t* sg = reinterpret_cast<int8_t*>(lc + (STAGED ? (size_t)span * A.m : 0));
   const int n = A.n, m = A.m, B = A.B;
   const size_t total = (size_t)A.ns * B;
   const int s_lo = (int)(first / B);
@@ -32,11 +38,12 @@ __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
   const int s_hi = (int)(last / B);
   const int cnt = s_hi - s_lo + 1;
   const R* theta = static_cast<const R*>(A.theta);
-  for (int q = threadIdx.x; q < cnt * m; q += 256) {
-    const R t = theta[(size_t)s_lo * m + q];
-    th[q] = t;
-    lc[q] = Math<R>::logcosh(t);
-  }
+  if (STAGED)
+    for (int q = threadIdx.x; q < cnt * m; q += 256) {
+      const R t = theta[(size_t)s_lo * m + q];
+      th[q] = t;
+      lc[q] = Math<R>::logcosh(t);
+    }
   for (int q = threadIdx.x; q < cnt * n; q += 256)
     sg[q] = A.sigma[(size_t)s_lo * n + q];
   __syncthreads();
@@ -59,12 +66,13 @@ __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
   const int8_t sk = sgs[k];
   const R twok = sk > 0 ? (R)2 : (R)-2;
   R res = ((R)-2 * ar[k]) * (R)sk;
-  const R* ths = th + (size_t)ls * m;
+  const R* ths = STAGED ? th + (size_t)ls * m : theta + (size_t)s * m;
   const R* lcs = lc + (size_t)ls * m;
+  auto lc_of = [&](int j) { return STAGED ? lcs[j] : Math<R>::logcosh(ths[j]); };
   if (l < 0) {
     for (int j = 0; j < m; ++j) {
       const R t = ths[j] - twok * WT[(size_t)j * n + k];
-      res = res + (Math<R>::logcosh(t) - lcs[j]);
+      res = res + (Math<R>::logcosh(t) - lc_of(j));
     }
   } else {
     const int8_t sl = sgs[l];
@@ -73,7 +81,7 @@ __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
     for (int j = 0; j < m; ++j) {
       R t = ths[j] - twok * WT[(size_t)j * n + k];
       t = t - twol * WT[(size_t)j * n + l];
-      res = res + (Math<R>::logcosh(t) - lcs[j]);
+      res = res + (Math<R>::logcosh(t) - lc_of(j));
     }
   }
   if (A.ap_ffs) {
@@ -88,10 +96,10 @@ __device__ void eloc_ratio_block(const ElocArgs& A, int span, size_t first,
   A.ratio[idx] = (double)Math<R>::exp(res);
 }
 
-template <typename R>
+template <typename R, bool STAGED>
 __global__ void __launch_bounds__(256) k_eloc_ratio(const ElocArgs A, int span) {
   extern __shared__ unsigned char lds_raw[];
-  eloc_ratio_block<R>(A, span, (size_t)blockIdx.x * 256, lds_raw);
+  eloc_ratio_block<R, STAGED>(A, span, (size_t)blockIdx.x * 256, lds_raw);
 }
 
 // many machines: blockIdx.y selects the machine (its own span of samples per block)
@@ -184,17 +192,27 @@ __global__ void __launch_bounds__(1024) k_eloc_stats_multi(const ElocArgs* __res
 
 }  // namespace
 
+// theta staging of a block's samples fits the LDS budget
+bool eloc_staged(const ElocArgs& a) {
+  const int span = 256 / a.B + 2;
+  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
+  return (size_t)span * a.m * 2 * rs <= kElocStageBytes;
+}
+
 void launch_eloc_ratio(const ElocArgs& a, hipStream_t st) {
   const size_t total = (size_t)a.ns * a.B;
   const int span = 256 / a.B + 2;
-  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
-  const size_t lds = (size_t)span * a.m * 2 * rs + (size_t)span * a.n + 16;
+  const size_t lds = eloc_lds_bytes(a);
   const dim3 grid((unsigned)((total + 255) / 256)), block(256);
   ProfGuard g("eloc_ratio", st);
-  if (a.use_float)
-    hipLaunchKernelGGL(k_eloc_ratio<float>, grid, block, lds, st, a, span);
-  else
-    hipLaunchKernelGGL(k_eloc_ratio<double>, grid, block, lds, st, a, span);
+  const bool staged = eloc_staged(a);
+  if (a.use_float) {
+    if (staged) hipLaunchKernelGGL((k_eloc_ratio<float, true>), grid, block, lds, st, a, span);
+    else hipLaunchKernelGGL((k_eloc_ratio<float, false>), grid, block, lds, st, a, span);
+  } else {
+    if (staged) hipLaunchKernelGGL((k_eloc_ratio<double, true>), grid, block, lds, st, a, span);
+    else hipLaunchKernelGGL((k_eloc_ratio<double, false>), grid, block, lds, st, a, span);
+  }
 }
 
 void launch_eloc(const ElocArgs& a, hipStream_t st) {
@@ -203,10 +221,17 @@ void launch_eloc(const ElocArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_eloc_combine, dim3((a.ns + 255) / 256), dim3(256), 0, st, a);
 }
 
-size_t eloc_lds_bytes(const ElocArgs& a) {
+size_t eloc_stage_lds_bytes(const ElocArgs& a) {
   const int span = 256 / a.B + 2;
   const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
   return (size_t)span * a.m * 2 * rs + (size_t)span * a.n + 16;
+}
+
+size_t eloc_lds_bytes(const ElocArgs& a) {
+  const int span = 256 / a.B + 2;
+  const size_t rs = a.use_float ? sizeof(float) : sizeof(double);
+  const size_t stage = eloc_staged(a) ? (size_t)span * a.m * 2 * rs : 0;
+  return stage + (size_t)span * a.n + 16;
 }
 
 void launch_eloc_multi(const ElocArgs* inst_dev, int count, int max_blocks, int max_ns,

@@ -3,6 +3,9 @@
 // One wavefront owns one Markov chain.  Hidden unit j lives in lane j % 64,
 // register slot j / 64, so theta_j and logcosh(theta_j) stay in VGPRs for the
 // whole launch; the spins of the chain sit in a 1-byte-per-site LDS strip.
+// Past m = 512 a chain takes a whole block of G = 64 NW threads (unit j in
+// thread j % G, slot j / G, still at most 8 slots; chain_group), and the
+// log-ratio's tree over G lanes crosses the waves through LDS (group_sum).
 // A proposal costs one Philox4x32-10 block (counter = the chain's proposal
 // index), one coalesced read of the flipped column(s) of W (L2 resident),
 // ceil(m/64) logcosh evaluations per lane, and a 6-level xor-butterfly that
@@ -25,9 +28,10 @@ namespace yq {
 namespace {
 
 // theta_j = (sum_i +/-W_ji, index order, fp64) + b_j, then rounded to R, and
-// lc_j = logcosh(theta_j), for the hidden units of this lane
-template <typename R, int RMAX>
-__device__ __forceinline__ void refresh_theta(const SampleArgs& A, const int8_t* sig, int lane,
+// lc_j = logcosh(theta_j), for the hidden units of thread gl of a G-thread
+// chain group
+template <typename R, int RMAX, int G = kWave>
+__device__ __forceinline__ void refresh_theta(const SampleArgs& A, const int8_t* sig, int gl,
                                               R (&th)[RMAX], R (&lc)[RMAX]) {
   const int n = A.n, m = A.m;
   double accd[RMAX];
@@ -38,7 +42,7 @@ __device__ __forceinline__ void refresh_theta(const SampleArgs& A, const int8_t*
     const double* wc = A.W64 + (size_t)i * m;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       if (j < m) {
         const double w = wc[j];
         accd[r] = up ? accd[r] + w : accd[r] - w;
@@ -47,7 +51,7 @@ __device__ __forceinline__ void refresh_theta(const SampleArgs& A, const int8_t*
   }
 #pragma unroll
   for (int r = 0; r < RMAX; ++r) {
-    const int j = lane + kWave * r;
+    const int j = gl + G * r;
     if (j < m) {
       th[r] = (R)(accd[r] + A.b64[j]);
       lc[r] = Math<R>::logcosh(th[r]);
@@ -59,12 +63,17 @@ __device__ __forceinline__ void refresh_theta(const SampleArgs& A, const int8_t*
 }
 
 // One sweep (A.nprop proposals) of one chain at inverse temperature beta;
-// returns the accepted count.  ev is the chain's proposal counter.
-template <typename R, int KIND, int RMAX>
+// returns the accepted count.  ev is the chain's proposal counter.  gl is
+// the thread's index in its G-thread chain group; red and par are
+// group_sum's LDS exchange and parity (unused when G = 64).  With G > 64
+// every wave of the block stores each accepted flip itself (the same byte),
+// so every wave reads the spins it wrote; the barrier inside group_sum
+// orders each proposal's reads before any wave's flip of that proposal.
+template <typename R, int KIND, int RMAX, int G = kWave>
 __device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, int8_t* sig,
-                                                int lane, R (&th)[RMAX], R (&lc)[RMAX],
+                                                int gl, R (&th)[RMAX], R (&lc)[RMAX],
                                                 R (&tt)[RMAX], R (&lt)[RMAX], uint64_t& ev,
-                                                R beta) {
+                                                R beta, R* red = nullptr, int* par = nullptr) {
   const int n = A.n, m = A.m;
   const R* Wr = static_cast<const R*>(A.Wr);
   const R* ar = static_cast<const R*>(A.ar);
@@ -98,7 +107,7 @@ __device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, 
     R part = (R)0;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       if (j < m) {
         R t = th[r] - twok * wk[j];
         if (KIND == 1) t = t - twol * wl[j];
@@ -107,8 +116,12 @@ __device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, 
         part = part + (lt[r] - lc[r]);
       }
     }
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) part = part + __shfl_xor(part, s);
+    if constexpr (G > kWave) {
+      part = group_sum<R, G>(part, red, *par);
+    } else {
+      int p0 = 0;
+      part = group_sum<R, G>(part, nullptr, p0);
+    }
     const R delta = cst + part;
     bool ok;
     if (sizeof(R) == 4)
@@ -118,13 +131,13 @@ __device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, 
     if (ok) {
 #pragma unroll
       for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
+        const int j = gl + G * r;
         if (j < m) {
           th[r] = tt[r];
           lc[r] = lt[r];
         }
       }
-      sig[k] = (int8_t)-sk;  // every lane stores the same byte
+      sig[k] = (int8_t)-sk;  // every thread stores the same byte
       if (KIND == 1) sig[l] = (int8_t)-sl;
       ++nacc;
     }
@@ -134,23 +147,23 @@ __device__ __forceinline__ uint64_t sweep_chain(const SampleArgs& A, int chain, 
 
 // sample row `row`: spins and theta (mixed precision: theta refreshed from
 // the fp64 parameters first, fresh is then set)
-template <typename R, int RMAX>
+template <typename R, int RMAX, int G = kWave>
 __device__ __forceinline__ void record_row(const SampleArgs& A, size_t row, const int8_t* sig,
-                                           int lane, R (&th)[RMAX], R (&lc)[RMAX],
+                                           int gl, R (&th)[RMAX], R (&lc)[RMAX],
                                            bool& fresh) {
   const int n = A.n, m = A.m;
-  for (int i = lane; i < n; i += kWave) A.out_sigma[row * n + i] = sig[i];
+  for (int i = gl; i < n; i += G) A.out_sigma[row * n + i] = sig[i];
   // mixed precision with the MFMA theta pass: the pass that follows the
   // launch records theta refreshed from FP64 (one sweep per launch)
   if (A.mixed && A.theta_in) return;
   if (A.mixed) {
-    refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+    refresh_theta<R, RMAX, G>(A, sig, gl, th, lc);
     fresh = true;
   }
   R* ot = static_cast<R*>(A.out_theta) + row * m;
 #pragma unroll
   for (int r = 0; r < RMAX; ++r) {
-    const int j = lane + kWave * r;
+    const int j = gl + G * r;
     if (j < m) ot[j] = th[r];
   }
 }
@@ -161,14 +174,21 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One chain of one machine, run by one wavefront: sig is the wave's LDS spin
-// strip.  RMAX >= ceil(m/64); larger values only add predicated-off slots,
-// so the arithmetic (and the replay) does not depend on it.
-template <typename R, int KIND, int RMAX>
-__device__ void run_chain(const SampleArgs& A, int chain, int8_t* sig, int lane) {
+// One chain of one machine, run by a G-thread chain group (one wavefront,
+// or the whole block when G > 64): sig is the group's LDS spin strip, gl the
+// thread's index in the group, red group_sum's LDS exchange.  RMAX >=
+// ceil(m/G); larger values only add predicated-off slots, so the arithmetic
+// (and the replay) does not depend on it.
+template <typename R, int KIND, int RMAX, int G = kWave>
+__device__ void run_chain(const SampleArgs& A, int chain, int8_t* sig, int gl,
+                          R* red = nullptr) {
   const int n = A.n;
-  for (int i = lane; i < n; i += kWave) sig[i] = A.sigma[(size_t)chain * n + i];
-  wave_sync();
+  for (int i = gl; i < n; i += G) sig[i] = A.sigma[(size_t)chain * n + i];
+  if (G > kWave)
+    __syncthreads();
+  else
+    wave_sync();
+  int par = 0;
   R th[RMAX], lc[RMAX], tt[RMAX], lt[RMAX];
   uint64_t ev = A.ctr[chain];
   uint64_t nacc = 0;
@@ -177,28 +197,29 @@ __device__ void run_chain(const SampleArgs& A, int chain, int8_t* sig, int lane)
     const R* ti = static_cast<const R*>(A.theta_in) + (size_t)chain * A.m;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       th[r] = j < A.m ? ti[j] : (R)0;
       lc[r] = j < A.m ? Math<R>::logcosh(th[r]) : (R)0;
     }
   } else {
-    refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+    refresh_theta<R, RMAX, G>(A, sig, gl, th, lc);
   }
   bool fresh = true;
   const int total = A.n_discard + A.n_sweeps;
   for (int sw = 0; sw < total; ++sw) {
-    if (A.mixed && !fresh) refresh_theta<R, RMAX>(A, sig, lane, th, lc);
+    if (A.mixed && !fresh) refresh_theta<R, RMAX, G>(A, sig, gl, th, lc);
     fresh = false;
-    nacc += sweep_chain<R, KIND, RMAX>(A, chain, sig, lane, th, lc, tt, lt, ev, beta);
+    nacc += sweep_chain<R, KIND, RMAX, G>(A, chain, sig, gl, th, lc, tt, lt, ev, beta, red, &par);
     if (sw >= A.n_discard)
-      record_row<R, RMAX>(A, (size_t)(sw - A.n_discard) * A.C + chain, sig, lane, th, lc, fresh);
+      record_row<R, RMAX, G>(A, (size_t)(sw - A.n_discard) * A.C + chain, sig, gl, th, lc,
+                             fresh);
   }
-  for (int i = lane; i < n; i += kWave) A.sigma[(size_t)chain * n + i] = sig[i];
-  if (lane == 0) {
+  for (int i = gl; i < n; i += G) A.sigma[(size_t)chain * n + i] = sig[i];
+  if (gl == 0) {
     A.ctr[chain] = ev;
     A.acc[chain] += nacc;
   }
-  wave_sync();  // the strip is reused by the wave's next chain
+  if (G == kWave) wave_sync();  // the strip is reused by the wave's next chain
 }
 
 // Parallel tempering (SamplerMT.hpp:44-60, 112-204 with nTmps = T > 1): one
@@ -228,6 +249,8 @@ __global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
   int8_t* strips = reinterpret_cast<int8_t*>(lds_raw + 448);        // T x npad
   R* ths = reinterpret_cast<R*>(lds_raw + 448 + ((T * npad + 15) & ~15));  // T x m
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int G = kWave;  // one wave per temperature slot
+  const int gl = lane;
   const int g = blockIdx.x;
   const int groups = A.C / T;
   const R* ar = static_cast<const R*>(A.ar);
@@ -241,7 +264,7 @@ __global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
     refresh_theta<R, RMAX>(A, sig, lane, th, lc);
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
-      const int j = lane + kWave * r;
+      const int j = gl + G * r;
       if (j < m) ths[t * m + j] = th[r];
     }
     if (lane == 0) {
@@ -262,7 +285,7 @@ __global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
       } else {
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) {
-          const int j = lane + kWave * r;
+          const int j = gl + G * r;
           th[r] = j < m ? ths[st * m + j] : (R)0;
           lc[r] = j < m ? Math<R>::logcosh(th[r]) : (R)0;
         }
@@ -273,7 +296,7 @@ __global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
                                                       beta);
 #pragma unroll
       for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
+        const int j = gl + G * r;
         if (j < m) ths[st * m + j] = th[r];
       }
       // lp = a . sigma + sum_j logcosh(theta_j)
@@ -281,7 +304,7 @@ __global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
       for (int i = lane; i < n; i += kWave) sa = sa + ar[i] * (R)sig[i];
 #pragma unroll
       for (int r = 0; r < RMAX; ++r)
-        if (lane + kWave * r < m) sl = sl + lc[r];
+        if (gl + G * r < m) sl = sl + lc[r];
 #pragma unroll
       for (int s = 32; s >= 1; s >>= 1) {
         sa = sa + __shfl_xor(sa, s);
@@ -323,7 +346,7 @@ __global__ void __launch_bounds__(256) k_sample_pt(const SampleArgs A) {
       R* ot = static_cast<R*>(A.out_theta) + row * m;
 #pragma unroll
       for (int r = 0; r < RMAX; ++r) {
-        const int j = lane + kWave * r;
+        const int j = gl + G * r;
         if (j < m) ot[j] = A.mixed ? th[r] : ths[st * m + j];
       }
     }
@@ -350,6 +373,15 @@ __global__ void __launch_bounds__(256) k_sample(const SampleArgs A) {
   run_chain<R, KIND, RMAX>(A, chain, lds_sig + wave * A.npad, lane);
 }
 
+// m > 512: one chain per block of G threads.  LDS: the spin strip (npad
+// bytes), then group_sum's exchange (2 G values of R).
+template <typename R, int KIND, int RMAX, int G>
+__global__ void __launch_bounds__(G) k_sample_wide(const SampleArgs A) {
+  extern __shared__ __align__(16) unsigned char lds_w[];
+  R* red = reinterpret_cast<R*>(lds_w + ((A.npad + 15) & ~15));
+  run_chain<R, KIND, RMAX, G>(A, blockIdx.x, reinterpret_cast<int8_t*>(lds_w), threadIdx.x, red);
+}
+
 // Many machines in one persistent launch: every wave walks the list of
 // (machine, chain) items with a grid-sized stride.  The list is ordered by
 // decreasing cost (n * m * sweep factor), so the strided walk hands every
@@ -366,12 +398,29 @@ __global__ void __launch_bounds__(256)
   for (int q = blockIdx.x * 4 + wave; q < n_items; q += nw) {
     const int2 it = items[q];
     const SampleArgs A = inst[it.x];
-    run_chain<R, KIND, kMaxHidden / kWave>(A, it.y, lds_sig + wave * npad, lane);
+    run_chain<R, KIND, kWaveHidden / kWave>(A, it.y, lds_sig + wave * npad, lane);
   }
+}
+
+template <typename R, int KIND, int G>
+void launch_wide(const SampleArgs& a, hipStream_t st) {
+  const size_t lds = (size_t)((a.npad + 15) & ~15) + 2 * G * sizeof(R);
+  // 6 slots when they suffice, else 8 (at most a third of the slots idle)
+  if (a.m <= 6 * G)
+    hipLaunchKernelGGL((k_sample_wide<R, KIND, 6, G>), dim3(a.C), dim3(G), lds, st, a);
+  else
+    hipLaunchKernelGGL((k_sample_wide<R, KIND, 8, G>), dim3(a.C), dim3(G), lds, st, a);
 }
 
 template <typename R, int KIND>
 void launch_r(const SampleArgs& a, hipStream_t st) {
+  switch (chain_group(a.m)) {
+    case kWave: break;
+    case 128: launch_wide<R, KIND, 128>(a, st); return;
+    case 256: launch_wide<R, KIND, 256>(a, st); return;
+    case 512: launch_wide<R, KIND, 512>(a, st); return;
+    default: launch_wide<R, KIND, 1024>(a, st); return;
+  }
   const dim3 grid((a.C + 3) / 4), block(256);
   const size_t lds = (size_t)4 * a.npad;
   // exactly ceil(m/64) register slots, so no slot is computed for nothing

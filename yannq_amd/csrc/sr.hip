@@ -167,7 +167,7 @@ __global__ void k_add_diag(double* M, int dim, double shift) {
 // produced on the fly from the int8 spins, T and the parameter vector.
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kF64Tiles = 4;                     // column tiles per forward group
-constexpr int kMaxIt64 = (kMaxSites + 1 + 15) / 16;
+constexpr int kMaxIt64 = 9;                      // row tiles per backward wave
 
 __device__ inline double sig_aug(const int8_t* __restrict__ sigma, int ns, int n, int s, int i) {
   if (s >= ns || i > n) return 0.0;
@@ -230,7 +230,7 @@ __global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows
 }
 
 // part[kc][i][j] = sum_{s in chunk kc} sigma'_si r_s T'_sj ; one wave per
-// (column tile, chunk)
+// (column tile, chunk, group of up to kMaxIt64 row tiles: blockIdx.z)
 __global__ void __launch_bounds__(256)
     k_bwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
                  int m, const double* __restrict__ r, int Kp, int Np, int chunk,
@@ -240,7 +240,8 @@ __global__ void __launch_bounds__(256)
   if (ct >= Np / 16) return;
   const int kc = blockIdx.y;
   const int li = l & 15, lg = l >> 4;
-  const int nit = Kp / 16;
+  const int it0 = blockIdx.z * kMaxIt64;  // first row tile of this group
+  const int nit = min(Kp / 16 - it0, kMaxIt64);
   const int j = ct * 16 + li;
   doublex4 acc[kMaxIt64];
 #pragma unroll
@@ -254,15 +255,17 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int it = 0; it < kMaxIt64; ++it)
       if (it < nit)
-        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, s, it * 16 + li), b,
-                                                       acc[it], 0, 0, 0);
+        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, s,
+                                                               (it0 + it) * 16 + li),
+                                                       b, acc[it], 0, 0, 0);
   }
   double* out = part + (size_t)kc * Kp * Np;
 #pragma unroll
   for (int it = 0; it < kMaxIt64; ++it)
     if (it < nit)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) out[(size_t)(it * 16 + lg + 4 * q) * Np + j] = acc[it][q];
+      for (int q = 0; q < 4; ++q)
+        out[(size_t)((it0 + it) * 16 + lg + 4 * q) * Np + j] = acc[it][q];
 }
 
 // g[c] = scale * sum_k part[k][aug(c)] on the padded (Kp x Np) grid, mapped to
@@ -516,8 +519,9 @@ void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
   const int chunks = bwd64_chunks(ns);
   const int Kp = kp16(n), Np = np16(m);
   ProfGuard pg("factored_bwd64", st);
-  hipLaunchKernelGGL(k_bwd64_mfma, dim3((Np / 16 + 3) / 4, chunks), dim3(256), 0, st, sigma, T,
-                     ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial);
+  const int rgroups = (Kp / 16 + kMaxIt64 - 1) / kMaxIt64;
+  hipLaunchKernelGGL(k_bwd64_mfma, dim3((Np / 16 + 3) / 4, chunks, rgroups), dim3(256), 0, st,
+                     sigma, T, ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial);
   const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
   hipLaunchKernelGGL(k_reduce_aug, dim3(nblk(dim)), dim3(256), 0, st, partial, chunks, n, m,
                      use_bias, Kp, Np, 1.0, g);

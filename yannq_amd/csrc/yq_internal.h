@@ -50,8 +50,55 @@ void launch_prof(const char* label, F kernel, dim3 grid, dim3 block, size_t lds,
 }
 
 constexpr int kWave = 64;
-constexpr int kMaxHidden = 512;  // theta registers per lane: 8 per 64 lanes
-constexpr int kMaxSites = 128;
+// One wave per chain keeps up to 8 theta registers per lane (m <= 512); a
+// wider machine runs one chain per block of G = 64 NW threads (chain_group),
+// still at most 8 registers per thread, up to kMaxHidden hidden units.
+constexpr int kWaveHidden = 512;
+constexpr int kMaxHidden = 8192;
+constexpr int kMaxSites = 4096;
+
+// Threads that share one chain's hidden units for m hidden units: 64 (one
+// wave) up to m = 512, else 64 NW with NW the smallest power of two that
+// leaves at most 8 units per thread.  Hidden unit j lives in thread j % G of
+// the group; the chain's log-ratio is the canonical tree over G lanes.
+inline int chain_group(int m) {
+  int g = kWave;
+  while (g * 8 < m) g *= 2;
+  return g;
+}
+
+// Sum of v over the G threads of a chain group in the canonical order: the
+// tree with strides G/2, ..., 1 over the per-thread partials (what an xor
+// butterfly produces within one wave; addition is commutative, so every
+// thread ends with the same bits).  G = 64: the butterfly.  G > 64 (the
+// group is the whole block): the waves' partials meet in LDS (red: 2 G
+// values, alternating halves by call parity, so one barrier per call
+// suffices: a half is written again only after every wave has passed the
+// next call's barrier, i.e. finished reading it), every wave forms the tree
+// levels above 64 for its own lane, then the butterfly.  Every thread of the
+// block must call it.
+template <typename R, int G>
+__device__ __forceinline__ R group_sum(R v, R* red, int& par) {
+  if constexpr (G > kWave) {
+    R* buf = red + par * G;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    constexpr int NW = G / kWave;
+    R w[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) w[q] = buf[lane + kWave * q];
+#pragma unroll
+    for (int s = NW / 2; s >= 1; s /= 2)
+#pragma unroll
+      for (int q = 0; q < s; ++q) w[q] = w[q] + w[q + s];
+    v = w[0];
+    par ^= 1;
+  }
+#pragma unroll
+  for (int s = kWave / 2; s >= 1; s >>= 1) v = v + __shfl_xor(v, s);
+  return v;
+}
 
 // ------------------------------------------------------------------ sampler
 struct SampleArgs {
@@ -140,7 +187,14 @@ int num_bonds(const yq_ham_desc& h);
 void launch_eloc(const ElocArgs& a, hipStream_t st);
 // the ratio pass alone (ratios of every (sample, term) pair)
 void launch_eloc_ratio(const ElocArgs& a, hipStream_t st);
+// LDS of the single-machine ratio launch: theta and logcosh(theta) of the
+// block's samples are staged when they fit kElocStageBytes (eloc_staged),
+// else only the spins
+constexpr size_t kElocStageBytes = 48 * 1024;
+bool eloc_staged(const ElocArgs& a);
 size_t eloc_lds_bytes(const ElocArgs& a);
+// LDS of the staged form whatever its size (the many-machine launch)
+size_t eloc_stage_lds_bytes(const ElocArgs& a);
 // Many machines in one launch per pass: inst_dev holds one ElocArgs per
 // machine; stats[2 i], stats[2 i + 1] = mean and variance of machine i.
 void launch_eloc_multi(const ElocArgs* inst_dev, int count, int max_blocks, int max_ns,
