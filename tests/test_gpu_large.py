@@ -8,8 +8,9 @@ tree with strides G/2 .. 1).  E_loc of wide machines reads theta from global
 memory instead of staging it; it must keep the bits of the kernel replay.
 Chains of up to 4096 sites sample, form E_loc and the FP64 factored SR
 operator (whose backward product splits the rows into groups of 9 tiles) to
-the oracle's values.  The mixed factored solver keeps its n <= 159 limit and
-says so."""
+the oracle's values.  Past n = 159 the mixed factored operator runs its
+chunked forward and row-grouped backward products, to float accuracy against
+a dense FP64 operator, and its refined solve agrees with the FP64 one."""
 import numpy as np
 import pytest
 
@@ -160,15 +161,57 @@ def test_long_chain_sample_eloc_and_factored_operator(ctx, orc, yq):
     assert _rel(ctx.sr_apply(1e-3, v), Oc.T @ (Oc @ v) / ns + 1e-3 * v) < 1e-12
 
 
-def test_site_limit_4096_and_mixed_limit(ctx, yq):
+def test_site_limits(ctx, yq):
     ctx.rbm_create(4096, 8)
     with pytest.raises(yq.YqError):
         ctx.rbm_create(4097, 8)
     with pytest.raises(yq.YqError):
         ctx.rbm_create(16, 8193)
-    ctx.rbm_create(200, 16)
-    ctx.ham_set(0, 200, J=-1.0, h=-1.0)
-    rng = np.random.default_rng(1)
-    ctx.samples_set(rng.choice([-1, 1], size=(64, 200)).astype(np.int8), precision=1)
-    with pytest.raises(yq.YqError, match="n <= 159"):
+
+
+@pytest.mark.parametrize("n,m,ns", [(200, 48, 2048), (333, 40, 1536)])
+def test_mixed_factored_long_chain(ctx, orc, n, m, ns):
+    """n >= 160: the chunked forward product (K in chunks of 4 k-steps) and the
+    row-grouped backward product compute the operator to float accuracy for
+    both column-group widths and every backward chunk; the refined solve
+    reaches 1e-8 and agrees with the FP64 factored solve of the same batch; captured graphs
+    and eager launches give the same bits."""
+    print(f"mixed long chain n={n} m={m} ns={ns}", flush=True)
+    ctx.rbm_create(n, m)
+    rng = np.random.default_rng(n)
+    rbm = orc.RBM.random(n, m, 0.05, rng)
+    ctx.set_params(rbm.params())
+    ctx.ham_set(0, n, J=-1.0, h=-1.0)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig, precision=1)
+    th = orc.theta_replay(rbm, sig).astype(np.float32).astype(np.float64)
+    T = np.tanh(th)
+    Ow = (sig[:, :, None] * T[:, None, :]).reshape(ns, -1)
+    Od = np.concatenate([Ow, sig.astype(float), T], axis=1)
+    Oc = Od - Od.mean(axis=0)
+    v = np.random.default_rng(4).normal(size=Od.shape[1])
+    ref = Oc.T @ (Oc @ v) / ns + 1e-3 * v
+    for ct in (2, 4):
+        for chunk in (128, 256, 512):
+            ctx.tiles(ct, chunk)
+            ctx.sr_build(2, precision=1)
+            assert _rel(ctx.sr_apply(1e-3, v), ref) < 2e-6, (ct, chunk)
+    ctx.tiles(8, 256)
+    with pytest.raises(Exception):
         ctx.sr_build(2, precision=1)
+    ctx.tiles(0, 0)
+    ctx.sr_build(2, precision=1)
+    out = {}
+    for mode in (False, True):
+        ctx.graph_mode(mode)
+        out[mode] = ctx.sr_solve(1e-3, tol=1e-8, max_iter=20000)
+    ctx.graph_mode(True)
+    (xe, re_, ie), (xg, rg, ig) = out[False], out[True]
+    assert re_ < 1e-8 and ie == ig
+    np.testing.assert_array_equal(xe, xg)
+    # the FP64 factored solve of the same (float theta) batch; both answers
+    # are within 1e-8 relative residual, so apart by up to cond(A) 1e-8
+    ctx.sr_build(2)
+    x64, r64, _ = ctx.sr_solve(1e-3, tol=1e-10, max_iter=20000)
+    assert r64 < 1e-10
+    assert _rel(xg, x64) < 1e-5

@@ -37,7 +37,8 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBwdChunkMin = 128;  // smallest backward chunk (sizes the partials)
 constexpr int kRowPad = 512;    // ns padding
-constexpr int kMaxKs = 5;       // Kp <= 160 (n <= 159)
+constexpr int kMaxKs = 5;       // register-resident k-steps (Kp <= 160); past that the
+                                // chunked forward and the row-grouped backward run
 constexpr int kColTilesMin = 2;  // narrowest forward column group (sizes ypart)
 
 // device control block of the CG loop: ints [stop flag, completed iterations,
@@ -211,12 +212,14 @@ __device__ inline size_t aug_index(size_t c, int n, int m, int Np) {
 // are never read, and a padded sample has T' = 0 and y = 0, so their signs do
 // not matter.  A kernel reads 1 bit per spin instead of a bf16 and expands
 // the bits into bf16 +-1 in registers (expand8).
-constexpr int kSigRow = 32;  // bytes per row-major bit row (Kp <= 256)
+constexpr int kSigRow = 32;  // bytes per row-major bit row up to Kp = 256
+// bytes of a row-major bit row for Kp (a multiple of 16, at least kSigRow)
+inline int sig_row_bytes(int Kp) { return std::max(kSigRow, (Kp / 8 + 15) / 16 * 16); }
 __global__ void k_prep_sig_rows(const int8_t* __restrict__ sigma, int ns, int n, int nsp,
-                                uint8_t* __restrict__ A) {
+                                int row_bytes, uint8_t* __restrict__ A) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (size_t)nsp * kSigRow) return;
-  const int s = (int)(q / kSigRow), b = (int)(q % kSigRow);
+  if (q >= (size_t)nsp * row_bytes) return;
+  const int s = (int)(q / row_bytes), b = (int)(q % row_bytes);
   unsigned v = 0;
   if (s < ns)
     for (int e = 0; e < 8; ++e) {
@@ -454,6 +457,108 @@ auto fwd_kernel(int ct, int nks) {
   return ct == 2 ? fwd_kernel_ks<2>(nks) : ct == 8 ? fwd_kernel_ks<8>(nks) : fwd_kernel_ks<4>(nks);
 }
 
+// Forward product past Kp = 32 kMaxKs (n >= 160): K in chunks of kFwdKc
+// k-steps, each chunk's V' rows staged through LDS for the column group, the
+// U' accumulators of the CT column tiles kept in registers across the
+// chunks, and this lane's spin fragments read per chunk (16 bytes of the
+// sig_row-byte bit row).  Same block shape, operand layouts and epilogue as
+// fwd_body; the sums over k run chunk by chunk (layer, then k-step, within a
+// chunk).
+constexpr int kFwdKc = 4;
+template <int CT>
+__global__ void __launch_bounds__(256)
+    k_fwd_mfma_g(const uint8_t* __restrict__ sigA, int sig_row, const uint16_t* __restrict__ vs,
+                 const float* __restrict__ Trm, int Kp, int Np, int nsp,
+                 float* __restrict__ ypart, int* __restrict__ ictl, int count) {
+  constexpr int ldk = 32 * kFwdKc + kLdsPad;
+  extern __shared__ uint16_t bsh[];  // [3][CT*16][ldk]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, l = tid & 63;
+  const int li = l & 15, lg = l >> 4;
+  const int cg = blockIdx.y;
+  const int col0 = cg * CT * 16;
+  const int ncols = min(CT * 16, Np - col0);
+  const int nct = ncols / 16;
+  const int rbase = blockIdx.x * 256 + wave * 64;
+  if (ictl[kCtlStop]) return;  // uniform per launch, before any barrier
+  if (count && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) ictl[kCtlIters] += 1;
+  const int nks = Kp / 32;
+  floatx4 c[CT][4];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+    for (int st = 0; st < 4; ++st) c[ct][st] = (floatx4){0.f, 0.f, 0.f, 0.f};
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  for (int k0 = 0; k0 < nks; k0 += kFwdKc) {
+    const int kn = min(kFwdKc, nks - k0);  // k-steps in this chunk
+    __syncthreads();  // the previous chunk's operands are read
+    // V' rows [32 k0, 32 (k0 + kn)) of the group's columns, in 8-k pieces
+    const int kch = 4 * kn;
+    const int total = 3 * ncols * kch;
+    for (int q = tid; q < total; q += 256) {
+      const int kc8 = q % kch, col = (q / kch) % ncols, sp = q / (kch * ncols);
+      *reinterpret_cast<u16x8*>(bsh + (sp * CT * 16 + col) * ldk + kc8 * 8) =
+          *reinterpret_cast<const u16x8*>(vs + ((size_t)sp * Np + col0 + col) * Kp + k0 * 32 +
+                                          kc8 * 8);
+    }
+    // k-step k0 + ks of this lane's row: byte lg of dword ks of the row's
+    // 16-byte piece at byte 4 k0
+    bf16x8 sfr[4][kFwdKc];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const u32x4 w4 = *reinterpret_cast<const u32x4*>(
+          sigA + (size_t)(rbase + st * 16 + li) * sig_row + 4 * k0);
+#pragma unroll
+      for (int ks = 0; ks < kFwdKc; ++ks)
+        sfr[st][ks] = __builtin_bit_cast(bf16x8, expand8((w4[ks] >> (8 * lg)) & 0xFFu));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      if (ct >= nct) break;
+      bf16x8 vv[3][kFwdKc];
+#pragma unroll
+      for (int sp = 0; sp < 3; ++sp)
+#pragma unroll
+        for (int ks = 0; ks < kFwdKc; ++ks)
+          vv[sp][ks] = ld8(bsh + (sp * CT * 16 + ct * 16 + li) * ldk + ks * 32 + lg * 8);
+#pragma unroll
+      for (int sp = 0; sp < 3; ++sp)
+#pragma unroll
+        for (int ks = 0; ks < kFwdKc; ++ks) {
+          if (ks >= kn) break;  // uniform
+#pragma unroll
+          for (int st = 0; st < 4; ++st)
+            c[ct][st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv[sp][ks], sfr[st][ks],
+                                                                c[ct][st], 0, 0, 0);
+        }
+    }
+  }
+  // c[ct][st][e] = U'^T[col0 + ct*16 + 4*lg + e][rbase + st*16 + li]
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    if (ct >= nct) break;
+    const int j0 = col0 + ct * 16 + 4 * lg;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const floatx4 t =
+          *reinterpret_cast<const floatx4*>(Trm + (size_t)(rbase + st * 16 + li) * Np + j0);
+      acc[st] += ((t[0] * c[ct][st][0] + t[1] * c[ct][st][1]) +
+                  (t[2] * c[ct][st][2] + t[3] * c[ct][st][3]));
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    float v = acc[st];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    if (lg == 0) ypart[(size_t)cg * nsp + rbase + st * 16 + li] = v;
+  }
+}
+// k width of the forward's LDS slab: all of Kp, or one chunk past kMaxKs
+inline int fwd_k_width(int Kp) { return Kp <= 32 * kMaxKs ? Kp : 32 * kFwdKc; }
+
 // part[kc][i][j] = sum_{s in chunk kc} sigma'_si y_s T'_sj with the centred
 // forward result y_s = (sum_cg ypart - <omean, p>) / ns (0 for padded rows),
 // formed in LDS for the chunk; blocks of column group 0 also store the chunk
@@ -480,8 +585,10 @@ __device__ inline int xcd_block(int b, int total) {
 // ((Np / 16 + 3) / 4) x chunks grid, the grid's x extent and its size
 // NIT = Kp / 16 row tiles (a template parameter, so that the row-tile loop
 // unrolls without branches and the LDS reads of its A operands are issued
-// together)
-template <int CHUNK, int SLICE, int NIT>
+// together).  RG (Kp > 32 kMaxKs): the grid is ((Np / 16 + 3) / 4) x chunks x
+// row groups, each block forming rows [rg NIT 16, (rg + 1) NIT 16) of G'
+// (the last group may be shorter), with only those rows' sigma'^T bits in LDS.
+template <int CHUNK, int SLICE, int NIT, bool RG = false>
 __device__ __forceinline__ void bwd_body(int b, int gx, int total,
                                          const uint8_t* __restrict__ sigT,
                                          const float* __restrict__ Tt,
@@ -495,7 +602,11 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
   const int tid = threadIdx.x;
   const int wave = tid >> 6, l = tid & 63;
   const int lb = xcd_block(b, total);
-  const int cx = lb % gx, kc = lb / gx;
+  const int cx = lb % gx;
+  const int kc = RG ? (lb / gx) % (nsp / CHUNK) : lb / gx;
+  const int rg = RG ? (lb / gx) / (nsp / CHUNK) : 0;
+  const int r0 = rg * NIT * 16;                         // first row of G' formed here
+  const int nrow = RG ? min(NIT * 16, Kp - r0) : Kp;   // rows formed here
   const int ct = cx * 4 + wave;
   const bool active = ct < Np / 16;
   const int li = l & 15, lg = l >> 4;
@@ -510,7 +621,7 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
   constexpr int kKs = SLICE / 32;  // k-steps of 32 samples per slice
   constexpr int kHalves = CHUNK / SLICE;
   constexpr int kDepth = kHalves < 2 ? kHalves : 2;  // slices in flight
-  const int nslab = Kp * kWpr;
+  const int nslab = nrow * kWpr;
   const int rbytes = nsp / 8;  // bytes of a transposed bit row
   uint32_t sg[kDepth][kSlab];
   floatx4 tv[kDepth][2 * kKs];  // T'[ct*16 + li][sbase + ks*32 + lg*8 + 0..7]
@@ -522,7 +633,7 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
       const int q = tid + 256 * i;
       if (q < nslab) {
         const int row = q / kWpr, wd = q % kWpr;
-        sg[slot][i] = *reinterpret_cast<const uint32_t*>(sigT + (size_t)row * rbytes +
+        sg[slot][i] = *reinterpret_cast<const uint32_t*>(sigT + (size_t)(r0 + row) * rbytes +
                                                          sbase / 8 + 4 * wd);
       }
     }
@@ -580,7 +691,7 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
       dv2 += (double)yv * (double)yv;
     }
     if (stop) return;  // uniform per launch, before any barrier or global store
-    if (cx == 0) {  // uniform per block
+    if (cx == 0 && rg == 0) {  // uniform per block
       const double v2[2] = {dv, dv2};
       double* const o2[2] = {ys, yy};
       block_sums<2>(v2, o2, kc, 4);
@@ -619,7 +730,8 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
       // their products, each would wait on its own)
       bf16x8 av[NIT];
 #pragma unroll
-      for (int it = 0; it < NIT; ++it) av[it] = ld8(ash + (it * 16 + li) * lda + ks * 32 + lg * 8);
+      for (int it = 0; it < NIT; ++it)
+        if (!RG || it * 16 < nrow) av[it] = ld8(ash + (it * 16 + li) * lda + ks * 32 + lg * 8);
       __builtin_amdgcn_sched_barrier(0);  // (the scheduler would sink them again)
       u16x8 bh, bm, bl;
 #pragma unroll
@@ -636,6 +748,7 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
       const bf16x8 b2 = __builtin_bit_cast(bf16x8, bl);
 #pragma unroll
       for (int it = 0; it < NIT; ++it) {
+        if (RG && it * 16 >= nrow) break;  // uniform
         acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[it], b0, acc[it], 0, 0, 0);
         acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[it], b1, acc[it], 0, 0, 0);
         acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[it], b2, acc[it], 0, 0, 0);
@@ -645,19 +758,23 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
   if (!active) return;
   float* out = part + (size_t)kc * Kp * Np;
 #pragma unroll
-  for (int it = 0; it < NIT; ++it)
+  for (int it = 0; it < NIT; ++it) {
+    if (RG && it * 16 >= nrow) break;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        out[(size_t)(it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
+    for (int q = 0; q < 4; ++q)
+      out[(size_t)(r0 + it * 16 + 4 * lg + q) * Np + ct * 16 + li] = acc[it][q];
+  }
 }
-template <int CHUNK, int SLICE, int NIT>
+template <int CHUNK, int SLICE, int NIT, bool RG = false>
 __global__ void __launch_bounds__(256)
     k_bwd_mfma(const uint8_t* __restrict__ sigT, const float* __restrict__ Tt,
                const float* __restrict__ ypart, int ncg, int ns, const double* __restrict__ op,
                int nop, int Kp, int Np, int nsp, float* __restrict__ part,
                double* __restrict__ ys, double* __restrict__ yy, const int* __restrict__ ictl) {
-  bwd_body<CHUNK, SLICE, NIT>(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.x * gridDim.y,
-                         sigT, Tt, ypart, ncg, ns, op, nop, Kp, Np, nsp, part, ys, yy, ictl);
+  // grid (x, chunks x row groups): the linear index runs over all of it
+  bwd_body<CHUNK, SLICE, NIT, RG>(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x,
+                                  gridDim.x * gridDim.y, sigT, Tt, ypart, ncg, ns, op, nop, Kp,
+                                  Np, nsp, part, ys, yy, ictl);
 }
 
 // backward kernel for a chunk length; slices of 64 samples (longer slices,
@@ -678,6 +795,13 @@ auto bwd_kernel_it(int nit) {
 auto bwd_kernel(int chunk, int nit) {
   return chunk == 128 ? bwd_kernel_it<128>(nit)
                       : chunk == 512 ? bwd_kernel_it<512>(nit) : bwd_kernel_it<256>(nit);
+}
+// past Kp = 160: row groups of kBwdGroupTiles tiles
+constexpr int kBwdGroupTiles = 2 * kMaxKs;
+auto bwd_kernel_rg(int chunk) {
+  return chunk == 128   ? k_bwd_mfma<128, kBwdSlice, kBwdGroupTiles, true>
+         : chunk == 512 ? k_bwd_mfma<512, kBwdSlice, kBwdGroupTiles, true>
+                        : k_bwd_mfma<256, kBwdSlice, kBwdGroupTiles, true>;
 }
 
 // One Chronopoulos-Gear CG step for (S + shift) x = b with preconditioner M
@@ -1085,14 +1209,27 @@ void publish_launch(MixedWork& w, hipStream_t st) {
 // CG iteration.
 void product_launches(MixedWork& w, int count, hipStream_t st) {
   const RedLayout L = red_layout(w);
-  const size_t lds_f = sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad);
-  launch_prof("fwd_mfma", fwd_kernel(w.ct, w.Kp / 32), dim3(w.nsp / 256, w.ncg), dim3(256),
-              lds_f, st, w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
+  const size_t lds_f = sizeof(uint16_t) * 3 * w.ct * 16 * (fwd_k_width(w.Kp) + kLdsPad);
   const int chunks = w.nsp / w.chunk;
-  const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad);
-  launch_prof("bwd_mfma", bwd_kernel(w.chunk, w.Kp / 16), dim3((w.Np / 16 + 3) / 4, chunks), dim3(256), lds,
-              st, w.sigT, w.Tt, w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp,
-              w.part, w.red + L.ys, w.red + L.yy, w.ictl);
+  if (w.Kp <= 32 * kMaxKs) {
+    launch_prof("fwd_mfma", fwd_kernel(w.ct, w.Kp / 32), dim3(w.nsp / 256, w.ncg), dim3(256),
+                lds_f, st, w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
+    const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad);
+    launch_prof("bwd_mfma", bwd_kernel(w.chunk, w.Kp / 16), dim3((w.Np / 16 + 3) / 4, chunks),
+                dim3(256), lds, st, w.sigT, w.Tt, w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv,
+                w.Kp, w.Np, w.nsp, w.part, w.red + L.ys, w.red + L.yy, w.ictl);
+    return;
+  }
+  const int srow = sig_row_bytes(w.Kp);
+  launch_prof("fwd_mfma", w.ct == 2 ? k_fwd_mfma_g<2> : k_fwd_mfma_g<4>,
+              dim3(w.nsp / 256, w.ncg), dim3(256), lds_f, st, w.sigA, srow, w.vsplit, w.Trm,
+              w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
+  const int groups = (w.Kp / 16 + kBwdGroupTiles - 1) / kBwdGroupTiles;
+  const size_t lds = sizeof(uint16_t) * 16 * kBwdGroupTiles * (kBwdSlice + kLdsPad);
+  launch_prof("bwd_mfma", bwd_kernel_rg(w.chunk),
+              dim3((w.Np / 16 + 3) / 4, chunks * groups), dim3(256), lds, st, w.sigT, w.Tt,
+              w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp, w.part,
+              w.red + L.ys, w.red + L.yy, w.ictl);
 }
 
 // thread groups of 256 per k_cg_step block for a number of chunk partials
@@ -1367,7 +1504,8 @@ int tune(MixedWork& w, hipStream_t st, std::string& err) {
   float best = 1e30f;
   int bct = 4, bch = 256;
   for (int ct : {2, 4, 8}) {
-    if (sizeof(uint16_t) * 3 * ct * 16 * (w.Kp + kLdsPad) > 64 * 1024) continue;
+    if (sizeof(uint16_t) * 3 * ct * 16 * (fwd_k_width(w.Kp) + kLdsPad) > 64 * 1024) continue;
+    if (ct == 8 && w.Kp > 32 * kMaxKs) continue;  // the chunked forward takes 2 or 4
     for (int ch : {128, 256, 512}) {
       w.ct = ct;
       w.chunk = ch;
@@ -1411,10 +1549,6 @@ void mixed_drop_graphs(MixedWork& w) {
 
 int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, int n, int m,
                   int use_bias, hipStream_t st, std::string& err) {
-  if (n + 1 > 32 * kMaxKs) {
-    err = "mixed solver supports n <= 159";
-    return YQ_ERR_UNSUPPORTED;
-  }
   w.ns = ns;
   w.n = n;
   w.m = m;
@@ -1433,7 +1567,8 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
     const size_t chunks = rows / kBwdChunkMin;
     const size_t ncg_max = (w.Np / 16 + kColTilesMin - 1) / kColTilesMin;
     int rc;
-    if ((rc = alloc(w.sigA, rows * kSigRow, err)) || (rc = alloc(w.sigT, rows / 8 * w.Kp, err)) ||
+    if ((rc = alloc(w.sigA, rows * sig_row_bytes(w.Kp), err)) ||
+        (rc = alloc(w.sigT, rows / 8 * w.Kp, err)) ||
         (rc = alloc(w.Trm, rows * w.Np, err)) || (rc = alloc(w.Tt, rows * w.Np, err)) ||
         (rc = alloc(w.ypart, rows * ncg_max, err)) ||
         (rc = alloc(w.part, chunks * w.Kp * w.Np, err)) ||
@@ -1460,8 +1595,9 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
     w.cap_dim = w.dim;
     w.cap_red = red_total;
   }
-  hipLaunchKernelGGL(k_prep_sig_rows, dim3(nb(rows * kSigRow)), dim3(256), 0, st, sigma, ns, n,
-                     w.nsp, w.sigA);
+  const int srow = sig_row_bytes(w.Kp);
+  hipLaunchKernelGGL(k_prep_sig_rows, dim3(nb(rows * srow)), dim3(256), 0, st, sigma, ns, n,
+                     w.nsp, srow, w.sigA);
   hipLaunchKernelGGL(k_prep_sig_cols, dim3(nb(rows / 8 * w.Kp)), dim3(256), 0, st, sigma, ns, n,
                      w.Kp, w.nsp, w.sigT);
   hipLaunchKernelGGL(k_prep_T, dim3(nb(rows * w.Np)), dim3(256), 0, st, T32, ns, m, w.Np, w.nsp,
@@ -1476,7 +1612,11 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
     w.ct = 4;
     w.chunk = 256;
   }
-  if (sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad) > 64 * 1024) {
+  if (w.ct == 8 && w.Kp > 32 * kMaxKs) {
+    err = "the chunked forward product (n >= 160) takes 2 or 4 column tiles";
+    return YQ_ERR_UNSUPPORTED;
+  }
+  if (sizeof(uint16_t) * 3 * w.ct * 16 * (fwd_k_width(w.Kp) + kLdsPad) > 64 * 1024) {
     err = "forward column group too wide for this n (LDS)";
     return YQ_ERR_UNSUPPORTED;
   }
