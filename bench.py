@@ -616,6 +616,12 @@ def main():
         rel_e = _rel_diff(ene[:ws], enc[:len(ene)])
         last = outs[-1]
         last_c = outc[-1]
+        # the timed steps' energy differences against their Monte Carlo
+        # errors (16,384 independent chains per step)
+        zs = [abs(a["energy"] - b["energy"]) / np.sqrt((a["energy_var"] + b["energy_var"]) / NS)
+              for a, b in zip(outs, outc)]
+        em = float(np.mean([o["energy"] for o in outs]))
+        ef = float(np.mean([o["energy"] for o in outc]))
         extra = {
             "fp64_factored_cg_steps_per_s": K / dtc,
             "fp64_factored_cg_steps_per_s_repeat": K / dtc2,
@@ -642,6 +648,13 @@ def main():
                 "the two Markov chains and then the parameters drift apart stochastically. "
                 "mixed_solver_vs_fp64: the mixed solve fed by the FP64 path's own chains, all "
                 "%d steps: the difference the mixed arithmetic itself makes" % (ws, ws)),
+            "energy_timed_window": {
+                "mixed_mean": em, "fp64_mean": ef, "rel_diff_of_means": abs(em - ef) / abs(ef),
+                "diff_over_mc_err_max": float(max(zs)),
+                "diff_over_mc_err_mean": float(np.mean(zs)),
+                "note": "per timed step |E_mixed - E_fp64| / sqrt((var_mixed + var_fp64) / Ns): "
+                        "the two paths' energies differ by what their Monte Carlo errors "
+                        "allow"},
             "energy_last_step": {"mixed": energies[-1], "fp64": enc[-1],
                                  "mixed_solver_fp64_sampling": ens[-1],
                                  "mc_std_err_fp64": float(np.sqrt(last_c["energy_var"] / NS)),

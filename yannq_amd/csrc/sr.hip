@@ -188,36 +188,81 @@ __device__ inline double v_aug(const double* __restrict__ v, int n, int m, int u
   return 0.0;
 }
 
-// ypart[cg][s] = sum over the column tiles of group cg of T'_sj (sigma' V')_sj
+// the parameter vector on the padded (Kp x Np) grid of sigma'^T T', so the
+// forward product reads its B operand with plain coalesced loads
+__global__ void k_aug64(const double* __restrict__ v, int n, int m, int use_bias, int Kp, int Np,
+                        double* __restrict__ va) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)Kp * Np) return;
+  va[q] = v_aug(v, n, m, use_bias, (int)(q / Np), (int)(q % Np));
+}
+
+// ypart[cg][s] = sum over the column tiles of group cg of T'_sj (sigma' V')_sj.
+// One wave: 16 samples x the kF64Tiles column tiles of group blockIdx.y, one
+// tile after the other.  The B operands come from Vaug (the padded parameter
+// grid, L2-resident) with plain coalesced loads and the A operands (this
+// lane's spins, +-1, 1 for the bias column, 0 past it) from its spin row,
+// both without branches, so the unrolled k loop keeps several k-steps' loads
+// in flight.  Each tile's sum over k runs in increasing k and the tiles are
+// added in order.
 __global__ void __launch_bounds__(256)
     k_fwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
-                 int m, int use_bias, const double* __restrict__ v, int Kp, int Np, int nrows,
+                 int m, const double* __restrict__ va, int Kp, int Np, int nrows,
                  double* __restrict__ ypart) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int row0 = (blockIdx.x * 4 + wave) * 16;
-  if (row0 >= nrows) return;
+  if (row0 >= nrows) return;  // wave-uniform; no barrier in this kernel
   const int li = l & 15, lg = l >> 4;
   const int ct0 = blockIdx.y * kF64Tiles, ct1 = min(ct0 + kF64Tiles, Np / 16);
   const int nk = Kp / 4;
+  const int srow = row0 + li;
+  const bool rin = srow < ns;
+  const int8_t* sp = sigma + (size_t)(rin ? srow : 0) * n;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  constexpr int kUf = 8;  // k-steps per load batch
   for (int ct = ct0; ct < ct1; ++ct) {
     doublex4 c = {0.0, 0.0, 0.0, 0.0};
     const int j = ct * 16 + li;
-#pragma unroll 8
-    for (int q = 0; q < nk; ++q)
-      c = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, row0 + li, 4 * q + lg),
-                                               v_aug(v, n, m, use_bias, 4 * q + lg, j), c, 0, 0,
-                                               0);
+    const double* vcol = va + j;
+    // operands of k-steps k0 .. k0 + kUf - 1 (clamped addresses past nk)
+    auto load = [&](int k0, double (&ob)[kUf], int8_t (&oa)[kUf]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += t_aug(T, ns, m, row0 + lg + 4 * q, j) * c[q];
+      for (int u = 0; u < kUf; ++u) {
+        const int i = 4 * min(k0 + u, nk - 1) + lg;
+        ob[u] = vcol[(size_t)i * Np];
+        oa[u] = sp[min(i, n - 1)];
+      }
+    };
+    double bv[kUf];
+    int8_t av[kUf];
+    load(0, bv, av);
+    for (int k0 = 0; k0 < nk; k0 += kUf) {
+      double bn[kUf];
+      int8_t an[kUf];
+      if (k0 + kUf < nk) load(k0 + kUf, bn, an);
+#pragma unroll
+      for (int u = 0; u < kUf; ++u) {
+        if (k0 + u >= nk) break;  // uniform
+        const int i = 4 * (k0 + u) + lg;
+        const double a = (!rin || i > n) ? 0.0 : (i == n ? 1.0 : (av[u] > 0 ? 1.0 : -1.0));
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[u], c, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < kUf; ++u) {
+        bv[u] = bn[u];
+        av[u] = an[u];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] += t_aug(T, ns, m, row0 + lg + 4 * e, j) * c[e];
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int e = 0; e < 4; ++e)
 #pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) acc[q] += __shfl_xor(acc[q], s);
+    for (int s = 8; s >= 1; s >>= 1) acc[e] += __shfl_xor(acc[e], s);
   if (li == 0)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ypart[(size_t)blockIdx.y * nrows + row0 + lg + 4 * q] = acc[q];
+    for (int e = 0; e < 4; ++e) ypart[(size_t)blockIdx.y * nrows + row0 + lg + 4 * e] = acc[e];
 }
 
 __global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows, int ns,
@@ -229,43 +274,82 @@ __global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows
   y[s] = v;
 }
 
-// part[kc][i][j] = sum_{s in chunk kc} sigma'_si r_s T'_sj ; one wave per
-// (column tile, chunk, group of up to kMaxIt64 row tiles: blockIdx.z)
+constexpr int kU64 = 8;  // k-steps per load batch of the backward product
+
+// part[kc][i][j] = sum_{s in chunk kc} sigma'_si r_s T'_sj.  One wave per
+// (column tile, sample chunk blockIdx.y, group of NIT row tiles blockIdx.z).
+// Per k-step a lane needs its B value r_s T'_sj and the spins of its NIT rows
+// of sample s; both are loaded a batch of kU64 k-steps ahead of the MFMAs
+// that use them.  NIT is a template parameter so that the row-tile loop
+// unrolls and the accumulators stay in registers.  Each tile's sum over the
+// samples runs in increasing sample order.
+template <int NIT>
 __global__ void __launch_bounds__(256)
     k_bwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
                  int m, const double* __restrict__ r, int Kp, int Np, int chunk,
                  double* __restrict__ part) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int ct = blockIdx.x * 4 + wave;
-  if (ct >= Np / 16) return;
-  const int kc = blockIdx.y;
+  if (ct >= Np / 16) return;  // wave-uniform; no barrier in this kernel
   const int li = l & 15, lg = l >> 4;
-  const int it0 = blockIdx.z * kMaxIt64;  // first row tile of this group
-  const int nit = min(Kp / 16 - it0, kMaxIt64);
+  const int kc = blockIdx.y;
+  const int i0 = blockIdx.z * NIT * 16;  // first row of G' formed here
+  const int s0 = kc * chunk;
   const int j = ct * 16 + li;
-  doublex4 acc[kMaxIt64];
+  doublex4 acc[NIT];
 #pragma unroll
-  for (int it = 0; it < kMaxIt64; ++it) acc[it] = (doublex4){0.0, 0.0, 0.0, 0.0};
-  // unrolled so that several k-steps' loads are in flight together (the
-  // accumulation order over k is unchanged)
-#pragma unroll 8
-  for (int k0 = kc * chunk; k0 < kc * chunk + chunk; k0 += 4) {
-    const int s = k0 + lg;
-    const double b = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j);
+  for (int it = 0; it < NIT; ++it) acc[it] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  const int nks = chunk / 4;
+  // spin (row i0 + it * 16 + li, sample s) as +-1, 1 for the bias row, 0 past
+  // it or past the batch: loaded at clamped addresses, selected afterwards
+  double bv[kU64];
+  int8_t av[kU64][NIT];
+  auto load = [&](int k0, double (&ob)[kU64], int8_t (&oa)[kU64][NIT]) {
 #pragma unroll
-    for (int it = 0; it < kMaxIt64; ++it)
-      if (it < nit)
-        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(sig_aug(sigma, ns, n, s,
-                                                               (it0 + it) * 16 + li),
-                                                       b, acc[it], 0, 0, 0);
+    for (int u = 0; u < kU64; ++u) {
+      const int s = s0 + min(k0 + u, nks - 1) * 4 + lg;  // clamped: unused past nks
+      const int sc = min(s, ns - 1);
+      ob[u] = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j);
+#pragma unroll
+      for (int it = 0; it < NIT; ++it)
+        oa[u][it] = sigma[(size_t)sc * n + min(i0 + it * 16 + li, n - 1)];
+    }
+  };
+  auto spin = [&](int8_t raw, int it, int k) -> double {
+    const int i = i0 + it * 16 + li;
+    const int s = s0 + k * 4 + lg;
+    if (s >= ns || i > n) return 0.0;
+    if (i == n) return 1.0;
+    return raw > 0 ? 1.0 : -1.0;
+  };
+  load(0, bv, av);
+  for (int k0 = 0; k0 < nks; k0 += kU64) {
+    double bn[kU64];
+    int8_t an[kU64][NIT];
+    if (k0 + kU64 < nks) load(k0 + kU64, bn, an);
+#pragma unroll
+    for (int u = 0; u < kU64; ++u) {
+      if (k0 + u >= nks) break;  // uniform
+#pragma unroll
+      for (int it = 0; it < NIT; ++it)
+        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(spin(av[u][it], it, k0 + u), bv[u],
+                                                       acc[it], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kU64; ++u) {
+      bv[u] = bn[u];
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) av[u][it] = an[u][it];
+    }
   }
   double* out = part + (size_t)kc * Kp * Np;
 #pragma unroll
-  for (int it = 0; it < kMaxIt64; ++it)
-    if (it < nit)
+  for (int it = 0; it < NIT; ++it)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        out[(size_t)((it0 + it) * 16 + lg + 4 * q) * Np + j] = acc[it][q];
+    for (int q = 0; q < 4; ++q) {
+      const int row = i0 + it * 16 + lg + 4 * q;
+      if (row < Kp) out[(size_t)row * Np + j] = acc[it][q];
+    }
 }
 
 // g[c] = scale * sum_k part[k][aug(c)] on the padded (Kp x Np) grid, mapped to
@@ -498,7 +582,9 @@ void launch_copy_lower_shift(const double* S, double* M, int dim,
 
 size_t factored_partial_doubles(int ns, int n, int m) {
   const size_t grid = (size_t)bwd64_chunks(ns) * kp16(n) * np16(m);
-  const size_t fwd = (size_t)((np16(m) / 16 + kF64Tiles - 1) / kF64Tiles) * ((ns + 15) / 16 * 16);
+  // forward: the column-group partials, then the padded parameter grid
+  const size_t fwd = (size_t)((np16(m) / 16 + kF64Tiles - 1) / kF64Tiles) * ((ns + 15) / 16 * 16) +
+                     (size_t)kp16(n) * np16(m);
   return (grid > fwd ? grid : fwd) + ns;
 }
 
@@ -507,10 +593,28 @@ void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
                            double* scratch, hipStream_t st) {
   const int Kp = kp16(n), Np = np16(m), nrows = (ns + 15) / 16 * 16;
   const int ncg = (Np / 16 + kF64Tiles - 1) / kF64Tiles;
+  double* va = scratch + (size_t)ncg * nrows;
   ProfGuard g("factored_fwd64", st);
+  hipLaunchKernelGGL(k_aug64, dim3(nblk((size_t)Kp * Np)), dim3(256), 0, st, v, n, m, use_bias,
+                     Kp, Np, va);
   hipLaunchKernelGGL(k_fwd64_mfma, dim3((nrows / 16 + 3) / 4, ncg), dim3(256), 0, st, sigma, T,
-                     ns, n, m, use_bias, v, Kp, Np, nrows, scratch);
+                     ns, n, m, (const double*)va, Kp, Np, nrows, scratch);
   hipLaunchKernelGGL(k_sum_ypart, dim3(nblk(ns)), dim3(256), 0, st, scratch, ncg, nrows, ns, y);
+}
+
+// the backward kernel for nit row tiles per block
+auto bwd64_kernel(int nit) {
+  switch (nit) {
+    case 1: return k_bwd64_mfma<1>;
+    case 2: return k_bwd64_mfma<2>;
+    case 3: return k_bwd64_mfma<3>;
+    case 4: return k_bwd64_mfma<4>;
+    case 5: return k_bwd64_mfma<5>;
+    case 6: return k_bwd64_mfma<6>;
+    case 7: return k_bwd64_mfma<7>;
+    case 8: return k_bwd64_mfma<8>;
+    default: return k_bwd64_mfma<kMaxIt64>;
+  }
 }
 
 void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
@@ -519,9 +623,11 @@ void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
   const int chunks = bwd64_chunks(ns);
   const int Kp = kp16(n), Np = np16(m);
   ProfGuard pg("factored_bwd64", st);
-  const int rgroups = (Kp / 16 + kMaxIt64 - 1) / kMaxIt64;
-  hipLaunchKernelGGL(k_bwd64_mfma, dim3((Np / 16 + 3) / 4, chunks, rgroups), dim3(256), 0, st,
-                     sigma, T, ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial);
+  // all row tiles in one group up to kMaxIt64, else groups of kMaxIt64
+  const int nit = std::min(Kp / 16, kMaxIt64);
+  const int rgroups = (Kp / 16 + nit - 1) / nit;
+  hipLaunchKernelGGL(bwd64_kernel(nit), dim3((Np / 16 + 3) / 4, chunks, rgroups), dim3(256), 0,
+                     st, sigma, T, ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial);
   const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
   hipLaunchKernelGGL(k_reduce_aug, dim3(nblk(dim)), dim3(256), 0, st, partial, chunks, n, m,
                      use_bias, Kp, Np, 1.0, g);
