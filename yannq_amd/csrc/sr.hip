@@ -198,71 +198,110 @@ __global__ void k_aug64(const double* __restrict__ v, int n, int m, int use_bias
 }
 
 // ypart[cg][s] = sum over the column tiles of group cg of T'_sj (sigma' V')_sj.
-// One wave: 16 samples x the kF64Tiles column tiles of group blockIdx.y, one
-// tile after the other.  The B operands come from Vaug (the padded parameter
-// grid, L2-resident) with plain coalesced loads and the A operands (this
-// lane's spins, +-1, 1 for the bias column, 0 past it) from its spin row,
-// both without branches, so the unrolled k loop keeps several k-steps' loads
-// in flight.  Each tile's sum over k runs in increasing k and the tiles are
-// added in order.
+// Block: 4 waves x two 16-sample row tiles (128 samples), column group
+// blockIdx.y of kF64Tiles tiles.  The group's rows of Vaug (the padded
+// parameter grid) are staged through LDS kF64Kc k at a time and shared by the
+// waves, and every B operand read from LDS feeds the MFMAs of both row tiles;
+// the A operands are the lane's spins (+-1, 1 for the bias column, 0 past
+// it).  Each (row tile, column tile) sum over k runs in increasing k and the
+// column tiles are added in order, so the values are those of a sequential
+// loop over the tiles.
+constexpr int kF64Kc = 32;  // k per LDS stage (8 k-steps)
 __global__ void __launch_bounds__(256)
     k_fwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
                  int m, const double* __restrict__ va, int Kp, int Np, int nrows,
                  double* __restrict__ ypart) {
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int row0 = (blockIdx.x * 4 + wave) * 16;
-  if (row0 >= nrows) return;  // wave-uniform; no barrier in this kernel
+  constexpr int kCols = kF64Tiles * 16;
+  __shared__ double vsh[kF64Kc][kCols + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   const int li = l & 15, lg = l >> 4;
-  const int ct0 = blockIdx.y * kF64Tiles, ct1 = min(ct0 + kF64Tiles, Np / 16);
+  const int row0 = (blockIdx.x * 8 + wave * 2) * 16;  // this wave's two row tiles
+  const int ct0 = blockIdx.y * kF64Tiles;
+  const int nct = min(kF64Tiles, Np / 16 - ct0);
   const int nk = Kp / 4;
-  const int srow = row0 + li;
-  const bool rin = srow < ns;
-  const int8_t* sp = sigma + (size_t)(rin ? srow : 0) * n;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  constexpr int kUf = 8;  // k-steps per load batch
-  for (int ct = ct0; ct < ct1; ++ct) {
-    doublex4 c = {0.0, 0.0, 0.0, 0.0};
-    const int j = ct * 16 + li;
-    const double* vcol = va + j;
-    // operands of k-steps k0 .. k0 + kUf - 1 (clamped addresses past nk)
-    auto load = [&](int k0, double (&ob)[kUf], int8_t (&oa)[kUf]) {
+  // (waves past nrows run on clamped rows, for the barriers, and store nothing)
+  bool rin[2];
+  const int8_t* sp[2];
 #pragma unroll
-      for (int u = 0; u < kUf; ++u) {
-        const int i = 4 * min(k0 + u, nk - 1) + lg;
-        ob[u] = vcol[(size_t)i * Np];
-        oa[u] = sp[min(i, n - 1)];
-      }
-    };
-    double bv[kUf];
-    int8_t av[kUf];
-    load(0, bv, av);
-    for (int k0 = 0; k0 < nk; k0 += kUf) {
-      double bn[kUf];
-      int8_t an[kUf];
-      if (k0 + kUf < nk) load(k0 + kUf, bn, an);
+  for (int rt = 0; rt < 2; ++rt) {
+    const int srow = row0 + rt * 16 + li;
+    rin[rt] = srow < ns;
+    sp[rt] = sigma + (size_t)(rin[rt] ? srow : 0) * n;
+  }
+  doublex4 c[2][kF64Tiles];
 #pragma unroll
-      for (int u = 0; u < kUf; ++u) {
-        if (k0 + u >= nk) break;  // uniform
-        const int i = 4 * (k0 + u) + lg;
-        const double a = (!rin || i > n) ? 0.0 : (i == n ? 1.0 : (av[u] > 0 ? 1.0 : -1.0));
-        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[u], c, 0, 0, 0);
-      }
+  for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-      for (int u = 0; u < kUf; ++u) {
-        bv[u] = bn[u];
-        av[u] = an[u];
-      }
+    for (int q = 0; q < kF64Tiles; ++q) c[rt][q] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  // the next stage's slab entries and spins are loaded into registers while
+  // the current stage's MFMAs run
+  constexpr int kPer = kF64Kc * kCols / 256;
+  double stg[kPer];
+  int8_t an[2][kF64Kc / 4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int q = tid + 256 * u, kk = q / kCols, jj = q % kCols;
+      const int k = min(k0 + kk, Kp - 1), j = ct0 * 16 + min(jj, nct * 16 - 1);
+      stg[u] = va[(size_t)k * Np + j];  // clamped: entries past Kp / the group are not read
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] += t_aug(T, ns, m, row0 + lg + 4 * e, j) * c[e];
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ks = 0; ks < kF64Kc / 4; ++ks) an[rt][ks] = sp[rt][min(k0 + 4 * ks + lg, n - 1)];
+  };
+  load(0);
+  for (int k0 = 0; k0 < Kp; k0 += kF64Kc) {
+    __syncthreads();  // the previous stage is read
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int q = tid + 256 * u;
+      vsh[q / kCols][q % kCols] = stg[u];
+    }
+    int8_t av[2][kF64Kc / 4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ks = 0; ks < kF64Kc / 4; ++ks) av[rt][ks] = an[rt][ks];
+    __syncthreads();
+    if (k0 + kF64Kc < Kp) load(k0 + kF64Kc);
+#pragma unroll
+    for (int ks = 0; ks < kF64Kc / 4; ++ks) {
+      if (k0 / 4 + ks >= nk) break;  // uniform
+      const int i = k0 + 4 * ks + lg;  // this lane's spin index
+      double a[2];
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+        a[rt] = (!rin[rt] || i > n) ? 0.0 : (i == n ? 1.0 : (av[rt][ks] > 0 ? 1.0 : -1.0));
+#pragma unroll
+      for (int q = 0; q < kF64Tiles; ++q) {
+        if (q >= nct) break;  // uniform
+        const double b = vsh[4 * ks + lg][q * 16 + li];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+          c[rt][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b, c[rt][q], 0, 0, 0);
+      }
+    }
   }
 #pragma unroll
-  for (int e = 0; e < 4; ++e)
+  for (int rt = 0; rt < 2; ++rt) {
+    const int r0 = row0 + rt * 16;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) acc[e] += __shfl_xor(acc[e], s);
-  if (li == 0)
+    for (int q = 0; q < kF64Tiles; ++q)
+      if (q < nct) {
+        const int j = (ct0 + q) * 16 + li;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) ypart[(size_t)blockIdx.y * nrows + row0 + lg + 4 * e] = acc[e];
+        for (int e = 0; e < 4; ++e) acc[e] += t_aug(T, ns, m, r0 + lg + 4 * e, j) * c[rt][q][e];
+      }
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int s = 8; s >= 1; s >>= 1) acc[e] += __shfl_xor(acc[e], s);
+    if (li == 0 && r0 < nrows)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ypart[(size_t)blockIdx.y * nrows + r0 + lg + 4 * e] = acc[e];
+  }
 }
 
 __global__ void k_sum_ypart(const double* __restrict__ ypart, int ncg, int nrows, int ns,
@@ -366,8 +405,18 @@ __global__ void k_reduce_aug(const double* __restrict__ part, int chunks, int n,
   else if (c < nm + n) a = (c - nm) * Np + m;
   else a = (size_t)n * Np + (c - nm - n);
   const size_t stride = (size_t)Kp * Np;
+  // the chunks in order, 16 loads in flight at a time (clamped indices, the
+  // values past the last chunk left out of the sum)
+  constexpr int kB = 16;
   double acc = 0.0;
-  for (int k = 0; k < chunks; ++k) acc += part[(size_t)k * stride + a];
+  for (int k0 = 0; k0 < chunks; k0 += kB) {
+    double v[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) v[u] = part[(size_t)min(k0 + u, chunks - 1) * stride + a];
+#pragma unroll
+    for (int u = 0; u < kB; ++u)
+      if (k0 + u < chunks) acc += v[u];
+  }
   g[c] = acc * scale;
 }
 
@@ -597,7 +646,7 @@ void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
   ProfGuard g("factored_fwd64", st);
   hipLaunchKernelGGL(k_aug64, dim3(nblk((size_t)Kp * Np)), dim3(256), 0, st, v, n, m, use_bias,
                      Kp, Np, va);
-  hipLaunchKernelGGL(k_fwd64_mfma, dim3((nrows / 16 + 3) / 4, ncg), dim3(256), 0, st, sigma, T,
+  hipLaunchKernelGGL(k_fwd64_mfma, dim3((nrows + 127) / 128, ncg), dim3(256), 0, st, sigma, T,
                      ns, n, m, (const double*)va, Kp, Np, nrows, scratch);
   hipLaunchKernelGGL(k_sum_ypart, dim3(nblk(ns)), dim3(256), 0, st, scratch, ncg, nrows, ns, y);
 }
