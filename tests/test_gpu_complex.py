@@ -145,6 +145,35 @@ def test_complex_sr_build_and_solve(ctx, orc, solver):
         ctx.sr_build(2)  # the factored solver is real-only
 
 
+@pytest.mark.parametrize("n,m,ns", [(12, 20, 700), (32, 64, 2048)])
+def test_complex_explicit_fp64_and_mixed_s(ctx, orc, n, m, ns):
+    """The explicit S of a complex machine comes from the hand-written SYRK of
+    the centred conj(O) rows read as 2 dim reals (FP64 MFMA): equal to the
+    oracle's S to 1e-12.  With YQ_MIXED the same product runs in split-bf16
+    (six hi/mid/lo products, fp32 accumulation): S to float accuracy, and the
+    FP64-refined Cholesky solve reaches the FP64 solution."""
+    rbm = _setup(ctx, orc, n, m, scale=0.1)
+    ctx.ham_set(1, n, J=1.0, delta=1.0)
+    rng = np.random.default_rng(n)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig)
+    ctx.sr_build(0)
+    S64 = ctx.sr_get(3)
+    th = orc.c_theta(rbm, sig)
+    el = orc.c_eloc(orc.ham(1, n, J=1.0, delta=1.0), rbm, sig, th)
+    _, _, F, S = orc.c_sr_build(rbm, sig, th, el)
+    assert _rel(S64, S) < 1e-12
+    lam = 1e-3
+    x_ref = np.linalg.solve(S + lam * np.eye(len(F)), F)
+    ctx.sr_build(0, precision=1)
+    Sm = ctx.sr_get(3)
+    assert np.abs(Sm - S).max() <= 3e-6 * np.abs(S).max()
+    assert _rel(ctx.sr_get(0), F) < 1e-10
+    x, res, rounds = ctx.sr_solve(lam, tol=1e-12)
+    assert res < 1e-12 and 1 <= rounds <= 30
+    assert _rel(x, x_ref) < 1e-9
+
+
 def test_complex_mixed_batch_matches_fp64(ctx, orc):
     """C2 shape (N=64, alpha=2, complex): the same configurations evaluated
     with float E_loc and with FP64 give the same energy and force to float

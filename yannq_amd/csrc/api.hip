@@ -121,6 +121,7 @@ struct yq_ctx {
   bool have_sr = false;
   int sr_solver = 0, sr_prec = 0;
   Buf T64, T32, O, omean, F, S, M, partial, ecent, info;
+  Buf hxx;  // complex explicit build: X^T X of the rows read as reals
   Buf vx, vr, vp, vAp, vy, vtmp, vb, dotp;
   // device scalars: [0, 1] <omean, v> and sum y of the FP64 factored operator,
   // [2, 3] <b, b> and <r, r> of the mixed refinement
@@ -389,47 +390,30 @@ int compute_eloc(yq_ctx* c) {
 // ---------------------------------------------------------------- SR build
 size_t aug_partial_doubles(int ns, int n, int m) { return factored_partial_doubles(ns, n, m); }
 
-// Complex S = alpha A A^H (lower triangle) with A = conj(O_c) dim x ns
-// column-major, on rocBLAS HERK.  HERK parallelises over the output tiles,
-// which leaves most of the chip idle when dim is small and ns large (an exact
-// basis of 2^20 configurations); then the samples are split into P chunks,
-// one strided-batched HERK forms the P partial products side by side, and a
-// fixed-order sum adds them (plus the remainder chunk).
-int herk_samples(yq_ctx* c, const double* A, size_t dim, int ns, double alpha, double* S) {
-  TRY(blas(c));
-  const int kc = 16384;
-  int P = ns / kc;
-  const size_t cap = (size_t)2 << 30;  // bytes of partial products
-  while (P > 1 && (size_t)P * dim * dim * 2 * sizeof(double) > cap) --P;
-  const double zero = 0.0, one = 1.0;
-  auto* Sz = reinterpret_cast<rocblas_double_complex*>(S);
-  if (P < 4) {
-    ProfGuard g("zherk", c->st);
-    BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                             (rocblas_int)dim, ns, &alpha,
-                             reinterpret_cast<const rocblas_double_complex*>(A),
-                             (rocblas_int)dim, &zero, Sz, (rocblas_int)dim));
-    return YQ_OK;
+// Complex S = alpha A A^H (lower triangle, interleaved) with A = conj(O_c)
+// (the stored rows, ns x dim complex row-major) on the hand-written SYRKs:
+// the rows read as 2 dim reals X give XX = alpha X^T X (FP64 MFMA, or split-
+// bf16 with fp32 accumulation for the mixed build), from whose 2 x 2 blocks
+// k_herk_combine forms Re S and Im S.  A batch with few tiles and many samples
+// is split into sample chunks summed in chunk order (partials in c->M).
+int herk_mfma(yq_ctx* c, const double* A, size_t dim, int ns, double alpha, double* S,
+              bool bf16x3) {
+  const size_t d2 = 2 * dim;
+  TRY(ensure(c, c->hxx, sizeof(double) * d2 * d2));
+  const size_t cap = ((size_t)2 << 30) / sizeof(double);
+  const int P = syrk64_chunks(ns, d2, cap);
+  double* part = nullptr;
+  if (P > 1) {
+    TRY(ensure(c, c->M, sizeof(double) * (size_t)P * d2 * d2));
+    part = c->M.as<double>();
   }
-  TRY(ensure(c, c->M, sizeof(double) * 2 * (size_t)P * dim * dim));
-  double* Pout = c->M.as<double>();
-  const rocblas_stride sa = (rocblas_stride)kc * dim, sc = (rocblas_stride)(dim * dim);
-  {
-    ProfGuard g("zherk", c->st);
-    BLASCHK(c, rocblas_zherk_strided_batched(
-                   c->blas, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)dim, kc,
-                   &alpha, reinterpret_cast<const rocblas_double_complex*>(A), (rocblas_int)dim,
-                   sa, &zero, reinterpret_cast<rocblas_double_complex*>(Pout), (rocblas_int)dim,
-                   sc, P));
-  }
-  launch_sum_lower(Pout, P, dim, 2, S, c->st);
+  const int rc = bf16x3 ? launch_syrk_bf16x3_dense(A, ns, d2, alpha, c->hxx.as<double>(), P,
+                                                   part, c->st)
+                        : launch_syrk64_dense(A, ns, d2, alpha, c->hxx.as<double>(), P, part,
+                                              c->st);
+  if (rc) return fail(c, rc, "SYRK arguments");
+  launch_herk_combine(c->hxx.as<double>(), dim, S, c->st);
   LAUNCHCHK(c);
-  const int rem = ns - P * kc;
-  if (rem > 0)
-    BLASCHK(c, rocblas_zherk(c->blas, rocblas_fill_lower, rocblas_operation_none,
-                             (rocblas_int)dim, rem, &alpha,
-                             reinterpret_cast<const rocblas_double_complex*>(A + 2 * (size_t)P * kc * dim),
-                             (rocblas_int)dim, &one, Sz, (rocblas_int)dim));
   return YQ_OK;
 }
 
@@ -459,9 +443,10 @@ int syrk_real(yq_ctx* c, const double* Odense, size_t dim, int ns, double alpha,
   return YQ_OK;
 }
 
-// Complex machine: dense conj(O) (see complex.hip), FP64 SR whatever the
-// batch precision; S only for the explicit solver.
-int sr_build_c(yq_ctx* c, int solver) {
+// Complex machine: dense conj(O) (see complex.hip); S only for the explicit
+// solver, in FP64, or with YQ_MIXED in split-bf16 (solved by FP64-refined
+// Cholesky); the dense CG runs in FP64 whatever the precision.
+int sr_build_c(yq_ctx* c, int solver, int precision) {
   const int ns = c->ns, n = c->n, m = c->m;
   const size_t dim = c->dim;
   if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG)
@@ -492,7 +477,7 @@ int sr_build_c(yq_ctx* c, int solver) {
   LAUNCHCHK(c);
   if (solver == YQ_SOLVE_EXPLICIT) {
     TRY(ensure(c, c->S, sizeof(double) * 2 * dim * dim));
-    TRY(herk_samples(c, Oh, dim, ns, 1.0 / ns, c->S.as<double>()));
+    TRY(herk_mfma(c, Oh, dim, ns, 1.0 / ns, c->S.as<double>(), precision == YQ_MIXED));
   }
   // diag(S) = mean |O_c|^2, duplicated per (re, im) for the Jacobi option
   TRY(ensure(c, c->diagS, sizeof(double) * 2 * dim));
@@ -506,7 +491,7 @@ int sr_build_c(yq_ctx* c, int solver) {
   c->have_sr = true;
   c->diag_ready = true;  // formed above
   c->sr_solver = solver;
-  c->sr_prec = YQ_FP64;
+  c->sr_prec = solver == YQ_SOLVE_EXPLICIT ? precision : YQ_FP64;
   return YQ_OK;
 }
 
@@ -602,7 +587,7 @@ int sr_build(yq_ctx* c, int solver, int precision) {
   if (precision != YQ_FP64 && precision != YQ_MIXED)
     return fail(c, YQ_ERR_ARG, "unknown precision");
   if (c->ap) return sr_build_ap(c, solver, precision);
-  if (c->cplx) return sr_build_c(c, solver);
+  if (c->cplx) return sr_build_c(c, solver, precision);
   const int ns = c->ns, n = c->n, m = c->m;
   const size_t dim = c->dim;
   if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG &&
@@ -893,7 +878,7 @@ int residual64(yq_ctx* c, double shift, const double* x, const double* b, double
   const size_t dim = srlen(c);
   TRY(ensure(c, c->vtmp, sizeof(double) * std::max(dim, (size_t)c->ns * c->m)));
   double* t = c->vtmp.as<double>();
-  if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->cplx) {
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->cplx && c->sr_prec != YQ_MIXED) {
     TRY(blas(c));
     HIPCHK(c, hipMemcpyAsync(t, x, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->st));
     const rocblas_double_complex one(1.0, 0.0), sh(shift, 0.0);
@@ -1038,6 +1023,62 @@ int solve_explicit_mixed(yq_ctx* c, double shift, const double* b, double* x, do
   return *err < tol ? YQ_OK : YQ_ERR_NOT_CONVERGED;
 }
 
+// Mixed explicit solve of a complex machine: FP64 Cholesky (rocSOLVER zpotrf)
+// of the split-bf16 S + shift I, then FP64 iterative refinement x += (S16 +
+// shift)^-1 r, r = b - (S + shift) x with the exact FP64 operator on the
+// stored conj(O) (apply64), until |r| / |b| < tol (30 rounds at most).
+int solve_explicit_mixed_c(yq_ctx* c, double shift, const double* b, double* x, double tol,
+                           int* rounds, double* err) {
+  const size_t dim = c->dim, len = 2 * dim;
+  TRY(blas(c));
+  TRY(ensure(c, c->M, sizeof(double) * 2 * dim * dim));
+  TRY(ensure(c, c->info, sizeof(rocblas_int)));
+  TRY(ensure(c, c->vr, sizeof(double) * len));
+  TRY(ensure(c, c->vp, sizeof(double) * len));
+  TRY(ensure(c, c->vAp, sizeof(double) * len));
+  launch_copy_shift_c(c->S.as<double>(), c->M.as<double>(), (int)dim, shift, c->st);
+  LAUNCHCHK(c);
+  auto* M = reinterpret_cast<rocblas_double_complex*>(c->M.p);
+  {
+    ProfGuard g("zpotrf", c->st);
+    BLASCHK(c, rocsolver_zpotrf(c->blas, rocblas_fill_lower, (rocblas_int)dim, M,
+                                (rocblas_int)dim, c->info.as<rocblas_int>()));
+  }
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  TRY(sync(c));
+  if (info != 0)
+    return fail(c, YQ_ERR_NOT_SPD,
+                "Cholesky of the split-bf16 S + shift I failed at pivot " + std::to_string(info));
+  double *r = c->vr.as<double>(), *d = c->vp.as<double>(), *Ax = c->vAp.as<double>();
+  double bn2;
+  TRY(dot(c, b, b, len, &bn2));
+  HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double) * len, c->st));
+  HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * len, hipMemcpyDeviceToDevice, c->st));
+  *rounds = 0;
+  *err = bn2 > 0.0 ? 1.0 : 0.0;
+  if (!std::isfinite(bn2)) return fail(c, YQ_ERR_NAN, "non-finite right-hand side");
+  for (int k = 0; k < 30 && bn2 > 0.0; ++k) {
+    HIPCHK(c, hipMemcpyAsync(d, r, sizeof(double) * len, hipMemcpyDeviceToDevice, c->st));
+    BLASCHK(c, rocsolver_zpotrs(c->blas, rocblas_fill_lower, (rocblas_int)dim, 1, M,
+                                (rocblas_int)dim, reinterpret_cast<rocblas_double_complex*>(d),
+                                (rocblas_int)dim));
+    launch_axpy(1.0, d, x, len, c->st);
+    LAUNCHCHK(c);
+    TRY(apply64(c, shift, x, Ax));
+    HIPCHK(c, hipMemcpyAsync(r, b, sizeof(double) * len, hipMemcpyDeviceToDevice, c->st));
+    launch_axpy(-1.0, Ax, r, len, c->st);
+    LAUNCHCHK(c);
+    double rn2;
+    TRY(dot(c, r, r, len, &rn2));
+    *rounds = k + 1;
+    *err = sqrt(rn2 / bn2);
+    if (!std::isfinite(*err)) return fail(c, YQ_ERR_NAN, "non-finite residual");
+    if (*err < tol) break;
+  }
+  return *err < tol ? YQ_OK : YQ_ERR_NOT_CONVERGED;
+}
+
 // FP64 iterative refinement around the mixed-precision inner CG:
 // x += CG_mixed((S+shift) d = r), r = b - (S+shift) x in FP64.
 int solve_mixed(yq_ctx* c, double shift, const double* b, double* x, double tol,
@@ -1120,8 +1161,11 @@ int solve(yq_ctx* c, double shift, double tol, int max_iter, const double* b_dev
   int it = 0;
   double err = 0.0;
   int rc = YQ_OK;
-  if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->sr_prec == YQ_MIXED && !c->cplx && !c->ap) {
-    rc = solve_explicit_mixed(c, shift, b_dev, x_dev, tol > 0 ? tol : 1e-12, &it, &err);
+  if (c->sr_solver == YQ_SOLVE_EXPLICIT && c->sr_prec == YQ_MIXED && !c->ap) {
+    rc = c->cplx ? solve_explicit_mixed_c(c, shift, b_dev, x_dev, tol > 0 ? tol : 1e-12, &it,
+                                          &err)
+                 : solve_explicit_mixed(c, shift, b_dev, x_dev, tol > 0 ? tol : 1e-12, &it,
+                                        &err);
     if (rc != YQ_OK && rc != YQ_ERR_NOT_CONVERGED) return rc;
   } else if (c->sr_solver == YQ_SOLVE_EXPLICIT) {
     rc = solve_explicit(c, shift, b_dev, x_dev);

@@ -17,7 +17,9 @@
 //
 // SR: O is held conjugated, Oh = conj(O) (ns x dim complex, row-major), so
 // that with Oh read as a dim x ns column-major matrix A:
-//   S   = A A^H / ns          (rocBLAS zherk, trans N)
+//   S   = A A^H / ns          (the hand-written SYRK on the rows read as
+//                              2 dim reals, then Re / Im S from its 2 x 2
+//                              blocks: herk_mfma in api.hip)
 //   F   = A E_c / ns          (column sums below)
 //   O_c v = A^H v,  O_c^H y = A y   (zgemv C and N)
 // which are the adjoint products of SRMat.hpp:89-103, 142-149, 170-189.

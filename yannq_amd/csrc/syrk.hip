@@ -244,7 +244,7 @@ __device__ inline uint16_t to_bf16(float x) {
 }
 __device__ inline float from_bf16(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
 
-template <bool RS>
+template <bool DENSE, bool RS>
 __global__ void __launch_bounds__(256, 2) k_syrk_bf16x3(const SyrkArgs a) {
   __shared__ uint16_t sh[2][3][kT][kLd3];  // [J, I slab][hi, mid, lo][column][sample]
   int bi, bj;
@@ -256,8 +256,9 @@ __global__ void __launch_bounds__(256, 2) k_syrk_bf16x3(const SyrkArgs a) {
   const int s1 = min(a.ns, s0 + a.kchunk);
   // this thread stages column cl of both slabs for 16 consecutive samples
   const int cl = tid & (kT - 1), sb = (tid >> 7) * kPer3;
-  const Col cj = make_col<false>(a, (long long)bj * kT + cl);
-  const Col ci = make_col<false>(a, (long long)bi * kT + cl);
+  const Col cj = make_col<DENSE>(a, (long long)bj * kT + cl);
+  const Col ci = make_col<DENSE>(a, (long long)bi * kT + cl);
+  const size_t tstride = DENSE ? (size_t)a.dim : (size_t)a.m;
   double tj[kPer3], ti[kPer3], rsv[kPer3];
   int8_t gj[kPer3], gi[kPer3];
   int k_loaded = s0;
@@ -266,17 +267,22 @@ __global__ void __launch_bounds__(256, 2) k_syrk_bf16x3(const SyrkArgs a) {
 #pragma unroll
     for (int p = 0; p < kPer3; ++p) {
       const int s = min(k0 + sb + p, s1 - 1);
-      tj[p] = cj.tp[(size_t)s * a.m];
-      ti[p] = ci.tp[(size_t)s * a.m];
-      gj[p] = cj.sp[(size_t)s * a.n];
-      gi[p] = ci.sp[(size_t)s * a.n];
+      tj[p] = cj.tp[(size_t)s * tstride];
+      ti[p] = ci.tp[(size_t)s * tstride];
+      if (!DENSE) {
+        gj[p] = cj.sp[(size_t)s * a.n];
+        gi[p] = ci.sp[(size_t)s * a.n];
+      }
       if (RS) rsv[p] = a.rs[s];
     }
   };
   auto value = [&](const Col& q, double t, int8_t g, double r, bool in) {
-    double v = q.use_t ? t : 1.0;
-    v = (q.use_s && g < 0) ? -v : v;
-    v = v - q.om;
+    double v = t;
+    if (!DENSE) {
+      v = q.use_t ? t : 1.0;
+      v = (q.use_s && g < 0) ? -v : v;
+      v = v - q.om;
+    }
     if (RS) v = v * r;
     return in && q.live ? (float)v : 0.0f;
   };
@@ -288,8 +294,8 @@ __global__ void __launch_bounds__(256, 2) k_syrk_bf16x3(const SyrkArgs a) {
       for (int e = 0; e < 8; ++e) {
         const int p = half * 8 + e;
         const bool in = k_loaded + sb + p < s1;
-        const float x = value(cj, tj[p], gj[p], RS ? rsv[p] : 1.0, in);
-        const float y = value(ci, ti[p], gi[p], RS ? rsv[p] : 1.0, in);
+        const float x = value(cj, tj[p], DENSE ? 0 : gj[p], RS ? rsv[p] : 1.0, in);
+        const float y = value(ci, ti[p], DENSE ? 0 : gi[p], RS ? rsv[p] : 1.0, in);
         const uint16_t xh = to_bf16(x), yh = to_bf16(y);
         const float xr = x - from_bf16(xh), yr = y - from_bf16(yh);
         const uint16_t xm = to_bf16(xr), ym = to_bf16(yr);
@@ -368,6 +374,25 @@ __global__ void __launch_bounds__(256, 2) k_syrk_bf16x3(const SyrkArgs a) {
     }
 }
 
+// A complex S = A A^H (A = conj(O_c), the stored rows) from the real product
+// XX = X^T X of X = the same rows read as 2 dim reals (re, im interleaved):
+// with A = R + i I, S_ij = sum_s (R_si + i I_si)(R_sj - i I_sj), so
+// Re S_ij = XX(2i, 2j) + XX(2i+1, 2j+1) and Im S_ij = XX(2i+1, 2j) -
+// XX(2i, 2j+1), both in XX's lower triangle for i > j (Im S_ii = 0).  XX and S
+// are column-major lower triangles; S is complex interleaved.
+__global__ void k_herk_combine(const double* __restrict__ XX, size_t dim, double* __restrict__ S) {
+  const size_t total = dim * dim, d2 = 2 * dim;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = q % dim, j = q / dim;
+    if (i < j) continue;
+    const double re = XX[2 * i + 2 * j * d2] + XX[(2 * i + 1) + (2 * j + 1) * d2];
+    const double im = i == j ? 0.0 : XX[(2 * i + 1) + 2 * j * d2] - XX[2 * i + (2 * j + 1) * d2];
+    S[2 * q] = re;
+    S[2 * q + 1] = im;
+  }
+}
+
 SyrkArgs base_args(int ns, size_t dim, double alpha, double* S, int chunks, double* part) {
   SyrkArgs a{};
   a.ns = ns;
@@ -440,10 +465,32 @@ int launch_syrk_bf16x3(const int8_t* sigma, const double* T, const double* omean
   {
     ProfGuard g("syrk_bf16x3", st);
     const dim3 grid((unsigned)tiles, (unsigned)chunks), block(256);
-    hipLaunchKernelGGL((k_syrk_bf16x3<false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_syrk_bf16x3<false, false>), grid, block, 0, st, a);
   }
   if (chunks > 1) launch_sum_lower(a.S, chunks, dim, 1, S, st);
   return YQ_OK;
+}
+
+int launch_syrk_bf16x3_dense(const double* O, int ns, size_t dim, double alpha, double* S,
+                             int chunks, double* part, hipStream_t st) {
+  if (ns < 1 || dim < 1 || chunks < 1 || (chunks > 1 && !part)) return YQ_ERR_ARG;
+  SyrkArgs a = base_args(ns, dim, alpha, S, chunks, part);
+  a.O = O;
+  a.dense = 1;
+  const long long tiles = (long long)a.nb * (a.nb + 1) / 2;
+  {
+    ProfGuard g("syrk_bf16x3", st);
+    const dim3 grid((unsigned)tiles, (unsigned)chunks), block(256);
+    hipLaunchKernelGGL((k_syrk_bf16x3<true, false>), grid, block, 0, st, a);
+  }
+  if (chunks > 1) launch_sum_lower(a.S, chunks, dim, 1, S, st);
+  return YQ_OK;
+}
+
+void launch_herk_combine(const double* XX, size_t dim, double* S, hipStream_t st) {
+  const size_t total = dim * dim;
+  const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_herk_combine, dim3(grid), dim3(256), 0, st, XX, dim, S);
 }
 
 int launch_syrk64_dense(const double* O, int ns, size_t dim, double alpha, double* S, int chunks,

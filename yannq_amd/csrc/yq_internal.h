@@ -305,6 +305,12 @@ int launch_syrk_bf16x3(const int8_t* sigma, const double* T, const double* omean
                        double* S, int chunks, double* part, hipStream_t st);
 int launch_syrk64_dense(const double* O, int ns, size_t dim, double alpha, double* S, int chunks,
                         double* part, hipStream_t st);
+// the split-bf16 S of a dense row-major O (ns x dim)
+int launch_syrk_bf16x3_dense(const double* O, int ns, size_t dim, double alpha, double* S,
+                             int chunks, double* part, hipStream_t st);
+// complex S = A A^H (lower, interleaved, dim x dim) from XX = X^T X (lower,
+// 2 dim x 2 dim) of the same rows read as 2 dim reals
+void launch_herk_combine(const double* XX, size_t dim, double* S, hipStream_t st);
 // ---------------------------------------------------------------- exact sums
 // full-basis / sector configurations as spins, |psi|^2 weights (and their
 // row scales sqrt(B w)), weighted E_loc statistics, the CDF and the draws of
