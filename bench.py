@@ -230,8 +230,9 @@ def precision_errors(yq):
     operator S v (random v) and the SR update delta-theta ((S + 1e-3 I) x = F,
     the refined mixed solve vs an FP64 solve to 1e-12).  Complex XXZ N=64,
     alpha=2, 4096 chains x 4 sweeps (Sz = 0, exchange moves): E_loc mean and F
-    with float theta and float E_loc (the SR build is FP64 for complex
-    machines).  Prints one JSON line."""
+    with float theta and float E_loc, S v with the split-bf16 explicit S, and
+    delta-theta from its FP64-refined Cholesky solve, against the FP64 explicit
+    build.  Prints one JSON line."""
     def rel(a, b):
         return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
     out = {}
@@ -268,17 +269,25 @@ def precision_errors(yq):
     ctx.randomize(n2 // 2)
     ctx.sample(4, 20)
     sig, _ = ctx.samples()
+    vc = None  # a random complex vector for S v, drawn once the dimension is known
     res = {}
     for prec in (yq.FP64, yq.MIXED):
         ctx.samples_set(sig, precision=prec)
         e, mean, _ = ctx.eloc()
-        ctx.sr_build(yq.SOLVE_CG, precision=yq.FP64)
-        res[prec] = (e, mean, ctx.sr_get(yq.SR_F))
+        # the explicit build: FP64 S, or split-bf16 S with the FP64-refined solve
+        ctx.sr_build(yq.SOLVE_EXPLICIT, precision=prec)
+        if vc is None:
+            r = np.random.default_rng(SEED)
+            vc = r.normal(size=ctx.sr_dim) + 1j * r.normal(size=ctx.sr_dim)
+        Sv = ctx.sr_apply(0.0, vc)
+        x, _, _ = ctx.sr_solve(LAMBDA, tol=1e-12 if prec == yq.FP64 else TOL, max_iter=1)
+        res[prec] = (e, mean, ctx.sr_get(yq.SR_F), Sv, x)
     a, b = res[yq.MIXED], res[yq.FP64]
     out["xxz_n64_a2_complex"] = {
         "samples": int(sig.shape[0]),
         "energy_rel_err": abs(a[1] - b[1]) / abs(b[1]), "eloc_rel_err": rel(a[0], b[0]),
-        "F_rel_err": rel(a[2], b[2])}
+        "F_rel_err": rel(a[2], b[2]), "Sv_rel_err": rel(a[3], b[3]),
+        "dtheta_rel_err": rel(a[4], b[4])}
     ctx.close()
     print(json.dumps(out), flush=True)
 
