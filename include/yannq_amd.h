@@ -98,11 +98,12 @@ YQ_API const char* yq_last_error(const yq_ctx* ctx);
 
 /* ---- Machine: Machines/RBM.hpp ---------------------------------------- */
 /* RBM(n, m, useBias), all parameters zero: RBM.hpp:50-56.  dtype is a
- * yq_dtype; 2 <= n <= 4096 and 1 <= m <= 8192.  Complex machines run the
- * explicit and dense CG solvers (with YQ_MIXED sampling and E_loc run in
- * float, and the explicit S is formed in split-bf16 and solved by FP64-
- * refined Cholesky); the factored solver is real-only.  Parallel tempering
- * and the many-machine calls need m <= 512. */
+ * yq_dtype; 2 <= n <= 4096 and 1 <= m <= 8192.  Complex machines run every
+ * solver: with YQ_MIXED sampling and E_loc run in float, the explicit S is
+ * formed in split-bf16 and solved by FP64-refined Cholesky, and the dense and
+ * factored CG stay FP64 (the factored complex CG composes four real factored
+ * products per complex one and has no Jacobi option).  Parallel tempering and
+ * the many-machine calls need m <= 512. */
 YQ_API int yq_rbm_create(yq_ctx* ctx, int dtype, int n, int m, int use_bias);
 /* getN / getM / getDim: RBM.hpp:115-130 */
 YQ_API int yq_rbm_dims(const yq_ctx* ctx, int* n, int* m, int* dim);

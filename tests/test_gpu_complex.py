@@ -141,8 +141,48 @@ def test_complex_sr_build_and_solve(ctx, orc, solver):
         x2, res2, _ = ctx.sr_solve(lam, tol=1e-11, max_iter=20000)
         ctx.sr_options(0, 0)
         assert res2 < 1e-10 and _rel(x2, x_ref) < 1e-8
+
+
+@pytest.mark.parametrize("n,m,ns", [(16, 24, 500), (64, 128, 2048)])
+def test_complex_factored_build_and_solve(ctx, orc, n, m, ns):
+    """The factored build of a complex machine (O never stored; every product
+    four real FP64 MFMA factored products on the planes of T): omean, F and the
+    operator against the oracle, and the CG solution against a direct solve
+    (small shape) or the dense CG (C2 shape)."""
+    rbm = _setup(ctx, orc, n, m, scale=0.1 if n < 64 else 0.02)
+    ctx.ham_set(1, n, J=1.0, delta=1.0)
+    rng = np.random.default_rng(n + 1)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig)
+    ctx.sr_build(2)
+    lam = 1e-3
+    v = rng.normal(size=ctx.sr_dim) + 1j * rng.normal(size=ctx.sr_dim)
     with pytest.raises(Exception):
-        ctx.sr_build(2)  # the factored solver is real-only
+        ctx.sr_get(2)  # O is not materialised
+    if n < 64:
+        th = orc.c_theta(rbm, sig)
+        el = orc.c_eloc(orc.ham(1, n, J=1.0, delta=1.0), rbm, sig, th)
+        O, om, F, S = orc.c_sr_build(rbm, sig, th, el)
+        assert _rel(ctx.sr_get(1), om) < 1e-12
+        assert _rel(ctx.sr_get(0), F) < 1e-10
+        A = S + lam * np.eye(len(F))
+        assert _rel(ctx.sr_apply(lam, v), A @ v) < 1e-12
+        x_ref = np.linalg.solve(A, F)
+    else:
+        F = ctx.sr_get(0)
+        Av = ctx.sr_apply(lam, v)
+        ctx.sr_build(1)   # the dense build of the same batch
+        assert _rel(ctx.sr_get(0), F) < 1e-12
+        assert _rel(ctx.sr_apply(lam, v), Av) < 1e-12
+        x_ref, r_ref, _ = ctx.sr_solve(lam, tol=1e-11, max_iter=20000)
+        ctx.sr_build(2)
+    x, res, it = ctx.sr_solve(lam, tol=1e-11, max_iter=20000)
+    assert res < 1e-10 and it > 0
+    assert _rel(x, x_ref) < 1e-8
+    ctx.sr_options(precond=1)
+    with pytest.raises(Exception):
+        ctx.sr_solve(lam, tol=1e-11, max_iter=20000)
+    ctx.sr_options(0, 0)
 
 
 @pytest.mark.parametrize("n,m,ns", [(12, 20, 700), (32, 64, 2048)])

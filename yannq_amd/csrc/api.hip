@@ -122,6 +122,7 @@ struct yq_ctx {
   int sr_solver = 0, sr_prec = 0;
   Buf T64, T32, O, omean, F, S, M, partial, ecent, info;
   Buf hxx;  // complex explicit build: X^T X of the rows read as reals
+  Buf cft, cfs;  // complex factored build: T planes; operator scratch
   Buf vx, vr, vp, vAp, vy, vtmp, vb, dotp;
   // device scalars: [0, 1] <omean, v> and sum y of the FP64 factored operator,
   // [2, 3] <b, b> and <r, r> of the mixed refinement
@@ -446,19 +447,122 @@ int syrk_real(yq_ctx* c, const double* Odense, size_t dim, int ns, double alpha,
 // Complex machine: dense conj(O) (see complex.hip); S only for the explicit
 // solver, in FP64, or with YQ_MIXED in split-bf16 (solved by FP64-refined
 // Cholesky); the dense CG runs in FP64 whatever the precision.
+// Scratch of the complex factored operator (doubles from c->cfs): the planes
+// of v (2 dim), the four forward products (4 ns), y as planes and interleaved
+// (4 ns), the four backward products (4 dim) and three complex scalars.
+struct CfScratch {
+  double *vr, *vi, *P[4], *yr, *yi, *yz, *B[4], *ov, *sy, *zero;
+};
+CfScratch cf_scratch(yq_ctx* c) {
+  const size_t ns = (size_t)c->ns, dim = c->dim;
+  double* q = c->cfs.as<double>();
+  CfScratch w;
+  w.vr = q;
+  w.vi = q + dim;
+  q += 2 * dim;
+  for (int k = 0; k < 4; ++k) w.P[k] = q + k * ns;
+  q += 4 * ns;
+  w.yr = q;
+  w.yi = q + ns;
+  w.yz = q + 2 * ns;
+  q += 4 * ns;
+  for (int k = 0; k < 4; ++k) w.B[k] = q + k * dim;
+  q += 4 * dim;
+  w.ov = q;
+  w.sy = q + 2;
+  w.zero = q + 4;
+  return w;
+}
+size_t cf_scratch_doubles(int ns, size_t dim) { return 6 * dim + 8 * (size_t)ns + 6; }
+
+// The four real backward products of a complex y (planes yr, yi) on the
+// factored T planes: B0 = O_r^T yr, B1 = O_i^T yi, B2 = O_r^T yi, B3 = O_i^T yr
+// with O_r / O_i the factored rows of Re / Im T
+int cf_backward(yq_ctx* c, const double* yr, const double* yi, double* const (&B)[4]) {
+  const int ns = c->ns, n = c->n, m = c->m;
+  const double* Tr = c->cft.as<double>();
+  const double* Ti = Tr + (size_t)ns * m;
+  const int8_t* sg = c->bsig.as<int8_t>();
+  double* part = c->partial.as<double>();
+  launch_factored_bwd64(sg, Tr, ns, n, m, c->use_bias, yr, B[0], part, c->st, 1.0);
+  launch_factored_bwd64(sg, Ti, ns, n, m, c->use_bias, yi, B[1], part, c->st, 0.0);
+  launch_factored_bwd64(sg, Tr, ns, n, m, c->use_bias, yi, B[2], part, c->st, 1.0);
+  launch_factored_bwd64(sg, Ti, ns, n, m, c->use_bias, yr, B[3], part, c->st, 0.0);
+  LAUNCHCHK(c);
+  return YQ_OK;
+}
+
+// Factored complex build (FP64): O_s = sigma'_s (x) T'_s with complex T' =
+// [tanh theta | 1] is never stored; every product is four real factored
+// products on the planes of T (sr.hip's FP64 MFMA kernels).  The device keeps
+// conj(omean) and F as the dense build does.
+int sr_build_cf(yq_ctx* c) {
+  const int ns = c->ns, n = c->n, m = c->m;
+  const size_t dim = c->dim;
+  TRY(ensure(c, c->cft, sizeof(double) * 2 * (size_t)ns * m));
+  TRY(ensure(c, c->cfs, sizeof(double) * cf_scratch_doubles(ns, dim)));
+  TRY(ensure(c, c->partial, sizeof(double) * aug_partial_doubles(ns, n, m)));
+  double* Tr = c->cft.as<double>();
+  double* Ti = Tr + (size_t)ns * m;
+  launch_split_c(c->T64.as<double>(), (size_t)ns * m, Tr, Ti, c->st);
+  const CfScratch w = cf_scratch(c);
+  // omean = (mean of the Re T rows, mean of the Im T rows); kept conjugated
+  launch_factored_mean64(c->bsig.as<int8_t>(), Tr, ns, n, m, c->use_bias, w.B[0],
+                         c->partial.as<double>(), c->st);
+  launch_factored_mean64(c->bsig.as<int8_t>(), Ti, ns, n, m, c->use_bias, w.B[1],
+                         c->partial.as<double>(), c->st, 0.0);
+  launch_conj_merge(w.B[0], w.B[1], dim, c->omean.as<double>(), c->st);
+  LAUNCHCHK(c);
+  // F = conj(O)^T (E - mean E) / ns = O_c^H E / ns (SRMat.hpp:89-103): the
+  // centred energies formed on the host in sample order
+  std::vector<double> e(2 * (size_t)ns);
+  HIPCHK(c, hipMemcpyAsync(e.data(), c->eloc.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost,
+                           c->st));
+  TRY(sync(c));
+  double er = 0.0, ei = 0.0;
+  for (int s = 0; s < ns; ++s) {
+    er += e[2 * s];
+    ei += e[2 * s + 1];
+  }
+  er /= ns;
+  ei /= ns;
+  std::vector<double> h(2 * (size_t)ns);
+  for (int s = 0; s < ns; ++s) {
+    h[s] = (e[2 * s] - er) / ns;
+    h[ns + s] = (e[2 * s + 1] - ei) / ns;
+  }
+  HIPCHK(c, hipMemcpyAsync(w.yr, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                           c->st));
+  TRY(cf_backward(c, w.yr, w.yi, w.B));
+  HIPCHK(c, hipMemsetAsync(w.zero, 0, sizeof(double) * 2, c->st));
+  // (B0 + B1, B2 - B3): conj(O)^T of the centred energies
+  launch_cbwd_combine(w.B[0], w.B[1], w.B[2], w.B[3], c->omean.as<double>(), w.zero,
+                      c->omean.as<double>(), 0.0, dim, c->F.as<double>(), c->st);
+  LAUNCHCHK(c);
+  TRY(sync(c));
+  c->have_sr = true;
+  c->diag_ready = false;  // the Jacobi option is not offered for this build
+  c->sr_solver = YQ_SOLVE_CG_FACTORED;
+  c->sr_prec = YQ_FP64;
+  return YQ_OK;
+}
+
 int sr_build_c(yq_ctx* c, int solver, int precision) {
   const int ns = c->ns, n = c->n, m = c->m;
   const size_t dim = c->dim;
-  if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG)
+  if (solver != YQ_SOLVE_EXPLICIT && solver != YQ_SOLVE_CG && solver != YQ_SOLVE_CG_FACTORED)
+    return fail(c, YQ_ERR_ARG, "unknown solver");
+  if (solver == YQ_SOLVE_CG_FACTORED && c->weighted)
     return fail(c, YQ_ERR_UNSUPPORTED,
-                "complex machines support the explicit and dense CG solvers");
+                "an exact weighted batch uses the explicit or the dense CG solver");
   TRY(ensure(c, c->omean, sizeof(double) * 2 * dim));
   TRY(ensure(c, c->F, sizeof(double) * 2 * dim));
   TRY(ensure(c, c->T64, sizeof(double) * 2 * (size_t)ns * m));
+  launch_tanh_c(c->btheta.p, c->bprec == YQ_MIXED, c->T64.as<double>(), (size_t)ns * m, c->st);
+  if (solver == YQ_SOLVE_CG_FACTORED) return sr_build_cf(c);  // O is not stored
   TRY(ensure(c, c->O, sizeof(double) * 2 * (size_t)ns * dim));
   const int chunks = colsum_chunks(ns);
   TRY(ensure(c, c->partial, sizeof(double) * 2 * (size_t)chunks * dim));
-  launch_tanh_c(c->btheta.p, c->bprec == YQ_MIXED, c->T64.as<double>(), (size_t)ns * m, c->st);
   double* Oh = c->O.as<double>();
   launch_build_Oh(c->bsig.as<int8_t>(), c->T64.as<double>(), ns, n, m, c->use_bias, Oh, c->st);
   // column means of conj(O) (complex columns as pairs of doubles); an exact
@@ -691,6 +795,9 @@ int sr_build(yq_ctx* c, int solver, int precision) {
 // scaled O_c^2); the complex and AmplitudePhase builds form it themselves
 int ensure_diag(yq_ctx* c) {
   if (c->diag_ready) return YQ_OK;
+  if (c->cplx)  // the dense complex builds form it; the factored one does not
+    return fail(c, YQ_ERR_UNSUPPORTED,
+                "the Jacobi option is not offered for the factored build of a complex machine");
   const int ns = c->ns, n = c->n, m = c->m;
   const size_t dim = c->dim;
   TRY(ensure(c, c->diagS, sizeof(double) * dim));
@@ -727,6 +834,30 @@ int apply64(yq_ctx* c, double shift, const double* v, double* out) {
       BLASCHK(c, rocblas_dgemv(c->blas, rocblas_operation_none, d, ns, &one, b.O, d, y, 1,
                                &shift, out + b.off, 1));
     }
+  } else if (c->cplx && c->sr_solver == YQ_SOLVE_CG_FACTORED) {
+    // y = O_c v / ns = (O v - (omean . v)) / ns, out = conj(O)^T y -
+    // conj(omean) sum(y) + shift v, each complex product four real factored
+    // ones on the planes of T; the device keeps conj(omean)
+    const int n = c->n, m = c->m;
+    const double* Tr = c->cft.as<double>();
+    const double* Ti = Tr + (size_t)ns * m;
+    const int8_t* sg = c->bsig.as<int8_t>();
+    double* part = c->partial.as<double>();
+    const CfScratch w = cf_scratch(c);
+    launch_split_c(v, dim, w.vr, w.vi, c->st);
+    launch_factored_fwd64(sg, Tr, ns, n, m, c->use_bias, w.vr, w.P[0], part, c->st, 1.0);
+    launch_factored_fwd64(sg, Ti, ns, n, m, c->use_bias, w.vi, w.P[1], part, c->st, 0.0);
+    launch_factored_fwd64(sg, Tr, ns, n, m, c->use_bias, w.vi, w.P[2], part, c->st, 1.0);
+    launch_factored_fwd64(sg, Ti, ns, n, m, c->use_bias, w.vr, w.P[3], part, c->st, 0.0);
+    launch_cdot1(c->omean.as<double>(), v, dim, w.ov, c->st);  // omean . v
+    launch_cfwd_combine(w.P[0], w.P[1], w.P[2], w.P[3], w.ov, ns, 1.0 / ns, w.yr, w.yi, w.yz,
+                        c->st);
+    launch_cdot1(w.yz, nullptr, ns, w.sy, c->st);  // sum y
+    LAUNCHCHK(c);
+    TRY(cf_backward(c, w.yr, w.yi, w.B));
+    launch_cbwd_combine(w.B[0], w.B[1], w.B[2], w.B[3], c->omean.as<double>(), w.sy, v, shift,
+                        dim, out, c->st);
+    LAUNCHCHK(c);
   } else if (c->cplx) {
     // y = A^H v / ns = O_c v / ns, out = A y + shift v = O_c^H y + shift v
     TRY(ensure(c, c->vy, sizeof(double) * 2 * ns));
@@ -1917,6 +2048,8 @@ YQ_API int yq_sr_get(yq_ctx* c, int item, double* out) {
         return sync(c);
       case YQ_SR_OMEAN:
       case YQ_SR_O: {
+        if (item == YQ_SR_O && c->sr_solver == YQ_SOLVE_CG_FACTORED)
+          return fail(c, YQ_ERR_STATE, "the factored solver does not materialise O");
         const size_t cnt = item == YQ_SR_O ? 2 * dim * c->ns : 2 * dim;
         const Buf& b = item == YQ_SR_O ? c->O : c->omean;
         HIPCHK(c, hipMemcpyAsync(out, b.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->st));

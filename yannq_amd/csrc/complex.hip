@@ -825,6 +825,109 @@ void launch_eloc_stats_c(const double* eloc, int ns, double* out, hipStream_t st
   hipLaunchKernelGGL(k_eloc_stats_c, dim3(1), dim3(1024), 0, st, eloc, ns, out);
 }
 
+// ---- factored complex SR operator (FP64), composed of real factored products
+namespace {
+__global__ void k_split_c(const double* __restrict__ z, size_t n, double* __restrict__ re,
+                          double* __restrict__ im) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  re[i] = z[2 * i];
+  im[i] = z[2 * i + 1];
+}
+// y = (P1 - P2 - Re ov, P3 + P4 - Im ov) * scale, as planes and interleaved
+__global__ void k_cfwd_combine(const double* __restrict__ P1, const double* __restrict__ P2,
+                               const double* __restrict__ P3, const double* __restrict__ P4,
+                               const double* __restrict__ ov, size_t n, double scale,
+                               double* __restrict__ yr, double* __restrict__ yi,
+                               double* __restrict__ yz) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = ((P1[i] - P2[i]) - ov[0]) * scale, b = ((P3[i] + P4[i]) - ov[1]) * scale;
+  yr[i] = a;
+  yi[i] = b;
+  yz[2 * i] = a;
+  yz[2 * i + 1] = b;
+}
+// out = (B1 + B2, B3 - B4) - a * sy + shift v, interleaved; a and v
+// interleaved, sy one complex scalar
+__global__ void k_cbwd_combine(const double* __restrict__ B1, const double* __restrict__ B2,
+                               const double* __restrict__ B3, const double* __restrict__ B4,
+                               const double* __restrict__ a, const double* __restrict__ sy,
+                               const double* __restrict__ v, double shift, size_t n,
+                               double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ar = a[2 * i], ai = a[2 * i + 1];
+  const double cr = ar * sy[0] - ai * sy[1], ci = ar * sy[1] + ai * sy[0];
+  out[2 * i] = ((B1[i] + B2[i]) - cr) + shift * v[2 * i];
+  out[2 * i + 1] = ((B3[i] - B4[i]) - ci) + shift * v[2 * i + 1];
+}
+// out[0..1] = sum_i conj(a_i) b_i (a, b interleaved), or sum_i a_i when b is
+// null; one block, fixed order: thread-strided sums, then a tree over the
+// block
+__global__ void __launch_bounds__(1024) k_cdot1(const double* __restrict__ a,
+                                                const double* __restrict__ b, size_t n,
+                                                double* __restrict__ out) {
+  __shared__ double sr[1024], si[1024];
+  double xr = 0.0, xi = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 1024) {
+    const double ar = a[2 * i], ai = a[2 * i + 1];
+    if (b) {
+      const double br = b[2 * i], bi = b[2 * i + 1];
+      xr += ar * br + ai * bi;
+      xi += ar * bi - ai * br;
+    } else {
+      xr += ar;
+      xi += ai;
+    }
+  }
+  sr[threadIdx.x] = xr;
+  si[threadIdx.x] = xi;
+  __syncthreads();
+  for (int w = 512; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sr[0];
+    out[1] = si[0];
+  }
+}
+// z = (re, -im): the conjugate, interleaved
+__global__ void k_conj_merge(const double* __restrict__ re, const double* __restrict__ im,
+                             size_t n, double* __restrict__ z) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  z[2 * i] = re[i];
+  z[2 * i + 1] = -im[i];
+}
+}  // namespace
+
+void launch_split_c(const double* z, size_t n, double* re, double* im, hipStream_t st) {
+  hipLaunchKernelGGL(k_split_c, dim3(nblk(n)), dim3(256), 0, st, z, n, re, im);
+}
+void launch_cfwd_combine(const double* P1, const double* P2, const double* P3, const double* P4,
+                         const double* ov, size_t n, double scale, double* yr, double* yi,
+                         double* yz, hipStream_t st) {
+  hipLaunchKernelGGL(k_cfwd_combine, dim3(nblk(n)), dim3(256), 0, st, P1, P2, P3, P4, ov, n,
+                     scale, yr, yi, yz);
+}
+void launch_cbwd_combine(const double* B1, const double* B2, const double* B3, const double* B4,
+                         const double* a, const double* sy, const double* v, double shift,
+                         size_t n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cbwd_combine, dim3(nblk(n)), dim3(256), 0, st, B1, B2, B3, B4, a, sy, v,
+                     shift, n, out);
+}
+void launch_cdot1(const double* a, const double* b, size_t n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cdot1, dim3(1), dim3(1024), 0, st, a, b, n, out);
+}
+void launch_conj_merge(const double* re, const double* im, size_t n, double* z, hipStream_t st) {
+  hipLaunchKernelGGL(k_conj_merge, dim3(nblk(n)), dim3(256), 0, st, re, im, n, z);
+}
+
 void launch_tanh_c(const void* theta, int in_float, double* T, size_t count, hipStream_t st) {
   if (in_float)
     hipLaunchKernelGGL(k_tanh_c<float>, dim3(nblk(count)), dim3(256), 0, st,

@@ -174,9 +174,12 @@ __device__ inline double sig_aug(const int8_t* __restrict__ sigma, int ns, int n
   if (i == n) return 1.0;
   return sigma[(size_t)s * n + i] > 0 ? 1.0 : -1.0;
 }
-__device__ inline double t_aug(const double* __restrict__ T, int ns, int m, int s, int j) {
+// T' = [T | tb]: tb = 1 for a real machine (and the real plane of a complex
+// one), 0 for the imaginary plane of a complex machine's T
+__device__ inline double t_aug(const double* __restrict__ T, int ns, int m, int s, int j,
+                               double tb = 1.0) {
   if (s >= ns || j > m) return 0.0;
-  if (j == m) return 1.0;
+  if (j == m) return tb;
   return T[(size_t)s * m + j];
 }
 __device__ inline double v_aug(const double* __restrict__ v, int n, int m, int use_bias, int k,
@@ -210,7 +213,7 @@ constexpr int kF64Kc = 32;  // k per LDS stage (8 k-steps)
 __global__ void __launch_bounds__(256)
     k_fwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
                  int m, const double* __restrict__ va, int Kp, int Np, int nrows,
-                 double* __restrict__ ypart) {
+                 double* __restrict__ ypart, double tb) {
   constexpr int kCols = kF64Tiles * 16;
   __shared__ double vsh[kF64Kc][kCols + 1];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
@@ -292,7 +295,8 @@ __global__ void __launch_bounds__(256)
       if (q < nct) {
         const int j = (ct0 + q) * 16 + li;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) acc[e] += t_aug(T, ns, m, r0 + lg + 4 * e, j) * c[rt][q][e];
+        for (int e = 0; e < 4; ++e)
+          acc[e] += t_aug(T, ns, m, r0 + lg + 4 * e, j, tb) * c[rt][q][e];
       }
 #pragma unroll
     for (int e = 0; e < 4; ++e)
@@ -326,7 +330,7 @@ template <int NIT>
 __global__ void __launch_bounds__(256)
     k_bwd64_mfma(const int8_t* __restrict__ sigma, const double* __restrict__ T, int ns, int n,
                  int m, const double* __restrict__ r, int Kp, int Np, int chunk,
-                 double* __restrict__ part) {
+                 double* __restrict__ part, double tb) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int ct = blockIdx.x * 4 + wave;
   if (ct >= Np / 16) return;  // wave-uniform; no barrier in this kernel
@@ -348,7 +352,7 @@ __global__ void __launch_bounds__(256)
     for (int u = 0; u < kU64; ++u) {
       const int s = s0 + min(k0 + u, nks - 1) * 4 + lg;  // clamped: unused past nks
       const int sc = min(s, ns - 1);
-      ob[u] = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j);
+      ob[u] = (s < ns ? r[s] : 0.0) * t_aug(T, ns, m, s, j, tb);
 #pragma unroll
       for (int it = 0; it < NIT; ++it)
         oa[u][it] = sigma[(size_t)sc * n + min(i0 + it * 16 + li, n - 1)];
@@ -639,7 +643,7 @@ size_t factored_partial_doubles(int ns, int n, int m) {
 
 void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* v, double* y,
-                           double* scratch, hipStream_t st) {
+                           double* scratch, hipStream_t st, double tb) {
   const int Kp = kp16(n), Np = np16(m), nrows = (ns + 15) / 16 * 16;
   const int ncg = (Np / 16 + kF64Tiles - 1) / kF64Tiles;
   double* va = scratch + (size_t)ncg * nrows;
@@ -647,7 +651,7 @@ void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
   hipLaunchKernelGGL(k_aug64, dim3(nblk((size_t)Kp * Np)), dim3(256), 0, st, v, n, m, use_bias,
                      Kp, Np, va);
   hipLaunchKernelGGL(k_fwd64_mfma, dim3((nrows + 127) / 128, ncg), dim3(256), 0, st, sigma, T,
-                     ns, n, m, (const double*)va, Kp, Np, nrows, scratch);
+                     ns, n, m, (const double*)va, Kp, Np, nrows, scratch, tb);
   hipLaunchKernelGGL(k_sum_ypart, dim3(nblk(ns)), dim3(256), 0, st, scratch, ncg, nrows, ns, y);
 }
 
@@ -668,7 +672,7 @@ auto bwd64_kernel(int nit) {
 
 void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* r, double* g,
-                           double* partial, hipStream_t st) {
+                           double* partial, hipStream_t st, double tb) {
   const int chunks = bwd64_chunks(ns);
   const int Kp = kp16(n), Np = np16(m);
   ProfGuard pg("factored_bwd64", st);
@@ -676,7 +680,7 @@ void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
   const int nit = std::min(Kp / 16, kMaxIt64);
   const int rgroups = (Kp / 16 + nit - 1) / nit;
   hipLaunchKernelGGL(bwd64_kernel(nit), dim3((Np / 16 + 3) / 4, chunks, rgroups), dim3(256), 0,
-                     st, sigma, T, ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial);
+                     st, sigma, T, ns, n, m, r, Kp, Np, bwd64_chunk(ns), partial, tb);
   const size_t dim = use_bias ? (size_t)n * m + n + m : (size_t)n * m;
   hipLaunchKernelGGL(k_reduce_aug, dim3(nblk(dim)), dim3(256), 0, st, partial, chunks, n, m,
                      use_bias, Kp, Np, 1.0, g);
@@ -684,11 +688,11 @@ void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
 
 void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
                             int n, int m, int use_bias, double* omean,
-                            double* partial, hipStream_t st) {
+                            double* partial, hipStream_t st, double tb) {
   // r = 1/ns lives in the tail of the partial buffer
   double* ones = partial + factored_partial_doubles(ns, n, m) - ns;
   hipLaunchKernelGGL(k_fill, dim3(nblk(ns)), dim3(256), 0, st, ones, 1.0 / ns, ns);
-  launch_factored_bwd64(sigma, T, ns, n, m, use_bias, ones, omean, partial, st);
+  launch_factored_bwd64(sigma, T, ns, n, m, use_bias, ones, omean, partial, st, tb);
 }
 
 void launch_diag_S(const double* T, int ns, int n, int m, int use_bias, const double* omean,

@@ -229,17 +229,19 @@ void launch_copy_lower_shift(const double* S, double* M, int dim,
 // scratch: factored_partial_doubles(ns, n, m) doubles (shared with the
 // backward product's partials)
 size_t factored_partial_doubles(int ns, int n, int m);
+// tb: the augmented column of T' (1; 0 for the imaginary plane of a complex
+// machine's T, whose bias column is real)
 void launch_factored_fwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* v, double* y,
-                           double* scratch, hipStream_t st);
+                           double* scratch, hipStream_t st, double tb = 1.0);
 // g = O^T r (uncentred), length dim; partial has colsum_chunks(ns)*dim doubles
 void launch_factored_bwd64(const int8_t* sigma, const double* T, int ns, int n,
                            int m, int use_bias, const double* r, double* g,
-                           double* partial, hipStream_t st);
+                           double* partial, hipStream_t st, double tb = 1.0);
 // column mean of the factored O: omean = O^T 1 / ns
 void launch_factored_mean64(const int8_t* sigma, const double* T, int ns,
                             int n, int m, int use_bias, double* omean,
-                            double* partial, hipStream_t st);
+                            double* partial, hipStream_t st, double tb = 1.0);
 
 // diag(S) = Var(O_c) per parameter from T = tanh(theta) and the column mean;
 // partial needs (colsum_chunks(ns) + 1) * m doubles
@@ -270,6 +272,19 @@ void launch_eloc_combine_c(const ElocArgs& a, hipStream_t st);
 void launch_eloc_stats_c(const double* eloc, int ns, double* out, hipStream_t st);
 // T = tanh(theta), complex, from an interleaved double or float theta
 void launch_tanh_c(const void* theta, int in_float, double* T, size_t count, hipStream_t st);
+// factored complex SR operator pieces (complex.hip): interleaved complex <->
+// real and imaginary planes, the combinations of the four real forward and
+// backward products, a one-block complex dot (conj(a) . b, or sum a when b is
+// null) and the conjugate of a split vector, interleaved
+void launch_split_c(const double* z, size_t n, double* re, double* im, hipStream_t st);
+void launch_cfwd_combine(const double* P1, const double* P2, const double* P3, const double* P4,
+                         const double* ov, size_t n, double scale, double* yr, double* yi,
+                         double* yz, hipStream_t st);
+void launch_cbwd_combine(const double* B1, const double* B2, const double* B3, const double* B4,
+                         const double* a, const double* sy, const double* v, double shift,
+                         size_t n, double* out, hipStream_t st);
+void launch_cdot1(const double* a, const double* b, size_t n, double* out, hipStream_t st);
+void launch_conj_merge(const double* re, const double* im, size_t n, double* z, hipStream_t st);
 // Oh = conj(O), ns x dim complex row-major
 void launch_build_Oh(const int8_t* sigma, const double* T, int ns, int n, int m, int use_bias,
                      double* Oh, hipStream_t st);
