@@ -233,21 +233,44 @@ def test_complex_mixed_batch_matches_fp64(ctx, orc):
     assert _rel(out[1][2], out[0][2]) < 1e-4
 
 
-def test_complex_xxz_reaches_ground_state(ctx, orc):
+def test_complex_factored_float_batch_matches_dense(ctx, orc):
+    """A batch sampled in float (theta stored as float): the factored build
+    reads tanh of the widened theta and agrees with the dense build of the
+    same batch."""
+    n, m, ns = 16, 24, 500
+    _setup(ctx, orc, n, m, scale=0.1)
+    ctx.ham_set(1, n, J=1.0, delta=1.0)
+    rng = np.random.default_rng(5)
+    sig = rng.choice([-1, 1], size=(ns, n)).astype(np.int8)
+    ctx.samples_set(sig, precision=1)
+    ctx.eloc()
+    v = rng.normal(size=ctx.sr_dim) + 1j * rng.normal(size=ctx.sr_dim)
+    out = {}
+    for solver in (1, 2):
+        ctx.sr_build(solver)
+        out[solver] = (ctx.sr_get(0), ctx.sr_get(1), ctx.sr_apply(1e-3, v))
+    for a, b in zip(out[2], out[1]):
+        assert _rel(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("solver,prec", [(0, 0), (2, 1)])
+def test_complex_xxz_reaches_ground_state(ctx, orc, solver, prec):
     """SR with exchange moves on the Sz = 0 sector of the N=10 Heisenberg
-    chain (Marshall sign rule) reaches the exact ground-state energy."""
+    chain (Marshall sign rule) reaches the exact ground-state energy: the
+    explicit solver on FP64 chains, and the factored CG on float chains."""
     n = 10
     H = orc.ham(1, n, J=1.0, delta=1.0, sign_rule=1)
     e0 = np.linalg.eigvalsh(orc.ham_matrix(H).toarray())[0]
     ctx.rbm_create(n, 2 * n, dtype=np.complex128)
     ctx.init_random(17, 0.01)
     ctx.ham_set(1, n, J=1.0, delta=1.0, sign_rule=1)
-    ctx.sampler_config(1, 256, precision=0, seed=17)
+    ctx.sampler_config(1, 256, precision=prec, seed=17)
     ctx.randomize(n // 2)
     es = []
     for i in range(150):
-        o = ctx.sr_step(n_sweeps=4, n_discard=20 if i == 0 else 2, solver=0, precision=0,
-                        lam=1e-3, lr=0.02)
+        o = ctx.sr_step(n_sweeps=4, n_discard=20 if i == 0 else 2, solver=solver,
+                        precision=prec, lam=1e-3, lr=0.02, cg_tol=1e-6)
+        assert o["cg_err"] < 1e-6 or solver == 0
         es.append(o["energy"])
     e = np.mean(es[-20:])
     assert abs(e - e0) < 0.01 * abs(e0), (e, e0)
