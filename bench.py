@@ -212,6 +212,34 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_per_copy": 4 * n, "copies": 4}), flush=True)
 
 
+def stream_probe(reps=400):
+    """What plain streaming kernels achieve at the byte counts of one CG
+    product: torch's sum over the product's algorithmic bytes (a read), and
+    copies of that size and of the backward product's chunk partials (a read
+    and a write each), timed back to back with events on torch's stream."""
+    import torch
+    out = {}
+    sizes = {"sum_28MB": KERNEL_WORK["bwd_mfma"], "copy_28MB": KERNEL_WORK["bwd_mfma"],
+             "copy_13.6MB": 64 * KP * NP * 4}
+    for name, nbytes in sizes.items():
+        a = torch.ones(nbytes // 4, device="cuda")
+        b = torch.empty_like(a)
+        op = (lambda: a.sum()) if name.startswith("sum") else (lambda: b.copy_(a))
+        moved = nbytes * (1 if name.startswith("sum") else 2)
+        for _ in range(20):
+            op()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out[name] = {"bytes_moved": moved, "avg_us": us, "gbs": moved / us / 1e3,
+                     "hbm_frac": moved / us / 1e3 / HBM_PEAK_GBS}
+    print(json.dumps({"stream_probe": out, "reps": reps}), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC measurement, or
     None when there is none for it."""
@@ -500,6 +528,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="time plain streaming kernels at the CG products' byte counts")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the copy that calibrates the PMC traffic counters")
     ap.add_argument("--batched", action="store_true",
@@ -515,6 +545,9 @@ def main():
                     help="only 64 eager mixed CG iterations at this shape (a short load for "
                          "counter passes of the CG kernels)")
     args = ap.parse_args()
+    if args.stream_probe:
+        stream_probe()
+        return
     if args.copy_probe:
         copy_probe()
         return

@@ -513,26 +513,12 @@ int sr_build_cf(yq_ctx* c) {
                          c->partial.as<double>(), c->st, 0.0);
   launch_conj_merge(w.B[0], w.B[1], dim, c->omean.as<double>(), c->st);
   LAUNCHCHK(c);
-  // F = conj(O)^T (E - mean E) / ns = O_c^H E / ns (SRMat.hpp:89-103): the
-  // centred energies formed on the host in sample order
-  std::vector<double> e(2 * (size_t)ns);
-  HIPCHK(c, hipMemcpyAsync(e.data(), c->eloc.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost,
-                           c->st));
-  TRY(sync(c));
-  double er = 0.0, ei = 0.0;
-  for (int s = 0; s < ns; ++s) {
-    er += e[2 * s];
-    ei += e[2 * s + 1];
-  }
-  er /= ns;
-  ei /= ns;
-  std::vector<double> h(2 * (size_t)ns);
-  for (int s = 0; s < ns; ++s) {
-    h[s] = (e[2 * s] - er) / ns;
-    h[ns + s] = (e[2 * s + 1] - ei) / ns;
-  }
-  HIPCHK(c, hipMemcpyAsync(w.yr, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
-                           c->st));
+  // F = conj(O)^T (E - mean E) / ns = O_c^H E / ns (SRMat.hpp:89-103), the
+  // energies centred on the device (their sum held in w.sy, which every
+  // apply overwrites)
+  launch_cdot1(c->eloc.as<double>(), nullptr, (size_t)ns, w.sy, c->st);
+  launch_center_c(c->eloc.as<double>(), w.sy, (size_t)ns, w.yr, w.yi, c->st);
+  LAUNCHCHK(c);
   TRY(cf_backward(c, w.yr, w.yi, w.B));
   HIPCHK(c, hipMemsetAsync(w.zero, 0, sizeof(double) * 2, c->st));
   // (B0 + B1, B2 - B3): conj(O)^T of the centred energies

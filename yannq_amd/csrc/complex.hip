@@ -896,6 +896,16 @@ __global__ void __launch_bounds__(1024) k_cdot1(const double* __restrict__ a,
     out[1] = si[0];
   }
 }
+// (yr, yi) = (e - esum / n) / n as planes: the centred energies over n
+// (e interleaved, esum their complex sum)
+__global__ void k_center_c(const double* __restrict__ e, const double* __restrict__ esum,
+                           size_t n, double* __restrict__ yr, double* __restrict__ yi) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dn = (double)n;
+  yr[i] = (e[2 * i] - esum[0] / dn) / dn;
+  yi[i] = (e[2 * i + 1] - esum[1] / dn) / dn;
+}
 // z = (re, -im): the conjugate, interleaved
 __global__ void k_conj_merge(const double* __restrict__ re, const double* __restrict__ im,
                              size_t n, double* __restrict__ z) {
@@ -923,6 +933,10 @@ void launch_cbwd_combine(const double* B1, const double* B2, const double* B3, c
 }
 void launch_cdot1(const double* a, const double* b, size_t n, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_cdot1, dim3(1), dim3(1024), 0, st, a, b, n, out);
+}
+void launch_center_c(const double* e, const double* esum, size_t n, double* yr, double* yi,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_center_c, dim3(nblk(n)), dim3(256), 0, st, e, esum, n, yr, yi);
 }
 void launch_conj_merge(const double* re, const double* im, size_t n, double* z, hipStream_t st) {
   hipLaunchKernelGGL(k_conj_merge, dim3(nblk(n)), dim3(256), 0, st, re, im, n, z);

@@ -291,6 +291,9 @@ __device__ __forceinline__ void fwd_body(int bx, int by, const uint8_t* __restri
                                          const float* __restrict__ Trm, int Kp, int Np, int nsp,
                                          float* __restrict__ ypart, int* __restrict__ ictl,
                                          int count) {
+  // fused multiply-adds in the products (their sums need no replay; the
+  // split of bwd's y T' is exact either way, every piece a truncation)
+  #pragma clang fp contract(fast)
   extern __shared__ uint16_t bsh[];  // [3][CT*16][Kp + kLdsPad]
   const int tid = threadIdx.x;
   const int cg = by;
@@ -470,6 +473,9 @@ __global__ void __launch_bounds__(256)
     k_fwd_mfma_g(const uint8_t* __restrict__ sigA, int sig_row, const uint16_t* __restrict__ vs,
                  const float* __restrict__ Trm, int Kp, int Np, int nsp,
                  float* __restrict__ ypart, int* __restrict__ ictl, int count) {
+  // fused multiply-adds in the products (their sums need no replay; the
+  // split of bwd's y T' is exact either way, every piece a truncation)
+  #pragma clang fp contract(fast)
   constexpr int ldk = 32 * kFwdKc + kLdsPad;
   extern __shared__ uint16_t bsh[];  // [3][CT*16][ldk]
   const int tid = threadIdx.x;
@@ -597,6 +603,9 @@ __device__ __forceinline__ void bwd_body(int b, int gx, int total,
                                          int nsp, float* __restrict__ part,
                                          double* __restrict__ ys, double* __restrict__ yy,
                                          const int* __restrict__ ictl) {
+  // fused multiply-adds in the products (their sums need no replay; the
+  // split of bwd's y T' is exact either way, every piece a truncation)
+  #pragma clang fp contract(fast)
   extern __shared__ uint16_t ash[];  // [Kp][SLICE + kLdsPad]
   __shared__ float ysh[CHUNK];
   const int tid = threadIdx.x;

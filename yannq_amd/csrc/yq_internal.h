@@ -284,6 +284,8 @@ void launch_cbwd_combine(const double* B1, const double* B2, const double* B3, c
                          const double* a, const double* sy, const double* v, double shift,
                          size_t n, double* out, hipStream_t st);
 void launch_cdot1(const double* a, const double* b, size_t n, double* out, hipStream_t st);
+void launch_center_c(const double* e, const double* esum, size_t n, double* yr, double* yi,
+                     hipStream_t st);
 void launch_conj_merge(const double* re, const double* im, size_t n, double* z, hipStream_t st);
 // Oh = conj(O), ns x dim complex row-major
 void launch_build_Oh(const int8_t* sigma, const double* T, int ns, int n, int m, int use_bias,
