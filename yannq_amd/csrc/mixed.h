@@ -13,6 +13,8 @@ namespace yq {
 struct MixedWork {
   int ns = 0, n = 0, m = 0, use_bias = 1;
   int Kp = 0;    // n+1 rounded up to 32 (k-steps of the forward product)
+  int Kr = 0;    // n+1 rounded up to 16 (rows of the backward product's result,
+                 // of its chunk partials and of the CG-step grid)
   int Np = 0;    // m+1 rounded up to 16 (column tiles)
   int nsp = 0;   // ns rounded up to 512
   int ncg = 0;   // column groups of the forward product (Np/16 over ct)

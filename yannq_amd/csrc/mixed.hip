@@ -793,14 +793,19 @@ constexpr int kBwdSlice = 64;
 template <int CHUNK>
 auto bwd_kernel_it(int nit) {
   switch (nit) {
+    case 1: return k_bwd_mfma<CHUNK, kBwdSlice, 1>;
     case 2: return k_bwd_mfma<CHUNK, kBwdSlice, 2>;
+    case 3: return k_bwd_mfma<CHUNK, kBwdSlice, 3>;
     case 4: return k_bwd_mfma<CHUNK, kBwdSlice, 4>;
+    case 5: return k_bwd_mfma<CHUNK, kBwdSlice, 5>;
     case 6: return k_bwd_mfma<CHUNK, kBwdSlice, 6>;
+    case 7: return k_bwd_mfma<CHUNK, kBwdSlice, 7>;
     case 8: return k_bwd_mfma<CHUNK, kBwdSlice, 8>;
+    case 9: return k_bwd_mfma<CHUNK, kBwdSlice, 9>;
     default: return k_bwd_mfma<CHUNK, kBwdSlice, 10>;
   }
 }
-// (Kp is a multiple of 32 up to 160: nit = Kp / 16 is 2, 4, 6, 8 or 10)
+// (Kr = n + 1 rounded up to 16, at most 160 here: nit = Kr / 16 is 1 .. 10)
 auto bwd_kernel(int chunk, int nit) {
   return chunk == 128 ? bwd_kernel_it<128>(nit)
                       : chunk == 512 ? bwd_kernel_it<512>(nit) : bwd_kernel_it<256>(nit);
@@ -880,7 +885,8 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
   // the values are used in the order of the code below.
   constexpr int kPre = 16;
   float pre[kPre];
-  const size_t pstride = (size_t)Kp * Np;
+  // the chunk partials hold the grid's rows (Kr = n + 1 rounded up to 16)
+  const size_t pstride = (size_t)((total / gx) * 16) * Np;
   const size_t paug = (size_t)k * Np + j;  // aug(c)
   if (mode != 0 && c >= 0 && G == 4) {  // clamped chunk index: unconditional loads
 #pragma unroll
@@ -978,8 +984,8 @@ __device__ __forceinline__ void cg_step_body(int b, int gx, int total, float* __
       }
       grp[g][e] = acc;
     } else if (c >= 0) {
-      const size_t a = (size_t)k * Np + j;  // aug(c)
-      const size_t stride = (size_t)Kp * Np;
+      const size_t a = paug;
+      const size_t stride = pstride;
       float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       for (int q0 = 0; q0 < chunks; q0 += 32) {
         float v[4][8];
@@ -1190,7 +1196,7 @@ struct RedLayout {
 RedLayout red_layout(const MixedWork& w, int chunk = 0) {
   RedLayout L;
   if (chunk == 0) chunk = w.chunk;
-  L.nbv = (w.Kp / 16) * (w.Np / 16);         // k_cg_step tiles
+  L.nbv = (w.Kr / 16) * (w.Np / 16);         // k_cg_step tiles
   L.nby = w.nsp / chunk;                      // k_bwd_mfma chunks
   size_t o = 0;
   for (int q = 0; q < 2; ++q) {
@@ -1223,21 +1229,21 @@ void product_launches(MixedWork& w, int count, hipStream_t st) {
   if (w.Kp <= 32 * kMaxKs) {
     launch_prof("fwd_mfma", fwd_kernel(w.ct, w.Kp / 32), dim3(w.nsp / 256, w.ncg), dim3(256),
                 lds_f, st, w.sigA, w.vsplit, w.Trm, w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
-    const size_t lds = sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad);
-    launch_prof("bwd_mfma", bwd_kernel(w.chunk, w.Kp / 16), dim3((w.Np / 16 + 3) / 4, chunks),
+    const size_t lds = sizeof(uint16_t) * w.Kr * (kBwdSlice + kLdsPad);
+    launch_prof("bwd_mfma", bwd_kernel(w.chunk, w.Kr / 16), dim3((w.Np / 16 + 3) / 4, chunks),
                 dim3(256), lds, st, w.sigT, w.Tt, w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv,
-                w.Kp, w.Np, w.nsp, w.part, w.red + L.ys, w.red + L.yy, w.ictl);
+                w.Kr, w.Np, w.nsp, w.part, w.red + L.ys, w.red + L.yy, w.ictl);
     return;
   }
   const int srow = sig_row_bytes(w.Kp);
   launch_prof("fwd_mfma", w.ct == 2 ? k_fwd_mfma_g<2> : k_fwd_mfma_g<4>,
               dim3(w.nsp / 256, w.ncg), dim3(256), lds_f, st, w.sigA, srow, w.vsplit, w.Trm,
               w.Kp, w.Np, w.nsp, w.ypart, w.ictl, count);
-  const int groups = (w.Kp / 16 + kBwdGroupTiles - 1) / kBwdGroupTiles;
+  const int groups = (w.Kr / 16 + kBwdGroupTiles - 1) / kBwdGroupTiles;
   const size_t lds = sizeof(uint16_t) * 16 * kBwdGroupTiles * (kBwdSlice + kLdsPad);
   launch_prof("bwd_mfma", bwd_kernel_rg(w.chunk),
               dim3((w.Np / 16 + 3) / 4, chunks * groups), dim3(256), lds, st, w.sigT, w.Tt,
-              w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kp, w.Np, w.nsp, w.part,
+              w.ypart, w.ncg, w.ns, w.red + L.op, L.nbv, w.Kr, w.Np, w.nsp, w.part,
               w.red + L.ys, w.red + L.yy, w.ictl);
 }
 
@@ -1249,7 +1255,7 @@ int cg_groups(int chunks) { return chunks > 16 ? 4 : chunks > 8 ? 2 : 1; }
 void step_launch(MixedWork& w, const float* dinv, int mode, int par, hipStream_t st) {
   const RedLayout L = red_layout(w);
   const int wr = mode == 0 ? 0 : par ^ 1;
-  launch_prof("cg_step", k_cg_step, dim3(w.Np / 16, w.Kp / 16),
+  launch_prof("cg_step", k_cg_step, dim3(w.Np / 16, w.Kr / 16),
               dim3(256 * cg_groups(w.nsp / w.chunk)), 0, st, w.x, w.r,
               w.p, w.s, w.u, w.Ap, dinv, mode, par, w.red + L.g[par], w.red + L.rr[par],
               w.red + L.uu[par], L.nbv, w.red + L.ys, w.red + L.yy, L.nby, w.ns,
@@ -1296,7 +1302,7 @@ struct GroupDesc {
   long long dim;
   long long g[2], rr[2], uu[2], op, ys, yy;  // offsets into red (red_layout)
   double tol;            // relative tolerance of this call
-  int ns, n, m, use_bias, Kp, Np, nsp, ncg, ct, chunk, nbv, nby;
+  int ns, n, m, use_bias, Kp, Kr, Np, nsp, ncg, ct, chunk, nbv, nby;
   int fwd_gx, bwd_gx, bwd_total, cg_gx, cg_total;
   int in;                // takes part in this call
   int max_it;            // iteration budget of this call
@@ -1330,14 +1336,19 @@ __global__ void __launch_bounds__(256)
   const int2 e = bt[blockIdx.x];
   const GroupDesc& d = gd[e.x];
 #define YQ_BWD_ARGS                                                                            \
-  e.y, d.bwd_gx, d.bwd_total, d.sigT, d.Tt, d.ypart, d.ncg, d.ns, d.red + d.op, d.nbv, d.Kp,   \
+  e.y, d.bwd_gx, d.bwd_total, d.sigT, d.Tt, d.ypart, d.ncg, d.ns, d.red + d.op, d.nbv, d.Kr,   \
       d.Np, d.nsp, d.part, d.red + d.ys, d.red + d.yy, d.ictl
 #define YQ_BWD_NIT(CH)                                                                         \
-  switch (d.Kp / 16) {                                                                         \
+  switch (d.Kr / 16) {                                                                         \
+    case 1: bwd_body<CH, kBwdSlice, 1>(YQ_BWD_ARGS); break;                                    \
     case 2: bwd_body<CH, kBwdSlice, 2>(YQ_BWD_ARGS); break;                                    \
+    case 3: bwd_body<CH, kBwdSlice, 3>(YQ_BWD_ARGS); break;                                    \
     case 4: bwd_body<CH, kBwdSlice, 4>(YQ_BWD_ARGS); break;                                    \
+    case 5: bwd_body<CH, kBwdSlice, 5>(YQ_BWD_ARGS); break;                                    \
     case 6: bwd_body<CH, kBwdSlice, 6>(YQ_BWD_ARGS); break;                                    \
+    case 7: bwd_body<CH, kBwdSlice, 7>(YQ_BWD_ARGS); break;                                    \
     case 8: bwd_body<CH, kBwdSlice, 8>(YQ_BWD_ARGS); break;                                    \
+    case 9: bwd_body<CH, kBwdSlice, 9>(YQ_BWD_ARGS); break;                                    \
     default: bwd_body<CH, kBwdSlice, 10>(YQ_BWD_ARGS); break;                                  \
   }
   if (d.chunk == 128) {
@@ -1563,6 +1574,7 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
   w.m = m;
   w.use_bias = use_bias;
   w.Kp = rup(n + 1, 32);
+  w.Kr = rup(n + 1, 16);
   w.Np = rup(m + 1, 16);
   w.nsp = rup(ns, kRowPad);
   if (w.ct == 0) w.ct = 4;
@@ -1611,6 +1623,12 @@ int mixed_prepare(MixedWork& w, const int8_t* sigma, const float* T32, int ns, i
                      w.Kp, w.nsp, w.sigT);
   hipLaunchKernelGGL(k_prep_T, dim3(nb(rows * w.Np)), dim3(256), 0, st, T32, ns, m, w.Np, w.nsp,
                      w.Trm, w.Tt);
+  // the CG step writes rows [0, Kr) of the split vector; the forward product
+  // reads Kp rows, the rest zero
+  {
+    const hipError_t e = hipMemsetAsync(w.vsplit, 0, sizeof(uint16_t) * 3 * w.Np * w.Kp, st);
+    if (e != hipSuccess) return hipfail(err, "mixed_prepare", e);
+  }
   if (w.pin_ct && w.pin_chunk) {
     w.ct = w.pin_ct;
     w.chunk = w.pin_chunk;
@@ -1834,6 +1852,7 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
     d.m = w.m;
     d.use_bias = w.use_bias;
     d.Kp = w.Kp;
+    d.Kr = w.Kr;
     d.Np = w.Np;
     d.nsp = w.nsp;
     d.ncg = w.ncg;
@@ -1845,7 +1864,7 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
     d.bwd_gx = (w.Np / 16 + 3) / 4;
     d.bwd_total = d.bwd_gx * (w.nsp / w.chunk);
     d.cg_gx = w.Np / 16;
-    d.cg_total = d.cg_gx * (w.Kp / 16);
+    d.cg_total = d.cg_gx * (w.Kr / 16);
     d.in = j.active ? 1 : 0;
     d.max_it = j.max_iter;
     desc[q] = d;
@@ -1853,7 +1872,7 @@ int mixed_cg_group(MixedGroup& g, std::vector<GroupJob>& jobs, double shift, hip
     for (int b = 0; b < d.bwd_total; ++b) bb.push_back(make_int2(q, b));
     for (int b = 0; b < d.cg_total; ++b) bc.push_back(make_int2(q, b));
     lds_f = std::max(lds_f, sizeof(uint16_t) * 3 * w.ct * 16 * (w.Kp + kLdsPad));
-    lds_b = std::max(lds_b, sizeof(uint16_t) * w.Kp * (kBwdSlice + kLdsPad));
+    lds_b = std::max(lds_b, sizeof(uint16_t) * w.Kr * (kBwdSlice + kLdsPad));
     maxdim = std::max(maxdim, d.dim);
     cgg = std::max(cgg, cg_groups(w.nsp / w.chunk));
   }
